@@ -1,0 +1,1611 @@
+/*
+ * CPU oracle for the moment-image hot path. TEST INFRASTRUCTURE ONLY: the
+ * checker for the HIP kernels and the CPU baseline timed by bench.py. See
+ * mray_oracle.h for the precision modes.
+ *
+ * Build: gcc -std=gnu11 -O2 -fopenmp -ffp-contract=off (no -march, no
+ * -ffast-math) so that float expressions round exactly like the reference's
+ * g++ -O2 build on x86-64 (SSE2, FLT_EVAL_METHOD == 0).
+ *
+ * Where the reference mixes float and double (M_PI literals, 0.5 literals,
+ * 1.0 / x, unqualified cos/sin/exp/atan2 on floats, which resolve to the
+ * double overloads), the same promotion is written out explicitly.
+ */
+#include "mray_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define F_PI ((float)M_PI)
+
+/* ------------------------------------------------------------------------ */
+/* helpers: cut::min / cut::max / clamp (src/cut/cuda_math.h:72-85,825-828)  */
+/* ------------------------------------------------------------------------ */
+static inline float fminr(float x, float y) { return (x < y) ? x : y; }
+static inline float fmaxr(float x, float y) { return (x > y) ? x : y; }
+static inline float fclampr(float v, float lo, float hi) { return fminr(fmaxr(v, lo), hi); }
+static inline int iclampr(int v, int lo, int hi)
+{
+    int a = (v > lo) ? v : lo;
+    return (a < hi) ? a : hi;
+}
+static inline double dminr(double x, double y) { return (x < y) ? x : y; }
+static inline double dmaxr(double x, double y) { return (x > y) ? x : y; }
+static inline double dclampr(double v, double lo, double hi) { return dminr(dmaxr(v, lo), hi); }
+
+/* float_complex (src/mese/complex_algebra.h:34-158) */
+typedef struct
+{
+    float x, y;
+} cf;
+static inline cf cf_make(float x, float y)
+{
+    cf r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+static inline cf cf_add(cf a, cf b) { return cf_make(a.x + b.x, a.y + b.y); }
+static inline cf cf_sub(cf a, cf b) { return cf_make(a.x - b.x, a.y - b.y); }
+static inline cf cf_mul(cf a, cf b) { return cf_make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+static inline cf cf_rmul(float s, cf a) { return cf_make(s * a.x, s * a.y); }
+static inline cf cf_conj(cf a) { return cf_make(a.x, -a.y); }
+static inline float cf_abs2(cf a) { return a.x * a.x + a.y * a.y; }
+static inline cf cf_div(cf n, cf d)
+{
+    float factor = 1.0f / cf_abs2(d);
+    return cf_make((n.x * d.x + n.y * d.y) * factor, (-n.x * d.y + n.y * d.x) * factor);
+}
+static inline cf cf_divr(cf n, float d)
+{
+    float factor = 1.0f / d;
+    return cf_make(n.x * factor, n.y * factor);
+}
+/* euler_exp: the reference's unqualified cos/sin on a float resolve to the
+ * double overloads (src/mese/complex_algebra.h:155-158). */
+static inline cf euler_exp(float x) { return cf_make((float)cos((double)x), (float)sin((double)x)); }
+
+typedef struct
+{
+    double x, y;
+} cd;
+static inline cd cd_make(double x, double y)
+{
+    cd r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+static inline cd cd_add(cd a, cd b) { return cd_make(a.x + b.x, a.y + b.y); }
+static inline cd cd_sub(cd a, cd b) { return cd_make(a.x - b.x, a.y - b.y); }
+static inline cd cd_mul(cd a, cd b) { return cd_make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+static inline cd cd_rmul(double s, cd a) { return cd_make(s * a.x, s * a.y); }
+static inline cd cd_conj(cd a) { return cd_make(a.x, -a.y); }
+static inline double cd_abs2(cd a) { return a.x * a.x + a.y * a.y; }
+
+static int set_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        return nthreads;
+    return omp_get_max_threads();
+#else
+    (void)nthreads;
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* camera / rays                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* get_eye_ray_persp / _ortho (src/cut/raytracing.h:39-65), Mat4f*Vec4f
+ * (src/cut/matrix.h:658-664), normalize (src/cut/cuda_math.h:502-511). */
+void orc_eye_ray(const mray_camera *cam, float ndcx, float ndcy, float org[3], float dir[3])
+{
+    if (cam->orthographic)
+    {
+        for (int i = 0; i < 3; ++i)
+        {
+            float a = cam->position[i] + ndcx * cam->right[i];
+            org[i] = a + ndcy * cam->up[i];
+            dir[i] = cam->dir[i];
+        }
+    }
+    else
+    {
+        org[0] = cam->position[0];
+        org[1] = cam->position[1];
+        org[2] = cam->position[2];
+        float x = ndcx * cam->aspect * cam->scale;
+        float y = ndcy * cam->scale;
+        const float *M = cam->transform;
+        float v[3];
+        for (int r = 0; r < 3; ++r)
+            v[r] = M[r * 4 + 0] * x + M[r * 4 + 1] * y + M[r * 4 + 2] * -1.0f + M[r * 4 + 3] * 0.0f;
+        float inv = 1 / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        dir[0] = v[0] * inv;
+        dir[1] = v[1] * inv;
+        dir[2] = v[2] * inv;
+    }
+}
+
+/* Pixel -> NDC without half-pixel offset (src/moment_raymarcher.cpp:133,298). */
+void orc_pixel_ray(const mray_camera *cam, int W, int H, int x, int y, float org[3], float dir[3])
+{
+    float ndcx = x / (float)W * 2.f - 1.f;
+    float ndcy = y / (float)H * 2.f - 1.f;
+    orc_eye_ray(cam, ndcx, ndcy, org, dir);
+}
+
+/* Slab test with NaN-dropping operand order (src/cut/raytracing.h:67-86). */
+int orc_intersect_aabb(const float org[3], const float dir[3], const float bmin[3], const float bmax[3],
+                       float *t_near, float *t_far)
+{
+    float tmin[3], tmax[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        float invR = 1.0f / dir[i];
+        float tbot = invR * (bmin[i] - org[i]);
+        float ttop = invR * (bmax[i] - org[i]);
+        tmin[i] = fminr(ttop, tbot);
+        tmax[i] = fmaxr(ttop, tbot);
+    }
+    float largest_tmin = fmaxr(fmaxr(tmin[0], tmin[1]), fmaxr(tmin[0], tmin[2]));
+    float smallest_tmax = fminr(fminr(tmax[0], tmax[1]), fminr(tmax[0], tmax[2]));
+    *t_near = largest_tmin;
+    *t_far = smallest_tmax;
+    return smallest_tmax > largest_tmin;
+}
+
+/* ------------------------------------------------------------------------ */
+/* volume sampling (src/volume.h:59-123)                                     */
+/* ------------------------------------------------------------------------ */
+static inline float vol_get(const orc_volume *v, int x, int y, int z)
+{
+    x = iclampr(x, 0, v->nx - 1);
+    y = iclampr(y, 0, v->ny - 1);
+    z = iclampr(z, 0, v->nz - 1);
+    size_t idx = (size_t)z * (size_t)(v->ny * v->nx) + (size_t)(y * v->nx + x);
+    if (v->dtype == MRAY_DTYPE_U8)
+        return (float)(((const uint8_t *)v->data)[idx] / 255.f);
+    return ((const float *)v->data)[idx];
+}
+
+float orc_volume_sample(const orc_volume *v, const float p[3])
+{
+    float ip[3], a[3];
+    int il[3];
+    const int n[3] = {v->nx, v->ny, v->nz};
+    for (int i = 0; i < 3; ++i)
+    {
+        float np = (p[i] - v->dmin[i]) / (v->dmax[i] - v->dmin[i]);
+        ip[i] = np * (float)n[i];
+        a[i] = ip[i] - floorf(ip[i]);
+        il[i] = (int)floorf(ip[i]);
+    }
+    int xl = il[0], yl = il[1], zl = il[2];
+    float lll = vol_get(v, xl, yl, zl);
+    float lrl = vol_get(v, xl, yl + 1, zl);
+    float rrl = vol_get(v, xl + 1, yl + 1, zl);
+    float rll = vol_get(v, xl + 1, yl, zl);
+    float llr = vol_get(v, xl, yl, zl + 1);
+    float lrr = vol_get(v, xl, yl + 1, zl + 1);
+    float rrr = vol_get(v, xl + 1, yl + 1, zl + 1);
+    float rlr = vol_get(v, xl + 1, yl, zl + 1);
+    float ax = a[0], ay = a[1], az = a[2];
+    return (1.f - ax) * (1.f - ay) * (1.f - az) * lll + ax * (1.f - ay) * (1.f - az) * rll +
+           (1.f - ax) * ay * (1.f - az) * lrl + ax * ay * (1.f - az) * rrl + (1.f - ax) * (1.f - ay) * az * llr +
+           ax * (1.f - ay) * az * rlr + (1.f - ax) * ay * az * lrr + ax * ay * az * rrr;
+}
+
+double orc_volume_sample_f64(const orc_volume *v, const double p[3])
+{
+    double a[3];
+    int il[3];
+    const int n[3] = {v->nx, v->ny, v->nz};
+    for (int i = 0; i < 3; ++i)
+    {
+        double ip = (p[i] - v->dmin[i]) / ((double)v->dmax[i] - v->dmin[i]) * n[i];
+        double fl = floor(ip);
+        a[i] = ip - fl;
+        il[i] = (int)fl;
+    }
+    int xl = il[0], yl = il[1], zl = il[2];
+    double ax = a[0], ay = a[1], az = a[2];
+    double c[2][2][2];
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                c[dz][dy][dx] = vol_get(v, xl + dx, yl + dy, zl + dz);
+    double r = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                r += (dx ? ax : 1.0 - ax) * (dy ? ay : 1.0 - ay) * (dz ? az : 1.0 - az) * c[dz][dy][dx];
+    return r;
+}
+
+/* map_density (src/parameters.h:31-34) */
+static inline float map_density(float s, const mray_gen_params *p)
+{
+    return fclampr((s - p->data_min) / (p->data_max - p->data_min), 0.0f, 1.0f);
+}
+
+/* DensityBound::set (src/common.h:143-147): truncating 16-bit quantisation */
+static inline uint32_t pack_bound(float l, float u)
+{
+    uint16_t lo = (uint16_t)(l * 65535.f);
+    uint16_t hi = (uint16_t)(u * 65535.f);
+    return (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
+/* ------------------------------------------------------------------------ */
+/* generate (src/raymarch_generate.h)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* update_moments (src/raymarch_generate.h:35-58) */
+static void update_moments(int m, float *trig, float p_next, float phase_step_size, float fs, float bs)
+{
+    float p_prev = p_next - phase_step_size;
+    float gradient = (bs - fs) / phase_step_size;
+    float yintercept = fs - gradient * p_prev;
+
+    for (int j = 1; j < m; ++j)
+    {
+        float jSq = (float)j * (float)j;
+        cf common = cf_make(gradient / jSq, yintercept / (float)j);
+
+        cf a = cf_add(common, cf_make(0, gradient * (float)j * p_next / jSq));
+        trig[j] += cf_mul(a, euler_exp((float)(-j) * p_next)).x;
+        cf b = cf_add(common, cf_make(0, gradient * (float)j * p_prev / jSq));
+        trig[j] -= cf_mul(b, euler_exp((float)(-j) * p_prev)).x;
+    }
+    trig[0] = (float)((double)trig[0] + (0.5 * gradient * (double)(p_next * p_next) + (double)(yintercept * p_next)));
+    trig[0] = (float)((double)trig[0] - (0.5 * gradient * (double)(p_prev * p_prev) + (double)(yintercept * p_prev)));
+}
+
+static inline void sample_at(const orc_volume *v, const float org[3], const float dir[3], float t, float out[3])
+{
+    for (int i = 0; i < 3; ++i)
+        out[i] = org[i] + t * dir[i];
+}
+
+/* generate_ray (src/raymarch_generate.h:69-148) */
+uint32_t orc_generate_ray(const orc_volume *v, const float org[3], const float dir[3], float znear, float *moments,
+                          int m, const mray_gen_params *gp, int32_t *n_samples)
+{
+    float trig[512];
+    for (int j = 0; j < m; ++j)
+        trig[j] = 0.0f;
+
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, v->dmin, v->dmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    int ns = 0;
+
+    if (hit && (t_far > 0))
+    {
+        float step = gp->step_size;
+        float bound_lo = 1.f, bound_hi = 0.f;
+        float t = t_near;
+        float pi_trange_inv = (float)(M_PI / (double)(t_far - t_near));
+        float p = (float)(-M_PI);
+        float phase_step_size = (float)(M_PI * (double)(step / (t_far - t_near)));
+        float pos[3];
+
+        sample_at(v, org, dir, t, pos);
+        float fs = map_density(orc_volume_sample(v, pos), gp);
+        ++ns;
+        bound_lo = fminr(bound_lo, fs);
+        bound_hi = fmaxr(bound_hi, fs);
+        float bs;
+        while (t + step < t_far)
+        {
+            t += step;
+            p = (float)((double)((t - t_near) * pi_trange_inv) - M_PI);
+            sample_at(v, org, dir, t, pos);
+            bs = map_density(orc_volume_sample(v, pos), gp);
+            ++ns;
+            bound_lo = fminr(bound_lo, bs);
+            bound_hi = fmaxr(bound_hi, bs);
+            update_moments(m, trig, p, phase_step_size, fs, bs);
+            fs = bs;
+        }
+        float last_step = 0.0f - p;
+        sample_at(v, org, dir, t_far, pos);
+        bs = map_density(orc_volume_sample(v, pos), gp);
+        ++ns;
+        bound_lo = fminr(bound_lo, bs);
+        bound_hi = fmaxr(bound_hi, bs);
+        update_moments(m, trig, 0.0f, last_step, fs, bs);
+
+        if (gp->transform_to_bounds)
+        {
+            bound_lo = fmaxr(bound_lo - gp->transform_bound_eps, 0.0f);
+            bound_hi = fminr(bound_hi + gp->transform_bound_eps, 1.0f);
+            double inv_relaxed_bound = 1.0 / (double)(bound_hi - bound_lo);
+            moments[0] = (float)((double)(trig[0] * 0.3183098861837907f - bound_lo) * inv_relaxed_bound);
+            for (int j = 1; j < m; ++j)
+                moments[j] = (float)((double)(trig[j] * 0.3183098861837907f) * inv_relaxed_bound);
+        }
+        else
+        {
+            for (int j = 0; j < m; ++j)
+                moments[j] = trig[j] * 0.3183098861837907f;
+        }
+        if (n_samples)
+            *n_samples = ns;
+        return pack_bound(bound_lo, bound_hi);
+    }
+    if (n_samples)
+        *n_samples = 0;
+    if (gp->transform_to_bounds && gp->compact_image)
+        return 0; /* moments deliberately left unwritten */
+    for (int j = 0; j < m; ++j)
+        moments[j] = 0.f;
+    return 0;
+}
+
+/* generate_host's per-pixel loop (src/moment_raymarcher.cpp:121-145). The
+ * optional compaction / prediction-coding stages are separate functions. */
+void orc_generate_image(const orc_volume *v, const mray_camera *cam, int W, int H, int m, const mray_gen_params *p,
+                        float *moments, uint32_t *bounds, int32_t *n_samples, int row0, int row_step, int nthreads)
+{
+    if (row_step < 1)
+        row_step = 1;
+    int nrows = (H - row0 + row_step - 1) / row_step;
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int r = 0; r < nrows; ++r)
+    {
+        int y = row0 + r * row_step;
+        for (int x = 0; x < W; ++x)
+        {
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            size_t pix = (size_t)y * W + x;
+            int32_t ns = 0;
+            uint32_t b = orc_generate_ray(v, org, dir, cam->znear, moments + pix * m, m, p, &ns);
+            if (bounds)
+                bounds[pix] = b;
+            if (n_samples)
+                n_samples[pix] = ns;
+        }
+    }
+}
+
+/* FP64 generate over the same nodes: the float t sequence and node count of
+ * the reference loop, everything else in double with exact sin/cos. */
+uint32_t orc_generate_ray_f64(const orc_volume *v, const float org[3], const float dir[3], float znear,
+                              double *moments, int m, const mray_gen_params *gp)
+{
+    double trig[512];
+    for (int j = 0; j < m; ++j)
+        trig[j] = 0.0;
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, v->dmin, v->dmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    if (!(hit && (t_far > 0)))
+    {
+        if (!(gp->transform_to_bounds && gp->compact_image))
+            for (int j = 0; j < m; ++j)
+                moments[j] = 0.0;
+        return 0;
+    }
+    double dmin = gp->data_min, dmax = gp->data_max;
+    float step = gp->step_size;
+    double tn = t_near, tf = t_far;
+    double phase_step = M_PI * ((double)step / (tf - tn));
+    double lo = 1.0, hi = 0.0;
+
+    double pos[3];
+    float t = t_near;
+    for (int i = 0; i < 3; ++i)
+        pos[i] = (double)org[i] + (double)t * dir[i];
+    double fs = dclampr((orc_volume_sample_f64(v, pos) - dmin) / (dmax - dmin), 0.0, 1.0);
+    lo = dminr(lo, fs);
+    hi = dmaxr(hi, fs);
+    double p = -M_PI;
+    int first = 1;
+
+    for (;;)
+    {
+        int last = !(t + step < t_far);
+        double p_next, seg;
+        double bs;
+        if (!last)
+        {
+            t += step;
+            p_next = ((double)t - tn) * (M_PI / (tf - tn)) - M_PI;
+            seg = phase_step;
+            for (int i = 0; i < 3; ++i)
+                pos[i] = (double)org[i] + (double)t * dir[i];
+        }
+        else
+        {
+            p_next = 0.0;
+            seg = 0.0 - p;
+            for (int i = 0; i < 3; ++i)
+                pos[i] = (double)org[i] + tf * dir[i];
+        }
+        bs = dclampr((orc_volume_sample_f64(v, pos) - dmin) / (dmax - dmin), 0.0, 1.0);
+        lo = dminr(lo, bs);
+        hi = dmaxr(hi, bs);
+        double p_prev = p_next - seg;
+        double g = (bs - fs) / seg;
+        double y0 = fs - g * p_prev;
+        for (int j = 1; j < m; ++j)
+        {
+            double jd = j;
+            /* Re[(g/j^2 + i(y + g phi)/j) e^{-i j phi}] evaluated at both ends */
+            double cn = cos(jd * p_next), sn = sin(jd * p_next);
+            double cp = cos(jd * p_prev), sp = sin(jd * p_prev);
+            double an = g / (jd * jd), bn = (y0 + g * p_next) / jd;
+            double bp = (y0 + g * p_prev) / jd;
+            trig[j] += (an * cn + bn * sn) - (an * cp + bp * sp);
+        }
+        trig[0] += (0.5 * g * p_next * p_next + y0 * p_next) - (0.5 * g * p_prev * p_prev + y0 * p_prev);
+        fs = bs;
+        p = p_next;
+        first = 0;
+        if (last)
+            break;
+    }
+    (void)first;
+    if (gp->transform_to_bounds)
+    {
+        /* the relaxed bound is formed in float exactly like the reference so
+         * that the stored 16-bit bound is the same pixel value */
+        float flo = fmaxr((float)lo - gp->transform_bound_eps, 0.0f);
+        float fhi = fminr((float)hi + gp->transform_bound_eps, 1.0f);
+        double inv = 1.0 / ((double)fhi - (double)flo);
+        moments[0] = (trig[0] / M_PI - flo) * inv;
+        for (int j = 1; j < m; ++j)
+            moments[j] = trig[j] / M_PI * inv;
+        return pack_bound(flo, fhi);
+    }
+    for (int j = 0; j < m; ++j)
+        moments[j] = trig[j] / M_PI;
+    return pack_bound((float)lo, (float)hi);
+}
+
+void orc_generate_image_f64(const orc_volume *v, const mray_camera *cam, int W, int H, int m,
+                            const mray_gen_params *p, double *moments, uint32_t *bounds, int row0, int row_step,
+                            int nthreads)
+{
+    if (row_step < 1)
+        row_step = 1;
+    int nrows = (H - row0 + row_step - 1) / row_step;
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int r = 0; r < nrows; ++r)
+    {
+        int y = row0 + r * row_step;
+        for (int x = 0; x < W; ++x)
+        {
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            size_t pix = (size_t)y * W + x;
+            uint32_t b = orc_generate_ray_f64(v, org, dir, cam->znear, moments + pix * m, m, p);
+            if (bounds)
+                bounds[pix] = b;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* bounded MESE, FP32 (src/mese/MESE_dynamic.h, src/mese/complex_algebra.h) */
+/* ------------------------------------------------------------------------ */
+
+/* fast_atan (src/mese/complex_algebra.h:131-152) */
+float orc_fast_atan(float tan)
+{
+    int negative = (tan < 0.0f);
+    float abs_tan = negative ? (-tan) : tan;
+    int greater_one = (abs_tan > 1.0f);
+    float x = greater_one ? abs_tan : 1.0f;
+    x = 1.0f / x;
+    float y = greater_one ? 1.0f : abs_tan;
+    x = x * y;
+    y = x * x;
+    float z = y * 0.020835f - 0.085133f;
+    z = y * z + 0.180141f;
+    z = y * z - 0.330299f;
+    y = y * z + 0.999866f;
+    z = y * x;
+    z = z * -2.0f + 1.570796f;
+    z = greater_one ? z : 0.0f;
+    x = x * y + z;
+    return negative ? (-x) : x;
+}
+
+/* trigonometricToExponentialMoments, real input (src/mese/MESE_dynamic.h:35-54) */
+static void trig_to_exp_f(int m, cf *E, const float *t)
+{
+    float ph = 3.14159265f * t[0] - 1.57079633f;
+    E[0] = cf_make(cosf(ph), sinf(ph));
+    E[0] = cf_rmul(0.0795774715f, E[0]);
+    for (int l = 1; l < m; ++l)
+    {
+        E[l] = cf_make(0.0f, 0.0f);
+        for (int j = 0; j < l; ++j)
+            E[l] = cf_add(E[l], cf_mul(cf_rmul((float)(l - j), E[j]), cf_make(t[l - j], 0.0f)));
+        E[l] = cf_mul(E[l], cf_make(0.0f, (float)((double)(2.0f * M_PI) / (double)(float)l)));
+    }
+    E[0] = cf_rmul(2.0f, E[0]);
+}
+
+/* complex-input overload (src/mese/MESE_dynamic.h:56-75) */
+static void trig_to_exp_fc(int m, cf *E, const cf *t)
+{
+    float ph = 3.14159265f * t[0].x - 1.57079633f;
+    E[0] = cf_make(cosf(ph), sinf(ph));
+    E[0] = cf_rmul(0.0795774715f, E[0]);
+    for (int l = 1; l < m; ++l)
+    {
+        E[l] = cf_make(0.0f, 0.0f);
+        for (int j = 0; j < l; ++j)
+            E[l] = cf_add(E[l], cf_mul(cf_rmul((float)(l - j), E[j]), t[l - j]));
+        E[l] = cf_mul(E[l], cf_make(0.0f, (float)((double)(2.0f * M_PI) / (double)(float)l)));
+    }
+    E[0] = cf_rmul(2.0f, E[0]);
+}
+
+/* exponentialToTrigonometricMoments (src/mese/MESE_dynamic.h:77-98) */
+static void exp_to_trig_f(int m, cf *T, const cf *E)
+{
+    cf gamma_0 = cf_divr(E[0], 2.f);
+    T[0] = cf_divr(gamma_0, 0.0795774715f);
+    float ph = (float)atan2((double)T[0].y, (double)T[0].x);
+    T[0] = cf_make((ph + 1.57079633f) / 3.14159265f, 0.0f);
+    T[0].y = 0.0f;
+    for (int l = 1; l < m; ++l)
+    {
+        T[l] = cf_make(0.0f, 0.0f);
+        for (int j = 1; j < l; ++j)
+        {
+            cf c = cf_div(cf_make((float)(l - j), 0.0f), cf_rmul((float)l, gamma_0));
+            T[l] = cf_add(T[l], cf_mul(cf_mul(c, E[j]), T[l - j]));
+        }
+        cf den = cf_mul(cf_make(0.0f, (float)(2.0f * M_PI)), gamma_0);
+        T[l] = cf_sub(cf_div(E[l], den), T[l]);
+    }
+}
+
+/* levinsonsAlgorithm (src/mese/MESE_dynamic.h:100-135) */
+static void levinson_f(int m, cf *sol, const cf *col, cf *tmp)
+{
+    sol[0] = cf_make(1.0f / col[0].x, 0.0f);
+    int invalid = 0;
+    for (int j = 1; j < m; ++j)
+    {
+        cf u = cf_make(0.0f, 0.0f);
+        if (!invalid)
+            for (int k = 0; k < j; ++k)
+                u = cf_add(u, cf_mul(sol[k], col[j - k]));
+        float a2 = cf_abs2(u);
+        if (a2 >= 1.0f)
+        {
+            u = cf_make(0.0f, 0.0f);
+            a2 = 0;
+            invalid = 1;
+        }
+        float factor = 1.0f / (1.0f - a2);
+        for (int k = 1; k < j; ++k)
+            tmp[k] = cf_conj(sol[j - k]);
+        tmp[j] = cf_make(sol[0].x, 0.0f);
+        sol[0] = cf_make(factor * sol[0].x, 0.0f);
+        for (int k = 1; k < j; ++k)
+            sol[k] = cf_rmul(factor, cf_sub(sol[k], cf_mul(tmp[k], u)));
+        sol[j] = cf_rmul(factor, cf_rmul(-tmp[j].x, u));
+    }
+}
+
+/* computeAutocorrelation (src/mese/MESE_dynamic.h:137-147) */
+static void autocorr_f(int m, cf *out, const cf *s)
+{
+    for (int k = 0; k < m; ++k)
+    {
+        out[k] = cf_make(0.0f, 0.0f);
+        for (int j = 0; j < m - k; ++j)
+            out[k] = cf_add(out[k], cf_mul(s[j], cf_conj(s[j + k])));
+    }
+}
+
+/* computeCorrelation (src/mese/MESE_dynamic.h:149-159) */
+static void corr_f(int m, cf *out, const cf *lhs, const cf *rhs)
+{
+    for (int k = 0; k < m; ++k)
+    {
+        out[k] = cf_make(0.0f, 0.0f);
+        for (int j = 0; j < m - k; ++j)
+            out[k] = cf_add(out[k], cf_mul(lhs[k + j], rhs[j]));
+    }
+}
+
+/* computeImaginaryCorrelation (src/mese/MESE_dynamic.h:161-171) */
+static void imag_corr_f(int m, float *out, const cf *lhs, const cf *rhs)
+{
+    for (int k = 0; k < m; ++k)
+    {
+        out[k] = 0.0f;
+        for (int j = 0; j < m - k; ++j)
+            out[k] += lhs[k + j].x * rhs[j].y + lhs[k + j].y * rhs[j].x;
+    }
+}
+
+/* prepareMESELagrange, real overload (src/mese/MESE_dynamic.h:194-211) */
+static void prepare_mese_f(int m, float *out, const float *trig, cf *E, cf *P, cf *R)
+{
+    trig_to_exp_f(m, E, trig);
+    levinson_f(m, P, E, R);
+    for (int i = 0; i < m; ++i)
+        P[i] = cf_rmul(6.28318531f, P[i]);
+    autocorr_f(m, R, P);
+    E[0] = cf_rmul(0.5f, E[0]);
+    imag_corr_f(m, out, R, E);
+    float nf = 1.0f / (3.14159265f * P[0].x);
+    for (int i = 0; i < m; ++i)
+        out[i] = nf * out[i];
+}
+
+/* prepareMESELagrange, complex overload (src/mese/MESE_dynamic.h:173-192) */
+static void prepare_mese_fc(int m, cf *out, const cf *trig, cf *E, cf *P, cf *R)
+{
+    trig_to_exp_fc(m, E, trig);
+    levinson_f(m, P, E, R);
+    for (int i = 0; i < m; ++i)
+        P[i] = cf_rmul(6.28318531f, P[i]);
+    autocorr_f(m, R, P);
+    E[0] = cf_rmul(0.5f, E[0]);
+    corr_f(m, out, R, E);
+    float nf = 1.0f / (3.14159265f * P[0].x);
+    for (int i = 0; i < m; ++i)
+        out[i] = cf_rmul(nf, cf_mul(cf_make(-0.0f, -1.0f), out[i]));
+    out[0] = cf_make(out[0].x, 0.0f);
+}
+
+#define MAXM 512
+
+void orc_trig_to_exp(int m, const float *trig, float *exp_c) { trig_to_exp_f(m, (cf *)exp_c, trig); }
+void orc_trig_to_exp_c(int m, const float *trig_c, float *exp_c)
+{
+    trig_to_exp_fc(m, (cf *)exp_c, (const cf *)trig_c);
+}
+void orc_exp_to_trig(int m, const float *exp_c, float *trig_c) { exp_to_trig_f(m, (cf *)trig_c, (const cf *)exp_c); }
+void orc_levinson(int m, const float *first_col_c, float *sol_c)
+{
+    cf tmp[MAXM];
+    levinson_f(m, (cf *)sol_c, (const cf *)first_col_c, tmp);
+}
+void orc_autocorrelation(int m, const float *sig_c, float *out_c) { autocorr_f(m, (cf *)out_c, (const cf *)sig_c); }
+void orc_correlation(int m, const float *lhs_c, const float *rhs_c, float *out_c)
+{
+    corr_f(m, (cf *)out_c, (const cf *)lhs_c, (const cf *)rhs_c);
+}
+void orc_imag_correlation(int m, const float *lhs_c, const float *rhs_c, float *out)
+{
+    imag_corr_f(m, out, (const cf *)lhs_c, (const cf *)rhs_c);
+}
+void orc_prepare_mese(int m, const float *trig, float *lagrange)
+{
+    cf E[MAXM], P[MAXM], R[MAXM];
+    prepare_mese_f(m, lagrange, trig, E, P, R);
+}
+void orc_prepare_mese_c(int m, const float *trig_c, float *lagrange_c)
+{
+    cf E[MAXM], P[MAXM], R[MAXM];
+    prepare_mese_fc(m, (cf *)lagrange_c, (const cf *)trig_c, E, P, R);
+}
+
+/* evaluateFourierSeries (src/mese/MESE_dynamic.h:213-226) */
+float orc_evaluate_fourier(int m, float cx, float cy, const float *coeff)
+{
+    float result = 0;
+    cf z = cf_make(cx, cy);
+    cf pw = cf_make(1.f, 0.0f);
+    for (int l = 1; l < m; ++l)
+    {
+        pw = cf_mul(pw, z);
+        result += coeff[l] * pw.x;
+    }
+    return 2.0f * result + coeff[0];
+}
+
+/* evaluateMESELagrange (src/mese/MESE_dynamic.h:242-250) */
+float orc_evaluate_mese(int m, float phase, const float *lagrange)
+{
+    float s = orc_evaluate_fourier(m, cosf(-phase), sinf(-phase), lagrange);
+    return orc_fast_atan(s) * 0.318309886f + 0.5f;
+}
+
+/* prepare_moments_from_std_coding (src/moment_preparation.h:55-64) on a copy,
+ * as MomentImageHost::prepare_moments does (src/moment_image.cpp:170-180). */
+void orc_prepare_moments_std(int m, const float *moments, float bias, float *pmoments)
+{
+    float mc[MAXM];
+    mc[0] = (1 - bias) * moments[0] + bias * 0.5f;
+    for (int i = 1; i < m; ++i)
+        mc[i] = (1 - bias) * moments[i] + bias * 0.0f;
+    orc_prepare_mese(m, mc, pmoments);
+}
+
+/* ------------------------------------------------------------------------ */
+/* bounded MESE, FP64 (same algorithm, double arithmetic, exact functions)   */
+/* ------------------------------------------------------------------------ */
+void orc_prepare_moments_std_f64(int m, const double *moments, double bias, double *out)
+{
+    cd E[MAXM], P[MAXM], R[MAXM], tmp[MAXM];
+    double t[MAXM];
+    t[0] = (1 - bias) * moments[0] + bias * 0.5;
+    for (int i = 1; i < m; ++i)
+        t[i] = (1 - bias) * moments[i];
+    double ph = M_PI * t[0] - M_PI / 2;
+    E[0] = cd_rmul(1.0 / (4.0 * M_PI), cd_make(cos(ph), sin(ph)));
+    for (int l = 1; l < m; ++l)
+    {
+        cd acc = cd_make(0, 0);
+        for (int j = 0; j < l; ++j)
+            acc = cd_add(acc, cd_rmul((double)(l - j) * t[l - j], E[j]));
+        E[l] = cd_mul(acc, cd_make(0.0, 2.0 * M_PI / l));
+    }
+    E[0] = cd_rmul(2.0, E[0]);
+    /* Levinson with the reference's invalid guard */
+    P[0] = cd_make(1.0 / E[0].x, 0.0);
+    int invalid = 0;
+    for (int j = 1; j < m; ++j)
+    {
+        cd u = cd_make(0, 0);
+        if (!invalid)
+            for (int k = 0; k < j; ++k)
+                u = cd_add(u, cd_mul(P[k], E[j - k]));
+        double a2 = cd_abs2(u);
+        if (a2 >= 1.0)
+        {
+            u = cd_make(0, 0);
+            a2 = 0;
+            invalid = 1;
+        }
+        double f = 1.0 / (1.0 - a2);
+        for (int k = 1; k < j; ++k)
+            tmp[k] = cd_conj(P[j - k]);
+        tmp[j] = cd_make(P[0].x, 0.0);
+        P[0] = cd_make(f * P[0].x, 0.0);
+        for (int k = 1; k < j; ++k)
+            P[k] = cd_rmul(f, cd_sub(P[k], cd_mul(tmp[k], u)));
+        P[j] = cd_rmul(f, cd_rmul(-tmp[j].x, u));
+    }
+    for (int i = 0; i < m; ++i)
+        P[i] = cd_rmul(2.0 * M_PI, P[i]);
+    for (int k = 0; k < m; ++k)
+    {
+        cd acc = cd_make(0, 0);
+        for (int j = 0; j < m - k; ++j)
+            acc = cd_add(acc, cd_mul(P[j], cd_conj(P[j + k])));
+        R[k] = acc;
+    }
+    E[0] = cd_rmul(0.5, E[0]);
+    double nf = 1.0 / (M_PI * P[0].x);
+    for (int k = 0; k < m; ++k)
+    {
+        double acc = 0.0;
+        for (int j = 0; j < m - k; ++j)
+            acc += R[k + j].x * E[j].y + R[k + j].y * E[j].x;
+        out[k] = nf * acc;
+    }
+}
+
+double orc_evaluate_mese_f64(int m, double phase, const double *lagrange)
+{
+    double s = 0.0;
+    for (int l = 1; l < m; ++l)
+        s += lagrange[l] * cos(l * phase);
+    s = 2.0 * s + lagrange[0];
+    return atan(s) / M_PI + 0.5;
+}
+
+/* ------------------------------------------------------------------------ */
+/* transfer function (src/transfer_function.cpp:46-119, .h:44-59)            */
+/* ------------------------------------------------------------------------ */
+
+/* read_tf + the value preintegrate keeps for every table row
+ * (src/transfer_function.cpp:86-92,108-116). */
+void orc_tf_from_rgba8(const uint8_t *rgba8, float step_size, float *tf)
+{
+    for (int i = 0; i < 256; ++i)
+    {
+        float r = powf(rgba8[i * 4 + 0] / 255.f, 2.2f);
+        float g = powf(rgba8[i * 4 + 1] / 255.f, 2.2f);
+        float b = powf(rgba8[i * 4 + 2] / 255.f, 2.2f);
+        float a = rgba8[i * 4 + 3] / 255.f;
+        a *= step_size;
+        tf[i * 4 + 0] = r * a;
+        tf[i * 4 + 1] = g * a;
+        tf[i * 4 + 2] = b * a;
+        tf[i * 4 + 3] = a;
+    }
+}
+
+/* TransferFunction::sample on the row-constant table (src/transfer_function.h:44-59). */
+void orc_tf_sample(const float *tf, float fs, float bs, float out[4])
+{
+    float ipx = fs * 256.0f, ipy = bs * 256.0f;
+    float ax = ipx - floorf(ipx);
+    float ay = ipy - floorf(ipy);
+    int xl = (int)floorf(ipx);
+    int x0 = iclampr(xl, 0, 255), x1 = iclampr(xl + 1, 0, 255);
+    const float *ll = tf + x0 * 4, *lr = tf + x0 * 4, *rr = tf + x1 * 4, *rl = tf + x1 * 4;
+    float w0 = (1.f - ax) * (1.f - ay), w1 = ax * (1.f - ay), w2 = (1.f - ax) * ay, w3 = ax * ay;
+    for (int c = 0; c < 4; ++c)
+        out[c] = w0 * ll[c] + w1 * lr[c] + w2 * rl[c] + w3 * rr[c];
+}
+
+static void tf_sample_f64(const float *tf, double fs, double bs, double out[4])
+{
+    double ipx = fs * 256.0, ipy = bs * 256.0;
+    double ax = ipx - floor(ipx), ay = ipy - floor(ipy);
+    int xl = (int)floor(ipx);
+    int x0 = iclampr(xl, 0, 255), x1 = iclampr(xl + 1, 0, 255);
+    (void)ax;
+    for (int c = 0; c < 4; ++c)
+        out[c] = (1.0 - ay) * tf[x0 * 4 + c] + ay * tf[x1 * 4 + c];
+}
+
+/* ------------------------------------------------------------------------ */
+/* reconstruct (src/raymarch_reconstruct.h:127-200)                          */
+/* ------------------------------------------------------------------------ */
+static inline float mese_reconstruct(int m, const float *pm, float blo, float bhi, float p, int fourier)
+{
+    p = fclampr(p, (float)(-M_PI), 0.0f);
+    float s;
+    if (fourier)
+    {
+        /* TruncatedFourierReconstruction (src/reconstruction_techniques.h:43-64) */
+        s = orc_evaluate_fourier(m, cosf(-p), sinf(-p), pm);
+    }
+    else
+        s = orc_evaluate_mese(m, p, pm);
+    return s * (bhi - blo) + blo;
+}
+
+void orc_reconstruct_ray(const float org[3], const float dir[3], const float bmin[3], const float bmax[3], float znear,
+                         int m, const float *pm, float blo, float bhi, const float *tf, const mray_rec_params *rp,
+                         float out[4], int32_t *n_steps)
+{
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    float step = rp->step_size;
+    int fourier = rp->use_truncated_fourier;
+    int ns = 0;
+    if (hit && (t_far > 0))
+    {
+        float t = t_near;
+        float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+        float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+        float L[3] = {0.0f, 0.0f, 0.0f};
+        float alpha = 0.0f, T = 1.0f, tau = 0.0f;
+        float fs = mese_reconstruct(m, pm, blo, bhi, p, fourier);
+        float bs;
+        while (t <= t_far)
+        {
+            t += step;
+            p += phase_step;
+            ++ns;
+            bs = mese_reconstruct(m, pm, blo, bhi, p, fourier);
+            float rgba[4];
+            orc_tf_sample(tf, fs, bs, rgba);
+            float sigma_t = rgba[3];
+            for (int c = 0; c < 3; ++c)
+            {
+                float Le = rp->medium_emission * rgba[c];
+                L[c] += (Le + 0.0f * rp->medium_scattering) * T;
+            }
+            tau += sigma_t;
+            T = (float)exp((double)(-tau));
+            alpha = 1.0f - T;
+            fs = bs;
+            if (alpha >= rp->early_ray_termination)
+            {
+                alpha = 1.f;
+                break;
+            }
+        }
+        out[0] = L[0];
+        out[1] = L[1];
+        out[2] = L[2];
+        out[3] = alpha;
+    }
+    else
+        out[0] = out[1] = out[2] = out[3] = 0.f;
+    if (n_steps)
+        *n_steps = ns;
+}
+
+void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                           const float *data, const uint32_t *index, const uint32_t *bounds, int prediction_coded,
+                           int coding_warp, const float *coding_params, const float *tf, const mray_rec_params *p,
+                           float *rgba, int32_t *n_steps, int row0, int row_step, int nthreads)
+{
+    if (row_step < 1)
+        row_step = 1;
+    int nrows = (H - row0 + row_step - 1) / row_step;
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int r = 0; r < nrows; ++r)
+    {
+        int y = row0 + r * row_step;
+        float pm[MAXM];
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            int nm = index ? (int)(index[pix + 1] - index[pix]) : m;
+            size_t off = index ? index[pix] : pix * (size_t)m;
+            float *out = rgba + pix * 4;
+            if (nm == 0)
+            {
+                /* reconstruct_zero_ray (src/raymarch_reconstruct.h:127-131) */
+                float c[4];
+                orc_tf_sample(tf, 0.0f, 0.0f, c);
+                out[0] = c[0];
+                out[1] = c[1];
+                out[2] = c[2];
+                out[3] = 0.0f;
+                if (n_steps)
+                    n_steps[pix] = 0;
+                continue;
+            }
+            if (p->use_truncated_fourier)
+            {
+                if (prediction_coded)
+                {
+                    /* get_trig_moments decode path is only needed for the Fourier
+                     * variant of coded images; not on the MESE hot path. */
+                    for (int l = 0; l < nm; ++l)
+                        pm[l] = 0.0f;
+                }
+                else
+                    for (int l = 0; l < nm; ++l)
+                        pm[l] = data[off + l];
+            }
+            else if (prediction_coded)
+                orc_prepare_moments_pred(nm, data + off, coding_warp, coding_params, pm);
+            else
+                orc_prepare_moments_std(nm, data + off, p->bias, pm);
+            float blo = 0.f, bhi = 1.f;
+            if (bounds)
+            {
+                blo = MRAY_BOUND_LO(bounds[pix]) / 65535.f;
+                bhi = MRAY_BOUND_HI(bounds[pix]) / 65535.f;
+            }
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            int32_t ns = 0;
+            orc_reconstruct_ray(org, dir, bmin, bmax, cam->znear, nm, pm, blo, bhi, tf, p, out, &ns);
+            if (n_steps)
+                n_steps[pix] = ns;
+        }
+    }
+}
+
+/* FP64 reconstruct: same t loop and the same float phase sequence (the
+ * reference increments the phase in float), densities, TF lerp and
+ * compositing in double with exact atan and exp. */
+void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                               const double *moments, const uint32_t *bounds, const float *tf,
+                               const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
+{
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+    {
+        double pm[MAXM];
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            double *out = rgba + pix * 4;
+            orc_prepare_moments_std_f64(m, moments + pix * (size_t)m, rp->bias, pm);
+            double blo = 0.0, bhi = 1.0;
+            if (bounds)
+            {
+                blo = MRAY_BOUND_LO(bounds[pix]) / 65535.0;
+                bhi = MRAY_BOUND_HI(bounds[pix]) / 65535.0;
+            }
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            float t_near, t_far;
+            int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
+            t_near = fmaxr(t_near, cam->znear);
+            int ns = 0;
+            out[0] = out[1] = out[2] = out[3] = 0.0;
+            if (hit && (t_far > 0))
+            {
+                float step = rp->step_size;
+                float t = t_near;
+                float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+                float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+                double L[3] = {0, 0, 0}, tau = 0.0, T = 1.0, alpha = 0.0;
+                double pc = dclampr(p, -M_PI, 0.0);
+                double fs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
+                while (t <= t_far)
+                {
+                    t += step;
+                    p += phase_step;
+                    ++ns;
+                    pc = dclampr(p, -M_PI, 0.0);
+                    double bs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
+                    double c[4];
+                    tf_sample_f64(tf, fs, bs, c);
+                    for (int k = 0; k < 3; ++k)
+                        L[k] += rp->medium_emission * c[k] * T;
+                    tau += c[3];
+                    T = exp(-tau);
+                    alpha = 1.0 - T;
+                    fs = bs;
+                    if (alpha >= rp->early_ray_termination)
+                    {
+                        alpha = 1.0;
+                        break;
+                    }
+                }
+                out[0] = L[0];
+                out[1] = L[1];
+                out[2] = L[2];
+                out[3] = alpha;
+            }
+            if (n_steps)
+                n_steps[pix] = ns;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* compaction (src/moment_compaction.h)                                      */
+/* ------------------------------------------------------------------------ */
+
+/* evaluate_unknown_moments (src/moment_compaction.h:35-60) */
+static void eval_unknown_f(int num_moments, const cf *E, int max_moments, cf *newE, cf *lev)
+{
+    int mm = max_moments - 1;
+    int n = num_moments - 1;
+    levinson_f(num_moments, lev, E, newE);
+    for (int i = 0; i < n + 1; ++i)
+        lev[i] = cf_rmul(6.28318531f, lev[i]);
+    for (int l = 0; l < n + 1; ++l)
+        newE[l] = E[l];
+    cf inv_lev0 = cf_div(cf_make(-1.0f, 0.0f), lev[0]);
+    for (int k = 0; k < mm - n; ++k)
+    {
+        newE[n + k + 1] = cf_make(0.f, 0.0f);
+        for (int j = 1; j < n + 1; ++j)
+            newE[n + k + 1] = cf_add(newE[n + k + 1], cf_mul(newE[j + k], lev[n + 1 - j]));
+        newE[n + k + 1] = cf_mul(newE[n + k + 1], inv_lev0);
+    }
+}
+
+/* exponential_to_bounded_moments, including its use of exp_moments[l]
+ * inside the j loop (src/moment_compaction.h:65-79). */
+static void exp_to_bounded_f(int m, const cf *E, cf *B, int n)
+{
+    cf gamma_0 = cf_divr(E[0], 2.f);
+    for (int l = n; l < m; ++l)
+    {
+        B[l] = cf_make(0.0f, 0.0f);
+        for (int j = 0; j < l; ++j)
+        {
+            cf c = cf_div(cf_make((float)(l - j), 0.0f), cf_rmul((float)l, gamma_0));
+            B[l] = cf_add(B[l], cf_mul(cf_mul(c, E[l]), B[l - j]));
+        }
+        cf den = cf_mul(cf_mul(cf_make(0.0f, 2.0f), cf_make((float)M_PI, 0.0f)), gamma_0);
+        B[l] = cf_sub(cf_div(E[l], den), B[l]);
+    }
+}
+
+/* rmse (src/moment_compaction.h:84-94) */
+static float rmse_f(int n, int m, const cf *pred, const float *target)
+{
+    float sum = 0.0f;
+    for (int k = n; k < m; ++k)
+    {
+        float diff = pred[k].x - target[k];
+        sum += diff * diff;
+    }
+    return sqrtf(sum);
+}
+
+void orc_evaluate_unknown_moments(int n, const float *exp_c, int m, float *new_exp_c)
+{
+    cf lev[MAXM];
+    eval_unknown_f(n, (const cf *)exp_c, m, (cf *)new_exp_c, lev);
+}
+void orc_exp_to_bounded_moments(int m, const float *exp_c, float *bounded_c, int n)
+{
+    exp_to_bounded_f(m, (const cf *)exp_c, (cf *)bounded_c, n);
+}
+float orc_rmse(int n, int m, const float *pred_c, const float *target) { return rmse_f(n, m, (const cf *)pred_c, target); }
+
+/* evaluate_rrmse (src/moment_compaction.h:99-111) */
+static float eval_rrmse_f(int n, int m, const float *moments, const cf *tE, cf *tExt, cf *tLev, float norm)
+{
+    eval_unknown_f(n, tE, m, tExt, tLev);
+    cf *ext = tLev;
+    for (int l = 0; l < n; ++l)
+        ext[l] = cf_make(moments[l], 0.0f);
+    exp_to_bounded_f(m, tExt, ext, n);
+    return rmse_f(n, m, ext, moments) * norm;
+}
+
+/* determine_best_number_moments (src/moment_compaction.h:113-141) */
+int orc_determine_best_number_moments(int m, const float *moments, float error_threshold)
+{
+    if (error_threshold == 0.0f)
+        return m;
+    cf tE[MAXM], tExt[MAXM], tLev[MAXM];
+    float norm = 1.0f / moments[0];
+    trig_to_exp_f(m, tE, moments);
+    int n = m / 2;
+    int s = m / 4;
+    while (s > 0)
+    {
+        float rrmse = eval_rrmse_f(n, m, moments, tE, tExt, tLev, norm);
+        float d = rrmse - error_threshold;
+        if (((d < 0) ? -d : d) < 1e-8f)
+            return n;
+        else if (rrmse > error_threshold)
+            n += s;
+        else
+            n -= s;
+        s = s / 2;
+    }
+    return n;
+}
+
+/* compute_num_moments (src/moment_compaction.h:147-174) */
+void orc_compute_num_moments_image(int W, int H, int m, const float *dense, const uint32_t *bounds,
+                                   float error_threshold, uint32_t *counts, int nthreads)
+{
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            const float *mo = dense + pix * (size_t)m;
+            int is_zero;
+            if (bounds)
+            {
+                float lo = MRAY_BOUND_LO(bounds[pix]) / 65535.f;
+                float hi = MRAY_BOUND_HI(bounds[pix]) / 65535.f;
+                is_zero = lo == hi;
+            }
+            else
+                is_zero = mo[0] == 0.0f;
+            int n = 0;
+            if (!is_zero)
+                n = orc_determine_best_number_moments(m, mo, error_threshold);
+            counts[pix] = (uint32_t)n;
+        }
+}
+
+/* MomentImageHost::compact (src/moment_image.cpp:69-94) */
+uint32_t orc_compact(int W, int H, int m, const float *dense, const uint32_t *counts, uint32_t *index, float *out)
+{
+    uint32_t sum = 0;
+    for (size_t pix = 0; pix < (size_t)W * H; ++pix)
+    {
+        uint32_t n = counts[pix];
+        index[pix] = sum;
+        if (out)
+            for (uint32_t l = 0; l < n; ++l)
+                out[sum + l] = dense[pix * (size_t)m + l];
+        sum += n;
+    }
+    index[(size_t)W * H] = sum;
+    return sum;
+}
+
+/* ------------------------------------------------------------------------ */
+/* prediction coding (src/moment_prediction_coding.h)                        */
+/* ------------------------------------------------------------------------ */
+
+/* encode (src/moment_prediction_coding.h:123-159) */
+static void encode_f(int m, const cf *E, cf *code, cf *P, cf *tmp)
+{
+    code[0] = E[0];
+    P[0] = cf_make(1.0f / E[0].x, 0.0f);
+    int invalid = 0;
+    for (int l = 1; l < m; ++l)
+    {
+        cf u = cf_make(0.0f, 0.0f);
+        for (int k = 0; k < l; ++k)
+            u = cf_add(u, cf_mul(P[k], E[l - k]));
+        code[l] = u;
+        float a2 = cf_abs2(u);
+        if (invalid || a2 >= 1.0f)
+        {
+            u = cf_make(0.0f, 0.0f);
+            a2 = 0;
+            invalid = 1;
+        }
+        float factor = 1.0f / (1.0f - a2);
+        for (int k = 1; k < l; ++k)
+            tmp[k] = cf_conj(P[l - k]);
+        tmp[l] = cf_make(P[0].x, 0.0f);
+        P[0] = cf_make(factor * P[0].x, 0.0f);
+        for (int k = 1; k < l; ++k)
+            P[k] = cf_rmul(factor, cf_sub(P[k], cf_mul(tmp[k], u)));
+        P[l] = cf_rmul(factor, cf_rmul(-tmp[l].x, u));
+    }
+}
+
+/* decode (src/moment_prediction_coding.h:164-206) */
+static void decode_f(int m, const cf *code, cf *E, cf *P, cf *tmp)
+{
+    E[0] = code[0];
+    P[0] = cf_make(1.0f / E[0].x, 0.0f);
+    const float eps = 1e-6f;
+    int invalid = 0;
+    for (int l = 1; l < m; ++l)
+    {
+        cf u = code[l];
+        float a2 = cf_abs2(u);
+        if (invalid || a2 >= (1.0f - eps))
+        {
+            u = cf_make(0.0f, 0.0f);
+            a2 = 0;
+            invalid = 1;
+        }
+        cf g = cf_make(0.0f, 0.0f);
+        for (int k = 1; k < l; ++k)
+            g = cf_add(g, cf_mul(P[k], E[l - k]));
+        E[l] = cf_mul(cf_div(cf_make((float)1.0, 0.0f), P[0]), cf_sub(u, g));
+        float factor = 1.0f / (1.0f - a2);
+        for (int k = 1; k < l; ++k)
+            tmp[k] = cf_conj(P[l - k]);
+        tmp[l] = cf_make(P[0].x, 0.0f);
+        P[0] = cf_make(factor * P[0].x, 0.0f);
+        for (int k = 1; k < l; ++k)
+            P[k] = cf_rmul(factor, cf_sub(P[k], cf_mul(tmp[k], u)));
+        P[l] = cf_rmul(factor, cf_rmul(-tmp[l].x, u));
+    }
+}
+
+void orc_encode(int m, const float *exp_c, float *code_c)
+{
+    cf P[MAXM], tmp[MAXM];
+    encode_f(m, (const cf *)exp_c, (cf *)code_c, P, tmp);
+}
+void orc_decode(int m, const float *code_c, float *exp_c, float *eval_poly_c)
+{
+    cf tmp[MAXM];
+    decode_f(m, (const cf *)code_c, (cf *)exp_c, (cf *)eval_poly_c, tmp);
+}
+
+/* prediction_coding_warp / _inv_warp (src/moment_prediction_coding.h:47-67) */
+static inline float pc_warp(int l, float x, int warp, const float *params)
+{
+    float w;
+    if (warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED)
+        w = (x + params[(l - 1) * 2]) / (params[(l - 1) * 2 + 1] * 2.f);
+    else
+        w = x * 0.5f + 0.5f;
+    return fclampr(w, 0.0f, 1.0f);
+}
+static inline float pc_inv_warp(int l, float x, int warp, const float *params)
+{
+    if (warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED)
+        return (x * params[(l - 1) * 2 + 1] * 2.f) - params[(l - 1) * 2];
+    return x * 2.f - 1.f;
+}
+
+/* transform_quantization_real (src/moment_prediction_coding.h:81-93) */
+void orc_transform_quantization_real(int m, const float *code_c, float c0, float *out)
+{
+    const cf *code = (const cf *)code_c;
+    out[0] = c0;
+    cf axis = cf_make(-code[0].y, code[0].x);
+    float length_axis = sqrtf(cf_abs2(axis));
+    cf inv_axis = cf_div(cf_make(length_axis, 0.0f), axis);
+    for (int l = 1; l < m; ++l)
+        out[l] = cf_mul(code[l], inv_axis).x;
+}
+
+/* transform_dequantization_real (src/moment_prediction_coding.h:98-118) */
+void orc_transform_dequantization_real(int m, const float *in, float *code_c, int warp, const float *params)
+{
+    cf *out = (cf *)code_c;
+    float c0 = in[0];
+    float ph = 3.14159265f * c0 - 1.57079633f;
+    out[0] = cf_make(cosf(ph), sinf(ph));
+    out[0] = cf_rmul(2.0f * 0.0795774715f, out[0]);
+    cf axis = cf_make(-out[0].y, out[0].x);
+    float length_axis = sqrtf(cf_abs2(axis));
+    cf factor = cf_divr(axis, length_axis);
+    for (int l = 1; l < m; ++l)
+    {
+        float ql = pc_inv_warp(l, in[l], warp, params);
+        out[l] = cf_rmul(ql, factor);
+    }
+}
+
+/* find_coding_transform_parameters (src/coding_transform.h:37-68), including
+ * the inclusion of pixels whose count equals l (these read the next pixel's
+ * first value; the read past the end of the data for the very last pixel is
+ * skipped here). */
+void orc_find_coding_params(int W, int H, int m, const float *data, const uint32_t *index, float *params)
+{
+    size_t total = index[(size_t)W * H];
+    for (int i = 0; i < (m - 1) * 2; ++i)
+        params[i] = 0.0f;
+    for (int l = 1; l < m; ++l)
+    {
+        float mn = 0.0f, mx = 0.0f;
+        for (size_t pix = 0; pix < (size_t)W * H; ++pix)
+        {
+            int num = (int)(index[pix + 1] - index[pix]);
+            if (num < l)
+                continue;
+            size_t at = index[pix] + (size_t)l;
+            if (at >= total)
+                continue;
+            float value = fclampr(data[at], -1.0f, 1.0f);
+            mn = (value < mn) ? value : mn; /* std::min(a, b) = (b < a) ? b : a */
+            mx = (mx < value) ? value : mx; /* std::max(a, b) = (a < b) ? b : a */
+        }
+        params[(l - 1) * 2] = mn;
+        params[(l - 1) * 2 + 1] = mx;
+    }
+    for (int pidx = 0; pidx < m - 1; ++pidx)
+    {
+        float d = params[pidx * 2 + 1] - params[pidx * 2];
+        params[pidx * 2 + 1] = ((d < 0) ? -d : d) * 0.5f;
+        float a = params[pidx * 2];
+        params[pidx * 2] = (a < 0) ? -a : a;
+    }
+}
+
+/* CodingParamType::quantize / dequantize (src/coding_params.h:56-78) */
+void orc_quantize_coding_params(int m, const float *params, uint16_t *q)
+{
+    for (int i = 0; i < (m - 1) * 2; ++i)
+        q[i] = (uint16_t)ceilf(fclampr(params[i], 0.0f, 1.0f) * 65535.f);
+}
+void orc_dequantize_coding_params(int m, const uint16_t *q, float *params)
+{
+    for (int i = 0; i < (m - 1) * 2; ++i)
+        params[i] = q[i] / 65535.f;
+}
+
+/* MomentImageHost::prediction_encode (src/moment_image.cpp:96-152) */
+void orc_prediction_encode_image(int W, int H, int m, float *data, const uint32_t *index, int warp, float bias,
+                                 uint16_t *coding_params_q, int nthreads)
+{
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+    {
+        cf E[MAXM], code[MAXM], P[MAXM], tmp[MAXM];
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            size_t idx = index[pix];
+            int nm = (int)(index[pix + 1] - index[pix]);
+            if (nm <= 0)
+                continue;
+            float *d = data + idx;
+            d[0] = (1 - bias) * d[0] + bias * 0.5f;
+            for (int i = 1; i < nm; ++i)
+                d[i] = (1 - bias) * d[i] + bias * 0.0f;
+            trig_to_exp_f(nm, E, d);
+            encode_f(nm, E, code, P, tmp);
+            orc_transform_quantization_real(nm, (const float *)code, d[0], d);
+        }
+    }
+    float params_f[2 * MAXM];
+    for (int i = 0; i < 2 * (m - 1); ++i)
+        params_f[i] = 0.0f;
+    if (warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED)
+    {
+        float raw[2 * MAXM];
+        orc_find_coding_params(W, H, m, data, index, raw);
+        orc_quantize_coding_params(m, raw, coding_params_q);
+        orc_dequantize_coding_params(m, coding_params_q, params_f);
+    }
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            int nm = (int)(index[pix + 1] - index[pix]);
+            if (nm <= 0)
+                continue;
+            float *d = data + index[pix];
+            for (int l = 1; l < nm; ++l)
+                d[l] = pc_warp(l, d[l], warp, params_f);
+        }
+}
+
+/* prepare_moments_from_pred_coding (src/moment_preparation.h:32-53) */
+void orc_prepare_moments_pred(int m, const float *moments, int warp, const float *params, float *pm)
+{
+    cf code[MAXM], E[MAXM], P[MAXM], tmp[MAXM];
+    orc_transform_dequantization_real(m, moments, (float *)code, warp, params);
+    decode_f(m, code, E, P, tmp);
+    for (int i = 0; i < m; ++i)
+        P[i] = cf_rmul(6.28318531f, P[i]);
+    autocorr_f(m, tmp, P);
+    E[0] = cf_rmul(0.5f, E[0]);
+    imag_corr_f(m, pm, tmp, E);
+    float nf = 1.0f / (3.14159265f * P[0].x);
+    for (int i = 0; i < m; ++i)
+        pm[i] = nf * pm[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* SPH gather (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh})            */
+/* ------------------------------------------------------------------------ */
+
+/* cubicSpline (src/cut/kernels.h:55-64) */
+float orc_cubic_spline(float x)
+{
+    const float sigma = 0.5641895835477563f;
+    if (x < 1.f)
+        return sigma * (1.f - (3.f / 2.f) * x * x * (1.f - x * 0.5f));
+    else if (x < 2.f)
+        return sigma / 4.f * (2.f - x) * (2.f - x) * (2.f - x);
+    return 0.0f;
+}
+
+typedef struct
+{
+    uint32_t sx, sy, sz; /* padded grid size */
+    float minp[3];
+    float inv_cell;
+    float r2;
+    uint32_t total;
+} sph_grid;
+
+static inline uint32_t next_multiple(uint32_t x, int mul) { return ((x - 1) / mul + 1) * mul; }
+
+/* getIndexFromCell with 4^3 tiles (src/particle_grid.cuh:64-83) */
+static inline uint32_t sph_cell_index(const sph_grid *g, int cx, int cy, int cz)
+{
+    int tsx = 4, tsy = 4, tsz = (g->sz == 1) ? 1 : 4;
+    int widthInTiles = ((int)g->sx + tsx - 1) / tsx;
+    int heightInTiles = ((int)g->sy + tsy - 1) / tsy;
+    int tileX = cx / tsx, tileY = cy / tsy, tileZ = cz / tsz;
+    int inX = cx % tsx, inY = cy % tsy, inZ = cz % tsz;
+    return (uint32_t)((tileZ * heightInTiles * widthInTiles + tileY * widthInTiles + tileX) * (tsx * tsy * tsz) +
+                      inZ * tsx * tsy + inY * tsx + inX);
+}
+
+static inline void sph_get_cell(const sph_grid *g, const float p[3], int c[3])
+{
+    for (int i = 0; i < 3; ++i)
+        c[i] = (int)((p[i] - g->minp[i]) * g->inv_cell);
+}
+
+typedef struct
+{
+    uint32_t key, idx;
+} kv;
+static int kv_cmp(const void *a, const void *b)
+{
+    const kv *x = (const kv *)a, *y = (const kv *)b;
+    if (x->key != y->key)
+        return x->key < y->key ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* Particles::create_grid + Particles::sample evaluated at the voxel positions
+ * of an nx*ny*nz grid over [dmin, dmax] (voxel i at dmin + i*(dmax-dmin)/n,
+ * the RegularGrid::sample convention). Particles are ordered by cell with a
+ * stable order inside a cell (the reference's std::sort leaves that order
+ * unspecified). */
+void orc_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
+                           const float dmin[3], const float dmax[3], int nx, int ny, int nz, float *grid,
+                           int nthreads)
+{
+    /* createGridFromDomainCPU (src/particle_grid.cpp:38-55) + createGridFromDomain (.cuh:389-400) */
+    float mn[3] = {x[0], y[0], z[0]}, mx[3] = {x[0], y[0], z[0]};
+    for (uint32_t i = 1; i < n; ++i)
+    {
+        const float q[3] = {x[i], y[i], z[i]};
+        for (int k = 0; k < 3; ++k)
+        {
+            mn[k] = (q[k] < mn[k]) ? q[k] : mn[k];
+            mx[k] = (mx[k] < q[k]) ? q[k] : mx[k];
+        }
+    }
+    sph_grid g;
+    float cell = h;
+    float inv_cellf = 1 / cell; /* Vec3 / scalar multiplies by the reciprocal (src/cut/cuda_math.h:467-472) */
+    uint32_t gs[3];
+    for (int k = 0; k < 3; ++k)
+        gs[k] = (uint32_t)((mx[k] - mn[k]) * inv_cellf + 1);
+    g.sx = next_multiple(gs[0], 4);
+    g.sy = next_multiple(gs[1], 4);
+    g.sz = (gs[2] > 1) ? next_multiple(gs[2], 4) : gs[2];
+    for (int k = 0; k < 3; ++k)
+        g.minp[k] = mn[k];
+    g.inv_cell = (float)(1.0 / (double)cell);
+    g.r2 = cell * cell;
+    g.total = g.sx * g.sy * g.sz;
+
+    kv *keys = (kv *)malloc(sizeof(kv) * n);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const float q[3] = {x[i], y[i], z[i]};
+        int c[3];
+        sph_get_cell(&g, q, c);
+        keys[i].key = sph_cell_index(&g, c[0], c[1], c[2]);
+        keys[i].idx = i;
+    }
+    qsort(keys, n, sizeof(kv), kv_cmp);
+    float *sx = (float *)malloc(sizeof(float) * n * 4);
+    float *sy = sx + n, *sz = sy + n, *sv = sz + n;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        sx[i] = x[keys[i].idx];
+        sy[i] = y[keys[i].idx];
+        sz[i] = z[keys[i].idx];
+        sv[i] = v[keys[i].idx];
+    }
+    uint32_t *start = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)g.total);
+    uint32_t *count = (uint32_t *)calloc((size_t)g.total, sizeof(uint32_t));
+    for (uint32_t c = 0; c < g.total; ++c)
+        start[c] = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        uint32_t c = keys[i].key;
+        if (i == 0 || c != keys[i - 1].key)
+            start[c] = i;
+        count[c]++;
+    }
+    free(keys);
+
+    float inv_h = 1.f / h;
+#pragma omp parallel for num_threads(set_threads(nthreads)) collapse(2)
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix)
+            {
+                float p[3];
+                const int ii[3] = {ix, iy, iz};
+                const int nn[3] = {nx, ny, nz};
+                for (int k = 0; k < 3; ++k)
+                    p[k] = dmin[k] + (float)ii[k] / (float)nn[k] * (dmax[k] - dmin[k]);
+                int c[3];
+                sph_get_cell(&g, p, c);
+                float value = 0.0f, wsum = 0.0f;
+                int z0 = (c[2] - 1 > 0) ? c[2] - 1 : 0, z1 = (c[2] + 2 < (int)g.sz) ? c[2] + 2 : (int)g.sz;
+                int y0 = (c[1] - 1 > 0) ? c[1] - 1 : 0, y1 = (c[1] + 2 < (int)g.sy) ? c[1] + 2 : (int)g.sy;
+                int x0 = (c[0] - 1 > 0) ? c[0] - 1 : 0, x1 = (c[0] + 2 < (int)g.sx) ? c[0] + 2 : (int)g.sx;
+                for (int cz = z0; cz < z1; ++cz)
+                    for (int cy = y0; cy < y1; ++cy)
+                        for (int cx = x0; cx < x1; ++cx)
+                        {
+                            uint32_t ci = sph_cell_index(&g, cx, cy, cz);
+                            if (start[ci] == 0xFFFFFFFFu)
+                                continue;
+                            for (uint32_t q = start[ci]; q < start[ci] + count[ci]; ++q)
+                            {
+                                float dx = p[0] - sx[q], dy = p[1] - sy[q], dz = p[2] - sz[q];
+                                float d2 = dx * dx + dy * dy + dz * dz;
+                                if (d2 < g.r2)
+                                {
+                                    /* InterpolateParticlesOp (src/particle_interpolate.h:45-51) */
+                                    float ex = sx[q] - p[0], ey = sy[q] - p[1], ez = sz[q] - p[2];
+                                    float w = orc_cubic_spline(sqrtf(ex * ex + ey * ey + ez * ez) * inv_h);
+                                    wsum += w;
+                                    value += w * sv[q];
+                                }
+                            }
+                        }
+                if (wsum != 0.0f)
+                    value /= wsum;
+                grid[((size_t)iz * ny + iy) * nx + ix] = value;
+            }
+    free(sx);
+    free(start);
+    free(count);
+}

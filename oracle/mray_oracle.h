@@ -1,0 +1,128 @@
+/*
+ * CPU oracle for the moment-image hot path. TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain-C restatement of the reference's CPU algorithm
+ * (TobiasRp/mray, src/...) used as the checker for the HIP kernels, for the
+ * golden-vector tests and as the timed CPU baseline in bench.py. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product (mray_amd) never does.
+ *
+ * Two precisions are provided:
+ *   - *_f32 / unsuffixed: reference-faithful FP32, reproducing the reference's
+ *     operation order and its float/double promotions (including the
+ *     double-precision cos/sin/exp that the reference's unqualified calls
+ *     resolve to). Pinned bit-exact against the compiled reference
+ *     (oracle/_ref) by tests/test_oracle_ref.py.
+ *   - *_f64: the same discretisation (identical sample positions, loop counts
+ *     and float phase sequence) evaluated in FP64 with exact cos/sin/atan/exp.
+ *     This is the "FP64 oracle" the precision targets are stated against.
+ *
+ * Each function cites the reference file:line it restates.
+ */
+#ifndef MRAY_ORACLE_H
+#define MRAY_ORACLE_H
+
+#include "../include/mray_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_volume
+{
+    const void *data; /* F32: float[nz][ny][nx], U8: uint8_t[nz][ny][nx] */
+    int32_t dtype;    /* MRAY_DTYPE_F32 / MRAY_DTYPE_U8 */
+    int32_t nx, ny, nz;
+    float dmin[3], dmax[3];
+} orc_volume;
+
+/* ---- camera / rays (src/cut/raytracing.h:39-86) ---- */
+void orc_eye_ray(const mray_camera *cam, float ndcx, float ndcy, float org[3], float dir[3]);
+int orc_intersect_aabb(const float org[3], const float dir[3], const float bmin[3], const float bmax[3],
+                       float *t_near, float *t_far);
+void orc_pixel_ray(const mray_camera *cam, int W, int H, int x, int y, float org[3], float dir[3]);
+
+/* ---- volume (src/volume.h:59-123) ---- */
+float orc_volume_sample(const orc_volume *v, const float p[3]);
+double orc_volume_sample_f64(const orc_volume *v, const double p[3]);
+
+/* ---- generate (src/raymarch_generate.h:35-148, src/moment_raymarcher.cpp:121-166) ---- */
+uint32_t orc_generate_ray(const orc_volume *v, const float org[3], const float dir[3], float znear, float *moments,
+                          int m, const mray_gen_params *p, int32_t *n_samples);
+void orc_generate_image(const orc_volume *v, const mray_camera *cam, int W, int H, int m, const mray_gen_params *p,
+                        float *moments, uint32_t *bounds, int32_t *n_samples, int row0, int row_step, int nthreads);
+uint32_t orc_generate_ray_f64(const orc_volume *v, const float org[3], const float dir[3], float znear,
+                              double *moments, int m, const mray_gen_params *p);
+void orc_generate_image_f64(const orc_volume *v, const mray_camera *cam, int W, int H, int m,
+                            const mray_gen_params *p, double *moments, uint32_t *bounds, int row0, int row_step,
+                            int nthreads);
+
+/* ---- bounded MESE, FP32 (src/mese/MESE_dynamic.h, src/mese/complex_algebra.h) ----
+ * complex arrays are interleaved (re, im) float pairs. */
+float orc_fast_atan(float t);
+void orc_trig_to_exp(int m, const float *trig, float *exp_c);
+void orc_trig_to_exp_c(int m, const float *trig_c, float *exp_c);
+void orc_exp_to_trig(int m, const float *exp_c, float *trig_c);
+void orc_levinson(int m, const float *first_col_c, float *sol_c);
+void orc_autocorrelation(int m, const float *sig_c, float *out_c);
+void orc_correlation(int m, const float *lhs_c, const float *rhs_c, float *out_c);
+void orc_imag_correlation(int m, const float *lhs_c, const float *rhs_c, float *out);
+void orc_prepare_mese(int m, const float *trig, float *lagrange);
+void orc_prepare_mese_c(int m, const float *trig_c, float *lagrange_c);
+float orc_evaluate_fourier(int m, float cx, float cy, const float *coeff);
+float orc_evaluate_mese(int m, float phase, const float *lagrange);
+void orc_prepare_moments_std(int m, const float *moments, float bias, float *pmoments);
+
+/* ---- bounded MESE, FP64 ---- */
+void orc_prepare_moments_std_f64(int m, const double *moments, double bias, double *pmoments);
+double orc_evaluate_mese_f64(int m, double phase, const double *lagrange);
+
+/* ---- transfer function (src/transfer_function.{h,cpp}) ---- */
+void orc_tf_from_rgba8(const uint8_t *rgba8, float step_size, float *tf);
+void orc_tf_sample(const float *tf, float fs, float bs, float out[4]);
+
+/* ---- reconstruct (src/raymarch_reconstruct.h:127-200, src/moment_raymarcher.cpp:283-327) ---- */
+void orc_reconstruct_ray(const float org[3], const float dir[3], const float bmin[3], const float bmax[3], float znear,
+                         int m, const float *pmoments, float blo, float bhi, const float *tf,
+                         const mray_rec_params *p, float out[4], int32_t *n_steps);
+void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                           const float *data, const uint32_t *index, const uint32_t *bounds, int prediction_coded,
+                           int coding_warp, const float *coding_params, const float *tf, const mray_rec_params *p,
+                           float *rgba, int32_t *n_steps, int row0, int row_step, int nthreads);
+void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                               const double *moments, const uint32_t *bounds, const float *tf,
+                               const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);
+
+/* ---- compaction (src/moment_compaction.h, src/moment_image.cpp:69-94) ---- */
+void orc_evaluate_unknown_moments(int n, const float *exp_c, int m, float *new_exp_c);
+void orc_exp_to_bounded_moments(int m, const float *exp_c, float *bounded_c, int n);
+float orc_rmse(int n, int m, const float *pred_c, const float *target);
+int orc_determine_best_number_moments(int m, const float *moments, float error_threshold);
+void orc_compute_num_moments_image(int W, int H, int m, const float *dense, const uint32_t *bounds,
+                                   float error_threshold, uint32_t *counts, int nthreads);
+uint32_t orc_compact(int W, int H, int m, const float *dense, const uint32_t *counts, uint32_t *index, float *out);
+
+/* ---- prediction coding (src/moment_prediction_coding.h, src/coding_transform.h, src/coding_params.h,
+ *      src/moment_image.cpp:96-152, src/moment_preparation.h:32-53) ---- */
+void orc_encode(int m, const float *exp_c, float *code_c);
+void orc_decode(int m, const float *code_c, float *exp_c, float *eval_poly_c);
+void orc_transform_quantization_real(int m, const float *code_c, float c0, float *out);
+void orc_transform_dequantization_real(int m, const float *in, float *code_c, int warp, const float *params);
+void orc_find_coding_params(int W, int H, int m, const float *data, const uint32_t *index, float *params);
+void orc_quantize_coding_params(int m, const float *params, uint16_t *q);
+void orc_dequantize_coding_params(int m, const uint16_t *q, float *params);
+void orc_prediction_encode_image(int W, int H, int m, float *data, const uint32_t *index, int warp, float bias,
+                                 uint16_t *coding_params_q, int nthreads);
+void orc_prepare_moments_pred(int m, const float *moments, int warp, const float *params, float *pmoments);
+
+/* ---- SPH gather (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh}, src/particle_interpolate.h) ---- */
+float orc_cubic_spline(float q);
+void orc_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
+                           const float dmin[3], const float dmax[3], int nx, int ny, int nz, float *grid,
+                           int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,218 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so) and, where it was
+built, the compiled reference (oracle/_ref/libmrayref.so). Test-side only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+from mray_amd.types import Camera, GenParams, RecParams
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libmrayref.so"
+
+_f = C.POINTER(C.c_float)
+_d = C.POINTER(C.c_double)
+_u32 = C.POINTER(C.c_uint32)
+_i32 = C.POINTER(C.c_int32)
+_u16 = C.POINTER(C.c_uint16)
+_u8 = C.POINTER(C.c_uint8)
+
+
+class OrcVolume(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("dtype", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("nz", C.c_int32), ("dmin", C.c_float * 3), ("dmax", C.c_float * 3)]
+
+
+def ptr(a, t):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)
+
+
+def _load(path):
+    return C.CDLL(str(path)) if path.exists() else None
+
+
+def oracle():
+    lib = _load(ORACLE_SO)
+    if lib is None:
+        raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+    return lib
+
+
+def reference():
+    return _load(REF_SO)
+
+
+def nthreads():
+    return int(os.environ.get("MRAY_ORACLE_THREADS", "0"))
+
+
+class Oracle:
+    """numpy-level wrappers over liboracle.so."""
+
+    def __init__(self):
+        self.lib = oracle()
+        L = self.lib
+        L.orc_fast_atan.restype = C.c_float
+        L.orc_fast_atan.argtypes = [C.c_float]
+        L.orc_evaluate_mese.restype = C.c_float
+        L.orc_evaluate_mese.argtypes = [C.c_int, C.c_float, _f]
+        L.orc_evaluate_mese_f64.restype = C.c_double
+        L.orc_evaluate_mese_f64.argtypes = [C.c_int, C.c_double, _d]
+        L.orc_determine_best_number_moments.restype = C.c_int
+        L.orc_determine_best_number_moments.argtypes = [C.c_int, _f, C.c_float]
+        L.orc_rmse.restype = C.c_float
+        L.orc_compact.restype = C.c_uint32
+        L.orc_volume_sample.restype = C.c_float
+        L.orc_cubic_spline.restype = C.c_float
+        L.orc_cubic_spline.argtypes = [C.c_float]
+        L.orc_generate_ray.restype = C.c_uint32
+
+    @staticmethod
+    def volume(vol: np.ndarray, dmin, dmax):
+        v = OrcVolume()
+        v.data = vol.ctypes.data
+        v.dtype = 1 if vol.dtype == np.uint8 else 0
+        v.nz, v.ny, v.nx = vol.shape
+        for i in range(3):
+            v.dmin[i] = dmin[i]
+            v.dmax[i] = dmax[i]
+        v._keep = vol
+        return v
+
+    def generate(self, vol, dmin, dmax, cam: dict, W, H, m, gp: GenParams, row0=0, row_step=1, f64=False):
+        v = self.volume(vol, dmin, dmax)
+        c = Camera.from_dict(cam)
+        bounds = np.zeros(W * H, np.uint32)
+        if f64:
+            mo = np.zeros((H, W, m), np.float64)
+            self.lib.orc_generate_image_f64(C.byref(v), C.byref(c), W, H, m, C.byref(gp), ptr(mo, _d),
+                                            ptr(bounds, _u32), row0, row_step, nthreads())
+            return mo, bounds
+        mo = np.zeros((H, W, m), np.float32)
+        ns = np.zeros(W * H, np.int32)
+        self.lib.orc_generate_image(C.byref(v), C.byref(c), W, H, m, C.byref(gp), ptr(mo, _f), ptr(bounds, _u32),
+                                    ptr(ns, _i32), row0, row_step, nthreads())
+        return mo, bounds, ns
+
+    def tf(self, rgba8: np.ndarray, step: float) -> np.ndarray:
+        out = np.zeros((256, 4), np.float32)
+        self.lib.orc_tf_from_rgba8(ptr(np.ascontiguousarray(rgba8, np.uint8), _u8), C.c_float(step), ptr(out, _f))
+        return out
+
+    def reconstruct(self, cam, dmin, dmax, W, H, m, data, tf, rp: RecParams, index=None, bounds=None,
+                    prediction_coded=0, coding_warp=0, coding_params=None, row0=0, row_step=1):
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float32)
+        ns = np.zeros(W * H, np.int32)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_reconstruct_image(C.byref(c), bmin, bmax, W, H, m, ptr(data, _f), ptr(index, _u32),
+                                       ptr(bounds, _u32), prediction_coded, coding_warp, ptr(coding_params, _f),
+                                       ptr(tf, _f), C.byref(rp), ptr(rgba, _f), ptr(ns, _i32), row0, row_step,
+                                       nthreads())
+        return rgba, ns
+
+    def reconstruct_f64(self, cam, dmin, dmax, W, H, m, moments64, tf, rp: RecParams, bounds=None):
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float64)
+        ns = np.zeros(W * H, np.int32)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_reconstruct_image_f64(C.byref(c), bmin, bmax, W, H, m, ptr(moments64, _d), ptr(bounds, _u32),
+                                           ptr(tf, _f), C.byref(rp), ptr(rgba, _d), ptr(ns, _i32), nthreads())
+        return rgba, ns
+
+    def prepare_std(self, m, moments, bias):
+        out = np.zeros(m, np.float32)
+        self.lib.orc_prepare_moments_std(m, ptr(np.ascontiguousarray(moments, np.float32), _f), C.c_float(bias),
+                                         ptr(out, _f))
+        return out
+
+    def prepare_std_f64(self, m, moments, bias):
+        out = np.zeros(m, np.float64)
+        self.lib.orc_prepare_moments_std_f64(m, ptr(np.ascontiguousarray(moments, np.float64), _d),
+                                             C.c_double(bias), ptr(out, _d))
+        return out
+
+    def num_moments(self, W, H, m, dense, bounds, thr):
+        counts = np.zeros(W * H, np.uint32)
+        self.lib.orc_compute_num_moments_image(W, H, m, ptr(dense, _f), ptr(bounds, _u32), C.c_float(thr),
+                                               ptr(counts, _u32), nthreads())
+        return counts
+
+    def compact(self, W, H, m, dense, counts):
+        index = np.zeros(W * H + 1, np.uint32)
+        total = int(counts.astype(np.int64).sum())
+        out = np.zeros(max(total, 1), np.float32)
+        self.lib.orc_compact(W, H, m, ptr(dense, _f), ptr(counts, _u32), ptr(index, _u32), ptr(out, _f))
+        return index, out[:total]
+
+    def prediction_encode(self, W, H, m, data, index, warp, bias):
+        d = np.ascontiguousarray(data, np.float32).copy()
+        q = np.zeros(max(2 * (m - 1), 1), np.uint16)
+        self.lib.orc_prediction_encode_image(W, H, m, ptr(d, _f), ptr(index, _u32), warp, C.c_float(bias),
+                                             ptr(q, _u16), nthreads())
+        return d, q
+
+    def particles_to_grid(self, x, y, z, v, h, dmin, dmax, nx, ny, nz):
+        g = np.zeros((nz, ny, nx), np.float32)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_particles_to_grid(ptr(x, _f), ptr(y, _f), ptr(z, _f), ptr(v, _f), C.c_uint32(len(x)),
+                                       C.c_float(h), bmin, bmax, nx, ny, nz, ptr(g, _f), nthreads())
+        return g
+
+
+class Reference:
+    """numpy-level wrappers over the compiled reference (build container only)."""
+
+    def __init__(self):
+        self.lib = reference()
+        if self.lib is None:
+            raise RuntimeError("oracle/_ref/libmrayref.so not built")
+        L = self.lib
+        L.ref_generate.restype = C.c_long
+        L.ref_evaluate_mese.restype = C.c_float
+        L.ref_evaluate_mese.argtypes = [C.c_int, C.c_float, _f]
+        L.ref_fast_atan.restype = C.c_float
+        L.ref_fast_atan.argtypes = [C.c_float]
+        L.ref_best_num_moments.restype = C.c_int
+        L.ref_best_num_moments.argtypes = [C.c_int, _f, C.c_float]
+        L.ref_fixed_evaluate4.restype = C.c_float
+        L.ref_fixed_evaluate4.argtypes = [C.c_float, _f]
+        L.ref_fixed_evaluate10.restype = C.c_float
+        L.ref_fixed_evaluate10.argtypes = [C.c_float, _f]
+
+    def generate(self, vol, dmin, dmax, cam, W, H, m, gp: GenParams):
+        c = Camera.from_dict(cam)
+        data = np.zeros(W * H * m, np.float32)
+        bounds = np.zeros(W * H, np.uint32)
+        index = np.zeros(W * H + 1, np.uint32)
+        cp = np.zeros(max(2 * (m - 1), 1), np.uint16)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        nz, ny, nx = vol.shape
+        n = self.lib.ref_generate(vol.ctypes.data_as(C.c_void_p), 1 if vol.dtype == np.uint8 else 0, nx, ny, nz,
+                                  bmin, bmax, C.byref(c), W, H, m, C.byref(gp), ptr(data, _f), ptr(bounds, _u32),
+                                  ptr(index, _u32), ptr(cp, _u16))
+        return data[:n], bounds, index, cp
+
+    def reconstruct(self, cam, dmin, dmax, W, H, m, data, tf_rgba8, tf_step, rp: RecParams, index=None,
+                    bounds=None, prediction_coded=0, coding_warp=0, coding_params=None, want_rgba8=False):
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float32)
+        rgba8 = np.zeros((H, W, 4), np.uint8) if want_rgba8 else None
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.ref_reconstruct(C.byref(c), bmin, bmax, W, H, m, ptr(data, _f), C.c_long(len(data)),
+                                 ptr(index, _u32), ptr(bounds, _u32), prediction_coded, coding_warp,
+                                 ptr(coding_params, _u16), ptr(np.ascontiguousarray(tf_rgba8, np.uint8), _u8),
+                                 C.c_float(tf_step), C.byref(rp), ptr(rgba, _f), ptr(rgba8, _u8))
+        return rgba, rgba8
