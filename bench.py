@@ -1,0 +1,211 @@
+"""Moment-image benchmark on one MI355X.
+
+Workload (the precision-study configuration): synthetic 1024^3 fp32
+mixing-layer volume, perspective camera, 1024x1024 rays, 50 moments,
+step 0.5 voxel. One step = generate + prepare + reconstruct of one frame
+with the volume already resident in HBM.
+
+Prints ONE JSON line: Mrays/s of the MIXED form (value), the PLAIN form
+beside it, per-kernel times, the roofline of the dominant kernel and a CPU
+baseline (the C oracle, FP32 reference arithmetic, OpenMP) timed on a bounded
+row sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed) FLOP/s
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024, help="volume edge")
+    ap.add_argument("--res", type=int, default=1024, help="image edge")
+    ap.add_argument("--moments", type=int, default=50)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-row-step", type=int, default=64, help="CPU baseline: every k-th row")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def flop_generate(ng, m):
+    # contract formula, generate part: N_g (8(m-1) + 30) per ray
+    return float(np.sum(ng.astype(np.float64))) * (8 * (m - 1) + 30)
+
+
+def flop_reconstruct(nr, m):
+    return float(np.sum(nr.astype(np.float64))) * (8 * (m - 1) + 48)
+
+
+def flop_prepare(npix, m):
+    return float(npix) * 20 * m * m
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is described in DESIGN.md "
+                                   "and not run", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from mray_amd import hip, synth, tf as tfmod
+    from mray_amd.types import GenParams, RecParams
+
+    hip.init(0)
+    n, W, H, m = args.n, args.res, args.res, args.moments
+    dmin, dmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=25.0, znear=0.01, aspect=W / H)
+    step = 0.5 / n
+    gp = GenParams(step_size=step, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0, bias=1e-4)
+    rp = RecParams(step_size=step, bias=1e-4, early_ray_termination=0.99, medium_emission=1.0)
+    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size)).cuda()
+
+    print(f"[bench] generating {n}^3 mixing layer on the GPU", file=sys.stderr, flush=True)
+    vox = hip.synth_mixing_layer(n, seed=1)
+    vol = hip.Volume(vox, dmin, dmax)
+    torch.cuda.synchronize()
+
+    mo = torch.empty((H, W, m), dtype=torch.float32, device="cuda")
+    lag = torch.empty_like(mo)
+    bnd = torch.empty(H * W, dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+
+    # untimed pass: per-ray sample counts for the algorithmic FLOP count
+    _, _, ng = hip.generate(vol, cam, gp, W, H, m, form="mixed", moments=mo, bounds=bnd, want_samples=True)
+    hip.prepare(mo, W, H, m, rp.bias, form="mixed", out=lag)
+    _, nr = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form="mixed", bounds=bnd, rgba=rgba,
+                            want_steps=True)
+    torch.cuda.synchronize()
+    ng, nr = ng.cpu().numpy(), nr.cpu().numpy()
+
+    def run(form, steps, warmup):
+        ev = {k: [] for k in ("gen", "prep", "rec")}
+        for i in range(warmup + steps):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e[0].record()
+            hip.generate(vol, cam, gp, W, H, m, form=form, moments=mo, bounds=bnd)
+            e[1].record()
+            hip.prepare(mo, W, H, m, rp.bias, form=form, out=lag)
+            e[2].record()
+            hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form=form, bounds=bnd, rgba=rgba)
+            e[3].record()
+            if i >= warmup:
+                ev["gen"].append(e)
+        torch.cuda.synchronize()
+        return ev
+
+    print(f"[bench] mixed: {args.warmup} warmup + {args.steps} timed steps", file=sys.stderr, flush=True)
+    run("mixed", 0, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev = run("mixed", args.steps, 0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_gen = np.mean([e[0].elapsed_time(e[1]) for e in ev["gen"]])
+    ms_prep = np.mean([e[1].elapsed_time(e[2]) for e in ev["gen"]])
+    ms_rec = np.mean([e[2].elapsed_time(e[3]) for e in ev["gen"]])
+    ms_step = wall * 1e3 / args.steps
+    mrays = W * H / (ms_step * 1e-3) / 1e6
+
+    print(f"[bench] plain: 1 warmup + {args.plain_steps} timed steps", file=sys.stderr, flush=True)
+    run("plain", 0, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evp = run("plain", args.plain_steps, 0)
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - t0
+    ms_step_p = wall_p * 1e3 / args.plain_steps
+    mrays_p = W * H / (ms_step_p * 1e-3) / 1e6
+    p_gen = np.mean([e[0].elapsed_time(e[1]) for e in evp["gen"]])
+    p_prep = np.mean([e[1].elapsed_time(e[2]) for e in evp["gen"]])
+    p_rec = np.mean([e[2].elapsed_time(e[3]) for e in evp["gen"]])
+
+    f_gen, f_rec, f_prep = flop_generate(ng, m), flop_reconstruct(nr, m), flop_prepare(W * H, m)
+    kernels = {"generate": (ms_gen, f_gen), "reconstruct": (ms_rec, f_rec), "prepare": (ms_prep, f_prep)}
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    ms_dom, f_dom = kernels[dom]
+    achieved = f_dom / (ms_dom * 1e-3)
+    total_flop = f_gen + f_rec + f_prep
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step)
+
+    line = {
+        "metric": "Mrays/s gen+reconstruct, mixed vs plain form at RMSE<=1e-4; adjoint/forward",
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"mixing layer {n}^3 f32 (resident in HBM), perspective {W}x{H} rays, {m} moments, "
+                               f"step 0.5 voxel; generate + prepare + reconstruct, mixed form",
+                   "volume": [n, n, n], "image": [W, H], "moments": m, "form": "mixed"},
+        "plain": {"value": round(mrays_p, 3), "ms_per_step": round(ms_step_p, 3),
+                  "ms": {"generate": round(p_gen, 3), "prepare": round(p_prep, 3), "reconstruct": round(p_rec, 3)}},
+        "mixed_over_plain": round(mrays / mrays_p, 3),
+        "ms": {"generate": round(ms_gen, 3), "prepare": round(ms_prep, 3), "reconstruct": round(ms_rec, 3)},
+        "samples_per_ray": {"generate": round(float(ng.mean()), 1), "reconstruct": round(float(nr.mean()), 1)},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved / 1e12, 3),
+                     "peak": PEAK_FP32_VALU / 1e12, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_VALU, 4), "traffic": None},
+        "roofline_frame": {"achieved_TFLOPs": round(total_flop / (ms_step * 1e-3) / 1e12, 3),
+                           "frac": round(total_flop / (ms_step * 1e-3) / PEAK_FP32_VALU, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, row_step):
+    """The C oracle (reference FP32 arithmetic, OpenMP over rows) on every
+    row_step-th row of the same frame; extrapolated to Mrays/s."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_ffi import Oracle
+    from mray_amd import synth, tf as tfmod
+
+    O = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    os.environ["MRAY_ORACLE_THREADS"] = str(threads)
+    print(f"[bench] cpu baseline: copying volume to host, {threads} threads, every {row_step}th row",
+          file=sys.stderr, flush=True)
+    vol = vox.cpu().numpy()
+    table = tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size)
+    t0 = time.perf_counter()
+    mo, b, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp, row0=0, row_step=row_step)
+    t1 = time.perf_counter()
+    rgba, _ = O.reconstruct(cam, dmin, dmax, W, H, m, mo.reshape(-1), table, rp, bounds=b, row0=0,
+                            row_step=row_step)
+    t2 = time.perf_counter()
+    rows = (H + row_step - 1) // row_step
+    rays = rows * W
+    return {"value": round(rays / (t2 - t0) / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"rows 0,{row_step},... ({rays} of {W * H} rays) of the same frame; generate "
+                      f"{t1 - t0:.1f} s + prepare/reconstruct {t2 - t1:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,230 @@
+// C ABI of libmray_hip.so (declared in include/mray_hip.h): argument
+// validation, device/volume handles, status codes and the thread-local error
+// message. Kernels live in generate.hip, prepare.hip, reconstruct.hip,
+// synth.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/mray_hip.h"
+#include "launch.hpp"
+
+namespace mray
+{
+void set_lds_limit(size_t bytes);
+}
+
+struct mray_volume_s
+{
+    const void *data;
+    bool owned;
+    int dtype;
+    int nx, ny, nz;
+    float dmin[3], dmax[3];
+    int device;
+};
+
+namespace
+{
+thread_local char g_err[512] = "";
+bool g_init = false;
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(MRAY_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return MRAY_OK;
+}
+
+int check_status(int rc, const char *what)
+{
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "%s: configuration not supported by this build", what);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "%s: device allocation failed", what);
+    if (rc != 0)
+        return fail(rc, "%s: failed (%d)", what, rc);
+    return check_launch(what);
+}
+
+bool bad_image(int W, int H, int m) { return W <= 0 || H <= 0 || m <= 0 || m > 255 || (long long)W * H > (1LL << 31); }
+} // namespace
+
+extern "C" {
+
+const char *mray_hip_last_error(void) { return g_err; }
+
+int mray_hip_init(int device)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MRAY_ERR_HIP, "mray_hip_init: no HIP device available");
+    if (device < 0 || device >= count)
+        return fail(MRAY_ERR_ARG, "mray_hip_init: device %d out of range [0, %d)", device, count);
+    if (hipSetDevice(device) != hipSuccess)
+        return fail(MRAY_ERR_HIP, "mray_hip_init: hipSetDevice(%d) failed", device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return fail(MRAY_ERR_HIP, "mray_hip_init: hipGetDeviceProperties failed");
+    mray::set_lds_limit(prop.sharedMemPerBlock);
+    g_init = true;
+    return MRAY_OK;
+}
+
+int mray_hip_volume_create(const void *host_voxels, int dtype, int nx, int ny, int nz, const float dom_min[3],
+                           const float dom_max[3], mray_volume *out)
+{
+    if (!out || !host_voxels || !dom_min || !dom_max)
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_create: null argument");
+    if ((dtype != MRAY_DTYPE_F32 && dtype != MRAY_DTYPE_U8) || nx <= 0 || ny <= 0 || nz <= 0)
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_create: bad dtype or size");
+    size_t bytes = (size_t)nx * ny * nz * (dtype == MRAY_DTYPE_F32 ? 4 : 1);
+    void *d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess)
+        return fail(MRAY_ERR_OOM, "mray_hip_volume_create: hipMalloc(%zu) failed", bytes);
+    if (hipMemcpy(d, host_voxels, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    {
+        (void)hipFree(d);
+        return fail(MRAY_ERR_HIP, "mray_hip_volume_create: upload failed");
+    }
+    int rc = mray_hip_volume_wrap(d, dtype, nx, ny, nz, dom_min, dom_max, out);
+    if (rc != MRAY_OK)
+    {
+        (void)hipFree(d);
+        return rc;
+    }
+    (*out)->owned = true;
+    return MRAY_OK;
+}
+
+int mray_hip_volume_wrap(const void *d_voxels, int dtype, int nx, int ny, int nz, const float dom_min[3],
+                         const float dom_max[3], mray_volume *out)
+{
+    if (!out || !d_voxels || !dom_min || !dom_max)
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_wrap: null argument");
+    if ((dtype != MRAY_DTYPE_F32 && dtype != MRAY_DTYPE_U8) || nx <= 0 || ny <= 0 || nz <= 0)
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_wrap: bad dtype or size");
+    if ((long long)nx * ny > (1LL << 31))
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_wrap: nx*ny must fit in int");
+    for (int i = 0; i < 3; ++i)
+        if (!(dom_max[i] > dom_min[i]))
+            return fail(MRAY_ERR_ARG, "mray_hip_volume_wrap: empty domain on axis %d", i);
+    auto *v = new (std::nothrow) mray_volume_s;
+    if (!v)
+        return fail(MRAY_ERR_OOM, "mray_hip_volume_wrap: out of host memory");
+    v->data = d_voxels;
+    v->owned = false;
+    v->dtype = dtype;
+    v->nx = nx;
+    v->ny = ny;
+    v->nz = nz;
+    std::memcpy(v->dmin, dom_min, sizeof(v->dmin));
+    std::memcpy(v->dmax, dom_max, sizeof(v->dmax));
+    (void)hipGetDevice(&v->device);
+    *out = v;
+    return MRAY_OK;
+}
+
+int mray_hip_volume_destroy(mray_volume vol)
+{
+    if (!vol)
+        return MRAY_OK;
+    int rc = MRAY_OK;
+    if (vol->owned && hipFree(const_cast<void *>(vol->data)) != hipSuccess)
+        rc = fail(MRAY_ERR_HIP, "mray_hip_volume_destroy: hipFree failed");
+    delete vol;
+    return rc;
+}
+
+const void *mray_hip_volume_data(mray_volume vol) { return vol ? vol->data : nullptr; }
+
+int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int form, int W, int H,
+                      int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_generate: call mray_hip_init first");
+    if (!vol || !cam || !p || !d_moments)
+        return fail(MRAY_ERR_ARG, "mray_hip_generate: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_generate: bad image size %dx%d or moment count %d", W, H, m);
+    if (form != MRAY_FORM_PLAIN && form != MRAY_FORM_MIXED)
+        return fail(MRAY_ERR_ARG, "mray_hip_generate: unknown form %d", form);
+    if (p->transform_to_bounds && !d_bounds)
+        return fail(MRAY_ERR_ARG, "mray_hip_generate: transform_to_bounds needs d_bounds");
+    if (!(p->step_size > 0.0f) || !(p->data_max != p->data_min))
+        return fail(MRAY_ERR_ARG, "mray_hip_generate: step_size must be > 0 and data_max != data_min");
+    int rc = mray::launch_generate(vol->data, vol->dtype, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax, *cam, *p,
+                                   form, W, H, m, d_moments, d_bounds, d_samples, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_generate");
+}
+
+int mray_hip_prepare(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias, int form,
+                     float *d_lagrange, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_prepare: call mray_hip_init first");
+    if (!d_moments || !d_lagrange)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare: bad image size %dx%d or moment count %d", W, H, m);
+    if (form != MRAY_FORM_PLAIN && form != MRAY_FORM_MIXED)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare: unknown form %d", form);
+    int rc = mray::launch_prepare(d_moments, d_index, W, H, m, bias, form, d_lagrange, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_prepare");
+}
+
+int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                         const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W, int H,
+                         int m, const float *d_tf, const mray_rec_params *p, int form, float *d_rgba,
+                         int32_t *d_steps, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_reconstruct: call mray_hip_init first");
+    if (!cam || !dom_min || !dom_max || !d_lagrange || !d_tf || !p || !d_rgba)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct: bad image size %dx%d or moment count %d", W, H, m);
+    if (form != MRAY_FORM_PLAIN && form != MRAY_FORM_MIXED)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct: unknown form %d", form);
+    if (!(p->step_size > 0.0f))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct: step_size must be > 0");
+    int rc = mray::launch_reconstruct(*cam, dom_min, dom_max, d_lagrange, d_index, d_bounds, W, H, m, d_tf, *p, form,
+                                      d_rgba, d_steps, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_reconstruct");
+}
+
+int mray_hip_synth_mixing_layer(float *d_out, int n, const double *amp, const double *theta, const double *phi,
+                                uint64_t seed, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_synth_mixing_layer: call mray_hip_init first");
+    if (!d_out || !amp || !theta || !phi || n <= 0)
+        return fail(MRAY_ERR_ARG, "mray_hip_synth_mixing_layer: bad argument");
+    int rc = mray::launch_synth_mixing_layer(d_out, n, amp, theta, phi, seed, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_synth_mixing_layer");
+}
+
+int mray_hip_synth_marschner_lobb(float *d_out, int n, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_synth_marschner_lobb: call mray_hip_init first");
+    if (!d_out || n <= 0)
+        return fail(MRAY_ERR_ARG, "mray_hip_synth_marschner_lobb: bad argument");
+    int rc = mray::launch_synth_marschner_lobb(d_out, n, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_synth_marschner_lobb");
+}
+
+} // extern "C"

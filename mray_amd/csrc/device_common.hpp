@@ -1,0 +1,240 @@
+// Device-side building blocks shared by the moment-image kernels (gfx950).
+//
+// Everything that decides *which* samples a ray takes and *what value* each
+// sample has (pixel -> ray, slab test, step loop, trilinear fetch,
+// map_density, bound quantisation, TF lookup) follows the reference CPU
+// semantics operation by operation, so that sample values and 16-bit bounds
+// are bit-identical to the CPU oracle. This file is compiled with
+// -ffp-contract=off; kernels request FMAs explicitly with __builtin_fmaf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mray_types.h"
+
+namespace mray
+{
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr float kInvPiF = 0.3183098861837907f; // M_INV_PI (src/raymarch_generate.h:28)
+
+struct Vec3
+{
+    float x, y, z;
+};
+
+// cut::min / cut::max: (x < y) ? x : y, NaN handling by operand order
+// (src/cut/cuda_math.h:77-85). Never fminf/fmaxf.
+__device__ __forceinline__ float rmin(float x, float y) { return (x < y) ? x : y; }
+__device__ __forceinline__ float rmax(float x, float y) { return (x > y) ? x : y; }
+__device__ __forceinline__ float rclamp(float v, float lo, float hi) { return rmin(rmax(v, lo), hi); }
+__device__ __forceinline__ int iclamp(int v, int lo, int hi)
+{
+    int a = (v > lo) ? v : lo;
+    return (a < hi) ? a : hi;
+}
+
+// Camera passed by value as a kernel argument (no __constant__ state).
+struct DevCamera
+{
+    int orthographic;
+    float m[12]; // rows 0..2 of the camera-to-world matrix
+    Vec3 pos, right, up, dir;
+    float scale, aspect, znear;
+};
+
+inline DevCamera make_dev_camera(const mray_camera &c)
+{
+    DevCamera d;
+    d.orthographic = c.orthographic;
+    for (int i = 0; i < 12; ++i)
+        d.m[i] = c.transform[i];
+    d.pos = {c.position[0], c.position[1], c.position[2]};
+    d.right = {c.right[0], c.right[1], c.right[2]};
+    d.up = {c.up[0], c.up[1], c.up[2]};
+    d.dir = {c.dir[0], c.dir[1], c.dir[2]};
+    d.scale = c.scale;
+    d.aspect = c.aspect;
+    d.znear = c.znear;
+    return d;
+}
+
+struct Ray
+{
+    Vec3 o, d;
+};
+
+// get_eye_ray (src/cut/raytracing.h:39-65) with the pixel -> NDC mapping of
+// src/moment_raymarcher.cpp:133 (no half-pixel offset).
+__device__ __forceinline__ Ray pixel_ray(const DevCamera &cam, int W, int H, int x, int y)
+{
+    float ndcx = x / (float)W * 2.f - 1.f;
+    float ndcy = y / (float)H * 2.f - 1.f;
+    Ray r;
+    if (cam.orthographic)
+    {
+        r.o.x = (cam.pos.x + ndcx * cam.right.x) + ndcy * cam.up.x;
+        r.o.y = (cam.pos.y + ndcx * cam.right.y) + ndcy * cam.up.y;
+        r.o.z = (cam.pos.z + ndcx * cam.right.z) + ndcy * cam.up.z;
+        r.d = cam.dir;
+    }
+    else
+    {
+        r.o = cam.pos;
+        float px = ndcx * cam.aspect * cam.scale;
+        float py = ndcy * cam.scale;
+        float v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            v[k] = cam.m[k * 4 + 0] * px + cam.m[k * 4 + 1] * py + cam.m[k * 4 + 2] * -1.0f + cam.m[k * 4 + 3] * 0.0f;
+        float inv = 1 / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        r.d = {v[0] * inv, v[1] * inv, v[2] * inv};
+    }
+    return r;
+}
+
+// intersect_AABB (src/cut/raytracing.h:67-86)
+__device__ __forceinline__ bool intersect_aabb(const Ray &r, const Vec3 &bmin, const Vec3 &bmax, float &t_near,
+                                               float &t_far)
+{
+    float ix = 1.0f / r.d.x, iy = 1.0f / r.d.y, iz = 1.0f / r.d.z;
+    float tbx = ix * (bmin.x - r.o.x), tby = iy * (bmin.y - r.o.y), tbz = iz * (bmin.z - r.o.z);
+    float ttx = ix * (bmax.x - r.o.x), tty = iy * (bmax.y - r.o.y), ttz = iz * (bmax.z - r.o.z);
+    float tminx = rmin(ttx, tbx), tminy = rmin(tty, tby), tminz = rmin(ttz, tbz);
+    float tmaxx = rmax(ttx, tbx), tmaxy = rmax(tty, tby), tmaxz = rmax(ttz, tbz);
+    float largest_tmin = rmax(rmax(tminx, tminy), rmax(tminx, tminz));
+    float smallest_tmax = rmin(rmin(tmaxx, tmaxy), rmin(tmaxx, tmaxz));
+    t_near = largest_tmin;
+    t_far = smallest_tmax;
+    return smallest_tmax > largest_tmin;
+}
+
+// Device view of a regular grid, voxel (x,y,z) at data[z*nx*ny + y*nx + x].
+template <typename T> struct VolumeView
+{
+    const T *__restrict__ data;
+    int nx, ny, nz;
+    Vec3 dmin, dmax;
+};
+
+__device__ __forceinline__ float voxel_value(float v) { return v; }
+__device__ __forceinline__ float voxel_value(uint8_t v) { return (float)(v / 255.f); }
+
+// RegularGrid::sample (src/volume.h:97-123): corner-aligned trilinear with
+// clamp-to-edge indices and the reference's weight order.
+template <typename T> __device__ __forceinline__ float sample_volume(const VolumeView<T> &v, float px, float py, float pz)
+{
+    float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
+    float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
+    float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
+    float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
+    float ax = ipx - flx, ay = ipy - fly, az = ipz - flz;
+    int xl = (int)flx, yl = (int)fly, zl = (int)flz;
+    int x0 = iclamp(xl, 0, v.nx - 1), x1 = iclamp(xl + 1, 0, v.nx - 1);
+    int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
+    int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
+    size_t sxy = (size_t)(v.ny * v.nx);
+    size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
+    size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
+    size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
+    size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
+    float lll = voxel_value(v.data[r00 + x0]);
+    float rll = voxel_value(v.data[r00 + x1]);
+    float lrl = voxel_value(v.data[r01 + x0]);
+    float rrl = voxel_value(v.data[r01 + x1]);
+    float llr = voxel_value(v.data[r10 + x0]);
+    float rlr = voxel_value(v.data[r10 + x1]);
+    float lrr = voxel_value(v.data[r11 + x0]);
+    float rrr = voxel_value(v.data[r11 + x1]);
+    return (1.f - ax) * (1.f - ay) * (1.f - az) * lll + ax * (1.f - ay) * (1.f - az) * rll +
+           (1.f - ax) * ay * (1.f - az) * lrl + ax * ay * (1.f - az) * rrl + (1.f - ax) * (1.f - ay) * az * llr +
+           ax * (1.f - ay) * az * rlr + (1.f - ax) * ay * az * lrr + ax * ay * az * rrr;
+}
+
+// map_density (src/parameters.h:31-34)
+__device__ __forceinline__ float map_density(float s, float dmin, float dmax)
+{
+    return rclamp((s - dmin) / (dmax - dmin), 0.0f, 1.0f);
+}
+
+// DensityBound::set (src/common.h:143-147)
+__device__ __forceinline__ uint32_t pack_bound(float l, float u)
+{
+    uint32_t lo = (uint16_t)(l * 65535.f);
+    uint32_t hi = (uint16_t)(u * 65535.f);
+    return lo | (hi << 16);
+}
+
+// fast_atan (src/mese/complex_algebra.h:131-152)
+__device__ __forceinline__ float fast_atan(float tan)
+{
+    bool negative = (tan < 0.0f);
+    float abs_tan = negative ? (-tan) : tan;
+    bool greater_one = (abs_tan > 1.0f);
+    float x = greater_one ? abs_tan : 1.0f;
+    x = 1.0f / x;
+    float y = greater_one ? 1.0f : abs_tan;
+    x = x * y;
+    y = x * x;
+    float z = y * 0.020835f - 0.085133f;
+    z = y * z + 0.180141f;
+    z = y * z - 0.330299f;
+    y = y * z + 0.999866f;
+    z = y * x;
+    z = z * -2.0f + 1.570796f;
+    z = greater_one ? z : 0.0f;
+    x = x * y + z;
+    return negative ? (-x) : x;
+}
+
+// TransferFunction::sample on the row-constant 256-entry table
+// (src/transfer_function.h:44-59): only frac(256*bs) weights the two entries.
+__device__ __forceinline__ float4 tf_sample(const float4 *tf, float fs, float bs)
+{
+    float ipx = fs * 256.0f, ipy = bs * 256.0f;
+    float ax = ipx - floorf(ipx);
+    float ay = ipy - floorf(ipy);
+    int xl = (int)floorf(ipx);
+    int x0 = iclamp(xl, 0, 255), x1 = iclamp(xl + 1, 0, 255);
+    float4 l = tf[x0], r = tf[x1];
+    float w0 = (1.f - ax) * (1.f - ay), w1 = ax * (1.f - ay), w2 = (1.f - ax) * ay, w3 = ax * ay;
+    float4 o;
+    o.x = w0 * l.x + w1 * l.x + w2 * r.x + w3 * r.x;
+    o.y = w0 * l.y + w1 * l.y + w2 * r.y + w3 * r.y;
+    o.z = w0 * l.z + w1 * l.z + w2 * r.z + w3 * r.z;
+    o.w = w0 * l.w + w1 * l.w + w2 * r.w + w3 * r.w;
+    return o;
+}
+
+// Upper bound on the iterations of a march loop from t_near to t_far with
+// step s. Float rounding can shorten each increment by up to half an ulp, so
+// the bound has 1% plus 16 iterations of slack; it only ends loops whose
+// t + s no longer advances (which would never terminate).
+__device__ __forceinline__ int march_cap(float t_near, float t_far, float s)
+{
+    float n = (t_far - t_near) / s;
+    n = rmin(rmax(n, 0.0f), 1.0e8f);
+    return (int)(n * 1.01f) + 16;
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD gets a
+// contiguous run of tiles (neighbouring rays share its L2).
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks)
+{
+    constexpr uint32_t kXcd = 8;
+    uint32_t per = nblocks / kXcd;
+    if (per == 0 || bid >= per * kXcd)
+        return bid;
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+// 8x8 pixel tile per 64-lane wavefront; tiles in row-major order.
+__device__ __forceinline__ void tile_pixel(uint32_t tile, int W, int lane, int &x, int &y)
+{
+    int tiles_x = (W + 7) / 8;
+    x = (int)(tile % tiles_x) * 8 + (lane & 7);
+    y = (int)(tile / tiles_x) * 8 + (lane >> 3);
+}
+
+} // namespace mray

@@ -1,0 +1,455 @@
+// Moment generation: ray march accumulating bounded trigonometric moments of
+// the density along each ray (src/raymarch_generate.h:35-148).
+//
+// Two forms share the sample loop (pixel ray, slab test, t loop, trilinear
+// fetch, map_density, min/max bound) so both see exactly the reference's
+// samples and produce its 16-bit bounds bit for bit:
+//
+//  PLAIN  the reference's per-segment update_moments: 2(m-1) sin/cos per
+//         sample, one thread per ray, accumulators in LDS (what a direct port
+//         of generate_moments_kernel does).
+//
+//  MIXED  the same integral (exact for the piecewise-linear signal) written
+//         per node instead of per segment. For segments with slopes g_s the
+//         i/j terms telescope and
+//             trig_j = (1/j^2) sum_k (g_k - g_{k+1}) cos(j phi_k)
+//                      - (f_0 / j) sin(j phi_0),
+//         so a node costs one phasor rotation (re-seeded with an exact
+//         sincos every kReseed nodes) and a Chebyshev recurrence over j:
+//         2 FMAs per moment instead of 2 sin/cos. Accumulation is blocked:
+//         kFold nodes go into a partial sum that is then added with Kahan
+//         compensation; trig_0 (the trapezoid rule) is accumulated in FP64.
+//         Moments live in registers (template M >= m).
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace mray
+{
+
+struct GenArgs
+{
+    int W, H, m;
+    float step;
+    float data_min, data_max;
+    int transform_to_bounds;
+    int compact_image;
+    float bound_eps;
+};
+
+// Sample at t along r (origin + t * dir in float, as the reference).
+template <typename T>
+__device__ __forceinline__ float sample_at(const VolumeView<T> &vol, const Ray &r, float t, const GenArgs &a)
+{
+    float px = r.o.x + t * r.d.x, py = r.o.y + t * r.d.y, pz = r.o.z + t * r.d.z;
+    return map_density(sample_volume(vol, px, py, pz), a.data_min, a.data_max);
+}
+
+// ---------------------------------------------------------------------------
+// PLAIN
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
+                                                             float *__restrict__ out, uint32_t *__restrict__ bounds,
+                                                             int32_t *__restrict__ samples)
+{
+    extern __shared__ float lds_trig[]; // [m][blockDim.x], strided per thread
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int stride = blockDim.x;
+    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x / 64) + (tid >> 6);
+    if (tile >= ntiles)
+        return;
+    int x, y;
+    tile_pixel(tile, a.W, lane, x, y);
+    if (x >= a.W || y >= a.H)
+        return;
+    const int m = a.m;
+    float *trig = lds_trig + tid;
+#define TRIG(j) trig[(j)*stride]
+    for (int j = 0; j < m; ++j)
+        TRIG(j) = 0.0f;
+
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    bool hit = intersect_aabb(r, vol.dmin, vol.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    size_t pix = (size_t)y * a.W + x;
+    float *mo = out + pix * (size_t)m;
+    if (!(hit && (t_far > 0)))
+    {
+        if (!(a.transform_to_bounds && a.compact_image))
+            for (int j = 0; j < m; ++j)
+                mo[j] = 0.f;
+        if (bounds)
+            bounds[pix] = 0;
+        if (samples)
+            samples[pix] = 0;
+        return;
+    }
+    const float s = a.step;
+    float lo = 1.f, hi = 0.f;
+    float t = t_near;
+    float pi_trange_inv = (float)(kPi / (double)(t_far - t_near));
+    float p = (float)(-kPi);
+    float dphi = (float)(kPi * (double)(s / (t_far - t_near)));
+    float fs = sample_at(vol, r, t, a);
+    int ns = 1;
+    lo = rmin(lo, fs);
+    hi = rmax(hi, fs);
+
+    auto update = [&](float p_next, float step_size, float f0, float f1) {
+        // update_moments (src/raymarch_generate.h:35-58)
+        float p_prev = p_next - step_size;
+        float gradient = (f1 - f0) / step_size;
+        float yint = f0 - gradient * p_prev;
+        for (int j = 1; j < m; ++j)
+        {
+            float jf = (float)j;
+            float jSq = jf * jf;
+            float csx = gradient / jSq, csy = yint / jf;
+            float sn, cn, sp, cp;
+            sincosf(-jf * p_next, &sn, &cn);
+            sincosf(-jf * p_prev, &sp, &cp);
+            float ay = csy + gradient * jf * p_next / jSq;
+            float by = csy + gradient * jf * p_prev / jSq;
+            float acc = TRIG(j);
+            acc += (csx + 0.0f) * cn - ay * sn;
+            acc -= (csx + 0.0f) * cp - by * sp;
+            TRIG(j) = acc;
+        }
+        float t0 = TRIG(0);
+        t0 = (float)((double)t0 + (0.5 * gradient * (double)(p_next * p_next) + (double)(yint * p_next)));
+        t0 = (float)((double)t0 - (0.5 * gradient * (double)(p_prev * p_prev) + (double)(yint * p_prev)));
+        TRIG(0) = t0;
+    };
+
+    float bs;
+    const int cap = march_cap(t_near, t_far, s);
+    for (int it = 0; t + s < t_far && it < cap; ++it)
+    {
+        t += s;
+        p = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
+        bs = sample_at(vol, r, t, a);
+        ++ns;
+        lo = rmin(lo, bs);
+        hi = rmax(hi, bs);
+        update(p, dphi, fs, bs);
+        fs = bs;
+    }
+    float last_step = 0.0f - p;
+    bs = sample_at(vol, r, t_far, a);
+    ++ns;
+    lo = rmin(lo, bs);
+    hi = rmax(hi, bs);
+    update(0.0f, last_step, fs, bs);
+
+    // normalisation (src/raymarch_generate.h:116-135)
+    if (a.transform_to_bounds)
+    {
+        lo = rmax(lo - a.bound_eps, 0.0f);
+        hi = rmin(hi + a.bound_eps, 1.0f);
+        double inv = 1.0 / (double)(hi - lo);
+        mo[0] = (float)((double)(TRIG(0) * kInvPiF - lo) * inv);
+        for (int j = 1; j < m; ++j)
+            mo[j] = (float)((double)(TRIG(j) * kInvPiF) * inv);
+    }
+    else
+        for (int j = 0; j < m; ++j)
+            mo[j] = TRIG(j) * kInvPiF;
+#undef TRIG
+    if (bounds)
+        bounds[pix] = pack_bound(lo, hi);
+    if (samples)
+        samples[pix] = ns;
+}
+
+// ---------------------------------------------------------------------------
+// MIXED
+// ---------------------------------------------------------------------------
+constexpr int kReseed = 16; // exact sincos of the node phase every kReseed nodes
+constexpr int kFold = 8;    // nodes per partial sum before the Kahan add
+
+template <int M>
+__device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float x)
+{
+    // part[j] += d * cos(j*phi) for j = 1..M-1, cos(j*phi) by the Chebyshev
+    // recurrence c_j = 2x c_{j-1} - c_{j-2} from x = cos(phi).
+    const float x2 = x + x;
+    float cm2 = 1.0f, cm1 = x;
+    part[1] = __builtin_fmaf(d, cm1, part[1]);
+#pragma unroll
+    for (int j = 2; j < M; ++j)
+    {
+        float c = __builtin_fmaf(x2, cm1, -cm2);
+        part[j] = __builtin_fmaf(d, c, part[j]);
+        cm2 = cm1;
+        cm1 = c;
+    }
+}
+
+template <int M> __device__ __forceinline__ void kahan_fold(float (&part)[M], float (&sum)[M], float (&comp)[M])
+{
+#pragma unroll
+    for (int j = 1; j < M; ++j)
+    {
+        float yv = part[j] - comp[j];
+        float tv = sum[j] + yv;
+        comp[j] = (tv - sum[j]) - yv;
+        sum[j] = tv;
+        part[j] = 0.0f;
+    }
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
+                                                             float *__restrict__ out, uint32_t *__restrict__ bounds,
+                                                             int32_t *__restrict__ samples)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    if (tile >= ntiles)
+        return;
+    int x, y;
+    tile_pixel(tile, a.W, lane, x, y);
+    if (x >= a.W || y >= a.H)
+        return;
+    const int m = a.m;
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    bool hit = intersect_aabb(r, vol.dmin, vol.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    size_t pix = (size_t)y * a.W + x;
+    float *mo = out + pix * (size_t)m;
+    if (!(hit && (t_far > 0)))
+    {
+        if (!(a.transform_to_bounds && a.compact_image))
+            for (int j = 0; j < m; ++j)
+                mo[j] = 0.f;
+        if (bounds)
+            bounds[pix] = 0;
+        if (samples)
+            samples[pix] = 0;
+        return;
+    }
+
+    const float s = a.step;
+    const float pi_trange_inv = (float)(kPi / (double)(t_far - t_near));
+    const float dphi = (float)(kPi * (double)(s / (t_far - t_near)));
+    const float inv_dphi = 1.0f / dphi;
+    float rot_s, rot_c;
+    sincosf(dphi, &rot_s, &rot_c);
+
+    float t = t_near;
+    float f_prev = sample_at(vol, r, t, a);
+    const float f_first = f_prev;
+    float lo = rmin(1.f, f_prev), hi = rmax(0.f, f_prev);
+    int ns = 1;
+
+    // phase of node 0: start of the first segment, p_1 - dphi (or -pi when the
+    // ray has no interior sample), as update_moments forms p_prev.
+    float node_phase;
+    if (t + s < t_far)
+    {
+        float t1 = t + s;
+        float p1 = (float)((double)((t1 - t_near) * pi_trange_inv) - kPi);
+        node_phase = p1 - dphi;
+    }
+    else
+        node_phase = (float)(-kPi);
+    float zs, zc;
+    sincosf(node_phase, &zs, &zc);
+    const float z0s = zs, z0c = zc;
+
+    float part[M], sum[M], comp[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+    {
+        part[j] = 0.0f;
+        sum[j] = 0.0f;
+        comp[j] = 0.0f;
+    }
+    double trap = 0.0;
+    float g_left = 0.0f;
+    int node = 0;
+
+    const int cap = march_cap(t_near, t_far, s);
+    for (int it = 0; t + s < t_far && it < cap; ++it)
+    {
+        t += s;
+        const float p = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
+        const float f = sample_at(vol, r, t, a);
+        ++ns;
+        lo = rmin(lo, f);
+        hi = rmax(hi, f);
+        const float g = (f - f_prev) * inv_dphi;
+        trap += (double)dphi * (0.5 * ((double)f_prev + (double)f));
+        accumulate_node<M>(part, g_left - g, zc);
+        g_left = g;
+        f_prev = f;
+        node_phase = p;
+        ++node;
+        if (node % kReseed == 0)
+            sincosf(p, &zs, &zc);
+        else
+        {
+            float nc = zc * rot_c - zs * rot_s;
+            float nsn = zs * rot_c + zc * rot_s;
+            zc = nc;
+            zs = nsn;
+        }
+        if (node % kFold == 0)
+            kahan_fold<M>(part, sum, comp);
+    }
+    // last segment [p_K, 0] (src/raymarch_generate.h:107-114)
+    const float f_last = sample_at(vol, r, t_far, a);
+    ++ns;
+    lo = rmin(lo, f_last);
+    hi = rmax(hi, f_last);
+    const float dl = 0.0f - node_phase;
+    const float g_last = (f_last - f_prev) / dl;
+    trap += (double)dl * (0.5 * ((double)f_prev + (double)f_last));
+    accumulate_node<M>(part, g_left - g_last, zc);
+    // node at phase 0: cos(j*0) = 1
+#pragma unroll
+    for (int j = 1; j < M; ++j)
+        part[j] += g_last;
+    kahan_fold<M>(part, sum, comp);
+
+    // trig_j = sum_j / j^2 - f_0 sin(j phi_0) / j, sin by Chebyshev from phi_0.
+    float trig[M];
+    {
+        const float x2 = z0c + z0c;
+        float sm2 = 0.0f, sm1 = z0s;
+#pragma unroll
+        for (int j = 1; j < M; ++j)
+        {
+            float sj;
+            if (j == 1)
+                sj = z0s;
+            else
+            {
+                sj = __builtin_fmaf(x2, sm1, -sm2);
+                sm2 = sm1;
+                sm1 = sj;
+            }
+            const float jf = (float)j;
+            trig[j] = (sum[j] - comp[j]) / (jf * jf) - f_first * sj / jf;
+        }
+    }
+    trig[0] = (float)trap;
+
+    uint32_t b;
+    if (a.transform_to_bounds)
+    {
+        lo = rmax(lo - a.bound_eps, 0.0f);
+        hi = rmin(hi + a.bound_eps, 1.0f);
+        const double inv = 1.0 / (double)(hi - lo);
+        mo[0] = (float)((double)(trig[0] * kInvPiF - lo) * inv);
+#pragma unroll
+        for (int j = 1; j < M; ++j)
+            if (j < m)
+                mo[j] = (float)((double)(trig[j] * kInvPiF) * inv);
+    }
+    else
+    {
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m)
+                mo[j] = trig[j] * kInvPiF;
+    }
+    b = pack_bound(lo, hi);
+    if (bounds)
+        bounds[pix] = b;
+    if (samples)
+        samples[pix] = ns;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T, int M>
+static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
+                         int32_t *s, hipStream_t st, uint32_t blocks)
+{
+    hipLaunchKernelGGL((generate_mixed_kernel<T, M>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s);
+}
+
+template <typename T>
+static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
+                          int32_t *s, hipStream_t st, uint32_t blocks)
+{
+    const int m = a.m;
+#define MRAY_GEN_CASE(MM)                                                                                             \
+    if (m <= MM)                                                                                                      \
+    {                                                                                                                 \
+        launch_mixed<T, MM>(v, c, a, o, b, s, st, blocks);                                                            \
+        return 0;                                                                                                     \
+    }
+    MRAY_GEN_CASE(4)
+    MRAY_GEN_CASE(8)
+    MRAY_GEN_CASE(12)
+    MRAY_GEN_CASE(16)
+    MRAY_GEN_CASE(20)
+    MRAY_GEN_CASE(24)
+    MRAY_GEN_CASE(28)
+    MRAY_GEN_CASE(32)
+    MRAY_GEN_CASE(36)
+    MRAY_GEN_CASE(40)
+    MRAY_GEN_CASE(44)
+    MRAY_GEN_CASE(48)
+    MRAY_GEN_CASE(52)
+    MRAY_GEN_CASE(56)
+    MRAY_GEN_CASE(60)
+    MRAY_GEN_CASE(64)
+#undef MRAY_GEN_CASE
+    return MRAY_ERR_UNSUPPORTED;
+}
+
+// PLAIN keeps m floats per thread in LDS: 256-thread blocks while that fits
+// in 64 KiB, else one wave per block.
+static int plain_block(int m) { return (256 * m * (int)sizeof(float) <= 65536) ? 256 : 64; }
+
+int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                    const mray_camera &cam, const mray_gen_params &p, int form, int W, int H, int m, float *moments,
+                    uint32_t *bounds, int32_t *samples, hipStream_t stream)
+{
+    GenArgs a;
+    a.W = W;
+    a.H = H;
+    a.m = m;
+    a.step = p.step_size;
+    a.data_min = p.data_min;
+    a.data_max = p.data_max;
+    a.transform_to_bounds = p.transform_to_bounds;
+    a.compact_image = p.compact_image;
+    a.bound_eps = p.transform_bound_eps;
+    DevCamera c = make_dev_camera(cam);
+    uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
+    uint32_t blocks = (ntiles + 3) / 4;
+    if (dtype == MRAY_DTYPE_F32)
+    {
+        VolumeView<float> v{(const float *)vox, nx, ny, nz, {dmin[0], dmin[1], dmin[2]}, {dmax[0], dmax[1], dmax[2]}};
+        if (form == MRAY_FORM_PLAIN)
+        {
+            int bs = plain_block(m);
+            hipLaunchKernelGGL(generate_plain_kernel<float>, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
+                               bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples);
+            return 0;
+        }
+        return dispatch_mixed<float>(v, c, a, moments, bounds, samples, stream, blocks);
+    }
+    VolumeView<uint8_t> v{(const uint8_t *)vox, nx, ny, nz, {dmin[0], dmin[1], dmin[2]}, {dmax[0], dmax[1], dmax[2]}};
+    if (form == MRAY_FORM_PLAIN)
+    {
+        int bs = plain_block(m);
+        hipLaunchKernelGGL(generate_plain_kernel<uint8_t>, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
+                           bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples);
+        return 0;
+    }
+    return dispatch_mixed<uint8_t>(v, c, a, moments, bounds, samples, stream, blocks);
+}
+
+} // namespace mray

@@ -1,0 +1,28 @@
+// Host-side launchers implemented in the kernel translation units and called
+// by the C ABI (capi.hip). Each returns 0 on success or a negative mray_status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mray_types.h"
+
+namespace mray
+{
+
+int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                    const mray_camera &cam, const mray_gen_params &p, int form, int W, int H, int m, float *moments,
+                    uint32_t *bounds, int32_t *samples, hipStream_t stream);
+
+int launch_prepare(const float *moments, const uint32_t *index, int W, int H, int m, float bias, int form,
+                   float *lagrange, hipStream_t stream);
+
+int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
+                       const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                       const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream);
+
+int launch_synth_mixing_layer(float *out, int n, const double *amp, const double *theta, const double *phi,
+                              uint64_t seed, hipStream_t stream);
+int launch_synth_marschner_lobb(float *out, int n, hipStream_t stream);
+
+} // namespace mray

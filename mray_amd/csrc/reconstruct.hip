@@ -1,0 +1,332 @@
+// Reconstruct-and-composite march (reconstruct_ray,
+// src/raymarch_reconstruct.h:137-200) over prepared Lagrange multipliers.
+//
+// Both forms replay the reference's phase sequence exactly: the phase is
+// *incremented* in float (p += phase_step) while t <= t_far, clamped to
+// [-pi, 0] per sample, and compositing stops when alpha >= the early ray
+// termination threshold.
+//
+//  PLAIN  evaluateMESELagrange as written: cos/sin of the phase, powers of
+//         the circle point by repeated complex multiplication, multipliers
+//         strided through LDS (as reconstruct_moments_kernel).
+//
+//  MIXED  multipliers in registers (template M >= m); the series
+//         lambda_0 + 2 sum_l lambda_l cos(l theta) by Clenshaw's recurrence
+//         b_l = lambda_l + 2 cos(theta) b_{l+1} - b_{l+2}: 2 VALU ops per
+//         term and a single cos per sample. The 256-entry transfer function
+//         is staged in LDS once per workgroup.
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace mray
+{
+
+struct RecArgs
+{
+    int W, H, m;
+    float step;
+    float emission;
+    float ert;
+    int fourier;
+    Vec3 dmin, dmax;
+};
+
+__device__ __forceinline__ void zero_ray(const float4 *tf, float *o)
+{
+    // reconstruct_zero_ray (src/raymarch_reconstruct.h:127-131)
+    float4 c = tf_sample(tf, 0.0f, 0.0f);
+    o[0] = c.x;
+    o[1] = c.y;
+    o[2] = c.z;
+    o[3] = 0.0f;
+}
+
+__device__ __forceinline__ void store_rgba(float *rgba, size_t pix, float r, float g, float b, float a)
+{
+    *reinterpret_cast<float4 *>(rgba + pix * 4) = make_float4(r, g, b, a);
+}
+
+// ---------------------------------------------------------------------------
+// PLAIN
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, RecArgs a,
+                                                                const float *__restrict__ lag,
+                                                                const uint32_t *__restrict__ index,
+                                                                const uint32_t *__restrict__ bounds,
+                                                                const float4 *__restrict__ tf_g,
+                                                                float *__restrict__ rgba, int32_t *__restrict__ steps)
+{
+    __shared__ float4 tf[256];
+    extern __shared__ float lds_lag[]; // [m][blockDim]
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 256; i += blockDim.x)
+        tf[i] = tf_g[i];
+    __syncthreads();
+
+    const int lane = tid & 63;
+    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x / 64) + (tid >> 6);
+    if (tile >= ntiles)
+        return;
+    int x, y;
+    tile_pixel(tile, a.W, lane, x, y);
+    if (x >= a.W || y >= a.H)
+        return;
+    const size_t pix = (size_t)y * a.W + x;
+    const int nm = index ? min((int)(index[pix + 1] - index[pix]), a.m) : a.m;
+    const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
+    if (nm == 0)
+    {
+        float o[4];
+        zero_ray(tf, o);
+        store_rgba(rgba, pix, o[0], o[1], o[2], o[3]);
+        if (steps)
+            steps[pix] = 0;
+        return;
+    }
+    float *L = lds_lag + tid;
+    const int stride = blockDim.x;
+    for (int l = 0; l < nm; ++l)
+        L[l * stride] = lag[off + l];
+    float blo = 0.f, bhi = 1.f;
+    if (bounds)
+    {
+        uint32_t b = bounds[pix];
+        blo = (b & 0xFFFFu) / 65535.f;
+        bhi = (b >> 16) / 65535.f;
+    }
+    auto recon = [&](float p) {
+        p = rclamp(p, (float)(-kPi), 0.0f);
+        float cx = cosf(-p), cy = sinf(-p);
+        // evaluateFourierSeries (src/mese/MESE_dynamic.h:213-226)
+        float result = 0.0f;
+        float pwx = 1.0f, pwy = 0.0f;
+        for (int l = 1; l < nm; ++l)
+        {
+            float nx = pwx * cx - pwy * cy;
+            float ny = pwx * cy + pwy * cx;
+            pwx = nx;
+            pwy = ny;
+            result += L[l * stride] * pwx;
+        }
+        float sres = 2.0f * result + L[0];
+        float rho = a.fourier ? sres : fast_atan(sres) * 0.318309886f + 0.5f;
+        return rho * (bhi - blo) + blo;
+    };
+
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    bool hit = intersect_aabb(r, a.dmin, a.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    int ns = 0;
+    float Lr = 0.f, Lg = 0.f, Lb = 0.f, alpha = 0.f;
+    if (hit && (t_far > 0))
+    {
+        float t = t_near;
+        float p = (float)((double)((t - t_near) / (t_far - t_near)) * kPi - kPi);
+        const float dphi = (float)(kPi * (double)(a.step / (t_far - t_near)));
+        float T = 1.0f, tau = 0.0f;
+        float fs = recon(p);
+        const int cap = march_cap(t_near, t_far, a.step) + 1;
+        for (int it = 0; t <= t_far && it < cap; ++it)
+        {
+            t += a.step;
+            p += dphi;
+            ++ns;
+            float bs = recon(p);
+            float4 c = tf_sample(tf, fs, bs);
+            Lr += (a.emission * c.x) * T;
+            Lg += (a.emission * c.y) * T;
+            Lb += (a.emission * c.z) * T;
+            tau += c.w;
+            T = expf(-tau);
+            alpha = 1.0f - T;
+            fs = bs;
+            if (alpha >= a.ert)
+            {
+                alpha = 1.f;
+                break;
+            }
+        }
+    }
+    store_rgba(rgba, pix, Lr, Lg, Lb, alpha);
+    if (steps)
+        steps[pix] = ns;
+}
+
+// ---------------------------------------------------------------------------
+// MIXED
+// ---------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ float series_clenshaw(const float (&lam)[M], int nm, float x)
+{
+    // lambda_0 + 2 sum_{l=1}^{nm-1} lambda_l cos(l theta), x = cos(theta).
+    // Entries l >= nm hold 0, so running the recurrence over all of M is exact.
+    const float x2 = x + x;
+    float b1 = 0.0f, b2 = 0.0f;
+#pragma unroll
+    for (int l = M - 1; l >= 1; --l)
+    {
+        float b0 = __builtin_fmaf(x2, b1, lam[l] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    // sum_{l>=1} lambda_l cos(l theta) = x b_1 - b_2
+    return __builtin_fmaf(2.0f, __builtin_fmaf(x, b1, -b2), lam[0]);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, RecArgs a,
+                                                                const float *__restrict__ lag,
+                                                                const uint32_t *__restrict__ index,
+                                                                const uint32_t *__restrict__ bounds,
+                                                                const float4 *__restrict__ tf_g,
+                                                                float *__restrict__ rgba, int32_t *__restrict__ steps)
+{
+    __shared__ float4 tf[256];
+    const int tid = threadIdx.x;
+    tf[tid] = tf_g[tid];
+    __syncthreads();
+
+    const int lane = tid & 63;
+    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    if (tile >= ntiles)
+        return;
+    int x, y;
+    tile_pixel(tile, a.W, lane, x, y);
+    if (x >= a.W || y >= a.H)
+        return;
+    const size_t pix = (size_t)y * a.W + x;
+    const int nm = index ? min((int)(index[pix + 1] - index[pix]), a.m) : a.m;
+    const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
+    if (nm == 0)
+    {
+        float o[4];
+        zero_ray(tf, o);
+        store_rgba(rgba, pix, o[0], o[1], o[2], o[3]);
+        if (steps)
+            steps[pix] = 0;
+        return;
+    }
+    float lam[M];
+#pragma unroll
+    for (int l = 0; l < M; ++l)
+        lam[l] = (l < nm) ? lag[off + l] : 0.0f;
+    float blo = 0.f, bhi = 1.f;
+    if (bounds)
+    {
+        uint32_t b = bounds[pix];
+        blo = (b & 0xFFFFu) / 65535.f;
+        bhi = (b >> 16) / 65535.f;
+    }
+    const float brange = bhi - blo;
+    const bool fourier = a.fourier != 0;
+    auto recon = [&](float p) {
+        p = rclamp(p, (float)(-kPi), 0.0f);
+        float s = series_clenshaw<M>(lam, nm, cosf(p));
+        float rho = fourier ? s : __builtin_fmaf(fast_atan(s), 0.318309886f, 0.5f);
+        return __builtin_fmaf(rho, brange, blo);
+    };
+
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    bool hit = intersect_aabb(r, a.dmin, a.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    int ns = 0;
+    float Lr = 0.f, Lg = 0.f, Lb = 0.f, alpha = 0.f;
+    if (hit && (t_far > 0))
+    {
+        float t = t_near;
+        float p = (float)((double)((t - t_near) / (t_far - t_near)) * kPi - kPi);
+        const float dphi = (float)(kPi * (double)(a.step / (t_far - t_near)));
+        float T = 1.0f, tau = 0.0f;
+        float fs = recon(p);
+        const int cap = march_cap(t_near, t_far, a.step) + 1;
+        for (int it = 0; t <= t_far && it < cap; ++it)
+        {
+            t += a.step;
+            p += dphi;
+            ++ns;
+            const float bs = recon(p);
+            const float4 c = tf_sample(tf, fs, bs);
+            Lr = __builtin_fmaf(a.emission * c.x, T, Lr);
+            Lg = __builtin_fmaf(a.emission * c.y, T, Lg);
+            Lb = __builtin_fmaf(a.emission * c.z, T, Lb);
+            tau += c.w;
+            T = __expf(-tau);
+            alpha = 1.0f - T;
+            fs = bs;
+            if (alpha >= a.ert)
+            {
+                alpha = 1.f;
+                break;
+            }
+        }
+    }
+    store_rgba(rgba, pix, Lr, Lg, Lb, alpha);
+    if (steps)
+        steps[pix] = ns;
+}
+
+int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
+                       const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                       const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream)
+{
+    RecArgs a;
+    a.W = W;
+    a.H = H;
+    a.m = m;
+    a.step = p.step_size;
+    a.emission = p.medium_emission;
+    a.ert = p.early_ray_termination;
+    a.fourier = p.use_truncated_fourier;
+    a.dmin = {dmin[0], dmin[1], dmin[2]};
+    a.dmax = {dmax[0], dmax[1], dmax[2]};
+    DevCamera c = make_dev_camera(cam);
+    const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
+    const float4 *tf4 = reinterpret_cast<const float4 *>(tf);
+    if (form == MRAY_FORM_PLAIN)
+    {
+        int bs = (256 * m * (int)sizeof(float) <= 60000) ? 256 : 64;
+        hipLaunchKernelGGL(reconstruct_plain_kernel, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
+                           (size_t)bs * m * sizeof(float), stream, c, a, lagrange, index, bounds, tf4, rgba, steps);
+        return 0;
+    }
+    const uint32_t blocks = (ntiles + 3) / 4;
+#define MRAY_REC_CASE(MM)                                                                                             \
+    if (m <= MM)                                                                                                      \
+    {                                                                                                                 \
+        hipLaunchKernelGGL(reconstruct_mixed_kernel<MM>, dim3(blocks), dim3(256), 0, stream, c, a, lagrange, index,   \
+                           bounds, tf4, rgba, steps);                                                                 \
+        return 0;                                                                                                     \
+    }
+    MRAY_REC_CASE(4)
+    MRAY_REC_CASE(8)
+    MRAY_REC_CASE(12)
+    MRAY_REC_CASE(16)
+    MRAY_REC_CASE(20)
+    MRAY_REC_CASE(24)
+    MRAY_REC_CASE(28)
+    MRAY_REC_CASE(32)
+    MRAY_REC_CASE(36)
+    MRAY_REC_CASE(40)
+    MRAY_REC_CASE(44)
+    MRAY_REC_CASE(48)
+    MRAY_REC_CASE(52)
+    MRAY_REC_CASE(56)
+    MRAY_REC_CASE(60)
+    MRAY_REC_CASE(64)
+    MRAY_REC_CASE(72)
+    MRAY_REC_CASE(80)
+    MRAY_REC_CASE(88)
+    MRAY_REC_CASE(96)
+    MRAY_REC_CASE(104)
+    MRAY_REC_CASE(112)
+    MRAY_REC_CASE(120)
+    MRAY_REC_CASE(128)
+#undef MRAY_REC_CASE
+    return MRAY_ERR_UNSUPPORTED;
+}
+
+} // namespace mray

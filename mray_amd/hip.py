@@ -1,0 +1,189 @@
+"""ctypes binding of libmray_hip.so (include/mray_hip.h) over torch device tensors.
+
+torch is used for device memory and streams only; all compute runs in the HIP
+kernels of mray_amd/csrc. There is no CPU fallback: a missing library raises.
+Every call is issued on torch's current stream unless a stream is given.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import torch
+
+from .types import Camera, GenParams, RecParams, MRAY_DTYPE_F32, MRAY_DTYPE_U8, MRAY_FORM_MIXED, MRAY_FORM_PLAIN
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmray_hip.so"
+
+_vp = C.c_void_p
+_f3 = C.c_float * 3
+
+_SIGNATURES = {
+    "mray_hip_init": ([C.c_int], C.c_int),
+    "mray_hip_last_error": ([], C.c_char_p),
+    "mray_hip_volume_create": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f3, _f3, C.POINTER(_vp)], C.c_int),
+    "mray_hip_volume_wrap": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f3, _f3, C.POINTER(_vp)], C.c_int),
+    "mray_hip_volume_destroy": ([_vp], C.c_int),
+    "mray_hip_volume_data": ([_vp], _vp),
+    "mray_hip_generate": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, C.c_int, _vp,
+                           _vp, _vp, _vp], C.c_int),
+    "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_reconstruct": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                              C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_synth_mixing_layer": ([_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.c_uint64, _vp], C.c_int),
+    "mray_hip_synth_marschner_lobb": ([_vp, C.c_int, _vp], C.c_int),
+}
+
+_lib = None
+_initialised_device = None
+
+
+class MrayError(RuntimeError):
+    pass
+
+
+def load_library() -> C.CDLL:
+    """Loads libmray_hip.so and declares the C ABI; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise MrayError(f"{LIB_PATH} is missing: build it with `make -C mray_amd` (no CPU fallback exists)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (argtypes, restype) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = load_library().mray_hip_last_error().decode(errors="replace")
+        raise MrayError(f"{what} failed ({rc}): {msg}")
+
+
+def init(device: int = 0):
+    global _initialised_device
+    lib = load_library()
+    if _initialised_device != device:
+        torch.cuda.set_device(device)
+        _check(lib.mray_hip_init(device), "mray_hip_init")
+        _initialised_device = device
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise MrayError("expected a device tensor")
+    if not t.is_contiguous():
+        raise MrayError("expected a contiguous tensor")
+    return _vp(t.data_ptr())
+
+
+def _stream(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return _vp(s.cuda_stream)
+
+
+def form_id(form) -> int:
+    if isinstance(form, str):
+        return {"plain": MRAY_FORM_PLAIN, "mixed": MRAY_FORM_MIXED}[form]
+    return int(form)
+
+
+class Volume:
+    """Device-resident RegularGrid (F32 or U8, layout [nz][ny][nx])."""
+
+    def __init__(self, voxels: torch.Tensor, dmin, dmax):
+        init(voxels.device.index or 0)
+        if voxels.dtype not in (torch.float32, torch.uint8) or voxels.dim() != 3:
+            raise MrayError("voxels must be a 3-D float32 or uint8 tensor")
+        self.voxels = voxels.contiguous()
+        self.dmin = tuple(float(v) for v in dmin)
+        self.dmax = tuple(float(v) for v in dmax)
+        nz, ny, nx = self.voxels.shape
+        dtype = MRAY_DTYPE_U8 if voxels.dtype == torch.uint8 else MRAY_DTYPE_F32
+        h = _vp()
+        _check(load_library().mray_hip_volume_wrap(_ptr(self.voxels), dtype, nx, ny, nz, _f3(*self.dmin),
+                                                   _f3(*self.dmax), C.byref(h)), "mray_hip_volume_wrap")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load_library().mray_hip_volume_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, form="mixed", moments=None,
+             bounds=None, samples=None, want_samples=False, stream=None):
+    """Dense moments (H, W, m), packed bounds (H*W,) uint32 (or None), samples per ray (or None)."""
+    dev = vol.voxels.device
+    if moments is None:
+        moments = torch.empty((H, W, m), dtype=torch.float32, device=dev)
+    if bounds is None and gp.transform_to_bounds:
+        bounds = torch.empty(H * W, dtype=torch.int32, device=dev)
+    if samples is None and want_samples:
+        samples = torch.empty(H * W, dtype=torch.int32, device=dev)
+    c = Camera.from_dict(cam)
+    _check(load_library().mray_hip_generate(vol.handle, C.byref(c), C.byref(gp), form_id(form), W, H, m,
+                                            _ptr(moments), _ptr(bounds), _ptr(samples), _stream(stream)),
+           "mray_hip_generate")
+    return moments, bounds, samples
+
+
+def prepare(moments: torch.Tensor, W: int, H: int, m: int, bias: float, form="mixed", index=None, out=None,
+            stream=None):
+    if out is None:
+        out = torch.empty_like(moments)
+    _check(load_library().mray_hip_prepare(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias), form_id(form),
+                                           _ptr(out), _stream(stream)), "mray_hip_prepare")
+    return out
+
+
+def reconstruct(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m: int, tf: torch.Tensor,
+                rp: RecParams, form="mixed", index=None, bounds=None, rgba=None, steps=None, want_steps=False,
+                stream=None):
+    """Float RGBA (H, W, 4) and optionally the compositing steps per ray."""
+    dev = lagrange.device
+    if rgba is None:
+        rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    if steps is None and want_steps:
+        steps = torch.empty(H * W, dtype=torch.int32, device=dev)
+    c = Camera.from_dict(cam)
+    tf = tf.to(device=dev, dtype=torch.float32).contiguous()
+    if tf.numel() != 1024:
+        raise MrayError("transfer function table must have 256 RGBA entries")
+    _check(load_library().mray_hip_reconstruct(C.byref(c), _f3(*dmin), _f3(*dmax), _ptr(lagrange), _ptr(index),
+                                               _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp), form_id(form),
+                                               _ptr(rgba), _ptr(steps), _stream(stream)), "mray_hip_reconstruct")
+    return rgba, steps
+
+
+def synth_mixing_layer(n: int, seed: int = 1, device="cuda", stream=None) -> torch.Tensor:
+    from .synth import mixing_layer_modes
+
+    init(torch.device(device).index or 0)
+    amp, theta, phi = mixing_layer_modes(seed)
+    out = torch.empty((n, n, n), dtype=torch.float32, device=device)
+    arr = C.c_double * 16
+    _check(load_library().mray_hip_synth_mixing_layer(_ptr(out), n, arr(*amp), arr(*theta), arr(*phi),
+                                                      C.c_uint64(seed), _stream(stream)),
+           "mray_hip_synth_mixing_layer")
+    return out
+
+
+def synth_marschner_lobb(n: int, device="cuda", stream=None) -> torch.Tensor:
+    init(torch.device(device).index or 0)
+    out = torch.empty((n, n, n), dtype=torch.float32, device=device)
+    _check(load_library().mray_hip_synth_marschner_lobb(_ptr(out), n, _stream(stream)),
+           "mray_hip_synth_marschner_lobb")
+    return out

@@ -1,0 +1,257 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Run in the build container (needs oracle/_ref/libmrayref.so, built by
+`make -C oracle ref` from the reference sources):
+
+    python tests/golden/make_golden.py
+
+Inputs are (a) the vectors of the reference's own unit tests
+(tests/test_mese.cpp, tests/test_prediction_coding.cpp,
+tests/test_moment_compaction.cpp) and (b) small synthetic workloads from
+mray_amd.synth. Outputs are whatever the reference functions return. Only the
+resulting arrays are committed (golden_*.npz); no reference code is stored.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import sys
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+from mray_amd import synth  # noqa: E402
+from mray_amd.types import GenParams, RecParams  # noqa: E402
+from oracle_ffi import Reference, ptr  # noqa: E402
+
+OUT = pathlib.Path(__file__).resolve().parent
+_f = C.POINTER(C.c_float)
+
+# vectors of the reference unit tests (tests/test_mese.cpp:33,47,61,97-103,155-156,187,205-206;
+# tests/test_prediction_coding.cpp:32,74; tests/test_moment_compaction.cpp:32-33,85-104)
+TRIG4 = [0.22008814, 0.0, 0.01400074, 0.00271224]
+MOM4 = [0.5, 0.25, 0.25, 0.1]
+TRIG10 = [0.22008814, 0.0, 0.01400074, 0.00271224, 0.00426201, 0.0, 0.0, 0.0, 0.0, 0.0]
+TRIG28 = [5.2070445e-01, 7.4165657e-02, -1.3283056e-02, 1.2487045e-02, 1.3437086e-02, -9.5647079e-04,
+          -1.2216626e-03, 2.1146263e-03, -5.4606516e-04, 1.0199437e-02, 1.1894707e-02, -1.9188672e-02,
+          -5.5241492e-03, 2.8166445e-02, 6.7452774e-03, -1.9198583e-02, 4.3223342e-03, -6.1640824e-04,
+          7.8988671e-03, 2.4080461e-02, -9.6553434e-03, -1.9514188e-02, 1.1532067e-02, -4.1229022e-03,
+          2.3510400e-02, 1.7623754e-02, -1.4590423e-02, 1.1135901e-03]
+TRIG_C4 = [(0.22008814, 0.0), (0.0, 0.0), (0.01400074, 0.5), (0.00271224, 0.1)]
+SIG4 = [(2.0, 0.0), (4.0, 0.0), (8.0, 0.0), (16.0, 0.0)]
+RHS4 = [(1.0, 0.0), (42.0, 0.0), (0.1, 0.0), (0.2, 0.0)]
+PREP_C4 = [(0.22008814, 0.05), (-0.02, 0.01), (0.01400074, 0.04), (0.00271224, 0.1)]
+PC5 = [0.22, 0.0, 0.014, 0.0027, 0.004]
+PC5B = [0.22008814, 0.0, 0.01400074, 0.00271224, 0.00426201]
+COMP10 = [1.9176584e-01, -3.7339486e-02, -1.4143054e-02, -1.6614912e-02, -2.2262430e-02, -1.2005451e-02,
+          -7.5180288e-03, 2.5505440e-02, 2.8378680e-02, 1.6212923e-02]
+
+
+def f32(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32))
+
+
+def c64(pairs):
+    return np.ascontiguousarray(np.asarray(pairs, dtype=np.float32).reshape(-1))
+
+
+def mese_vectors(R: Reference) -> dict:
+    L = R.lib
+    out = {}
+    for name, t in (("trig4", TRIG4), ("trig10", TRIG10), ("trig28", TRIG28), ("comp10", COMP10)):
+        t = f32(t)
+        m = len(t)
+        e = np.zeros(2 * m, np.float32)
+        L.ref_trig_to_exp(m, ptr(t, _f), ptr(e, _f))
+        out[f"{name}_in"] = t
+        out[f"{name}_exp"] = e
+        back = np.zeros(2 * m, np.float32)
+        L.ref_exp_to_trig(m, ptr(e, _f), ptr(back, _f))
+        out[f"{name}_exp_to_trig"] = back
+        lev = np.zeros(2 * m, np.float32)
+        L.ref_levinson(m, ptr(e, _f), ptr(lev, _f))
+        out[f"{name}_levinson"] = lev
+        pm = np.zeros(m, np.float32)
+        L.ref_prepare_mese(m, ptr(t, _f), ptr(pm, _f))
+        out[f"{name}_prepare"] = pm
+        phases = f32(np.linspace(-np.pi, 0.0, 33))
+        out[f"{name}_phases"] = phases
+        out[f"{name}_evaluate"] = f32([L.ref_evaluate_mese(m, float(p), ptr(pm, _f)) for p in phases])
+        for bias in (1e-4, 5e-5):
+            pb = np.zeros(m, np.float32)
+            L.ref_prepare_dense(m, ptr(t, _f), C.c_float(bias), ptr(pb, _f))
+            out[f"{name}_prepare_bias{bias:g}"] = pb
+    # fixed-size versions the reference tests compare against (src/mese/MESE_4_10.h)
+    m4 = f32(MOM4)
+    pf = np.zeros(4, np.float32)
+    L.ref_fixed_prepare4(ptr(m4, _f), ptr(pf, _f))
+    out["mom4_in"] = m4
+    out["mom4_fixed_prepare"] = pf
+    p = np.float32(-np.pi + np.pi * 0.5)
+    out["mom4_fixed_eval"] = f32([L.ref_fixed_evaluate4(float(p), ptr(pf, _f))])
+    t10 = f32(TRIG10)
+    pf10 = np.zeros(10, np.float32)
+    L.ref_fixed_prepare10(ptr(t10, _f), ptr(pf10, _f))
+    out["trig10_fixed_prepare"] = pf10
+    out["trig10_fixed_eval"] = f32([L.ref_fixed_evaluate10(float(p), ptr(pf10, _f))])
+    e4 = np.zeros(8, np.float32)
+    L.ref_fixed_trig_to_exp4(ptr(f32(TRIG4), _f), ptr(e4, _f))
+    out["trig4_fixed_exp"] = e4
+    # complex-input overloads
+    tc = c64(TRIG_C4)
+    ec = np.zeros(8, np.float32)
+    L.ref_trig_to_exp_c(4, ptr(tc, _f), ptr(ec, _f))
+    out["trigc4_in"] = tc
+    out["trigc4_exp"] = ec
+    pc = c64(PREP_C4)
+    bias = np.float32(1e-2)
+    pcb = pc.reshape(4, 2).copy()
+    pcb[0] = (1 - bias) * pcb[0] + bias * np.float32(0.5) * np.array([1, 0], np.float32)
+    pcb[1:] = (1 - bias) * pcb[1:]
+    pcb = np.ascontiguousarray(pcb.reshape(-1))
+    lc = np.zeros(8, np.float32)
+    L.ref_prepare_mese_c(4, ptr(pcb, _f), ptr(lc, _f))
+    out["prepc4_in"] = pcb
+    out["prepc4_out"] = lc
+    s, r = c64(SIG4), c64(RHS4)
+    ac = np.zeros(8, np.float32)
+    cc = np.zeros(8, np.float32)
+    ic = np.zeros(4, np.float32)
+    L.ref_autocorrelation(4, ptr(s, _f), ptr(ac, _f))
+    L.ref_correlation(4, ptr(s, _f), ptr(r, _f), ptr(cc, _f))
+    L.ref_imag_correlation(4, ptr(s, _f), ptr(r, _f), ptr(ic, _f))
+    out.update(sig4=s, rhs4=r, autocorr4=ac, corr4=cc, imagcorr4=ic)
+    xs = f32(np.concatenate([np.linspace(-50, 50, 401), [0.0, 1.0, -1.0, 1e-8, 1e6]]))
+    out["atan_in"] = xs
+    out["atan_out"] = f32([L.ref_fast_atan(float(x)) for x in xs])
+    return out
+
+
+def coding_vectors(R: Reference) -> dict:
+    L = R.lib
+    out = {}
+    for name, t in (("pc5", PC5), ("pc5b", PC5B), ("comp10", COMP10), ("trig28", TRIG28)):
+        t = f32(t)
+        m = len(t)
+        e = np.zeros(2 * m, np.float32)
+        L.ref_trig_to_exp(m, ptr(t, _f), ptr(e, _f))
+        code = np.zeros(2 * m, np.float32)
+        L.ref_encode(m, ptr(e, _f), ptr(code, _f))
+        dec_e = np.zeros(2 * m, np.float32)
+        dec_p = np.zeros(2 * m, np.float32)
+        L.ref_decode(m, ptr(code, _f), ptr(dec_e, _f), ptr(dec_p, _f))
+        tq = np.zeros(m, np.float32)
+        L.ref_transform_quant(m, ptr(code, _f), C.c_float(t[0]), ptr(tq, _f))
+        dq = np.zeros(2 * m, np.float32)
+        L.ref_transform_dequant(m, ptr(tq, _f), ptr(dq, _f), 1, None)
+        out.update({f"{name}_in": t, f"{name}_exp": e, f"{name}_code": code, f"{name}_dec_exp": dec_e,
+                    f"{name}_dec_poly": dec_p, f"{name}_tq": tq, f"{name}_dq_none": dq})
+    # compaction (determine_best_number_moments) on the reference vectors and synthetic moments
+    thr = f32([0.0, 1e-4, 1e-3, 1e-2, 0.05, 0.2])
+    rng_moments = []
+    vol = synth.marschner_lobb(32)
+    gp = GenParams(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                   prediction_coding=0)
+    data, _, _, _ = R.generate(vol, (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 8, 6, 24, gp)
+    rng_moments = data.reshape(48, 24)
+    counts = np.zeros((len(thr), 48), np.int32)
+    for i, th in enumerate(thr):
+        for p in range(48):
+            counts[i, p] = L.ref_best_num_moments(24, ptr(np.ascontiguousarray(rng_moments[p]), _f), C.c_float(th))
+    c10 = [L.ref_best_num_moments(10, ptr(f32(COMP10), _f), C.c_float(th)) for th in thr]
+    out.update(compaction_thr=thr, compaction_moments=f32(rng_moments), compaction_counts=counts,
+               comp10_counts=np.asarray(c10, np.int32))
+    ne = np.zeros(20, np.float32)
+    e10 = out["comp10_exp"]
+    L.ref_eval_unknown(4, ptr(e10, _f), 10, ptr(ne, _f))
+    out["comp10_unknown_n4"] = ne
+    bm = np.zeros(20, np.float32)
+    bm[0] = COMP10[0]
+    L.ref_exp_to_bounded(10, ptr(ne, _f), ptr(bm, _f), 1)
+    out["comp10_bounded_n1"] = bm
+    return out
+
+
+def image_vectors(R: Reference) -> dict:
+    out = {}
+    tf8 = synth.tf_ramps(0)
+    out["tf_rgba8"] = tf8
+    cases = {
+        # name: (volume, dmin, dmax, camera, W, H, m, gen kwargs, rec kwargs)
+        "ml_ortho": (synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 24, 20, 12,
+                     dict(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                          prediction_coding=0, bias=5e-5),
+                     dict(step_size=2.0 / 64, bias=5e-5, early_ray_termination=0.99, medium_emission=1.0)),
+        "ml_persp_plain": (synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1),
+                           synth.perspective_camera(position=(0.1, -0.05, -3.0), fov_half_deg=20.0, znear=0.01,
+                                                    aspect=20 / 16),
+                           20, 16, 10, dict(step_size=2.0 / 48, compact_image=0, prediction_coding=0),
+                           dict(step_size=2.0 / 48, early_ray_termination=0.95)),
+        "mix_coded": (synth.mixing_layer(24), (0, 0, 0), (1, 1, 1),
+                      synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=16 / 12), 16, 12, 16,
+                      dict(step_size=1.0 / 48, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=1,
+                           prediction_coding=1, error_threshold=0.01, bias=1e-4),
+                      dict(step_size=1.0 / 48, bias=1e-4, early_ray_termination=0.98, medium_emission=1.0)),
+    }
+    u8 = np.clip(synth.mixing_layer(24) * 255.0, 0, 255).astype(np.uint8)
+    cases["u8_ortho"] = (u8, (0, 0, 0), (1, 1, 1),
+                         synth.ortho_camera(pos=(0.5, 0.5, -1.0), right=(0.5, 0, 0), up=(0, 0.5, 0)), 16, 16, 8,
+                         dict(step_size=1.0 / 32, transform_to_bounds=1, compact_image=0, prediction_coding=0),
+                         dict(step_size=1.0 / 32, early_ray_termination=0.9))
+    for name, (vol, dmin, dmax, cam, W, H, m, gk, rk) in cases.items():
+        gp = GenParams(**gk)
+        rp = RecParams(**rk)
+        data, bounds, index, cp = R.generate(vol, dmin, dmax, cam, W, H, m, gp)
+        tf_step = rp.step_size
+        rgba, rgba8 = R.reconstruct(cam, dmin, dmax, W, H, m, data, tf8, tf_step, rp,
+                                    index=index if gp.compact_image else None,
+                                    bounds=bounds if gp.transform_to_bounds else None,
+                                    prediction_coded=int(gp.prediction_coding), coding_warp=gp.coding_warp,
+                                    coding_params=cp if gp.prediction_coding else None, want_rgba8=True)
+        out[f"{name}_volume"] = vol
+        out[f"{name}_data"] = data
+        out[f"{name}_bounds"] = bounds
+        out[f"{name}_index"] = index
+        out[f"{name}_coding_params"] = cp
+        out[f"{name}_rgba"] = rgba
+        out[f"{name}_rgba8"] = rgba8
+    return out
+
+
+def sph_vectors(R: Reference) -> dict:
+    x, y, z, v = synth.particle_mixture(3000, seed=5, n_gauss=6)
+    v = (0.5 + 0.5 * synth.hash_uniform(7, 1, np.arange(3000))).astype(np.float32)
+    h = np.float32(0.05)
+    dmin, dmax = (0.0, 0.0, 0.0), (1.0, 1.0, 0.5)
+    n = (16, 16, 8)
+    pts = []
+    for iz in range(n[2]):
+        for iy in range(n[1]):
+            for ix in range(n[0]):
+                pts.append([np.float32(dmin[0]) + np.float32(ix) / np.float32(n[0]) * np.float32(dmax[0] - dmin[0]),
+                            np.float32(dmin[1]) + np.float32(iy) / np.float32(n[1]) * np.float32(dmax[1] - dmin[1]),
+                            np.float32(dmin[2]) + np.float32(iz) / np.float32(n[2]) * np.float32(dmax[2] - dmin[2])])
+    pts = f32(pts)
+    res = np.zeros(len(pts), np.float32)
+    bmin = (C.c_float * 3)(*dmin)
+    bmax = (C.c_float * 3)(*dmax)
+    R.lib.ref_particles_sample(ptr(x, _f), ptr(y, _f), ptr(z, _f), ptr(v, _f), C.c_uint32(len(x)), C.c_float(h),
+                               bmin, bmax, ptr(pts, _f), C.c_uint32(len(pts)), ptr(res, _f))
+    return dict(sph_x=x, sph_y=y, sph_z=z, sph_v=v, sph_h=f32([h]), sph_grid=res.reshape(n[2], n[1], n[0]))
+
+
+def main():
+    R = Reference()
+    np.savez_compressed(OUT / "golden_mese.npz", **mese_vectors(R))
+    np.savez_compressed(OUT / "golden_coding.npz", **coding_vectors(R))
+    np.savez_compressed(OUT / "golden_images.npz", **image_vectors(R))
+    np.savez_compressed(OUT / "golden_sph.npz", **sph_vectors(R))
+    for f in sorted(OUT.glob("golden_*.npz")):
+        print(f.name, f.stat().st_size)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,284 @@
+"""HIP kernels (through the C ABI) against the CPU oracle on the same inputs.
+
+Bars (stated per test):
+  * moments: per-pixel normwise relative error <= 1e-5 against the
+    reference-faithful FP32 oracle; the 16-bit bounds and the per-ray sample
+    counts are bit-exact;
+  * Lagrange multipliers: checked through the densities they reconstruct;
+  * float RGBA: <= 1e-4 absolute against the FP32 oracle (PLAIN) and RMSE
+    <= 1e-4 against the FP64 oracle (MIXED); rays whose step count differs
+    (early ray termination flips on rounding) are counted separately.
+"""
+import ctypes as C
+import pathlib
+
+import numpy as np
+import pytest
+
+from mray_amd import synth, tf as tfmod
+from mray_amd.types import GenParams, RecParams
+from oracle_ffi import Oracle, ptr
+
+pytestmark = pytest.mark.gpu
+
+GOLD = pathlib.Path(__file__).resolve().parent / "golden"
+_f = C.POINTER(C.c_float)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import torch
+    from mray_amd import hip as H
+
+    H.init(0)
+    return H
+
+
+@pytest.fixture(scope="module")
+def O():
+    return Oracle()
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def bounds_np(b):
+    return to_np(b).view(np.uint32)
+
+
+def normwise_rel(a, b):
+    a = a.reshape(-1, a.shape[-1]).astype(np.float64)
+    b = b.reshape(-1, b.shape[-1]).astype(np.float64)
+    den = np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+    return np.linalg.norm(a - b, axis=1) / den
+
+
+CASES = {
+    # name: volume factory, dmin, dmax, camera, W, H, m, gen kwargs
+    "ml_ortho_m30": (lambda: synth.marschner_lobb(64), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 48, 40, 30,
+                     dict(step_size=2.0 / 128, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                          prediction_coding=0, bias=5e-5)),
+    "ml_persp_m40": (lambda: synth.marschner_lobb(48), (-1, -1, -1), (1, 1, 1),
+                     synth.perspective_camera(position=(0.15, -0.1, -2.6), fov_half_deg=24.0, znear=0.01,
+                                              aspect=44 / 36), 44, 36, 40,
+                     dict(step_size=2.0 / 96, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                          prediction_coding=0)),
+    "mix_persp_m50": (lambda: synth.mixing_layer(48), (0, 0, 0), (1, 1, 1),
+                      synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=1.0), 40, 40, 50,
+                      dict(step_size=0.5 / 48, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                           prediction_coding=0)),
+    "ml_nobounds_m7": (lambda: synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1),
+                       synth.ortho_camera(pos=(0.3, 0.0, -3.0), right=(1.3, 0, 0), up=(0, 1.3, 0)), 21, 13, 7,
+                       dict(step_size=2.0 / 50, transform_to_bounds=0, compact_image=0, prediction_coding=0)),
+    "u8_ortho_m16": (lambda: np.clip(synth.mixing_layer(32) * 255, 0, 255).astype(np.uint8), (0, 0, 0), (1, 1, 1),
+                     synth.ortho_camera(pos=(0.5, 0.5, -1.0), right=(0.6, 0, 0), up=(0, 0.6, 0)), 24, 24, 16,
+                     dict(step_size=1.0 / 64, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
+    "tiny_m1": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 1,
+                dict(step_size=2.0 / 20, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
+    "tiny_m2": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 2,
+                dict(step_size=2.0 / 20, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
+}
+
+_cache = {}
+
+
+def case_inputs(name):
+    if name not in _cache:
+        f, dmin, dmax, cam, W, H, m, gk = CASES[name]
+        _cache[name] = (f(), dmin, dmax, cam, W, H, m, GenParams(**gk))
+    return _cache[name]
+
+
+def gpu_volume(hip, vol, dmin, dmax):
+    import torch
+
+    return hip.Volume(torch.from_numpy(vol).cuda(), dmin, dmax)
+
+
+def rec_params(gp, ert=0.99):
+    return RecParams(step_size=gp.step_size, bias=5e-5, early_ray_termination=ert, medium_emission=1.0)
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_generate_parity(hip, O, name, form):
+    import torch
+
+    vol, dmin, dmax, cam, W, H, m, gp = case_inputs(name)
+    print(f"[generate] case={name} form={form} W={W} H={H} m={m}", flush=True)
+    mo_ref, b_ref, ns_ref = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    v = gpu_volume(hip, vol, dmin, dmax)
+    mo, b, ns = hip.generate(v, cam, gp, W, H, m, form=form, want_samples=True)
+    torch.cuda.synchronize()
+    mo = to_np(mo)
+    assert np.array_equal(to_np(ns), ns_ref), "sample counts differ"
+    if gp.transform_to_bounds:
+        assert np.array_equal(bounds_np(b), b_ref), "16-bit bounds differ"
+    hit = ns_ref > 0
+    err = normwise_rel(mo, mo_ref)[hit]
+    assert err.max() <= 1e-5, f"max normwise rel err {err.max():.3e}"
+    missed = ~hit
+    if missed.any() and not (gp.transform_to_bounds and gp.compact_image):
+        assert np.all(mo.reshape(-1, m)[missed] == 0)
+
+
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "mix_persp_m50"])
+def test_generate_mixed_closer_to_f64_than_reference(hip, O, name):
+    import torch
+
+    vol, dmin, dmax, cam, W, H, m, gp = case_inputs(name)
+    mo_ref, b_ref, ns_ref = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    mo64, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp, f64=True)
+    v = gpu_volume(hip, vol, dmin, dmax)
+    mo, _, _ = hip.generate(v, cam, gp, W, H, m, form="mixed")
+    torch.cuda.synchronize()
+    hit = ns_ref > 0
+    e_gpu = normwise_rel(to_np(mo), mo64)[hit]
+    e_ref = normwise_rel(mo_ref, mo64)[hit]
+    print(f"[f64] case={name} gpu-mixed max {e_gpu.max():.2e} median {np.median(e_gpu):.2e} | "
+          f"fp32-oracle max {e_ref.max():.2e} median {np.median(e_ref):.2e}")
+    assert e_gpu.max() <= 1e-5
+    assert np.median(e_gpu) <= 2.0 * np.median(e_ref) + 1e-7
+
+
+def densities(O, m, lag, phases):
+    return np.float32([O.lib.orc_evaluate_mese(m, float(p), ptr(np.ascontiguousarray(lag, np.float32), _f))
+                       for p in phases])
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "mix_persp_m50", "tiny_m1", "tiny_m2", "ml_nobounds_m7"])
+def test_prepare_parity(hip, O, name, form):
+    import torch
+
+    vol, dmin, dmax, cam, W, H, m, gp = case_inputs(name)
+    mo_ref, b_ref, ns_ref = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    bias = 5e-5
+    lag = hip.prepare(torch.from_numpy(mo_ref).cuda(), W, H, m, bias, form=form)
+    torch.cuda.synchronize()
+    lag = to_np(lag).reshape(-1, m)
+    phases = np.linspace(-np.pi, 0.0, 41)
+    worst = 0.0
+    hit = np.nonzero(ns_ref > 0)[0]
+    for p in hit[:: max(1, len(hit) // 64)]:
+        mom = mo_ref.reshape(-1, m)[p]
+        if form == "plain":
+            ref = O.prepare_std(m, mom, bias)
+            rho_ref = densities(O, m, ref, phases)
+        else:
+            ref = O.prepare_std_f64(m, mom.astype(np.float64), bias)
+            rho_ref = np.array([O.lib.orc_evaluate_mese_f64(m, float(ph), ptr(ref, C.POINTER(C.c_double)))
+                                for ph in phases])
+        rho = densities(O, m, lag[p], phases)
+        worst = max(worst, float(np.abs(rho - rho_ref).max()))
+    print(f"[prepare] case={name} form={form} max density diff {worst:.2e}")
+    assert worst <= (5e-5 if form == "mixed" else 1e-4)
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "ml_persp_m40", "mix_persp_m50", "u8_ortho_m16", "tiny_m2"])
+def test_reconstruct_parity(hip, O, name, form):
+    import torch
+
+    vol, dmin, dmax, cam, W, H, m, gp = case_inputs(name)
+    mo_ref, b_ref, ns_ref = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    rp = rec_params(gp)
+    lag_ref = np.stack([O.prepare_std(m, mo_ref.reshape(-1, m)[p], rp.bias) for p in range(W * H)])
+    tf8 = synth.tf_ramps(0)
+    table = tfmod.table_from_rgba8(tf8, rp.step_size)
+    bnd = b_ref if gp.transform_to_bounds else None
+    rgba_ref, st_ref = O.reconstruct(cam, dmin, dmax, W, H, m, lag_ref.reshape(-1), table, rp, bounds=bnd)
+    print(f"[reconstruct] case={name} form={form}", flush=True)
+    lag_d = torch.from_numpy(np.ascontiguousarray(lag_ref, np.float32)).cuda()
+    b_d = torch.from_numpy(bnd.view(np.int32)).cuda() if bnd is not None else None
+    rgba, st = hip.reconstruct(cam, dmin, dmax, lag_d, W, H, m, torch.from_numpy(table), rp, form=form, bounds=b_d,
+                               want_steps=True)
+    torch.cuda.synchronize()
+    rgba, st = to_np(rgba), to_np(st)
+    same = st == st_ref
+    diff = np.abs(rgba - rgba_ref).reshape(-1, 4)[same]
+    print(f"  max |diff| {diff.max():.2e}, rays with other step count {np.count_nonzero(~same)}/{W * H}")
+    assert np.count_nonzero(~same) <= max(2, 0.01 * W * H)
+    assert diff.max() <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "mix_persp_m50", "ml_persp_m40"])
+def test_pipeline_mixed_vs_f64(hip, O, name):
+    import torch
+
+    vol, dmin, dmax, cam, W, H, m, gp = case_inputs(name)
+    rp = rec_params(gp)
+    mo64, b64 = O.generate(vol, dmin, dmax, cam, W, H, m, gp, f64=True)
+    _, b32, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    tf8 = synth.tf_ramps(0)
+    table = tfmod.table_from_rgba8(tf8, rp.step_size)
+    rgba64, st64 = O.reconstruct_f64(cam, dmin, dmax, W, H, m, mo64, table, rp, bounds=b64)
+    v = gpu_volume(hip, vol, dmin, dmax)
+    mo, b, _ = hip.generate(v, cam, gp, W, H, m, form="mixed")
+    lag = hip.prepare(mo, W, H, m, rp.bias, form="mixed")
+    rgba, st = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, torch.from_numpy(table), rp, form="mixed", bounds=b,
+                               want_steps=True)
+    torch.cuda.synchronize()
+    rgba, st = to_np(rgba), to_np(st)
+    same = (st == st64) & (bounds_np(b) == b64)
+    d = (rgba - rgba64).reshape(-1, 4)
+    rmse = float(np.sqrt((d[same] ** 2).mean()))
+    print(f"[pipeline mixed vs f64] case={name} RMSE {rmse:.2e} max {np.abs(d[same]).max():.2e} "
+          f"separate rays {np.count_nonzero(~same)}")
+    assert rmse <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "mix_persp_m50"])
+def test_pipeline_plain_vs_f32(hip, O, name):
+    import torch
+
+    vol, dmin, dmax, cam, W, H, m, gp = case_inputs(name)
+    rp = rec_params(gp)
+    mo_ref, b_ref, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    tf8 = synth.tf_ramps(0)
+    table = tfmod.table_from_rgba8(tf8, rp.step_size)
+    rgba_ref, st_ref = O.reconstruct(cam, dmin, dmax, W, H, m, mo_ref.reshape(-1), table, rp, bounds=b_ref)
+    v = gpu_volume(hip, vol, dmin, dmax)
+    mo, b, _ = hip.generate(v, cam, gp, W, H, m, form="plain")
+    lag = hip.prepare(mo, W, H, m, rp.bias, form="plain")
+    rgba, st = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, torch.from_numpy(table), rp, form="plain", bounds=b,
+                               want_steps=True)
+    torch.cuda.synchronize()
+    rgba, st = to_np(rgba), to_np(st)
+    same = st == st_ref
+    d = np.abs(rgba - rgba_ref).reshape(-1, 4)[same]
+    print(f"[pipeline plain vs f32] case={name} max {d.max():.2e} rms {np.sqrt((d ** 2).mean()):.2e}")
+    assert np.sqrt((d ** 2).mean()) <= 1e-4
+
+
+def test_golden_images_on_gpu(hip, O):
+    import torch
+
+    g = dict(np.load(GOLD / "golden_images.npz"))
+    case = "ml_ortho"
+    W, H, m = 24, 20, 12
+    gp = GenParams(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                   prediction_coding=0, bias=5e-5)
+    v = gpu_volume(hip, g[f"{case}_volume"], (-1, -1, -1), (1, 1, 1))
+    for form in ("plain", "mixed"):
+        mo, b, _ = hip.generate(v, synth.ortho_camera(), gp, W, H, m, form=form)
+        torch.cuda.synchronize()
+        assert np.array_equal(bounds_np(b), g[f"{case}_bounds"])
+        err = normwise_rel(to_np(mo), g[f"{case}_data"].reshape(H, W, m))
+        assert err.max() <= 1e-5
+
+
+def test_error_paths(hip):
+    import torch
+
+    from mray_amd.hip import MrayError
+
+    vol = torch.zeros((4, 4, 4), dtype=torch.float32, device="cuda")
+    v = hip.Volume(vol, (0, 0, 0), (1, 1, 1))
+    with pytest.raises(MrayError):
+        hip.generate(v, synth.ortho_camera(), GenParams(step_size=0.0), 4, 4, 4)
+    with pytest.raises(MrayError):
+        hip.generate(v, synth.ortho_camera(), GenParams(step_size=0.1), 4, 4, 300)
+    with pytest.raises(MrayError):
+        hip.Volume(vol, (0, 0, 0), (0, 1, 1))
