@@ -16,7 +16,8 @@
 //                      - (f_0 / j) sin(j phi_0),
 //         so a node costs one phasor rotation (re-seeded with an exact
 //         sincos every kReseed nodes) and a Chebyshev recurrence over j:
-//         2 FMAs per moment instead of 2 sin/cos. Accumulation is blocked:
+//         3 FMAs per moment instead of 2 sin/cos (the third applies the
+//         first-order correction for the reference's overlapping segments). Accumulation is blocked:
 //         kFold nodes go into a partial sum that is then added with Kahan
 //         compensation; trig_0 (the trapezoid rule) is accumulated in FP64.
 //         Moments live in registers (template M >= m).
@@ -171,18 +172,21 @@ constexpr int kReseed = 16; // exact sincos of the node phase every kReseed node
 constexpr int kFold = 8;    // nodes per partial sum before the Kahan add
 
 template <int M>
-__device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float x)
+__device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float e, float x)
 {
-    // part[j] += d * cos(j*phi) for j = 1..M-1, cos(j*phi) by the Chebyshev
-    // recurrence c_j = 2x c_{j-1} - c_{j-2} from x = cos(phi).
+    // part[j] += (d + j^2 e) * cos(j*phi) for j = 1..M-1, cos(j*phi) by the
+    // Chebyshev recurrence c_j = 2x c_{j-1} - c_{j-2} from x = cos(phi).
+    // e carries the first-order term of the reference's segment overlap at
+    // this node (its segments are [p_k - dphi, p_k] with p_k recomputed from
+    // t, so consecutive segments overlap or leave a gap of a few ulp).
     const float x2 = x + x;
     float cm2 = 1.0f, cm1 = x;
-    part[1] = __builtin_fmaf(d, cm1, part[1]);
+    part[1] = __builtin_fmaf(__builtin_fmaf(1.0f, e, d), cm1, part[1]);
 #pragma unroll
     for (int j = 2; j < M; ++j)
     {
-        float c = __builtin_fmaf(x2, cm1, -cm2);
-        part[j] = __builtin_fmaf(d, c, part[j]);
+        const float c = __builtin_fmaf(x2, cm1, -cm2);
+        part[j] = __builtin_fmaf(__builtin_fmaf((float)(j * j), e, d), c, part[j]);
         cm2 = cm1;
         cm1 = c;
     }
@@ -286,7 +290,12 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         hi = rmax(hi, f);
         const float g = (f - f_prev) * inv_dphi;
         trap += (double)dphi * (0.5 * ((double)f_prev + (double)f));
-        accumulate_node<M>(part, g_left - g, zc);
+        // the next segment starts at p - dphi (update_moments' p_prev), which
+        // misses this node by delta; to first order that adds
+        // -delta * f_k * cos(j p_k) to trig_j (node 0 and the last node have
+        // delta = 0 by construction).
+        const float delta = (node > 0) ? (p - dphi) - node_phase : 0.0f;
+        accumulate_node<M>(part, g_left - g, -delta * f_prev, zc);
         g_left = g;
         f_prev = f;
         node_phase = p;
@@ -311,7 +320,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float dl = 0.0f - node_phase;
     const float g_last = (f_last - f_prev) / dl;
     trap += (double)dl * (0.5 * ((double)f_prev + (double)f_last));
-    accumulate_node<M>(part, g_left - g_last, zc);
+    accumulate_node<M>(part, g_left - g_last, 0.0f, zc);
     // node at phase 0: cos(j*0) = 1
 #pragma unroll
     for (int j = 1; j < M; ++j)
