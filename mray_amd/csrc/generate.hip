@@ -9,18 +9,23 @@
 //         sample, one thread per ray, accumulators in LDS (what a direct port
 //         of generate_moments_kernel does).
 //
-//  MIXED  the same integral (exact for the piecewise-linear signal) written
-//         per node instead of per segment. For segments with slopes g_s the
-//         i/j terms telescope and
+//  MIXED  the same integral written per node instead of per segment. For
+//         contiguous segments with slopes g_s the i/j terms telescope and
 //             trig_j = (1/j^2) sum_k (g_k - g_{k+1}) cos(j phi_k)
-//                      - (f_0 / j) sin(j phi_0),
-//         so a node costs one phasor rotation (re-seeded with an exact
-//         sincos every kReseed nodes) and a Chebyshev recurrence over j:
-//         3 FMAs per moment instead of 2 sin/cos (the third applies the
-//         first-order correction for the reference's overlapping segments). Accumulation is blocked:
-//         kFold nodes go into a partial sum that is then added with Kahan
-//         compensation; trig_0 (the trapezoid rule) is accumulated in FP64.
-//         Moments live in registers (template M >= m).
+//                      - (f_0 / j) sin(j phi_0).
+//         The reference's segments are [p_k - dphi, p_k] with p_k recomputed
+//         from t, so neighbours overlap by a few ulp (delta), and its lines
+//         use the slope (f_k - f_{k-1}) / dphi, so they miss f_k by a few ulp
+//         (eps); both enter to first order, exactly enough to reproduce the
+//         reference's arithmetic-free result to ~1e-13:
+//             + sum_k [ -delta_k f_k cos(j p_k) + h_k sin(j p_k) / j ].
+//         A node costs one phasor rotation (re-seeded with an exact sincos
+//         every kReseed nodes) and two Chebyshev recurrences over j (cos and
+//         sin of j phi): 6 VALU ops per moment instead of 2(m-1) sin/cos.
+//         Accumulation is blocked: kFold nodes go into a partial sum that is
+//         then added with Kahan compensation; trig_0 (the exact integral of the
+//         reference's lines) is accumulated in FP64. Moments live in
+//         registers (template M >= m).
 #include "device_common.hpp"
 #include "launch.hpp"
 
@@ -172,23 +177,26 @@ constexpr int kReseed = 16; // exact sincos of the node phase every kReseed node
 constexpr int kFold = 8;    // nodes per partial sum before the Kahan add
 
 template <int M>
-__device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float e, float x)
+__device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float e, float h, float x, float y)
 {
-    // part[j] += (d + j^2 e) * cos(j*phi) for j = 1..M-1, cos(j*phi) by the
-    // Chebyshev recurrence c_j = 2x c_{j-1} - c_{j-2} from x = cos(phi).
-    // e carries the first-order term of the reference's segment overlap at
-    // this node (its segments are [p_k - dphi, p_k] with p_k recomputed from
-    // t, so consecutive segments overlap or leave a gap of a few ulp).
+    // part[j] += (d + j^2 e) cos(j phi) + j h sin(j phi) for j = 1..M-1 (the
+    // partial sums are scaled by j^2, so the h sin / j term enters as j h sin);
+    // cos and sin of j phi by the Chebyshev recurrence v_j = 2x v_{j-1} - v_{j-2}
+    // from x = cos(phi), y = sin(phi).
     const float x2 = x + x;
     float cm2 = 1.0f, cm1 = x;
-    part[1] = __builtin_fmaf(__builtin_fmaf(1.0f, e, d), cm1, part[1]);
+    float sm2 = 0.0f, sm1 = y;
+    part[1] = __builtin_fmaf(h, sm1, __builtin_fmaf(d + e, cm1, part[1]));
 #pragma unroll
     for (int j = 2; j < M; ++j)
     {
         const float c = __builtin_fmaf(x2, cm1, -cm2);
-        part[j] = __builtin_fmaf(__builtin_fmaf((float)(j * j), e, d), c, part[j]);
+        const float sn = __builtin_fmaf(x2, sm1, -sm2);
+        part[j] = __builtin_fmaf((float)j * h, sn, __builtin_fmaf(__builtin_fmaf((float)(j * j), e, d), c, part[j]));
         cm2 = cm1;
         cm1 = c;
+        sm2 = sm1;
+        sm1 = sn;
     }
 }
 
@@ -276,7 +284,8 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         comp[j] = 0.0f;
     }
     double trap = 0.0;
-    float g_left = 0.0f;
+    float g_left = 0.0f;   // slope of the segment ending at the current node
+    float eps_left = 0.0f; // its width minus dphi
     int node = 0;
 
     const int cap = march_cap(t_near, t_far, s);
@@ -288,15 +297,16 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         ++ns;
         lo = rmin(lo, f);
         hi = rmax(hi, f);
-        const float g = (f - f_prev) * inv_dphi;
-        trap += (double)dphi * (0.5 * ((double)f_prev + (double)f));
-        // the next segment starts at p - dphi (update_moments' p_prev), which
-        // misses this node by delta; to first order that adds
-        // -delta * f_k * cos(j p_k) to trig_j (node 0 and the last node have
-        // delta = 0 by construction).
-        const float delta = (node > 0) ? (p - dphi) - node_phase : 0.0f;
-        accumulate_node<M>(part, g_left - g, -delta * f_prev, zc);
+        // the reference's segment [u, p]: u = p - dphi in float, slope (f - f_prev) / dphi
+        const float u = p - dphi;
+        const float g = (f - f_prev) / dphi;
+        const double width = (double)p - (double)u;
+        trap += width * ((double)f_prev + 0.5 * (double)g * width);
+        const float delta = (node > 0) ? (float)((double)u - (double)node_phase) : 0.0f;
+        const float h = __builtin_fmaf(g, delta, g_left * eps_left);
+        accumulate_node<M>(part, g_left - g, -delta * f_prev, h, zc, zs);
         g_left = g;
+        eps_left = (float)(width - (double)dphi);
         f_prev = f;
         node_phase = p;
         ++node;
@@ -312,16 +322,17 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         if (node % kFold == 0)
             kahan_fold<M>(part, sum, comp);
     }
-    // last segment [p_K, 0] (src/raymarch_generate.h:107-114)
+    // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
+    // exactly at p_K and its line ends exactly at f_last
     const float f_last = sample_at(vol, r, t_far, a);
     ++ns;
     lo = rmin(lo, f_last);
     hi = rmax(hi, f_last);
     const float dl = 0.0f - node_phase;
     const float g_last = (f_last - f_prev) / dl;
-    trap += (double)dl * (0.5 * ((double)f_prev + (double)f_last));
-    accumulate_node<M>(part, g_left - g_last, 0.0f, zc);
-    // node at phase 0: cos(j*0) = 1
+    trap += (double)dl * ((double)f_prev + 0.5 * (double)g_last * (double)dl);
+    accumulate_node<M>(part, g_left - g_last, 0.0f, g_left * eps_left, zc, zs);
+    // node at phase 0: cos(j*0) = 1, sin(j*0) = 0
 #pragma unroll
     for (int j = 1; j < M; ++j)
         part[j] += g_last;
@@ -404,11 +415,13 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
     MRAY_GEN_CASE(20)
     MRAY_GEN_CASE(24)
     MRAY_GEN_CASE(28)
+    MRAY_GEN_CASE(30)
     MRAY_GEN_CASE(32)
     MRAY_GEN_CASE(36)
     MRAY_GEN_CASE(40)
     MRAY_GEN_CASE(44)
     MRAY_GEN_CASE(48)
+    MRAY_GEN_CASE(50)
     MRAY_GEN_CASE(52)
     MRAY_GEN_CASE(56)
     MRAY_GEN_CASE(60)

@@ -308,11 +308,13 @@ int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *d
     MRAY_REC_CASE(20)
     MRAY_REC_CASE(24)
     MRAY_REC_CASE(28)
+    MRAY_REC_CASE(30)
     MRAY_REC_CASE(32)
     MRAY_REC_CASE(36)
     MRAY_REC_CASE(40)
     MRAY_REC_CASE(44)
     MRAY_REC_CASE(48)
+    MRAY_REC_CASE(50)
     MRAY_REC_CASE(52)
     MRAY_REC_CASE(56)
     MRAY_REC_CASE(60)

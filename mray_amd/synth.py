@@ -113,8 +113,10 @@ def value_noise(x, y, z, seed: int, octaves: int, base: int) -> np.ndarray:
     return total / norm
 
 
-def mixing_layer(n: int, seed: int = 1) -> np.ndarray:
-    """Mixing-layer density on [0,1]^3, n^3 float32, layout [z][y][x]."""
+def mixing_layer(n: int, seed: int = 1, width: float = 0.01) -> np.ndarray:
+    """Mixing-layer density on [0,1]^3, n^3 float32, layout [z][y][x].
+
+    width is the tanh thickness of the interface (0.01 in the benchmark)."""
     amp, theta, phi = mixing_layer_modes(seed)
     c = voxel_coords(n, 0.0, 1.0)
     y2, x2 = np.meshgrid(c, c, indexing="ij")
@@ -124,7 +126,7 @@ def mixing_layer(n: int, seed: int = 1) -> np.ndarray:
     out = np.empty((n, n, n), dtype=np.float32)
     for iz in range(n):
         z = c[iz]
-        layer = 0.5 + 0.5 * np.tanh((z - 0.5 - h) / 0.01)
+        layer = 0.5 + 0.5 * np.tanh((z - 0.5 - h) / width)
         noise = value_noise(x2, y2, np.full_like(x2, z), seed, 4, 8)
         out[iz] = (layer + 0.05 * noise).astype(np.float32)
     return out

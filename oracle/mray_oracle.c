@@ -378,8 +378,12 @@ void orc_generate_image(const orc_volume *v, const mray_camera *cam, int W, int 
     }
 }
 
-/* FP64 generate over the same nodes: the float t sequence and node count of
- * the reference loop, everything else in double with exact sin/cos. */
+/* FP64 generate over the same discretisation: the reference's float t
+ * sequence, node count and float sample values (trilinear in float, as
+ * RegularGrid::sample), so that bounds are the same 16-bit values; the phases
+ * of the reference's segments [p_k - dphi, p_k] are the reference's float
+ * values, and the moment integral of each segment is evaluated in double with
+ * exact sin/cos. */
 uint32_t orc_generate_ray_f64(const orc_volume *v, const float org[3], const float dir[3], float znear,
                               double *moments, int m, const mray_gen_params *gp)
 {
@@ -396,81 +400,71 @@ uint32_t orc_generate_ray_f64(const orc_volume *v, const float org[3], const flo
                 moments[j] = 0.0;
         return 0;
     }
-    double dmin = gp->data_min, dmax = gp->data_max;
     float step = gp->step_size;
-    double tn = t_near, tf = t_far;
-    double phase_step = M_PI * ((double)step / (tf - tn));
-    double lo = 1.0, hi = 0.0;
-
-    double pos[3];
+    float lo = 1.f, hi = 0.f;
     float t = t_near;
-    for (int i = 0; i < 3; ++i)
-        pos[i] = (double)org[i] + (double)t * dir[i];
-    double fs = dclampr((orc_volume_sample_f64(v, pos) - dmin) / (dmax - dmin), 0.0, 1.0);
-    lo = dminr(lo, fs);
-    hi = dmaxr(hi, fs);
-    double p = -M_PI;
-    int first = 1;
-
+    float pi_trange_inv = (float)(M_PI / (double)(t_far - t_near));
+    float p = (float)(-M_PI);
+    float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+    float pos[3];
+    sample_at(v, org, dir, t, pos);
+    float fs = map_density(orc_volume_sample(v, pos), gp);
+    lo = fminr(lo, fs);
+    hi = fmaxr(hi, fs);
     for (;;)
     {
         int last = !(t + step < t_far);
-        double p_next, seg;
-        double bs;
+        float p_next, seg, bs;
         if (!last)
         {
             t += step;
-            p_next = ((double)t - tn) * (M_PI / (tf - tn)) - M_PI;
+            p_next = (float)((double)((t - t_near) * pi_trange_inv) - M_PI);
             seg = phase_step;
-            for (int i = 0; i < 3; ++i)
-                pos[i] = (double)org[i] + (double)t * dir[i];
+            sample_at(v, org, dir, t, pos);
         }
         else
         {
-            p_next = 0.0;
-            seg = 0.0 - p;
-            for (int i = 0; i < 3; ++i)
-                pos[i] = (double)org[i] + tf * dir[i];
+            p_next = 0.0f;
+            seg = 0.0f - p;
+            sample_at(v, org, dir, t_far, pos);
         }
-        bs = dclampr((orc_volume_sample_f64(v, pos) - dmin) / (dmax - dmin), 0.0, 1.0);
-        lo = dminr(lo, bs);
-        hi = dmaxr(hi, bs);
-        double p_prev = p_next - seg;
-        double g = (bs - fs) / seg;
-        double y0 = fs - g * p_prev;
+        bs = map_density(orc_volume_sample(v, pos), gp);
+        lo = fminr(lo, bs);
+        hi = fmaxr(hi, bs);
+        /* segment [pp, pn] = [p_next - seg, p_next] with the reference's float
+         * end points; the line has the reference's slope (bs - fs) / seg and
+         * passes through (pp, fs) (update_moments, src/raymarch_generate.h:37-40) */
+        double pn = p_next, pp = (double)(p_next - seg), sg = pn - pp;
+        double f0 = fs, f1 = bs;
+        double g = (f1 - f0) / (double)seg;
+        double f1l = f0 + g * sg; /* the line's value at pn */
         for (int j = 1; j < m; ++j)
         {
             double jd = j;
-            /* Re[(g/j^2 + i(y + g phi)/j) e^{-i j phi}] evaluated at both ends */
-            double cn = cos(jd * p_next), sn = sin(jd * p_next);
-            double cp = cos(jd * p_prev), sp = sin(jd * p_prev);
-            double an = g / (jd * jd), bn = (y0 + g * p_next) / jd;
-            double bp = (y0 + g * p_prev) / jd;
-            trig[j] += (an * cn + bn * sn) - (an * cp + bp * sp);
+            /* Re[(g/j^2 + i(f0 + g(phi - pp))/j) e^{-i j phi}] from pp to pn */
+            double cn = cos(jd * pn), sn = sin(jd * pn);
+            double cp = cos(jd * pp), sp = sin(jd * pp);
+            trig[j] += (g / (jd * jd)) * (cn - cp) + (f1l * sn - f0 * sp) / jd;
         }
-        trig[0] += (0.5 * g * p_next * p_next + y0 * p_next) - (0.5 * g * p_prev * p_prev + y0 * p_prev);
+        trig[0] += sg * (f0 + 0.5 * g * sg);
         fs = bs;
         p = p_next;
-        first = 0;
         if (last)
             break;
     }
-    (void)first;
     if (gp->transform_to_bounds)
     {
-        /* the relaxed bound is formed in float exactly like the reference so
-         * that the stored 16-bit bound is the same pixel value */
-        float flo = fmaxr((float)lo - gp->transform_bound_eps, 0.0f);
-        float fhi = fminr((float)hi + gp->transform_bound_eps, 1.0f);
-        double inv = 1.0 / ((double)fhi - (double)flo);
-        moments[0] = (trig[0] / M_PI - flo) * inv;
+        lo = fmaxr(lo - gp->transform_bound_eps, 0.0f);
+        hi = fminr(hi + gp->transform_bound_eps, 1.0f);
+        double inv = 1.0 / ((double)hi - (double)lo);
+        moments[0] = (trig[0] / M_PI - lo) * inv;
         for (int j = 1; j < m; ++j)
             moments[j] = trig[j] / M_PI * inv;
-        return pack_bound(flo, fhi);
     }
-    for (int j = 0; j < m; ++j)
-        moments[j] = trig[j] / M_PI;
-    return pack_bound((float)lo, (float)hi);
+    else
+        for (int j = 0; j < m; ++j)
+            moments[j] = trig[j] / M_PI;
+    return pack_bound(lo, hi);
 }
 
 void orc_generate_image_f64(const orc_volume *v, const mray_camera *cam, int W, int H, int m,
