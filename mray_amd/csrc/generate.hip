@@ -19,9 +19,12 @@
 //         (eps); both enter to first order, exactly enough to reproduce the
 //         reference's arithmetic-free result to ~1e-13:
 //             + sum_k [ -delta_k f_k cos(j p_k) + h_k sin(j p_k) / j ].
-//         A node costs one phasor rotation (re-seeded with an exact sincos
-//         every kReseed nodes) and two Chebyshev recurrences over j (cos and
-//         sin of j phi): 6 VALU ops per moment instead of 2(m-1) sin/cos.
+//         A node costs one sincos of its (float) phase and two Chebyshev
+//         recurrences over j (cos and sin of j phi): 6 VALU ops per moment
+//         instead of 2(m-1) sin/cos. The phasor is not carried from node to
+//         node by rotation: the reference's node phases jitter by ulps around
+//         the uniform grid, and a rotated phasor drifts off cos(p_k) by more
+//         than the high orders tolerate.
 //         Accumulation is blocked: kFold nodes go into a partial sum that is
 //         then added with Kahan compensation; trig_0 (the exact integral of the
 //         reference's lines) is accumulated in FP64. Moments live in
@@ -173,7 +176,6 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 // MIXED
 // ---------------------------------------------------------------------------
-constexpr int kReseed = 16; // exact sincos of the node phase every kReseed nodes
 constexpr int kFold = 8;    // nodes per partial sum before the Kahan add
 
 template <int M>
@@ -250,9 +252,6 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float s = a.step;
     const float pi_trange_inv = (float)(kPi / (double)(t_far - t_near));
     const float dphi = (float)(kPi * (double)(s / (t_far - t_near)));
-    const float inv_dphi = 1.0f / dphi;
-    float rot_s, rot_c;
-    sincosf(dphi, &rot_s, &rot_c);
 
     float t = t_near;
     float f_prev = sample_at(vol, r, t, a);
@@ -310,15 +309,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         f_prev = f;
         node_phase = p;
         ++node;
-        if (node % kReseed == 0)
-            sincosf(p, &zs, &zc);
-        else
-        {
-            float nc = zc * rot_c - zs * rot_s;
-            float nsn = zs * rot_c + zc * rot_s;
-            zc = nc;
-            zs = nsn;
-        }
+        sincosf(p, &zs, &zc);
         if (node % kFold == 0)
             kahan_fold<M>(part, sum, comp);
     }

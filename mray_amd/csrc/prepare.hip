@@ -176,15 +176,17 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     dc *E = reinterpret_cast<dc *>(lds_d + 8 * MP) + q * MP;
     dc *A = reinterpret_cast<dc *>(lds_d + 8 * MP) + 8 * MP + q * MP;
 
-    // biased moments (lerp in float, as the reference)
+    // biased moments, lerp in FP64 (near the boundary of the moment space the
+    // recursion below amplifies a one-ulp change of the input)
+    const double b64 = (double)bias;
 #pragma unroll
     for (int s = 0; s < C; ++s)
     {
         const int e = g + 8 * s;
         if (e < nm)
         {
-            float v = mom[off + e];
-            T[e] = (double)(e == 0 ? lerp_bias(v, 0.5f, bias) : lerp_bias(v, 0.0f, bias));
+            const double v = (double)mom[off + e];
+            T[e] = (1.0 - b64) * v + (e == 0 ? b64 * 0.5 : 0.0);
         }
     }
     wave_sync();

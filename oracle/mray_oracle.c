@@ -984,9 +984,57 @@ void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const fl
     }
 }
 
-/* FP64 reconstruct: same t loop and the same float phase sequence (the
- * reference increments the phase in float), densities, TF lerp and
- * compositing in double with exact atan and exp. */
+/* FP64 reconstruct of one ray from given multipliers: the reference's t
+ * loop and float phase sequence (it increments the phase in float),
+ * densities, TF lerp and compositing in double with exact atan and exp. */
+static void reconstruct_ray_f64(const float org[3], const float dir[3], const float bmin[3], const float bmax[3],
+                                float znear, int m, const double *pm, double blo, double bhi, const float *tf,
+                                const mray_rec_params *rp, double *out, int32_t *n_steps)
+{
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    int ns = 0;
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    if (hit && (t_far > 0))
+    {
+        float step = rp->step_size;
+        float t = t_near;
+        float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+        float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+        double L[3] = {0, 0, 0}, tau = 0.0, T = 1.0, alpha = 0.0;
+        double pc = dclampr(p, -M_PI, 0.0);
+        double fs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
+        while (t <= t_far)
+        {
+            t += step;
+            p += phase_step;
+            ++ns;
+            pc = dclampr(p, -M_PI, 0.0);
+            double bs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
+            double c[4];
+            tf_sample_f64(tf, fs, bs, c);
+            for (int k = 0; k < 3; ++k)
+                L[k] += rp->medium_emission * c[k] * T;
+            tau += c[3];
+            T = exp(-tau);
+            alpha = 1.0 - T;
+            fs = bs;
+            if (alpha >= rp->early_ray_termination)
+            {
+                alpha = 1.0;
+                break;
+            }
+        }
+        out[0] = L[0];
+        out[1] = L[1];
+        out[2] = L[2];
+        out[3] = alpha;
+    }
+    if (n_steps)
+        *n_steps = ns;
+}
+
 void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
                                const double *moments, const uint32_t *bounds, const float *tf,
                                const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
@@ -998,58 +1046,38 @@ void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], cons
         for (int x = 0; x < W; ++x)
         {
             size_t pix = (size_t)y * W + x;
-            double *out = rgba + pix * 4;
             orc_prepare_moments_std_f64(m, moments + pix * (size_t)m, rp->bias, pm);
-            double blo = 0.0, bhi = 1.0;
-            if (bounds)
-            {
-                blo = MRAY_BOUND_LO(bounds[pix]) / 65535.0;
-                bhi = MRAY_BOUND_HI(bounds[pix]) / 65535.0;
-            }
+            double blo = bounds ? MRAY_BOUND_LO(bounds[pix]) / 65535.0 : 0.0;
+            double bhi = bounds ? MRAY_BOUND_HI(bounds[pix]) / 65535.0 : 1.0;
             float org[3], dir[3];
             orc_pixel_ray(cam, W, H, x, y, org, dir);
-            float t_near, t_far;
-            int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
-            t_near = fmaxr(t_near, cam->znear);
-            int ns = 0;
-            out[0] = out[1] = out[2] = out[3] = 0.0;
-            if (hit && (t_far > 0))
-            {
-                float step = rp->step_size;
-                float t = t_near;
-                float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
-                float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
-                double L[3] = {0, 0, 0}, tau = 0.0, T = 1.0, alpha = 0.0;
-                double pc = dclampr(p, -M_PI, 0.0);
-                double fs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
-                while (t <= t_far)
-                {
-                    t += step;
-                    p += phase_step;
-                    ++ns;
-                    pc = dclampr(p, -M_PI, 0.0);
-                    double bs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
-                    double c[4];
-                    tf_sample_f64(tf, fs, bs, c);
-                    for (int k = 0; k < 3; ++k)
-                        L[k] += rp->medium_emission * c[k] * T;
-                    tau += c[3];
-                    T = exp(-tau);
-                    alpha = 1.0 - T;
-                    fs = bs;
-                    if (alpha >= rp->early_ray_termination)
-                    {
-                        alpha = 1.0;
-                        break;
-                    }
-                }
-                out[0] = L[0];
-                out[1] = L[1];
-                out[2] = L[2];
-                out[3] = alpha;
-            }
-            if (n_steps)
-                n_steps[pix] = ns;
+            reconstruct_ray_f64(org, dir, bmin, bmax, cam->znear, m, pm, blo, bhi, tf, rp, rgba + pix * 4,
+                                n_steps ? n_steps + pix : NULL);
+        }
+    }
+}
+
+/* FP64 evaluation of given (float) multipliers: isolates the reconstruct
+ * march from the preparation. */
+void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                  int m, const float *lagrange, const uint32_t *bounds, const float *tf,
+                                  const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
+{
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+    {
+        double pm[MAXM];
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            for (int l = 0; l < m; ++l)
+                pm[l] = lagrange[pix * (size_t)m + l];
+            double blo = bounds ? MRAY_BOUND_LO(bounds[pix]) / 65535.0 : 0.0;
+            double bhi = bounds ? MRAY_BOUND_HI(bounds[pix]) / 65535.0 : 1.0;
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            reconstruct_ray_f64(org, dir, bmin, bmax, cam->znear, m, pm, blo, bhi, tf, rp, rgba + pix * 4,
+                                n_steps ? n_steps + pix : NULL);
         }
     }
 }

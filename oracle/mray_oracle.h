@@ -12,10 +12,11 @@
  *     operation order and its float/double promotions (including the
  *     double-precision cos/sin/exp that the reference's unqualified calls
  *     resolve to). Pinned bit-exact against the compiled reference
- *     (oracle/_ref) by tests/test_oracle_ref.py.
- *   - *_f64: the same discretisation (identical sample positions, loop counts
- *     and float phase sequence) evaluated in FP64 with exact cos/sin/atan/exp.
- *     This is the "FP64 oracle" the precision targets are stated against.
+ *     (oracle/_ref) by tests/test_oracle_golden.py.
+ *   - *_f64: the same discretisation (the reference's float sample values,
+ *     16-bit bounds, loop counts, segment end points and float phase
+ *     sequence) evaluated in FP64 with exact cos/sin/atan/exp. This is the
+ *     "FP64 oracle" the precision targets are stated against.
  *
  * Each function cites the reference file:line it restates.
  */
@@ -92,6 +93,9 @@ void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const fl
 void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
                                const double *moments, const uint32_t *bounds, const float *tf,
                                const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);
+void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                  int m, const float *lagrange, const uint32_t *bounds, const float *tf,
+                                  const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);
 
 /* ---- compaction (src/moment_compaction.h, src/moment_image.cpp:69-94) ---- */
 void orc_evaluate_unknown_moments(int n, const float *exp_c, int m, float *new_exp_c);

@@ -129,6 +129,17 @@ class Oracle:
                                            ptr(tf, _f), C.byref(rp), ptr(rgba, _d), ptr(ns, _i32), nthreads())
         return rgba, ns
 
+    def reconstruct_lagrange_f64(self, cam, dmin, dmax, W, H, m, lag32, tf, rp: RecParams, bounds=None):
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float64)
+        ns = np.zeros(W * H, np.int32)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_reconstruct_lagrange_f64(C.byref(c), bmin, bmax, W, H, m,
+                                              ptr(np.ascontiguousarray(lag32, np.float32), _f), ptr(bounds, _u32),
+                                              ptr(tf, _f), C.byref(rp), ptr(rgba, _d), ptr(ns, _i32), nthreads())
+        return rgba, ns
+
     def prepare_std(self, m, moments, bias):
         out = np.zeros(m, np.float32)
         self.lib.orc_prepare_moments_std(m, ptr(np.ascontiguousarray(moments, np.float32), _f), C.c_float(bias),
