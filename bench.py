@@ -78,7 +78,11 @@ def main():
     # MESE bias 1e-3: at 50 moments the sharp mixing layer needs it for float
     # moment images to be well posed (lerp toward uniform-density moments)
     rp = RecParams(step_size=step, bias=1e-3, early_ray_termination=0.99, medium_emission=1.0)
-    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size)).cuda()
+    # opacity scale (the reference's preintegration_step_size_factor): the ramps
+    # of the seed-0 transfer function only cover the thin interface, so a large
+    # factor gives a partly opaque image (mean alpha ~0.2)
+    tf_factor = 256.0
+    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size * tf_factor)).cuda()
 
     print(f"[bench] generating {n}^3 mixing layer on the GPU", file=sys.stderr, flush=True)
     vox = hip.synth_mixing_layer(n, seed=1)
@@ -150,7 +154,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step)
+        cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step, tf_factor)
 
     line = {
         "metric": "Mrays/s gen+reconstruct, mixed vs plain form at RMSE<=1e-4; adjoint/forward",
@@ -184,7 +188,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, row_step):
+def cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, row_step, tf_factor):
     """The C oracle (reference FP32 arithmetic, OpenMP over rows) on every
     row_step-th row of the same frame; extrapolated to Mrays/s."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -197,7 +201,7 @@ def cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, row_step):
     print(f"[bench] cpu baseline: copying volume to host, {threads} threads, every {row_step}th row",
           file=sys.stderr, flush=True)
     vol = vox.cpu().numpy()
-    table = tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size)
+    table = tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size * tf_factor)
     t0 = time.perf_counter()
     mo, b, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp, row0=0, row_step=row_step)
     t1 = time.perf_counter()
