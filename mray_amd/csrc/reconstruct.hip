@@ -11,10 +11,13 @@
 //         strided through LDS (as reconstruct_moments_kernel).
 //
 //  MIXED  multipliers in registers (template M >= m); the series
-//         lambda_0 + 2 sum_l lambda_l cos(l theta) by Clenshaw's recurrence
-//         b_l = lambda_l + 2 cos(theta) b_{l+1} - b_{l+2}: 2 VALU ops per
-//         term and a single cos per sample. The 256-entry transfer function
-//         is staged in LDS once per workgroup.
+//         lambda_0 + 2 sum_l lambda_l cos(l theta) by Reinsch's form of
+//         Clenshaw's recurrence in FP64 (3 VALU ops per term, one sincos of
+//         the half angle per sample): at 50 moments the multipliers reach
+//         |lambda| ~ 1e3 and an FP32 series loses ~1e-4 of density to
+//         cancellation, while FP64 FMAs cost the same issue slots as FP32
+//         ones on gfx950. The 256-entry transfer function is staged in LDS
+//         once per workgroup.
 #include "device_common.hpp"
 #include "launch.hpp"
 
@@ -158,21 +161,32 @@ __global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, R
 // MIXED
 // ---------------------------------------------------------------------------
 template <int M>
-__device__ __forceinline__ float series_clenshaw(const float (&lam)[M], int nm, float x)
+__device__ __forceinline__ double series_reinsch(const double (&lam)[M], float p)
 {
-    // lambda_0 + 2 sum_{l=1}^{nm-1} lambda_l cos(l theta), x = cos(theta).
-    // Entries l >= nm hold 0, so running the recurrence over all of M is exact.
-    const float x2 = x + x;
-    float b1 = 0.0f, b2 = 0.0f;
+    // lambda_0 + 2 sum_{l=1}^{M-1} lambda_l cos(l theta), theta = -p in [0, pi]
+    // (entries l >= m hold 0). Reinsch's form of Clenshaw's recurrence,
+    //     d_l = lambda_l + u b_{l+1} + d_{l+1},  b_l = d_l + b_{l+1},
+    //     sum_{l>=1} lambda_l cos(l theta) = d_1 + (u/2) b_1,
+    // with u = -4 sin^2(theta/2) taken from the half angle, stays accurate
+    // where cos(theta) is close to 1 (plain Clenshaw loses digits there). For
+    // theta > pi/2 it runs on theta' = pi - theta with the odd coefficients
+    // negated, so every lane takes the same path. FP64: gfx950 issues FP64
+    // FMAs at the non-packed FP32 rate.
+    float sh, ch;
+    sincosf(-0.5f * p, &sh, &ch);
+    const bool reflect = ch < sh;
+    const double q = reflect ? (double)ch : (double)sh;
+    const double u = -4.0 * q * q;
+    const double sg = reflect ? -1.0 : 1.0;
+    double b = 0.0, d = 0.0;
 #pragma unroll
     for (int l = M - 1; l >= 1; --l)
     {
-        float b0 = __builtin_fmaf(x2, b1, lam[l] - b2);
-        b2 = b1;
-        b1 = b0;
+        const double lk = (l & 1) ? fma(sg, lam[l], d) : lam[l] + d;
+        d = fma(u, b, lk);
+        b = d + b;
     }
-    // sum_{l>=1} lambda_l cos(l theta) = x b_1 - b_2
-    return __builtin_fmaf(2.0f, __builtin_fmaf(x, b1, -b2), lam[0]);
+    return fma(2.0, fma(0.5 * u, b, d), lam[0]);
 }
 
 template <int M>
@@ -209,10 +223,10 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
             steps[pix] = 0;
         return;
     }
-    float lam[M];
+    double lam[M];
 #pragma unroll
     for (int l = 0; l < M; ++l)
-        lam[l] = (l < nm) ? lag[off + l] : 0.0f;
+        lam[l] = (l < nm) ? (double)lag[off + l] : 0.0;
     float blo = 0.f, bhi = 1.f;
     if (bounds)
     {
@@ -224,7 +238,7 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
     const bool fourier = a.fourier != 0;
     auto recon = [&](float p) {
         p = rclamp(p, (float)(-kPi), 0.0f);
-        float s = series_clenshaw<M>(lam, nm, cosf(p));
+        float s = (float)series_reinsch<M>(lam, p);
         float rho = fourier ? s : __builtin_fmaf(fast_atan(s), 0.318309886f, 0.5f);
         return __builtin_fmaf(rho, brange, blo);
     };

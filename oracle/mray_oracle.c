@@ -792,13 +792,36 @@ void orc_prepare_moments_std_f64(int m, const double *moments, double bias, doub
     }
 }
 
+/* fast_atan (src/mese/complex_algebra.h:131-152) evaluated in double: the
+ * polynomial is part of the reference's definition of the MESE density. */
+static double fast_atan_f64(double tan)
+{
+    int negative = (tan < 0.0);
+    double abs_tan = negative ? (-tan) : tan;
+    int greater_one = (abs_tan > 1.0);
+    double x = greater_one ? abs_tan : 1.0;
+    x = 1.0 / x;
+    double y = greater_one ? 1.0 : abs_tan;
+    x = x * y;
+    y = x * x;
+    double z = y * (double)0.020835f - (double)0.085133f;
+    z = y * z + (double)0.180141f;
+    z = y * z - (double)0.330299f;
+    y = y * z + (double)0.999866f;
+    z = y * x;
+    z = z * -2.0 + (double)1.570796f;
+    z = greater_one ? z : 0.0;
+    x = x * y + z;
+    return negative ? (-x) : x;
+}
+
 double orc_evaluate_mese_f64(int m, double phase, const double *lagrange)
 {
     double s = 0.0;
     for (int l = 1; l < m; ++l)
         s += lagrange[l] * cos(l * phase);
     s = 2.0 * s + lagrange[0];
-    return atan(s) / M_PI + 0.5;
+    return fast_atan_f64(s) / M_PI + 0.5;
 }
 
 /* ------------------------------------------------------------------------ */
