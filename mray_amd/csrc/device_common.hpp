@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "../../include/mray_types.h"
+#include "host_sincosf.hpp"
 
 namespace mray
 {

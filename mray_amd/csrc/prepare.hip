@@ -3,8 +3,11 @@
 // prepare_moments_from_std_coding, src/moment_preparation.h:55-64 ->
 // prepareMESELagrange, src/mese/MESE_dynamic.h:194-211).
 //
-//  PLAIN  the reference FP32 arithmetic, one thread per pixel, the three
-//         complex temporaries strided through LDS (as prepareMomentsKernel).
+//  PLAIN  the reference FP32 arithmetic operation for operation, one thread
+//         per pixel, the three complex temporaries strided through LDS (as
+//         prepareMomentsKernel). sinf / cosf of the zeroth exponential
+//         moment reproduce the host C library's results (host_sincosf.hpp),
+//         so the multipliers are bit-identical to the reference CPU path's.
 //
 //  MIXED  the Herglotz transform (trigonometric -> exponential moments) and
 //         the Levinson recursion in FP64, where conditioning at 50+ moments
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(64) void prepare_plain_kernel(const float *__restri
 
     // trigonometricToExponentialMoments (src/mese/MESE_dynamic.h:35-54)
     float ph = 3.14159265f * tb(0) - 1.57079633f;
-    AT(E, 0) = crmul(0.0795774715f, cmk(cosf(ph), sinf(ph)));
+    AT(E, 0) = crmul(0.0795774715f, cmk(host_cosf(ph), host_sinf(ph)));
     for (int l = 1; l < nm; ++l)
     {
         cf acc = cmk(0.0f, 0.0f);

@@ -6,9 +6,12 @@
 // [-pi, 0] per sample, and compositing stops when alpha >= the early ray
 // termination threshold.
 //
-//  PLAIN  evaluateMESELagrange as written: cos/sin of the phase, powers of
-//         the circle point by repeated complex multiplication, multipliers
-//         strided through LDS (as reconstruct_moments_kernel).
+//  PLAIN  evaluateMESELagrange as written: cosf/sinf of the phase (the host
+//         C library's results, host_sincosf.hpp), powers of the circle point
+//         by repeated complex multiplication, transmittance by exp in double
+//         (the reference's unqualified exp of a float), multipliers strided
+//         through LDS (as reconstruct_moments_kernel). Same FP32 operations
+//         as the reference CPU path.
 //
 //  MIXED  multipliers in registers (template M >= m); the series
 //         lambda_0 + 2 sum_l lambda_l cos(l theta) by Reinsch's form of
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, R
     }
     auto recon = [&](float p) {
         p = rclamp(p, (float)(-kPi), 0.0f);
-        float cx = cosf(-p), cy = sinf(-p);
+        float cx = host_cosf(-p), cy = host_sinf(-p);
         // evaluateFourierSeries (src/mese/MESE_dynamic.h:213-226)
         float result = 0.0f;
         float pwx = 1.0f, pwy = 0.0f;
@@ -142,7 +145,7 @@ __global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, R
             Lg += (a.emission * c.y) * T;
             Lb += (a.emission * c.z) * T;
             tau += c.w;
-            T = expf(-tau);
+            T = (float)exp((double)(-tau));
             alpha = 1.0f - T;
             fs = bs;
             if (alpha >= a.ert)
