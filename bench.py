@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-row-step", type=int, default=64, help="CPU baseline: every k-th row")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -55,6 +57,26 @@ def flop_reconstruct(nr, m):
 
 def flop_prepare(npix, m):
     return float(npix) * 20 * m * m
+
+
+KERNEL_SYMBOL = {"generate": "generate_mixed_kernel", "prepare": "prepare_mixed_kernel",
+                 "reconstruct": "reconstruct_mixed_kernel"}
+
+
+def measured_traffic(kernel, n, res, m):
+    """HBM bytes per dispatch of a mixed-form kernel on the default workload,
+    from the calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised in
+    profiles/ (counters cannot be read from inside the profiled program)."""
+    if (n, res, m) != (1024, 1024, 50):
+        return None
+    f = ROOT / "profiles" / "hbm_traffic_mixing_layer_1024.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == KERNEL_SYMBOL[kernel]:
+            return int(parts[2])
+    return None
 
 
 def main():
@@ -88,6 +110,15 @@ def main():
     vox = hip.synth_mixing_layer(n, seed=1)
     vol = hip.Volume(vox, dmin, dmax)
     torch.cuda.synchronize()
+
+    if args.copy_mib:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the per-dispatch HBM traffic read for the kernels below
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     mo = torch.empty((H, W, m), dtype=torch.float32, device="cuda")
     lag = torch.empty_like(mo)
@@ -132,18 +163,19 @@ def main():
     ms_step = wall * 1e3 / args.steps
     mrays = W * H / (ms_step * 1e-3) / 1e6
 
-    print(f"[bench] plain: 1 warmup + {args.plain_steps} timed steps", file=sys.stderr, flush=True)
-    run("plain", 0, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evp = run("plain", args.plain_steps, 0)
-    torch.cuda.synchronize()
-    wall_p = time.perf_counter() - t0
-    ms_step_p = wall_p * 1e3 / args.plain_steps
-    mrays_p = W * H / (ms_step_p * 1e-3) / 1e6
-    p_gen = np.mean([e[0].elapsed_time(e[1]) for e in evp["gen"]])
-    p_prep = np.mean([e[1].elapsed_time(e[2]) for e in evp["gen"]])
-    p_rec = np.mean([e[2].elapsed_time(e[3]) for e in evp["gen"]])
+    plain = None
+    if args.plain_steps > 0:
+        print(f"[bench] plain: 1 warmup + {args.plain_steps} timed steps", file=sys.stderr, flush=True)
+        run("plain", 0, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evp = run("plain", args.plain_steps, 0)
+        torch.cuda.synchronize()
+        wall_p = time.perf_counter() - t0
+        ms_step_p = wall_p * 1e3 / args.plain_steps
+        plain = {"value": round(W * H / (ms_step_p * 1e-3) / 1e6, 3), "ms_per_step": round(ms_step_p, 3),
+                 "ms": {k: round(float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evp["gen"]])), 3)
+                        for i, k in enumerate(("generate", "prepare", "reconstruct"))}}
 
     f_gen, f_rec, f_prep = flop_generate(ng, m), flop_reconstruct(nr, m), flop_prepare(W * H, m)
     kernels = {"generate": (ms_gen, f_gen), "reconstruct": (ms_rec, f_rec), "prepare": (ms_prep, f_prep)}
@@ -172,15 +204,15 @@ def main():
         "config": {"workload": f"mixing layer {n}^3 f32 (resident in HBM), perspective {W}x{H} rays, {m} moments, "
                                f"step 0.5 voxel; generate + prepare + reconstruct, mixed form",
                    "volume": [n, n, n], "image": [W, H], "moments": m, "form": "mixed"},
-        "plain": {"value": round(mrays_p, 3), "ms_per_step": round(ms_step_p, 3),
-                  "ms": {"generate": round(p_gen, 3), "prepare": round(p_prep, 3), "reconstruct": round(p_rec, 3)}},
-        "mixed_over_plain": round(mrays / mrays_p, 3),
+        "plain": plain,
+        "mixed_over_plain": round(mrays / plain["value"], 3) if plain else None,
         "ms": {"generate": round(ms_gen, 3), "prepare": round(ms_prep, 3), "reconstruct": round(ms_rec, 3)},
         "samples_per_ray": {"generate": round(float(ng.mean()), 1), "reconstruct": round(float(nr.mean()), 1)},
         "image_alpha_mean": round(alpha_mean, 3),
         "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved / 1e12, 3),
                      "peak": PEAK_FP32_VALU / 1e12, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_VALU, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_FP32_VALU, 4),
+                     "traffic": measured_traffic(dom, n, W, m)},
         "roofline_frame": {"achieved_TFLOPs": round(total_flop / (ms_step * 1e-3) / 1e12, 3),
                            "frac": round(total_flop / (ms_step * 1e-3) / PEAK_FP32_VALU, 4)},
         "cpu_baseline": cpu,
