@@ -67,7 +67,8 @@ const void *mray_hip_volume_data(mray_volume vol);
  *   d_bounds   W*H uint32, written when p->transform_to_bounds (else may be NULL)
  *   d_samples  optional W*H int32: samples taken per ray (N_g), NULL to skip
  * form: MRAY_FORM_PLAIN or MRAY_FORM_MIXED. Compaction and prediction coding
- * are separate calls (mray_hip_compute_num_moments, ...). */
+ * are separate calls (mray_hip_compute_num_moments, mray_hip_moment_index,
+ * mray_hip_compact, mray_hip_prediction_encode). */
 int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int form, int W, int H,
                       int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, mray_stream stream);
 
@@ -98,6 +99,54 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
                          const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W, int H,
                          int m, const float *d_tf, const mray_rec_params *p, int form, float *d_rgba,
                          int32_t *d_steps, mray_stream stream);
+
+/* ---- compaction (src/moment_compaction.h, src/moment_image.cpp:69-94) ---- */
+
+/* Number of moments to keep per pixel (compute_num_moments,
+ * src/moment_compaction.h:147-174, and its CUDA kernel in
+ * src/moment_image_device.cu): binary search on the relative RMSE of the
+ * moments that the MESE predicts from the first n, against error_threshold
+ * (0 keeps all m). Pixels whose bounds are equal (or, without bounds, whose
+ * m_0 is 0) get 0. Bit-identical to the reference CPU path.
+ *   d_moments  dense W*H*m (bound-transformed moments as generated)
+ *   d_bounds   W*H or NULL;  d_counts  W*H uint32 out */
+int mray_hip_compute_num_moments(const float *d_moments, const uint32_t *d_bounds, int W, int H, int m,
+                                 float error_threshold, uint32_t *d_counts, mray_stream stream);
+
+/* Exclusive scan of the counts into index[W*H+1] (index[W*H] = total moments):
+ * the thrust::exclusive_scan of MomentImageDevice::compact
+ * (src/moment_image_device.cu:83-110). d_index may be d_counts (W*H+1
+ * entries). Uses stream-ordered scratch (hipMallocAsync). */
+int mray_hip_moment_index(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream);
+
+/* Copies the first count(pixel) moments of each pixel of a dense image into
+ * the compact layout (compaction_copy_kernel, src/moment_image_device.cu:65-81;
+ * MomentImageHost::compact, src/moment_image.cpp:69-94). d_compact holds
+ * index[W*H] floats. */
+int mray_hip_compact(const float *d_dense, const uint32_t *d_index, int W, int H, int m, float *d_compact,
+                     mray_stream stream);
+
+/* ---- prediction coding (src/moment_image.cpp:96-152) ------------------- */
+
+/* In-place prediction coding of a compact image (MomentImageHost::prediction_encode,
+ * src/moment_image.cpp:96-152; MomentImageDevice::prediction_encode,
+ * src/moment_image_device.cu:112-200): bias, exponential moments, reflection
+ * coefficients rotated to the real axis (src/moment_prediction_coding.h:81-159),
+ * then, for MRAY_CODING_WARP_DEFAULT_TRANSFORMED, the coding parameters
+ * (find_coding_transform_parameters, src/coding_transform.h:37-68, quantized
+ * as src/coding_params.h:56-66) into d_coding_params (2*(m-1) uint16: p1, p2
+ * per index) and the warp into [0, 1]. Bit-identical to the reference CPU path.
+ * Uses stream-ordered scratch (hipMallocAsync). */
+int mray_hip_prediction_encode(float *d_data, const uint32_t *d_index, int W, int H, int m, int coding_warp,
+                               float bias, uint16_t *d_coding_params, mray_stream stream);
+
+/* Lagrange multipliers of a prediction-coded image
+ * (prepare_moments_from_pred_coding, src/moment_preparation.h:32-53, as
+ * MomentImageHost::prepare_moments runs it when prediction_code is set,
+ * src/moment_image.cpp:154-183). FP32, bit-identical to the reference CPU
+ * path. d_index NULL = dense. */
+int mray_hip_prepare_coded(const float *d_moments, const uint32_t *d_index, int W, int H, int m, int coding_warp,
+                           const uint16_t *d_coding_params, float *d_lagrange, mray_stream stream);
 
 /* ---- synthetic volumes (device-side generators for the benchmark inputs) --- */
 

@@ -206,6 +206,90 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
     return check_status(rc, "mray_hip_reconstruct");
 }
 
+int mray_hip_compute_num_moments(const float *d_moments, const uint32_t *d_bounds, int W, int H, int m,
+                                 float error_threshold, uint32_t *d_counts, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_compute_num_moments: call mray_hip_init first");
+    if (!d_moments || !d_counts)
+        return fail(MRAY_ERR_ARG, "mray_hip_compute_num_moments: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_compute_num_moments: bad image size %dx%d or moment count %d", W, H, m);
+    int rc = mray::launch_num_moments(d_moments, d_bounds, W, H, m, error_threshold, d_counts, (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_compute_num_moments: %d moments do not fit the LDS", m);
+    return check_status(rc, "mray_hip_compute_num_moments");
+}
+
+int mray_hip_moment_index(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_moment_index: call mray_hip_init first");
+    if (!d_counts || !d_index)
+        return fail(MRAY_ERR_ARG, "mray_hip_moment_index: null argument");
+    if (bad_image(W, H, 1))
+        return fail(MRAY_ERR_ARG, "mray_hip_moment_index: bad image size %dx%d", W, H);
+    int rc = mray::launch_moment_index(d_counts, W * H, d_index, (hipStream_t)stream);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_moment_index: scratch allocation failed");
+    return check_status(rc, "mray_hip_moment_index");
+}
+
+int mray_hip_compact(const float *d_dense, const uint32_t *d_index, int W, int H, int m, float *d_compact,
+                     mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_compact: call mray_hip_init first");
+    if (!d_dense || !d_index || !d_compact)
+        return fail(MRAY_ERR_ARG, "mray_hip_compact: null argument");
+    if (bad_image(W, H, m) || (long long)W * H * m > 0xFFFFFFFFLL)
+        return fail(MRAY_ERR_ARG, "mray_hip_compact: bad image size %dx%d or moment count %d", W, H, m);
+    int rc = mray::launch_compact(d_dense, d_index, W, H, m, d_compact, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_compact");
+}
+
+int mray_hip_prediction_encode(float *d_data, const uint32_t *d_index, int W, int H, int m, int coding_warp,
+                               float bias, uint16_t *d_coding_params, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_prediction_encode: call mray_hip_init first");
+    if (!d_data || !d_index)
+        return fail(MRAY_ERR_ARG, "mray_hip_prediction_encode: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_prediction_encode: bad image size %dx%d or moment count %d", W, H, m);
+    if (coding_warp != MRAY_CODING_WARP_DEFAULT_TRANSFORMED && coding_warp != MRAY_CODING_WARP_NONE)
+        return fail(MRAY_ERR_ARG, "mray_hip_prediction_encode: unknown coding warp %d", coding_warp);
+    if (coding_warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED && m > 1 && !d_coding_params)
+        return fail(MRAY_ERR_ARG, "mray_hip_prediction_encode: the default warp needs d_coding_params");
+    int rc = mray::launch_prediction_encode(d_data, d_index, W, H, m, coding_warp, bias, d_coding_params,
+                                            (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_prediction_encode: %d moments do not fit the LDS", m);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_prediction_encode: scratch allocation failed");
+    return check_status(rc, "mray_hip_prediction_encode");
+}
+
+int mray_hip_prepare_coded(const float *d_moments, const uint32_t *d_index, int W, int H, int m, int coding_warp,
+                           const uint16_t *d_coding_params, float *d_lagrange, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_prepare_coded: call mray_hip_init first");
+    if (!d_moments || !d_lagrange)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_coded: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_coded: bad image size %dx%d or moment count %d", W, H, m);
+    if (coding_warp != MRAY_CODING_WARP_DEFAULT_TRANSFORMED && coding_warp != MRAY_CODING_WARP_NONE)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_coded: unknown coding warp %d", coding_warp);
+    if (coding_warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED && m > 1 && !d_coding_params)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_coded: the default warp needs d_coding_params");
+    int rc = mray::launch_prepare_coded(d_moments, d_index, W, H, m, coding_warp, d_coding_params, d_lagrange,
+                                        (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_prepare_coded: %d moments do not fit the LDS", m);
+    return check_status(rc, "mray_hip_prepare_coded");
+}
+
 int mray_hip_synth_mixing_layer(float *d_out, int n, const double *amp, const double *theta, const double *phi,
                                 uint64_t seed, mray_stream stream)
 {

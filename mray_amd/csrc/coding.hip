@@ -1,0 +1,599 @@
+// Moment-image compaction and prediction coding, reproducing the reference
+// CPU path bit for bit (FP32, the reference's operation order, the host C
+// library's sinf/cosf):
+//
+//   compute_num_moments     per-pixel moment count from the rRMSE of moments
+//                           predicted by the MESE extension
+//                           (src/moment_compaction.h:35-174)
+//   index scan, compaction  exclusive scan of the counts into index[W*H+1]
+//                           and the copy into the compact layout
+//                           (MomentImageHost::compact, src/moment_image.cpp:69-94;
+//                           src/moment_image_device.cu:65-110)
+//   prediction_encode       bias, Herglotz transform, encoding into reflection
+//                           coefficients, rotation to the real axis, coding
+//                           parameters and warp
+//                           (src/moment_image.cpp:96-152,
+//                           src/moment_prediction_coding.h:47-159,
+//                           src/coding_transform.h:37-68, src/coding_params.h:56-78)
+//   prepare_coded           decoding of a prediction-coded image into
+//                           Lagrange multipliers
+//                           (prepare_moments_from_pred_coding, src/moment_preparation.h:32-53)
+//
+// Per-pixel work is sequential (the reference's sums must be formed in its
+// order to round alike), one thread per pixel with its complex arrays in LDS
+// columns; the workgroup size adapts so that the columns fit the LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "launch.hpp"
+#include "mese_f32.hpp"
+
+namespace mray
+{
+
+// ---------------------------------------------------------------------------
+// compaction: number of moments per pixel
+// ---------------------------------------------------------------------------
+
+// evaluate_unknown_moments (src/moment_compaction.h:35-60): extend the first n
+// exponential moments to m with the MESE (Peters 2019, Eq. B.3).
+__device__ void evaluate_unknown_moments(int n_known, Col E, int m, Col newE, Col lev)
+{
+    const int mm = m - 1;
+    const int n = n_known - 1;
+    levinson(n_known, lev, E);
+    for (int i = 0; i < n + 1; ++i)
+        lev[i] = crmul(6.28318531f, lev[i]);
+    for (int l = 0; l < n + 1; ++l)
+        newE[l] = E[l];
+    const cf inv_lev0 = cdiv(cmk(-1.0f, 0.0f), lev[0]);
+    for (int k = 0; k < mm - n; ++k)
+    {
+        cf acc = cmk(0.0f, 0.0f);
+        for (int j = 1; j < n + 1; ++j)
+            acc = cadd(acc, cmul(newE[j + k], lev[n + 1 - j]));
+        newE[n + k + 1] = cmul(acc, inv_lev0);
+    }
+}
+
+// exponential_to_bounded_moments (src/moment_compaction.h:65-79), including
+// its use of the l-th exponential moment inside the j loop.
+__device__ void exponential_to_bounded_moments(int m, Col E, Col B, int n)
+{
+    const cf gamma_0 = cdivr(E[0], 2.f);
+    for (int l = n; l < m; ++l)
+    {
+        B[l] = cmk(0.0f, 0.0f);
+        for (int j = 0; j < l; ++j)
+        {
+            const cf c = cdiv(cmk((float)(l - j), 0.0f), crmul((float)l, gamma_0));
+            B[l] = cadd(B[l], cmul(cmul(c, E[l]), B[l - j]));
+        }
+        const cf den = cmul(cmul(cmk(0.0f, 2.0f), cmk((float)kPi, 0.0f)), gamma_0);
+        B[l] = csub(cdiv(E[l], den), B[l]);
+    }
+}
+
+// evaluate_rrmse (src/moment_compaction.h:84-111)
+__device__ float evaluate_rrmse(int n, int m, const float *moments, Col tE, Col tExt, Col tLev, float norm)
+{
+    evaluate_unknown_moments(n, tE, m, tExt, tLev);
+    const Col ext = tLev;
+    for (int l = 0; l < n; ++l)
+        ext[l] = cmk(moments[l], 0.0f);
+    exponential_to_bounded_moments(m, tExt, ext, n);
+    float sum = 0.0f;
+    for (int k = n; k < m; ++k)
+    {
+        const float diff = ext[k].x - moments[k];
+        sum += diff * diff;
+    }
+    return sqrtf(sum) * norm;
+}
+
+// determine_best_number_moments (src/moment_compaction.h:113-141): binary
+// search on the rRMSE starting at m/2 with step m/4.
+__device__ int determine_best_number_moments(int m, const float *moments, float error_threshold, Col tE, Col tExt,
+                                             Col tLev)
+{
+    if (error_threshold == 0.0f)
+        return m;
+    const float norm = 1.0f / moments[0];
+    trig_to_exp(m, tE, [&](int i) { return moments[i]; });
+    int n = m / 2;
+    int s = m / 4;
+    while (s > 0)
+    {
+        const float rrmse = evaluate_rrmse(n, m, moments, tE, tExt, tLev, norm);
+        const float d = rrmse - error_threshold;
+        if (((d < 0) ? -d : d) < 1e-8f)
+            return n;
+        else if (rrmse > error_threshold)
+            n += s;
+        else
+            n -= s;
+        s = s / 2;
+    }
+    return n;
+}
+
+// compute_num_moments (src/moment_compaction.h:147-174), one pixel per thread
+__global__ __launch_bounds__(64) void num_moments_kernel(const float *__restrict__ dense,
+                                                         const uint32_t *__restrict__ bounds, int npix, int m,
+                                                         float error_threshold, uint32_t *__restrict__ counts)
+{
+    extern __shared__ cf lds_cf[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int pix = blockIdx.x * nt + tid;
+    if (pix >= npix)
+        return;
+    const float *mo = dense + (size_t)pix * m;
+    bool is_zero;
+    if (bounds)
+        is_zero = (bounds[pix] & 0xFFFFu) == (bounds[pix] >> 16);
+    else
+        is_zero = mo[0] == 0.0f;
+    int n = 0;
+    if (!is_zero)
+    {
+        const Col tE{lds_cf + tid, nt}, tExt{lds_cf + nt * m + tid, nt}, tLev{lds_cf + 2 * nt * m + tid, nt};
+        n = determine_best_number_moments(m, mo, error_threshold, tE, tExt, tLev);
+    }
+    counts[pix] = (uint32_t)n;
+}
+
+// ---------------------------------------------------------------------------
+// compaction: exclusive scan of the counts, copy into the compact layout
+// ---------------------------------------------------------------------------
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        red[w] = v;
+    __syncthreads();
+    uint32_t s = 0;
+    for (int i = 0; i < kScanThreads / 64; ++i)
+        s += red[i];
+    return s;
+}
+
+// pass 1: the sum of each tile of counts
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sums(const uint32_t *__restrict__ counts, int n,
+                                                               uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t red[kScanThreads / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (int i = 0; i < kScanItems; ++i)
+    {
+        const size_t e = base + (size_t)i * kScanThreads + threadIdx.x;
+        if (e < (size_t)n)
+            s += counts[e];
+    }
+    s = block_sum_u32(s, red);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = s;
+}
+
+// inclusive scan of the 256 per-thread totals in LDS (Hillis-Steele)
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *buf)
+{
+    const int t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1)
+    {
+        const uint32_t add = (t >= o) ? buf[t - o] : 0u;
+        __syncthreads();
+        buf[t] += add;
+        __syncthreads();
+    }
+    return buf[t] - v;
+}
+
+// pass 2: exclusive scan of the tile sums (one workgroup, chunks of 256 with
+// a running carry); writes the grand total to index[n]
+__global__ __launch_bounds__(kScanThreads) void scan_tile_offsets(uint32_t *__restrict__ sums, int ntiles,
+                                                                  uint32_t *__restrict__ index, int n)
+{
+    __shared__ uint32_t buf[kScanThreads];
+    uint32_t carry = 0;
+    for (int c = 0; c < ntiles; c += kScanThreads)
+    {
+        const int i = c + threadIdx.x;
+        const uint32_t v = (i < ntiles) ? sums[i] : 0u;
+        const uint32_t ex = block_exclusive_scan(v, buf);
+        const uint32_t chunk_total = buf[kScanThreads - 1];
+        if (i < ntiles)
+            sums[i] = carry + ex;
+        carry += chunk_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        index[n] = carry;
+}
+
+// pass 3: exclusive scan inside each tile plus the tile's offset. The tile is
+// staged in LDS, so counts and index may be the same array.
+__global__ __launch_bounds__(kScanThreads) void scan_tile_apply(const uint32_t *counts, int n,
+                                                                const uint32_t *__restrict__ offsets, uint32_t *index)
+{
+    __shared__ uint32_t tile[kScanTile];
+    __shared__ uint32_t buf[kScanThreads];
+    const int t = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    for (int i = 0; i < kScanItems; ++i)
+    {
+        const size_t e = base + (size_t)i * kScanThreads + t;
+        tile[i * kScanThreads + t] = (e < (size_t)n) ? counts[e] : 0u;
+    }
+    __syncthreads();
+    uint32_t s = 0;
+    for (int i = 0; i < kScanItems; ++i)
+        s += tile[t * kScanItems + i];
+    uint32_t run = offsets[blockIdx.x] + block_exclusive_scan(s, buf);
+    for (int i = 0; i < kScanItems; ++i)
+    {
+        const uint32_t v = tile[t * kScanItems + i];
+        tile[t * kScanItems + i] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int i = 0; i < kScanItems; ++i)
+    {
+        const size_t e = base + (size_t)i * kScanThreads + t;
+        if (e < (size_t)n)
+            index[e] = tile[i * kScanThreads + t];
+    }
+}
+
+// compaction_copy_kernel (src/moment_image_device.cu:65-81): one thread per
+// dense element, the first count(pixel) moments of each pixel are kept
+__global__ __launch_bounds__(256) void compact_copy_kernel(const float *__restrict__ dense,
+                                                           const uint32_t *__restrict__ index, size_t npix, int m,
+                                                           float *__restrict__ out)
+{
+    const size_t total = npix * (size_t)m;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    {
+        const size_t pix = e / (size_t)m;
+        const uint32_t l = (uint32_t)(e - pix * (size_t)m);
+        const uint32_t start = index[pix];
+        if (l < index[pix + 1] - start)
+            out[start + l] = dense[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// prediction coding
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float fmaxr(float x, float y) { return (x > y) ? x : y; }
+__device__ __forceinline__ float fminr(float x, float y) { return (x < y) ? x : y; }
+__device__ __forceinline__ float fclampr(float v, float lo, float hi) { return fminr(fmaxr(v, lo), hi); }
+
+// prediction_coding_warp / prediction_coding_inv_warp (src/moment_prediction_coding.h:47-67);
+// params[2(l-1)] = p1, params[2(l-1)+1] = p2 (dequantized coding parameters)
+__device__ __forceinline__ float coding_warp(int l, float x, int warp, const float *params)
+{
+    float w;
+    if (warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED)
+        w = (x + params[(l - 1) * 2]) / (params[(l - 1) * 2 + 1] * 2.f);
+    else
+        w = x * 0.5f + 0.5f;
+    return fclampr(w, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float coding_inv_warp(int l, float x, int warp, const uint16_t *q)
+{
+    if (warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED)
+    {
+        const float p1 = q[(l - 1) * 2] / 65535.f, p2 = q[(l - 1) * 2 + 1] / 65535.f;
+        return (x * p2 * 2.f) - p1;
+    }
+    return x * 2.f - 1.f;
+}
+
+// MomentImageHost::prediction_encode, first loop (src/moment_image.cpp:100-123):
+// bias, exponential moments, encode (src/moment_prediction_coding.h:123-159)
+// and transform_quantization_real (:81-93), in place. The rotation to the
+// real axis depends only on code[0] = E[0], so each coefficient is written as
+// soon as it is known.
+__global__ __launch_bounds__(64) void prediction_encode_kernel(float *__restrict__ data,
+                                                               const uint32_t *__restrict__ index, int npix, int m,
+                                                               float bias)
+{
+    extern __shared__ cf lds_cf[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int pix = blockIdx.x * nt + tid;
+    if (pix >= npix)
+        return;
+    const int nm = min((int)(index[pix + 1] - index[pix]), m);
+    if (nm <= 0)
+        return;
+    float *d = data + index[pix];
+    d[0] = lerp_bias(d[0], 0.5f, bias);
+    for (int i = 1; i < nm; ++i)
+        d[i] = lerp_bias(d[i], 0.0f, bias);
+    const Col E{lds_cf + tid, nt}, P{lds_cf + nt * m + tid, nt};
+    trig_to_exp(nm, E, [&](int i) { return d[i]; });
+
+    const cf code0 = E[0];
+    const cf axis = cmk(-code0.y, code0.x);
+    const float length_axis = sqrtf(cabs2(axis));
+    const cf inv_axis = cdiv(cmk(length_axis, 0.0f), axis);
+    P[0] = cmk(1.0f / E[0].x, 0.0f);
+    bool invalid = false;
+    for (int l = 1; l < nm; ++l)
+    {
+        cf u = cmk(0.0f, 0.0f);
+        for (int k = 0; k < l; ++k)
+            u = cadd(u, cmul(P[k], E[l - k]));
+        d[l] = cmul(u, inv_axis).x;
+        float a2 = cabs2(u);
+        if (invalid || a2 >= 1.0f)
+        {
+            u = cmk(0.0f, 0.0f);
+            a2 = 0;
+            invalid = true;
+        }
+        levinson_update(l, P, u, 1.0f / (1.0f - a2));
+    }
+}
+
+// order-preserving integer keys of floats (for atomic min / max)
+__device__ __forceinline__ int float_key(float f)
+{
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float key_float(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
+
+// find_coding_transform_parameters (src/coding_transform.h:37-68), min / max
+// part: for every l in [1, m) the min and max (both starting at 0) of
+// clamp(data[index[p] + l], -1, 1) over pixels with count >= l. A pixel whose
+// count equals l contributes the first value of the next pixel, as in the
+// reference; the value past the end of the data is skipped. keys holds
+// (min, max) per l as float_key, initialised to float_key(0).
+__global__ __launch_bounds__(256) void coding_minmax_kernel(const float *__restrict__ data,
+                                                            const uint32_t *__restrict__ index, int npix, int m,
+                                                            int *__restrict__ keys)
+{
+    extern __shared__ int lds_keys[]; // [2 (m - 1)]
+    for (int i = threadIdx.x; i < 2 * (m - 1); i += blockDim.x)
+        lds_keys[i] = 0;
+    __syncthreads();
+    const uint32_t total = index[npix];
+    const int lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < npix; base += gridDim.x * blockDim.x)
+    {
+        const int pix = base + lane;
+        const bool valid = pix < npix;
+        const uint32_t start = valid ? index[pix] : 0u;
+        const int num = valid ? (int)(index[pix + 1] - start) : 0;
+        for (int l = 1; l < m; ++l)
+        {
+            // lanes without a value contribute the neutral 0 (the start value)
+            float v = 0.0f;
+            if (num >= l && start + (uint32_t)l < total)
+            {
+                const float x = data[start + l];
+                if (x == x)
+                    v = fclampr(x, -1.0f, 1.0f) + 0.0f;
+            }
+            float lo = v, hi = v;
+            for (int o = 32; o > 0; o >>= 1)
+            {
+                lo = fminf(lo, __shfl_xor(lo, o));
+                hi = fmaxf(hi, __shfl_xor(hi, o));
+            }
+            if (lane == 0)
+            {
+                atomicMin(&lds_keys[2 * (l - 1)], float_key(lo));
+                atomicMax(&lds_keys[2 * (l - 1) + 1], float_key(hi));
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * (m - 1); i += blockDim.x)
+    {
+        if (i & 1)
+            atomicMax(&keys[i], lds_keys[i]);
+        else
+            atomicMin(&keys[i], lds_keys[i]);
+    }
+}
+
+__global__ void fill_i32_kernel(int *__restrict__ p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// the rest of find_coding_transform_parameters (p1 = |min|, p2 = |max - min| / 2)
+// and CodingParamType::quantize / dequantize (src/coding_params.h:56-78)
+__global__ void coding_params_kernel(const int *__restrict__ keys, int m, uint16_t *__restrict__ q,
+                                     float *__restrict__ params)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m - 1)
+        return;
+    const float mn = key_float(keys[2 * p]), mx = key_float(keys[2 * p + 1]);
+    const float d = mx - mn;
+    const float raw[2] = {(mn < 0) ? -mn : mn, ((d < 0) ? -d : d) * 0.5f};
+    for (int i = 0; i < 2; ++i)
+    {
+        const uint16_t v = (uint16_t)ceilf(fclampr(raw[i], 0.0f, 1.0f) * 65535.f);
+        q[2 * p + i] = v;
+        params[2 * p + i] = v / 65535.f;
+    }
+}
+
+// MomentImageHost::prediction_encode, warping loop (src/moment_image.cpp:137-151)
+__global__ __launch_bounds__(256) void coding_warp_kernel(float *__restrict__ data, const uint32_t *__restrict__ index,
+                                                          int npix, int m, int warp, const float *__restrict__ params)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix)
+        return;
+    const int nm = min((int)(index[pix + 1] - index[pix]), m);
+    float *d = data + index[pix];
+    for (int l = 1; l < nm; ++l)
+        d[l] = coding_warp(l, d[l], warp, params);
+}
+
+// prepare_moments_from_pred_coding (src/moment_preparation.h:32-53):
+// transform_dequantization_real (src/moment_prediction_coding.h:98-118, the
+// coefficients recomputed where decode reads them), decode (:164-206), then
+// the correlations of prepareMESELagrange.
+__global__ __launch_bounds__(64) void prepare_coded_kernel(const float *__restrict__ mom,
+                                                           const uint32_t *__restrict__ index, int npix, int m,
+                                                           int warp, const uint16_t *__restrict__ qparams,
+                                                           float *__restrict__ out)
+{
+    extern __shared__ cf lds_cf[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int pix = blockIdx.x * nt + tid;
+    if (pix >= npix)
+        return;
+    const int nm = index ? min((int)(index[pix + 1] - index[pix]), m) : m;
+    const size_t off = index ? (size_t)index[pix] : (size_t)pix * m;
+    if (nm <= 0)
+        return;
+    const float *in = mom + off;
+    const Col E{lds_cf + tid, nt}, P{lds_cf + nt * m + tid, nt}, R{lds_cf + 2 * nt * m + tid, nt};
+
+    const float ph = 3.14159265f * in[0] - 1.57079633f;
+    const cf code0 = crmul(2.0f * 0.0795774715f, cmk(host_cosf(ph), host_sinf(ph)));
+    const cf axis = cmk(-code0.y, code0.x);
+    const cf factor_axis = cdivr(axis, sqrtf(cabs2(axis)));
+
+    E[0] = code0;
+    P[0] = cmk(1.0f / E[0].x, 0.0f);
+    const float eps = 1e-6f;
+    bool invalid = false;
+    for (int l = 1; l < nm; ++l)
+    {
+        cf u = crmul(coding_inv_warp(l, in[l], warp, qparams), factor_axis);
+        float a2 = cabs2(u);
+        if (invalid || a2 >= (1.0f - eps))
+        {
+            u = cmk(0.0f, 0.0f);
+            a2 = 0;
+            invalid = true;
+        }
+        cf g = cmk(0.0f, 0.0f);
+        for (int k = 1; k < l; ++k)
+            g = cadd(g, cmul(P[k], E[l - k]));
+        E[l] = cmul(cdiv(cmk(1.0f, 0.0f), P[0]), csub(u, g));
+        levinson_update(l, P, u, 1.0f / (1.0f - a2));
+    }
+    lagrange_from_levinson(nm, out + off, E, P, R);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+extern size_t g_lds_limit;
+
+// the largest of 64, 32, 16 threads per workgroup whose `cols` LDS columns of
+// m complex numbers fit; 0 if none does
+static int column_block(int cols, int m)
+{
+    for (int nt = 64; nt >= 16; nt /= 2)
+        if ((size_t)cols * m * nt * sizeof(cf) <= g_lds_limit)
+            return nt;
+    return 0;
+}
+
+int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H, int m, float error_threshold,
+                       uint32_t *counts, hipStream_t stream)
+{
+    const int npix = W * H;
+    const int nt = column_block(3, m);
+    if (!nt)
+        return MRAY_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(num_moments_kernel, dim3((npix + nt - 1) / nt), dim3(nt), (size_t)3 * m * nt * sizeof(cf),
+                       stream, dense, bounds, npix, m, error_threshold, counts);
+    return 0;
+}
+
+int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream)
+{
+    const int ntiles = (npix + kScanTile - 1) / kScanTile;
+    uint32_t *sums = nullptr;
+    if (hipMallocAsync((void **)&sums, (size_t)ntiles * sizeof(uint32_t), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    hipLaunchKernelGGL(scan_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, stream, counts, npix, sums);
+    hipLaunchKernelGGL(scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, stream, sums, ntiles, index, npix);
+    hipLaunchKernelGGL(scan_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, stream, counts, npix, sums, index);
+    return hipFreeAsync(sums, stream) == hipSuccess ? 0 : MRAY_ERR_HIP;
+}
+
+int launch_compact(const float *dense, const uint32_t *index, int W, int H, int m, float *out, hipStream_t stream)
+{
+    const size_t total = (size_t)W * H * m;
+    const size_t blocks = std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(compact_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, dense, index,
+                       (size_t)W * H, m, out);
+    return 0;
+}
+
+int launch_prediction_encode(float *data, const uint32_t *index, int W, int H, int m, int warp, float bias,
+                             uint16_t *qparams, hipStream_t stream)
+{
+    const int npix = W * H;
+    const int nt = column_block(2, m);
+    if (!nt)
+        return MRAY_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(prediction_encode_kernel, dim3((npix + nt - 1) / nt), dim3(nt), (size_t)2 * m * nt * sizeof(cf),
+                       stream, data, index, npix, m, bias);
+    float *params = nullptr;
+    int *keys = nullptr;
+    const size_t np = m > 1 ? (size_t)2 * (m - 1) : 1;
+    if (hipMallocAsync((void **)&params, np * sizeof(float), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    if (warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED && m > 1)
+    {
+        if (hipMallocAsync((void **)&keys, np * sizeof(int), stream) != hipSuccess)
+        {
+            (void)hipFreeAsync(params, stream);
+            return MRAY_ERR_OOM;
+        }
+        // float_key(0.0f) == 0: both min and max start at 0 as in the reference
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream, keys, (int)np, 0);
+        const int blocks = std::min((npix + 255) / 256, 1024);
+        hipLaunchKernelGGL(coding_minmax_kernel, dim3(blocks), dim3(256), np * sizeof(int), stream, data, index, npix,
+                           m, keys);
+        hipLaunchKernelGGL(coding_params_kernel, dim3((m - 1 + 63) / 64), dim3(64), 0, stream, keys, m, qparams,
+                           params);
+        if (hipFreeAsync(keys, stream) != hipSuccess)
+            return MRAY_ERR_HIP;
+    }
+    hipLaunchKernelGGL(coding_warp_kernel, dim3((npix + 255) / 256), dim3(256), 0, stream, data, index, npix, m, warp,
+                       params);
+    return hipFreeAsync(params, stream) == hipSuccess ? 0 : MRAY_ERR_HIP;
+}
+
+int launch_prepare_coded(const float *moments, const uint32_t *index, int W, int H, int m, int warp,
+                         const uint16_t *qparams, float *lagrange, hipStream_t stream)
+{
+    const int npix = W * H;
+    const int nt = column_block(3, m);
+    if (!nt)
+        return MRAY_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(prepare_coded_kernel, dim3((npix + nt - 1) / nt), dim3(nt), (size_t)3 * m * nt * sizeof(cf),
+                       stream, moments, index, npix, m, warp, qparams, lagrange);
+    return 0;
+}
+
+} // namespace mray

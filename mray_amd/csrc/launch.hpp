@@ -21,6 +21,15 @@ int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *d
                        const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
                        const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream);
 
+int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H, int m, float error_threshold,
+                       uint32_t *counts, hipStream_t stream);
+int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);
+int launch_compact(const float *dense, const uint32_t *index, int W, int H, int m, float *out, hipStream_t stream);
+int launch_prediction_encode(float *data, const uint32_t *index, int W, int H, int m, int warp, float bias,
+                             uint16_t *qparams, hipStream_t stream);
+int launch_prepare_coded(const float *moments, const uint32_t *index, int W, int H, int m, int warp,
+                         const uint16_t *qparams, float *lagrange, hipStream_t stream);
+
 int launch_synth_mixing_layer(float *out, int n, const double *amp, const double *theta, const double *phi,
                               uint64_t seed, hipStream_t stream);
 int launch_synth_marschner_lobb(float *out, int n, hipStream_t stream);

@@ -18,32 +18,17 @@
 //         across the 8 lanes with DPP butterflies (quad_perm xor 1, xor 2,
 //         row_half_mirror), never leaving the 8-lane row group. Per-pixel
 //         arrays live in LDS (320 bytes per padded moment per wavefront).
-#include "device_common.hpp"
 #include "launch.hpp"
+#include "mese_f32.hpp"
 
 namespace mray
 {
-
-// ---------------------------------------------------------------------------
-// shared helpers
-// ---------------------------------------------------------------------------
-struct cf
-{
-    float x, y;
-};
-__device__ __forceinline__ cf cmk(float x, float y) { return cf{x, y}; }
-__device__ __forceinline__ cf cadd(cf a, cf b) { return cf{a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ cf csub(cf a, cf b) { return cf{a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ cf cmul(cf a, cf b) { return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-__device__ __forceinline__ cf crmul(float s, cf a) { return cf{s * a.x, s * a.y}; }
-__device__ __forceinline__ cf cconj(cf a) { return cf{a.x, -a.y}; }
 
 struct dc
 {
     double x, y;
 };
 
-__device__ __forceinline__ float lerp_bias(float v, float target, float b) { return (1 - b) * v + b * target; }
 
 // ---------------------------------------------------------------------------
 // PLAIN (FP32, one thread per pixel)
@@ -61,72 +46,14 @@ __global__ __launch_bounds__(64) void prepare_plain_kernel(const float *__restri
     const size_t off = index ? (size_t)index[pix] : (size_t)pix * m;
     if (nm <= 0)
         return;
-    cf *E = lds_cf + tid;
-    cf *P = lds_cf + 64 * m + tid;
-    cf *R = lds_cf + 2 * 64 * m + tid;
-#define AT(arr, i) arr[(i)*64]
+    const Col E{lds_cf + tid, 64}, P{lds_cf + 64 * m + tid, 64}, R{lds_cf + 2 * 64 * m + tid, 64};
     const float *t = mom + off;
-    auto tb = [&](int i) { return i == 0 ? lerp_bias(t[0], 0.5f, bias) : lerp_bias(t[i], 0.0f, bias); };
-
-    // trigonometricToExponentialMoments (src/mese/MESE_dynamic.h:35-54)
-    float ph = 3.14159265f * tb(0) - 1.57079633f;
-    AT(E, 0) = crmul(0.0795774715f, cmk(host_cosf(ph), host_sinf(ph)));
-    for (int l = 1; l < nm; ++l)
-    {
-        cf acc = cmk(0.0f, 0.0f);
-        for (int j = 0; j < l; ++j)
-            acc = cadd(acc, cmul(crmul((float)(l - j), AT(E, j)), cmk(tb(l - j), 0.0f)));
-        AT(E, l) = cmul(acc, cmk(0.0f, (float)((double)(2.0f * kPi) / (double)(float)l)));
-    }
-    AT(E, 0) = crmul(2.0f, AT(E, 0));
-
-    // levinsonsAlgorithm (src/mese/MESE_dynamic.h:100-135), R as temp
-    AT(P, 0) = cmk(1.0f / AT(E, 0).x, 0.0f);
-    bool invalid = false;
-    for (int j = 1; j < nm; ++j)
-    {
-        cf u = cmk(0.0f, 0.0f);
-        if (!invalid)
-            for (int k = 0; k < j; ++k)
-                u = cadd(u, cmul(AT(P, k), AT(E, j - k)));
-        float a2 = u.x * u.x + u.y * u.y;
-        if (a2 >= 1.0f)
-        {
-            u = cmk(0.0f, 0.0f);
-            a2 = 0;
-            invalid = true;
-        }
-        float factor = 1.0f / (1.0f - a2);
-        for (int k = 1; k < j; ++k)
-            AT(R, k) = cconj(AT(P, j - k));
-        AT(R, j) = cmk(AT(P, 0).x, 0.0f);
-        AT(P, 0) = cmk(factor * AT(P, 0).x, 0.0f);
-        for (int k = 1; k < j; ++k)
-            AT(P, k) = crmul(factor, csub(AT(P, k), cmul(AT(R, k), u)));
-        AT(P, j) = crmul(factor, crmul(-AT(R, j).x, u));
-    }
-    for (int i = 0; i < nm; ++i)
-        AT(P, i) = crmul(6.28318531f, AT(P, i));
-    // computeAutocorrelation (src/mese/MESE_dynamic.h:137-147)
-    for (int k = 0; k < nm; ++k)
-    {
-        cf acc = cmk(0.0f, 0.0f);
-        for (int j = 0; j < nm - k; ++j)
-            acc = cadd(acc, cmul(AT(P, j), cconj(AT(P, j + k))));
-        AT(R, k) = acc;
-    }
-    AT(E, 0) = crmul(0.5f, AT(E, 0));
-    // computeImaginaryCorrelation (src/mese/MESE_dynamic.h:161-171)
-    const float nf = 1.0f / (3.14159265f * AT(P, 0).x);
-    float *o = out + off;
-    for (int k = 0; k < nm; ++k)
-    {
-        float acc = 0.0f;
-        for (int j = 0; j < nm - k; ++j)
-            acc += AT(R, k + j).x * AT(E, j).y + AT(R, k + j).y * AT(E, j).x;
-        o[k] = nf * acc;
-    }
-#undef AT
+    // prepare_moments_from_std_coding (src/moment_preparation.h:55-64):
+    // bias lerp toward the uniform density, then prepareMESELagrange
+    // (src/mese/MESE_dynamic.h:194-211)
+    trig_to_exp(nm, E, [&](int i) { return i == 0 ? lerp_bias(t[0], 0.5f, bias) : lerp_bias(t[i], 0.0f, bias); });
+    levinson(nm, P, E);
+    lagrange_from_levinson(nm, out + off, E, P, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -353,7 +280,7 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     }
 }
 
-static size_t g_lds_limit = 65536;
+size_t g_lds_limit = 65536;
 
 void set_lds_limit(size_t bytes) { g_lds_limit = bytes; }
 

@@ -30,6 +30,11 @@ _SIGNATURES = {
     "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_reconstruct": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
+    "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_prediction_encode": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
+    "mray_hip_prepare_coded": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
     "mray_hip_synth_mixing_layer": ([_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.c_uint64, _vp], C.c_int),
     "mray_hip_synth_marschner_lobb": ([_vp, C.c_int, _vp], C.c_int),
@@ -81,6 +86,11 @@ def _ptr(t):
     if not t.is_contiguous():
         raise MrayError("expected a contiguous tensor")
     return _vp(t.data_ptr())
+
+
+def _like(t: torch.Tensor) -> torch.Tensor:
+    """empty_like that never hands out a null pointer (empty compact images)."""
+    return torch.empty(max(t.numel(), 1), dtype=t.dtype, device=t.device)[: t.numel()].view(t.shape)
 
 
 def _stream(stream):
@@ -143,7 +153,7 @@ def generate(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, form
 def prepare(moments: torch.Tensor, W: int, H: int, m: int, bias: float, form="mixed", index=None, out=None,
             stream=None):
     if out is None:
-        out = torch.empty_like(moments)
+        out = _like(moments)
     _check(load_library().mray_hip_prepare(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias), form_id(form),
                                            _ptr(out), _stream(stream)), "mray_hip_prepare")
     return out
@@ -166,6 +176,69 @@ def reconstruct(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m
                                                _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp), form_id(form),
                                                _ptr(rgba), _ptr(steps), _stream(stream)), "mray_hip_reconstruct")
     return rgba, steps
+
+
+def compute_num_moments(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None,
+                        out=None, stream=None) -> torch.Tensor:
+    """Moments to keep per pixel, (H*W,) int32 holding uint32 counts
+    (compute_num_moments, src/moment_compaction.h:147-174)."""
+    if out is None:
+        out = torch.empty(H * W, dtype=torch.int32, device=moments.device)
+    _check(load_library().mray_hip_compute_num_moments(_ptr(moments), _ptr(bounds), W, H, m,
+                                                       C.c_float(error_threshold), _ptr(out), _stream(stream)),
+           "mray_hip_compute_num_moments")
+    return out
+
+
+def moment_index(counts: torch.Tensor, W: int, H: int, out=None, stream=None) -> torch.Tensor:
+    """Exclusive scan of the counts: (H*W+1,) int32 holding uint32 offsets, last = total."""
+    if out is None:
+        out = torch.empty(H * W + 1, dtype=torch.int32, device=counts.device)
+    _check(load_library().mray_hip_moment_index(_ptr(counts), W, H, _ptr(out), _stream(stream)),
+           "mray_hip_moment_index")
+    return out
+
+
+def compact(moments: torch.Tensor, index: torch.Tensor, W: int, H: int, m: int, stream=None) -> torch.Tensor:
+    """Compact copy of a dense moment image (reads the total from index[-1])."""
+    s = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(s):
+        total = int(index[-1].to(torch.int64).item()) & 0xFFFFFFFF
+    out = torch.empty(max(total, 1), dtype=torch.float32, device=moments.device)
+    _check(load_library().mray_hip_compact(_ptr(moments), _ptr(index), W, H, m, _ptr(out), _stream(stream)),
+           "mray_hip_compact")
+    return out[:total]
+
+
+def compact_image(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None, stream=None):
+    """The compaction of generate with compact_image set: (compact data, index)."""
+    counts = compute_num_moments(moments, W, H, m, error_threshold, bounds=bounds, stream=stream)
+    index = moment_index(counts, W, H, stream=stream)
+    return compact(moments, index, W, H, m, stream=stream), index
+
+
+def prediction_encode(data: torch.Tensor, index: torch.Tensor, W: int, H: int, m: int, coding_warp: int = 0,
+                      bias: float = 1e-4, stream=None):
+    """In-place prediction coding of a compact image; returns the quantized
+    coding parameters, (2*(m-1),) int16 holding uint16 values (None for the
+    warp without parameters)."""
+    params = None
+    if coding_warp == 0 and m > 1:
+        params = torch.zeros(2 * (m - 1), dtype=torch.int16, device=data.device)
+    _check(load_library().mray_hip_prediction_encode(_ptr(data), _ptr(index), W, H, m, coding_warp, C.c_float(bias),
+                                                     _ptr(params), _stream(stream)), "mray_hip_prediction_encode")
+    return params
+
+
+def prepare_coded(moments: torch.Tensor, W: int, H: int, m: int, coding_warp: int = 0, coding_params=None,
+                  index=None, out=None, stream=None) -> torch.Tensor:
+    """Lagrange multipliers of a prediction-coded image (same layout as the input)."""
+    if out is None:
+        out = _like(moments)
+    _check(load_library().mray_hip_prepare_coded(_ptr(moments), _ptr(index), W, H, m, coding_warp,
+                                                 _ptr(coding_params), _ptr(out), _stream(stream)),
+           "mray_hip_prepare_coded")
+    return out
 
 
 def synth_mixing_layer(n: int, seed: int = 1, device="cuda", stream=None) -> torch.Tensor:

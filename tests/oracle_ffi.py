@@ -172,6 +172,14 @@ class Oracle:
                                              ptr(q, _u16), nthreads())
         return d, q
 
+    def prepare_pred(self, nm, moments, warp, params_f):
+        """prepare_moments_from_pred_coding on one pixel's nm coded moments."""
+        out = np.zeros(max(nm, 1), np.float32)
+        pf = np.ascontiguousarray(params_f if params_f is not None else np.zeros(max(2 * (nm - 1), 1)), np.float32)
+        self.lib.orc_prepare_moments_pred(nm, ptr(np.ascontiguousarray(moments, np.float32), _f), warp, ptr(pf, _f),
+                                          ptr(out, _f))
+        return out[:nm]
+
     def particles_to_grid(self, x, y, z, v, h, dmin, dmax, nx, ny, nz):
         g = np.zeros((nz, ny, nx), np.float32)
         bmin = (C.c_float * 3)(*dmin)
