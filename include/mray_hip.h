@@ -148,6 +148,22 @@ int mray_hip_prediction_encode(float *d_data, const uint32_t *d_index, int W, in
 int mray_hip_prepare_coded(const float *d_moments, const uint32_t *d_index, int W, int H, int m, int coding_warp,
                            const uint16_t *d_coding_params, float *d_lagrange, mray_stream stream);
 
+/* ---- SPH particles (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh}) --- */
+
+/* Samples particles (device arrays of n positions and values) at every voxel
+ * position of an nx*ny*nz grid over [dom_min, dom_max] (voxel i at
+ * dom_min + i/n (dom_max - dom_min)), as Particles::sample does through its
+ * uniform grid of cell size h (Particles::create_grid, src/volume.cpp:55-81;
+ * Particles::sample, :83-94; InterpolateParticlesOp,
+ * src/particle_interpolate.h:35-58): the cubic-spline weighted mean of the
+ * values of particles closer than h. Particles are ordered by cell with a
+ * stable sort, so the result is deterministic. Synchronises the stream once
+ * (the cell grid is sized from the particles' bounding box); uses
+ * stream-ordered scratch of about 28 bytes per particle + 8 per cell. */
+int mray_hip_particles_to_grid(const float *d_x, const float *d_y, const float *d_z, const float *d_v, uint32_t n,
+                               float h, const float dom_min[3], const float dom_max[3], int nx, int ny, int nz,
+                               float *d_grid, mray_stream stream);
+
 /* ---- synthetic volumes (device-side generators for the benchmark inputs) --- */
 
 /* Mixing layer on [0,1]^3: 0.5 + 0.5 tanh((z - 0.5 - h(x,y)) / 0.01) +

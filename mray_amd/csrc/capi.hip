@@ -290,6 +290,25 @@ int mray_hip_prepare_coded(const float *d_moments, const uint32_t *d_index, int 
     return check_status(rc, "mray_hip_prepare_coded");
 }
 
+int mray_hip_particles_to_grid(const float *d_x, const float *d_y, const float *d_z, const float *d_v, uint32_t n,
+                               float h, const float dom_min[3], const float dom_max[3], int nx, int ny, int nz,
+                               float *d_grid, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_particles_to_grid: call mray_hip_init first");
+    if (!d_x || !d_y || !d_z || !d_v || !dom_min || !dom_max || !d_grid)
+        return fail(MRAY_ERR_ARG, "mray_hip_particles_to_grid: null argument");
+    if (n == 0 || n > 0x7fffffffu || !(h > 0.0f) || nx <= 0 || ny <= 0 || nz <= 0)
+        return fail(MRAY_ERR_ARG, "mray_hip_particles_to_grid: bad particle count, smoothing length or grid size");
+    int rc = mray::launch_particles_to_grid(d_x, d_y, d_z, d_v, n, h, dom_min, dom_max, nx, ny, nz, d_grid,
+                                            (hipStream_t)stream);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_particles_to_grid: device allocation failed");
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_particles_to_grid: the cell grid of size h exceeds 2^32 cells");
+    return check_status(rc, "mray_hip_particles_to_grid");
+}
+
 int mray_hip_synth_mixing_layer(float *d_out, int n, const double *amp, const double *theta, const double *phi,
                                 uint64_t seed, mray_stream stream)
 {

@@ -30,6 +30,10 @@ int launch_prediction_encode(float *data, const uint32_t *index, int W, int H, i
 int launch_prepare_coded(const float *moments, const uint32_t *index, int W, int H, int m, int warp,
                          const uint16_t *qparams, float *lagrange, hipStream_t stream);
 
+int launch_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
+                             const float *dmin, const float *dmax, int nx, int ny, int nz, float *grid,
+                             hipStream_t stream);
+
 int launch_synth_mixing_layer(float *out, int n, const double *amp, const double *theta, const double *phi,
                               uint64_t seed, hipStream_t stream);
 int launch_synth_marschner_lobb(float *out, int n, hipStream_t stream);

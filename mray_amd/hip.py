@@ -35,6 +35,8 @@ _SIGNATURES = {
     "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_prediction_encode": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_prepare_coded": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_particles_to_grid": ([_vp, _vp, _vp, _vp, C.c_uint32, C.c_float, _f3, _f3, C.c_int, C.c_int, C.c_int,
+                                    _vp, _vp], C.c_int),
     "mray_hip_synth_mixing_layer": ([_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.c_uint64, _vp], C.c_int),
     "mray_hip_synth_marschner_lobb": ([_vp, C.c_int, _vp], C.c_int),
@@ -85,6 +87,11 @@ def _ptr(t):
         raise MrayError("expected a device tensor")
     if not t.is_contiguous():
         raise MrayError("expected a contiguous tensor")
+    if t.numel() == 0:
+        # an empty compact image: torch reports a null data pointer for a
+        # zero-length view; its storage (allocated non-empty by _like /
+        # compact) is a valid address that no kernel dereferences
+        return _vp(t.untyped_storage().data_ptr())
     return _vp(t.data_ptr())
 
 
@@ -238,6 +245,17 @@ def prepare_coded(moments: torch.Tensor, W: int, H: int, m: int, coding_warp: in
     _check(load_library().mray_hip_prepare_coded(_ptr(moments), _ptr(index), W, H, m, coding_warp,
                                                  _ptr(coding_params), _ptr(out), _stream(stream)),
            "mray_hip_prepare_coded")
+    return out
+
+
+def particles_to_grid(x: torch.Tensor, y: torch.Tensor, z: torch.Tensor, v: torch.Tensor, h: float, dmin, dmax,
+                      nx: int, ny: int, nz: int, out=None, stream=None) -> torch.Tensor:
+    """SPH particles sampled at the voxel positions of an (nz, ny, nx) grid."""
+    if out is None:
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device=x.device)
+    _check(load_library().mray_hip_particles_to_grid(_ptr(x), _ptr(y), _ptr(z), _ptr(v), x.numel(), C.c_float(h),
+                                                     _f3(*dmin), _f3(*dmax), nx, ny, nz, _ptr(out),
+                                                     _stream(stream)), "mray_hip_particles_to_grid")
     return out
 
 

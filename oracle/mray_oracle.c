@@ -1582,6 +1582,11 @@ void orc_particles_to_grid(const float *x, const float *y, const float *z, const
         const float q[3] = {x[i], y[i], z[i]};
         int c[3];
         sph_get_cell(&g, q, c);
+        /* the largest coordinate can round to one cell past a padded size
+         * (the reference then indexes past its cell array): clamp */
+        const uint32_t s3[3] = {g.sx, g.sy, g.sz};
+        for (int k = 0; k < 3; ++k)
+            c[k] = c[k] < 0 ? 0 : (c[k] >= (int)s3[k] ? (int)s3[k] - 1 : c[k]);
         keys[i].key = sph_cell_index(&g, c[0], c[1], c[2]);
         keys[i].idx = i;
     }
