@@ -1,0 +1,33 @@
+// generate / reconstruct with the reference's signatures
+// (src/moment_raymarcher.h:48-64), running on one MI355X through
+// libmray_hip.so. Errors are thrown as std::runtime_error carrying the C
+// ABI's message.
+#pragma once
+
+#include "color_image.h"
+#include "cut/camera.h"
+#include "moment_image.h"
+#include "parameters.h"
+#include "transfer_function.h"
+#include "volume.h"
+
+// Kernel form used by generate / prepare / reconstruct: MRAY_FORM_MIXED (1,
+// the default) or MRAY_FORM_PLAIN (0, the reference's FP32 arithmetic).
+void set_kernel_form(int form);
+int get_kernel_form();
+
+// Ray-marches the volume into img (dense moments, bounds when img has
+// bounds), then compacts and prediction-codes it as params ask.
+void generate(cut::Camera cam, const RegularGrid &volume, MomentImageHost &img, const GenerationParameters &params);
+
+// Particles are first sampled onto a grid of spacing smoothing_length / 2
+// (Particles::get_step_size) over their domain, then marched as a grid.
+void generate(cut::Camera cam, const Particles &volume, MomentImageHost &img, const GenerationParameters &params);
+
+// Renders the moment image into RGBA8 (clamped and truncated, as the reference).
+void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, ColorImageHost &cimg,
+                 const ReconstructionParameters &params);
+
+// Float-RGBA overload (un-clamped), which the reference lacks.
+void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, vector<Vec4f> &rgba,
+                 const ReconstructionParameters &params);

@@ -1,0 +1,42 @@
+// Generation / reconstruction parameters with the reference's names and
+// defaults (src/parameters.h:49-115, 231-270), and the "k=v" parsers the CLI uses.
+#pragma once
+
+#include <unordered_map>
+
+#include "common.h"
+
+struct GenerationParameters
+{
+    float step_size = 1.f;
+    float step_size_write_samples = 1.5f;
+    float data_min = 0.0f;
+    float data_max = 1.0f;
+    float error_threshold = 1.f;
+    float bias = 1e-4f;
+    bool compact_image = true;
+    bool prediction_coding = true;
+    bool compress = false;          // moment-image file options: not part of this library
+    bool entropy_coding = false;    // (accepted, ignored)
+    Byte quantization_bits = 32;    //
+    int coding_warp = 0;
+    int quantization_curve_sampling = 64;
+    bool transform_to_bounds = false;
+    float transform_bound_eps = 0.01f;
+};
+
+struct ReconstructionParameters
+{
+    float bias = 1e-6f;
+    float medium_emission = 0.1f;
+    float medium_scattering = 0.1f;
+    float early_ray_termination = 0.95f;
+    float data_min = 0.0f;
+    float data_max = 1.0f;
+    float step_size = 0.01f;
+    float step_size_write_samples = 1.5f;
+    bool use_truncated_fourier = false;
+};
+
+GenerationParameters parse_gen_params(std::unordered_map<string, string> &params);
+ReconstructionParameters parse_rec_params(std::unordered_map<string, string> &params);

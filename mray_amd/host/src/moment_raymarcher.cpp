@@ -1,0 +1,220 @@
+// generate / reconstruct (src/moment_raymarcher.cpp:121-183, 283-353) over
+// the C ABI: host images are uploaded, the kernels run on the current HIP
+// device, results are downloaded into the caller's MomentImageHost /
+// ColorImageHost.
+#include <algorithm>
+#include <cmath>
+
+#include "../include/moment_raymarcher.h"
+#include "device.hpp"
+
+using mray_host::check;
+using mray_host::DeviceBuffer;
+
+namespace
+{
+int g_form = MRAY_FORM_MIXED;
+
+struct VolumeHandle
+{
+    mray_volume v = nullptr;
+    ~VolumeHandle()
+    {
+        if (v)
+            (void)mray_hip_volume_destroy(v);
+    }
+};
+
+void require_size(const MomentImageHost &img)
+{
+    if (img.width <= 0 || img.height <= 0 || img.num_moments <= 0 || img.num_moments > 255)
+        throw invalid_args("moment image needs width, height > 0 and 1..255 moments");
+}
+
+// the generate kernel on a device-resident volume, then the compaction and
+// prediction coding that generate_host applies
+void generate_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &img, const GenerationParameters &params)
+{
+    require_size(img);
+    if (params.prediction_coding && !params.compact_image)
+        throw invalid_args("prediction coding needs a compact image (compact_image=1)");
+    const int W = img.width, H = img.height, m = img.num_moments;
+    const size_t npix = (size_t)W * H;
+    mray_camera c = mray_host::to_c(cam);
+    mray_gen_params g = mray_host::to_c(params);
+    DeviceBuffer d_mom(npix * m * sizeof(float));
+    const bool want_bounds = img.has_bounds() || params.transform_to_bounds;
+    DeviceBuffer d_bounds(want_bounds ? npix * 4 : 0);
+    check(mray_hip_generate(vol, &c, &g, g_form, W, H, m, d_mom.as<float>(), d_bounds.as<uint32_t>(), nullptr,
+                            nullptr),
+          "generate");
+    if (img.has_bounds())
+        d_bounds.download(img.bounds.data(), npix * 4);
+
+    if (!params.compact_image)
+    {
+        img.is_compact = false;
+        img.data.resize(npix * m);
+        d_mom.download(img.data.data(), npix * m * sizeof(float));
+        return;
+    }
+    DeviceBuffer d_index((npix + 1) * 4);
+    check(mray_hip_compute_num_moments(d_mom.as<float>(), img.has_bounds() ? d_bounds.as<uint32_t>() : nullptr, W,
+                                       H, m, params.error_threshold, d_index.as<uint32_t>(), nullptr),
+          "compute_num_moments");
+    check(mray_hip_moment_index(d_index.as<uint32_t>(), W, H, d_index.as<uint32_t>(), nullptr), "moment_index");
+    img.index.resize(npix + 1);
+    d_index.download(img.index.data(), (npix + 1) * 4);
+    const size_t total = img.index[npix];
+    DeviceBuffer d_comp(std::max<size_t>(total, 1) * sizeof(float));
+    check(mray_hip_compact(d_mom.as<float>(), d_index.as<uint32_t>(), W, H, m, d_comp.as<float>(), nullptr),
+          "compact");
+    img.is_compact = true;
+    if (params.prediction_coding)
+    {
+        const bool with_params = requires_coding_parameters(params.coding_warp) && m > 1;
+        DeviceBuffer d_q(with_params ? (size_t)(m - 1) * sizeof(CodingParamType) : 0);
+        check(mray_hip_prediction_encode(d_comp.as<float>(), d_index.as<uint32_t>(), W, H, m, params.coding_warp,
+                                         params.bias, d_q.as<uint16_t>(), nullptr),
+              "prediction_encode");
+        img.prediction_code = true;
+        img.coding_warp = params.coding_warp;
+        img.coding_params.clear();
+        if (with_params)
+        {
+            img.coding_params.resize(m - 1);
+            d_q.download(img.coding_params.data(), (size_t)(m - 1) * sizeof(CodingParamType));
+        }
+    }
+    img.data.resize(total);
+    d_comp.download(img.data.data(), total * sizeof(float));
+}
+
+// multipliers (or trigonometric moments for the truncated Fourier
+// reconstruction) of every pixel into d_lag, layout of mi.data
+void prepare_on_device(const MomentImageHost &mi, const DeviceBuffer &d_data, const DeviceBuffer &d_index,
+                       float bias, bool truncated_fourier, DeviceBuffer &d_lag)
+{
+    const int W = mi.width, H = mi.height, m = mi.num_moments;
+    const uint32_t *idx = mi.is_compact ? d_index.as<uint32_t>() : nullptr;
+    if (truncated_fourier)
+    {
+        if (mi.prediction_code)
+            throw invalid_args("the truncated Fourier reconstruction of a prediction-coded image is not supported");
+        if (!mi.data.empty())
+            mray_host::check_hip(hipMemcpy(d_lag.ptr, d_data.ptr, mi.data.size() * sizeof(float),
+                                           hipMemcpyDeviceToDevice),
+                                 "copy");
+        return;
+    }
+    if (mi.prediction_code)
+    {
+        const bool with_params = requires_coding_parameters(mi.coding_warp) && m > 1;
+        if (with_params && mi.coding_params.size() != (size_t)(m - 1))
+            throw invalid_data("prediction-coded image without its coding parameters");
+        DeviceBuffer d_q(mi.coding_params.data(), with_params ? (size_t)(m - 1) * sizeof(CodingParamType) : 0);
+        check(mray_hip_prepare_coded(d_data.as<float>(), idx, W, H, m, mi.coding_warp, d_q.as<uint16_t>(),
+                                     d_lag.as<float>(), nullptr),
+              "prepare_coded");
+        return;
+    }
+    check(mray_hip_prepare(d_data.as<float>(), idx, W, H, m, bias, g_form, d_lag.as<float>(), nullptr), "prepare");
+}
+
+void reconstruct_float(const cut::Camera &cam, const MomentImageHost &mi, const TransferFunction &tf,
+                       vector<Vec4f> &rgba, const ReconstructionParameters &params)
+{
+    mray_host::ensure_init();
+    require_size(mi);
+    const int W = mi.width, H = mi.height, m = mi.num_moments;
+    const size_t npix = (size_t)W * H;
+    if (mi.is_compact && mi.index.size() != npix + 1)
+        throw invalid_data("compact moment image without index");
+    // data may be empty for a compact image without moments: keep a valid pointer
+    std::unique_ptr<DeviceBuffer> d_data_p(mray_host::upload_nonempty(mi.data));
+    const DeviceBuffer &d_data = *d_data_p;
+    DeviceBuffer d_index(mi.is_compact ? mi.index.data() : nullptr, mi.is_compact ? (npix + 1) * 4 : 0);
+    DeviceBuffer d_bounds(mi.has_bounds() ? mi.bounds.data() : nullptr, mi.has_bounds() ? npix * 4 : 0);
+    DeviceBuffer d_lag(std::max<size_t>(mi.data.size(), 1) * sizeof(float));
+    prepare_on_device(mi, d_data, d_index, params.bias, params.use_truncated_fourier, d_lag);
+    DeviceBuffer d_tf(tf.data.data(), sizeof(tf.data));
+    DeviceBuffer d_rgba(npix * sizeof(Vec4f));
+    mray_camera c = mray_host::to_c(cam);
+    mray_rec_params r = mray_host::to_c(params);
+    const float dmin[3] = {mi.domain.min.x, mi.domain.min.y, mi.domain.min.z};
+    const float dmax[3] = {mi.domain.max.x, mi.domain.max.y, mi.domain.max.z};
+    check(mray_hip_reconstruct(&c, dmin, dmax, d_lag.as<float>(), mi.is_compact ? d_index.as<uint32_t>() : nullptr,
+                               mi.has_bounds() ? d_bounds.as<uint32_t>() : nullptr, W, H, m, d_tf.as<float>(), &r,
+                               g_form, d_rgba.as<float>(), nullptr, nullptr),
+          "reconstruct");
+    rgba.resize(npix);
+    d_rgba.download(rgba.data(), npix * sizeof(Vec4f));
+}
+} // namespace
+
+void set_kernel_form(int form)
+{
+    if (form != MRAY_FORM_PLAIN && form != MRAY_FORM_MIXED)
+        throw invalid_args("kernel form must be MRAY_FORM_PLAIN or MRAY_FORM_MIXED");
+    g_form = form;
+}
+
+int get_kernel_form() { return g_form; }
+
+void generate(cut::Camera cam, const RegularGrid &volume, MomentImageHost &img, const GenerationParameters &params)
+{
+    mray_host::ensure_init();
+    img.domain = volume.domain;
+    if (volume.data.size() * sizeof(float) < volume.num_voxels() * volume.get_byte_size())
+        throw invalid_data("volume data smaller than its size");
+    const float dmin[3] = {volume.domain.min.x, volume.domain.min.y, volume.domain.min.z};
+    const float dmax[3] = {volume.domain.max.x, volume.domain.max.y, volume.domain.max.z};
+    VolumeHandle vh;
+    check(mray_hip_volume_create(volume.voxels(),
+                                 volume.type == RegularGrid::DataType::U8 ? MRAY_DTYPE_U8 : MRAY_DTYPE_F32,
+                                 volume.size.x, volume.size.y, volume.size.z, dmin, dmax, &vh.v),
+          "volume_create");
+    generate_on(vh.v, cam, img, params);
+}
+
+void generate(cut::Camera cam, const Particles &volume, MomentImageHost &img, const GenerationParameters &params)
+{
+    mray_host::ensure_init();
+    img.domain = volume.domain;
+    const uint32_t n = volume.size();
+    if (n == 0 || volume.y.size() != n || volume.z.size() != n || volume.values.size() != n)
+        throw invalid_data("particle arrays must be non-empty and of equal length");
+    const float spacing = volume.get_step_size();
+    const Vec3f lo = volume.domain.min, hi = volume.domain.max;
+    auto cells = [&](float a, float b) { return std::max(1, (int)std::ceil((b - a) / spacing)); };
+    const int nx = cells(lo.x, hi.x), ny = cells(lo.y, hi.y), nz = cells(lo.z, hi.z);
+    DeviceBuffer dx(volume.x.data(), n * 4), dy(volume.y.data(), n * 4), dz(volume.z.data(), n * 4),
+        dv(volume.values.data(), n * 4);
+    DeviceBuffer grid((size_t)nx * ny * nz * sizeof(float));
+    const float dmin[3] = {lo.x, lo.y, lo.z};
+    const float dmax[3] = {hi.x, hi.y, hi.z};
+    check(mray_hip_particles_to_grid(dx.as<float>(), dy.as<float>(), dz.as<float>(), dv.as<float>(), n,
+                                     volume.smoothing_length, dmin, dmax, nx, ny, nz, grid.as<float>(), nullptr),
+          "particles_to_grid");
+    VolumeHandle vh;
+    check(mray_hip_volume_wrap(grid.ptr, MRAY_DTYPE_F32, nx, ny, nz, dmin, dmax, &vh.v), "volume_wrap");
+    generate_on(vh.v, cam, img, params);
+}
+
+void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, vector<Vec4f> &rgba,
+                 const ReconstructionParameters &params)
+{
+    reconstruct_float(cam, mi, tf, rgba, params);
+}
+
+void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, ColorImageHost &cimg,
+                 const ReconstructionParameters &params)
+{
+    if (cimg.width != mi.width || cimg.height != mi.height)
+        throw invalid_args("colour image and moment image differ in size");
+    vector<Vec4f> rgba;
+    reconstruct_float(cam, mi, tf, rgba, params);
+    for (int y = 0; y < mi.height; ++y)
+        for (int x = 0; x < mi.width; ++x)
+            cimg.set_color(x, y, rgba[(size_t)y * mi.width + x]);
+}

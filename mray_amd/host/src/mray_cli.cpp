@@ -1,0 +1,181 @@
+// mray command line (src/main.cpp:563-660): the generate and reconstruct
+// modes with the reference's "k=v,k=v" parameter strings, on one MI355X.
+//
+//   mray --generate <moment_image.bin> <volume.raw> "k=v,..."
+//       width, height, num_moments, camera, GenerationParameters keys,
+//       dims=NX:NY:NZ, dtype=f32|u8, domain=X0:Y0:Z0:X1:Y1:Z1 (default the
+//       unit cube); volumes are raw voxel files [z][y][x] (netCDF and
+//       .h5part loaders are not part of this build)
+//   mray --reconstruct <moment_image.bin> <output.ppm> "k=v,..."
+//       tf, camera, preintegration_step_size_factor, background_r/g/b,
+//       ReconstructionParameters keys; float_rgba=<file> also writes the
+//       un-clamped float RGBA (4 floats per pixel, rows top-down)
+//   both: form=mixed (default) | plain
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <unordered_map>
+
+#include "../include/moment_image_io.h"
+#include "../include/moment_raymarcher.h"
+
+using std::unordered_map;
+
+namespace
+{
+
+unordered_map<string, string> parse_parameters(const string &params)
+{
+    unordered_map<string, string> out;
+    size_t start = 0;
+    while (start <= params.size())
+    {
+        size_t end = params.find(',', start);
+        const string pair = params.substr(start, end == string::npos ? string::npos : end - start);
+        const size_t eq = pair.find('=');
+        if (!pair.empty())
+            out[pair.substr(0, eq)] = eq == string::npos ? "" : pair.substr(eq + 1);
+        if (end == string::npos)
+            break;
+        start = end + 1;
+    }
+    return out;
+}
+
+vector<float> split_floats(const string &s)
+{
+    vector<float> v;
+    std::stringstream ss(s);
+    string item;
+    while (std::getline(ss, item, ':'))
+        v.push_back(std::stof(item));
+    return v;
+}
+
+const string &need(unordered_map<string, string> &p, const char *key)
+{
+    auto it = p.find(key);
+    if (it == p.end())
+        throw invalid_args(string("missing parameter ") + key);
+    return it->second;
+}
+
+void apply_form(unordered_map<string, string> &p)
+{
+    auto it = p.find("form");
+    if (it != p.end())
+    {
+        if (it->second == "plain")
+            set_kernel_form(0);
+        else if (it->second == "mixed")
+            set_kernel_form(1);
+        else
+            throw invalid_args("form must be plain or mixed");
+    }
+}
+
+RegularGrid load_raw(const string &file, unordered_map<string, string> &p)
+{
+    auto dims = split_floats(need(p, "dims"));
+    if (dims.size() != 3)
+        throw invalid_args("dims must be NX:NY:NZ");
+    const bool u8 = p.count("dtype") && p["dtype"] == "u8";
+    Range<Vec3f> dom(Vec3f(0, 0, 0), Vec3f(1, 1, 1));
+    if (p.count("domain"))
+    {
+        auto d = split_floats(p["domain"]);
+        if (d.size() != 6)
+            throw invalid_args("domain must be X0:Y0:Z0:X1:Y1:Z1");
+        dom = Range<Vec3f>(Vec3f(d[0], d[1], d[2]), Vec3f(d[3], d[4], d[5]));
+    }
+    RegularGrid g(Vec3i((int)dims[0], (int)dims[1], (int)dims[2]), dom,
+                  u8 ? RegularGrid::DataType::U8 : RegularGrid::DataType::F32);
+    const size_t bytes = g.num_voxels() * g.get_byte_size();
+    g.data.resize((bytes + 3) / 4);
+    std::ifstream in(file, std::ios::binary);
+    if (!in)
+        throw invalid_args("cannot open volume " + file);
+    in.read(reinterpret_cast<char *>(g.data.data()), bytes);
+    if ((size_t)in.gcount() != bytes)
+        throw invalid_data("volume file " + file + " is smaller than dims x dtype");
+    return g;
+}
+
+void run_generate(const string &mi_file, const string &dataset, unordered_map<string, string> p)
+{
+    apply_form(p);
+    const int width = std::stoi(need(p, "width")), height = std::stoi(need(p, "height"));
+    const int num_moments = std::stoi(need(p, "num_moments"));
+    if (num_moments > 255)
+        throw invalid_args("Number of moments cannot be greater than 255");
+    auto gen_params = parse_gen_params(p);
+    auto cam = cut::load_from_file(need(p, "camera"), width, height);
+    MomentImageHost mi(width, height, num_moments, gen_params.compact_image, gen_params.transform_to_bounds);
+    auto volume = load_raw(dataset, p);
+    mi.domain = volume.domain;
+    generate(cam, volume, mi, gen_params);
+    write_moment_image(mi, mi_file, gen_params.compress, gen_params.quantization_bits, gen_params.entropy_coding);
+}
+
+void run_reconstruct(const string &mi_file, const string &output, unordered_map<string, string> p)
+{
+    apply_form(p);
+    auto rec_params = parse_rec_params(p);
+    float factor = 1.f;
+    if (p.count("preintegration_step_size_factor"))
+        factor = std::stof(p["preintegration_step_size_factor"]);
+    auto tf = read_from_disk(need(p, "tf").c_str(), rec_params.step_size * factor);
+    auto mi = load_moment_image(mi_file);
+    auto cam = cut::load_from_file(need(p, "camera"), mi.width, mi.height);
+    vector<Vec4f> rgba;
+    reconstruct(cam, mi, tf, rgba, rec_params);
+    ColorImageHost img(mi.width, mi.height);
+    for (int y = 0; y < mi.height; ++y)
+        for (int x = 0; x < mi.width; ++x)
+            img.set_color(x, y, rgba[(size_t)y * mi.width + x]);
+    Vec3f bg(1.f, 1.f, 1.f);
+    if (p.count("background_r"))
+        bg.x = std::stof(p["background_r"]);
+    if (p.count("background_g"))
+        bg.y = std::stof(p["background_g"]);
+    if (p.count("background_b"))
+        bg.z = std::stof(p["background_b"]);
+    write_PPM(output, img, bg);
+    if (p.count("float_rgba"))
+    {
+        std::ofstream f(p["float_rgba"], std::ios::binary);
+        f.write(reinterpret_cast<const char *>(rgba.data()), rgba.size() * sizeof(Vec4f));
+    }
+}
+
+} // namespace
+
+int main(int argc, char *argv[])
+{
+    if (argc < 5)
+    {
+        std::cerr << "usage: mray --generate <moment_image> <volume.raw> \"k=v,...\"\n"
+                     "       mray --reconstruct <moment_image> <output.ppm> \"k=v,...\"\n";
+        return 2;
+    }
+    const string mode(argv[1]);
+    try
+    {
+        if (mode == "--generate")
+            run_generate(argv[2], argv[3], parse_parameters(argv[4]));
+        else if (mode == "--reconstruct")
+            run_reconstruct(argv[2], argv[3], parse_parameters(argv[4]));
+        else
+        {
+            std::cerr << "unknown mode " << mode << "\n";
+            return 2;
+        }
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "mray: " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}
