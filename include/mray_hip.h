@@ -100,6 +100,24 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
                          int m, const float *d_tf, const mray_rec_params *p, int form, float *d_rgba,
                          int32_t *d_steps, mray_stream stream);
 
+/* ---- adjoint (no counterpart in the reference, which has no gradients) ---- */
+
+/* Reverse mode of mray_hip_reconstruct (MIXED arithmetic) for
+ * L = sum(d_grad_rgba * rgba): the gradient with respect to the multipliers
+ * (d_grad_lagrange, layout of d_lagrange, overwritten; NULL to skip) and to
+ * the 256 x 4 table (d_grad_tf, ADDED to, so the caller zeroes it; NULL to
+ * skip). d_rgba is the forward output of mray_hip_reconstruct on the same
+ * inputs. The march is recomputed per ray (no per-sample storage: memory
+ * O(pixels x moments)); bounds are constants, the transfer-function lookup
+ * is differentiated as the reference evaluates it (entries from the front
+ * sample, weight from the back sample) and the density through the
+ * fast_atan polynomial. Up to 64 moments. */
+int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                                  const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W,
+                                  int H, int m, const float *d_tf, const mray_rec_params *p, const float *d_rgba,
+                                  const float *d_grad_rgba, float *d_grad_lagrange, float *d_grad_tf,
+                                  mray_stream stream);
+
 /* ---- compaction (src/moment_compaction.h, src/moment_image.cpp:69-94) ---- */
 
 /* Number of moments to keep per pixel (compute_num_moments,

@@ -1,0 +1,50 @@
+"""torch.autograd wrappers over the HIP kernels (forward and hand-written
+adjoint behind the C ABI).
+
+ReconstructFunction: float RGBA of the moment-image march as a function of the
+Lagrange multipliers and of the 256 x 4 transfer-function table. The backward
+pass calls mray_hip_reconstruct_backward, which re-marches each ray
+(per-ray checkpointing: memory stays O(pixels x moments)).
+
+transfer_table: the reference's preintegrated table T[x] = (rgb^2.2 a s, a s)
+from a float RGBA transfer function in [0, 1] (src/transfer_function.cpp:46-119),
+in torch ops so that gradients reach the RGBA entries."""
+from __future__ import annotations
+
+import torch
+
+from . import hip
+from .types import RecParams
+
+
+class ReconstructFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, lagrange, table, cam, dmin, dmax, W, H, m, rp: RecParams, index, bounds):
+        rgba, _ = hip.reconstruct(cam, dmin, dmax, lagrange.detach().contiguous(), W, H, m,
+                                  table.detach().contiguous(), rp, form="mixed", index=index, bounds=bounds)
+        ctx.save_for_backward(lagrange, table, rgba)
+        ctx.args = (cam, dmin, dmax, W, H, m, rp, index, bounds)
+        return rgba
+
+    @staticmethod
+    def backward(ctx, grad_rgba):
+        lagrange, table, rgba = ctx.saved_tensors
+        cam, dmin, dmax, W, H, m, rp, index, bounds = ctx.args
+        g_lag, g_tf = hip.reconstruct_backward(cam, dmin, dmax, lagrange.detach().contiguous(), W, H, m,
+                                               table.detach().contiguous(), rp, rgba,
+                                               grad_rgba.detach().to(torch.float32).contiguous(), index=index,
+                                               bounds=bounds, want_lagrange=ctx.needs_input_grad[0],
+                                               want_tf=ctx.needs_input_grad[1])
+        return g_lag, g_tf, None, None, None, None, None, None, None, None, None
+
+
+def reconstruct(lagrange, table, cam, dmin, dmax, W, H, m, rp: RecParams, index=None, bounds=None):
+    """Differentiable float RGBA (H, W, 4)."""
+    return ReconstructFunction.apply(lagrange, table, cam, dmin, dmax, W, H, m, rp, index, bounds)
+
+
+def transfer_table(tf_rgba: torch.Tensor, step_size: float) -> torch.Tensor:
+    """(256, 4) table (rgb^2.2 * a * s, a * s) of a float RGBA transfer function."""
+    rgb = tf_rgba[:, :3].clamp(0.0, 1.0).pow(2.2)
+    a = tf_rgba[:, 3:4].clamp(0.0, 1.0) * step_size
+    return torch.cat([rgb * a, a], dim=1)

@@ -206,6 +206,29 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
     return check_status(rc, "mray_hip_reconstruct");
 }
 
+int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                                  const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W,
+                                  int H, int m, const float *d_tf, const mray_rec_params *p, const float *d_rgba,
+                                  const float *d_grad_rgba, float *d_grad_lagrange, float *d_grad_tf,
+                                  mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_reconstruct_backward: call mray_hip_init first");
+    if (!cam || !dom_min || !dom_max || !d_lagrange || !d_tf || !p || !d_rgba || !d_grad_rgba)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_backward: null argument");
+    if (!d_grad_lagrange && !d_grad_tf)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_backward: no gradient requested");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_backward: bad image size %dx%d or moment count %d", W, H, m);
+    if (!(p->step_size > 0.0f))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_backward: step_size must be > 0");
+    int rc = mray::launch_reconstruct_adjoint(*cam, dom_min, dom_max, d_lagrange, d_index, d_bounds, W, H, m, d_tf, *p,
+                                              d_rgba, d_grad_rgba, d_grad_lagrange, d_grad_tf, (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_reconstruct_backward: at most 64 moments");
+    return check_status(rc, "mray_hip_reconstruct_backward");
+}
+
 int mray_hip_compute_num_moments(const float *d_moments, const uint32_t *d_bounds, int W, int H, int m,
                                  float error_threshold, uint32_t *d_counts, mray_stream stream)
 {

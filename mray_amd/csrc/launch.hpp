@@ -21,6 +21,11 @@ int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *d
                        const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
                        const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream);
 
+int launch_reconstruct_adjoint(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
+                               const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                               const mray_rec_params &p, const float *rgba, const float *grad_rgba,
+                               float *grad_lagrange, float *grad_tf, hipStream_t stream);
+
 int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H, int m, float error_threshold,
                        uint32_t *counts, hipStream_t stream);
 int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);

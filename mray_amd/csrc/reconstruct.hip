@@ -286,6 +286,255 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
         steps[pix] = ns;
 }
 
+// ---------------------------------------------------------------------------
+// ADJOINT of the MIXED march
+// ---------------------------------------------------------------------------
+
+// d fast_atan / dt of the polynomial itself (src/mese/complex_algebra.h:131-152):
+// for |t| <= 1 it is x P(x^2), otherwise pi/2 - x P(x^2) with x = 1/|t|.
+__device__ __forceinline__ float fast_atan_deriv(float t)
+{
+    const float at = fabsf(t);
+    const bool greater_one = at > 1.0f;
+    const float x = greater_one ? 1.0f / at : at;
+    const float y = x * x;
+    const float P = (((0.020835f * y - 0.085133f) * y + 0.180141f) * y - 0.330299f) * y + 0.999866f;
+    const float dP = ((4.0f * 0.020835f * y - 3.0f * 0.085133f) * y + 2.0f * 0.180141f) * y - 0.330299f;
+    const float dxp = P + 2.0f * y * dP; // d (x P(x^2)) / dx
+    return greater_one ? dxp * y : dxp;
+}
+
+template <int M>
+__device__ __forceinline__ double series_reinsch_f(const float (&lam)[M], float p)
+{
+    // series_reinsch on float multipliers (converted term by term)
+    float sh, ch;
+    sincosf(-0.5f * p, &sh, &ch);
+    const bool reflect = ch < sh;
+    const double q = reflect ? (double)ch : (double)sh;
+    const double u = -4.0 * q * q;
+    const double sg = reflect ? -1.0 : 1.0;
+    double b = 0.0, d = 0.0;
+#pragma unroll
+    for (int l = M - 1; l >= 1; --l)
+    {
+        const double ll = (double)lam[l];
+        const double lk = (l & 1) ? fma(sg, ll, d) : ll + d;
+        d = fma(u, b, lk);
+        b = d + b;
+    }
+    return fma(2.0, fma(0.5 * u, b, d), (double)lam[0]);
+}
+
+// Reverse mode of reconstruct_mixed_kernel for one ray, with the march
+// recomputed (per-ray checkpointing: nothing per sample is stored). With the
+// forward RGBA L known, the radiance that sample k still shades is L - P_k
+// (P_k the prefix sum), so one pass in forward order gives
+//     dL/dc_k     = g_rgb e T_k                       (emission of sample k)
+//     dL/dsigma_k = -g_rgb . (L - P_k) + g_a T_end     (extinction of sample k)
+// with T_k the transmittance in front of sample k and T_end = 1 - alpha
+// (0 after early termination). The transfer-function lookup takes its
+// entries from fs and its weight from frac(256 bs), so only bs carries a
+// derivative: d c / d bs = 256 (T[x1] - T[x0]). Then
+// d rho / d S = fast_atan'(S) / pi (hi - lo) and dS / d lambda_l = c_l cos(l theta)
+// (c_0 = 1, c_l = 2), cos(l theta) by the Chebyshev recurrence in FP64.
+// Transfer-function gradients are summed per workgroup in LDS and added to
+// grad_tf with one atomic per entry and workgroup.
+template <int M>
+__global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
+    DevCamera cam, RecArgs a, const float *__restrict__ lag, const uint32_t *__restrict__ index,
+    const uint32_t *__restrict__ bounds, const float4 *__restrict__ tf_g, const float4 *__restrict__ rgba_fwd,
+    const float4 *__restrict__ grad_rgba, float *__restrict__ grad_lag, float *__restrict__ grad_tf)
+{
+    __shared__ float4 tf[256];
+    __shared__ float gtf[256 * 4];
+    const int tid = threadIdx.x;
+    tf[tid] = tf_g[tid];
+    for (int i = 0; i < 4; ++i)
+        gtf[tid * 4 + i] = 0.0f;
+    __syncthreads();
+
+    const int lane = tid & 63;
+    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    int x = 0, y = 0;
+    bool active = tile < ntiles;
+    if (active)
+    {
+        tile_pixel(tile, a.W, lane, x, y);
+        active = x < a.W && y < a.H;
+    }
+    if (active)
+    {
+        const size_t pix = (size_t)y * a.W + x;
+        const int nm = index ? min((int)(index[pix + 1] - index[pix]), a.m) : a.m;
+        const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
+        const float4 g = grad_rgba[pix];
+        if (nm == 0)
+        {
+            // reconstruct_zero_ray: the colour is the table at (0, 0) = entry 0
+            if (grad_tf)
+            {
+                atomicAdd(&gtf[0], g.x);
+                atomicAdd(&gtf[1], g.y);
+                atomicAdd(&gtf[2], g.z);
+                atomicAdd(&gtf[3], g.w);
+            }
+        }
+        else
+        {
+            float lam[M], glam[M];
+#pragma unroll
+            for (int l = 0; l < M; ++l)
+            {
+                lam[l] = (l < nm) ? lag[off + l] : 0.0f;
+                glam[l] = 0.0f;
+            }
+            float blo = 0.f, bhi = 1.f;
+            if (bounds)
+            {
+                const uint32_t b = bounds[pix];
+                blo = (b & 0xFFFFu) / 65535.f;
+                bhi = (b >> 16) / 65535.f;
+            }
+            const float brange = bhi - blo;
+            const bool fourier = a.fourier != 0;
+            const float4 L = rgba_fwd[pix];
+            const double t_end = 1.0 - (double)L.w;
+            const double ge[3] = {(double)g.x * a.emission, (double)g.y * a.emission, (double)g.z * a.emission};
+
+            Ray r = pixel_ray(cam, a.W, a.H, x, y);
+            float t_near, t_far;
+            const bool hit = intersect_aabb(r, a.dmin, a.dmax, t_near, t_far);
+            t_near = rmax(t_near, cam.znear);
+            if (hit && (t_far > 0))
+            {
+                float t = t_near;
+                float p = (float)((double)((t - t_near) / (t_far - t_near)) * kPi - kPi);
+                const float dphi = (float)(kPi * (double)(a.step / (t_far - t_near)));
+                float T = 1.0f, tau = 0.0f;
+                auto rho_of = [&](float s) {
+                    const float rho = fourier ? s : __builtin_fmaf(fast_atan(s), 0.318309886f, 0.5f);
+                    return __builtin_fmaf(rho, brange, blo);
+                };
+                float fs = rho_of((float)series_reinsch_f<M>(lam, rclamp(p, (float)(-kPi), 0.0f)));
+                double pre[3] = {0.0, 0.0, 0.0};
+                const int cap = march_cap(t_near, t_far, a.step) + 1;
+                for (int it = 0; t <= t_far && it < cap; ++it)
+                {
+                    t += a.step;
+                    p += dphi;
+                    const float pc = rclamp(p, (float)(-kPi), 0.0f);
+                    const float S = (float)series_reinsch_f<M>(lam, pc);
+                    const float bs = rho_of(S);
+                    const float4 c = tf_sample(tf, fs, bs);
+
+                    // indices and weights of tf_sample
+                    const float ipx = fs * 256.0f, ipy = bs * 256.0f;
+                    const float ax = ipx - floorf(ipx), ay = ipy - floorf(ipy);
+                    const int xl = (int)floorf(ipx);
+                    const int x0 = iclamp(xl, 0, 255), x1 = iclamp(xl + 1, 0, 255);
+                    const float w01 = (1.f - ax) * (1.f - ay) + ax * (1.f - ay);
+                    const float w23 = (1.f - ax) * ay + ax * ay;
+                    const float4 lo4 = tf[x0], hi4 = tf[x1];
+
+                    const double Td = (double)T;
+                    const double gc[3] = {ge[0] * Td, ge[1] * Td, ge[2] * Td};
+                    pre[0] = fma((double)(a.emission * c.x), Td, pre[0]);
+                    pre[1] = fma((double)(a.emission * c.y), Td, pre[1]);
+                    pre[2] = fma((double)(a.emission * c.z), Td, pre[2]);
+                    const double gsig = -((double)g.x * ((double)L.x - pre[0]) + (double)g.y * ((double)L.y - pre[1]) +
+                                          (double)g.z * ((double)L.z - pre[2])) +
+                                        (double)g.w * t_end;
+                    if (grad_tf)
+                    {
+                        const float gv[4] = {(float)gc[0], (float)gc[1], (float)gc[2], (float)gsig};
+                        for (int ch = 0; ch < 4; ++ch)
+                        {
+                            atomicAdd(&gtf[x0 * 4 + ch], gv[ch] * w01);
+                            atomicAdd(&gtf[x1 * 4 + ch], gv[ch] * w23);
+                        }
+                    }
+                    const double drho = 256.0 * (gc[0] * (double)(hi4.x - lo4.x) + gc[1] * (double)(hi4.y - lo4.y) +
+                                                 gc[2] * (double)(hi4.z - lo4.z) + gsig * (double)(hi4.w - lo4.w));
+                    const float dS = (float)drho * brange * (fourier ? 1.0f : fast_atan_deriv(S) * 0.318309886f);
+                    if (dS != 0.0f)
+                    {
+                        // cos(l theta), theta = -pc, by the Chebyshev recurrence
+                        const double c1 = cos((double)pc);
+                        double cm = 1.0, cl = c1;
+                        glam[0] += dS;
+#pragma unroll
+                        for (int l = 1; l < M; ++l)
+                        {
+                            glam[l] = __builtin_fmaf(dS, (float)cl, glam[l]);
+                            const double cn = fma(2.0 * c1, cl, -cm);
+                            cm = cl;
+                            cl = cn;
+                        }
+                    }
+                    tau += c.w;
+                    T = __expf(-tau);
+                    fs = bs;
+                    if (1.0f - T >= a.ert)
+                        break;
+                }
+            }
+            if (grad_lag)
+                for (int l = 0; l < nm; ++l)
+                    grad_lag[off + l] = l == 0 ? glam[0] : 2.0f * glam[l];
+        }
+    }
+    __syncthreads();
+    if (grad_tf)
+        for (int i = tid; i < 1024; i += blockDim.x)
+        {
+            const float v = gtf[i];
+            if (v != 0.0f)
+                atomicAdd(&grad_tf[i], v);
+        }
+}
+
+int launch_reconstruct_adjoint(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
+                               const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                               const mray_rec_params &p, const float *rgba, const float *grad_rgba,
+                               float *grad_lagrange, float *grad_tf, hipStream_t stream)
+{
+    RecArgs a;
+    a.W = W;
+    a.H = H;
+    a.m = m;
+    a.step = p.step_size;
+    a.emission = p.medium_emission;
+    a.ert = p.early_ray_termination;
+    a.fourier = p.use_truncated_fourier;
+    a.dmin = {dmin[0], dmin[1], dmin[2]};
+    a.dmax = {dmax[0], dmax[1], dmax[2]};
+    DevCamera c = make_dev_camera(cam);
+    const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
+    const uint32_t blocks = (ntiles + 3) / 4;
+    const float4 *tf4 = reinterpret_cast<const float4 *>(tf);
+    const float4 *fw = reinterpret_cast<const float4 *>(rgba);
+    const float4 *gr = reinterpret_cast<const float4 *>(grad_rgba);
+#define MRAY_ADJ_CASE(MM)                                                                                             \
+    if (m <= MM)                                                                                                      \
+    {                                                                                                                 \
+        hipLaunchKernelGGL(reconstruct_adjoint_kernel<MM>, dim3(blocks), dim3(256), 0, stream, c, a, lagrange, index, \
+                           bounds, tf4, fw, gr, grad_lagrange, grad_tf);                                              \
+        return 0;                                                                                                     \
+    }
+    MRAY_ADJ_CASE(8)
+    MRAY_ADJ_CASE(16)
+    MRAY_ADJ_CASE(24)
+    MRAY_ADJ_CASE(32)
+    MRAY_ADJ_CASE(40)
+    MRAY_ADJ_CASE(48)
+    MRAY_ADJ_CASE(56)
+    MRAY_ADJ_CASE(64)
+#undef MRAY_ADJ_CASE
+    return MRAY_ERR_UNSUPPORTED;
+}
+
 int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
                        const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
                        const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream)

@@ -30,6 +30,8 @@ _SIGNATURES = {
     "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_reconstruct": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_reconstruct_backward": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                       C.POINTER(RecParams), _vp, _vp, _vp, _vp, _vp], C.c_int),
     "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
@@ -183,6 +185,23 @@ def reconstruct(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m
                                                _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp), form_id(form),
                                                _ptr(rgba), _ptr(steps), _stream(stream)), "mray_hip_reconstruct")
     return rgba, steps
+
+
+def reconstruct_backward(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m: int, tf: torch.Tensor,
+                         rp: RecParams, rgba: torch.Tensor, grad_rgba: torch.Tensor, index=None, bounds=None,
+                         want_lagrange=True, want_tf=True, stream=None):
+    """(dL/dlagrange or None, dL/dtable (256, 4) or None) for L = sum(grad_rgba * rgba)."""
+    dev = lagrange.device
+    g_lag = _like(lagrange) if want_lagrange else None
+    g_tf = torch.zeros((256, 4), dtype=torch.float32, device=dev) if want_tf else None
+    c = Camera.from_dict(cam)
+    tf = tf.to(device=dev, dtype=torch.float32).contiguous()
+    _check(load_library().mray_hip_reconstruct_backward(C.byref(c), _f3(*dmin), _f3(*dmax), _ptr(lagrange),
+                                                        _ptr(index), _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp),
+                                                        _ptr(rgba.contiguous()), _ptr(grad_rgba.contiguous()),
+                                                        _ptr(g_lag), _ptr(g_tf), _stream(stream)),
+           "mray_hip_reconstruct_backward")
+    return g_lag, g_tf
 
 
 def compute_num_moments(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None,

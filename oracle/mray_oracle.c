@@ -860,7 +860,7 @@ void orc_tf_sample(const float *tf, float fs, float bs, float out[4])
         out[c] = w0 * ll[c] + w1 * lr[c] + w2 * rl[c] + w3 * rr[c];
 }
 
-static void tf_sample_f64(const float *tf, double fs, double bs, double out[4])
+static void tf_sample_f64(const double *tf, double fs, double bs, double out[4])
 {
     double ipx = fs * 256.0, ipy = bs * 256.0;
     double ax = ipx - floor(ipx), ay = ipy - floor(ipy);
@@ -1011,7 +1011,7 @@ void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const fl
  * loop and float phase sequence (it increments the phase in float),
  * densities, TF lerp and compositing in double with exact atan and exp. */
 static void reconstruct_ray_f64(const float org[3], const float dir[3], const float bmin[3], const float bmax[3],
-                                float znear, int m, const double *pm, double blo, double bhi, const float *tf,
+                                float znear, int m, const double *pm, double blo, double bhi, const double *tf,
                                 const mray_rec_params *rp, double *out, int32_t *n_steps)
 {
     float t_near, t_far;
@@ -1062,6 +1062,9 @@ void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], cons
                                const double *moments, const uint32_t *bounds, const float *tf,
                                const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
 {
+    double tfd[1024];
+    for (int i = 0; i < 1024; ++i)
+        tfd[i] = tf[i];
 #pragma omp parallel for num_threads(set_threads(nthreads))
     for (int y = 0; y < H; ++y)
     {
@@ -1074,35 +1077,181 @@ void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], cons
             double bhi = bounds ? MRAY_BOUND_HI(bounds[pix]) / 65535.0 : 1.0;
             float org[3], dir[3];
             orc_pixel_ray(cam, W, H, x, y, org, dir);
-            reconstruct_ray_f64(org, dir, bmin, bmax, cam->znear, m, pm, blo, bhi, tf, rp, rgba + pix * 4,
+            reconstruct_ray_f64(org, dir, bmin, bmax, cam->znear, m, pm, blo, bhi, tfd, rp, rgba + pix * 4,
                                 n_steps ? n_steps + pix : NULL);
         }
     }
 }
 
+/* d fast_atan_f64 / dt: derivative of the polynomial (not of atan). */
+static double fast_atan_deriv_f64(double t)
+{
+    double at = t < 0.0 ? -t : t;
+    int greater_one = at > 1.0;
+    double x = greater_one ? 1.0 / at : at;
+    double y = x * x;
+    double P = (((double)0.020835f * y - (double)0.085133f) * y + (double)0.180141f) * y - (double)0.330299f;
+    P = P * y + (double)0.999866f;
+    double dP = ((4.0 * (double)0.020835f * y - 3.0 * (double)0.085133f) * y + 2.0 * (double)0.180141f) * y -
+                (double)0.330299f;
+    double dxp = P + 2.0 * y * dP;
+    return greater_one ? dxp * y : dxp;
+}
+
+/* Reverse mode of reconstruct_ray_f64 on given multipliers: the march is run
+ * forward storing every step, then swept backwards (the adjoint of
+ * L += e c T, tau += sigma, T = exp(-tau), alpha = 1 - T), through the
+ * transfer-function lookup (entries from fs, weight from frac(256 bs)) and
+ * the density rho = fast_atan(S)/pi + 1/2 mapped to the bounds, to the
+ * multipliers (dS/dlambda_l = c_l cos(l theta), c_0 = 1, c_l = 2) and to the
+ * 256 x 4 table. Test infrastructure: the checker of the GPU adjoint. */
+#define ORC_ADJ_MAXSTEPS 65536
+void orc_reconstruct_adjoint_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                 int m, const float *lagrange, const uint32_t *bounds, const float *tf,
+                                 const mray_rec_params *rp, const double *g_rgba, double *grad_lag, double *grad_tf,
+                                 int nthreads)
+{
+    for (int i = 0; i < 1024; ++i)
+        grad_tf[i] = 0.0;
+#pragma omp parallel num_threads(set_threads(nthreads))
+    {
+        double gtf[1024];
+        for (int i = 0; i < 1024; ++i)
+            gtf[i] = 0.0;
+        double *st = (double *)malloc(sizeof(double) * 8 * ORC_ADJ_MAXSTEPS);
+        int *sx = (int *)malloc(sizeof(int) * 2 * ORC_ADJ_MAXSTEPS);
+#pragma omp for
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+            {
+                size_t pix = (size_t)y * W + x;
+                double pm[MAXM];
+                for (int l = 0; l < m; ++l)
+                {
+                    pm[l] = lagrange[pix * (size_t)m + l];
+                    grad_lag[pix * (size_t)m + l] = 0.0;
+                }
+                double blo = bounds ? MRAY_BOUND_LO(bounds[pix]) / 65535.0 : 0.0;
+                double bhi = bounds ? MRAY_BOUND_HI(bounds[pix]) / 65535.0 : 1.0;
+                float org[3], dir[3];
+                orc_pixel_ray(cam, W, H, x, y, org, dir);
+                float t_near, t_far;
+                int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
+                t_near = fmaxr(t_near, cam->znear);
+                if (!(hit && t_far > 0))
+                    continue;
+                /* forward, storing per step: theta, S, ay, T before, c[4] */
+                float step = rp->step_size;
+                float t = t_near;
+                float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+                float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+                double tau = 0.0, T = 1.0;
+                double pc = dclampr(p, -M_PI, 0.0);
+                double fs = orc_evaluate_mese_f64(m, pc, pm) * (bhi - blo) + blo;
+                int K = 0, ert = 0;
+                while (t <= t_far && K < ORC_ADJ_MAXSTEPS)
+                {
+                    t += step;
+                    p += phase_step;
+                    pc = dclampr(p, -M_PI, 0.0);
+                    double S = 0.0;
+                    for (int l = 1; l < m; ++l)
+                        S += pm[l] * cos(l * pc);
+                    S = 2.0 * S + pm[0];
+                    double bs = (fast_atan_f64(S) / M_PI + 0.5) * (bhi - blo) + blo;
+                    double ipx = fs * 256.0, ipy = bs * 256.0;
+                    double ay = ipy - floor(ipy);
+                    int xl = (int)floor(ipx);
+                    int x0 = iclampr(xl, 0, 255), x1 = iclampr(xl + 1, 0, 255);
+                    double *r = st + 8 * K;
+                    r[0] = pc;
+                    r[1] = S;
+                    r[2] = ay;
+                    r[3] = T;
+                    for (int c = 0; c < 4; ++c)
+                        r[4 + c] = (1.0 - ay) * tf[x0 * 4 + c] + ay * tf[x1 * 4 + c];
+                    sx[2 * K] = x0;
+                    sx[2 * K + 1] = x1;
+                    ++K;
+                    tau += r[7];
+                    T = exp(-tau);
+                    fs = bs;
+                    if (1.0 - T >= rp->early_ray_termination)
+                    {
+                        ert = 1;
+                        break;
+                    }
+                }
+                /* reverse sweep */
+                const double *g = g_rgba + pix * 4;
+                double Tbar = ert ? 0.0 : -g[3]; /* alpha = 1 - T_final */
+                double taubar = 0.0;
+                double e = rp->medium_emission;
+                for (int k = K - 1; k >= 0; --k)
+                {
+                    const double *r = st + 8 * k;
+                    /* T_{k+1} = exp(-tau_{k+1}): the stored T of step k+1, or the final T */
+                    double Tk1 = (k + 1 < K) ? st[8 * (k + 1) + 3] : T;
+                    taubar += -Tk1 * Tbar;
+                    double sigbar = taubar;
+                    double cbar[4] = {g[0] * e * r[3], g[1] * e * r[3], g[2] * e * r[3], sigbar};
+                    Tbar = g[0] * e * r[4] + g[1] * e * r[5] + g[2] * e * r[6];
+                    int x0 = sx[2 * k], x1 = sx[2 * k + 1];
+                    double rhobar = 0.0;
+                    for (int c = 0; c < 4; ++c)
+                    {
+                        gtf[x0 * 4 + c] += cbar[c] * (1.0 - r[2]);
+                        gtf[x1 * 4 + c] += cbar[c] * r[2];
+                        rhobar += cbar[c] * 256.0 * ((double)tf[x1 * 4 + c] - (double)tf[x0 * 4 + c]);
+                    }
+                    double Sbar = rhobar * (bhi - blo) * fast_atan_deriv_f64(r[1]) / M_PI;
+                    grad_lag[pix * (size_t)m] += Sbar;
+                    for (int l = 1; l < m; ++l)
+                        grad_lag[pix * (size_t)m + l] += 2.0 * Sbar * cos(l * r[0]);
+                }
+            }
+#pragma omp critical
+        for (int i = 0; i < 1024; ++i)
+            grad_tf[i] += gtf[i];
+        free(st);
+        free(sx);
+    }
+}
+
 /* FP64 evaluation of given (float) multipliers: isolates the reconstruct
  * march from the preparation. */
-void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
-                                  int m, const float *lagrange, const uint32_t *bounds, const float *tf,
-                                  const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
+void orc_reconstruct_lagrange64_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                    int m, const double *lagrange, const uint32_t *bounds, const double *tf,
+                                    const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
 {
 #pragma omp parallel for num_threads(set_threads(nthreads))
     for (int y = 0; y < H; ++y)
     {
-        double pm[MAXM];
         for (int x = 0; x < W; ++x)
         {
             size_t pix = (size_t)y * W + x;
-            for (int l = 0; l < m; ++l)
-                pm[l] = lagrange[pix * (size_t)m + l];
             double blo = bounds ? MRAY_BOUND_LO(bounds[pix]) / 65535.0 : 0.0;
             double bhi = bounds ? MRAY_BOUND_HI(bounds[pix]) / 65535.0 : 1.0;
             float org[3], dir[3];
             orc_pixel_ray(cam, W, H, x, y, org, dir);
-            reconstruct_ray_f64(org, dir, bmin, bmax, cam->znear, m, pm, blo, bhi, tf, rp, rgba + pix * 4,
-                                n_steps ? n_steps + pix : NULL);
+            reconstruct_ray_f64(org, dir, bmin, bmax, cam->znear, m, lagrange + pix * (size_t)m, blo, bhi, tf, rp,
+                                rgba + pix * 4, n_steps ? n_steps + pix : NULL);
         }
     }
+}
+
+void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                  int m, const float *lagrange, const uint32_t *bounds, const float *tf,
+                                  const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
+{
+    double tfd[1024];
+    for (int i = 0; i < 1024; ++i)
+        tfd[i] = tf[i];
+    double *lag = (double *)malloc(sizeof(double) * (size_t)W * H * m);
+    for (size_t i = 0; i < (size_t)W * H * m; ++i)
+        lag[i] = lagrange[i];
+    orc_reconstruct_lagrange64_f64(cam, bmin, bmax, W, H, m, lag, bounds, tfd, rp, rgba, n_steps, nthreads);
+    free(lag);
 }
 
 /* ------------------------------------------------------------------------ */

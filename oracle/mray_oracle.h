@@ -97,6 +97,16 @@ void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], c
                                   int m, const float *lagrange, const uint32_t *bounds, const float *tf,
                                   const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);
 
+void orc_reconstruct_lagrange64_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                    int m, const double *lagrange, const uint32_t *bounds, const double *tf,
+                                    const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads);
+
+/* ---- adjoint of the march on given multipliers (FP64, store-all reverse sweep) ---- */
+void orc_reconstruct_adjoint_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
+                                 int m, const float *lagrange, const uint32_t *bounds, const float *tf,
+                                 const mray_rec_params *rp, const double *g_rgba, double *grad_lag, double *grad_tf,
+                                 int nthreads);
+
 /* ---- compaction (src/moment_compaction.h, src/moment_image.cpp:69-94) ---- */
 void orc_evaluate_unknown_moments(int n, const float *exp_c, int m, float *new_exp_c);
 void orc_exp_to_bounded_moments(int m, const float *exp_c, float *bounded_c, int n);

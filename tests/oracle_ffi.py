@@ -140,6 +140,32 @@ class Oracle:
                                               ptr(tf, _f), C.byref(rp), ptr(rgba, _d), ptr(ns, _i32), nthreads())
         return rgba, ns
 
+    def reconstruct_lagrange64_f64(self, cam, dmin, dmax, W, H, m, lag64, tf64, rp: RecParams, bounds=None):
+        """FP64 march on FP64 multipliers and an FP64 table (finite-difference reference)."""
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float64)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_reconstruct_lagrange64_f64(C.byref(c), bmin, bmax, W, H, m,
+                                                ptr(np.ascontiguousarray(lag64, np.float64), _d), ptr(bounds, _u32),
+                                                ptr(np.ascontiguousarray(tf64, np.float64), _d), C.byref(rp),
+                                                ptr(rgba, _d), None, nthreads())
+        return rgba
+
+    def reconstruct_adjoint_f64(self, cam, dmin, dmax, W, H, m, lag32, tf, rp: RecParams, g_rgba, bounds=None):
+        """(dL/dlambda [H*W*m], dL/dtable [256, 4]) of L = sum(g_rgba * rgba) in FP64."""
+        c = Camera.from_dict(cam)
+        gl = np.zeros(W * H * m, np.float64)
+        gt = np.zeros((256, 4), np.float64)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_reconstruct_adjoint_f64(C.byref(c), bmin, bmax, W, H, m,
+                                             ptr(np.ascontiguousarray(lag32, np.float32), _f), ptr(bounds, _u32),
+                                             ptr(np.ascontiguousarray(tf, np.float32), _f), C.byref(rp),
+                                             ptr(np.ascontiguousarray(g_rgba, np.float64), _d), ptr(gl, _d),
+                                             ptr(gt, _d), nthreads())
+        return gl, gt
+
     def prepare_std(self, m, moments, bias):
         out = np.zeros(m, np.float32)
         self.lib.orc_prepare_moments_std(m, ptr(np.ascontiguousarray(moments, np.float32), _f), C.c_float(bias),
