@@ -118,6 +118,14 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
                                   const float *d_grad_rgba, float *d_grad_lagrange, float *d_grad_tf,
                                   mray_stream stream);
 
+/* Reverse mode of mray_hip_prepare (MIXED mathematics, FP64): given
+ * dL/dlagrange, writes dL/dmoments (layout of d_moments) for the bias lerp,
+ * Herglotz transform, Levinson recursion (including its |u|^2 >= 1 guard)
+ * and correlations. Uses stream-ordered scratch of 256 x 128 x 7 x m x 16
+ * bytes (hipMallocAsync). */
+int mray_hip_prepare_backward(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
+                              const float *d_grad_lagrange, float *d_grad_moments, mray_stream stream);
+
 /* ---- compaction (src/moment_compaction.h, src/moment_image.cpp:69-94) ---- */
 
 /* Number of moments to keep per pixel (compute_num_moments,

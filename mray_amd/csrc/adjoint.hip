@@ -1,0 +1,280 @@
+// Adjoint of the bounded-MESE preparation (MIXED, FP64): given dL/dlambda,
+// dL/dmoments of lambda = prepare(moments) (prepare_mixed_kernel's
+// mathematics: bias lerp, Herglotz transform, Levinson recursion,
+// autocorrelation, imaginary correlation; src/mese/MESE_dynamic.h:35-211).
+// The reference has no gradients; this is new.
+//
+// Reverse mode of the algorithm itself, one thread per pixel in FP64:
+//  * the forward is recomputed and its arrays kept: exponential moments E,
+//    Levinson solution P, autocorrelation R and the reflection coefficient
+//    u_j of every Levinson step;
+//  * the Levinson recursion is swept backwards without storing its iterates:
+//    each step maps the pair (P_k, P_{j-k}) to
+//        a' = f (a - conj(b) u),  b' = f (b - conj(a) u),  f = 1 / (1 - |u|^2),
+//    which inverts to a = a' + u conj(b'), b = b' + u conj(a'), so the
+//    previous iterate is rebuilt from the current one and u_j;
+//  * steps after the |u|^2 >= 1 guard fired use u = 0 (constant).
+// Per-pixel arrays live in a global scratch laid out [array][element][thread]
+// so that the lanes of a wavefront touch consecutive addresses; the grid is a
+// fixed number of threads that stride over the pixels, which bounds the
+// scratch to threads x 7 arrays x m x 16 bytes.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace mray
+{
+
+namespace
+{
+
+struct dc
+{
+    double x, y;
+};
+__device__ __forceinline__ dc dmk(double x, double y) { return dc{x, y}; }
+__device__ __forceinline__ dc dadd(dc a, dc b) { return dc{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ dc dsub(dc a, dc b) { return dc{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ dc dmul(dc a, dc b) { return dc{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ dc dconj(dc a) { return dc{a.x, -a.y}; }
+__device__ __forceinline__ dc drmul(double s, dc a) { return dc{s * a.x, s * a.y}; }
+__device__ __forceinline__ double ddot(dc a, dc b) { return a.x * b.x + a.y * b.y; } // Re(conj(a) b)
+
+// strided per-thread column of a scratch array
+struct Dcol
+{
+    dc *p;
+    size_t s;
+    __device__ __forceinline__ dc &operator[](int i) const { return p[(size_t)i * s]; }
+};
+
+constexpr int kAdjThreads = 128;
+constexpr int kAdjBlocks = 256;
+constexpr int kArrays = 7;
+
+__global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const float *__restrict__ mom,
+                                                                      const uint32_t *__restrict__ index, int npix,
+                                                                      int m, float bias,
+                                                                      const float *__restrict__ grad_lag,
+                                                                      float *__restrict__ grad_mom,
+                                                                      dc *__restrict__ scratch)
+{
+    const size_t nthreads = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const Dcol E{scratch + tid, nthreads}, P{scratch + (size_t)m * nthreads + tid, nthreads},
+        R{scratch + (size_t)2 * m * nthreads + tid, nthreads}, U{scratch + (size_t)3 * m * nthreads + tid, nthreads},
+        Eb{scratch + (size_t)4 * m * nthreads + tid, nthreads}, Pb{scratch + (size_t)5 * m * nthreads + tid, nthreads},
+        Rb{scratch + (size_t)6 * m * nthreads + tid, nthreads};
+    const double b64 = (double)bias;
+
+    for (int pix = (int)tid; pix < npix; pix += (int)nthreads)
+    {
+        const int nm = index ? min((int)(index[pix + 1] - index[pix]), m) : m;
+        const size_t off = index ? (size_t)index[pix] : (size_t)pix * m;
+        if (nm <= 0)
+            continue;
+        const float *mo = mom + off;
+        auto Tm = [&](int i) { return (1.0 - b64) * (double)mo[i] + (i == 0 ? b64 * 0.5 : 0.0); };
+
+        // ---- forward (prepare_mixed_kernel's mathematics) ----
+        const double ph0 = kPi * Tm(0) - 0.5 * kPi;
+        double s0, c0;
+        sincos(ph0, &s0, &c0);
+        E[0] = dmk(c0 * (0.25 / kPi), s0 * (0.25 / kPi));
+        for (int l = 1; l < nm; ++l)
+        {
+            dc acc = dmk(0.0, 0.0);
+            for (int j = 0; j < l; ++j)
+                acc = dadd(acc, drmul((double)(l - j) * Tm(l - j), E[j]));
+            const double c = 2.0 * kPi / (double)l;
+            E[l] = dmk(-acc.y * c, acc.x * c);
+        }
+        E[0] = drmul(2.0, E[0]);
+        P[0] = dmk(1.0 / E[0].x, 0.0);
+        int j_invalid = nm; // first step whose u was forced to 0
+        for (int j = 1; j < nm; ++j)
+        {
+            dc u = dmk(0.0, 0.0);
+            if (j < j_invalid)
+                for (int k = 0; k < j; ++k)
+                    u = dadd(u, dmul(P[k], E[j - k]));
+            double a2 = u.x * u.x + u.y * u.y;
+            if (j < j_invalid && a2 >= 1.0)
+                j_invalid = j;
+            if (j >= j_invalid)
+            {
+                u = dmk(0.0, 0.0);
+                a2 = 0.0;
+            }
+            U[j] = u;
+            const double f = 1.0 / (1.0 - a2);
+            const double p0 = P[0].x;
+            for (int k = 1; 2 * k <= j; ++k)
+            {
+                const int q = j - k;
+                const dc a = P[k], b = P[q];
+                P[k] = drmul(f, dsub(a, dmul(dconj(b), u)));
+                if (q != k)
+                    P[q] = drmul(f, dsub(b, dmul(dconj(a), u)));
+            }
+            P[j] = drmul(-f * p0, u);
+            P[0] = dmk(f * p0, 0.0);
+        }
+        for (int i = 0; i < nm; ++i)
+            P[i] = drmul(2.0 * kPi, P[i]);
+        for (int k = 0; k < nm; ++k)
+        {
+            dc acc = dmk(0.0, 0.0);
+            for (int j = 0; j < nm - k; ++j)
+                acc = dadd(acc, dmul(P[j], dconj(P[j + k])));
+            R[k] = acc;
+        }
+        const double nf = 1.0 / (kPi * P[0].x);
+        auto Eh = [&](int j) { return j == 0 ? drmul(0.5, E[0]) : E[j]; }; // E with E_0 halved
+
+        // ---- reverse ----
+        for (int i = 0; i < nm; ++i)
+        {
+            Eb[i] = dmk(0.0, 0.0);
+            Pb[i] = dmk(0.0, 0.0);
+            Rb[i] = dmk(0.0, 0.0);
+        }
+        // lambda_k = nf C_k, C_k = sum_j R_{k+j}.x Eh_j.y + R_{k+j}.y Eh_j.x
+        double nfb = 0.0;
+        for (int k = 0; k < nm; ++k)
+        {
+            const double lb = (double)grad_lag[off + k];
+            double Ck = 0.0;
+            for (int j = 0; j < nm - k; ++j)
+            {
+                const dc r = R[k + j], e = Eh(j);
+                Ck += r.x * e.y + r.y * e.x;
+            }
+            nfb += lb * Ck;
+            const double cb = lb * nf;
+            if (cb != 0.0)
+                for (int j = 0; j < nm - k; ++j)
+                {
+                    const dc r = R[k + j], e = Eh(j);
+                    Rb[k + j] = dadd(Rb[k + j], dmk(cb * e.y, cb * e.x));
+                    // Eh_j.y += cb r.x, Eh_j.x += cb r.y; E_0 = 2 Eh_0 so dEh_0 = dE_0 / 2
+                    const double h = j == 0 ? 0.5 : 1.0;
+                    Eb[j] = dadd(Eb[j], dmk(h * cb * r.y, h * cb * r.x));
+                }
+        }
+        // nf = 1 / (pi P0.x)
+        Pb[0].x += -nfb * nf / P[0].x;
+        // R_k = sum_j P_j conj(P_{j+k})
+        for (int k = 0; k < nm; ++k)
+        {
+            const dc rb = Rb[k];
+            if (rb.x == 0.0 && rb.y == 0.0)
+                continue;
+            for (int j = 0; j < nm - k; ++j)
+            {
+                const dc a = P[j], b = P[j + k];
+                Pb[j] = dadd(Pb[j], dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
+                Pb[j + k] = dadd(Pb[j + k], dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
+            }
+        }
+        // P = 2 pi P_levinson
+        for (int i = 0; i < nm; ++i)
+        {
+            P[i] = drmul(1.0 / (2.0 * kPi), P[i]);
+            Pb[i] = drmul(2.0 * kPi, Pb[i]);
+        }
+        // Levinson, backwards
+        for (int j = nm - 1; j >= 1; --j)
+        {
+            const dc u = U[j];
+            const bool valid = j < j_invalid;
+            const double a2 = u.x * u.x + u.y * u.y;
+            const double f = 1.0 / (1.0 - a2);
+            // fb = <Pb', dP'/df> = <Pb', P'> / f over k = 0..j
+            double fb = 0.0;
+            for (int k = 0; k <= j; ++k)
+                fb += ddot(Pb[k], P[k]);
+            fb /= f;
+            const dc pjb = Pb[j];
+            const double p0b = Pb[0].x;
+            // previous iterate
+            const double p0 = P[0].x / f;
+            for (int k = 1; 2 * k <= j; ++k)
+            {
+                const int q = j - k;
+                const dc a1 = P[k], b1 = P[q];
+                P[k] = dadd(a1, dmul(u, dconj(b1)));
+                if (q != k)
+                    P[q] = dadd(b1, dmul(u, dconj(a1)));
+            }
+            P[0] = dmk(p0, 0.0);
+            P[j] = dmk(0.0, 0.0);
+            // ub = sum_k -f Pb'_k P_{j-k} - f p0 Pb'_j + 2 f^2 fb u
+            dc ub = drmul(2.0 * f * f * fb, u);
+            for (int k = 1; k < j; ++k)
+                ub = dsub(ub, drmul(f, dmul(Pb[k], P[j - k])));
+            ub = dsub(ub, drmul(f * p0, pjb));
+            // adjoints of the previous iterate
+            for (int k = 1; 2 * k <= j; ++k)
+            {
+                const int q = j - k;
+                const dc ak = Pb[k], bq = Pb[q];
+                Pb[k] = drmul(f, dsub(ak, dmul(dconj(bq), u)));
+                if (q != k)
+                    Pb[q] = drmul(f, dsub(bq, dmul(dconj(ak), u)));
+            }
+            Pb[0] = dmk(f * p0b - f * ddot(pjb, u), 0.0);
+            Pb[j] = dmk(0.0, 0.0);
+            if (valid)
+                for (int k = 0; k < j; ++k)
+                {
+                    // u = sum_k P_k E_{j-k}
+                    Pb[k] = dadd(Pb[k], dmul(ub, dconj(E[j - k])));
+                    Eb[j - k] = dadd(Eb[j - k], dmul(ub, dconj(P[k])));
+                }
+        }
+        // P_0 = 1 / E_0.x (E_0 doubled)
+        Eb[0].x += -Pb[0].x / (E[0].x * E[0].x);
+        // the recurrence used E_0 before doubling: dE0_undoubled = 2 dE0
+        Eb[0] = drmul(2.0, Eb[0]);
+        const dc e0 = drmul(0.5, E[0]);
+        // Herglotz recurrence, backwards: E_l = i c_l sum_j (l-j) T_{l-j} E_j;
+        // dL/dT_i is kept in Rb[i].x (Rb is no longer needed)
+        for (int i = 0; i < nm; ++i)
+            Rb[i] = dmk(0.0, 0.0);
+        for (int l = nm - 1; l >= 1; --l)
+        {
+            const double c = 2.0 * kPi / (double)l;
+            const dc accb = dmk(c * Eb[l].y, -c * Eb[l].x); // conj(i c) Eb_l
+            for (int j = 0; j < l; ++j)
+            {
+                const double w = (double)(l - j);
+                const dc ej = j == 0 ? e0 : E[j];
+                Eb[j] = dadd(Eb[j], drmul(w * Tm(l - j), accb));
+                Rb[l - j].x += w * ddot(accb, ej);
+            }
+        }
+        // E_0 = (cos(ph0), sin(ph0)) / (4 pi), ph0 = pi T_0 - pi / 2
+        Rb[0].x += 0.25 * (-Eb[0].x * s0 + Eb[0].y * c0);
+        for (int i = 0; i < nm; ++i)
+            grad_mom[off + i] = (float)((1.0 - b64) * Rb[i].x);
+    }
+}
+
+} // namespace
+
+size_t prepare_adjoint_scratch_bytes(int m) { return (size_t)kAdjBlocks * kAdjThreads * kArrays * m * sizeof(dc); }
+
+int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, int H, int m, float bias,
+                           const float *grad_lagrange, float *grad_moments, hipStream_t stream)
+{
+    void *scratch = nullptr;
+    if (hipMallocAsync(&scratch, prepare_adjoint_scratch_bytes(m), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    hipLaunchKernelGGL(prepare_adjoint_kernel, dim3(kAdjBlocks), dim3(kAdjThreads), 0, stream, moments, index, W * H,
+                       m, bias, grad_lagrange, grad_moments, static_cast<dc *>(scratch));
+    return hipFreeAsync(scratch, stream) == hipSuccess ? 0 : MRAY_ERR_HIP;
+}
+
+} // namespace mray

@@ -229,6 +229,22 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
     return check_status(rc, "mray_hip_reconstruct_backward");
 }
 
+int mray_hip_prepare_backward(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
+                              const float *d_grad_lagrange, float *d_grad_moments, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_prepare_backward: call mray_hip_init first");
+    if (!d_moments || !d_grad_lagrange || !d_grad_moments)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_backward: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_backward: bad image size %dx%d or moment count %d", W, H, m);
+    int rc = mray::launch_prepare_adjoint(d_moments, d_index, W, H, m, bias, d_grad_lagrange, d_grad_moments,
+                                          (hipStream_t)stream);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_prepare_backward: scratch allocation failed");
+    return check_status(rc, "mray_hip_prepare_backward");
+}
+
 int mray_hip_compute_num_moments(const float *d_moments, const uint32_t *d_bounds, int W, int H, int m,
                                  float error_threshold, uint32_t *d_counts, mray_stream stream)
 {

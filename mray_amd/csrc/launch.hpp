@@ -26,6 +26,9 @@ int launch_reconstruct_adjoint(const mray_camera &cam, const float *dmin, const 
                                const mray_rec_params &p, const float *rgba, const float *grad_rgba,
                                float *grad_lagrange, float *grad_tf, hipStream_t stream);
 
+int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, int H, int m, float bias,
+                           const float *grad_lagrange, float *grad_moments, hipStream_t stream);
+
 int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H, int m, float error_threshold,
                        uint32_t *counts, hipStream_t stream);
 int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);

@@ -32,6 +32,7 @@ _SIGNATURES = {
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
     "mray_hip_reconstruct_backward": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                        C.POINTER(RecParams), _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "mray_hip_prepare_backward": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp], C.c_int),
     "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
@@ -202,6 +203,16 @@ def reconstruct_backward(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, 
                                                         _ptr(g_lag), _ptr(g_tf), _stream(stream)),
            "mray_hip_reconstruct_backward")
     return g_lag, g_tf
+
+
+def prepare_backward(moments: torch.Tensor, W: int, H: int, m: int, bias: float, grad_lagrange: torch.Tensor,
+                     index=None, stream=None) -> torch.Tensor:
+    """dL/dmoments for a given dL/dlagrange of prepare (MIXED mathematics)."""
+    out = _like(moments)
+    _check(load_library().mray_hip_prepare_backward(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias),
+                                                    _ptr(grad_lagrange.contiguous()), _ptr(out), _stream(stream)),
+           "mray_hip_prepare_backward")
+    return out
 
 
 def compute_num_moments(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None,

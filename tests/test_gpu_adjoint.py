@@ -6,8 +6,12 @@ torch.autograd (the differentiation workload, scaled down).
 Bars: the GPU forward is FP32 with an FP64 series, the oracle is FP64
 throughout; the two forwards differ by ~1e-5 relative, so gradients must agree
 to 2e-3 normwise (table) and per pixel (multipliers, norm floored at 1% of the
-mean pixel norm), on rays whose step count is the same in both (early ray
-termination makes the image discontinuous in tau). Finite differences of the
+mean pixel norm). Rays where the two forwards take different discrete
+branches are compared with zero upstream gradient: another step count (early
+ray termination is discontinuous in tau) or images more than 1e-4 apart (the
+reference's lookup takes its table cell from the front sample, so a density
+within rounding of a cell edge can land in neighbouring cells in FP32 and
+FP64; the image jumps there). Finite differences of the
 FP32 forward with a relative step of 1e-3 (the configuration's eps) resolve a
 directional derivative to ~1e-4: bar 2e-3."""
 import numpy as np
@@ -60,33 +64,38 @@ def test_adjoint_matches_oracle(hip, name):
     dmin, dmax, cam, W, H, m, rp, lag, table, b = setup(O, name)
     rng = np.random.default_rng(5)
     g = rng.normal(size=(H, W, 4)).astype(np.float32)
-    gl_ref, gt_ref = O.reconstruct_adjoint_f64(cam, dmin, dmax, W, H, m, lag.reshape(-1), table, rp,
-                                               g.astype(np.float64), bounds=b)
-    _, st64 = O.reconstruct_lagrange_f64(cam, dmin, dmax, W, H, m, lag, table, rp, bounds=b)
+    rgba64, st64 = O.reconstruct_lagrange_f64(cam, dmin, dmax, W, H, m, lag, table, rp, bounds=b)
     lag_d = torch.from_numpy(lag).cuda()
     b_d = torch.from_numpy(b.view(np.int32)).cuda()
     tab_d = torch.from_numpy(table).cuda()
     print(f"[adjoint] case={name}", flush=True)
     rgba, st = hip.reconstruct(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, form="mixed", bounds=b_d,
                                want_steps=True)
+    torch.cuda.synchronize()
+    jump = np.abs(rgba.cpu().numpy() - rgba64).max(axis=-1).reshape(-1) > 1e-4
+    same = (st.cpu().numpy() == st64) & ~jump
+    g[~same.reshape(H, W)] = 0.0
+    gl_ref, gt_ref = O.reconstruct_adjoint_f64(cam, dmin, dmax, W, H, m, lag.reshape(-1), table, rp,
+                                               g.astype(np.float64), bounds=b)
     g_lag, g_tf = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, rgba,
                                            torch.from_numpy(g).cuda(), bounds=b_d)
     torch.cuda.synchronize()
-    same = st.cpu().numpy() == st64
     g_lag = g_lag.cpu().numpy().reshape(-1, m).astype(np.float64)
     gl_ref = gl_ref.reshape(-1, m)
     norms = np.linalg.norm(gl_ref, axis=1)
-    floor = 0.01 * norms.mean()
+    floor = 0.01 * norms[same].mean()
     rel_lag = np.linalg.norm(g_lag - gl_ref, axis=1) / np.maximum(norms, floor)
     g_tf = g_tf.cpu().numpy().astype(np.float64)
-    # the table gradient sums over all rays: take the rays with another step count out of both
     rel_tf = np.linalg.norm(g_tf - gt_ref) / np.linalg.norm(gt_ref)
-    print(f"  rays with other step count {np.count_nonzero(~same)}/{W * H}; multipliers max rel "
-          f"{rel_lag[same].max():.2e}; table rel {rel_tf:.2e}")
-    assert np.count_nonzero(~same) <= max(2, 0.01 * W * H)
+    print(f"  rays on other branches: step count {np.count_nonzero(st.cpu().numpy() != st64)}, image jump "
+          f"{np.count_nonzero(jump)} of {W * H}; multipliers max rel {rel_lag[same].max():.2e}; "
+          f"table rel {rel_tf:.2e}")
+    dif = np.abs(g_tf - gt_ref).reshape(-1)
+    for i in np.argsort(dif)[::-1][:3]:
+        print(f"    entry {i // 4} ch {i % 4}: gpu {g_tf.reshape(-1)[i]:.6e} fp64 {gt_ref.reshape(-1)[i]:.6e}")
+    assert np.count_nonzero(~same) <= max(2, 0.02 * W * H)
     assert rel_lag[same].max() <= 2e-3
-    if same.all():
-        assert rel_tf <= 2e-3
+    assert rel_tf <= 2e-3
 
 
 def test_adjoint_finite_differences_on_gpu(hip):
@@ -109,13 +118,22 @@ def test_adjoint_finite_differences_on_gpu(hip):
     rgba, _ = hip.reconstruct(cam, dmin, dmax, lag_d, W, H, m, tab, rp, form="mixed", bounds=b_d)
     _, g_tf = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab, rp, rgba, g, bounds=b_d,
                                        want_lagrange=False)
-    for trial in range(3):
-        d = torch.from_numpy(rng.normal(size=(256, 4)).astype(np.float32)).cuda() * tab.abs().clamp_min(1e-4)
+    # rounding noise of the FP32 forward in L: ~16 ulps per output value
+    # (about a hundred rounded compositing steps), independent between the
+    # two evaluations; divided by 2 eps it bounds the noise of the difference
+    r = rgba.cpu().numpy().astype(np.float64)
+    gn = g.cpu().numpy().astype(np.float64)
+    noise = 16.0 * np.sqrt(2.0 * ((np.spacing(np.abs(r).astype(np.float32)) * gn) ** 2).sum())
+    dirs = [g_tf * tab.abs().clamp_min(1e-4) ** 2 / (g_tf * tab.abs().clamp_min(1e-4)).abs().max()]
+    dirs += [torch.from_numpy(rng.normal(size=(256, 4)).astype(np.float32)).cuda() * tab.abs().clamp_min(1e-4)
+             for _ in range(2)]
+    for d in dirs:
         eps = 1e-3
         fd = (loss(tab + eps * d) - loss(tab - eps * d)) / (2 * eps)
         an = float((g_tf.double() * d.double()).sum())
-        print(f"[adjoint fd] table: fd {fd:.6e} adjoint {an:.6e}")
-        assert abs(fd - an) <= 2e-3 * abs(an) + 1e-6
+        bar = 2e-3 * abs(an) + noise / (2 * eps)
+        print(f"[adjoint fd] table: fd {fd:.6e} adjoint {an:.6e} bar {bar:.1e}")
+        assert abs(fd - an) <= bar
 
 
 def test_transfer_function_fitting(hip):
@@ -156,3 +174,81 @@ def test_transfer_function_fitting(hip):
         losses.append(float(loss))
     print(f"[fit] loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     assert losses[-1] <= 0.1 * losses[0]
+
+
+@pytest.mark.parametrize("name", ["ml_ortho_m40", "mixw_persp_m30", "ml_persp_m16"])
+def test_prepare_adjoint_matches_fp64_finite_differences(hip, name):
+    """dL/dmoments of prepare against central differences of the FP64 oracle
+    preparation (relative step 1e-6 of the moment scale, FP64 throughout):
+    per pixel L = sum_k gbar_k lambda_k, one random direction per pixel.
+    Bar: 1e-5 relative to the sum of the magnitudes of the terms (the
+    conditioning of the Levinson recursion amplifies FP64 rounding)."""
+    import torch
+
+    O = Oracle()
+    dmin, dmax, cam, W, H, m, rp, lag, table, b = setup(O, name)
+    f, _, _, _, _, _, _, step, _, _ = CASES[name]
+    gp = GenParams(step_size=step, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    mo, _, _ = O.generate(f(), dmin, dmax, cam, W, H, m, gp)
+    mo = mo.reshape(-1, m)
+    rng = np.random.default_rng(21)
+    gbar = rng.normal(size=(W * H, m)).astype(np.float32)
+    gm = hip.prepare_backward(torch.from_numpy(np.ascontiguousarray(mo)).cuda(), W, H, m, rp.bias,
+                              torch.from_numpy(gbar).cuda()).cpu().numpy().astype(np.float64)
+    worst = 0.0
+    for p in range(0, W * H, max(1, W * H // 200)):
+        x = mo[p].astype(np.float64)
+        d = rng.normal(size=m) * 0.1
+        eps = 1e-6
+        lp = O.prepare_std_f64(m, x + eps * d, rp.bias)
+        lm = O.prepare_std_f64(m, x - eps * d, rp.bias)
+        fd = float(gbar[p].astype(np.float64) @ (lp - lm)) / (2 * eps)
+        an = float(gm[p] @ d)
+        scale = float(np.abs(gm[p] * d).sum()) + 1e-12
+        worst = max(worst, abs(fd - an) / scale)
+    print(f"[prepare adjoint] case={name}: worst |fd - adjoint| / sum|terms| {worst:.2e}")
+    assert worst <= 1e-5
+
+
+def test_pipeline_adjoint_reconstruct_prepare(hip):
+    """dL/dmoments through the march and the preparation (GPU adjoints
+    chained) against central differences of the FP64 pipeline (oracle
+    preparation in FP64, FP64 march) along random directions, without early
+    termination. Bar: 2e-3 relative (the GPU forward is FP32 with an FP64
+    series; its image differs from FP64 by ~1e-5 relative)."""
+    import torch
+
+    O = Oracle()
+    dmin, dmax, cam, W, H, m, rp, lag, table, b = setup(O, "ml_persp_m16", ert=1.5)
+    f, _, _, _, _, _, _, step, _, _ = CASES["ml_persp_m16"]
+    gp = GenParams(step_size=step, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    mo, _, _ = O.generate(f(), dmin, dmax, cam, W, H, m, gp)
+    rng = np.random.default_rng(31)
+    g = rng.normal(size=(H, W, 4)).astype(np.float32)
+    mo_d = torch.from_numpy(np.ascontiguousarray(mo)).cuda()
+    b_d = torch.from_numpy(b.view(np.int32)).cuda()
+    tab_d = torch.from_numpy(table).cuda()
+    lag_d = hip.prepare(mo_d, W, H, m, rp.bias, form="mixed")
+    rgba, _ = hip.reconstruct(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, form="mixed", bounds=b_d)
+    g_lag, _ = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, rgba,
+                                        torch.from_numpy(g).cuda(), bounds=b_d, want_tf=False)
+    g_mo = hip.prepare_backward(mo_d, W, H, m, rp.bias, g_lag).cpu().numpy().astype(np.float64).reshape(-1)
+
+    mo64 = mo.astype(np.float64).reshape(-1, m)
+    tf64 = table.astype(np.float64)
+
+    def loss(x):
+        lam = np.stack([O.prepare_std_f64(m, x[p], rp.bias) for p in range(W * H)])
+        return float((g.astype(np.float64) * O.reconstruct_lagrange64_f64(cam, dmin, dmax, W, H, m,
+                                                                          lam.reshape(-1), tf64, rp,
+                                                                          bounds=b)).sum())
+
+    for trial in range(2):
+        d = rng.normal(size=mo64.shape) * 0.01
+        eps = 1e-6
+        fd = (loss(mo64 + eps * d) - loss(mo64 - eps * d)) / (2 * eps)
+        an = float(g_mo @ d.reshape(-1))
+        print(f"[pipeline adjoint] fd {fd:.6e} adjoint {an:.6e}")
+        assert abs(fd - an) <= 2e-3 * abs(fd) + 1e-6
