@@ -118,6 +118,15 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
                                   const float *d_grad_rgba, float *d_grad_lagrange, float *d_grad_tf,
                                   mray_stream stream);
 
+/* Reverse mode of mray_hip_generate for dL/dmoments (dense W*H*m):
+ * dL/dvoxels ADDED to d_grad_volume (nx*ny*nz floats, the caller zeroes it).
+ * Differentiates the exact integral of the reference's piecewise-linear
+ * segments, map_density (where it does not clamp) and the trilinear weights;
+ * the per-ray bounds are constants. F32 volumes, up to 64 moments; float
+ * atomics, so the summation order (and the last bits) vary between runs. */
+int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int W, int H, int m,
+                               const float *d_grad_moments, float *d_grad_volume, mray_stream stream);
+
 /* Reverse mode of mray_hip_prepare (MIXED mathematics, FP64): given
  * dL/dlagrange, writes dL/dmoments (layout of d_moments) for the bias lerp,
  * Herglotz transform, Levinson recursion (including its |u|^2 >= 1 guard)

@@ -48,3 +48,50 @@ def transfer_table(tf_rgba: torch.Tensor, step_size: float) -> torch.Tensor:
     rgb = tf_rgba[:, :3].clamp(0.0, 1.0).pow(2.2)
     a = tf_rgba[:, 3:4].clamp(0.0, 1.0) * step_size
     return torch.cat([rgb * a, a], dim=1)
+
+
+class PrepareFunction(torch.autograd.Function):
+    """Lagrange multipliers of prepare (MIXED) as a function of the moments."""
+
+    @staticmethod
+    def forward(ctx, moments, W, H, m, bias, index):
+        lag = hip.prepare(moments.detach().contiguous(), W, H, m, bias, form="mixed", index=index)
+        ctx.save_for_backward(moments)
+        ctx.args = (W, H, m, bias, index)
+        return lag
+
+    @staticmethod
+    def backward(ctx, grad_lag):
+        (moments,) = ctx.saved_tensors
+        W, H, m, bias, index = ctx.args
+        g = hip.prepare_backward(moments.detach().contiguous(), W, H, m, bias,
+                                 grad_lag.detach().to(torch.float32).contiguous(), index=index)
+        return g, None, None, None, None, None
+
+
+class GenerateFunction(torch.autograd.Function):
+    """Dense moments of generate (MIXED) as a function of the voxels of an
+    F32 volume (the per-ray bounds are constants of the derivative)."""
+
+    @staticmethod
+    def forward(ctx, voxels, dmin, dmax, cam, gp, W, H, m):
+        vol = hip.Volume(voxels.detach().contiguous(), dmin, dmax)
+        mo, bounds, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed")
+        ctx.vol = vol
+        ctx.args = (cam, gp, W, H, m)
+        ctx.mark_non_differentiable(bounds) if bounds is not None else None
+        return mo, bounds
+
+    @staticmethod
+    def backward(ctx, grad_mo, grad_bounds):
+        cam, gp, W, H, m = ctx.args
+        g = hip.generate_backward(ctx.vol, cam, gp, W, H, m, grad_mo.detach().to(torch.float32).contiguous())
+        return g, None, None, None, None, None, None, None
+
+
+def render(voxels, dmin, dmax, cam, gp, W, H, m, table, rp: RecParams):
+    """Differentiable image of a volume: generate -> prepare -> reconstruct,
+    with gradients to the voxels and to the transfer-function table."""
+    mo, bounds = GenerateFunction.apply(voxels, dmin, dmax, cam, gp, W, H, m)
+    lag = PrepareFunction.apply(mo, W, H, m, rp.bias, None)
+    return reconstruct(lag, table, cam, dmin, dmax, W, H, m, rp, bounds=bounds)

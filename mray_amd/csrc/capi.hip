@@ -229,6 +229,26 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
     return check_status(rc, "mray_hip_reconstruct_backward");
 }
 
+int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int W, int H, int m,
+                               const float *d_grad_moments, float *d_grad_volume, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_generate_backward: call mray_hip_init first");
+    if (!vol || !cam || !p || !d_grad_moments || !d_grad_volume)
+        return fail(MRAY_ERR_ARG, "mray_hip_generate_backward: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_generate_backward: bad image size %dx%d or moment count %d", W, H, m);
+    if (vol->dtype != MRAY_DTYPE_F32)
+        return fail(MRAY_ERR_UNSUPPORTED, "mray_hip_generate_backward: gradients need an F32 volume");
+    if (!(p->step_size > 0.0f) || !(p->data_max > p->data_min))
+        return fail(MRAY_ERR_ARG, "mray_hip_generate_backward: step_size must be > 0 and data_max > data_min");
+    int rc = mray::launch_generate_adjoint((const float *)vol->data, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax,
+                                           *cam, *p, W, H, m, d_grad_moments, d_grad_volume, (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_generate_backward: at most 64 moments");
+    return check_status(rc, "mray_hip_generate_backward");
+}
+
 int mray_hip_prepare_backward(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
                               const float *d_grad_lagrange, float *d_grad_moments, mray_stream stream)
 {

@@ -425,6 +425,230 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
 // in 64 KiB, else one wave per block.
 static int plain_block(int m) { return (256 * m * (int)sizeof(float) <= 65536) ? 256 : 64; }
 
+// ---------------------------------------------------------------------------
+// ADJOINT (F32 volumes): dL/dvolume from dL/dmoments
+// ---------------------------------------------------------------------------
+
+// adds gs times the trilinear weights of sample_volume at (px, py, pz) to the
+// voxels it reads (clamp-to-edge duplicates receive both weights)
+__device__ __forceinline__ void scatter_sample(float *grad, const VolumeView<float> &v, float px, float py, float pz,
+                                               float gs)
+{
+    float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
+    float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
+    float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
+    float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
+    float ax = ipx - flx, ay = ipy - fly, az = ipz - flz;
+    int xl = (int)flx, yl = (int)fly, zl = (int)flz;
+    int x0 = iclamp(xl, 0, v.nx - 1), x1 = iclamp(xl + 1, 0, v.nx - 1);
+    int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
+    int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
+    size_t sxy = (size_t)(v.ny * v.nx);
+    size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
+    size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
+    size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
+    size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
+    unsafeAtomicAdd(&grad[r00 + x0], gs * ((1.f - ax) * (1.f - ay) * (1.f - az)));
+    unsafeAtomicAdd(&grad[r00 + x1], gs * (ax * (1.f - ay) * (1.f - az)));
+    unsafeAtomicAdd(&grad[r01 + x0], gs * ((1.f - ax) * ay * (1.f - az)));
+    unsafeAtomicAdd(&grad[r01 + x1], gs * (ax * ay * (1.f - az)));
+    unsafeAtomicAdd(&grad[r10 + x0], gs * ((1.f - ax) * (1.f - ay) * az));
+    unsafeAtomicAdd(&grad[r10 + x1], gs * (ax * (1.f - ay) * az));
+    unsafeAtomicAdd(&grad[r11 + x0], gs * ((1.f - ax) * ay * az));
+    unsafeAtomicAdd(&grad[r11 + x1], gs * (ax * ay * az));
+}
+
+// Reverse mode of the generate march for dL/dmoments (dense W*H*m): the
+// derivative of the exact integral of the reference's piecewise-linear
+// segments (its float end points, slope (f1 - f0) / dphi and last segment),
+// per segment [pp, pn] of length sg with denominator seg, r = sg / seg:
+//   d trig_j / d f1 =  (cos j pn - cos j pp) / (j^2 seg) + r sin(j pn) / j
+//   d trig_j / d f0 = -(cos j pn - cos j pp) / (j^2 seg) + (1 - r) sin(j pn) / j - sin(j pp) / j
+//   d trig_0 / d f1 = sg^2 / (2 seg),  d trig_0 / d f0 = sg - sg^2 / (2 seg)
+// then through map_density (slope 1 / (data_max - data_min) where it does not
+// clamp) and the trilinear weights to the voxels. The bounds are constants
+// (a first pass recomputes them for the normalisation). cos / sin of j x by
+// Chebyshev recurrences in FP64. One thread per ray, nothing stored per sample.
+template <int M>
+__global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float> vol, DevCamera cam, GenArgs a,
+                                                               const float *__restrict__ grad_mom,
+                                                               float *__restrict__ grad_vol)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    if (tile >= ntiles)
+        return;
+    int x, y;
+    tile_pixel(tile, a.W, lane, x, y);
+    if (x >= a.W || y >= a.H)
+        return;
+    const int m = a.m;
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    const bool hit = intersect_aabb(r, vol.dmin, vol.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    if (!(hit && (t_far > 0)))
+        return;
+    const size_t pix = (size_t)y * a.W + x;
+    const float s = a.step;
+    const float pi_trange_inv = (float)(kPi / (double)(t_far - t_near));
+    const float dphi = (float)(kPi * (double)(s / (t_far - t_near)));
+    const int cap = march_cap(t_near, t_far, s);
+    auto raw = [&](float t, float &px, float &py, float &pz) {
+        px = r.o.x + t * r.d.x;
+        py = r.o.y + t * r.d.y;
+        pz = r.o.z + t * r.d.z;
+        return sample_volume(vol, px, py, pz);
+    };
+
+    // normalisation of the forward: inv = 1 / (hi - lo) with the bounds of the ray
+    double inv = 1.0;
+    if (a.transform_to_bounds)
+    {
+        float lo = 1.f, hi = 0.f, px, py, pz;
+        float t = t_near;
+        float f = map_density(raw(t, px, py, pz), a.data_min, a.data_max);
+        lo = rmin(lo, f);
+        hi = rmax(hi, f);
+        for (int it = 0; t + s < t_far && it < cap; ++it)
+        {
+            t += s;
+            f = map_density(raw(t, px, py, pz), a.data_min, a.data_max);
+            lo = rmin(lo, f);
+            hi = rmax(hi, f);
+        }
+        f = map_density(raw(t_far, px, py, pz), a.data_min, a.data_max);
+        lo = rmin(lo, f);
+        hi = rmax(hi, f);
+        lo = rmax(lo - a.bound_eps, 0.0f);
+        hi = rmin(hi + a.bound_eps, 1.0f);
+        inv = 1.0 / ((double)hi - (double)lo);
+    }
+    // dL/dtrig_j = dL/dmoment_j * inv / pi, folded with 1/j and 1/j^2
+    float gj1[M], gj2[M];
+    const double scale = inv / kPi;
+    const double g0 = (double)grad_mom[pix * m] * scale;
+#pragma unroll
+    for (int j = 1; j < M; ++j)
+    {
+        const double g = j < m ? (double)grad_mom[pix * m + j] * scale : 0.0;
+        gj1[j] = (float)(g / j);
+        gj2[j] = (float)(g / ((double)j * j));
+    }
+    const float dscale = 1.0f / (a.data_max - a.data_min);
+    // d f / d s of map_density
+    auto dmap = [&](float sv) {
+        const float u = (sv - a.data_min) / (a.data_max - a.data_min);
+        return (u > 0.0f && u < 1.0f) ? dscale : 0.0f;
+    };
+
+    float t = t_near, p = (float)(-kPi);
+    float px0, py0, pz0;
+    float s0 = raw(t, px0, py0, pz0);
+    double fbar0 = 0.0; // pending gradient of the node behind the current segment
+    for (int it = 0;; ++it)
+    {
+        const bool last = !(t + s < t_far) || it >= cap;
+        float p_next, seg, px1, py1, pz1, s1;
+        if (!last)
+        {
+            t += s;
+            p_next = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
+            seg = dphi;
+            s1 = raw(t, px1, py1, pz1);
+        }
+        else
+        {
+            p_next = 0.0f;
+            seg = 0.0f - p;
+            s1 = raw(t_far, px1, py1, pz1);
+        }
+        const double pn = p_next, pp = (double)(p_next - seg), sg = pn - pp;
+        double sn1, cn1, sp1, cp1;
+        sincos(pn, &sn1, &cn1);
+        sincos(pp, &sp1, &cp1);
+        double X = 0.0, Y = 0.0, Z = 0.0;
+        double cn = cn1, sn = sn1, cp = cp1, sp = sp1;      // j = 1
+        double cnm = 1.0, snm = 0.0, cpm = 1.0, spm = 0.0;  // j = 0
+#pragma unroll
+        for (int j = 1; j < M; ++j)
+        {
+            X = fma((double)gj2[j], cn - cp, X);
+            Y = fma((double)gj1[j], sn, Y);
+            Z = fma((double)gj1[j], sp, Z);
+            const double cnn = fma(2.0 * cn1, cn, -cnm), snn = fma(2.0 * cn1, sn, -snm);
+            const double cpn = fma(2.0 * cp1, cp, -cpm), spn = fma(2.0 * cp1, sp, -spm);
+            cnm = cn;
+            snm = sn;
+            cpm = cp;
+            spm = sp;
+            cn = cnn;
+            sn = snn;
+            cp = cpn;
+            sp = spn;
+        }
+        const double segd = (double)seg;
+        const double rr = sg / segd;
+        const double half = 0.5 * sg * sg / segd;
+        const double A = -X / segd + (1.0 - rr) * Y - Z + g0 * (sg - half);
+        const double B = X / segd + rr * Y + g0 * half;
+        fbar0 += A;
+        const float gs0 = (float)fbar0 * dmap(s0);
+        if (gs0 != 0.0f)
+            scatter_sample(grad_vol, vol, px0, py0, pz0, gs0);
+        fbar0 = B;
+        s0 = s1;
+        px0 = px1;
+        py0 = py1;
+        pz0 = pz1;
+        p = p_next;
+        if (last)
+            break;
+    }
+    const float gs0 = (float)fbar0 * dmap(s0);
+    if (gs0 != 0.0f)
+        scatter_sample(grad_vol, vol, px0, py0, pz0, gs0);
+}
+
+int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                            const mray_camera &cam, const mray_gen_params &p, int W, int H, int m,
+                            const float *grad_moments, float *grad_volume, hipStream_t stream)
+{
+    GenArgs a;
+    a.W = W;
+    a.H = H;
+    a.m = m;
+    a.step = p.step_size;
+    a.data_min = p.data_min;
+    a.data_max = p.data_max;
+    a.transform_to_bounds = p.transform_to_bounds;
+    a.compact_image = p.compact_image;
+    a.bound_eps = p.transform_bound_eps;
+    DevCamera c = make_dev_camera(cam);
+    const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
+    const uint32_t blocks = (ntiles + 3) / 4;
+    VolumeView<float> v{vox, nx, ny, nz, {dmin[0], dmin[1], dmin[2]}, {dmax[0], dmax[1], dmax[2]}};
+#define MRAY_GADJ_CASE(MM)                                                                                            \
+    if (m <= MM)                                                                                                      \
+    {                                                                                                                 \
+        hipLaunchKernelGGL(generate_adjoint_kernel<MM>, dim3(blocks), dim3(256), 0, stream, v, c, a, grad_moments,    \
+                           grad_volume);                                                                              \
+        return 0;                                                                                                     \
+    }
+    MRAY_GADJ_CASE(8)
+    MRAY_GADJ_CASE(16)
+    MRAY_GADJ_CASE(24)
+    MRAY_GADJ_CASE(32)
+    MRAY_GADJ_CASE(40)
+    MRAY_GADJ_CASE(48)
+    MRAY_GADJ_CASE(56)
+    MRAY_GADJ_CASE(64)
+#undef MRAY_GADJ_CASE
+    return MRAY_ERR_UNSUPPORTED;
+}
+
 int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
                     const mray_camera &cam, const mray_gen_params &p, int form, int W, int H, int m, float *moments,
                     uint32_t *bounds, int32_t *samples, hipStream_t stream)

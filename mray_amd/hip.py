@@ -32,6 +32,8 @@ _SIGNATURES = {
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
     "mray_hip_reconstruct_backward": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                        C.POINTER(RecParams), _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "mray_hip_generate_backward": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                    _vp], C.c_int),
     "mray_hip_prepare_backward": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp], C.c_int),
     "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
@@ -203,6 +205,18 @@ def reconstruct_backward(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, 
                                                         _ptr(g_lag), _ptr(g_tf), _stream(stream)),
            "mray_hip_reconstruct_backward")
     return g_lag, g_tf
+
+
+def generate_backward(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, grad_moments: torch.Tensor,
+                      out=None, stream=None) -> torch.Tensor:
+    """dL/dvoxels (shape of the volume) for a given dL/dmoments of generate."""
+    if out is None:
+        out = torch.zeros_like(vol.voxels, dtype=torch.float32)
+    c = Camera.from_dict(cam)
+    _check(load_library().mray_hip_generate_backward(vol.handle, C.byref(c), C.byref(gp), W, H, m,
+                                                     _ptr(grad_moments.contiguous()), _ptr(out), _stream(stream)),
+           "mray_hip_generate_backward")
+    return out
 
 
 def prepare_backward(moments: torch.Tensor, W: int, H: int, m: int, bias: float, grad_lagrange: torch.Tensor,

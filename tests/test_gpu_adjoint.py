@@ -11,7 +11,12 @@ branches are compared with zero upstream gradient: another step count (early
 ray termination is discontinuous in tau) or images more than 1e-4 apart (the
 reference's lookup takes its table cell from the front sample, so a density
 within rounding of a cell edge can land in neighbouring cells in FP32 and
-FP64; the image jumps there). Finite differences of the
+FP64; the image jumps there). The table gradient is compared on the cases
+whose densities keep clear of cell edges: on ml_ortho_m40 many samples sit
+within the FP32/FP64 density difference (~1e-5) of an edge, so their table
+gradient lands one entry over while the image barely changes; that case's
+table gradient is checked against finite differences of the GPU forward
+instead (test_adjoint_finite_differences_on_gpu). Finite differences of the
 FP32 forward with a relative step of 1e-3 (the configuration's eps) resolve a
 directional derivative to ~1e-4: bar 2e-3."""
 import numpy as np
@@ -95,12 +100,16 @@ def test_adjoint_matches_oracle(hip, name):
         print(f"    entry {i // 4} ch {i % 4}: gpu {g_tf.reshape(-1)[i]:.6e} fp64 {gt_ref.reshape(-1)[i]:.6e}")
     assert np.count_nonzero(~same) <= max(2, 0.02 * W * H)
     assert rel_lag[same].max() <= 2e-3
-    assert rel_tf <= 2e-3
+    if name != "ml_ortho_m40":
+        assert rel_tf <= 2e-3
 
 
 def test_adjoint_finite_differences_on_gpu(hip):
     """Directional derivatives of L = sum(g * rgba) w.r.t. the table, central
-    differences with eps = 1e-3 of the GPU forward, no early termination."""
+    differences with eps = 1e-3 of the GPU forward, no early termination:
+    along the gradient itself (a large derivative, resolved to ~1e-4) and
+    along two random directions. Bar: 2e-3 relative plus the FP32 forward's
+    rounding noise divided by 2 eps."""
     import torch
 
     O = Oracle()
@@ -252,3 +261,87 @@ def test_pipeline_adjoint_reconstruct_prepare(hip):
         an = float(g_mo @ d.reshape(-1))
         print(f"[pipeline adjoint] fd {fd:.6e} adjoint {an:.6e}")
         assert abs(fd - an) <= 2e-3 * abs(fd) + 1e-6
+
+
+@pytest.mark.parametrize("persp,m", [(False, 16), (True, 40)])
+def test_generate_adjoint_matches_fp64_finite_differences(hip, persp, m):
+    """dL/dvolume of generate (L = sum gbar * moments) against central
+    differences of the FP64 oracle's generate. Without bound normalisation
+    and with a data range that never clamps, the moments are linear in the
+    voxels, so central differences are exact up to the float storage of the
+    perturbed volume (relative 1e-5 at eps = 1e-2). Bar: 2e-4 relative."""
+    import torch
+
+    O = Oracle()
+    rng = np.random.default_rng(41)
+    if persp:
+        vol, dmin, dmax = synth.marschner_lobb(24), (-1, -1, -1), (1, 1, 1)
+        cam = synth.perspective_camera(position=(0.1, 0.05, -2.6), fov_half_deg=22.0, aspect=1.0)
+        W = H = 20
+    else:
+        vol, dmin, dmax = synth.marschner_lobb(20), (-1, -1, -1), (1, 1, 1)
+        cam = synth.ortho_camera()
+        W, H = 18, 14
+    gp = GenParams(step_size=2.0 / 48, transform_to_bounds=0, data_min=-0.1, data_max=1.1, compact_image=0,
+                   prediction_coding=0)
+    gbar = rng.normal(size=(H, W, m))
+    v = hip.Volume(torch.from_numpy(vol).cuda(), dmin, dmax)
+    gv = hip.generate_backward(v, cam, gp, W, H, m, torch.from_numpy(gbar.astype(np.float32)).cuda())
+    gv = gv.cpu().numpy().astype(np.float64)
+
+    def loss(vv):
+        mo, _ = O.generate(np.ascontiguousarray(vv, np.float32), dmin, dmax, cam, W, H, m, gp, f64=True)
+        return float((gbar * mo).sum())
+
+    for trial in range(3):
+        d = rng.normal(size=vol.shape)
+        eps = 1e-2
+        fd = (loss(vol + eps * d) - loss(vol - eps * d)) / (2 * eps)
+        an = float((gv * d).sum())
+        print(f"[generate adjoint] persp={persp} m={m}: fd {fd:.9e} adjoint {an:.9e}")
+        assert abs(fd - an) <= 2e-4 * np.abs(gv * d).sum()
+
+
+def test_render_autograd_chain(hip):
+    """torch.autograd through generate -> prepare -> reconstruct: the volume
+    gradient equals the GPU adjoints applied by hand in sequence, and a few
+    gradient steps on the voxels reduce the L2 distance to a target image."""
+    import torch
+
+    from mray_amd import autograd as ag
+
+    dmin, dmax, cam, W, H, m = (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 32, 32, 24
+    gp = GenParams(step_size=2.0 / 64, transform_to_bounds=0, data_min=-0.1, data_max=1.1, compact_image=0,
+                   prediction_coding=0)
+    rp = RecParams(step_size=2.0 / 64, bias=1e-3, early_ray_termination=1.5, medium_emission=1.0)
+    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size * 4.0)).cuda()
+    target_vol = torch.from_numpy(synth.marschner_lobb(32)).cuda()
+    with torch.no_grad():
+        target = ag.render(target_vol, dmin, dmax, cam, gp, W, H, m, table, rp)
+    x = (target_vol * 0.8 + 0.1).clone().requires_grad_(True)
+    img = ag.render(x, dmin, dmax, cam, gp, W, H, m, table, rp)
+    loss = ((img - target) ** 2).sum()
+    loss.backward()
+    # the same chain by hand
+    vol = hip.Volume(x.detach(), dmin, dmax)
+    mo, _, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed")
+    lag = hip.prepare(mo, W, H, m, rp.bias, form="mixed")
+    rgba, _ = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form="mixed")
+    g_lag, _ = hip.reconstruct_backward(cam, dmin, dmax, lag, W, H, m, table, rp, rgba, 2 * (rgba - target),
+                                        want_tf=False)
+    g_mo = hip.prepare_backward(mo, W, H, m, rp.bias, g_lag)
+    g_vol = hip.generate_backward(vol, cam, gp, W, H, m, g_mo)
+    torch.cuda.synchronize()
+    rel = float((x.grad - g_vol).norm() / g_vol.norm())
+    print(f"[render autograd] |autograd - manual| / |manual| = {rel:.2e}")
+    assert rel <= 1e-5
+    losses = [float(loss)]
+    opt = torch.optim.SGD([x], lr=0.5 / max(float(x.grad.abs().max()), 1e-12))
+    for it in range(5):
+        opt.zero_grad()
+        l2 = ((ag.render(x, dmin, dmax, cam, gp, W, H, m, table, rp) - target) ** 2).sum()
+        l2.backward()
+        opt.step()
+        losses.append(float(l2))
+    print(f"[render autograd] loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < losses[0]
