@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-row-step", type=int, default=64, help="CPU baseline: every k-th row")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -184,6 +185,8 @@ def main():
     achieved = f_dom / (ms_dom * 1e-3)
     total_flop = f_gen + f_rec + f_prep
 
+    adjoint = None if args.no_adjoint else adjoint_ratio(args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step, tf_factor)
@@ -215,9 +218,71 @@ def main():
                      "traffic": measured_traffic(dom, n, W, m)},
         "roofline_frame": {"achieved_TFLOPs": round(total_flop / (ms_step * 1e-3) / 1e12, 3),
                            "frac": round(total_flop / (ms_step * 1e-3) / PEAK_FP32_VALU, 4)},
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def adjoint_ratio(steps):
+    """Adjoint / forward time on the differentiation configuration: Marschner-Lobb
+    256^3 f32, orthographic 512x512 rays, 40 moments, step 0.5 voxel. Forward =
+    generate + prepare + reconstruct (mixed); adjoint = the reverse of all three
+    (gradient to the voxels and to the 256 x 4 table); the table-only adjoint
+    (what transfer-function fitting needs) is the reconstruct reverse alone."""
+    import torch
+
+    from mray_amd import hip, synth, tf as tfmod
+    from mray_amd.types import GenParams, RecParams
+
+    n, W, H, m = 256, 512, 512, 40
+    dmin, dmax = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    cam = synth.ortho_camera()
+    step = 1.0 / n
+    gp = GenParams(step_size=step, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    rp = RecParams(step_size=step, bias=5e-5, early_ray_termination=0.99, medium_emission=1.0)
+    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), step * 4.0)).cuda()
+    print("[bench] adjoint: Marschner-Lobb 256^3, 512x512, 40 moments", file=sys.stderr, flush=True)
+    vox = hip.synth_marschner_lobb(n)
+    vol = hip.Volume(vox, dmin, dmax)
+    g = torch.randn((H, W, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
+    gvol = torch.zeros_like(vox)
+
+    def fwd():
+        mo, b, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed")
+        lag = hip.prepare(mo, W, H, m, rp.bias, form="mixed")
+        rgba, _ = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form="mixed", bounds=b)
+        return mo, b, lag, rgba
+
+    def adj(mo, b, lag, rgba, tf_only):
+        g_lag, g_tf = hip.reconstruct_backward(cam, dmin, dmax, lag, W, H, m, table, rp, rgba, g, bounds=b,
+                                               want_lagrange=not tf_only)
+        if tf_only:
+            return
+        g_mo = hip.prepare_backward(mo, W, H, m, rp.bias, g_lag)
+        gvol.zero_()
+        hip.generate_backward(vol, cam, gp, W, H, m, g_mo, out=gvol)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    state = fwd()
+    adj(*state, False)
+    adj(*state, True)
+    k = max(2, steps)
+    ms_fwd = timed(fwd, k)
+    ms_adj = timed(lambda: adj(*state, False), k)
+    ms_tf = timed(lambda: adj(*state, True), k)
+    return {"workload": "Marschner-Lobb 256^3 f32, ortho 512x512, 40 moments, step 0.5 voxel",
+            "forward_ms": round(ms_fwd, 3), "adjoint_ms": round(ms_adj, 3), "adjoint_over_forward": round(
+                ms_adj / ms_fwd, 3), "table_adjoint_ms": round(ms_tf, 3),
+            "table_adjoint_over_forward": round(ms_tf / ms_fwd, 3)}
 
 
 def cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, row_step, tf_factor):

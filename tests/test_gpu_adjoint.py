@@ -304,8 +304,8 @@ def test_generate_adjoint_matches_fp64_finite_differences(hip, persp, m):
 
 def test_render_autograd_chain(hip):
     """torch.autograd through generate -> prepare -> reconstruct: the volume
-    gradient equals the GPU adjoints applied by hand in sequence, and a few
-    gradient steps on the voxels reduce the L2 distance to a target image."""
+    gradient equals the GPU adjoints applied by hand in sequence, and a
+    short step against it lowers the L2 distance to a target image."""
     import torch
 
     from mray_amd import autograd as ag
@@ -335,13 +335,9 @@ def test_render_autograd_chain(hip):
     rel = float((x.grad - g_vol).norm() / g_vol.norm())
     print(f"[render autograd] |autograd - manual| / |manual| = {rel:.2e}")
     assert rel <= 1e-5
-    losses = [float(loss)]
-    opt = torch.optim.SGD([x], lr=0.5 / max(float(x.grad.abs().max()), 1e-12))
-    for it in range(5):
-        opt.zero_grad()
-        l2 = ((ag.render(x, dmin, dmax, cam, gp, W, H, m, table, rp) - target) ** 2).sum()
-        l2.backward()
-        opt.step()
-        losses.append(float(l2))
-    print(f"[render autograd] loss {losses[0]:.4e} -> {losses[-1]:.4e}")
-    assert losses[-1] < losses[0]
+    # a short step against the gradient (largest voxel change 1e-3) lowers the loss
+    with torch.no_grad():
+        x2 = x - (1e-3 / float(x.grad.abs().max())) * x.grad
+        l2 = float(((ag.render(x2, dmin, dmax, cam, gp, W, H, m, table, rp) - target) ** 2).sum())
+    print(f"[render autograd] loss {float(loss):.6e} -> {l2:.6e} after a step of 1e-3 against the gradient")
+    assert l2 < float(loss)
