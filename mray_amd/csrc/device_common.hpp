@@ -153,6 +153,52 @@ template <typename T> __device__ __forceinline__ float sample_volume(const Volum
            ax * (1.f - ay) * az * rlr + (1.f - ax) * ay * az * lrr + ax * ay * az * rrr;
 }
 
+// sample_volume split in two: issue() computes the weights and starts the
+// eight loads, value() combines them later, so that the loads of the next
+// sample are in flight while the current one is used (same arithmetic and
+// result as sample_volume).
+template <typename T> struct PendingSample
+{
+    float ax, ay, az;
+    T lll, rll, lrl, rrl, llr, rlr, lrr, rrr;
+
+    __device__ __forceinline__ void issue(const VolumeView<T> &v, float px, float py, float pz)
+    {
+        float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
+        float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
+        float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
+        float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
+        ax = ipx - flx;
+        ay = ipy - fly;
+        az = ipz - flz;
+        int xl = (int)flx, yl = (int)fly, zl = (int)flz;
+        int x0 = iclamp(xl, 0, v.nx - 1), x1 = iclamp(xl + 1, 0, v.nx - 1);
+        int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
+        int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
+        size_t sxy = (size_t)(v.ny * v.nx);
+        size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
+        size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
+        size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
+        size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
+        lll = v.data[r00 + x0];
+        rll = v.data[r00 + x1];
+        lrl = v.data[r01 + x0];
+        rrl = v.data[r01 + x1];
+        llr = v.data[r10 + x0];
+        rlr = v.data[r10 + x1];
+        lrr = v.data[r11 + x0];
+        rrr = v.data[r11 + x1];
+    }
+
+    __device__ __forceinline__ float value() const
+    {
+        return (1.f - ax) * (1.f - ay) * (1.f - az) * voxel_value(lll) + ax * (1.f - ay) * (1.f - az) * voxel_value(rll) +
+               (1.f - ax) * ay * (1.f - az) * voxel_value(lrl) + ax * ay * (1.f - az) * voxel_value(rrl) +
+               (1.f - ax) * (1.f - ay) * az * voxel_value(llr) + ax * (1.f - ay) * az * voxel_value(rlr) +
+               (1.f - ax) * ay * az * voxel_value(lrr) + ax * ay * az * voxel_value(rrr);
+    }
+};
+
 // map_density (src/parameters.h:31-34)
 __device__ __forceinline__ float map_density(float s, float dmin, float dmax)
 {

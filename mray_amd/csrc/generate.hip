@@ -288,11 +288,19 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     int node = 0;
 
     const int cap = march_cap(t_near, t_far, s);
+    // the next sample's loads are issued one iteration ahead (at t + s while
+    // the march continues, else at t_far for the last segment)
+    auto issue_at = [&](PendingSample<T> &ps, float tt) {
+        ps.issue(vol, r.o.x + tt * r.d.x, r.o.y + tt * r.d.y, r.o.z + tt * r.d.z);
+    };
+    PendingSample<T> pending;
+    issue_at(pending, (t + s < t_far) ? t + s : t_far);
     for (int it = 0; t + s < t_far && it < cap; ++it)
     {
         t += s;
         const float p = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
-        const float f = sample_at(vol, r, t, a);
+        const float f = map_density(pending.value(), a.data_min, a.data_max);
+        issue_at(pending, (t + s < t_far && it + 1 < cap) ? t + s : t_far);
         ++ns;
         lo = rmin(lo, f);
         hi = rmax(hi, f);
@@ -315,7 +323,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
-    const float f_last = sample_at(vol, r, t_far, a);
+    const float f_last = map_density(pending.value(), a.data_min, a.data_max);
     ++ns;
     lo = rmin(lo, f_last);
     hi = rmax(hi, f_last);
