@@ -16,9 +16,12 @@
 //  * steps after the |u|^2 >= 1 guard fired use u = 0 (constant).
 // Per-pixel arrays live in a global scratch laid out [array][element][thread]
 // so that the lanes of a wavefront touch consecutive addresses; the grid is a
-// fixed number of threads that stride over the pixels, which bounds the
-// scratch to threads x 7 arrays x m x 16 bytes.
+// fixed number of threads (16 wavefronts per CU) that stride over the
+// pixels, which bounds the scratch to threads x 7 arrays x m x 16 bytes
+// (1.2 GB at 40 moments; fewer threads are launched for smaller images).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device_common.hpp"
 #include "launch.hpp"
@@ -50,7 +53,7 @@ struct Dcol
 };
 
 constexpr int kAdjThreads = 128;
-constexpr int kAdjBlocks = 256;
+constexpr int kAdjBlocks = 8192; // 16 wavefronts per CU on 256 CUs: enough to hide the scratch latency
 constexpr int kArrays = 7;
 
 __global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const float *__restrict__ mom,
@@ -264,16 +267,17 @@ __global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const floa
 
 } // namespace
 
-size_t prepare_adjoint_scratch_bytes(int m) { return (size_t)kAdjBlocks * kAdjThreads * kArrays * m * sizeof(dc); }
-
 int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, int H, int m, float bias,
                            const float *grad_lagrange, float *grad_moments, hipStream_t stream)
 {
+    const int npix = W * H;
+    const int blocks = std::min(kAdjBlocks, (npix + kAdjThreads - 1) / kAdjThreads);
+    const size_t bytes = (size_t)blocks * kAdjThreads * kArrays * m * sizeof(dc);
     void *scratch = nullptr;
-    if (hipMallocAsync(&scratch, prepare_adjoint_scratch_bytes(m), stream) != hipSuccess)
+    if (hipMallocAsync(&scratch, bytes, stream) != hipSuccess)
         return MRAY_ERR_OOM;
-    hipLaunchKernelGGL(prepare_adjoint_kernel, dim3(kAdjBlocks), dim3(kAdjThreads), 0, stream, moments, index, W * H,
-                       m, bias, grad_lagrange, grad_moments, static_cast<dc *>(scratch));
+    hipLaunchKernelGGL(prepare_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), 0, stream, moments, index, npix, m,
+                       bias, grad_lagrange, grad_moments, static_cast<dc *>(scratch));
     return hipFreeAsync(scratch, stream) == hipSuccess ? 0 : MRAY_ERR_HIP;
 }
 

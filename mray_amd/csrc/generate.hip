@@ -437,33 +437,89 @@ static int plain_block(int m) { return (256 * m * (int)sizeof(float) <= 65536) ?
 // ADJOINT (F32 volumes): dL/dvolume from dL/dmoments
 // ---------------------------------------------------------------------------
 
-// adds gs times the trilinear weights of sample_volume at (px, py, pz) to the
-// voxels it reads (clamp-to-edge duplicates receive both weights)
-__device__ __forceinline__ void scatter_sample(float *grad, const VolumeView<float> &v, float px, float py, float pz,
-                                               float gs)
+// Accumulates gs times the trilinear weights of sample_volume at a position
+// into the voxels it reads. Consecutive samples of a ray that fall into the
+// same cell are merged in registers; a cell's eight sums are added to the
+// gradient (float atomics) when the ray leaves it. Clamp-to-edge duplicates
+// receive both of their weights.
+struct CellScatter
 {
-    float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
-    float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
-    float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
-    float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
-    float ax = ipx - flx, ay = ipy - fly, az = ipz - flz;
-    int xl = (int)flx, yl = (int)fly, zl = (int)flz;
-    int x0 = iclamp(xl, 0, v.nx - 1), x1 = iclamp(xl + 1, 0, v.nx - 1);
-    int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
-    int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
-    size_t sxy = (size_t)(v.ny * v.nx);
-    size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
-    size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
-    size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
-    size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
-    unsafeAtomicAdd(&grad[r00 + x0], gs * ((1.f - ax) * (1.f - ay) * (1.f - az)));
-    unsafeAtomicAdd(&grad[r00 + x1], gs * (ax * (1.f - ay) * (1.f - az)));
-    unsafeAtomicAdd(&grad[r01 + x0], gs * ((1.f - ax) * ay * (1.f - az)));
-    unsafeAtomicAdd(&grad[r01 + x1], gs * (ax * ay * (1.f - az)));
-    unsafeAtomicAdd(&grad[r10 + x0], gs * ((1.f - ax) * (1.f - ay) * az));
-    unsafeAtomicAdd(&grad[r10 + x1], gs * (ax * (1.f - ay) * az));
-    unsafeAtomicAdd(&grad[r11 + x0], gs * ((1.f - ax) * ay * az));
-    unsafeAtomicAdd(&grad[r11 + x1], gs * (ax * ay * az));
+    size_t base[4] = {0, 0, 0, 0}; // row offsets r00, r01, r10, r11
+    int x0 = -1, x1 = -1;
+    float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool live = false;
+
+    __device__ __forceinline__ void flush(float *grad)
+    {
+        if (!live)
+            return;
+        unsafeAtomicAdd(&grad[base[0] + x0], w[0]);
+        unsafeAtomicAdd(&grad[base[0] + x1], w[1]);
+        unsafeAtomicAdd(&grad[base[1] + x0], w[2]);
+        unsafeAtomicAdd(&grad[base[1] + x1], w[3]);
+        unsafeAtomicAdd(&grad[base[2] + x0], w[4]);
+        unsafeAtomicAdd(&grad[base[2] + x1], w[5]);
+        unsafeAtomicAdd(&grad[base[3] + x0], w[6]);
+        unsafeAtomicAdd(&grad[base[3] + x1], w[7]);
+        live = false;
+    }
+
+    __device__ __forceinline__ void add(float *grad, const VolumeView<float> &v, float px, float py, float pz, float gs)
+    {
+        float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
+        float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
+        float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
+        float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
+        float ax = ipx - flx, ay = ipy - fly, az = ipz - flz;
+        int xl = (int)flx, yl = (int)fly, zl = (int)flz;
+        int nx0 = iclamp(xl, 0, v.nx - 1), nx1 = iclamp(xl + 1, 0, v.nx - 1);
+        int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
+        int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
+        size_t sxy = (size_t)(v.ny * v.nx);
+        size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
+        size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
+        size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
+        size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
+        if (!(live && nx0 == x0 && nx1 == x1 && r00 == base[0] && r01 == base[1] && r10 == base[2] && r11 == base[3]))
+        {
+            flush(grad);
+            x0 = nx0;
+            x1 = nx1;
+            base[0] = r00;
+            base[1] = r01;
+            base[2] = r10;
+            base[3] = r11;
+            for (int i = 0; i < 8; ++i)
+                w[i] = 0.0f;
+            live = true;
+        }
+        w[0] += gs * ((1.f - ax) * (1.f - ay) * (1.f - az));
+        w[1] += gs * (ax * (1.f - ay) * (1.f - az));
+        w[2] += gs * ((1.f - ax) * ay * (1.f - az));
+        w[3] += gs * (ax * ay * (1.f - az));
+        w[4] += gs * ((1.f - ax) * (1.f - ay) * az);
+        w[5] += gs * (ax * (1.f - ay) * az);
+        w[6] += gs * ((1.f - ax) * ay * az);
+        w[7] += gs * (ax * ay * az);
+    }
+};
+
+// cos and sin of (a + d) from those of a, for the small differences between
+// consecutive node phases (FP64 Taylor terms up to d^9; sincos beyond 0.1)
+__device__ __forceinline__ void rotate_small(double &c, double &s, double a_plus_d, double d)
+{
+    if (fabs(d) > 0.1)
+    {
+        sincos(a_plus_d, &s, &c);
+        return;
+    }
+    const double d2 = d * d;
+    const double cd = 1.0 - d2 * (0.5 - d2 * (1.0 / 24 - d2 * (1.0 / 720 - d2 * (1.0 / 40320))));
+    const double sd = d * (1.0 - d2 * (1.0 / 6 - d2 * (1.0 / 120 - d2 * (1.0 / 5040 - d2 * (1.0 / 362880)))));
+    const double nc = c * cd - s * sd;
+    const double ns = s * cd + c * sd;
+    c = nc;
+    s = ns;
 }
 
 // Reverse mode of the generate march for dL/dmoments (dense W*H*m): the
@@ -476,7 +532,9 @@ __device__ __forceinline__ void scatter_sample(float *grad, const VolumeView<flo
 // then through map_density (slope 1 / (data_max - data_min) where it does not
 // clamp) and the trilinear weights to the voxels. The bounds are constants
 // (a first pass recomputes them for the normalisation). cos / sin of j x by
-// Chebyshev recurrences in FP64. One thread per ray, nothing stored per sample.
+// Chebyshev recurrences in FP64 from cos / sin of each segment's end points,
+// which are carried from node to node by exact phase differences. One
+// thread per ray, nothing stored per sample.
 template <int M>
 __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float> vol, DevCamera cam, GenArgs a,
                                                                const float *__restrict__ grad_mom,
@@ -534,17 +592,14 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         hi = rmin(hi + a.bound_eps, 1.0f);
         inv = 1.0 / ((double)hi - (double)lo);
     }
-    // dL/dtrig_j = dL/dmoment_j * inv / pi, folded with 1/j and 1/j^2
-    float gj1[M], gj2[M];
+    // dL/dtrig_j = dL/dmoment_j * inv / pi (1/j and 1/j^2 are applied in the
+    // unrolled loop as constants)
+    float gj[M];
     const double scale = inv / kPi;
     const double g0 = (double)grad_mom[pix * m] * scale;
 #pragma unroll
     for (int j = 1; j < M; ++j)
-    {
-        const double g = j < m ? (double)grad_mom[pix * m + j] * scale : 0.0;
-        gj1[j] = (float)(g / j);
-        gj2[j] = (float)(g / ((double)j * j));
-    }
+        gj[j] = j < m ? (float)((double)grad_mom[pix * m + j] * scale) : 0.0f;
     const float dscale = 1.0f / (a.data_max - a.data_min);
     // d f / d s of map_density
     auto dmap = [&](float sv) {
@@ -556,6 +611,11 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     float px0, py0, pz0;
     float s0 = raw(t, px0, py0, pz0);
     double fbar0 = 0.0; // pending gradient of the node behind the current segment
+    CellScatter cells;
+    // cos / sin of the previous node's phase, carried forward by exact
+    // differences (consecutive phases differ by ~dphi, or by ulps)
+    double ph_prev = (double)p, cprev, sprev;
+    sincos(ph_prev, &sprev, &cprev);
     for (int it = 0;; ++it)
     {
         const bool last = !(t + s < t_far) || it >= cap;
@@ -574,18 +634,24 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
             s1 = raw(t_far, px1, py1, pz1);
         }
         const double pn = p_next, pp = (double)(p_next - seg), sg = pn - pp;
-        double sn1, cn1, sp1, cp1;
-        sincos(pn, &sn1, &cn1);
-        sincos(pp, &sp1, &cp1);
+        double cp1 = cprev, sp1 = sprev;
+        rotate_small(cp1, sp1, pp, pp - ph_prev);
+        double cn1 = cp1, sn1 = sp1;
+        rotate_small(cn1, sn1, pn, sg);
+        ph_prev = pn;
+        cprev = cn1;
+        sprev = sn1;
         double X = 0.0, Y = 0.0, Z = 0.0;
         double cn = cn1, sn = sn1, cp = cp1, sp = sp1;      // j = 1
         double cnm = 1.0, snm = 0.0, cpm = 1.0, spm = 0.0;  // j = 0
 #pragma unroll
         for (int j = 1; j < M; ++j)
         {
-            X = fma((double)gj2[j], cn - cp, X);
-            Y = fma((double)gj1[j], sn, Y);
-            Z = fma((double)gj1[j], sp, Z);
+            const double g = (double)gj[j];
+            const double gi = g * (1.0 / j);
+            X = fma(g, (cn - cp) * (1.0 / ((double)j * j)), X);
+            Y = fma(gi, sn, Y);
+            Z = fma(gi, sp, Z);
             const double cnn = fma(2.0 * cn1, cn, -cnm), snn = fma(2.0 * cn1, sn, -snm);
             const double cpn = fma(2.0 * cp1, cp, -cpm), spn = fma(2.0 * cp1, sp, -spm);
             cnm = cn;
@@ -605,7 +671,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         fbar0 += A;
         const float gs0 = (float)fbar0 * dmap(s0);
         if (gs0 != 0.0f)
-            scatter_sample(grad_vol, vol, px0, py0, pz0, gs0);
+            cells.add(grad_vol, vol, px0, py0, pz0, gs0);
         fbar0 = B;
         s0 = s1;
         px0 = px1;
@@ -617,7 +683,8 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     }
     const float gs0 = (float)fbar0 * dmap(s0);
     if (gs0 != 0.0f)
-        scatter_sample(grad_vol, vol, px0, py0, pz0, gs0);
+        cells.add(grad_vol, vol, px0, py0, pz0, gs0);
+    cells.flush(grad_vol);
 }
 
 int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const float *dmin, const float *dmax,
