@@ -14,7 +14,9 @@
 //    which inverts to a = a' + u conj(b'), b = b' + u conj(a'), so the
 //    previous iterate is rebuilt from the current one and u_j;
 //  * steps after the |u|^2 >= 1 guard fired use u = 0 (constant).
-// Per-pixel arrays live in a global scratch laid out [array][element][thread]
+// Up to 64 moments one wavefront holds a pixel, element k of every array in
+// lane k's registers (prepare_adjoint_wave_kernel). Beyond that, per-pixel
+// arrays live in a global scratch laid out [array][element][thread]
 // so that the lanes of a wavefront touch consecutive addresses; the grid is a
 // fixed number of threads (16 wavefronts per CU) that stride over the
 // pixels, which bounds the scratch to threads x 7 arrays x m x 16 bytes
@@ -265,12 +267,221 @@ __global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const floa
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront per pixel (m <= 64): element k of every array lives in lane k, in
+// registers. Partners such as P[j - k] or E[j - k] are fetched with
+// cross-lane shuffles, inner products are butterfly reductions over the
+// wavefront; nothing but the input and output touches memory.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ dc wsum(dc v) { return dc{wsum(v.x), wsum(v.y)}; }
+// value of lane src (any int; callers mask lanes whose src is out of range)
+__device__ __forceinline__ double lane_of(double v, int src) { return __shfl(v, src & 63); }
+__device__ __forceinline__ dc lane_of(dc v, int src) { return dc{lane_of(v.x, src), lane_of(v.y, src)}; }
+
+__global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *__restrict__ mom,
+                                                                   const uint32_t *__restrict__ index, int npix,
+                                                                   int m, float bias,
+                                                                   const float *__restrict__ grad_lag,
+                                                                   float *__restrict__ grad_mom)
+{
+    const int k = threadIdx.x & 63;
+    const int pix = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (pix >= npix) // uniform per wavefront
+        return;
+    const int nm = index ? min((int)(index[pix + 1] - index[pix]), m) : m;
+    const size_t off = index ? (size_t)index[pix] : (size_t)pix * m;
+    if (nm <= 0)
+        return;
+    const bool act = k < nm;
+    const double b64 = (double)bias;
+    const double T = act ? (1.0 - b64) * (double)mom[off + k] + (k == 0 ? b64 * 0.5 : 0.0) : 0.0;
+    const dc zero = dmk(0.0, 0.0);
+
+    // ---- forward ----
+    const double ph0 = kPi * lane_of(T, 0) - 0.5 * kPi;
+    double s0, c0;
+    sincos(ph0, &s0, &c0);
+    dc E = (k == 0) ? dmk(c0 * (0.25 / kPi), s0 * (0.25 / kPi)) : zero;
+    for (int l = 1; l < nm; ++l)
+    {
+        const double Tl = lane_of(T, l - k);
+        const dc term = (k < l) ? drmul((double)(l - k) * Tl, E) : zero;
+        const dc acc = wsum(term);
+        if (k == l)
+        {
+            const double c = 2.0 * kPi / (double)l;
+            E = dmk(-acc.y * c, acc.x * c);
+        }
+    }
+    if (k == 0)
+        E = drmul(2.0, E);
+    const double e0x = lane_of(E.x, 0);
+    dc P = (k == 0) ? dmk(1.0 / e0x, 0.0) : zero;
+    dc U = zero;
+    int j_invalid = nm;
+    for (int j = 1; j < nm; ++j)
+    {
+        const dc Ejk = lane_of(E, j - k);
+        dc u = (j < j_invalid) ? wsum((k < j) ? dmul(P, Ejk) : zero) : zero;
+        double a2 = u.x * u.x + u.y * u.y;
+        if (j < j_invalid && a2 >= 1.0)
+            j_invalid = j;
+        if (j >= j_invalid)
+        {
+            u = zero;
+            a2 = 0.0;
+        }
+        if (k == j)
+            U = u;
+        const double f = 1.0 / (1.0 - a2);
+        const double p0 = lane_of(P.x, 0);
+        const dc Pq = lane_of(P, j - k);
+        if (k >= 1 && k < j)
+            P = drmul(f, dsub(P, dmul(dconj(Pq), u)));
+        else if (k == j)
+            P = drmul(-f * p0, u);
+        else if (k == 0)
+            P = dmk(f * p0, 0.0);
+    }
+    P = drmul(2.0 * kPi, P);
+    dc R = zero;
+    for (int kk = 0; kk < nm; ++kk)
+    {
+        const dc Pn = lane_of(P, k + kk);
+        const dc r = wsum((k + kk < nm) ? dmul(P, dconj(Pn)) : zero);
+        if (k == kk)
+            R = r;
+    }
+    const double p0s = lane_of(P.x, 0);
+    const double nf = 1.0 / (kPi * p0s);
+    const dc Eh = (k == 0) ? drmul(0.5, E) : E; // E_0 before doubling
+
+    // ---- reverse: imaginary correlation and normalisation ----
+    const double lb = act ? (double)grad_lag[off + k] : 0.0;
+    double nfb = 0.0;
+    dc Rb = zero, Eb = zero;
+    for (int kk = 0; kk < nm; ++kk)
+    {
+        const dc Rn = lane_of(R, k + kk); // R_{kk+j} on lane j
+        const bool in = k + kk < nm;
+        const double Ck = wsum(in ? Rn.x * Eh.y + Rn.y * Eh.x : 0.0);
+        const double lbk = lane_of(lb, kk);
+        nfb += lbk * Ck;
+        const double cb = lbk * nf;
+        // Rb_{kk+j} += cb (Eh_j.y, Eh_j.x): lane n takes Eh_{n-kk}
+        const dc Ehj = lane_of(Eh, k - kk);
+        if (k >= kk && k < nm)
+            Rb = dadd(Rb, dmk(cb * Ehj.y, cb * Ehj.x));
+        // Eh_j += cb (R_{kk+j}.y, R_{kk+j}.x); E_0 = 2 Eh_0
+        if (in)
+        {
+            const double h = (k == 0) ? 0.5 : 1.0;
+            Eb = dadd(Eb, dmk(h * cb * Rn.y, h * cb * Rn.x));
+        }
+    }
+    dc Pb = (k == 0) ? dmk(-nfb * nf / p0s, 0.0) : zero;
+    // R_kk = sum_j P_j conj(P_{j+kk})
+    for (int kk = 0; kk < nm; ++kk)
+    {
+        const dc rb = lane_of(Rb, kk);
+        const dc b = lane_of(P, k + kk);
+        const dc a = lane_of(P, k - kk);
+        if (k + kk < nm)
+            Pb = dadd(Pb, dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
+        if (k >= kk && k < nm)
+            Pb = dadd(Pb, dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
+    }
+    P = drmul(1.0 / (2.0 * kPi), P);
+    Pb = drmul(2.0 * kPi, Pb);
+
+    // ---- reverse: Levinson, rebuilding each previous iterate ----
+    for (int j = nm - 1; j >= 1; --j)
+    {
+        const dc u = lane_of(U, j);
+        const bool valid = j < j_invalid;
+        const double f = 1.0 / (1.0 - (u.x * u.x + u.y * u.y));
+        const double fb = wsum((k <= j) ? ddot(Pb, P) : 0.0) / f;
+        const dc pjb = lane_of(Pb, j);
+        const double p0b = lane_of(Pb.x, 0);
+        const double p0 = lane_of(P.x, 0) / f;
+        const dc Pq = lane_of(P, j - k);
+        dc Pold = P;
+        if (k >= 1 && k < j)
+            Pold = dadd(P, dmul(u, dconj(Pq)));
+        else if (k == 0)
+            Pold = dmk(p0, 0.0);
+        else if (k == j)
+            Pold = zero;
+        const dc Poq = lane_of(Pold, j - k); // P_{j-k} of the previous iterate
+        dc ub = drmul(2.0 * f * f * fb, u);
+        ub = dsub(ub, drmul(f, wsum((k >= 1 && k < j) ? dmul(Pb, Poq) : zero)));
+        ub = dsub(ub, drmul(f * p0, pjb));
+        const dc Pbq = lane_of(Pb, j - k);
+        dc Pbn = Pb;
+        if (k >= 1 && k < j)
+            Pbn = drmul(f, dsub(Pb, dmul(dconj(Pbq), u)));
+        else if (k == 0)
+            Pbn = dmk(f * p0b - f * ddot(pjb, u), 0.0);
+        else if (k == j)
+            Pbn = zero;
+        if (valid)
+        {
+            // u = sum_{k<j} P_k E_{j-k}
+            const dc Ejk = lane_of(E, j - k);
+            if (k < j)
+                Pbn = dadd(Pbn, dmul(ub, dconj(Ejk)));
+            if (k >= 1 && k <= j) // lane n = j - k takes P_k = Poq
+                Eb = dadd(Eb, dmul(ub, dconj(Poq)));
+        }
+        P = Pold;
+        Pb = Pbn;
+    }
+    // P_0 = 1 / E_0.x (E_0 doubled), then the recurrence's undoubled E_0
+    if (k == 0)
+    {
+        Eb.x += -Pb.x / (e0x * e0x);
+        Eb = drmul(2.0, Eb);
+    }
+    // ---- reverse: Herglotz recurrence E_l = i c_l sum_j (l-j) T_{l-j} E_j ----
+    double Tb = 0.0;
+    for (int l = nm - 1; l >= 1; --l)
+    {
+        const dc Ebl = lane_of(Eb, l);
+        const double c = 2.0 * kPi / (double)l;
+        const dc accb = dmk(c * Ebl.y, -c * Ebl.x);
+        const double Tl = lane_of(T, l - k);
+        const dc El = lane_of(Eh, l - k); // E_{l-n} on lane n
+        if (k < l)
+            Eb = dadd(Eb, drmul((double)(l - k) * Tl, accb));
+        if (k >= 1 && k <= l)
+            Tb += (double)k * ddot(accb, El);
+    }
+    if (k == 0)
+        Tb += 0.25 * (-Eb.x * s0 + Eb.y * c0);
+    if (act)
+        grad_mom[off + k] = (float)((1.0 - b64) * Tb);
+}
+
 } // namespace
 
 int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, int H, int m, float bias,
                            const float *grad_lagrange, float *grad_moments, hipStream_t stream)
 {
     const int npix = W * H;
+    if (m <= 64)
+    {
+        hipLaunchKernelGGL(prepare_adjoint_wave_kernel, dim3((npix + 3) / 4), dim3(256), 0, stream, moments, index,
+                           npix, m, bias, grad_lagrange, grad_moments);
+        return 0;
+    }
     const int blocks = std::min(kAdjBlocks, (npix + kAdjThreads - 1) / kAdjThreads);
     const size_t bytes = (size_t)blocks * kAdjThreads * kArrays * m * sizeof(dc);
     void *scratch = nullptr;
