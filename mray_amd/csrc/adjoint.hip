@@ -274,17 +274,62 @@ __global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const floa
 // wavefront; nothing but the input and output touches memory.
 // ---------------------------------------------------------------------------
 
+// lane value of a DPP move of a double (two 32-bit halves); lanes the move
+// has no source for read 0 (bound_ctrl)
+template <int CTRL, int ROW_MASK = 0xF> __device__ __forceinline__ double dpp_d(double v)
+{
+    int2 w = *reinterpret_cast<int2 *>(&v), o;
+    o.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, ROW_MASK, 0xF, true);
+    o.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, ROW_MASK, 0xF, true);
+    return *reinterpret_cast<double *>(&o);
+}
+
+// sum over the 64 lanes, returned to every lane: prefix sums inside each row
+// of 16 lanes by row_shr 1, 2, 4, 8, then row_bcast 15 / 31 carry the row
+// totals upwards (DPP, executed by the VALU), and lane 63 holds the total
 __device__ __forceinline__ double wsum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o);
-    return v;
+    v += dpp_d<0x111>(v); // row_shr:1
+    v += dpp_d<0x112>(v); // row_shr:2
+    v += dpp_d<0x114>(v); // row_shr:4
+    v += dpp_d<0x118>(v); // row_shr:8
+    v += dpp_d<0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
+    v += dpp_d<0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
+    int2 w = *reinterpret_cast<int2 *>(&v), o;
+    o.x = __builtin_amdgcn_readlane(w.x, 63);
+    o.y = __builtin_amdgcn_readlane(w.y, 63);
+    return *reinterpret_cast<double *>(&o);
 }
 __device__ __forceinline__ dc wsum(dc v) { return dc{wsum(v.x), wsum(v.y)}; }
-// value of lane src (any int; callers mask lanes whose src is out of range)
-__device__ __forceinline__ double lane_of(double v, int src) { return __shfl(v, src & 63); }
-__device__ __forceinline__ dc lane_of(dc v, int src) { return dc{lane_of(v.x, src), lane_of(v.y, src)}; }
+// value of a wave-uniform lane (v_readlane)
+__device__ __forceinline__ double bcast(double v, int src)
+{
+    int2 w = *reinterpret_cast<int2 *>(&v), o;
+    o.x = __builtin_amdgcn_readlane(w.x, src);
+    o.y = __builtin_amdgcn_readlane(w.y, src);
+    return *reinterpret_cast<double *>(&o);
+}
+__device__ __forceinline__ dc bcast(dc v, int src) { return dc{bcast(v.x, src), bcast(v.y, src)}; }
+
+// Orders the LDS accesses of one wavefront (its lanes run in lockstep).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Per-wavefront LDS copies of the arrays whose elements other lanes read at
+// per-lane offsets (j - k, k + kk, ...): a store of lane k's element and
+// masked reads of arbitrary elements.
+struct WaveArrays
+{
+    double *T;
+    dc *E, *Eh, *P, *R, *Q; // Q: scratch copy of the array being updated
+};
+constexpr int kWaveLdsBytes = 64 * 8 + 5 * 64 * 16;
+
+__device__ __forceinline__ dc ld(const dc *a, int i, bool ok) { return ok ? a[i] : dc{0.0, 0.0}; }
+__device__ __forceinline__ double ld(const double *a, int i, bool ok) { return ok ? a[i] : 0.0; }
 
 __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *__restrict__ mom,
                                                                    const uint32_t *__restrict__ index, int npix,
@@ -292,45 +337,63 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
                                                                    const float *__restrict__ grad_lag,
                                                                    float *__restrict__ grad_mom)
 {
+    __shared__ __align__(16) unsigned char lds_raw[4 * kWaveLdsBytes];
     const int k = threadIdx.x & 63;
-    const int pix = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int w = threadIdx.x >> 6;
+    const int pix = blockIdx.x * (blockDim.x / 64) + w;
     if (pix >= npix) // uniform per wavefront
         return;
     const int nm = index ? min((int)(index[pix + 1] - index[pix]), m) : m;
     const size_t off = index ? (size_t)index[pix] : (size_t)pix * m;
     if (nm <= 0)
         return;
+    unsigned char *base = lds_raw + w * kWaveLdsBytes;
+    WaveArrays L;
+    L.T = reinterpret_cast<double *>(base);
+    L.E = reinterpret_cast<dc *>(base + 64 * 8);
+    L.Eh = L.E + 64;
+    L.P = L.Eh + 64;
+    L.R = L.P + 64;
+    L.Q = L.R + 64;
+
     const bool act = k < nm;
     const double b64 = (double)bias;
     const double T = act ? (1.0 - b64) * (double)mom[off + k] + (k == 0 ? b64 * 0.5 : 0.0) : 0.0;
     const dc zero = dmk(0.0, 0.0);
+    L.T[k] = T;
+    wave_lds_sync();
 
     // ---- forward ----
-    const double ph0 = kPi * lane_of(T, 0) - 0.5 * kPi;
+    const double ph0 = kPi * bcast(T, 0) - 0.5 * kPi;
     double s0, c0;
     sincos(ph0, &s0, &c0);
     dc E = (k == 0) ? dmk(c0 * (0.25 / kPi), s0 * (0.25 / kPi)) : zero;
     for (int l = 1; l < nm; ++l)
     {
-        const double Tl = lane_of(T, l - k);
-        const dc term = (k < l) ? drmul((double)(l - k) * Tl, E) : zero;
-        const dc acc = wsum(term);
+        const double Tl = ld(L.T, l - k, k < l);
+        const dc acc = wsum(drmul((double)(l - k) * Tl, E)); // E = 0 on lanes >= l
         if (k == l)
         {
             const double c = 2.0 * kPi / (double)l;
             E = dmk(-acc.y * c, acc.x * c);
         }
     }
+    const dc Eh = E; // E_0 before doubling
     if (k == 0)
         E = drmul(2.0, E);
-    const double e0x = lane_of(E.x, 0);
+    L.E[k] = E;
+    L.Eh[k] = Eh;
+    wave_lds_sync();
+    const double e0x = bcast(E.x, 0);
     dc P = (k == 0) ? dmk(1.0 / e0x, 0.0) : zero;
     dc U = zero;
     int j_invalid = nm;
     for (int j = 1; j < nm; ++j)
     {
-        const dc Ejk = lane_of(E, j - k);
-        dc u = (j < j_invalid) ? wsum((k < j) ? dmul(P, Ejk) : zero) : zero;
+        L.P[k] = P;
+        wave_lds_sync();
+        const dc Ejk = ld(L.E, j - k, k < j);
+        dc u = (j < j_invalid) ? wsum(dmul(P, Ejk)) : zero;
         double a2 = u.x * u.x + u.y * u.y;
         if (j < j_invalid && a2 >= 1.0)
             j_invalid = j;
@@ -342,8 +405,9 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
         if (k == j)
             U = u;
         const double f = 1.0 / (1.0 - a2);
-        const double p0 = lane_of(P.x, 0);
-        const dc Pq = lane_of(P, j - k);
+        const double p0 = bcast(P.x, 0);
+        const dc Pq = ld(L.P, j - k, k >= 1 && k < j);
+        wave_lds_sync();
         if (k >= 1 && k < j)
             P = drmul(f, dsub(P, dmul(dconj(Pq), u)));
         else if (k == j)
@@ -352,17 +416,20 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
             P = dmk(f * p0, 0.0);
     }
     P = drmul(2.0 * kPi, P);
+    L.P[k] = P;
+    wave_lds_sync();
     dc R = zero;
     for (int kk = 0; kk < nm; ++kk)
     {
-        const dc Pn = lane_of(P, k + kk);
-        const dc r = wsum((k + kk < nm) ? dmul(P, dconj(Pn)) : zero);
+        const dc Pn = ld(L.P, k + kk, k + kk < nm);
+        const dc r = wsum(dmul(P, dconj(Pn)));
         if (k == kk)
             R = r;
     }
-    const double p0s = lane_of(P.x, 0);
+    L.R[k] = R;
+    wave_lds_sync();
+    const double p0s = bcast(P.x, 0);
     const double nf = 1.0 / (kPi * p0s);
-    const dc Eh = (k == 0) ? drmul(0.5, E) : E; // E_0 before doubling
 
     // ---- reverse: imaginary correlation and normalisation ----
     const double lb = act ? (double)grad_lag[off + k] : 0.0;
@@ -370,49 +437,48 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
     dc Rb = zero, Eb = zero;
     for (int kk = 0; kk < nm; ++kk)
     {
-        const dc Rn = lane_of(R, k + kk); // R_{kk+j} on lane j
         const bool in = k + kk < nm;
-        const double Ck = wsum(in ? Rn.x * Eh.y + Rn.y * Eh.x : 0.0);
-        const double lbk = lane_of(lb, kk);
+        const dc Rn = ld(L.R, k + kk, in); // R_{kk+j} on lane j
+        const double Ck = wsum(Rn.x * Eh.y + Rn.y * Eh.x);
+        const double lbk = bcast(lb, kk);
         nfb += lbk * Ck;
         const double cb = lbk * nf;
         // Rb_{kk+j} += cb (Eh_j.y, Eh_j.x): lane n takes Eh_{n-kk}
-        const dc Ehj = lane_of(Eh, k - kk);
-        if (k >= kk && k < nm)
-            Rb = dadd(Rb, dmk(cb * Ehj.y, cb * Ehj.x));
+        const dc Ehj = ld(L.Eh, k - kk, k >= kk && k < nm);
+        Rb = dadd(Rb, dmk(cb * Ehj.y, cb * Ehj.x));
         // Eh_j += cb (R_{kk+j}.y, R_{kk+j}.x); E_0 = 2 Eh_0
-        if (in)
-        {
-            const double h = (k == 0) ? 0.5 : 1.0;
-            Eb = dadd(Eb, dmk(h * cb * Rn.y, h * cb * Rn.x));
-        }
+        const double h = (k == 0) ? 0.5 : 1.0;
+        Eb = dadd(Eb, dmk(h * cb * Rn.y, h * cb * Rn.x));
     }
     dc Pb = (k == 0) ? dmk(-nfb * nf / p0s, 0.0) : zero;
-    // R_kk = sum_j P_j conj(P_{j+kk})
+    // R_kk = sum_j P_j conj(P_{j+kk}) (P is in L.P)
     for (int kk = 0; kk < nm; ++kk)
     {
-        const dc rb = lane_of(Rb, kk);
-        const dc b = lane_of(P, k + kk);
-        const dc a = lane_of(P, k - kk);
-        if (k + kk < nm)
-            Pb = dadd(Pb, dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
-        if (k >= kk && k < nm)
-            Pb = dadd(Pb, dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
+        const dc rb = bcast(Rb, kk);
+        const dc b = ld(L.P, k + kk, k + kk < nm);
+        const dc a = ld(L.P, k - kk, k >= kk && k < nm);
+        Pb = dadd(Pb, dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
+        Pb = dadd(Pb, dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
     }
     P = drmul(1.0 / (2.0 * kPi), P);
     Pb = drmul(2.0 * kPi, Pb);
 
     // ---- reverse: Levinson, rebuilding each previous iterate ----
+    dc *Pl = L.P, *Ql = L.Q, *Pbl = L.R; // L.R is free from here on
     for (int j = nm - 1; j >= 1; --j)
     {
-        const dc u = lane_of(U, j);
+        const dc u = bcast(U, j);
         const bool valid = j < j_invalid;
         const double f = 1.0 / (1.0 - (u.x * u.x + u.y * u.y));
         const double fb = wsum((k <= j) ? ddot(Pb, P) : 0.0) / f;
-        const dc pjb = lane_of(Pb, j);
-        const double p0b = lane_of(Pb.x, 0);
-        const double p0 = lane_of(P.x, 0) / f;
-        const dc Pq = lane_of(P, j - k);
+        const dc pjb = bcast(Pb, j);
+        const double p0b = bcast(Pb.x, 0);
+        const double p0 = bcast(P.x, 0) / f;
+        Pl[k] = P;
+        Pbl[k] = Pb;
+        wave_lds_sync();
+        const dc Pq = ld(Pl, j - k, k >= 1 && k < j);
+        const dc Pbq = ld(Pbl, j - k, k >= 1 && k < j);
         dc Pold = P;
         if (k >= 1 && k < j)
             Pold = dadd(P, dmul(u, dconj(Pq)));
@@ -420,11 +486,13 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
             Pold = dmk(p0, 0.0);
         else if (k == j)
             Pold = zero;
-        const dc Poq = lane_of(Pold, j - k); // P_{j-k} of the previous iterate
+        Ql[k] = Pold;
+        wave_lds_sync();
+        const dc Poq = ld(Ql, j - k, k <= j); // P_{j-k} of the previous iterate
+        wave_lds_sync();
         dc ub = drmul(2.0 * f * f * fb, u);
         ub = dsub(ub, drmul(f, wsum((k >= 1 && k < j) ? dmul(Pb, Poq) : zero)));
         ub = dsub(ub, drmul(f * p0, pjb));
-        const dc Pbq = lane_of(Pb, j - k);
         dc Pbn = Pb;
         if (k >= 1 && k < j)
             Pbn = drmul(f, dsub(Pb, dmul(dconj(Pbq), u)));
@@ -435,9 +503,8 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
         if (valid)
         {
             // u = sum_{k<j} P_k E_{j-k}
-            const dc Ejk = lane_of(E, j - k);
-            if (k < j)
-                Pbn = dadd(Pbn, dmul(ub, dconj(Ejk)));
+            const dc Ejk = ld(L.E, j - k, k < j);
+            Pbn = dadd(Pbn, dmul(ub, dconj(Ejk)));
             if (k >= 1 && k <= j) // lane n = j - k takes P_k = Poq
                 Eb = dadd(Eb, dmul(ub, dconj(Poq)));
         }
@@ -454,15 +521,14 @@ __global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *
     double Tb = 0.0;
     for (int l = nm - 1; l >= 1; --l)
     {
-        const dc Ebl = lane_of(Eb, l);
+        const dc Ebl = bcast(Eb, l);
         const double c = 2.0 * kPi / (double)l;
         const dc accb = dmk(c * Ebl.y, -c * Ebl.x);
-        const double Tl = lane_of(T, l - k);
-        const dc El = lane_of(Eh, l - k); // E_{l-n} on lane n
+        const double Tl = ld(L.T, l - k, k < l);
+        const dc El = ld(L.Eh, l - k, k >= 1 && k <= l); // E_{l-n} on lane n
         if (k < l)
             Eb = dadd(Eb, drmul((double)(l - k) * Tl, accb));
-        if (k >= 1 && k <= l)
-            Tb += (double)k * ddot(accb, El);
+        Tb += (double)k * ddot(accb, El);
     }
     if (k == 0)
         Tb += 0.25 * (-Eb.x * s0 + Eb.y * c0);

@@ -592,8 +592,8 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         hi = rmin(hi + a.bound_eps, 1.0f);
         inv = 1.0 / ((double)hi - (double)lo);
     }
-    // dL/dtrig_j = dL/dmoment_j * inv / pi (1/j and 1/j^2 are applied in the
-    // unrolled loop as constants)
+    // dL/dtrig_j = dL/dmoment_j * inv / pi (the factors 1/j and 1/j^2 are
+    // float literals of the unrolled loop)
     float gj[M];
     const double scale = inv / kPi;
     const double g0 = (double)grad_mom[pix * m] * scale;
@@ -641,17 +641,19 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         ph_prev = pn;
         cprev = cn1;
         sprev = sn1;
-        double X = 0.0, Y = 0.0, Z = 0.0;
+        // X = sum g_j / j^2 (cos j pn - cos j pp), V = sum g_j / j (sin j pn -
+        // sin j pp), Yn = sum g_j / j sin j pn: the differences are formed in FP64 (they are
+        // small next to their terms), the weighted sums in FP32
+        float X = 0.0f, V = 0.0f, Yn = 0.0f;
         double cn = cn1, sn = sn1, cp = cp1, sp = sp1;      // j = 1
         double cnm = 1.0, snm = 0.0, cpm = 1.0, spm = 0.0;  // j = 0
 #pragma unroll
         for (int j = 1; j < M; ++j)
         {
-            const double g = (double)gj[j];
-            const double gi = g * (1.0 / j);
-            X = fma(g, (cn - cp) * (1.0 / ((double)j * j)), X);
-            Y = fma(gi, sn, Y);
-            Z = fma(gi, sp, Z);
+            const float g1 = gj[j] * (1.0f / (float)j);
+            X = __builtin_fmaf(g1 * (1.0f / (float)j), (float)(cn - cp), X);
+            V = __builtin_fmaf(g1, (float)(sn - sp), V);
+            Yn = __builtin_fmaf(g1, (float)sn, Yn);
             const double cnn = fma(2.0 * cn1, cn, -cnm), snn = fma(2.0 * cn1, sn, -snm);
             const double cpn = fma(2.0 * cp1, cp, -cpm), spn = fma(2.0 * cp1, sp, -spm);
             cnm = cn;
@@ -666,8 +668,9 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         const double segd = (double)seg;
         const double rr = sg / segd;
         const double half = 0.5 * sg * sg / segd;
-        const double A = -X / segd + (1.0 - rr) * Y - Z + g0 * (sg - half);
-        const double B = X / segd + rr * Y + g0 * half;
+        // A = -X / seg + (1 - r) Y - Z with Y - Z = V, Y = Yn
+        const double A = -(double)X / segd + (double)V - rr * (double)Yn + g0 * (sg - half);
+        const double B = (double)X / segd + rr * (double)Yn + g0 * half;
         fbar0 += A;
         const float gs0 = (float)fbar0 * dmap(s0);
         if (gs0 != 0.0f)
