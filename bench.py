@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cpu-row-step", type=int, default=64, help="CPU baseline: every k-th row")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="time only the adjoint workload and print its figures (for kernel profiles)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -92,6 +94,9 @@ def main():
     from mray_amd.types import GenParams, RecParams
 
     hip.init(0)
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_ratio(args.steps)}), flush=True)
+        return
     n, W, H, m = args.n, args.res, args.res, args.moments
     dmin, dmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
     cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=25.0, znear=0.01, aspect=W / H)
