@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "../../include/mray_types.h"
+#include "div_rn.hpp"
 #include "host_sincosf.hpp"
 
 namespace mray
@@ -117,7 +118,33 @@ template <typename T> struct VolumeView
     const T *__restrict__ data;
     int nx, ny, nz;
     Vec3 dmin, dmax;
+    Vec3 rext; // host_reciprocal(dmax - dmin) per axis
 };
+
+// a / b with y = host_reciprocal(b) formed once (div_rn.hpp): the IEEE
+// quotient, by three operations where the operands are in range
+__device__ __forceinline__ float div_rn(float a, float b, float y)
+{
+    const float q = div_rn_core(a, b, y);
+    if (__builtin_expect(!div_rn_fast_ok(a, q), 0))
+        return a / b;
+    return q;
+}
+
+template <typename T>
+inline VolumeView<T> make_volume_view(const T *data, int nx, int ny, int nz, const float *dmin, const float *dmax)
+{
+    VolumeView<T> v;
+    v.data = data;
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    v.dmin = {dmin[0], dmin[1], dmin[2]};
+    v.dmax = {dmax[0], dmax[1], dmax[2]};
+    v.rext = {host_reciprocal(dmax[0] - dmin[0]), host_reciprocal(dmax[1] - dmin[1]),
+              host_reciprocal(dmax[2] - dmin[2])};
+    return v;
+}
 
 __device__ __forceinline__ float voxel_value(float v) { return v; }
 __device__ __forceinline__ float voxel_value(uint8_t v) { return (float)(v / 255.f); }
@@ -164,9 +191,9 @@ template <typename T> struct PendingSample
 
     __device__ __forceinline__ void issue(const VolumeView<T> &v, float px, float py, float pz)
     {
-        float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
-        float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
-        float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
+        float ipx = div_rn(px - v.dmin.x, v.dmax.x - v.dmin.x, v.rext.x) * (float)v.nx;
+        float ipy = div_rn(py - v.dmin.y, v.dmax.y - v.dmin.y, v.rext.y) * (float)v.ny;
+        float ipz = div_rn(pz - v.dmin.z, v.dmax.z - v.dmin.z, v.rext.z) * (float)v.nz;
         float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
         ax = ipx - flx;
         ay = ipy - fly;
@@ -175,19 +202,21 @@ template <typename T> struct PendingSample
         int x0 = iclamp(xl, 0, v.nx - 1), x1 = iclamp(xl + 1, 0, v.nx - 1);
         int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
         int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
-        size_t sxy = (size_t)(v.ny * v.nx);
-        size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
-        size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
-        size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
-        size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
-        lll = v.data[r00 + x0];
-        rll = v.data[r00 + x1];
-        lrl = v.data[r01 + x0];
-        rrl = v.data[r01 + x1];
-        llr = v.data[r10 + x0];
-        rlr = v.data[r10 + x1];
-        lrr = v.data[r11 + x0];
-        rrr = v.data[r11 + x1];
+        // one 64-bit corner address, the others by 32-bit offsets (the
+        // clamped neighbour is 0 or 1 cell away on each axis)
+        const uint32_t sxy = (uint32_t)(v.ny * v.nx);
+        const T *p0 = v.data + ((size_t)z0 * sxy + (uint32_t)(y0 * v.nx + x0));
+        const uint32_t dx = (uint32_t)(x1 - x0);
+        const uint32_t dy = (uint32_t)((y1 - y0) * v.nx);
+        const T *p1 = p0 + ((z1 > z0) ? sxy : 0u);
+        lll = p0[0];
+        rll = p0[dx];
+        lrl = p0[dy];
+        rrl = p0[dy + dx];
+        llr = p1[0];
+        rlr = p1[dx];
+        lrr = p1[dy];
+        rrr = p1[dy + dx];
     }
 
     __device__ __forceinline__ float value() const
@@ -203,6 +232,12 @@ template <typename T> struct PendingSample
 __device__ __forceinline__ float map_density(float s, float dmin, float dmax)
 {
     return rclamp((s - dmin) / (dmax - dmin), 0.0f, 1.0f);
+}
+
+// map_density with host_reciprocal(dmax - dmin) formed once (same result)
+__device__ __forceinline__ float map_density_r(float s, float dmin, float dmax, float rinv)
+{
+    return rclamp(div_rn(s - dmin, dmax - dmin, rinv), 0.0f, 1.0f);
 }
 
 // DensityBound::set (src/common.h:143-147)

@@ -40,6 +40,7 @@ struct GenArgs
     int W, H, m;
     float step;
     float data_min, data_max;
+    float data_rinv; // host_reciprocal(data_max - data_min)
     int transform_to_bounds;
     int compact_image;
     float bound_eps;
@@ -288,8 +289,9 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     int node = 0;
 
     const int cap = march_cap(t_near, t_far, s);
-    // the next sample's loads are issued one iteration ahead (at t + s while
-    // the march continues, else at t_far for the last segment)
+    const float rdphi = host_reciprocal(dphi); // for the segment slopes
+    // the next sample's loads are issued one step ahead (at t + s while the
+    // march continues, else at t_far for the last segment)
     auto issue_at = [&](PendingSample<T> &ps, float tt) {
         ps.issue(vol, r.o.x + tt * r.d.x, r.o.y + tt * r.d.y, r.o.z + tt * r.d.z);
     };
@@ -299,14 +301,18 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     {
         t += s;
         const float p = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
-        const float f = map_density(pending.value(), a.data_min, a.data_max);
+        const float f = map_density_r(pending.value(), a.data_min, a.data_max, a.data_rinv);
+        // the loaded values die here: keeping the next loads below this point
+        // lets them land in the same registers (no copies that would wait
+        // for them at the end of the iteration)
+        __builtin_amdgcn_sched_barrier(0);
         issue_at(pending, (t + s < t_far && it + 1 < cap) ? t + s : t_far);
         ++ns;
         lo = rmin(lo, f);
         hi = rmax(hi, f);
         // the reference's segment [u, p]: u = p - dphi in float, slope (f - f_prev) / dphi
         const float u = p - dphi;
-        const float g = (f - f_prev) / dphi;
+        const float g = div_rn(f - f_prev, dphi, rdphi);
         const double width = (double)p - (double)u;
         trap += width * ((double)f_prev + 0.5 * (double)g * width);
         const float delta = (node > 0) ? (float)((double)u - (double)node_phase) : 0.0f;
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
-    const float f_last = map_density(pending.value(), a.data_min, a.data_max);
+    const float f_last = map_density_r(pending.value(), a.data_min, a.data_max, a.data_rinv);
     ++ns;
     lo = rmin(lo, f_last);
     hi = rmax(hi, f_last);
@@ -701,13 +707,14 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
     a.step = p.step_size;
     a.data_min = p.data_min;
     a.data_max = p.data_max;
+    a.data_rinv = host_reciprocal(p.data_max - p.data_min);
     a.transform_to_bounds = p.transform_to_bounds;
     a.compact_image = p.compact_image;
     a.bound_eps = p.transform_bound_eps;
     DevCamera c = make_dev_camera(cam);
     const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
     const uint32_t blocks = (ntiles + 3) / 4;
-    VolumeView<float> v{vox, nx, ny, nz, {dmin[0], dmin[1], dmin[2]}, {dmax[0], dmax[1], dmax[2]}};
+    VolumeView<float> v = make_volume_view(vox, nx, ny, nz, dmin, dmax);
 #define MRAY_GADJ_CASE(MM)                                                                                            \
     if (m <= MM)                                                                                                      \
     {                                                                                                                 \
@@ -738,6 +745,7 @@ int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const fl
     a.step = p.step_size;
     a.data_min = p.data_min;
     a.data_max = p.data_max;
+    a.data_rinv = host_reciprocal(p.data_max - p.data_min);
     a.transform_to_bounds = p.transform_to_bounds;
     a.compact_image = p.compact_image;
     a.bound_eps = p.transform_bound_eps;
@@ -746,7 +754,7 @@ int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const fl
     uint32_t blocks = (ntiles + 3) / 4;
     if (dtype == MRAY_DTYPE_F32)
     {
-        VolumeView<float> v{(const float *)vox, nx, ny, nz, {dmin[0], dmin[1], dmin[2]}, {dmax[0], dmax[1], dmax[2]}};
+        VolumeView<float> v = make_volume_view((const float *)vox, nx, ny, nz, dmin, dmax);
         if (form == MRAY_FORM_PLAIN)
         {
             int bs = plain_block(m);
@@ -756,7 +764,7 @@ int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const fl
         }
         return dispatch_mixed<float>(v, c, a, moments, bounds, samples, stream, blocks);
     }
-    VolumeView<uint8_t> v{(const uint8_t *)vox, nx, ny, nz, {dmin[0], dmin[1], dmin[2]}, {dmax[0], dmax[1], dmax[2]}};
+    VolumeView<uint8_t> v = make_volume_view((const uint8_t *)vox, nx, ny, nz, dmin, dmax);
     if (form == MRAY_FORM_PLAIN)
     {
         int bs = plain_block(m);
