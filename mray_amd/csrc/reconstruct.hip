@@ -163,33 +163,68 @@ __global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, R
 // ---------------------------------------------------------------------------
 // MIXED
 // ---------------------------------------------------------------------------
-template <int M>
-__device__ __forceinline__ double series_reinsch(const double (&lam)[M], float p)
+// sin x for x in [0, pi/4]: Taylor series to x^9 (truncation below 2e-9),
+// about one float ulp; no range reduction needed on this interval
+__device__ __forceinline__ float sin_quarter(float x)
+{
+    const float x2 = x * x;
+    float c = __builtin_fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+    c = __builtin_fmaf(x2, c, 8.33333333e-3f);
+    c = __builtin_fmaf(x2, c, -0.166666667f);
+    return __builtin_fmaf(x * x2, c, x);
+}
+
+template <int M, typename L>
+__device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p)
 {
     // lambda_0 + 2 sum_{l=1}^{M-1} lambda_l cos(l theta), theta = -p in [0, pi]
-    // (entries l >= m hold 0). Reinsch's form of Clenshaw's recurrence,
-    //     d_l = lambda_l + u b_{l+1} + d_{l+1},  b_l = d_l + b_{l+1},
-    //     sum_{l>=1} lambda_l cos(l theta) = d_1 + (u/2) b_1,
-    // with u = -4 sin^2(theta/2) taken from the half angle, stays accurate
-    // where cos(theta) is close to 1 (plain Clenshaw loses digits there). For
-    // theta > pi/2 it runs on theta' = pi - theta with the odd coefficients
-    // negated, so every lane takes the same path. FP64: gfx950 issues FP64
-    // FMAs at the non-packed FP32 rate.
-    float sh, ch;
-    sincosf(-0.5f * p, &sh, &ch);
-    const bool reflect = ch < sh;
-    const double q = reflect ? (double)ch : (double)sh;
-    const double u = -4.0 * q * q;
-    const double sg = reflect ? -1.0 : 1.0;
-    double b = 0.0, d = 0.0;
+    // (entries l >= m hold 0), by Clenshaw's recurrence in FP64 (2 operations
+    // per term; gfx950 issues FP64 FMAs at the non-packed FP32 rate):
+    //     b_l = lambda_l + 2x b_{l+1} - b_{l+2},  sum_{l>=1} = x b_1 - b_2.
+    // x = cos(theta) comes from the half angle, x = 1 - 2 sin^2(theta/2) or,
+    // for theta/2 > pi/4, x = 2 cos^2(theta/2) - 1 with cos(theta/2) =
+    // sin(pi/2 - theta/2), so that 1 - x (resp. 1 + x) keeps its relative
+    // accuracy next to theta = 0 (resp. pi), where the high orders need it.
+    const float h = -0.5f * p;
+    const bool reflect = h > 0.785398163f;
+    // pi/2 - h with pi/2 = 1.57079637f - 4.37113900e-8f; the first difference
+    // is exact for h in [pi/4, pi/2]
+    const float xq = reflect ? (1.57079637f - h) + -4.37113900e-8f : h;
+    const double q = (double)sin_quarter(xq);
+    const double q2 = 2.0 * q * q;
+    const double x = reflect ? q2 - 1.0 : 1.0 - q2;
+    const double x2 = x + x;
+    double b1 = 0.0, b2 = 0.0;
 #pragma unroll
     for (int l = M - 1; l >= 1; --l)
     {
-        const double lk = (l & 1) ? fma(sg, lam[l], d) : lam[l] + d;
-        d = fma(u, b, lk);
-        b = d + b;
+        const double t = fma(x2, b1, (double)lam[l] - b2);
+        b2 = b1;
+        b1 = t;
     }
-    return fma(2.0, fma(0.5 * u, b, d), lam[0]);
+    return fma(2.0, fma(x, b1, -b2), (double)lam[0]);
+}
+
+// fast_atan with the reciprocal from the hardware approximation (about one
+// ulp) instead of the IEEE division; the MIXED form only
+__device__ __forceinline__ float fast_atan_rcp(float tan)
+{
+    const bool negative = (tan < 0.0f);
+    const float abs_tan = negative ? (-tan) : tan;
+    const bool greater_one = (abs_tan > 1.0f);
+    float x = __builtin_amdgcn_rcpf(greater_one ? abs_tan : 1.0f);
+    float y = greater_one ? 1.0f : abs_tan;
+    x = x * y;
+    y = x * x;
+    float z = y * 0.020835f - 0.085133f;
+    z = y * z + 0.180141f;
+    z = y * z - 0.330299f;
+    y = y * z + 0.999866f;
+    z = y * x;
+    z = z * -2.0f + 1.570796f;
+    z = greater_one ? z : 0.0f;
+    x = x * y + z;
+    return negative ? (-x) : x;
 }
 
 template <int M>
@@ -241,8 +276,8 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
     const bool fourier = a.fourier != 0;
     auto recon = [&](float p) {
         p = rclamp(p, (float)(-kPi), 0.0f);
-        float s = (float)series_reinsch<M>(lam, p);
-        float rho = fourier ? s : __builtin_fmaf(fast_atan(s), 0.318309886f, 0.5f);
+        float s = (float)series_clenshaw<M>(lam, p);
+        float rho = fourier ? s : __builtin_fmaf(fast_atan_rcp(s), 0.318309886f, 0.5f);
         return __builtin_fmaf(rho, brange, blo);
     };
 
@@ -302,28 +337,6 @@ __device__ __forceinline__ float fast_atan_deriv(float t)
     const float dP = ((4.0f * 0.020835f * y - 3.0f * 0.085133f) * y + 2.0f * 0.180141f) * y - 0.330299f;
     const float dxp = P + 2.0f * y * dP; // d (x P(x^2)) / dx
     return greater_one ? dxp * y : dxp;
-}
-
-template <int M>
-__device__ __forceinline__ double series_reinsch_f(const float (&lam)[M], float p)
-{
-    // series_reinsch on float multipliers (converted term by term)
-    float sh, ch;
-    sincosf(-0.5f * p, &sh, &ch);
-    const bool reflect = ch < sh;
-    const double q = reflect ? (double)ch : (double)sh;
-    const double u = -4.0 * q * q;
-    const double sg = reflect ? -1.0 : 1.0;
-    double b = 0.0, d = 0.0;
-#pragma unroll
-    for (int l = M - 1; l >= 1; --l)
-    {
-        const double ll = (double)lam[l];
-        const double lk = (l & 1) ? fma(sg, ll, d) : ll + d;
-        d = fma(u, b, lk);
-        b = d + b;
-    }
-    return fma(2.0, fma(0.5 * u, b, d), (double)lam[0]);
 }
 
 // Reverse mode of reconstruct_mixed_kernel for one ray, with the march
@@ -414,10 +427,10 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                 const float dphi = (float)(kPi * (double)(a.step / (t_far - t_near)));
                 float T = 1.0f, tau = 0.0f;
                 auto rho_of = [&](float s) {
-                    const float rho = fourier ? s : __builtin_fmaf(fast_atan(s), 0.318309886f, 0.5f);
+                    const float rho = fourier ? s : __builtin_fmaf(fast_atan_rcp(s), 0.318309886f, 0.5f);
                     return __builtin_fmaf(rho, brange, blo);
                 };
-                float fs = rho_of((float)series_reinsch_f<M>(lam, rclamp(p, (float)(-kPi), 0.0f)));
+                float fs = rho_of((float)series_clenshaw<M>(lam, rclamp(p, (float)(-kPi), 0.0f)));
                 double pre[3] = {0.0, 0.0, 0.0};
                 const int cap = march_cap(t_near, t_far, a.step) + 1;
                 for (int it = 0; t <= t_far && it < cap; ++it)
@@ -425,7 +438,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     t += a.step;
                     p += dphi;
                     const float pc = rclamp(p, (float)(-kPi), 0.0f);
-                    const float S = (float)series_reinsch_f<M>(lam, pc);
+                    const float S = (float)series_clenshaw<M>(lam, pc);
                     const float bs = rho_of(S);
                     const float4 c = tf_sample(tf, fs, bs);
 
