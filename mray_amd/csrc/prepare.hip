@@ -87,13 +87,126 @@ __device__ __forceinline__ dc dmul(dc a, dc b) { return dc{a.x * b.x - a.y * b.y
 // Orders LDS accesses of the lanes of one wavefront (they execute in lockstep).
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// 1 / d by the hardware reciprocal and two Newton steps (within an ulp of
+// the IEEE quotient; d is in (0, 1] here)
+__device__ __forceinline__ double recip_nr(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
+// One step of the Herglotz recursion for pixel arrays T, E (lane g of the
+// pixel's 8): E_l = (2 pi / l) i sum_{j<l} (l - j) T_{l-j} E_j, over the S
+// slots that hold j < l (E_j = 0 for j >= l, so surplus terms vanish).
+template <int S>
+__device__ __forceinline__ void herglotz_step(const double *T, dc *E, int g, int l)
+{
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        const int j = g + 8 * s;
+        const int i = (j < l) ? l - j : 0;
+        const double w = (double)i * T[i];
+        const dc ej = E[j];
+        ax = fma(w, ej.x, ax);
+        ay = fma(w, ej.y, ay);
+    }
+    ax = grp8_sum(ax);
+    ay = grp8_sum(ay);
+    if (g == (l & 7))
+    {
+        const double c = 2.0 * kPi / (double)l;
+        E[l] = dc{-ay * c, ax * c};
+    }
+    wave_sync();
+}
+
+// Steps l = 8(S-1)+1 .. min(8S, nm-1) use S slots; then the next octave.
+template <int S, int C>
+__device__ __forceinline__ void herglotz_octaves(const double *T, dc *E, int g, int nm)
+{
+    const int hi = min(8 * S, nm - 1);
+    for (int l = 8 * (S - 1) + 1; l <= hi; ++l)
+        herglotz_step<S>(T, E, g, l);
+    if constexpr (S < C)
+        herglotz_octaves<S + 1, C>(T, E, g, nm);
+}
+
+// One Levinson step j over the S slots that hold k <= j:
+// u = sum_{k<j} A_k E_{j-k}; A_k <- f (A_k - conj(A_{j-k}) u) for k = 0..j
+// (A_j = 0 before the step, so k = 0 gives f A_0 and k = j gives -f A_0 u),
+// f = 1 / (1 - |u|^2), with the reference's |u|^2 >= 1 guard.
+template <int S>
+__device__ __forceinline__ void levinson_step(dc *A, const dc *E, int g, int j, bool &invalid)
+{
+    dc ak[S], ar[S];
+    double ux = 0.0, uy = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        const int k = g + 8 * s;
+        const int r = (k < j) ? j - k : 0;
+        ak[s] = A[k];
+        ar[s] = A[r];
+        const dc e = E[r];
+        // A_k = 0 for k >= j, so those terms vanish
+        ux = fma(ak[s].x, e.x, fma(-ak[s].y, e.y, ux));
+        uy = fma(ak[s].x, e.y, fma(ak[s].y, e.x, uy));
+    }
+    ux = grp8_sum(ux);
+    uy = grp8_sum(uy);
+    double a2 = ux * ux + uy * uy;
+    if (invalid || a2 >= 1.0)
+    {
+        ux = uy = 0.0;
+        a2 = 0.0;
+        invalid = true;
+    }
+    const double f = recip_nr(1.0 - a2);
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        const int k = g + 8 * s;
+        if (k <= j)
+        {
+            // conj(A_{j-k}) u (for k = j the partner is A_0, which is real)
+            const dc c = ar[s];
+            const double px = c.x * ux + c.y * uy;
+            const double py = c.x * uy - c.y * ux;
+            A[k] = dc{f * (ak[s].x - px), f * (ak[s].y - py)};
+        }
+    }
+    wave_sync();
+}
+
+// Steps j = max(1, 8(S-1)) .. min(8S-1, nm-1) use S slots; then the next octave.
+template <int S, int C>
+__device__ __forceinline__ void levinson_octaves(dc *A, const dc *E, int g, int nm, bool &invalid)
+{
+    const int hi = min(8 * S - 1, nm - 1);
+    for (int j = (S == 1) ? 1 : 8 * (S - 1); j <= hi; ++j)
+        levinson_step<S>(A, E, g, j, invalid);
+    if constexpr (S < C)
+        levinson_octaves<S + 1, C>(A, E, g, nm, invalid);
+}
+
 template <int C>
 __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restrict__ mom,
                                                            const uint32_t *__restrict__ index, int npix, int m,
                                                            float bias, float *__restrict__ out)
 {
+    // Per pixel (8 per wavefront) two LDS arrays of MP complex doubles: E
+    // (exponential moments) and A (Levinson coefficients, later the
+    // autocorrelation); the real moments T share A's space until the
+    // recursion starts. Every array is zero past the entries written so far,
+    // so the inner products read all their operands unconditionally (the
+    // indices are clamped into range and the surplus terms are exact zeros):
+    // the loads of a step issue together instead of one round trip per term.
     constexpr int MP = 8 * C; // padded per-pixel stride
-    extern __shared__ double lds_d[];
+    extern __shared__ dc lds_c[];
     const int lane = threadIdx.x;
     const int g = lane & 7;
     const int q = lane >> 3;
@@ -102,22 +215,20 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     const int nm = valid ? (index ? min((int)(index[pix + 1] - index[pix]), m) : m) : 0;
     const size_t off = valid ? (index ? (size_t)index[pix] : (size_t)pix * m) : 0;
 
-    double *T = lds_d + q * MP;
-    dc *E = reinterpret_cast<dc *>(lds_d + 8 * MP) + q * MP;
-    dc *A = reinterpret_cast<dc *>(lds_d + 8 * MP) + 8 * MP + q * MP;
+    dc *E = lds_c + q * MP;
+    dc *A = lds_c + 8 * MP + q * MP;
+    double *T = reinterpret_cast<double *>(A);
 
     // biased moments, lerp in FP64 (near the boundary of the moment space the
-    // recursion below amplifies a one-ulp change of the input)
+    // recursion below amplifies a one-ulp change of the input); zero padding
     const double b64 = (double)bias;
 #pragma unroll
     for (int s = 0; s < C; ++s)
     {
         const int e = g + 8 * s;
-        if (e < nm)
-        {
-            const double v = (double)mom[off + e];
-            T[e] = (1.0 - b64) * v + (e == 0 ? b64 * 0.5 : 0.0);
-        }
+        const double v = (e < nm) ? (double)mom[off + e] : 0.0;
+        T[e] = (e < nm) ? (1.0 - b64) * v + (e == 0 ? b64 * 0.5 : 0.0) : 0.0;
+        E[e] = dc{0.0, 0.0};
     }
     wave_sync();
 
@@ -130,105 +241,22 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
         E[0] = dc{cs * (0.25 / kPi), sn * (0.25 / kPi)};
     }
     wave_sync();
-    for (int l = 1; l < m; ++l)
-    {
-        if (l < nm)
-        {
-            double ax = 0.0, ay = 0.0;
-#pragma unroll
-            for (int s = 0; s < C; ++s)
-            {
-                const int j = g + 8 * s;
-                if (j < l)
-                {
-                    const double w = (double)(l - j) * T[l - j];
-                    const dc ej = E[j];
-                    ax = fma(w, ej.x, ax);
-                    ay = fma(w, ej.y, ay);
-                }
-            }
-            ax = grp8_sum(ax);
-            ay = grp8_sum(ay);
-            if (g == (l & 7))
-            {
-                const double c = 2.0 * kPi / (double)l;
-                E[l] = dc{-ay * c, ax * c};
-            }
-        }
-        wave_sync();
-    }
+    herglotz_octaves<1, C>(T, E, g, nm);
     if (g == 0 && nm > 0)
         E[0] = dc{2.0 * E[0].x, 2.0 * E[0].y};
+    // A: zero, then A_0 = 1 / E_0
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+        A[g + 8 * s] = dc{0.0, 0.0};
+    wave_sync();
+    if (g == 0 && nm > 0)
+        A[0] = dc{1.0 / E[0].x, 0.0};
     wave_sync();
 
     // Levinson recursion on the Hermitian Toeplitz matrix of E, with the
     // reference's |u|^2 >= 1 guard (levinsonsAlgorithm, src/mese/MESE_dynamic.h:100-135)
-    if (g == 0 && nm > 0)
-        A[0] = dc{1.0 / E[0].x, 0.0};
-    wave_sync();
     bool invalid = false;
-    for (int j = 1; j < m; ++j)
-    {
-        if (j < nm)
-        {
-            double ux = 0.0, uy = 0.0;
-            if (!invalid)
-            {
-#pragma unroll
-                for (int s = 0; s < C; ++s)
-                {
-                    const int k = g + 8 * s;
-                    if (k < j)
-                    {
-                        const dc a = A[k], e = E[j - k];
-                        ux = fma(a.x, e.x, fma(-a.y, e.y, ux));
-                        uy = fma(a.x, e.y, fma(a.y, e.x, uy));
-                    }
-                }
-            }
-            ux = grp8_sum(ux);
-            uy = grp8_sum(uy);
-            double a2 = ux * ux + uy * uy;
-            if (a2 >= 1.0)
-            {
-                ux = uy = 0.0;
-                a2 = 0.0;
-                invalid = true;
-            }
-            const double f = 1.0 / (1.0 - a2);
-            dc oldk[C], oldf[C];
-            const dc a0 = A[0];
-#pragma unroll
-            for (int s = 0; s < C; ++s)
-            {
-                const int k = g + 8 * s;
-                if (k >= 1 && k < j)
-                {
-                    oldk[s] = A[k];
-                    oldf[s] = A[j - k];
-                }
-            }
-            wave_sync();
-#pragma unroll
-            for (int s = 0; s < C; ++s)
-            {
-                const int k = g + 8 * s;
-                if (k >= 1 && k < j)
-                {
-                    // A_k <- f (A_k - conj(A_{j-k}) u)
-                    const dc c = oldf[s];
-                    const double px = c.x * ux + c.y * uy;
-                    const double py = c.x * uy - c.y * ux;
-                    A[k] = dc{f * (oldk[s].x - px), f * (oldk[s].y - py)};
-                }
-            }
-            if (g == (j & 7))
-                A[j] = dc{f * (-a0.x * ux), f * (-a0.x * uy)};
-            if (g == 0)
-                A[0] = dc{f * a0.x, 0.0};
-        }
-        wave_sync();
-    }
+    levinson_octaves<1, C>(A, E, g, nm, invalid);
 
     // autocorrelation of P = 2 pi A (computeAutocorrelation), written over A
     const double a0x = (nm > 0) ? A[0].x : 1.0;
@@ -298,7 +326,7 @@ int launch_prepare(const float *moments, const uint32_t *index, int W, int H, in
         return 0;
     }
     const int C = (m + 7) / 8;
-    const size_t lds = (size_t)8 * 8 * C * (sizeof(double) + 2 * sizeof(dc));
+    const size_t lds = (size_t)8 * 8 * C * 2 * sizeof(dc);
     if (lds > g_lds_limit)
         return MRAY_ERR_UNSUPPORTED;
     dim3 grid((npix + 7) / 8);
