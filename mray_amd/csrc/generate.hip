@@ -443,62 +443,132 @@ static int plain_block(int m) { return (256 * m * (int)sizeof(float) <= 65536) ?
 // ADJOINT (F32 volumes): dL/dvolume from dL/dmoments
 // ---------------------------------------------------------------------------
 
-// Accumulates gs times the trilinear weights of sample_volume at a position
-// into the voxels it reads. Consecutive samples of a ray that fall into the
-// same cell are merged in registers; a cell's eight sums are added to the
-// gradient (float atomics) when the ray leaves it. Clamp-to-edge duplicates
-// receive both of their weights.
-struct CellScatter
+// A trilinear sample of the march that keeps what its reverse needs: the
+// cell, the weights and the voxel values (same arithmetic as sample_volume;
+// loads issued in issue(), used in value()).
+struct GradSample
 {
-    size_t base[4] = {0, 0, 0, 0}; // row offsets r00, r01, r10, r11
-    int x0 = -1, x1 = -1;
-    float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    bool live = false;
+    float ax, ay, az;
+    int cx, cy, cz; // clamped (x0, y0, z0)
+    uint32_t fl;    // bit 0/1/2: x1 > x0, y1 > y0, z1 > z0 (0 at a clamped edge)
+    float v[8];
 
-    __device__ __forceinline__ void flush(float *grad)
+    __device__ __forceinline__ void issue(const VolumeView<float> &vol, float px, float py, float pz)
     {
-        if (!live)
-            return;
-        unsafeAtomicAdd(&grad[base[0] + x0], w[0]);
-        unsafeAtomicAdd(&grad[base[0] + x1], w[1]);
-        unsafeAtomicAdd(&grad[base[1] + x0], w[2]);
-        unsafeAtomicAdd(&grad[base[1] + x1], w[3]);
-        unsafeAtomicAdd(&grad[base[2] + x0], w[4]);
-        unsafeAtomicAdd(&grad[base[2] + x1], w[5]);
-        unsafeAtomicAdd(&grad[base[3] + x0], w[6]);
-        unsafeAtomicAdd(&grad[base[3] + x1], w[7]);
-        live = false;
+        const float ipx = div_rn(px - vol.dmin.x, vol.dmax.x - vol.dmin.x, vol.rext.x) * (float)vol.nx;
+        const float ipy = div_rn(py - vol.dmin.y, vol.dmax.y - vol.dmin.y, vol.rext.y) * (float)vol.ny;
+        const float ipz = div_rn(pz - vol.dmin.z, vol.dmax.z - vol.dmin.z, vol.rext.z) * (float)vol.nz;
+        const float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
+        ax = ipx - flx;
+        ay = ipy - fly;
+        az = ipz - flz;
+        const int xl = (int)flx, yl = (int)fly, zl = (int)flz;
+        cx = iclamp(xl, 0, vol.nx - 1);
+        cy = iclamp(yl, 0, vol.ny - 1);
+        cz = iclamp(zl, 0, vol.nz - 1);
+        const int x1 = iclamp(xl + 1, 0, vol.nx - 1), y1 = iclamp(yl + 1, 0, vol.ny - 1),
+                  z1 = iclamp(zl + 1, 0, vol.nz - 1);
+        fl = (uint32_t)(x1 > cx) | ((uint32_t)(y1 > cy) << 1) | ((uint32_t)(z1 > cz) << 2);
+        const uint32_t sxy = (uint32_t)(vol.ny * vol.nx);
+        const uint32_t dx = fl & 1u, dy = (fl & 2u) ? (uint32_t)vol.nx : 0u;
+        const float *p0 = vol.data + ((size_t)cz * sxy + (uint32_t)(cy * vol.nx + cx));
+        const float *p1 = p0 + ((fl & 4u) ? sxy : 0u);
+        v[0] = p0[0];
+        v[1] = p0[dx];
+        v[2] = p0[dy];
+        v[3] = p0[dy + dx];
+        v[4] = p1[0];
+        v[5] = p1[dx];
+        v[6] = p1[dy];
+        v[7] = p1[dy + dx];
     }
 
-    __device__ __forceinline__ void add(float *grad, const VolumeView<float> &v, float px, float py, float pz, float gs)
+    __device__ __forceinline__ float value() const
     {
-        float ipx = ((px - v.dmin.x) / (v.dmax.x - v.dmin.x)) * (float)v.nx;
-        float ipy = ((py - v.dmin.y) / (v.dmax.y - v.dmin.y)) * (float)v.ny;
-        float ipz = ((pz - v.dmin.z) / (v.dmax.z - v.dmin.z)) * (float)v.nz;
-        float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
-        float ax = ipx - flx, ay = ipy - fly, az = ipz - flz;
-        int xl = (int)flx, yl = (int)fly, zl = (int)flz;
-        int nx0 = iclamp(xl, 0, v.nx - 1), nx1 = iclamp(xl + 1, 0, v.nx - 1);
-        int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
-        int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
-        size_t sxy = (size_t)(v.ny * v.nx);
-        size_t r00 = (size_t)z0 * sxy + (size_t)(y0 * v.nx);
-        size_t r01 = (size_t)z0 * sxy + (size_t)(y1 * v.nx);
-        size_t r10 = (size_t)z1 * sxy + (size_t)(y0 * v.nx);
-        size_t r11 = (size_t)z1 * sxy + (size_t)(y1 * v.nx);
-        if (!(live && nx0 == x0 && nx1 == x1 && r00 == base[0] && r01 == base[1] && r10 == base[2] && r11 == base[3]))
+        return (1.f - ax) * (1.f - ay) * (1.f - az) * v[0] + ax * (1.f - ay) * (1.f - az) * v[1] +
+               (1.f - ax) * ay * (1.f - az) * v[2] + ax * ay * (1.f - az) * v[3] + (1.f - ax) * (1.f - ay) * az * v[4] +
+               ax * (1.f - ay) * az * v[5] + (1.f - ax) * ay * az * v[6] + ax * ay * az * v[7];
+    }
+};
+
+// Where the gradient of a cell's eight voxels goes: a workgroup's LDS window
+// over a box of voxels (summed there, added to the volume once per voxel and
+// chunk of the march) or, for corners outside the window, straight to the
+// volume with float atomics.
+struct GradTarget
+{
+    float *grad;
+    float *win; // LDS window, x fastest
+    int ox, oy, oz, ex, ey, ez;
+    bool lds;
+    int nx;
+    uint32_t sxy;
+
+    __device__ __forceinline__ void add8(int cx, int cy, int cz, uint32_t fl, const float (&w)[8]) const
+    {
+        const int fx = fl & 1, fy = (fl >> 1) & 1, fz = (fl >> 2) & 1;
+        const int lx = cx - ox, ly = cy - oy, lz = cz - oz;
+        if (lds && lx >= 0 && ly >= 0 && lz >= 0 && lx + fx < ex && ly + fy < ey && lz + fz < ez)
         {
-            flush(grad);
-            x0 = nx0;
-            x1 = nx1;
-            base[0] = r00;
-            base[1] = r01;
-            base[2] = r10;
-            base[3] = r11;
+            float *q0 = win + (lx + ex * (ly + ey * lz)), *q1 = q0 + fz * ex * ey;
+            const int dy = fy * ex;
+            atomicAdd(q0, w[0]);
+            atomicAdd(q0 + fx, w[1]);
+            atomicAdd(q0 + dy, w[2]);
+            atomicAdd(q0 + dy + fx, w[3]);
+            atomicAdd(q1, w[4]);
+            atomicAdd(q1 + fx, w[5]);
+            atomicAdd(q1 + dy, w[6]);
+            atomicAdd(q1 + dy + fx, w[7]);
+            return;
+        }
+        float *p0 = grad + ((size_t)cz * sxy + (uint32_t)(cy * nx + cx)), *p1 = p0 + (fz ? sxy : 0u);
+        const uint32_t dy = fy ? (uint32_t)nx : 0u;
+        unsafeAtomicAdd(p0, w[0]);
+        unsafeAtomicAdd(p0 + fx, w[1]);
+        unsafeAtomicAdd(p0 + dy, w[2]);
+        unsafeAtomicAdd(p0 + dy + fx, w[3]);
+        unsafeAtomicAdd(p1, w[4]);
+        unsafeAtomicAdd(p1 + fx, w[5]);
+        unsafeAtomicAdd(p1 + dy, w[6]);
+        unsafeAtomicAdd(p1 + dy + fx, w[7]);
+    }
+};
+
+// Accumulates gs times the trilinear weights of a sample. Consecutive
+// samples of a ray in the same cell are merged in registers; a cell's eight
+// sums go to the target when the ray leaves it (or at a flush). Clamp-to-edge
+// duplicates receive both weights.
+struct CellScatter
+{
+    int cx = -1, cy = 0, cz = 0;
+    uint32_t fl = 0;
+    float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    __device__ __forceinline__ void flush(const GradTarget &tg)
+    {
+        if (cx < 0)
+            return;
+        tg.add8(cx, cy, cz, fl, w);
+        cx = -1;
+    }
+
+    __device__ __forceinline__ void add(const GradTarget &tg, const GradSample &sm, float gs)
+    {
+        // the same corner with other neighbour flags is another cell (a
+        // sample just outside the domain clamps onto the edge voxel)
+        if (!(sm.cx == cx && sm.cy == cy && sm.cz == cz && sm.fl == fl))
+        {
+            flush(tg);
+            cx = sm.cx;
+            cy = sm.cy;
+            cz = sm.cz;
+            fl = sm.fl;
+#pragma unroll
             for (int i = 0; i < 8; ++i)
                 w[i] = 0.0f;
-            live = true;
         }
+        const float ax = sm.ax, ay = sm.ay, az = sm.az;
         w[0] += gs * ((1.f - ax) * (1.f - ay) * (1.f - az));
         w[1] += gs * (ax * (1.f - ay) * (1.f - az));
         w[2] += gs * ((1.f - ax) * ay * (1.f - az));
@@ -510,190 +580,321 @@ struct CellScatter
     }
 };
 
-// cos and sin of (a + d) from those of a, for the small differences between
-// consecutive node phases (FP64 Taylor terms up to d^9; sincos beyond 0.1)
-__device__ __forceinline__ void rotate_small(double &c, double &s, double a_plus_d, double d)
+// sin and cos of x in FP64 for |x| <= 4 (node phases lie in [-pi, 0]): the
+// nearest multiple k of pi/2 is taken off in two parts (the high part has
+// 33 significant bits, so k * hi is exact), then Taylor series to r^20 on
+// |r| <= pi/4 (truncation below 1e-21).
+__device__ __forceinline__ void sincos_small(double x, double &s, double &c)
 {
-    if (fabs(d) > 0.1)
-    {
-        sincos(a_plus_d, &s, &c);
-        return;
-    }
-    const double d2 = d * d;
-    const double cd = 1.0 - d2 * (0.5 - d2 * (1.0 / 24 - d2 * (1.0 / 720 - d2 * (1.0 / 40320))));
-    const double sd = d * (1.0 - d2 * (1.0 / 6 - d2 * (1.0 / 120 - d2 * (1.0 / 5040 - d2 * (1.0 / 362880)))));
-    const double nc = c * cd - s * sd;
-    const double ns = s * cd + c * sd;
-    c = nc;
-    s = ns;
+    const double k = rint(x * 0.63661977236758134);
+    const double r = fma(-k, 6.07710050650619224932e-11, fma(-k, 1.57079632673412561417e+00, x));
+    const double r2 = r * r;
+    double ps = 1.0 / 121645100408832000.0;   // 1/19!
+    ps = fma(ps, -r2, 1.0 / 355687428096000.0); // 1/17!
+    ps = fma(ps, -r2, 1.0 / 1307674368000.0);
+    ps = fma(ps, -r2, 1.0 / 6227020800.0);
+    ps = fma(ps, -r2, 1.0 / 39916800.0);
+    ps = fma(ps, -r2, 1.0 / 362880.0);
+    ps = fma(ps, -r2, 1.0 / 5040.0);
+    ps = fma(ps, -r2, 1.0 / 120.0);
+    ps = fma(ps, -r2, 1.0 / 6.0);
+    const double sr = fma(-r * r2, ps, r);
+    double pc = 1.0 / 2432902008176640000.0;  // 1/20!
+    pc = fma(pc, -r2, 1.0 / 6402373705728000.0); // 1/18!
+    pc = fma(pc, -r2, 1.0 / 20922789888000.0);
+    pc = fma(pc, -r2, 1.0 / 87178291200.0);
+    pc = fma(pc, -r2, 1.0 / 479001600.0);
+    pc = fma(pc, -r2, 1.0 / 3628800.0);
+    pc = fma(pc, -r2, 1.0 / 40320.0);
+    pc = fma(pc, -r2, 1.0 / 720.0);
+    pc = fma(pc, -r2, 1.0 / 24.0);
+    pc = fma(pc, -r2, 0.5);
+    const double cr = fma(-r2, pc, 1.0);
+    const int q = (int)k & 3;
+    s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
+    c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
 }
 
-// Reverse mode of the generate march for dL/dmoments (dense W*H*m): the
+// Reverse mode of the generate march for dL/dmoments (dense W*H*m). The
 // derivative of the exact integral of the reference's piecewise-linear
-// segments (its float end points, slope (f1 - f0) / dphi and last segment),
-// per segment [pp, pn] of length sg with denominator seg, r = sg / seg:
+// segment [pp, pn] (length sg, slope denominator seg, r = sg / seg) is
 //   d trig_j / d f1 =  (cos j pn - cos j pp) / (j^2 seg) + r sin(j pn) / j
 //   d trig_j / d f0 = -(cos j pn - cos j pp) / (j^2 seg) + (1 - r) sin(j pn) / j - sin(j pp) / j
-//   d trig_0 / d f1 = sg^2 / (2 seg),  d trig_0 / d f0 = sg - sg^2 / (2 seg)
-// then through map_density (slope 1 / (data_max - data_min) where it does not
-// clamp) and the trilinear weights to the voxels. The bounds are constants
-// (a first pass recomputes them for the normalisation). cos / sin of j x by
-// Chebyshev recurrences in FP64 from cos / sin of each segment's end points,
-// which are carried from node to node by exact phase differences. One
-// thread per ray, nothing stored per sample.
+//   d trig_0 / d f1 = sg^2 / (2 seg),  d trig_0 / d f0 = sg - sg^2 / (2 seg).
+// Summed over j with the incoming gradients g_j these need, at each end
+// point, C(phi) = sum g_j / j^2 cos(j phi) and S(phi) = sum g_j / j sin(j phi):
+// two Clenshaw sums in FP64 per node (their differences between neighbouring
+// nodes are small next to their values). A segment starts where the previous
+// one ended up to the reference's float rounding, pp = phi_k + delta; that
+// offset enters to first order, C(pp) = C_k - delta S_k and S(pp) = S_k +
+// delta S'_k, with S' = sum g_j cos(j phi) from a central difference of S
+// over the neighbouring nodes (delta is a few ulps, so its accuracy hardly
+// matters). Then through map_density (slope 1 / (data_max - data_min) where
+// it does not clamp) and the trilinear weights to the voxels. The bounds are
+// constants (a first pass recomputes them for the normalisation); cos and
+// sin of the node phases come from sincos_small. One thread per ray, nothing
+// stored per sample.
 template <int M>
 __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float> vol, DevCamera cam, GenArgs a,
                                                                const float *__restrict__ grad_mom,
                                                                float *__restrict__ grad_vol)
 {
+    // The workgroup's 256 rays advance in chunks of kChunk nodes. Per chunk
+    // they agree on the box of voxels they can touch; gradients go to an LDS
+    // window over that box (when it fits), which is then added to the volume
+    // with one atomic per voxel instead of one per ray, sample and corner.
+    constexpr int kWin = 8192; // floats of LDS window
+    constexpr int kChunk = 32; // nodes per chunk and ray
+    __shared__ float win[kWin];
+    __shared__ int bbox[6];
     const int tid = threadIdx.x;
+    for (int i = tid; i < kWin; i += blockDim.x)
+        win[i] = 0.0f;
+
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
-    if (tile >= ntiles)
-        return;
-    int x, y;
-    tile_pixel(tile, a.W, lane, x, y);
-    if (x >= a.W || y >= a.H)
-        return;
-    const int m = a.m;
-    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    int x = 0, y = 0;
+    bool active = tile < ntiles;
+    if (active)
+    {
+        tile_pixel(tile, a.W, lane, x, y);
+        active = x < a.W && y < a.H;
+    }
+    Ray r = pixel_ray(cam, a.W, a.H, active ? x : 0, active ? y : 0);
     float t_near, t_far;
     const bool hit = intersect_aabb(r, vol.dmin, vol.dmax, t_near, t_far);
     t_near = rmax(t_near, cam.znear);
-    if (!(hit && (t_far > 0)))
-        return;
-    const size_t pix = (size_t)y * a.W + x;
+    active = active && hit && (t_far > 0);
+    const int m = a.m;
+    const size_t pix = active ? (size_t)y * a.W + x : 0;
     const float s = a.step;
     const float pi_trange_inv = (float)(kPi / (double)(t_far - t_near));
     const float dphi = (float)(kPi * (double)(s / (t_far - t_near)));
-    const int cap = march_cap(t_near, t_far, s);
-    auto raw = [&](float t, float &px, float &py, float &pz) {
-        px = r.o.x + t * r.d.x;
-        py = r.o.y + t * r.d.y;
-        pz = r.o.z + t * r.d.z;
-        return sample_volume(vol, px, py, pz);
+    const int cap = active ? march_cap(t_near, t_far, s) : 0;
+    auto issue_at = [&](GradSample &sm, float tt) {
+        sm.issue(vol, r.o.x + tt * r.d.x, r.o.y + tt * r.d.y, r.o.z + tt * r.d.z);
     };
+    auto density = [&](const GradSample &sm) { return map_density_r(sm.value(), a.data_min, a.data_max, a.data_rinv); };
 
     // normalisation of the forward: inv = 1 / (hi - lo) with the bounds of the ray
     double inv = 1.0;
-    if (a.transform_to_bounds)
+    if (active && a.transform_to_bounds)
     {
-        float lo = 1.f, hi = 0.f, px, py, pz;
+        float lo = 1.f, hi = 0.f;
+        GradSample sm;
         float t = t_near;
-        float f = map_density(raw(t, px, py, pz), a.data_min, a.data_max);
-        lo = rmin(lo, f);
-        hi = rmax(hi, f);
-        for (int it = 0; t + s < t_far && it < cap; ++it)
+        issue_at(sm, t);
+        for (int it = 0;; ++it)
         {
-            t += s;
-            f = map_density(raw(t, px, py, pz), a.data_min, a.data_max);
+            const bool more = t + s < t_far && it < cap;
+            const float f = density(sm);
             lo = rmin(lo, f);
             hi = rmax(hi, f);
+            if (!more && t == t_far)
+                break;
+            t = more ? t + s : t_far;
+            __builtin_amdgcn_sched_barrier(0);
+            issue_at(sm, t);
+            if (!more)
+            {
+                const float fl = density(sm);
+                lo = rmin(lo, fl);
+                hi = rmax(hi, fl);
+                break;
+            }
         }
-        f = map_density(raw(t_far, px, py, pz), a.data_min, a.data_max);
-        lo = rmin(lo, f);
-        hi = rmax(hi, f);
         lo = rmax(lo - a.bound_eps, 0.0f);
         hi = rmin(hi + a.bound_eps, 1.0f);
         inv = 1.0 / ((double)hi - (double)lo);
     }
-    // dL/dtrig_j = dL/dmoment_j * inv / pi (the factors 1/j and 1/j^2 are
-    // float literals of the unrolled loop)
-    float gj[M];
+    // dL/dtrig_j = dL/dmoment_j * inv / pi
+    // Chebyshev coefficients g_j / j^2 of C in FP64 registers
+    double cc[M];
     const double scale = inv / kPi;
-    const double g0 = (double)grad_mom[pix * m] * scale;
+    const double g0 = active ? (double)grad_mom[pix * m] * scale : 0.0;
+    cc[0] = 0.0;
 #pragma unroll
     for (int j = 1; j < M; ++j)
-        gj[j] = j < m ? (float)((double)grad_mom[pix * m + j] * scale) : 0.0f;
+        cc[j] = (active && j < m) ? (double)grad_mom[pix * m + j] * scale / (double)(j * j) : 0.0;
+    // C and S at a node with cos / sin (co, si) of its phase: Clenshaw's
+    // recurrence b_l = c_l + 2x b_{l+1} - b_{l+2} gives C = x b_1 - b_2, and
+    // its derivative in x, b'_l = 2 b_{l+1} + 2x b'_{l+1} - b'_{l+2}, gives
+    // S = -dC/dphi = sin(phi) dC/dx = sin(phi) (b_1 + x b'_1 - b'_2), since
+    // S = sum j c_j sin(j phi) for the same coefficients c_j = g_j / j^2.
+    auto sums = [&](double co, double si, double &C, double &S) {
+        const double x2 = co + co;
+        double b1 = 0.0, b2 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int l = M - 1; l >= 1; --l)
+        {
+            const double tb = fma(x2, b1, cc[l] - b2);
+            const double td = fma(x2, d1, fma(2.0, b1, -d2));
+            b2 = b1;
+            b1 = tb;
+            d2 = d1;
+            d1 = td;
+        }
+        C = fma(co, b1, -b2);
+        S = si * (b1 + fma(co, d1, -d2));
+    };
     const float dscale = 1.0f / (a.data_max - a.data_min);
     // d f / d s of map_density
     auto dmap = [&](float sv) {
-        const float u = (sv - a.data_min) / (a.data_max - a.data_min);
+        const float u = div_rn(sv - a.data_min, a.data_max - a.data_min, a.data_rinv);
         return (u > 0.0f && u < 1.0f) ? dscale : 0.0f;
     };
+    // clamped cell of the sample at t
+    auto cell_at = [&](float tt, int &qx, int &qy, int &qz) {
+        const float px = r.o.x + tt * r.d.x, py = r.o.y + tt * r.d.y, pz = r.o.z + tt * r.d.z;
+        qx = iclamp((int)floorf(div_rn(px - vol.dmin.x, vol.dmax.x - vol.dmin.x, vol.rext.x) * (float)vol.nx), 0,
+                    vol.nx - 1);
+        qy = iclamp((int)floorf(div_rn(py - vol.dmin.y, vol.dmax.y - vol.dmin.y, vol.rext.y) * (float)vol.ny), 0,
+                    vol.ny - 1);
+        qz = iclamp((int)floorf(div_rn(pz - vol.dmin.z, vol.dmax.z - vol.dmin.z, vol.rext.z) * (float)vol.nz), 0,
+                    vol.nz - 1);
+    };
 
-    float t = t_near, p = (float)(-kPi);
-    float px0, py0, pz0;
-    float s0 = raw(t, px0, py0, pz0);
-    double fbar0 = 0.0; // pending gradient of the node behind the current segment
-    CellScatter cells;
-    // cos / sin of the previous node's phase, carried forward by exact
-    // differences (consecutive phases differ by ~dphi, or by ulps)
-    double ph_prev = (double)p, cprev, sprev;
-    sincos(ph_prev, &sprev, &cprev);
-    for (int it = 0;; ++it)
+    const double rdphi = 1.0 / (double)dphi;
+    // node 0: the ray's first sample, at the start of the first segment
+    // (p_1 - dphi in float, or -pi when the ray has no interior sample)
+    float t = t_near;
+    float phi_prev;
+    if (t + s < t_far && cap > 0)
+        phi_prev = (float)((double)((t + s - t_near) * pi_trange_inv) - kPi) - dphi;
+    else
+        phi_prev = (float)(-kPi);
+    GradSample cur, nxt;
+    double co = 1.0, si = 0.0, C_prev = 0.0, S_prev = 0.0;
+    float raw_prev = 0.0f;
+    if (active)
     {
-        const bool last = !(t + s < t_far) || it >= cap;
-        float p_next, seg, px1, py1, pz1, s1;
-        if (!last)
-        {
-            t += s;
-            p_next = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
-            seg = dphi;
-            s1 = raw(t, px1, py1, pz1);
-        }
-        else
-        {
-            p_next = 0.0f;
-            seg = 0.0f - p;
-            s1 = raw(t_far, px1, py1, pz1);
-        }
-        const double pn = p_next, pp = (double)(p_next - seg), sg = pn - pp;
-        double cp1 = cprev, sp1 = sprev;
-        rotate_small(cp1, sp1, pp, pp - ph_prev);
-        double cn1 = cp1, sn1 = sp1;
-        rotate_small(cn1, sn1, pn, sg);
-        ph_prev = pn;
-        cprev = cn1;
-        sprev = sn1;
-        // X = sum g_j / j^2 (cos j pn - cos j pp), V = sum g_j / j (sin j pn -
-        // sin j pp), Yn = sum g_j / j sin j pn: the differences are formed in FP64 (they are
-        // small next to their terms), the weighted sums in FP32
-        float X = 0.0f, V = 0.0f, Yn = 0.0f;
-        double cn = cn1, sn = sn1, cp = cp1, sp = sp1;      // j = 1
-        double cnm = 1.0, snm = 0.0, cpm = 1.0, spm = 0.0;  // j = 0
-#pragma unroll
-        for (int j = 1; j < M; ++j)
-        {
-            const float g1 = gj[j] * (1.0f / (float)j);
-            X = __builtin_fmaf(g1 * (1.0f / (float)j), (float)(cn - cp), X);
-            V = __builtin_fmaf(g1, (float)(sn - sp), V);
-            Yn = __builtin_fmaf(g1, (float)sn, Yn);
-            const double cnn = fma(2.0 * cn1, cn, -cnm), snn = fma(2.0 * cn1, sn, -snm);
-            const double cpn = fma(2.0 * cp1, cp, -cpm), spn = fma(2.0 * cp1, sp, -spm);
-            cnm = cn;
-            snm = sn;
-            cpm = cp;
-            spm = sp;
-            cn = cnn;
-            sn = snn;
-            cp = cpn;
-            sp = spn;
-        }
-        const double segd = (double)seg;
-        const double rr = sg / segd;
-        const double half = 0.5 * sg * sg / segd;
-        // A = -X / seg + (1 - r) Y - Z with Y - Z = V, Y = Yn
-        const double A = -(double)X / segd + (double)V - rr * (double)Yn + g0 * (sg - half);
-        const double B = (double)X / segd + rr * (double)Yn + g0 * half;
-        fbar0 += A;
-        const float gs0 = (float)fbar0 * dmap(s0);
-        if (gs0 != 0.0f)
-            cells.add(grad_vol, vol, px0, py0, pz0, gs0);
-        fbar0 = B;
-        s0 = s1;
-        px0 = px1;
-        py0 = py1;
-        pz0 = pz1;
-        p = p_next;
-        if (last)
-            break;
+        issue_at(cur, t);
+        sincos_small((double)phi_prev, si, co);
+        sums(co, si, C_prev, S_prev);
+        raw_prev = cur.value();
     }
-    const float gs0 = (float)fbar0 * dmap(s0);
-    if (gs0 != 0.0f)
-        cells.add(grad_vol, vol, px0, py0, pz0, gs0);
-    cells.flush(grad_vol);
+    double S_prev2 = S_prev, phi_prev2 = (double)phi_prev; // node k-2 for S'
+    double fbar = 0.0; // pending gradient of the node behind the current segment
+    CellScatter cells;
+    GradTarget tg;
+    tg.grad = grad_vol;
+    tg.win = win;
+    tg.nx = vol.nx;
+    tg.sxy = (uint32_t)(vol.nx * vol.ny);
+    int it = 0;
+    bool done = !active;
+    while (__syncthreads_or(!done))
+    {
+        if (tid == 0)
+        {
+            bbox[0] = bbox[1] = bbox[2] = 0x7fffffff;
+            bbox[3] = bbox[4] = bbox[5] = -1;
+        }
+        __syncthreads();
+        if (!done)
+        {
+            // this chunk reads samples in [t, t + (kChunk + 1) s], or up to t_far
+            // when it ends the ray; one cell of margin on each side
+            float tb = t + (float)(kChunk + 1) * s;
+            if (tb > t_far || it + kChunk >= cap)
+                tb = t_far;
+            int ax0, ay0, az0, ax1, ay1, az1;
+            cell_at(t, ax0, ay0, az0);
+            cell_at(tb, ax1, ay1, az1);
+            atomicMin(&bbox[0], max(min(ax0, ax1) - 1, 0));
+            atomicMin(&bbox[1], max(min(ay0, ay1) - 1, 0));
+            atomicMin(&bbox[2], max(min(az0, az1) - 1, 0));
+            atomicMax(&bbox[3], min(max(ax0, ax1) + 2, vol.nx - 1));
+            atomicMax(&bbox[4], min(max(ay0, ay1) + 2, vol.ny - 1));
+            atomicMax(&bbox[5], min(max(az0, az1) + 2, vol.nz - 1));
+        }
+        __syncthreads();
+        tg.ox = __builtin_amdgcn_readfirstlane(bbox[0]);
+        tg.oy = __builtin_amdgcn_readfirstlane(bbox[1]);
+        tg.oz = __builtin_amdgcn_readfirstlane(bbox[2]);
+        tg.ex = __builtin_amdgcn_readfirstlane(bbox[3]) - tg.ox + 1;
+        tg.ey = __builtin_amdgcn_readfirstlane(bbox[4]) - tg.oy + 1;
+        tg.ez = __builtin_amdgcn_readfirstlane(bbox[5]) - tg.oz + 1;
+        tg.lds = tg.ex > 0 && tg.ey > 0 && tg.ez > 0 && (long long)tg.ex * tg.ey * tg.ez <= kWin;
+        for (int k = 0; k < kChunk && !done; ++k, ++it)
+        {
+            const bool last = !(t + s < t_far) || it >= cap;
+            float p_next, seg;
+            if (!last)
+            {
+                t += s;
+                p_next = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
+                seg = dphi;
+                issue_at(nxt, t);
+            }
+            else
+            {
+                p_next = 0.0f;
+                seg = 0.0f - phi_prev;
+                issue_at(nxt, t_far);
+            }
+            // the segment [pp, pn]: pp = p_next - seg in float (= phi_prev + delta)
+            const double pn = (double)p_next;
+            const double pp = (double)(p_next - seg);
+            const double sg = pn - pp;
+            const double delta = pp - (double)phi_prev;
+            sincos_small(pn, si, co);
+            double Cn, Sn;
+            sums(co, si, Cn, Sn);
+            // S' at the previous node by a central difference (one-sided at the
+            // first); it only multiplies delta, a few ulps, so float suffices
+            const float dS = __fdividef((float)((it == 0) ? Sn - S_prev : Sn - S_prev2),
+                                        (float)((it == 0) ? pn - (double)phi_prev : pn - phi_prev2));
+            const double Cpp = C_prev - delta * S_prev;
+            const double Spp = S_prev + delta * (double)dS;
+            const double X = Cn - Cpp, V = Sn - Spp;
+            const double rseg = last ? 1.0 / (double)seg : rdphi;
+            const double xs = X * rseg;
+            const double rr = sg * rseg;
+            const double half = 0.5 * sg * rr;
+            const double A = -xs + V - rr * Sn + g0 * (sg - half);
+            const double B = xs + rr * Sn + g0 * half;
+            fbar += A;
+            const float gs0 = (float)fbar * dmap(raw_prev);
+            if (gs0 != 0.0f)
+                cells.add(tg, cur, gs0);
+            fbar = B;
+            S_prev2 = S_prev;
+            phi_prev2 = (double)phi_prev;
+            C_prev = Cn;
+            S_prev = Sn;
+            phi_prev = p_next;
+            cur = nxt;
+            raw_prev = cur.value();
+            if (last)
+            {
+                const float gl = (float)fbar * dmap(raw_prev);
+                if (gl != 0.0f)
+                    cells.add(tg, cur, gl);
+                done = true;
+            }
+        }
+        cells.flush(tg);
+        __syncthreads();
+        if (tg.lds)
+        {
+            const int n = tg.ex * tg.ey * tg.ez;
+            for (int i = tid; i < n; i += blockDim.x)
+            {
+                const float v = win[i];
+                if (v != 0.0f)
+                {
+                    win[i] = 0.0f;
+                    const int lx = i % tg.ex, rest = i / tg.ex;
+                    const int ly = rest % tg.ey, lz = rest / tg.ey;
+                    unsafeAtomicAdd(grad_vol + ((size_t)(tg.oz + lz) * tg.sxy +
+                                                (uint32_t)((tg.oy + ly) * vol.nx + tg.ox + lx)),
+                                    v);
+                }
+            }
+        }
+    }
 }
 
 int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const float *dmin, const float *dmax,

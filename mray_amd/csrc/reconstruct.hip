@@ -175,7 +175,7 @@ __device__ __forceinline__ float sin_quarter(float x)
 }
 
 template <int M, typename L>
-__device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p)
+__device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p, double *cos_theta = nullptr)
 {
     // lambda_0 + 2 sum_{l=1}^{M-1} lambda_l cos(l theta), theta = -p in [0, pi]
     // (entries l >= m hold 0), by Clenshaw's recurrence in FP64 (2 operations
@@ -193,6 +193,8 @@ __device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p)
     const double q = (double)sin_quarter(xq);
     const double q2 = 2.0 * q * q;
     const double x = reflect ? q2 - 1.0 : 1.0 - q2;
+    if (cos_theta)
+        *cos_theta = x;
     const double x2 = x + x;
     double b1 = 0.0, b2 = 0.0;
 #pragma unroll
@@ -432,13 +434,28 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                 };
                 float fs = rho_of((float)series_clenshaw<M>(lam, rclamp(p, (float)(-kPi), 0.0f)));
                 double pre[3] = {0.0, 0.0, 0.0};
+                // table gradient of the current entry pair (x0, x1), not yet in gtf
+                int tf_x0 = -1, tf_x1 = -1;
+                float tf_a[4] = {0.f, 0.f, 0.f, 0.f}, tf_b[4] = {0.f, 0.f, 0.f, 0.f};
+                auto tf_flush = [&]() {
+                    if (tf_x0 < 0)
+                        return;
+#pragma unroll
+                    for (int ch = 0; ch < 4; ++ch)
+                    {
+                        atomicAdd(&gtf[tf_x0 * 4 + ch], tf_a[ch]);
+                        atomicAdd(&gtf[tf_x1 * 4 + ch], tf_b[ch]);
+                        tf_a[ch] = tf_b[ch] = 0.0f;
+                    }
+                };
                 const int cap = march_cap(t_near, t_far, a.step) + 1;
                 for (int it = 0; t <= t_far && it < cap; ++it)
                 {
                     t += a.step;
                     p += dphi;
                     const float pc = rclamp(p, (float)(-kPi), 0.0f);
-                    const float S = (float)series_clenshaw<M>(lam, pc);
+                    double c1;
+                    const float S = (float)series_clenshaw<M>(lam, pc, &c1);
                     const float bs = rho_of(S);
                     const float4 c = tf_sample(tf, fs, bs);
 
@@ -461,11 +478,22 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                                         (double)g.w * t_end;
                     if (grad_tf)
                     {
+                        // consecutive samples that read the same table pair are
+                        // summed in registers; the LDS atomics (contended: the
+                        // rays of a wavefront read similar entries) run when
+                        // the pair changes
+                        if (x0 != tf_x0 || x1 != tf_x1)
+                        {
+                            tf_flush();
+                            tf_x0 = x0;
+                            tf_x1 = x1;
+                        }
                         const float gv[4] = {(float)gc[0], (float)gc[1], (float)gc[2], (float)gsig};
+#pragma unroll
                         for (int ch = 0; ch < 4; ++ch)
                         {
-                            atomicAdd(&gtf[x0 * 4 + ch], gv[ch] * w01);
-                            atomicAdd(&gtf[x1 * 4 + ch], gv[ch] * w23);
+                            tf_a[ch] = __builtin_fmaf(gv[ch], w01, tf_a[ch]);
+                            tf_b[ch] = __builtin_fmaf(gv[ch], w23, tf_b[ch]);
                         }
                     }
                     const double drho = 256.0 * (gc[0] * (double)(hi4.x - lo4.x) + gc[1] * (double)(hi4.y - lo4.y) +
@@ -474,7 +502,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     if (dS != 0.0f)
                     {
                         // cos(l theta), theta = -pc, by the Chebyshev recurrence
-                        const double c1 = cos((double)pc);
+                        // from the series' own cos(theta)
                         double cm = 1.0, cl = c1;
                         glam[0] += dS;
 #pragma unroll
@@ -492,10 +520,17 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     if (1.0f - T >= a.ert)
                         break;
                 }
+                if (grad_tf)
+                    tf_flush();
             }
             if (grad_lag)
-                for (int l = 0; l < nm; ++l)
-                    grad_lag[off + l] = l == 0 ? glam[0] : 2.0f * glam[l];
+            {
+                // static indices keep glam in registers
+#pragma unroll
+                for (int l = 0; l < M; ++l)
+                    if (l < nm)
+                        grad_lag[off + l] = l == 0 ? glam[0] : 2.0f * glam[l];
+            }
         }
     }
     __syncthreads();
