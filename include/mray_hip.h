@@ -130,9 +130,10 @@ int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mr
 /* Reverse mode of mray_hip_prepare (MIXED mathematics, FP64): given
  * dL/dlagrange, writes dL/dmoments (layout of d_moments) for the bias lerp,
  * Herglotz transform, Levinson recursion (including its |u|^2 >= 1 guard)
- * and correlations. Uses stream-ordered scratch (hipMallocAsync) of
- * 112 x m bytes per pixel, at most 1,048,576 pixels' worth (1.2 GB at 40
- * moments); larger images are processed in turns by the same threads. */
+ * and correlations. Up to 64 moments it needs no memory beyond its
+ * arguments; above 64 it uses stream-ordered scratch (hipMallocAsync) of
+ * 112 x m bytes per pixel, at most 1,048,576 pixels' worth; larger images
+ * are processed in turns by the same threads. */
 int mray_hip_prepare_backward(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
                               const float *d_grad_lagrange, float *d_grad_moments, mray_stream stream);
 

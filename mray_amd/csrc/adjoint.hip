@@ -14,19 +14,21 @@
 //    which inverts to a = a' + u conj(b'), b = b' + u conj(a'), so the
 //    previous iterate is rebuilt from the current one and u_j;
 //  * steps after the |u|^2 >= 1 guard fired use u = 0 (constant).
-// Up to 64 moments one wavefront holds a pixel, element k of every array in
-// lane k's registers (prepare_adjoint_wave_kernel). Beyond that, per-pixel
-// arrays live in a global scratch laid out [array][element][thread]
-// so that the lanes of a wavefront touch consecutive addresses; the grid is a
-// fixed number of threads (16 wavefronts per CU) that stride over the
-// pixels, which bounds the scratch to threads x 7 arrays x m x 16 bytes
-// (1.2 GB at 40 moments; fewer threads are launched for smaller images).
+// Up to 64 moments eight lanes hold a pixel, as in the forward kernel
+// (prepare_adjoint_grp_kernel: registers per element slot, LDS for arrays read
+// at other elements' offsets, DPP sums over the 8 lanes). Beyond that, one
+// thread per pixel keeps its arrays in a global scratch laid out
+// [array][element][thread] so that the lanes of a wavefront touch consecutive
+// addresses; the grid is a fixed number of threads (16 wavefronts per CU)
+// that stride over the pixels, which bounds the scratch to threads x 7
+// arrays x m x 16 bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "device_common.hpp"
 #include "launch.hpp"
+#include "mese_f64.hpp"
 
 namespace mray
 {
@@ -34,10 +36,6 @@ namespace mray
 namespace
 {
 
-struct dc
-{
-    double x, y;
-};
 __device__ __forceinline__ dc dmk(double x, double y) { return dc{x, y}; }
 __device__ __forceinline__ dc dadd(dc a, dc b) { return dc{a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ dc dsub(dc a, dc b) { return dc{a.x - b.x, a.y - b.y}; }
@@ -268,272 +266,346 @@ __global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const floa
 }
 
 // ---------------------------------------------------------------------------
-// Wavefront per pixel (m <= 64): element k of every array lives in lane k, in
-// registers. Partners such as P[j - k] or E[j - k] are fetched with
-// cross-lane shuffles, inner products are butterfly reductions over the
-// wavefront; nothing but the input and output touches memory.
+// Eight lanes per pixel (m <= 64), eight pixels per wavefront, as the forward
+// prepare_mixed_kernel: lane g holds elements g, g + 8, ... (C slots) of the
+// per-element registers (P, its adjoint Pb, Eb, Tb and the recorded
+// reflection coefficients U); arrays that lanes read at other elements'
+// offsets live in LDS per pixel (E, P and two work arrays X, Y of 8C complex
+// doubles; T overlays X at the ends). Sums over a pixel's elements reduce
+// over its 8 lanes (grp8_sum), single elements come from their lane by a
+// shuffle inside the group.
 // ---------------------------------------------------------------------------
 
-// lane value of a DPP move of a double (two 32-bit halves); lanes the move
-// has no source for read 0 (bound_ctrl)
-template <int CTRL, int ROW_MASK = 0xF> __device__ __forceinline__ double dpp_d(double v)
-{
-    int2 w = *reinterpret_cast<int2 *>(&v), o;
-    o.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, ROW_MASK, 0xF, true);
-    o.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, ROW_MASK, 0xF, true);
-    return *reinterpret_cast<double *>(&o);
-}
+// the value v of lane src of this lane's group of 8
+__device__ __forceinline__ double grp8_get(double v, int src) { return __shfl(v, (int)(threadIdx.x & ~7u) | src, 64); }
+__device__ __forceinline__ dc grp8_get(dc v, int src) { return dc{grp8_get(v.x, src), grp8_get(v.y, src)}; }
 
-// sum over the 64 lanes, returned to every lane: prefix sums inside each row
-// of 16 lanes by row_shr 1, 2, 4, 8, then row_bcast 15 / 31 carry the row
-// totals upwards (DPP, executed by the VALU), and lane 63 holds the total
-__device__ __forceinline__ double wsum(double v)
+template <int C> struct AdjRegs
 {
-    v += dpp_d<0x111>(v); // row_shr:1
-    v += dpp_d<0x112>(v); // row_shr:2
-    v += dpp_d<0x114>(v); // row_shr:4
-    v += dpp_d<0x118>(v); // row_shr:8
-    v += dpp_d<0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
-    v += dpp_d<0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
-    int2 w = *reinterpret_cast<int2 *>(&v), o;
-    o.x = __builtin_amdgcn_readlane(w.x, 63);
-    o.y = __builtin_amdgcn_readlane(w.y, 63);
-    return *reinterpret_cast<double *>(&o);
-}
-__device__ __forceinline__ dc wsum(dc v) { return dc{wsum(v.x), wsum(v.y)}; }
-// value of a wave-uniform lane (v_readlane)
-__device__ __forceinline__ double bcast(double v, int src)
-{
-    int2 w = *reinterpret_cast<int2 *>(&v), o;
-    o.x = __builtin_amdgcn_readlane(w.x, src);
-    o.y = __builtin_amdgcn_readlane(w.y, src);
-    return *reinterpret_cast<double *>(&o);
-}
-__device__ __forceinline__ dc bcast(dc v, int src) { return dc{bcast(v.x, src), bcast(v.y, src)}; }
-
-// Orders the LDS accesses of one wavefront (its lanes run in lockstep).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
-// Per-wavefront LDS copies of the arrays whose elements other lanes read at
-// per-lane offsets (j - k, k + kk, ...): a store of lane k's element and
-// masked reads of arbitrary elements.
-struct WaveArrays
-{
-    double *T;
-    dc *E, *Eh, *P, *R, *Q; // Q: scratch copy of the array being updated
+    dc P[C], Pb[C], Eb[C], U[C];
+    double Tb[C];
 };
-constexpr int kWaveLdsBytes = 64 * 8 + 5 * 64 * 16;
 
-__device__ __forceinline__ dc ld(const dc *a, int i, bool ok) { return ok ? a[i] : dc{0.0, 0.0}; }
-__device__ __forceinline__ double ld(const double *a, int i, bool ok) { return ok ? a[i] : 0.0; }
-
-__global__ __launch_bounds__(256) void prepare_adjoint_wave_kernel(const float *__restrict__ mom,
-                                                                   const uint32_t *__restrict__ index, int npix,
-                                                                   int m, float bias,
-                                                                   const float *__restrict__ grad_lag,
-                                                                   float *__restrict__ grad_mom)
+// Reverse Levinson steps j = min(8S - 1, nm - 1) down to max(1, 8(S - 1)),
+// which touch the elements k <= j, i.e. slots s < S; then the octave below.
+template <int S, int C>
+__device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, dc *Yl, dc *Xl, const dc *E, int g,
+                                                         int nm, int j_invalid)
 {
-    __shared__ __align__(16) unsigned char lds_raw[4 * kWaveLdsBytes];
-    const int k = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int pix = blockIdx.x * (blockDim.x / 64) + w;
-    if (pix >= npix) // uniform per wavefront
-        return;
-    const int nm = index ? min((int)(index[pix + 1] - index[pix]), m) : m;
-    const size_t off = index ? (size_t)index[pix] : (size_t)pix * m;
-    if (nm <= 0)
-        return;
-    unsigned char *base = lds_raw + w * kWaveLdsBytes;
-    WaveArrays L;
-    L.T = reinterpret_cast<double *>(base);
-    L.E = reinterpret_cast<dc *>(base + 64 * 8);
-    L.Eh = L.E + 64;
-    L.P = L.Eh + 64;
-    L.R = L.P + 64;
-    L.Q = L.R + 64;
-
-    const bool act = k < nm;
-    const double b64 = (double)bias;
-    const double T = act ? (1.0 - b64) * (double)mom[off + k] + (k == 0 ? b64 * 0.5 : 0.0) : 0.0;
-    const dc zero = dmk(0.0, 0.0);
-    L.T[k] = T;
-    wave_lds_sync();
-
-    // ---- forward ----
-    const double ph0 = kPi * bcast(T, 0) - 0.5 * kPi;
-    double s0, c0;
-    sincos(ph0, &s0, &c0);
-    dc E = (k == 0) ? dmk(c0 * (0.25 / kPi), s0 * (0.25 / kPi)) : zero;
-    for (int l = 1; l < nm; ++l)
+    const int hi = min(8 * S - 1, nm - 1);
+    for (int j = hi; j >= ((S == 1) ? 1 : 8 * (S - 1)); --j)
     {
-        const double Tl = ld(L.T, l - k, k < l);
-        const dc acc = wsum(drmul((double)(l - k) * Tl, E)); // E = 0 on lanes >= l
-        if (k == l)
-        {
-            const double c = 2.0 * kPi / (double)l;
-            E = dmk(-acc.y * c, acc.x * c);
-        }
-    }
-    const dc Eh = E; // E_0 before doubling
-    if (k == 0)
-        E = drmul(2.0, E);
-    L.E[k] = E;
-    L.Eh[k] = Eh;
-    wave_lds_sync();
-    const double e0x = bcast(E.x, 0);
-    dc P = (k == 0) ? dmk(1.0 / e0x, 0.0) : zero;
-    dc U = zero;
-    int j_invalid = nm;
-    for (int j = 1; j < nm; ++j)
-    {
-        L.P[k] = P;
-        wave_lds_sync();
-        const dc Ejk = ld(L.E, j - k, k < j);
-        dc u = (j < j_invalid) ? wsum(dmul(P, Ejk)) : zero;
-        double a2 = u.x * u.x + u.y * u.y;
-        if (j < j_invalid && a2 >= 1.0)
-            j_invalid = j;
-        if (j >= j_invalid)
-        {
-            u = zero;
-            a2 = 0.0;
-        }
-        if (k == j)
-            U = u;
-        const double f = 1.0 / (1.0 - a2);
-        const double p0 = bcast(P.x, 0);
-        const dc Pq = ld(L.P, j - k, k >= 1 && k < j);
-        wave_lds_sync();
-        if (k >= 1 && k < j)
-            P = drmul(f, dsub(P, dmul(dconj(Pq), u)));
-        else if (k == j)
-            P = drmul(-f * p0, u);
-        else if (k == 0)
-            P = dmk(f * p0, 0.0);
-    }
-    P = drmul(2.0 * kPi, P);
-    L.P[k] = P;
-    wave_lds_sync();
-    dc R = zero;
-    for (int kk = 0; kk < nm; ++kk)
-    {
-        const dc Pn = ld(L.P, k + kk, k + kk < nm);
-        const dc r = wsum(dmul(P, dconj(Pn)));
-        if (k == kk)
-            R = r;
-    }
-    L.R[k] = R;
-    wave_lds_sync();
-    const double p0s = bcast(P.x, 0);
-    const double nf = 1.0 / (kPi * p0s);
-
-    // ---- reverse: imaginary correlation and normalisation ----
-    const double lb = act ? (double)grad_lag[off + k] : 0.0;
-    double nfb = 0.0;
-    dc Rb = zero, Eb = zero;
-    for (int kk = 0; kk < nm; ++kk)
-    {
-        const bool in = k + kk < nm;
-        const dc Rn = ld(L.R, k + kk, in); // R_{kk+j} on lane j
-        const double Ck = wsum(Rn.x * Eh.y + Rn.y * Eh.x);
-        const double lbk = bcast(lb, kk);
-        nfb += lbk * Ck;
-        const double cb = lbk * nf;
-        // Rb_{kk+j} += cb (Eh_j.y, Eh_j.x): lane n takes Eh_{n-kk}
-        const dc Ehj = ld(L.Eh, k - kk, k >= kk && k < nm);
-        Rb = dadd(Rb, dmk(cb * Ehj.y, cb * Ehj.x));
-        // Eh_j += cb (R_{kk+j}.y, R_{kk+j}.x); E_0 = 2 Eh_0
-        const double h = (k == 0) ? 0.5 : 1.0;
-        Eb = dadd(Eb, dmk(h * cb * Rn.y, h * cb * Rn.x));
-    }
-    dc Pb = (k == 0) ? dmk(-nfb * nf / p0s, 0.0) : zero;
-    // R_kk = sum_j P_j conj(P_{j+kk}) (P is in L.P)
-    for (int kk = 0; kk < nm; ++kk)
-    {
-        const dc rb = bcast(Rb, kk);
-        const dc b = ld(L.P, k + kk, k + kk < nm);
-        const dc a = ld(L.P, k - kk, k >= kk && k < nm);
-        Pb = dadd(Pb, dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
-        Pb = dadd(Pb, dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
-    }
-    P = drmul(1.0 / (2.0 * kPi), P);
-    Pb = drmul(2.0 * kPi, Pb);
-
-    // ---- reverse: Levinson, rebuilding each previous iterate ----
-    dc *Pl = L.P, *Ql = L.Q, *Pbl = L.R; // L.R is free from here on
-    for (int j = nm - 1; j >= 1; --j)
-    {
-        const dc u = bcast(U, j);
+        const dc u = grp8_get(R.U[S - 1], j & 7);
         const bool valid = j < j_invalid;
-        const double f = 1.0 / (1.0 - (u.x * u.x + u.y * u.y));
-        const double fb = wsum((k <= j) ? ddot(Pb, P) : 0.0) / f;
-        const dc pjb = bcast(Pb, j);
-        const double p0b = bcast(Pb.x, 0);
-        const double p0 = bcast(P.x, 0) / f;
-        Pl[k] = P;
-        Pbl[k] = Pb;
-        wave_lds_sync();
-        const dc Pq = ld(Pl, j - k, k >= 1 && k < j);
-        const dc Pbq = ld(Pbl, j - k, k >= 1 && k < j);
-        dc Pold = P;
-        if (k >= 1 && k < j)
-            Pold = dadd(P, dmul(u, dconj(Pq)));
-        else if (k == 0)
-            Pold = dmk(p0, 0.0);
-        else if (k == j)
-            Pold = zero;
-        Ql[k] = Pold;
-        wave_lds_sync();
-        const dc Poq = ld(Ql, j - k, k <= j); // P_{j-k} of the previous iterate
-        wave_lds_sync();
-        dc ub = drmul(2.0 * f * f * fb, u);
-        ub = dsub(ub, drmul(f, wsum((k >= 1 && k < j) ? dmul(Pb, Poq) : zero)));
-        ub = dsub(ub, drmul(f * p0, pjb));
-        dc Pbn = Pb;
-        if (k >= 1 && k < j)
-            Pbn = drmul(f, dsub(Pb, dmul(dconj(Pbq), u)));
-        else if (k == 0)
-            Pbn = dmk(f * p0b - f * ddot(pjb, u), 0.0);
-        else if (k == j)
-            Pbn = zero;
-        if (valid)
+        const double f = recip_nr(1.0 - (u.x * u.x + u.y * u.y));
+        double fbs = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (g + 8 * s <= j)
+                fbs += ddot(R.Pb[s], R.P[s]);
+        const double fb = grp8_sum(fbs) / f;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
         {
-            // u = sum_{k<j} P_k E_{j-k}
-            const dc Ejk = ld(L.E, j - k, k < j);
-            Pbn = dadd(Pbn, dmul(ub, dconj(Ejk)));
-            if (k >= 1 && k <= j) // lane n = j - k takes P_k = Poq
-                Eb = dadd(Eb, dmul(ub, dconj(Poq)));
+            Pl[g + 8 * s] = R.P[s];
+            Yl[g + 8 * s] = R.Pb[s];
         }
-        P = Pold;
-        Pb = Pbn;
+        wave_sync();
+        const dc pjb = Yl[j];
+        const double p0b = Yl[0].x;
+        const double p0 = Pl[0].x / f;
+        dc Pq[S], Pbq[S], Pold[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+        {
+            const int k = g + 8 * s;
+            const int r = (k >= 1 && k < j) ? j - k : 0;
+            Pq[s] = Pl[r];
+            Pbq[s] = Yl[r];
+            if (k >= 1 && k < j)
+                Pold[s] = dadd(R.P[s], dmul(u, dconj(Pq[s])));
+            else if (k == 0)
+                Pold[s] = dmk(p0, 0.0);
+            else if (k == j)
+                Pold[s] = dmk(0.0, 0.0);
+            else
+                Pold[s] = R.P[s];
+            Xl[k] = Pold[s];
+        }
+        wave_sync();
+        dc Poq[S];
+        dc acc = dmk(0.0, 0.0);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+        {
+            const int k = g + 8 * s;
+            Poq[s] = Xl[(k <= j) ? j - k : 0]; // P_{j-k} of the previous iterate
+            if (k >= 1 && k < j)
+                acc = dadd(acc, dmul(R.Pb[s], Poq[s]));
+        }
+        wave_sync();
+        acc = dc{grp8_sum(acc.x), grp8_sum(acc.y)};
+        dc ub = drmul(2.0 * f * f * fb, u);
+        ub = dsub(ub, drmul(f, acc));
+        ub = dsub(ub, drmul(f * p0, pjb));
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+        {
+            const int k = g + 8 * s;
+            dc Pbn = R.Pb[s];
+            if (k >= 1 && k < j)
+                Pbn = drmul(f, dsub(R.Pb[s], dmul(dconj(Pbq[s]), u)));
+            else if (k == 0)
+                Pbn = dmk(f * p0b - f * ddot(pjb, u), 0.0);
+            else if (k == j)
+                Pbn = dmk(0.0, 0.0);
+            if (valid)
+            {
+                // u = sum_{k<j} P_k E_{j-k}
+                if (k < j)
+                    Pbn = dadd(Pbn, dmul(ub, dconj(E[j - k])));
+                if (k >= 1 && k <= j) // P_{j-k} of the previous iterate is Poq
+                    R.Eb[s] = dadd(R.Eb[s], dmul(ub, dconj(Poq[s])));
+            }
+            R.P[s] = Pold[s];
+            R.Pb[s] = Pbn;
+        }
     }
-    // P_0 = 1 / E_0.x (E_0 doubled), then the recurrence's undoubled E_0
-    if (k == 0)
+    if constexpr (S > 1)
+        levinson_reverse_octaves<S - 1, C>(R, Pl, Yl, Xl, E, g, nm, j_invalid);
+}
+
+// Reverse Herglotz steps l = min(8S - 1, nm - 1) down to max(1, 8(S - 1)):
+// E_l = i c_l sum_{j<l} (l - j) T_{l-j} E_j, c_l = 2 pi / l, touching the
+// elements k <= l (slots s < S); then the octave below.
+template <int S, int C>
+__device__ __forceinline__ void herglotz_reverse_octaves(AdjRegs<C> &R, const double *T, const dc *E, int g, int nm)
+{
+    const int hi = min(8 * S - 1, nm - 1);
+    for (int l = hi; l >= ((S == 1) ? 1 : 8 * (S - 1)); --l)
     {
-        Eb.x += -Pb.x / (e0x * e0x);
-        Eb = drmul(2.0, Eb);
-    }
-    // ---- reverse: Herglotz recurrence E_l = i c_l sum_j (l-j) T_{l-j} E_j ----
-    double Tb = 0.0;
-    for (int l = nm - 1; l >= 1; --l)
-    {
-        const dc Ebl = bcast(Eb, l);
+        const dc Ebl = grp8_get(R.Eb[S - 1], l & 7);
         const double c = 2.0 * kPi / (double)l;
         const dc accb = dmk(c * Ebl.y, -c * Ebl.x);
-        const double Tl = ld(L.T, l - k, k < l);
-        const dc El = ld(L.Eh, l - k, k >= 1 && k <= l); // E_{l-n} on lane n
-        if (k < l)
-            Eb = dadd(Eb, drmul((double)(l - k) * Tl, accb));
-        Tb += (double)k * ddot(accb, El);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+        {
+            const int k = g + 8 * s;
+            const int r = (k <= l) ? l - k : 0;
+            const double Tl = T[r];
+            dc El = E[r];
+            if (r == 0) // the recurrence ran on E_0 before its doubling
+                El = drmul(0.5, El);
+            if (k < l)
+                R.Eb[s] = dadd(R.Eb[s], drmul((double)(l - k) * Tl, accb));
+            if (k >= 1 && k <= l)
+                R.Tb[s] += (double)k * ddot(accb, El);
+        }
     }
-    if (k == 0)
-        Tb += 0.25 * (-Eb.x * s0 + Eb.y * c0);
-    if (act)
-        grad_mom[off + k] = (float)((1.0 - b64) * Tb);
+    if constexpr (S > 1)
+        herglotz_reverse_octaves<S - 1, C>(R, T, E, g, nm);
+}
+
+template <int C>
+__global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__restrict__ mom,
+                                                                 const uint32_t *__restrict__ index, int npix,
+                                                                 int m, float bias,
+                                                                 const float *__restrict__ grad_lag,
+                                                                 float *__restrict__ grad_mom)
+{
+    constexpr int MP = 8 * C;
+    extern __shared__ dc lds_adj[];
+    const int lane = threadIdx.x;
+    const int g = lane & 7;
+    const int q = lane >> 3;
+    const int pix = blockIdx.x * 8 + q;
+    const bool valid = pix < npix;
+    const int nm = valid ? (index ? min((int)(index[pix + 1] - index[pix]), m) : m) : 0;
+    const size_t off = valid ? (index ? (size_t)index[pix] : (size_t)pix * m) : 0;
+    dc *E = lds_adj + q * 4 * MP, *P = E + MP, *X = P + MP, *Y = X + MP;
+    double *T = reinterpret_cast<double *>(X);
+    const double b64 = (double)bias;
+    auto tval = [&](int e) {
+        return (e < nm) ? (1.0 - b64) * (double)mom[off + e] + (e == 0 ? b64 * 0.5 : 0.0) : 0.0;
+    };
+    AdjRegs<C> R;
+
+    // ---- forward, arrays kept ----
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int e = g + 8 * s;
+        T[e] = tval(e);
+        E[e] = dc{0.0, 0.0};
+        P[e] = dc{0.0, 0.0};
+        R.U[s] = dc{0.0, 0.0};
+    }
+    wave_sync();
+    double s0, c0;
+    sincos(kPi * T[0] - 0.5 * kPi, &s0, &c0);
+    if (g == 0 && nm > 0)
+        E[0] = dc{c0 * (0.25 / kPi), s0 * (0.25 / kPi)};
+    wave_sync();
+    herglotz_octaves<1, C>(T, E, g, nm);
+    if (g == 0 && nm > 0)
+        E[0] = dc{2.0 * E[0].x, 2.0 * E[0].y};
+    wave_sync();
+    const double e0x = (nm > 0) ? E[0].x : 1.0;
+    if (g == 0 && nm > 0)
+        P[0] = dc{1.0 / e0x, 0.0};
+    wave_sync();
+    int j_invalid = nm;
+    levinson_octaves<1, C, true>(P, E, g, nm, j_invalid, R.U);
+    // P <- 2 pi P (registers and LDS), autocorrelation R_kk = sum_j P_j conj(P_{j+kk})
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int e = g + 8 * s;
+        R.P[s] = drmul(2.0 * kPi, P[e]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+        P[g + 8 * s] = R.P[s];
+    wave_sync();
+    dc Rk[C];
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int kk = g + 8 * s;
+        double rx = 0.0, ry = 0.0;
+        if (kk < nm)
+            for (int j = 0; j < nm - kk; ++j)
+            {
+                const dc a = P[j], b = P[j + kk];
+                rx = fma(a.x, b.x, fma(a.y, b.y, rx));
+                ry = fma(a.y, b.x, fma(-a.x, b.y, ry));
+            }
+        Rk[s] = dc{rx, ry};
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        X[g + 8 * s] = Rk[s];          // R
+        Y[g + 8 * s] = dc{0.0, 0.0};
+    }
+    wave_sync();
+    const double p0s = (nm > 0) ? P[0].x : 1.0;
+    const double nf = 1.0 / (kPi * p0s);
+    // incoming gradients lb_kk, in Y (real parts)
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int e = g + 8 * s;
+        Y[e] = dc{(e < nm) ? (double)grad_lag[off + e] : 0.0, 0.0};
+    }
+    wave_sync();
+
+    // ---- reverse: imaginary correlation and normalisation ----
+    // out_kk = nf C_kk, C_kk = sum_j Im-part(R_{kk+j}, Eh_j) with Eh the
+    // undoubled exponential moments (Eh_0 = E_0 / 2)
+    double nfb_s = 0.0;
+    dc Rb[C];
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int n = g + 8 * s;
+        double ck = 0.0;
+        dc rb = dc{0.0, 0.0}, eb = dc{0.0, 0.0};
+        if (n < nm)
+        {
+            for (int j = 0; j < nm - n; ++j)
+            {
+                const dc r = X[n + j];
+                dc eh = E[j];
+                if (j == 0)
+                    eh = drmul(0.5, eh);
+                ck = fma(r.x, eh.y, fma(r.y, eh.x, ck));
+            }
+            // Rb_n = sum_{kk <= n} cb_kk (Eh_{n-kk}.y, Eh_{n-kk}.x), cb = lb nf
+            for (int kk = 0; kk <= n; ++kk)
+            {
+                const double cb = Y[kk].x * nf;
+                dc eh = E[n - kk];
+                if (kk == n)
+                    eh = drmul(0.5, eh);
+                rb.x = fma(cb, eh.y, rb.x);
+                rb.y = fma(cb, eh.x, rb.y);
+            }
+            // Eb_n = h_n sum_{kk < nm - n} cb_kk (R_{kk+n}.y, R_{kk+n}.x), h_0 = 1/2
+            const double h = (n == 0) ? 0.5 : 1.0;
+            for (int kk = 0; kk < nm - n; ++kk)
+            {
+                const double cb = Y[kk].x * nf * h;
+                const dc r = X[kk + n];
+                eb.x = fma(cb, r.y, eb.x);
+                eb.y = fma(cb, r.x, eb.y);
+            }
+            nfb_s = fma(Y[n].x, ck, nfb_s);
+        }
+        Rb[s] = rb;
+        R.Eb[s] = eb;
+    }
+    const double nfb = grp8_sum(nfb_s);
+    wave_sync();
+    // ---- reverse: autocorrelation, R_kk = sum_j P_j conj(P_{j+kk}) ----
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+        X[g + 8 * s] = Rb[s];
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int j = g + 8 * s;
+        dc pb = (j == 0) ? dmk(-nfb * nf / p0s, 0.0) : dmk(0.0, 0.0);
+        if (j < nm)
+            for (int kk = 0; kk < nm; ++kk)
+            {
+                const dc rb = X[kk];
+                if (j + kk < nm)
+                {
+                    const dc b = P[j + kk];
+                    pb = dadd(pb, dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
+                }
+                if (j >= kk)
+                {
+                    const dc a = P[j - kk];
+                    pb = dadd(pb, dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
+                }
+            }
+        R.Pb[s] = drmul(2.0 * kPi, pb);
+        R.P[s] = drmul(1.0 / (2.0 * kPi), R.P[s]);
+    }
+    wave_sync();
+
+    // ---- reverse: Levinson, rebuilding each previous iterate ----
+    levinson_reverse_octaves<C, C>(R, P, Y, X, E, g, nm, j_invalid);
+    // P_0 = 1 / E_0.x (E_0 doubled), then the recurrence's undoubled E_0
+    if (g == 0)
+    {
+        R.Eb[0].x += -R.Pb[0].x / (e0x * e0x);
+        R.Eb[0] = drmul(2.0, R.Eb[0]);
+    }
+    wave_sync();
+    // ---- reverse: Herglotz recurrence ----
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        T[g + 8 * s] = tval(g + 8 * s);
+        R.Tb[s] = 0.0;
+    }
+    wave_sync();
+    herglotz_reverse_octaves<C, C>(R, T, E, g, nm);
+    if (g == 0)
+        R.Tb[0] += 0.25 * (-R.Eb[0].x * s0 + R.Eb[0].y * c0);
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
+        const int e = g + 8 * s;
+        if (e < nm)
+            grad_mom[off + e] = (float)((1.0 - b64) * R.Tb[s]);
+    }
 }
 
 } // namespace
@@ -544,9 +616,28 @@ int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, i
     const int npix = W * H;
     if (m <= 64)
     {
-        hipLaunchKernelGGL(prepare_adjoint_wave_kernel, dim3((npix + 3) / 4), dim3(256), 0, stream, moments, index,
-                           npix, m, bias, grad_lagrange, grad_moments);
+        const int C = (m + 7) / 8;
+        const size_t lds = (size_t)8 * 4 * 8 * C * sizeof(dc);
+        const dim3 grid((npix + 7) / 8);
+#define MRAY_PADJ_CASE(CC)                                                                                            \
+    case CC:                                                                                                          \
+        hipLaunchKernelGGL(prepare_adjoint_grp_kernel<CC>, grid, dim3(64), lds, stream, moments, index, npix, m,      \
+                           bias, grad_lagrange, grad_moments);                                                        \
         return 0;
+        switch (C)
+        {
+            MRAY_PADJ_CASE(1)
+            MRAY_PADJ_CASE(2)
+            MRAY_PADJ_CASE(3)
+            MRAY_PADJ_CASE(4)
+            MRAY_PADJ_CASE(5)
+            MRAY_PADJ_CASE(6)
+            MRAY_PADJ_CASE(7)
+            MRAY_PADJ_CASE(8)
+        default:
+            break;
+        }
+#undef MRAY_PADJ_CASE
     }
     const int blocks = std::min(kAdjBlocks, (npix + kAdjThreads - 1) / kAdjThreads);
     const size_t bytes = (size_t)blocks * kAdjThreads * kArrays * m * sizeof(dc);
