@@ -68,7 +68,10 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     // so the inner products read all their operands unconditionally (the
     // indices are clamped into range and the surplus terms are exact zeros):
     // the loads of a step issue together instead of one round trip per term.
-    constexpr int MP = 8 * C; // padded per-pixel stride
+    constexpr int MP = 8 * C; // elements per array and pixel
+    // per-pixel stride MP + 1: the eight pixels' arrays start in different
+    // LDS banks (a stride of MP elements would put them all in the same ones)
+    constexpr int PS = MP + 1;
     extern __shared__ dc lds_c[];
     const int lane = threadIdx.x;
     const int g = lane & 7;
@@ -78,8 +81,8 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     const int nm = valid ? (index ? min((int)(index[pix + 1] - index[pix]), m) : m) : 0;
     const size_t off = valid ? (index ? (size_t)index[pix] : (size_t)pix * m) : 0;
 
-    dc *E = lds_c + q * MP;
-    dc *A = lds_c + 8 * MP + q * MP;
+    dc *E = lds_c + q * PS;
+    dc *A = lds_c + 8 * PS + q * PS;
     double *T = reinterpret_cast<double *>(A);
 
     // biased moments, lerp in FP64 (near the boundary of the moment space the
@@ -190,7 +193,7 @@ int launch_prepare(const float *moments, const uint32_t *index, int W, int H, in
         return 0;
     }
     const int C = (m + 7) / 8;
-    const size_t lds = (size_t)8 * 8 * C * 2 * sizeof(dc);
+    const size_t lds = (size_t)8 * (8 * C + 1) * 2 * sizeof(dc);
     if (lds > g_lds_limit)
         return MRAY_ERR_UNSUPPORTED;
     dim3 grid((npix + 7) / 8);

@@ -362,7 +362,9 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     const float4 *__restrict__ grad_rgba, float *__restrict__ grad_lag, float *__restrict__ grad_tf)
 {
     __shared__ float4 tf[256];
-    __shared__ float gtf[256 * 4];
+    // table gradient per workgroup, channel-major (gtf[ch * 256 + entry]) so
+    // that the lanes of one atomic instruction spread over the LDS banks
+    __shared__ float gtf[4 * 256];
     const int tid = threadIdx.x;
     tf[tid] = tf_g[tid];
     for (int i = 0; i < 4; ++i)
@@ -391,9 +393,9 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
             if (grad_tf)
             {
                 atomicAdd(&gtf[0], g.x);
-                atomicAdd(&gtf[1], g.y);
-                atomicAdd(&gtf[2], g.z);
-                atomicAdd(&gtf[3], g.w);
+                atomicAdd(&gtf[256], g.y);
+                atomicAdd(&gtf[512], g.z);
+                atomicAdd(&gtf[768], g.w);
             }
         }
         else
@@ -443,8 +445,8 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
 #pragma unroll
                     for (int ch = 0; ch < 4; ++ch)
                     {
-                        atomicAdd(&gtf[tf_x0 * 4 + ch], tf_a[ch]);
-                        atomicAdd(&gtf[tf_x1 * 4 + ch], tf_b[ch]);
+                        atomicAdd(&gtf[ch * 256 + tf_x0], tf_a[ch]);
+                        atomicAdd(&gtf[ch * 256 + tf_x1], tf_b[ch]);
                         tf_a[ch] = tf_b[ch] = 0.0f;
                     }
                 };
@@ -537,9 +539,9 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     if (grad_tf)
         for (int i = tid; i < 1024; i += blockDim.x)
         {
-            const float v = gtf[i];
+            const float v = gtf[i]; // channel i / 256 of entry i % 256
             if (v != 0.0f)
-                atomicAdd(&grad_tf[i], v);
+                atomicAdd(&grad_tf[(i & 255) * 4 + (i >> 8)], v);
         }
 }
 
