@@ -504,41 +504,38 @@ struct GradTarget
     int nx;
     uint32_t sxy;
 
-    __device__ __forceinline__ void add8(int cx, int cy, int cz, uint32_t fl, const float (&w)[8]) const
+    // adds w[i] to corner i of the cell for the corners in MASK (corner i:
+    // bit 0 / 1 / 2 of i select x1 / y1 / z1)
+    template <uint32_t MASK>
+    __device__ __forceinline__ void add(int cx, int cy, int cz, uint32_t fl, const float (&w)[8]) const
     {
         const int fx = fl & 1, fy = (fl >> 1) & 1, fz = (fl >> 2) & 1;
         const int lx = cx - ox, ly = cy - oy, lz = cz - oz;
         if (lds && lx >= 0 && ly >= 0 && lz >= 0 && lx + fx < ex && ly + fy < ey && lz + fz < ez)
         {
-            float *q0 = win + (lx + ex * (ly + ey * lz)), *q1 = q0 + fz * ex * ey;
-            const int dy = fy * ex;
-            atomicAdd(q0, w[0]);
-            atomicAdd(q0 + fx, w[1]);
-            atomicAdd(q0 + dy, w[2]);
-            atomicAdd(q0 + dy + fx, w[3]);
-            atomicAdd(q1, w[4]);
-            atomicAdd(q1 + fx, w[5]);
-            atomicAdd(q1 + dy, w[6]);
-            atomicAdd(q1 + dy + fx, w[7]);
+            float *q0 = win + (lx + ex * (ly + ey * lz));
+            const int dy = fy * ex, dz = fz * ex * ey;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (MASK & (1u << i))
+                    atomicAdd(q0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), w[i]);
             return;
         }
-        float *p0 = grad + ((size_t)cz * sxy + (uint32_t)(cy * nx + cx)), *p1 = p0 + (fz ? sxy : 0u);
-        const uint32_t dy = fy ? (uint32_t)nx : 0u;
-        unsafeAtomicAdd(p0, w[0]);
-        unsafeAtomicAdd(p0 + fx, w[1]);
-        unsafeAtomicAdd(p0 + dy, w[2]);
-        unsafeAtomicAdd(p0 + dy + fx, w[3]);
-        unsafeAtomicAdd(p1, w[4]);
-        unsafeAtomicAdd(p1 + fx, w[5]);
-        unsafeAtomicAdd(p1 + dy, w[6]);
-        unsafeAtomicAdd(p1 + dy + fx, w[7]);
+        float *p0 = grad + ((size_t)cz * sxy + (uint32_t)(cy * nx + cx));
+        const uint32_t dy = fy ? (uint32_t)nx : 0u, dz = fz ? sxy : 0u;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (MASK & (1u << i))
+                unsafeAtomicAdd(p0 + ((i & 1) ? (uint32_t)fx : 0u) + ((i & 2) ? dy : 0u) + ((i & 4) ? (size_t)dz : 0u),
+                                w[i]);
     }
 };
 
 // Accumulates gs times the trilinear weights of a sample. Consecutive
-// samples of a ray in the same cell are merged in registers; a cell's eight
-// sums go to the target when the ray leaves it (or at a flush). Clamp-to-edge
-// duplicates receive both weights.
+// samples of a ray in the same cell are merged in registers; when the ray
+// moves on to a neighbouring cell along one axis, the four corners of the
+// shared face stay in registers and only the other four go to the target,
+// else all eight do. Clamp-to-edge duplicates receive both weights.
 struct CellScatter
 {
     int cx = -1, cy = 0, cz = 0;
@@ -549,8 +546,31 @@ struct CellScatter
     {
         if (cx < 0)
             return;
-        tg.add8(cx, cy, cz, fl, w);
+        tg.add<0xFFu>(cx, cy, cz, fl, w);
         cx = -1;
+    }
+
+    // the ray moved by +1 (UP) or -1 along the axis of corner bit B: the face
+    // it leaves goes to the target, the shared face moves to the other side
+    template <int B, bool UP> __device__ __forceinline__ void step_face(const GradTarget &tg)
+    {
+        constexpr uint32_t kHigh = (B == 1) ? 0xAAu : (B == 2) ? 0xCCu : 0xF0u;
+        tg.add<UP ? (~kHigh & 0xFFu) : kHigh>(cx, cy, cz, fl, w);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(i & B))
+            {
+                if (UP)
+                {
+                    w[i] = w[i | B];
+                    w[i | B] = 0.0f;
+                }
+                else
+                {
+                    w[i | B] = w[i];
+                    w[i] = 0.0f;
+                }
+            }
     }
 
     __device__ __forceinline__ void add(const GradTarget &tg, const GradSample &sm, float gs)
@@ -559,14 +579,31 @@ struct CellScatter
         // sample just outside the domain clamps onto the edge voxel)
         if (!(sm.cx == cx && sm.cy == cy && sm.cz == cz && sm.fl == fl))
         {
-            flush(tg);
+            const int dx = sm.cx - cx, dy = sm.cy - cy, dz = sm.cz - cz;
+            const bool same_fl = cx >= 0 && sm.fl == fl;
+            if (same_fl && dy == 0 && dz == 0 && dx == 1 && (fl & 1u))
+                step_face<1, true>(tg);
+            else if (same_fl && dy == 0 && dz == 0 && dx == -1 && (fl & 1u))
+                step_face<1, false>(tg);
+            else if (same_fl && dx == 0 && dz == 0 && dy == 1 && (fl & 2u))
+                step_face<2, true>(tg);
+            else if (same_fl && dx == 0 && dz == 0 && dy == -1 && (fl & 2u))
+                step_face<2, false>(tg);
+            else if (same_fl && dx == 0 && dy == 0 && dz == 1 && (fl & 4u))
+                step_face<4, true>(tg);
+            else if (same_fl && dx == 0 && dy == 0 && dz == -1 && (fl & 4u))
+                step_face<4, false>(tg);
+            else
+            {
+                flush(tg);
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    w[i] = 0.0f;
+            }
             cx = sm.cx;
             cy = sm.cy;
             cz = sm.cz;
             fl = sm.fl;
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                w[i] = 0.0f;
         }
         const float ax = sm.ax, ay = sm.ay, az = sm.az;
         w[0] += gs * ((1.f - ax) * (1.f - ay) * (1.f - az));
@@ -764,15 +801,15 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         phi_prev = (float)((double)((t + s - t_near) * pi_trange_inv) - kPi) - dphi;
     else
         phi_prev = (float)(-kPi);
-    GradSample cur, nxt;
+    // cur: the sample of the node behind the current segment; its loads are
+    // issued one node ahead and used after the next node's sums
+    GradSample cur;
     double co = 1.0, si = 0.0, C_prev = 0.0, S_prev = 0.0;
-    float raw_prev = 0.0f;
     if (active)
     {
         issue_at(cur, t);
         sincos_small((double)phi_prev, si, co);
         sums(co, si, C_prev, S_prev);
-        raw_prev = cur.value();
     }
     double S_prev2 = S_prev, phi_prev2 = (double)phi_prev; // node k-2 for S'
     double fbar = 0.0; // pending gradient of the node behind the current segment
@@ -826,13 +863,11 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
                 t += s;
                 p_next = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
                 seg = dphi;
-                issue_at(nxt, t);
             }
             else
             {
                 p_next = 0.0f;
                 seg = 0.0f - phi_prev;
-                issue_at(nxt, t_far);
             }
             // the segment [pp, pn]: pp = p_next - seg in float (= phi_prev + delta)
             const double pn = (double)p_next;
@@ -856,7 +891,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
             const double A = -xs + V - rr * Sn + g0 * (sg - half);
             const double B = xs + rr * Sn + g0 * half;
             fbar += A;
-            const float gs0 = (float)fbar * dmap(raw_prev);
+            const float gs0 = (float)fbar * dmap(cur.value());
             if (gs0 != 0.0f)
                 cells.add(tg, cur, gs0);
             fbar = B;
@@ -865,11 +900,11 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
             C_prev = Cn;
             S_prev = Sn;
             phi_prev = p_next;
-            cur = nxt;
-            raw_prev = cur.value();
+            __builtin_amdgcn_sched_barrier(0);
+            issue_at(cur, last ? t_far : t);
             if (last)
             {
-                const float gl = (float)fbar * dmap(raw_prev);
+                const float gl = (float)fbar * dmap(cur.value());
                 if (gl != 0.0f)
                     cells.add(tg, cur, gl);
                 done = true;
