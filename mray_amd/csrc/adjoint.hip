@@ -493,12 +493,13 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     wave_sync();
     const double p0s = (nm > 0) ? P[0].x : 1.0;
     const double nf = 1.0 / (kPi * p0s);
-    // incoming gradients lb_kk, in Y (real parts)
+    // incoming gradients lb_kk, in Y (as doubles)
+    double *Yd = reinterpret_cast<double *>(Y);
 #pragma unroll
     for (int s = 0; s < C; ++s)
     {
         const int e = g + 8 * s;
-        Y[e] = dc{(e < nm) ? (double)grad_lag[off + e] : 0.0, 0.0};
+        Yd[e] = (e < nm) ? (double)grad_lag[off + e] : 0.0;
     }
     wave_sync();
 
@@ -515,34 +516,42 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
         dc rb = dc{0.0, 0.0}, eb = dc{0.0, 0.0};
         if (n < nm)
         {
-            for (int j = 0; j < nm - n; ++j)
+            // over t < nm - n: C_n = sum_t (R_{n+t}, Eh_t) terms and
+            // Eb_n = h_n nf sum_t lb_t (R_{n+t}.y, R_{n+t}.x), h_0 = 1/2;
+            // t = 0 (Eh_0 = E_0 / 2) peeled
             {
-                const dc r = X[n + j];
-                dc eh = E[j];
-                if (j == 0)
-                    eh = drmul(0.5, eh);
+                const dc r = X[n];
+                const dc eh = drmul(0.5, E[0]);
+                const double lbt = Yd[0];
+                ck = fma(r.x, eh.y, r.y * eh.x);
+                eb = dc{lbt * r.y, lbt * r.x};
+            }
+            for (int t = 1; t < nm - n; ++t)
+            {
+                const dc r = X[n + t];
+                const dc eh = E[t];
+                const double lbt = Yd[t];
                 ck = fma(r.x, eh.y, fma(r.y, eh.x, ck));
+                eb.x = fma(lbt, r.y, eb.x);
+                eb.y = fma(lbt, r.x, eb.y);
             }
-            // Rb_n = sum_{kk <= n} cb_kk (Eh_{n-kk}.y, Eh_{n-kk}.x), cb = lb nf
-            for (int kk = 0; kk <= n; ++kk)
-            {
-                const double cb = Y[kk].x * nf;
-                dc eh = E[n - kk];
-                if (kk == n)
-                    eh = drmul(0.5, eh);
-                rb.x = fma(cb, eh.y, rb.x);
-                rb.y = fma(cb, eh.x, rb.y);
-            }
-            // Eb_n = h_n sum_{kk < nm - n} cb_kk (R_{kk+n}.y, R_{kk+n}.x), h_0 = 1/2
             const double h = (n == 0) ? 0.5 : 1.0;
-            for (int kk = 0; kk < nm - n; ++kk)
+            eb = drmul(h * nf, eb);
+            // Rb_n = nf sum_{kk <= n} lb_kk (Eh_{n-kk}.y, Eh_{n-kk}.x), kk = n peeled
             {
-                const double cb = Y[kk].x * nf * h;
-                const dc r = X[kk + n];
-                eb.x = fma(cb, r.y, eb.x);
-                eb.y = fma(cb, r.x, eb.y);
+                const dc eh = drmul(0.5, E[0]);
+                const double lbk = Yd[n];
+                rb = dc{lbk * eh.y, lbk * eh.x};
             }
-            nfb_s = fma(Y[n].x, ck, nfb_s);
+            for (int kk = 0; kk < n; ++kk)
+            {
+                const double lbk = Yd[kk];
+                const dc eh = E[n - kk];
+                rb.x = fma(lbk, eh.y, rb.x);
+                rb.y = fma(lbk, eh.x, rb.y);
+            }
+            rb = drmul(nf, rb);
+            nfb_s = fma(Yd[n], ck, nfb_s);
         }
         Rb[s] = rb;
         R.Eb[s] = eb;
@@ -560,20 +569,21 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
         const int j = g + 8 * s;
         dc pb = (j == 0) ? dmk(-nfb * nf / p0s, 0.0) : dmk(0.0, 0.0);
         if (j < nm)
-            for (int kk = 0; kk < nm; ++kk)
+        {
+            // Rb_kk P_{j+kk} over kk < nm - j, conj(Rb_kk) P_{j-kk} over kk <= j
+            for (int kk = 0; kk < nm - j; ++kk)
             {
-                const dc rb = X[kk];
-                if (j + kk < nm)
-                {
-                    const dc b = P[j + kk];
-                    pb = dadd(pb, dmk(rb.x * b.x - rb.y * b.y, rb.x * b.y + rb.y * b.x));
-                }
-                if (j >= kk)
-                {
-                    const dc a = P[j - kk];
-                    pb = dadd(pb, dmk(rb.x * a.x + rb.y * a.y, rb.x * a.y - rb.y * a.x));
-                }
+                const dc rb = X[kk], b = P[j + kk];
+                pb.x = fma(rb.x, b.x, fma(-rb.y, b.y, pb.x));
+                pb.y = fma(rb.x, b.y, fma(rb.y, b.x, pb.y));
             }
+            for (int kk = 0; kk <= j; ++kk)
+            {
+                const dc rb = X[kk], a = P[j - kk];
+                pb.x = fma(rb.x, a.x, fma(rb.y, a.y, pb.x));
+                pb.y = fma(rb.x, a.y, fma(-rb.y, a.x, pb.y));
+            }
+        }
         R.Pb[s] = drmul(2.0 * kPi, pb);
         R.P[s] = drmul(1.0 / (2.0 * kPi), R.P[s]);
     }
