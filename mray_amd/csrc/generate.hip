@@ -25,9 +25,9 @@
 //         node by rotation: the reference's node phases jitter by ulps around
 //         the uniform grid, and a rotated phasor drifts off cos(p_k) by more
 //         than the high orders tolerate.
-//         Accumulation is blocked: kFold nodes go into a partial sum that is
-//         then added with Kahan compensation; trig_0 (the exact integral of the
-//         reference's lines) is accumulated in FP64. Moments live in
+//         Accumulation is blocked: kFold nodes go into an FP32 partial sum
+//         that is then added to an FP64 total; trig_0 (the exact integral of
+//         the reference's lines) is accumulated in FP64. Moments live in
 //         registers (template M >= m).
 #include "device_common.hpp"
 #include "launch.hpp"
@@ -177,25 +177,26 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 // MIXED
 // ---------------------------------------------------------------------------
-constexpr int kFold = 8;    // nodes per partial sum before the Kahan add
+constexpr int kFold = 8;    // nodes per FP32 partial sum before the FP64 add
 
-template <int M>
+// part[j] (+)= (d + j^2 e) cos(j phi) + j h sin(j phi) for j = 1..M-1 (the
+// partial sums are scaled by j^2, so the h sin / j term enters as j h sin);
+// FIRST starts a new partial sum. cos and sin of j phi by the Chebyshev
+// recurrence v_j = 2x v_{j-1} - v_{j-2} from x = cos(phi), y = sin(phi).
+template <int M, bool FIRST>
 __device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float e, float h, float x, float y)
 {
-    // part[j] += (d + j^2 e) cos(j phi) + j h sin(j phi) for j = 1..M-1 (the
-    // partial sums are scaled by j^2, so the h sin / j term enters as j h sin);
-    // cos and sin of j phi by the Chebyshev recurrence v_j = 2x v_{j-1} - v_{j-2}
-    // from x = cos(phi), y = sin(phi).
     const float x2 = x + x;
     float cm2 = 1.0f, cm1 = x;
     float sm2 = 0.0f, sm1 = y;
-    part[1] = __builtin_fmaf(h, sm1, __builtin_fmaf(d + e, cm1, part[1]));
+    part[1] = __builtin_fmaf(h, sm1, FIRST ? (d + e) * cm1 : __builtin_fmaf(d + e, cm1, part[1]));
 #pragma unroll
     for (int j = 2; j < M; ++j)
     {
         const float c = __builtin_fmaf(x2, cm1, -cm2);
         const float sn = __builtin_fmaf(x2, sm1, -sm2);
-        part[j] = __builtin_fmaf((float)j * h, sn, __builtin_fmaf(__builtin_fmaf((float)(j * j), e, d), c, part[j]));
+        const float w = __builtin_fmaf((float)(j * j), e, d);
+        part[j] = __builtin_fmaf((float)j * h, sn, FIRST ? w * c : __builtin_fmaf(w, c, part[j]));
         cm2 = cm1;
         cm1 = c;
         sm2 = sm1;
@@ -203,15 +204,23 @@ __device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float
     }
 }
 
-template <int M> __device__ __forceinline__ void kahan_fold(float (&part)[M], float (&sum)[M], float (&comp)[M])
+template <int M>
+__device__ __forceinline__ void accumulate_node(bool first, float (&part)[M], float d, float e, float h, float x,
+                                                float y)
+{
+    if (first)
+        accumulate_node<M, true>(part, d, e, h, x, y);
+    else
+        accumulate_node<M, false>(part, d, e, h, x, y);
+}
+
+// adds the partial sums to the FP64 totals and restarts them
+template <int M> __device__ __forceinline__ void fold(float (&part)[M], double (&sum)[M])
 {
 #pragma unroll
     for (int j = 1; j < M; ++j)
     {
-        float yv = part[j] - comp[j];
-        float tv = sum[j] + yv;
-        comp[j] = (tv - sum[j]) - yv;
-        sum[j] = tv;
+        sum[j] += (double)part[j];
         part[j] = 0.0f;
     }
 }
@@ -275,13 +284,13 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     sincosf(node_phase, &zs, &zc);
     const float z0s = zs, z0c = zc;
 
-    float part[M], sum[M], comp[M];
+    float part[M];
+    double sum[M];
 #pragma unroll
     for (int j = 0; j < M; ++j)
     {
         part[j] = 0.0f;
-        sum[j] = 0.0f;
-        comp[j] = 0.0f;
+        sum[j] = 0.0;
     }
     double trap = 0.0;
     float g_left = 0.0f;   // slope of the segment ending at the current node
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         trap += width * ((double)f_prev + 0.5 * (double)g * width);
         const float delta = (node > 0) ? (float)((double)u - (double)node_phase) : 0.0f;
         const float h = __builtin_fmaf(g, delta, g_left * eps_left);
-        accumulate_node<M>(part, g_left - g, -delta * f_prev, h, zc, zs);
+        accumulate_node<M, false>(part, g_left - g, -delta * f_prev, h, zc, zs);
         g_left = g;
         eps_left = (float)(width - (double)dphi);
         f_prev = f;
@@ -325,7 +334,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         ++node;
         sincosf(p, &zs, &zc);
         if (node % kFold == 0)
-            kahan_fold<M>(part, sum, comp);
+            fold<M>(part, sum);
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
@@ -336,12 +345,12 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float dl = 0.0f - node_phase;
     const float g_last = (f_last - f_prev) / dl;
     trap += (double)dl * ((double)f_prev + 0.5 * (double)g_last * (double)dl);
-    accumulate_node<M>(part, g_left - g_last, 0.0f, g_left * eps_left, zc, zs);
+    accumulate_node<M, false>(part, g_left - g_last, 0.0f, g_left * eps_left, zc, zs);
     // node at phase 0: cos(j*0) = 1, sin(j*0) = 0
 #pragma unroll
     for (int j = 1; j < M; ++j)
         part[j] += g_last;
-    kahan_fold<M>(part, sum, comp);
+    fold<M>(part, sum);
 
     // trig_j = sum_j / j^2 - f_0 sin(j phi_0) / j, sin by Chebyshev from phi_0.
     float trig[M];
@@ -361,7 +370,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
                 sm1 = sj;
             }
             const float jf = (float)j;
-            trig[j] = (sum[j] - comp[j]) / (jf * jf) - f_first * sj / jf;
+            trig[j] = (float)sum[j] / (jf * jf) - f_first * sj / jf;
         }
     }
     trig[0] = (float)trap;
