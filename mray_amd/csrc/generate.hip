@@ -179,6 +179,31 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 constexpr int kFold = 8;    // nodes per FP32 partial sum before the FP64 add
 
+// sin and cos of a node phase p in [-3.3, 0.1]: p = k pi/2 + r with k =
+// rint(2p / pi) in {-2, -1, 0}, r taken off in two float parts of pi/2 (k * hi
+// is exact for these k, so r is within an ulp), then Taylor polynomials on
+// |r| <= pi/4 (truncation below 3e-9 relative), about as accurate as the
+// device library's sincosf for this range and without its large-argument path.
+__device__ __forceinline__ void sincos_phase(float p, float &s, float &c)
+{
+    const float k = rintf(p * 0.636619772f);
+    float r = __builtin_fmaf(-k, 1.57079637f, p);
+    r = __builtin_fmaf(-k, -4.37113900e-8f, r);
+    const float r2 = r * r;
+    float ps = __builtin_fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);
+    ps = __builtin_fmaf(r2, ps, 8.33333333e-3f);
+    ps = __builtin_fmaf(r2, ps, -0.166666667f);
+    const float sr = __builtin_fmaf(r * r2, ps, r);
+    float pc = __builtin_fmaf(r2, -2.75573192e-7f, 2.48015873e-5f);
+    pc = __builtin_fmaf(r2, pc, -1.38888889e-3f);
+    pc = __builtin_fmaf(r2, pc, 4.16666667e-2f);
+    pc = __builtin_fmaf(r2, pc, -0.5f);
+    const float cr = __builtin_fmaf(r2, pc, 1.0f);
+    const int q = (int)k & 3; // 0, 3 (k = -1) or 2 (k = -2)
+    s = (q == 0) ? sr : (q == 3) ? -cr : -sr;
+    c = (q == 0) ? cr : (q == 3) ? sr : -cr;
+}
+
 // part[j] (+)= (d + j^2 e) cos(j phi) + j h sin(j phi) for j = 1..M-1 (the
 // partial sums are scaled by j^2, so the h sin / j term enters as j h sin);
 // FIRST starts a new partial sum. cos and sin of j phi by the Chebyshev
@@ -281,7 +306,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     else
         node_phase = (float)(-kPi);
     float zs, zc;
-    sincosf(node_phase, &zs, &zc);
+    sincos_phase(node_phase, zs, zc);
     const float z0s = zs, z0c = zc;
 
     float part[M];
@@ -332,7 +357,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         f_prev = f;
         node_phase = p;
         ++node;
-        sincosf(p, &zs, &zc);
+        sincos_phase(p, zs, zc);
         if (node % kFold == 0)
             fold<M>(part, sum);
     }
