@@ -422,7 +422,10 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     const bool valid = pix < npix;
     const int nm = valid ? (index ? min((int)(index[pix + 1] - index[pix]), m) : m) : 0;
     const size_t off = valid ? (index ? (size_t)index[pix] : (size_t)pix * m) : 0;
-    dc *E = lds_adj + q * 4 * MP, *P = E + MP, *X = P + MP, *Y = X + MP;
+    // per-pixel stride 4 MP + 4 elements: consecutive pixels start 64 bytes
+    // (16 banks) apart, so the 8-byte accesses of two pixels served together
+    // do not share banks
+    dc *E = lds_adj + q * (4 * MP + 4), *P = E + MP, *X = P + MP, *Y = X + MP;
     double *T = reinterpret_cast<double *>(X);
     const double b64 = (double)bias;
     auto tval = [&](int e) {
@@ -627,7 +630,7 @@ int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, i
     if (m <= 64)
     {
         const int C = (m + 7) / 8;
-        const size_t lds = (size_t)8 * 4 * 8 * C * sizeof(dc);
+        const size_t lds = (size_t)8 * (4 * 8 * C + 4) * sizeof(dc);
         const dim3 grid((npix + 7) / 8);
 #define MRAY_PADJ_CASE(CC)                                                                                            \
     case CC:                                                                                                          \

@@ -69,9 +69,10 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     // indices are clamped into range and the surplus terms are exact zeros):
     // the loads of a step issue together instead of one round trip per term.
     constexpr int MP = 8 * C; // elements per array and pixel
-    // per-pixel stride MP + 1: the eight pixels' arrays start in different
-    // LDS banks (a stride of MP elements would put them all in the same ones)
-    constexpr int PS = MP + 1;
+    // per-pixel stride MP + 4: the eight pixels' arrays start 64 bytes (16
+    // banks) apart (a stride of MP elements would put them all in the same
+    // banks)
+    constexpr int PS = MP + 4;
     extern __shared__ dc lds_c[];
     const int lane = threadIdx.x;
     const int g = lane & 7;
@@ -193,7 +194,7 @@ int launch_prepare(const float *moments, const uint32_t *index, int W, int H, in
         return 0;
     }
     const int C = (m + 7) / 8;
-    const size_t lds = (size_t)8 * (8 * C + 1) * 2 * sizeof(dc);
+    const size_t lds = (size_t)8 * (8 * C + 4) * 2 * sizeof(dc);
     if (lds > g_lds_limit)
         return MRAY_ERR_UNSUPPORTED;
     dim3 grid((npix + 7) / 8);
