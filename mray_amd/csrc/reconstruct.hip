@@ -208,7 +208,8 @@ __device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p, do
 }
 
 // fast_atan with the reciprocal from the hardware approximation (about one
-// ulp) instead of the IEEE division; the MIXED form only
+// ulp) instead of the IEEE division and fused multiply-adds in the
+// polynomial; the MIXED form only
 __device__ __forceinline__ float fast_atan_rcp(float tan)
 {
     const bool negative = (tan < 0.0f);
@@ -218,14 +219,13 @@ __device__ __forceinline__ float fast_atan_rcp(float tan)
     float y = greater_one ? 1.0f : abs_tan;
     x = x * y;
     y = x * x;
-    float z = y * 0.020835f - 0.085133f;
-    z = y * z + 0.180141f;
-    z = y * z - 0.330299f;
-    y = y * z + 0.999866f;
-    z = y * x;
-    z = z * -2.0f + 1.570796f;
+    float z = __builtin_fmaf(y, 0.020835f, -0.085133f);
+    z = __builtin_fmaf(y, z, 0.180141f);
+    z = __builtin_fmaf(y, z, -0.330299f);
+    y = __builtin_fmaf(y, z, 0.999866f);
+    z = __builtin_fmaf(y * x, -2.0f, 1.570796f);
     z = greater_one ? z : 0.0f;
-    x = x * y + z;
+    x = __builtin_fmaf(x, y, z);
     return negative ? (-x) : x;
 }
 
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
             p += dphi;
             ++ns;
             const float bs = recon(p);
-            const float4 c = tf_sample(tf, fs, bs);
+            const float4 c = tf_sample_mixed(tf, fs, bs);
             Lr = __builtin_fmaf(a.emission * c.x, T, Lr);
             Lg = __builtin_fmaf(a.emission * c.y, T, Lg);
             Lb = __builtin_fmaf(a.emission * c.z, T, Lb);
@@ -459,9 +459,9 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     double c1;
                     const float S = (float)series_clenshaw<M>(lam, pc, &c1);
                     const float bs = rho_of(S);
-                    const float4 c = tf_sample(tf, fs, bs);
+                    const float4 c = tf_sample_mixed(tf, fs, bs);
 
-                    // indices and weights of tf_sample
+                    // indices and weights of tf_sample_mixed
                     const float ipx = fs * 256.0f, ipy = bs * 256.0f;
                     const float ax = ipx - floorf(ipx), ay = ipy - floorf(ipy);
                     const int xl = (int)floorf(ipx);
