@@ -183,8 +183,10 @@ template <typename T> __device__ __forceinline__ float sample_volume(const Volum
 // sample_volume split in two: issue() computes the weights and starts the
 // eight loads, value() combines them later, so that the loads of the next
 // sample are in flight while the current one is used (same arithmetic and
-// result as sample_volume).
-template <typename T> struct PendingSample
+// result as sample_volume). OFF32: the volume spans at most 4 GiB, so the
+// corners are addressed by 32-bit byte offsets from the base (one scalar
+// base, no 64-bit address arithmetic per load).
+template <typename T, bool OFF32 = false> struct PendingSample
 {
     float ax, ay, az;
     T lll, rll, lrl, rrl, llr, rlr, lrr, rrr;
@@ -202,9 +204,27 @@ template <typename T> struct PendingSample
         int x0 = iclamp(xl, 0, v.nx - 1), x1 = iclamp(xl + 1, 0, v.nx - 1);
         int y0 = iclamp(yl, 0, v.ny - 1), y1 = iclamp(yl + 1, 0, v.ny - 1);
         int z0 = iclamp(zl, 0, v.nz - 1), z1 = iclamp(zl + 1, 0, v.nz - 1);
+        const uint32_t sxy = (uint32_t)(v.ny * v.nx);
+        if constexpr (OFF32)
+        {
+            constexpr uint32_t B = sizeof(T);
+            const char *base = reinterpret_cast<const char *>(v.data);
+            const uint32_t o0 = ((uint32_t)z0 * sxy + (uint32_t)(y0 * v.nx + x0)) * B;
+            const uint32_t dxb = (uint32_t)(x1 - x0) * B;
+            const uint32_t dyb = (uint32_t)((y1 - y0) * v.nx) * B;
+            const uint32_t o1 = o0 + ((z1 > z0) ? sxy * B : 0u);
+            lll = *reinterpret_cast<const T *>(base + o0);
+            rll = *reinterpret_cast<const T *>(base + (o0 + dxb));
+            lrl = *reinterpret_cast<const T *>(base + (o0 + dyb));
+            rrl = *reinterpret_cast<const T *>(base + (o0 + dyb + dxb));
+            llr = *reinterpret_cast<const T *>(base + o1);
+            rlr = *reinterpret_cast<const T *>(base + (o1 + dxb));
+            lrr = *reinterpret_cast<const T *>(base + (o1 + dyb));
+            rrr = *reinterpret_cast<const T *>(base + (o1 + dyb + dxb));
+            return;
+        }
         // one 64-bit corner address, the others by 32-bit offsets (the
         // clamped neighbour is 0 or 1 cell away on each axis)
-        const uint32_t sxy = (uint32_t)(v.ny * v.nx);
         const T *p0 = v.data + ((size_t)z0 * sxy + (uint32_t)(y0 * v.nx + x0));
         const uint32_t dx = (uint32_t)(x1 - x0);
         const uint32_t dy = (uint32_t)((y1 - y0) * v.nx);

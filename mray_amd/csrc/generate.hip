@@ -250,7 +250,7 @@ template <int M> __device__ __forceinline__ void fold(float (&part)[M], double (
     }
 }
 
-template <typename T, int M>
+template <typename T, int M, bool OFF32>
 __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
                                                              int32_t *__restrict__ samples)
@@ -326,10 +326,10 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float rdphi = host_reciprocal(dphi); // for the segment slopes
     // the next sample's loads are issued one step ahead (at t + s while the
     // march continues, else at t_far for the last segment)
-    auto issue_at = [&](PendingSample<T> &ps, float tt) {
+    auto issue_at = [&](PendingSample<T, OFF32> &ps, float tt) {
         ps.issue(vol, r.o.x + tt * r.d.x, r.o.y + tt * r.d.y, r.o.z + tt * r.d.z);
     };
-    PendingSample<T> pending;
+    PendingSample<T, OFF32> pending;
     issue_at(pending, (t + s < t_far) ? t + s : t_far);
     for (int it = 0; t + s < t_far && it < cap; ++it)
     {
@@ -433,7 +433,12 @@ template <typename T, int M>
 static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
                          int32_t *s, hipStream_t st, uint32_t blocks)
 {
-    hipLaunchKernelGGL((generate_mixed_kernel<T, M>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s);
+    // byte offsets of every voxel fit in 32 bits up to 4 GiB of volume
+    const bool off32 = (uint64_t)v.nx * v.ny * v.nz * sizeof(T) <= (uint64_t)0xFFFFFFFFu + 1u;
+    if (off32)
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s);
+    else
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s);
 }
 
 template <typename T>
