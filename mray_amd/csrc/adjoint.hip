@@ -289,7 +289,7 @@ template <int C> struct AdjRegs
 // Reverse Levinson steps j = min(8S - 1, nm - 1) down to max(1, 8(S - 1)),
 // which touch the elements k <= j, i.e. slots s < S; then the octave below.
 template <int S, int C>
-__device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, dc *Yl, dc *Xl, const dc *E, int g,
+__device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, dc *Yl, const dc *E, int g,
                                                          int nm, int j_invalid)
 {
     const int hi = min(8 * S - 1, nm - 1);
@@ -314,7 +314,11 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
         const dc pjb = Yl[j];
         const double p0b = Yl[0].x;
         const double p0 = Pl[0].x / f;
-        dc Pq[S], Pbq[S], Pold[S];
+        // previous iterate: Pold_k = P_k + u conj(P_{j-k}) (1 <= k < j),
+        // Pold_0 = p0, Pold_j = 0; its partner Poq_k = Pold_{j-k} comes from
+        // the same two values, P_{j-k} + u conj(P_k)
+        dc Pq[S], Pbq[S], Pold[S], Poq[S];
+        dc acc = dmk(0.0, 0.0);
 #pragma unroll
         for (int s = 0; s < S; ++s)
         {
@@ -323,25 +327,26 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
             Pq[s] = Pl[r];
             Pbq[s] = Yl[r];
             if (k >= 1 && k < j)
+            {
                 Pold[s] = dadd(R.P[s], dmul(u, dconj(Pq[s])));
-            else if (k == 0)
-                Pold[s] = dmk(p0, 0.0);
-            else if (k == j)
-                Pold[s] = dmk(0.0, 0.0);
-            else
-                Pold[s] = R.P[s];
-            Xl[k] = Pold[s];
-        }
-        wave_sync();
-        dc Poq[S];
-        dc acc = dmk(0.0, 0.0);
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-        {
-            const int k = g + 8 * s;
-            Poq[s] = Xl[(k <= j) ? j - k : 0]; // P_{j-k} of the previous iterate
-            if (k >= 1 && k < j)
+                Poq[s] = dadd(Pq[s], dmul(u, dconj(R.P[s])));
                 acc = dadd(acc, dmul(R.Pb[s], Poq[s]));
+            }
+            else if (k == 0)
+            {
+                Pold[s] = dmk(p0, 0.0);
+                Poq[s] = dmk(0.0, 0.0);
+            }
+            else if (k == j)
+            {
+                Pold[s] = dmk(0.0, 0.0);
+                Poq[s] = dmk(p0, 0.0);
+            }
+            else
+            {
+                Pold[s] = R.P[s];
+                Poq[s] = dmk(0.0, 0.0);
+            }
         }
         wave_sync();
         acc = dc{grp8_sum(acc.x), grp8_sum(acc.y)};
@@ -372,7 +377,7 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
         }
     }
     if constexpr (S > 1)
-        levinson_reverse_octaves<S - 1, C>(R, Pl, Yl, Xl, E, g, nm, j_invalid);
+        levinson_reverse_octaves<S - 1, C>(R, Pl, Yl, E, g, nm, j_invalid);
 }
 
 // Reverse Herglotz steps l = min(8S - 1, nm - 1) down to max(1, 8(S - 1)):
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     wave_sync();
 
     // ---- reverse: Levinson, rebuilding each previous iterate ----
-    levinson_reverse_octaves<C, C>(R, P, Y, X, E, g, nm, j_invalid);
+    levinson_reverse_octaves<C, C>(R, P, Y, E, g, nm, j_invalid);
     // P_0 = 1 / E_0.x (E_0 doubled), then the recurrence's undoubled E_0
     if (g == 0)
     {
