@@ -253,9 +253,12 @@ def adjoint_ratio(steps):
     vol = hip.Volume(vox, dmin, dmax)
     g = torch.randn((H, W, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
     gvol = torch.zeros_like(vox)
+    # per-ray normalisation written by the forward and read by the reverse
+    # (what the autograd wrapper keeps between the two)
+    rr = torch.empty((H * W, 2), dtype=torch.float32, device="cuda")
 
     def fwd():
-        mo, b, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed")
+        mo, b, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed", ray_range=rr)
         lag = hip.prepare(mo, W, H, m, rp.bias, form="mixed")
         rgba, _ = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form="mixed", bounds=b)
         return mo, b, lag, rgba
@@ -267,7 +270,7 @@ def adjoint_ratio(steps):
             return
         g_mo = hip.prepare_backward(mo, W, H, m, rp.bias, g_lag)
         gvol.zero_()
-        hip.generate_backward(vol, cam, gp, W, H, m, g_mo, out=gvol)
+        hip.generate_backward(vol, cam, gp, W, H, m, g_mo, out=gvol, ray_range=rr)
 
     def timed(fn, k):
         torch.cuda.synchronize()

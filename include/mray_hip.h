@@ -63,14 +63,18 @@ const void *mray_hip_volume_data(mray_volume vol);
 /* Ray marches every pixel and writes dense bounded trigonometric moments.
  * Replaces generate_moments_kernel (src/moment_raymarcher.cu:163-185) /
  * generate_ray (src/raymarch_generate.h:69-148).
- *   d_moments  W*H*m floats (dense)
- *   d_bounds   W*H uint32, written when p->transform_to_bounds (else may be NULL)
- *   d_samples  optional W*H int32: samples taken per ray (N_g), NULL to skip
+ *   d_moments    W*H*m floats (dense)
+ *   d_bounds     W*H uint32, written when p->transform_to_bounds (else may be NULL)
+ *   d_samples    optional W*H int32: samples taken per ray (N_g), NULL to skip
+ *   d_ray_range  optional W*H x 2 floats: per ray the (lo, hi) the moments
+ *                were normalised with (before the 16-bit packing of
+ *                d_bounds), for mray_hip_generate_backward; NULL to skip
  * form: MRAY_FORM_PLAIN or MRAY_FORM_MIXED. Compaction and prediction coding
  * are separate calls (mray_hip_compute_num_moments, mray_hip_moment_index,
  * mray_hip_compact, mray_hip_prediction_encode). */
 int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int form, int W, int H,
-                      int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, mray_stream stream);
+                      int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, float *d_ray_range,
+                      mray_stream stream);
 
 /* ---- prepare (src/moment_image.cpp:154-183, src/moment_image_device.cu:279-335) */
 
@@ -122,10 +126,13 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
  * dL/dvoxels ADDED to d_grad_volume (nx*ny*nz floats, the caller zeroes it).
  * Differentiates the exact integral of the reference's piecewise-linear
  * segments, map_density (where it does not clamp) and the trilinear weights;
- * the per-ray bounds are constants. F32 volumes, up to 64 moments; float
+ * the per-ray bounds are constants. d_ray_range: the forward's per-ray
+ * normalisation (its d_ray_range output), or NULL to recompute it with an
+ * extra pass over the samples. F32 volumes, up to 64 moments; float
  * atomics, so the summation order (and the last bits) vary between runs. */
 int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int W, int H, int m,
-                               const float *d_grad_moments, float *d_grad_volume, mray_stream stream);
+                               const float *d_grad_moments, const float *d_ray_range, float *d_grad_volume,
+                               mray_stream stream);
 
 /* Reverse mode of mray_hip_prepare (MIXED mathematics, FP64): given
  * dL/dlagrange, writes dL/dmoments (layout of d_moments) for the bias lerp,

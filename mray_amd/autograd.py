@@ -76,8 +76,11 @@ class GenerateFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, voxels, dmin, dmax, cam, gp, W, H, m):
         vol = hip.Volume(voxels.detach().contiguous(), dmin, dmax)
-        mo, bounds, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed")
+        # the per-ray normalisation is kept for the reverse pass
+        rr = torch.empty((H * W, 2), dtype=torch.float32, device=voxels.device) if gp.transform_to_bounds else None
+        mo, bounds, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed", ray_range=rr)
         ctx.vol = vol
+        ctx.ray_range = rr
         ctx.args = (cam, gp, W, H, m)
         ctx.mark_non_differentiable(bounds) if bounds is not None else None
         return mo, bounds
@@ -85,7 +88,8 @@ class GenerateFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_mo, grad_bounds):
         cam, gp, W, H, m = ctx.args
-        g = hip.generate_backward(ctx.vol, cam, gp, W, H, m, grad_mo.detach().to(torch.float32).contiguous())
+        g = hip.generate_backward(ctx.vol, cam, gp, W, H, m, grad_mo.detach().to(torch.float32).contiguous(),
+                                  ray_range=ctx.ray_range)
         return g, None, None, None, None, None, None, None
 
 

@@ -152,7 +152,8 @@ int mray_hip_volume_destroy(mray_volume vol)
 const void *mray_hip_volume_data(mray_volume vol) { return vol ? vol->data : nullptr; }
 
 int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int form, int W, int H,
-                      int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, mray_stream stream)
+                      int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, float *d_ray_range,
+                      mray_stream stream)
 {
     if (!g_init)
         return fail(MRAY_ERR_NOT_INIT, "mray_hip_generate: call mray_hip_init first");
@@ -167,7 +168,7 @@ int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_pa
     if (!(p->step_size > 0.0f) || !(p->data_max != p->data_min))
         return fail(MRAY_ERR_ARG, "mray_hip_generate: step_size must be > 0 and data_max != data_min");
     int rc = mray::launch_generate(vol->data, vol->dtype, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax, *cam, *p,
-                                   form, W, H, m, d_moments, d_bounds, d_samples, (hipStream_t)stream);
+                                   form, W, H, m, d_moments, d_bounds, d_samples, d_ray_range, (hipStream_t)stream);
     return check_status(rc, "mray_hip_generate");
 }
 
@@ -230,7 +231,8 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
 }
 
 int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int W, int H, int m,
-                               const float *d_grad_moments, float *d_grad_volume, mray_stream stream)
+                               const float *d_grad_moments, const float *d_ray_range, float *d_grad_volume,
+                               mray_stream stream)
 {
     if (!g_init)
         return fail(MRAY_ERR_NOT_INIT, "mray_hip_generate_backward: call mray_hip_init first");
@@ -243,7 +245,8 @@ int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mr
     if (!(p->step_size > 0.0f) || !(p->data_max > p->data_min))
         return fail(MRAY_ERR_ARG, "mray_hip_generate_backward: step_size must be > 0 and data_max > data_min");
     int rc = mray::launch_generate_adjoint((const float *)vol->data, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax,
-                                           *cam, *p, W, H, m, d_grad_moments, d_grad_volume, (hipStream_t)stream);
+                                           *cam, *p, W, H, m, d_grad_moments, d_ray_range, d_grad_volume,
+                                           (hipStream_t)stream);
     if (rc == MRAY_ERR_UNSUPPORTED)
         return fail(rc, "mray_hip_generate_backward: at most 64 moments");
     return check_status(rc, "mray_hip_generate_backward");

@@ -60,7 +60,8 @@ __device__ __forceinline__ float sample_at(const VolumeView<T> &vol, const Ray &
 template <typename T>
 __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
-                                                             int32_t *__restrict__ samples)
+                                                             int32_t *__restrict__ samples,
+                                                             float2 *__restrict__ ray_range)
 {
     extern __shared__ float lds_trig[]; // [m][blockDim.x], strided per thread
     const int tid = threadIdx.x;
@@ -93,6 +94,8 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
                 mo[j] = 0.f;
         if (bounds)
             bounds[pix] = 0;
+        if (ray_range)
+            ray_range[pix] = make_float2(0.0f, 1.0f);
         if (samples)
             samples[pix] = 0;
         return;
@@ -170,6 +173,8 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 #undef TRIG
     if (bounds)
         bounds[pix] = pack_bound(lo, hi);
+    if (ray_range)
+        ray_range[pix] = make_float2(lo, hi);
     if (samples)
         samples[pix] = ns;
 }
@@ -253,7 +258,8 @@ template <int M> __device__ __forceinline__ void fold(float (&part)[M], double (
 template <typename T, int M, bool OFF32>
 __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
-                                                             int32_t *__restrict__ samples)
+                                                             int32_t *__restrict__ samples,
+                                                             float2 *__restrict__ ray_range)
 {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -279,6 +285,8 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
                 mo[j] = 0.f;
         if (bounds)
             bounds[pix] = 0;
+        if (ray_range)
+            ray_range[pix] = make_float2(0.0f, 1.0f);
         if (samples)
             samples[pix] = 0;
         return;
@@ -422,6 +430,8 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     b = pack_bound(lo, hi);
     if (bounds)
         bounds[pix] = b;
+    if (ray_range)
+        ray_range[pix] = make_float2(lo, hi);
     if (samples)
         samples[pix] = ns;
 }
@@ -431,25 +441,26 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 template <typename T, int M>
 static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
-                         int32_t *s, hipStream_t st, uint32_t blocks)
+                         int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks)
 {
     // byte offsets of every voxel fit in 32 bits up to 4 GiB of volume
     const bool off32 = (uint64_t)v.nx * v.ny * v.nz * sizeof(T) <= (uint64_t)0xFFFFFFFFu + 1u;
     if (off32)
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s);
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s, rr);
     else
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s);
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s,
+                           rr);
 }
 
 template <typename T>
 static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
-                          int32_t *s, hipStream_t st, uint32_t blocks)
+                          int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks)
 {
     const int m = a.m;
 #define MRAY_GEN_CASE(MM)                                                                                             \
     if (m <= MM)                                                                                                      \
     {                                                                                                                 \
-        launch_mixed<T, MM>(v, c, a, o, b, s, st, blocks);                                                            \
+        launch_mixed<T, MM>(v, c, a, o, b, s, rr, st, blocks);                                                            \
         return 0;                                                                                                     \
     }
     MRAY_GEN_CASE(4)
@@ -713,6 +724,7 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &c)
 template <int M>
 __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float> vol, DevCamera cam, GenArgs a,
                                                                const float *__restrict__ grad_mom,
+                                                               const float2 *__restrict__ ray_range,
                                                                float *__restrict__ grad_vol)
 {
     // The workgroup's 256 rays advance in chunks of kChunk nodes. Per chunk
@@ -753,9 +765,15 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     };
     auto density = [&](const GradSample &sm) { return map_density_r(sm.value(), a.data_min, a.data_max, a.data_rinv); };
 
-    // normalisation of the forward: inv = 1 / (hi - lo) with the bounds of the ray
+    // normalisation of the forward: inv = 1 / (hi - lo) with the bounds of the
+    // ray, as the forward recorded them or from a pass over the samples
     double inv = 1.0;
-    if (active && a.transform_to_bounds)
+    if (active && a.transform_to_bounds && ray_range)
+    {
+        const float2 b = ray_range[pix];
+        inv = 1.0 / (double)(b.y - b.x);
+    }
+    else if (active && a.transform_to_bounds)
     {
         float lo = 1.f, hi = 0.f;
         GradSample sm;
@@ -782,7 +800,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         }
         lo = rmax(lo - a.bound_eps, 0.0f);
         hi = rmin(hi + a.bound_eps, 1.0f);
-        inv = 1.0 / ((double)hi - (double)lo);
+        inv = 1.0 / (double)(hi - lo);
     }
     // dL/dtrig_j = dL/dmoment_j * inv / pi
     // Chebyshev coefficients g_j / j^2 of C in FP64 registers
@@ -990,8 +1008,10 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
 
 int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const float *dmin, const float *dmax,
                             const mray_camera &cam, const mray_gen_params &p, int W, int H, int m,
-                            const float *grad_moments, float *grad_volume, hipStream_t stream)
+                            const float *grad_moments, const float *ray_range, float *grad_volume,
+                            hipStream_t stream)
 {
+    const float2 *rr = reinterpret_cast<const float2 *>(ray_range);
     GenArgs a;
     a.W = W;
     a.H = H;
@@ -1011,7 +1031,7 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
     if (m <= MM)                                                                                                      \
     {                                                                                                                 \
         hipLaunchKernelGGL(generate_adjoint_kernel<MM>, dim3(blocks), dim3(256), 0, stream, v, c, a, grad_moments,    \
-                           grad_volume);                                                                              \
+                           rr, grad_volume);                                                                          \
         return 0;                                                                                                     \
     }
     MRAY_GADJ_CASE(8)
@@ -1028,8 +1048,9 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
 
 int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
                     const mray_camera &cam, const mray_gen_params &p, int form, int W, int H, int m, float *moments,
-                    uint32_t *bounds, int32_t *samples, hipStream_t stream)
+                    uint32_t *bounds, int32_t *samples, float *ray_range, hipStream_t stream)
 {
+    float2 *rr = reinterpret_cast<float2 *>(ray_range);
     GenArgs a;
     a.W = W;
     a.H = H;
@@ -1051,20 +1072,20 @@ int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const fl
         {
             int bs = plain_block(m);
             hipLaunchKernelGGL(generate_plain_kernel<float>, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
-                               bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples);
+                               bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples, rr);
             return 0;
         }
-        return dispatch_mixed<float>(v, c, a, moments, bounds, samples, stream, blocks);
+        return dispatch_mixed<float>(v, c, a, moments, bounds, samples, rr, stream, blocks);
     }
     VolumeView<uint8_t> v = make_volume_view((const uint8_t *)vox, nx, ny, nz, dmin, dmax);
     if (form == MRAY_FORM_PLAIN)
     {
         int bs = plain_block(m);
         hipLaunchKernelGGL(generate_plain_kernel<uint8_t>, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
-                           bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples);
+                           bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples, rr);
         return 0;
     }
-    return dispatch_mixed<uint8_t>(v, c, a, moments, bounds, samples, stream, blocks);
+    return dispatch_mixed<uint8_t>(v, c, a, moments, bounds, samples, rr, stream, blocks);
 }
 
 } // namespace mray

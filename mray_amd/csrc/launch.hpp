@@ -12,11 +12,12 @@ namespace mray
 
 int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
                     const mray_camera &cam, const mray_gen_params &p, int form, int W, int H, int m, float *moments,
-                    uint32_t *bounds, int32_t *samples, hipStream_t stream);
+                    uint32_t *bounds, int32_t *samples, float *ray_range, hipStream_t stream);
 
 int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const float *dmin, const float *dmax,
                             const mray_camera &cam, const mray_gen_params &p, int W, int H, int m,
-                            const float *grad_moments, float *grad_volume, hipStream_t stream);
+                            const float *grad_moments, const float *ray_range, float *grad_volume,
+                            hipStream_t stream);
 
 int launch_prepare(const float *moments, const uint32_t *index, int W, int H, int m, float bias, int form,
                    float *lagrange, hipStream_t stream);

@@ -26,14 +26,14 @@ _SIGNATURES = {
     "mray_hip_volume_destroy": ([_vp], C.c_int),
     "mray_hip_volume_data": ([_vp], _vp),
     "mray_hip_generate": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, C.c_int, _vp,
-                           _vp, _vp, _vp], C.c_int),
+                           _vp, _vp, _vp, _vp], C.c_int),
     "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_reconstruct": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
     "mray_hip_reconstruct_backward": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                        C.POINTER(RecParams), _vp, _vp, _vp, _vp, _vp], C.c_int),
     "mray_hip_generate_backward": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, _vp, _vp,
-                                    _vp], C.c_int),
+                                    _vp, _vp], C.c_int),
     "mray_hip_prepare_backward": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp], C.c_int),
     "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
@@ -146,8 +146,9 @@ class Volume:
 
 
 def generate(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, form="mixed", moments=None,
-             bounds=None, samples=None, want_samples=False, stream=None):
-    """Dense moments (H, W, m), packed bounds (H*W,) uint32 (or None), samples per ray (or None)."""
+             bounds=None, samples=None, want_samples=False, ray_range=None, stream=None):
+    """Dense moments (H, W, m), packed bounds (H*W,) uint32 (or None), samples per ray (or None).
+    ray_range: optional (H*W, 2) float32 tensor that receives each ray's normalisation (lo, hi)."""
     dev = vol.voxels.device
     if moments is None:
         moments = torch.empty((H, W, m), dtype=torch.float32, device=dev)
@@ -157,7 +158,8 @@ def generate(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, form
         samples = torch.empty(H * W, dtype=torch.int32, device=dev)
     c = Camera.from_dict(cam)
     _check(load_library().mray_hip_generate(vol.handle, C.byref(c), C.byref(gp), form_id(form), W, H, m,
-                                            _ptr(moments), _ptr(bounds), _ptr(samples), _stream(stream)),
+                                            _ptr(moments), _ptr(bounds), _ptr(samples), _ptr(ray_range),
+                                            _stream(stream)),
            "mray_hip_generate")
     return moments, bounds, samples
 
@@ -208,13 +210,15 @@ def reconstruct_backward(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, 
 
 
 def generate_backward(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, grad_moments: torch.Tensor,
-                      out=None, stream=None) -> torch.Tensor:
-    """dL/dvoxels (shape of the volume) for a given dL/dmoments of generate."""
+                      out=None, ray_range=None, stream=None) -> torch.Tensor:
+    """dL/dvoxels (shape of the volume) for a given dL/dmoments of generate; ray_range is
+    the forward's per-ray normalisation (generate(..., ray_range=...)) or None to recompute it."""
     if out is None:
         out = torch.zeros_like(vol.voxels, dtype=torch.float32)
     c = Camera.from_dict(cam)
     _check(load_library().mray_hip_generate_backward(vol.handle, C.byref(c), C.byref(gp), W, H, m,
-                                                     _ptr(grad_moments.contiguous()), _ptr(out), _stream(stream)),
+                                                     _ptr(grad_moments.contiguous()), _ptr(ray_range), _ptr(out),
+                                                     _stream(stream)),
            "mray_hip_generate_backward")
     return out
 

@@ -45,7 +45,7 @@ void generate_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &img, 
     DeviceBuffer d_mom(npix * m * sizeof(float));
     const bool want_bounds = img.has_bounds() || params.transform_to_bounds;
     DeviceBuffer d_bounds(want_bounds ? npix * 4 : 0);
-    check(mray_hip_generate(vol, &c, &g, g_form, W, H, m, d_mom.as<float>(), d_bounds.as<uint32_t>(), nullptr,
+    check(mray_hip_generate(vol, &c, &g, g_form, W, H, m, d_mom.as<float>(), d_bounds.as<uint32_t>(), nullptr, nullptr,
                             nullptr),
           "generate");
     if (img.has_bounds())

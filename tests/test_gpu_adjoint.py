@@ -302,6 +302,34 @@ def test_generate_adjoint_matches_fp64_finite_differences(hip, persp, m):
         assert abs(fd - an) <= 2e-4 * np.abs(gv * d).sum()
 
 
+def test_generate_adjoint_with_forward_ray_range(hip):
+    """The forward's per-ray normalisation (ray_range) is what it packed into
+    the 16-bit bounds, and the reverse pass given it equals the reverse pass
+    that recomputes it (float atomics: the same sums in another order, bar
+    1e-5 of the largest gradient)."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    vol = synth.marschner_lobb(24)
+    dmin, dmax = (-1, -1, -1), (1, 1, 1)
+    cam = synth.perspective_camera(position=(0.1, 0.05, -2.6), fov_half_deg=22.0, aspect=1.0)
+    W = H = 20
+    m = 24
+    gp = GenParams(step_size=2.0 / 48, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    v = hip.Volume(torch.from_numpy(vol).cuda(), dmin, dmax)
+    rr = torch.empty((H * W, 2), dtype=torch.float32, device="cuda")
+    _, b, _ = hip.generate(v, cam, gp, W, H, m, form="mixed", ray_range=rr)
+    r = rr.cpu().numpy()
+    bb = b.cpu().numpy().astype(np.uint32)
+    assert np.array_equal(bb & 0xFFFF, (r[:, 0] * np.float32(65535.0)).astype(np.uint16))
+    assert np.array_equal(bb >> 16, (r[:, 1] * np.float32(65535.0)).astype(np.uint16))
+    gbar = torch.from_numpy(rng.normal(size=(H, W, m)).astype(np.float32)).cuda()
+    g1 = hip.generate_backward(v, cam, gp, W, H, m, gbar, ray_range=rr).cpu().numpy()
+    g2 = hip.generate_backward(v, cam, gp, W, H, m, gbar).cpu().numpy()
+    assert np.abs(g1 - g2).max() <= 1e-5 * np.abs(g2).max()
+
+
 def test_render_autograd_chain(hip):
     """torch.autograd through generate -> prepare -> reconstruct: the volume
     gradient equals the GPU adjoints applied by hand in sequence, and a
