@@ -131,6 +131,23 @@ __device__ __forceinline__ float div_rn(float a, float b, float y)
     return q;
 }
 
+// a / b with y = host_reciprocal(b): for a power-of-two b (POW2, see
+// exact_pow2) the exact product a * y, else div_rn
+template <bool POW2> __device__ __forceinline__ float div_c(float a, float b, float y)
+{
+    if constexpr (POW2)
+        return a * y;
+    else
+        return div_rn(a, b, y);
+}
+
+// b > 0 is a power of two whose reciprocal host_reciprocal gives (exactly)
+inline bool exact_pow2(float b)
+{
+    int e;
+    return b > 0.0f && frexpf(b, &e) == 0.5f && host_reciprocal(b) == host_reciprocal(b);
+}
+
 template <typename T>
 inline VolumeView<T> make_volume_view(const T *data, int nx, int ny, int nz, const float *dmin, const float *dmax)
 {
@@ -185,17 +202,19 @@ template <typename T> __device__ __forceinline__ float sample_volume(const Volum
 // sample are in flight while the current one is used (same arithmetic and
 // result as sample_volume). OFF32: the volume spans at most 4 GiB, so the
 // corners are addressed by 32-bit byte offsets from the base (one scalar
-// base, no 64-bit address arithmetic per load).
-template <typename T, bool OFF32 = false> struct PendingSample
+// base, no 64-bit address arithmetic per load). POW2: every extent of the
+// domain is a power of two, so dividing by it is multiplying by its exact
+// reciprocal.
+template <typename T, bool OFF32 = false, bool POW2 = false> struct PendingSample
 {
     float ax, ay, az;
     T lll, rll, lrl, rrl, llr, rlr, lrr, rrr;
 
     __device__ __forceinline__ void issue(const VolumeView<T> &v, float px, float py, float pz)
     {
-        float ipx = div_rn(px - v.dmin.x, v.dmax.x - v.dmin.x, v.rext.x) * (float)v.nx;
-        float ipy = div_rn(py - v.dmin.y, v.dmax.y - v.dmin.y, v.rext.y) * (float)v.ny;
-        float ipz = div_rn(pz - v.dmin.z, v.dmax.z - v.dmin.z, v.rext.z) * (float)v.nz;
+        const float ipx = div_c<POW2>(px - v.dmin.x, v.dmax.x - v.dmin.x, v.rext.x) * (float)v.nx;
+        const float ipy = div_c<POW2>(py - v.dmin.y, v.dmax.y - v.dmin.y, v.rext.y) * (float)v.ny;
+        const float ipz = div_c<POW2>(pz - v.dmin.z, v.dmax.z - v.dmin.z, v.rext.z) * (float)v.nz;
         float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
         ax = ipx - flx;
         ay = ipy - fly;
@@ -255,9 +274,10 @@ __device__ __forceinline__ float map_density(float s, float dmin, float dmax)
 }
 
 // map_density with host_reciprocal(dmax - dmin) formed once (same result)
+template <bool POW2 = false>
 __device__ __forceinline__ float map_density_r(float s, float dmin, float dmax, float rinv)
 {
-    return rclamp(div_rn(s - dmin, dmax - dmin, rinv), 0.0f, 1.0f);
+    return rclamp(div_c<POW2>(s - dmin, dmax - dmin, rinv), 0.0f, 1.0f);
 }
 
 // DensityBound::set (src/common.h:143-147)

@@ -255,7 +255,7 @@ template <int M> __device__ __forceinline__ void fold(float (&part)[M], double (
     }
 }
 
-template <typename T, int M, bool OFF32>
+template <typename T, int M, bool OFF32, bool POW2>
 __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
                                                              int32_t *__restrict__ samples,
@@ -334,16 +334,17 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float rdphi = host_reciprocal(dphi); // for the segment slopes
     // the next sample's loads are issued one step ahead (at t + s while the
     // march continues, else at t_far for the last segment)
-    auto issue_at = [&](PendingSample<T, OFF32> &ps, float tt) {
+    auto issue_at = [&](PendingSample<T, OFF32, POW2> &ps, float tt) {
         ps.issue(vol, r.o.x + tt * r.d.x, r.o.y + tt * r.d.y, r.o.z + tt * r.d.z);
     };
-    PendingSample<T, OFF32> pending;
+    PendingSample<T, OFF32, POW2> pending;
+    auto density = [&](float sv) { return map_density_r<POW2>(sv, a.data_min, a.data_max, a.data_rinv); };
     issue_at(pending, (t + s < t_far) ? t + s : t_far);
     for (int it = 0; t + s < t_far && it < cap; ++it)
     {
         t += s;
         const float p = (float)((double)((t - t_near) * pi_trange_inv) - kPi);
-        const float f = map_density_r(pending.value(), a.data_min, a.data_max, a.data_rinv);
+        const float f = density(pending.value());
         // the loaded values die here: keeping the next loads below this point
         // lets them land in the same registers (no copies that would wait
         // for them at the end of the iteration)
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
-    const float f_last = map_density_r(pending.value(), a.data_min, a.data_max, a.data_rinv);
+    const float f_last = density(pending.value());
     ++ns;
     lo = rmin(lo, f_last);
     hi = rmax(hi, f_last);
@@ -445,11 +446,18 @@ static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenAr
 {
     // byte offsets of every voxel fit in 32 bits up to 4 GiB of volume
     const bool off32 = (uint64_t)v.nx * v.ny * v.nz * sizeof(T) <= (uint64_t)0xFFFFFFFFu + 1u;
-    if (off32)
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s, rr);
-    else
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s,
+    // power-of-two domain extents and data range: divisions become products
+    const bool pow2 = exact_pow2(v.dmax.x - v.dmin.x) && exact_pow2(v.dmax.y - v.dmin.y) &&
+                      exact_pow2(v.dmax.z - v.dmin.z) && exact_pow2(a.data_max - a.data_min);
+    if (off32 && pow2)
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, true>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s,
                            rr);
+    else if (off32)
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b,
+                           s, rr);
+    else
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, false, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b,
+                           s, rr);
 }
 
 template <typename T>
@@ -496,7 +504,7 @@ static int plain_block(int m) { return (256 * m * (int)sizeof(float) <= 65536) ?
 // A trilinear sample of the march that keeps what its reverse needs: the
 // cell, the weights and the voxel values (same arithmetic as sample_volume;
 // loads issued in issue(), used in value()).
-struct GradSample
+template <bool POW2> struct GradSample
 {
     float ax, ay, az;
     int cx, cy, cz; // clamped (x0, y0, z0)
@@ -505,9 +513,9 @@ struct GradSample
 
     __device__ __forceinline__ void issue(const VolumeView<float> &vol, float px, float py, float pz)
     {
-        const float ipx = div_rn(px - vol.dmin.x, vol.dmax.x - vol.dmin.x, vol.rext.x) * (float)vol.nx;
-        const float ipy = div_rn(py - vol.dmin.y, vol.dmax.y - vol.dmin.y, vol.rext.y) * (float)vol.ny;
-        const float ipz = div_rn(pz - vol.dmin.z, vol.dmax.z - vol.dmin.z, vol.rext.z) * (float)vol.nz;
+        const float ipx = div_c<POW2>(px - vol.dmin.x, vol.dmax.x - vol.dmin.x, vol.rext.x) * (float)vol.nx;
+        const float ipy = div_c<POW2>(py - vol.dmin.y, vol.dmax.y - vol.dmin.y, vol.rext.y) * (float)vol.ny;
+        const float ipz = div_c<POW2>(pz - vol.dmin.z, vol.dmax.z - vol.dmin.z, vol.rext.z) * (float)vol.nz;
         const float flx = floorf(ipx), fly = floorf(ipy), flz = floorf(ipz);
         ax = ipx - flx;
         ay = ipy - fly;
@@ -623,7 +631,7 @@ struct CellScatter
             }
     }
 
-    __device__ __forceinline__ void add(const GradTarget &tg, const GradSample &sm, float gs)
+    template <bool POW2> __device__ __forceinline__ void add(const GradTarget &tg, const GradSample<POW2> &sm, float gs)
     {
         // the same corner with other neighbour flags is another cell (a
         // sample just outside the domain clamps onto the edge voxel)
@@ -721,7 +729,7 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &c)
 // constants (a first pass recomputes them for the normalisation); cos and
 // sin of the node phases come from sincos_small. One thread per ray, nothing
 // stored per sample.
-template <int M>
+template <int M, bool POW2>
 __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float> vol, DevCamera cam, GenArgs a,
                                                                const float *__restrict__ grad_mom,
                                                                const float2 *__restrict__ ray_range,
@@ -760,10 +768,12 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     const float pi_trange_inv = (float)(kPi / (double)(t_far - t_near));
     const float dphi = (float)(kPi * (double)(s / (t_far - t_near)));
     const int cap = active ? march_cap(t_near, t_far, s) : 0;
-    auto issue_at = [&](GradSample &sm, float tt) {
+    auto issue_at = [&](GradSample<POW2> &sm, float tt) {
         sm.issue(vol, r.o.x + tt * r.d.x, r.o.y + tt * r.d.y, r.o.z + tt * r.d.z);
     };
-    auto density = [&](const GradSample &sm) { return map_density_r(sm.value(), a.data_min, a.data_max, a.data_rinv); };
+    auto density = [&](const GradSample<POW2> &sm) {
+        return map_density_r<POW2>(sm.value(), a.data_min, a.data_max, a.data_rinv);
+    };
 
     // normalisation of the forward: inv = 1 / (hi - lo) with the bounds of the
     // ray, as the forward recorded them or from a pass over the samples
@@ -776,7 +786,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     else if (active && a.transform_to_bounds)
     {
         float lo = 1.f, hi = 0.f;
-        GradSample sm;
+        GradSample<POW2> sm;
         float t = t_near;
         issue_at(sm, t);
         for (int it = 0;; ++it)
@@ -835,17 +845,17 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     const float dscale = 1.0f / (a.data_max - a.data_min);
     // d f / d s of map_density
     auto dmap = [&](float sv) {
-        const float u = div_rn(sv - a.data_min, a.data_max - a.data_min, a.data_rinv);
+        const float u = div_c<POW2>(sv - a.data_min, a.data_max - a.data_min, a.data_rinv);
         return (u > 0.0f && u < 1.0f) ? dscale : 0.0f;
     };
     // clamped cell of the sample at t
     auto cell_at = [&](float tt, int &qx, int &qy, int &qz) {
         const float px = r.o.x + tt * r.d.x, py = r.o.y + tt * r.d.y, pz = r.o.z + tt * r.d.z;
-        qx = iclamp((int)floorf(div_rn(px - vol.dmin.x, vol.dmax.x - vol.dmin.x, vol.rext.x) * (float)vol.nx), 0,
+        qx = iclamp((int)floorf(div_c<POW2>(px - vol.dmin.x, vol.dmax.x - vol.dmin.x, vol.rext.x) * (float)vol.nx), 0,
                     vol.nx - 1);
-        qy = iclamp((int)floorf(div_rn(py - vol.dmin.y, vol.dmax.y - vol.dmin.y, vol.rext.y) * (float)vol.ny), 0,
+        qy = iclamp((int)floorf(div_c<POW2>(py - vol.dmin.y, vol.dmax.y - vol.dmin.y, vol.rext.y) * (float)vol.ny), 0,
                     vol.ny - 1);
-        qz = iclamp((int)floorf(div_rn(pz - vol.dmin.z, vol.dmax.z - vol.dmin.z, vol.rext.z) * (float)vol.nz), 0,
+        qz = iclamp((int)floorf(div_c<POW2>(pz - vol.dmin.z, vol.dmax.z - vol.dmin.z, vol.rext.z) * (float)vol.nz), 0,
                     vol.nz - 1);
     };
 
@@ -860,7 +870,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         phi_prev = (float)(-kPi);
     // cur: the sample of the node behind the current segment; its loads are
     // issued one node ahead and used after the next node's sums
-    GradSample cur;
+    GradSample<POW2> cur;
     double co = 1.0, si = 0.0, C_prev = 0.0, S_prev = 0.0;
     if (active)
     {
@@ -1027,11 +1037,17 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
     const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
     const uint32_t blocks = (ntiles + 3) / 4;
     VolumeView<float> v = make_volume_view(vox, nx, ny, nz, dmin, dmax);
+    const bool pow2 = exact_pow2(v.dmax.x - v.dmin.x) && exact_pow2(v.dmax.y - v.dmin.y) &&
+                      exact_pow2(v.dmax.z - v.dmin.z) && exact_pow2(a.data_max - a.data_min);
 #define MRAY_GADJ_CASE(MM)                                                                                            \
     if (m <= MM)                                                                                                      \
     {                                                                                                                 \
-        hipLaunchKernelGGL(generate_adjoint_kernel<MM>, dim3(blocks), dim3(256), 0, stream, v, c, a, grad_moments,    \
-                           rr, grad_volume);                                                                          \
+        if (pow2)                                                                                                     \
+            hipLaunchKernelGGL((generate_adjoint_kernel<MM, true>), dim3(blocks), dim3(256), 0, stream, v, c, a,      \
+                               grad_moments, rr, grad_volume);                                                        \
+        else                                                                                                          \
+            hipLaunchKernelGGL((generate_adjoint_kernel<MM, false>), dim3(blocks), dim3(256), 0, stream, v, c, a,     \
+                               grad_moments, rr, grad_volume);                                                        \
         return 0;                                                                                                     \
     }
     MRAY_GADJ_CASE(8)
