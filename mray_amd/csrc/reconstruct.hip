@@ -355,6 +355,9 @@ __device__ __forceinline__ float fast_atan_deriv(float t)
 // (c_0 = 1, c_l = 2), cos(l theta) by the Chebyshev recurrence in FP64.
 // Transfer-function gradients are summed per workgroup in LDS and added to
 // grad_tf with one atomic per entry and workgroup.
+constexpr int kTfCopies = 8;
+constexpr int kTfStride = 4 * 256 + 1;
+
 template <int M>
 __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     DevCamera cam, RecArgs a, const float *__restrict__ lag, const uint32_t *__restrict__ index,
@@ -362,16 +365,20 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     const float4 *__restrict__ grad_rgba, float *__restrict__ grad_lag, float *__restrict__ grad_tf)
 {
     __shared__ float4 tf[256];
-    // table gradient per workgroup, channel-major (gtf[ch * 256 + entry]) so
-    // that the lanes of one atomic instruction spread over the LDS banks
-    __shared__ float gtf[4 * 256];
+    // table gradient per workgroup, channel-major (ch * 256 + entry), in
+    // kTfCopies copies: lane l adds into copy l % kTfCopies. Neighbouring rays
+    // read the same entries, and the atomics of one instruction that hit one
+    // address are serialised; the copies start one bank apart (stride
+    // 1024 + 1 floats), so neighbouring lanes on one entry use different banks
+    __shared__ float gtf_all[kTfCopies * kTfStride];
     const int tid = threadIdx.x;
     tf[tid] = tf_g[tid];
-    for (int i = 0; i < 4; ++i)
-        gtf[tid * 4 + i] = 0.0f;
+    for (int i = tid; i < kTfCopies * kTfStride; i += blockDim.x)
+        gtf_all[i] = 0.0f;
     __syncthreads();
 
     const int lane = tid & 63;
+    float *gtf = gtf_all + (lane % kTfCopies) * kTfStride;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
     int x = 0, y = 0;
@@ -539,7 +546,10 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     if (grad_tf)
         for (int i = tid; i < 1024; i += blockDim.x)
         {
-            const float v = gtf[i]; // channel i / 256 of entry i % 256
+            float v = 0.0f; // channel i / 256 of entry i % 256
+#pragma unroll
+            for (int c = 0; c < kTfCopies; ++c)
+                v += gtf_all[c * kTfStride + i];
             if (v != 0.0f)
                 atomicAdd(&grad_tf[(i & 255) * 4 + (i >> 8)], v);
         }
