@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kAdjThreads) void prepare_adjoint_kernel(const floa
 // prepare_mixed_kernel: lane g holds elements g, g + 8, ... (C slots) of the
 // per-element registers (P, its adjoint Pb, Eb, Tb and the recorded
 // reflection coefficients U); arrays that lanes read at other elements'
-// offsets live in LDS per pixel (E, P and two work arrays X, Y of 8C complex
+// offsets live in LDS per pixel (E, P and a work array X of 8C complex
 // doubles; T overlays X at the ends). Sums over a pixel's elements reduce
 // over its 8 lanes (grp8_sum), single elements come from their lane by a
 // shuffle inside the group.
@@ -427,10 +427,10 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     const bool valid = pix < npix;
     const int nm = valid ? (index ? min((int)(index[pix + 1] - index[pix]), m) : m) : 0;
     const size_t off = valid ? (index ? (size_t)index[pix] : (size_t)pix * m) : 0;
-    // per-pixel stride 4 MP + 4 elements: consecutive pixels start 64 bytes
+    // per-pixel stride 3 MP + 4 elements: consecutive pixels start 64 bytes
     // (16 banks) apart, so the 8-byte accesses of two pixels served together
     // do not share banks
-    dc *E = lds_adj + q * (4 * MP + 4), *P = E + MP, *X = P + MP, *Y = X + MP;
+    dc *E = lds_adj + q * (3 * MP + 4), *P = E + MP, *X = P + MP;
     double *T = reinterpret_cast<double *>(X);
     const double b64 = (double)bias;
     auto tval = [&](int e) {
@@ -494,15 +494,14 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     wave_sync();
 #pragma unroll
     for (int s = 0; s < C; ++s)
-    {
-        X[g + 8 * s] = Rk[s];          // R
-        Y[g + 8 * s] = dc{0.0, 0.0};
-    }
+        X[g + 8 * s] = Rk[s]; // R
     wave_sync();
     const double p0s = (nm > 0) ? P[0].x : 1.0;
     const double nf = 1.0 / (kPi * p0s);
-    // incoming gradients lb_kk, in Y (as doubles)
-    double *Yd = reinterpret_cast<double *>(Y);
+    wave_sync();
+    // incoming gradients lb_kk, as doubles in P's space (P is in registers
+    // until the reverse autocorrelation)
+    double *Yd = reinterpret_cast<double *>(P);
 #pragma unroll
     for (int s = 0; s < C; ++s)
     {
@@ -569,7 +568,10 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     // ---- reverse: autocorrelation, R_kk = sum_j P_j conj(P_{j+kk}) ----
 #pragma unroll
     for (int s = 0; s < C; ++s)
+    {
         X[g + 8 * s] = Rb[s];
+        P[g + 8 * s] = R.P[s];
+    }
     wave_sync();
 #pragma unroll
     for (int s = 0; s < C; ++s)
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__
     wave_sync();
 
     // ---- reverse: Levinson, rebuilding each previous iterate ----
-    levinson_reverse_octaves<C, C>(R, P, Y, E, g, nm, j_invalid);
+    levinson_reverse_octaves<C, C>(R, P, X, E, g, nm, j_invalid);
     // P_0 = 1 / E_0.x (E_0 doubled), then the recurrence's undoubled E_0
     if (g == 0)
     {
@@ -635,7 +637,7 @@ int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, i
     if (m <= 64)
     {
         const int C = (m + 7) / 8;
-        const size_t lds = (size_t)8 * (4 * 8 * C + 4) * sizeof(dc);
+        const size_t lds = (size_t)8 * (3 * 8 * C + 4) * sizeof(dc);
         const dim3 grid((npix + 7) / 8);
 #define MRAY_PADJ_CASE(CC)                                                                                            \
     case CC:                                                                                                          \
