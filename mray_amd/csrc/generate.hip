@@ -209,24 +209,43 @@ __device__ __forceinline__ void sincos_phase(float p, float &s, float &c)
     c = (q == 0) ? cr : (q == 3) ? sr : -cr;
 }
 
-// part[j] (+)= (d + j^2 e) cos(j phi) + j h sin(j phi) for j = 1..M-1 (the
-// partial sums are scaled by j^2, so the h sin / j term enters as j h sin);
-// FIRST starts a new partial sum. cos and sin of j phi by the Chebyshev
-// recurrence v_j = 2x v_{j-1} - v_{j-2} from x = cos(phi), y = sin(phi).
-template <int M, bool FIRST>
-__device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float e, float h, float x, float y)
+// The moment's partial sum for j = 1..M-1, scaled by j^2, over a node is
+// (d + j^2 e) cos(j phi) + j h sin(j phi). SPLIT keeps the sine term apart,
+// part[j] += (d + j^2 e) cos(j phi), part_s[j] += h sin(j phi), the sum being
+// part[j] + j part_s[j]: five operations per moment instead of six, for M more
+// registers; taken for the moment counts where that keeps the kernel's
+// wavefronts per SIMD (8, 12, 20, 28 and 40 to 50 on gfx950).
+// cos and sin of j phi by the Chebyshev recurrence v_j = 2x v_{j-1} - v_{j-2}
+// from x = cos(phi), y = sin(phi).
+template <int M>
+constexpr bool kSplitSine = M == 8 || M == 12 || M == 20 || M == 28 || (M >= 40 && M <= 50);
+
+template <int M>
+__device__ __forceinline__ void accumulate_node(float (&part)[M], float (&part_s)[M], float d, float e, float h,
+                                                float x, float y)
 {
+    constexpr bool SPLIT = kSplitSine<M>;
     const float x2 = x + x;
     float cm2 = 1.0f, cm1 = x;
     float sm2 = 0.0f, sm1 = y;
-    part[1] = __builtin_fmaf(h, sm1, FIRST ? (d + e) * cm1 : __builtin_fmaf(d + e, cm1, part[1]));
+    part[1] = __builtin_fmaf(d + e, cm1, part[1]);
+    if constexpr (SPLIT)
+        part_s[1] = __builtin_fmaf(h, sm1, part_s[1]);
+    else
+        part[1] = __builtin_fmaf(h, sm1, part[1]);
 #pragma unroll
     for (int j = 2; j < M; ++j)
     {
         const float c = __builtin_fmaf(x2, cm1, -cm2);
         const float sn = __builtin_fmaf(x2, sm1, -sm2);
         const float w = __builtin_fmaf((float)(j * j), e, d);
-        part[j] = __builtin_fmaf((float)j * h, sn, FIRST ? w * c : __builtin_fmaf(w, c, part[j]));
+        if constexpr (SPLIT)
+        {
+            part[j] = __builtin_fmaf(w, c, part[j]);
+            part_s[j] = __builtin_fmaf(h, sn, part_s[j]);
+        }
+        else
+            part[j] = __builtin_fmaf((float)j * h, sn, __builtin_fmaf(w, c, part[j]));
         cm2 = cm1;
         cm1 = c;
         sm2 = sm1;
@@ -234,23 +253,19 @@ __device__ __forceinline__ void accumulate_node(float (&part)[M], float d, float
     }
 }
 
-template <int M>
-__device__ __forceinline__ void accumulate_node(bool first, float (&part)[M], float d, float e, float h, float x,
-                                                float y)
-{
-    if (first)
-        accumulate_node<M, true>(part, d, e, h, x, y);
-    else
-        accumulate_node<M, false>(part, d, e, h, x, y);
-}
-
 // adds the partial sums to the FP64 totals and restarts them
-template <int M> __device__ __forceinline__ void fold(float (&part)[M], double (&sum)[M])
+template <int M> __device__ __forceinline__ void fold(float (&part)[M], float (&part_s)[M], double (&sum)[M])
 {
 #pragma unroll
     for (int j = 1; j < M; ++j)
     {
-        sum[j] += (double)part[j];
+        if constexpr (kSplitSine<M>)
+        {
+            sum[j] += (double)__builtin_fmaf((float)j, part_s[j], part[j]);
+            part_s[j] = 0.0f;
+        }
+        else
+            sum[j] += (double)part[j];
         part[j] = 0.0f;
     }
 }
@@ -317,12 +332,13 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     sincos_phase(node_phase, zs, zc);
     const float z0s = zs, z0c = zc;
 
-    float part[M];
+    float part[M], part_s[M];
     double sum[M];
 #pragma unroll
     for (int j = 0; j < M; ++j)
     {
         part[j] = 0.0f;
+        part_s[j] = 0.0f;
         sum[j] = 0.0;
     }
     double trap = 0.0;
@@ -360,7 +376,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         trap += width * ((double)f_prev + 0.5 * (double)g * width);
         const float delta = (node > 0) ? (float)((double)u - (double)node_phase) : 0.0f;
         const float h = __builtin_fmaf(g, delta, g_left * eps_left);
-        accumulate_node<M, false>(part, g_left - g, -delta * f_prev, h, zc, zs);
+        accumulate_node<M>(part, part_s, g_left - g, -delta * f_prev, h, zc, zs);
         g_left = g;
         eps_left = (float)(width - (double)dphi);
         f_prev = f;
@@ -368,7 +384,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         ++node;
         sincos_phase(p, zs, zc);
         if (node % kFold == 0)
-            fold<M>(part, sum);
+            fold<M>(part, part_s, sum);
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
@@ -379,12 +395,12 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float dl = 0.0f - node_phase;
     const float g_last = (f_last - f_prev) / dl;
     trap += (double)dl * ((double)f_prev + 0.5 * (double)g_last * (double)dl);
-    accumulate_node<M, false>(part, g_left - g_last, 0.0f, g_left * eps_left, zc, zs);
+    accumulate_node<M>(part, part_s, g_left - g_last, 0.0f, g_left * eps_left, zc, zs);
     // node at phase 0: cos(j*0) = 1, sin(j*0) = 0
 #pragma unroll
     for (int j = 1; j < M; ++j)
         part[j] += g_last;
-    fold<M>(part, sum);
+    fold<M>(part, part_s, sum);
 
     // trig_j = sum_j / j^2 - f_0 sin(j phi_0) / j, sin by Chebyshev from phi_0.
     float trig[M];
