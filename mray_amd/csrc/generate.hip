@@ -20,8 +20,8 @@
 //         reference's arithmetic-free result to ~1e-13:
 //             + sum_k [ -delta_k f_k cos(j p_k) + h_k sin(j p_k) / j ].
 //         A node costs one sincos of its (float) phase and two Chebyshev
-//         recurrences over j (cos and sin of j phi): 6 VALU ops per moment
-//         instead of 2(m-1) sin/cos. The phasor is not carried from node to
+//         recurrences over j (cos and sin of j phi): 5 or 6 VALU ops per
+//         moment (accumulate_node) instead of 2(m-1) sin/cos. The phasor is not carried from node to
 //         node by rotation: the reference's node phases jitter by ulps around
 //         the uniform grid, and a rotated phasor drifts off cos(p_k) by more
 //         than the high orders tolerate.
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 // MIXED
 // ---------------------------------------------------------------------------
-constexpr int kFold = 8;    // nodes per FP32 partial sum before the FP64 add
+constexpr int kFold = 16;   // nodes per FP32 partial sum before the FP64 add
 
 // sin and cos of a node phase p in [-3.3, 0.1]: p = k pi/2 + r with k =
 // rint(2p / pi) in {-2, -1, 0}, r taken off in two float parts of pi/2 (k * hi
