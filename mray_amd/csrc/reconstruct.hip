@@ -354,9 +354,20 @@ __device__ __forceinline__ float fast_atan_deriv(float t)
 // d rho / d S = fast_atan'(S) / pi (hi - lo) and dS / d lambda_l = c_l cos(l theta)
 // (c_0 = 1, c_l = 2), cos(l theta) by the Chebyshev recurrence in FP64.
 // Transfer-function gradients are summed per workgroup in LDS and added to
-// grad_tf with one atomic per entry and workgroup.
-constexpr int kTfCopies = 8;
-constexpr int kTfStride = 4 * 256 + 1;
+// grad_tf with one atomic per entry and workgroup. The LDS sums are 64-bit
+// fixed point (integer atomics: float atomics on LDS cost several times as
+// much here, above all when the rays of a wavefront add to the same entry);
+// the workgroup's scale 2^kfix comes from a bound on what each ray can add to
+// one entry, so that no partial sum reaches 2^61.
+
+// v 2^k as a 64-bit integer (the fraction truncated); |v| 2^k < 2^53
+__device__ __forceinline__ unsigned long long to_fixed(float v, int k)
+{
+    const double x = ldexp((double)v, k);
+    const double hi = floor(x * 0x1p-32);
+    const double lo = fma(-hi, 0x1p32, x); // in [0, 2^32), exact
+    return ((unsigned long long)(uint32_t)(int)hi << 32) | (unsigned long long)(uint32_t)lo;
+}
 
 template <int M>
 __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
@@ -365,20 +376,22 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     const float4 *__restrict__ grad_rgba, float *__restrict__ grad_lag, float *__restrict__ grad_tf)
 {
     __shared__ float4 tf[256];
-    // table gradient per workgroup, channel-major (ch * 256 + entry), in
-    // kTfCopies copies: lane l adds into copy l % kTfCopies. Neighbouring rays
-    // read the same entries, and the atomics of one instruction that hit one
-    // address are serialised; the copies start one bank apart (stride
-    // 1024 + 1 floats), so neighbouring lanes on one entry use different banks
-    __shared__ float gtf_all[kTfCopies * kTfStride];
+    // table gradient per workgroup, channel-major (ch * 256 + entry), fixed point
+    __shared__ unsigned long long gtf[4 * 256];
+    __shared__ uint32_t cmax_bits, bound_bits;
     const int tid = threadIdx.x;
-    tf[tid] = tf_g[tid];
-    for (int i = tid; i < kTfCopies * kTfStride; i += blockDim.x)
-        gtf_all[i] = 0.0f;
+    const float4 tv = tf_g[tid];
+    tf[tid] = tv;
+    if (grad_tf)
+        for (int i = tid; i < 4 * 256; i += blockDim.x)
+            gtf[i] = 0ull;
+    if (tid == 0)
+        cmax_bits = bound_bits = 0u;
     __syncthreads();
+    // the largest colour of the table (non-negative floats order as their bits)
+    atomicMax(&cmax_bits, __float_as_uint(fmaxf(fabsf(tv.x), fmaxf(fabsf(tv.y), fabsf(tv.z)))));
 
     const int lane = tid & 63;
-    float *gtf = gtf_all + (lane % kTfCopies) * kTfStride;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
     int x = 0, y = 0;
@@ -388,21 +401,60 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
         tile_pixel(tile, a.W, lane, x, y);
         active = x < a.W && y < a.H;
     }
+    const size_t pix = active ? (size_t)y * a.W + x : 0;
+    const int nm = !active ? 0 : index ? min((int)(index[pix + 1] - index[pix]), a.m) : a.m;
+    const float4 g = active ? grad_rgba[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 L = active ? rgba_fwd[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    const bool hit = intersect_aabb(r, a.dmin, a.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    const int cap = march_cap(t_near, t_far, a.step) + 1;
+    __syncthreads();
+    // bound on what this ray adds to one table entry: per sample at most
+    // |dL/dc| = |g_rgb e| T and |dL/dsigma| <= sum |g_ch| (|L_ch| + |P_ch|) + |g_a| T_end
+    // (weights w01 + w23 = 1), at most cap samples; twice that for rounding
+    float lane_bound = 0.0f;
+    if (active && grad_tf)
+    {
+        const float ga = fabsf(g.x) + fabsf(g.y) + fabsf(g.z);
+        if (nm == 0)
+            lane_bound = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w)));
+        else if (hit && (t_far > 0))
+        {
+            const float e = fabsf(a.emission), n = (float)cap;
+            const float pmax = e * __uint_as_float(cmax_bits) * n;
+            const float gc = e * fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
+            const float gs = fabsf(g.x) * fabsf(L.x) + fabsf(g.y) * fabsf(L.y) + fabsf(g.z) * fabsf(L.z) + ga * pmax +
+                             fabsf(g.w) * fabsf(1.0f - L.w);
+            lane_bound = 2.0f * n * fmaxf(gc, gs);
+        }
+    }
+    if (grad_tf)
+        atomicMax(&bound_bits, __float_as_uint(lane_bound));
+    __syncthreads();
+    int kfix = 0;
+    {
+        int ex = 0;
+        const float bsum = __uint_as_float(bound_bits) * (float)blockDim.x;
+        if (bsum > 0.0f && bsum < 3.0e38f)
+        {
+            frexpf(bsum, &ex); // bsum < 2^ex
+            kfix = 61 - ex;
+        }
+    }
     if (active)
     {
-        const size_t pix = (size_t)y * a.W + x;
-        const int nm = index ? min((int)(index[pix + 1] - index[pix]), a.m) : a.m;
         const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
-        const float4 g = grad_rgba[pix];
         if (nm == 0)
         {
             // reconstruct_zero_ray: the colour is the table at (0, 0) = entry 0
             if (grad_tf)
             {
-                atomicAdd(&gtf[0], g.x);
-                atomicAdd(&gtf[256], g.y);
-                atomicAdd(&gtf[512], g.z);
-                atomicAdd(&gtf[768], g.w);
+                atomicAdd(&gtf[0], to_fixed(g.x, kfix));
+                atomicAdd(&gtf[256], to_fixed(g.y, kfix));
+                atomicAdd(&gtf[512], to_fixed(g.z, kfix));
+                atomicAdd(&gtf[768], to_fixed(g.w, kfix));
             }
         }
         else
@@ -423,14 +475,9 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
             }
             const float brange = bhi - blo;
             const bool fourier = a.fourier != 0;
-            const float4 L = rgba_fwd[pix];
             const double t_end = 1.0 - (double)L.w;
             const double ge[3] = {(double)g.x * a.emission, (double)g.y * a.emission, (double)g.z * a.emission};
 
-            Ray r = pixel_ray(cam, a.W, a.H, x, y);
-            float t_near, t_far;
-            const bool hit = intersect_aabb(r, a.dmin, a.dmax, t_near, t_far);
-            t_near = rmax(t_near, cam.znear);
             if (hit && (t_far > 0))
             {
                 float t = t_near;
@@ -452,12 +499,11 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
 #pragma unroll
                     for (int ch = 0; ch < 4; ++ch)
                     {
-                        atomicAdd(&gtf[ch * 256 + tf_x0], tf_a[ch]);
-                        atomicAdd(&gtf[ch * 256 + tf_x1], tf_b[ch]);
+                        atomicAdd(&gtf[ch * 256 + tf_x0], to_fixed(tf_a[ch], kfix));
+                        atomicAdd(&gtf[ch * 256 + tf_x1], to_fixed(tf_b[ch], kfix));
                         tf_a[ch] = tf_b[ch] = 0.0f;
                     }
                 };
-                const int cap = march_cap(t_near, t_far, a.step) + 1;
                 for (int it = 0; t <= t_far && it < cap; ++it)
                 {
                     t += a.step;
@@ -546,10 +592,8 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     if (grad_tf)
         for (int i = tid; i < 1024; i += blockDim.x)
         {
-            float v = 0.0f; // channel i / 256 of entry i % 256
-#pragma unroll
-            for (int c = 0; c < kTfCopies; ++c)
-                v += gtf_all[c * kTfStride + i];
+            // channel i / 256 of entry i % 256
+            const float v = (float)ldexp((double)(long long)gtf[i], -kfix);
             if (v != 0.0f)
                 atomicAdd(&grad_tf[(i & 255) * 4 + (i >> 8)], v);
         }

@@ -104,6 +104,40 @@ def test_adjoint_matches_oracle(hip, name):
         assert rel_tf <= 2e-3
 
 
+def test_table_gradient_scale_invariance(hip):
+    """The table gradient is summed per workgroup in 64-bit fixed point at a
+    scale taken from a bound on each ray's contribution: scaling dL/drgba by
+    2^-40 or 2^40 (exact in float) scales dL/dtable by the same factor, to
+    1e-6 normwise; a zero incoming gradient gives exactly zero."""
+    import torch
+
+    O = Oracle()
+    name = list(CASES)[0]
+    dmin, dmax, cam, W, H, m, rp, lag, table, b = setup(O, name)
+    rng = np.random.default_rng(11)
+    g = torch.from_numpy(rng.normal(size=(H, W, 4)).astype(np.float32)).cuda()
+    lag_d = torch.from_numpy(lag).cuda()
+    b_d = torch.from_numpy(b.view(np.int32)).cuda()
+    tab_d = torch.from_numpy(table).cuda()
+    print(f"[table gradient scale] case={name}", flush=True)
+    rgba, _ = hip.reconstruct(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, form="mixed", bounds=b_d)
+    ref = None
+    for e in (0, -40, 40):
+        _, g_tf = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, rgba, g * 2.0 ** e,
+                                           bounds=b_d, want_lagrange=False)
+        v = g_tf.cpu().numpy().astype(np.float64) * 2.0 ** (-e)
+        if ref is None:
+            ref = v
+            assert np.linalg.norm(ref) > 0
+        else:
+            rel = np.linalg.norm(v - ref) / np.linalg.norm(ref)
+            print(f"  scale 2^{e}: rel {rel:.2e}")
+            assert rel <= 1e-6
+    _, g0 = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, rgba, torch.zeros_like(g),
+                                     bounds=b_d, want_lagrange=False)
+    assert torch.count_nonzero(g0).item() == 0
+
+
 def test_adjoint_finite_differences_on_gpu(hip):
     """Directional derivatives of L = sum(g * rgba) w.r.t. the table, central
     differences with eps = 1e-3 of the GPU forward, no early termination:
