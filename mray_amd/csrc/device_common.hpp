@@ -360,6 +360,34 @@ __device__ __forceinline__ int march_cap(float t_near, float t_far, float s)
     return (int)(n * 1.01f) + 16;
 }
 
+// v 2^k as a 64-bit two's complement integer, the fraction truncated
+// (|v| 2^k < 2^53): the summand of the fixed-point LDS accumulators, whose
+// integer atomics cost a fraction of float atomics on LDS
+__device__ __forceinline__ unsigned long long to_fixed(float v, int k)
+{
+    const double x = ldexp((double)v, k);
+    const double hi = floor(x * 0x1p-32);
+    const double lo = fma(-hi, 0x1p32, x); // in [0, 2^32), exact
+    return ((unsigned long long)(uint32_t)(int)hi << 32) | (unsigned long long)(uint32_t)lo;
+}
+
+// the value of a fixed-point sum q at scale 2^k
+__device__ __forceinline__ float from_fixed(unsigned long long q, int k)
+{
+    return (float)ldexp((double)(long long)q, -k);
+}
+
+// exponent of a scale for sums bounded by b: 2^k b < 2^61 (0 for b = 0 or
+// not finite)
+__device__ __forceinline__ int fixed_scale(float b)
+{
+    int ex = 0;
+    if (!(b > 0.0f && b < 3.0e38f))
+        return 0;
+    frexpf(b, &ex); // b < 2^ex
+    return 61 - ex;
+}
+
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD gets a
 // contiguous run of tiles (neighbouring rays share its L2).
 __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks)

@@ -566,13 +566,14 @@ template <bool POW2> struct GradSample
 };
 
 // Where the gradient of a cell's eight voxels goes: a workgroup's LDS window
-// over a box of voxels (summed there, added to the volume once per voxel and
-// chunk of the march) or, for corners outside the window, straight to the
-// volume with float atomics.
+// over a box of voxels (summed there in 64-bit fixed point at scale 2^kfix,
+// added to the volume once per voxel and chunk of the march) or, for corners
+// outside the window, straight to the volume with float atomics.
 struct GradTarget
 {
     float *grad;
-    float *win; // LDS window, x fastest
+    unsigned long long *win; // LDS window, x fastest
+    int kfix;
     int ox, oy, oz, ex, ey, ez;
     bool lds;
     int nx;
@@ -587,12 +588,12 @@ struct GradTarget
         const int lx = cx - ox, ly = cy - oy, lz = cz - oz;
         if (lds && lx >= 0 && ly >= 0 && lz >= 0 && lx + fx < ex && ly + fy < ey && lz + fz < ez)
         {
-            float *q0 = win + (lx + ex * (ly + ey * lz));
+            unsigned long long *q0 = win + (lx + ex * (ly + ey * lz));
             const int dy = fy * ex, dz = fz * ex * ey;
 #pragma unroll
             for (int i = 0; i < 8; ++i)
                 if (MASK & (1u << i))
-                    atomicAdd(q0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), w[i]);
+                    atomicAdd(q0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), to_fixed(w[i], kfix));
             return;
         }
         float *p0 = grad + ((size_t)cz * sxy + (uint32_t)(cy * nx + cx));
@@ -755,13 +756,17 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     // they agree on the box of voxels they can touch; gradients go to an LDS
     // window over that box (when it fits), which is then added to the volume
     // with one atomic per voxel instead of one per ray, sample and corner.
-    constexpr int kWin = 8192; // floats of LDS window
+    constexpr int kWin = 8192; // voxels of LDS window (64-bit fixed point)
     constexpr int kChunk = 32; // nodes per chunk and ray
-    __shared__ float win[kWin];
+    __shared__ unsigned long long win[kWin];
     __shared__ int bbox[6];
+    __shared__ uint32_t bound_bits;
     const int tid = threadIdx.x;
     for (int i = tid; i < kWin; i += blockDim.x)
-        win[i] = 0.0f;
+        win[i] = 0ull;
+    if (tid == 0)
+        bound_bits = 0u;
+    __syncthreads();
 
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
@@ -859,6 +864,20 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         S = si * (b1 + fma(co, d1, -d2));
     };
     const float dscale = 1.0f / (a.data_max - a.data_min);
+    // fixed-point scale of the window: a node adds at most |fbar| |dscale| to
+    // a voxel (trilinear weights sum to 1), |fbar| <= |A| + |B| <= 6.2 sum_j j |c_j|
+    // + 2 pi |g0| (|C'| and |S| <= sum_j j |c_j|); 8 (...) with margin, times the
+    // kChunk + 1 nodes a ray adds per chunk, over the workgroup
+    {
+        double sj = 0.0;
+#pragma unroll
+        for (int j = 1; j < M; ++j)
+            sj = fma((double)j, fabs(cc[j]), sj);
+        const float lb = active ? (float)(8.0 * (sj + fabs(g0)) * (double)fabsf(dscale) * (kChunk + 1)) : 0.0f;
+        atomicMax(&bound_bits, __float_as_uint(lb));
+    }
+    __syncthreads();
+    const int kfix = fixed_scale(__uint_as_float(bound_bits) * (float)blockDim.x);
     // d f / d s of map_density
     auto dmap = [&](float sv) {
         const float u = div_c<POW2>(sv - a.data_min, a.data_max - a.data_min, a.data_rinv);
@@ -900,6 +919,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     GradTarget tg;
     tg.grad = grad_vol;
     tg.win = win;
+    tg.kfix = kfix;
     tg.nx = vol.nx;
     tg.sxy = (uint32_t)(vol.nx * vol.ny);
     int it = 0;
@@ -1017,10 +1037,11 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
             const int n = tg.ex * tg.ey * tg.ez;
             for (int i = tid; i < n; i += blockDim.x)
             {
-                const float v = win[i];
-                if (v != 0.0f)
+                const unsigned long long q = win[i];
+                if (q != 0ull)
                 {
-                    win[i] = 0.0f;
+                    win[i] = 0ull;
+                    const float v = from_fixed(q, kfix);
                     const int lx = i % tg.ex, rest = i / tg.ex;
                     const int ly = rest % tg.ey, lz = rest / tg.ey;
                     unsafeAtomicAdd(grad_vol + ((size_t)(tg.oz + lz) * tg.sxy +

@@ -360,14 +360,6 @@ __device__ __forceinline__ float fast_atan_deriv(float t)
 // the workgroup's scale 2^kfix comes from a bound on what each ray can add to
 // one entry, so that no partial sum reaches 2^61.
 
-// v 2^k as a 64-bit integer (the fraction truncated); |v| 2^k < 2^53
-__device__ __forceinline__ unsigned long long to_fixed(float v, int k)
-{
-    const double x = ldexp((double)v, k);
-    const double hi = floor(x * 0x1p-32);
-    const double lo = fma(-hi, 0x1p32, x); // in [0, 2^32), exact
-    return ((unsigned long long)(uint32_t)(int)hi << 32) | (unsigned long long)(uint32_t)lo;
-}
 
 template <int M>
 __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
@@ -433,16 +425,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     if (grad_tf)
         atomicMax(&bound_bits, __float_as_uint(lane_bound));
     __syncthreads();
-    int kfix = 0;
-    {
-        int ex = 0;
-        const float bsum = __uint_as_float(bound_bits) * (float)blockDim.x;
-        if (bsum > 0.0f && bsum < 3.0e38f)
-        {
-            frexpf(bsum, &ex); // bsum < 2^ex
-            kfix = 61 - ex;
-        }
-    }
+    const int kfix = fixed_scale(__uint_as_float(bound_bits) * (float)blockDim.x);
     if (active)
     {
         const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
@@ -593,7 +576,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
         for (int i = tid; i < 1024; i += blockDim.x)
         {
             // channel i / 256 of entry i % 256
-            const float v = (float)ldexp((double)(long long)gtf[i], -kfix);
+            const float v = from_fixed(gtf[i], kfix);
             if (v != 0.0f)
                 atomicAdd(&grad_tf[(i & 255) * 4 + (i >> 8)], v);
         }

@@ -364,6 +364,37 @@ def test_generate_adjoint_with_forward_ray_range(hip):
     assert np.abs(g1 - g2).max() <= 1e-5 * np.abs(g2).max()
 
 
+def test_generate_adjoint_scale_invariance(hip):
+    """The workgroup's voxel window sums in 64-bit fixed point at a scale
+    from a bound on each ray's contribution: dL/dmoments scaled by 2^-40 or
+    2^40 (exact in float) scales dL/dvoxels by the same factor (1e-6
+    normwise), and a zero incoming gradient gives exactly zero."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    vol = synth.marschner_lobb(24)
+    dmin, dmax = (-1, -1, -1), (1, 1, 1)
+    cam = synth.perspective_camera(position=(0.1, 0.05, -2.6), fov_half_deg=22.0, aspect=1.0)
+    W = H = 20
+    m = 24
+    gp = GenParams(step_size=2.0 / 48, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    print(f"[generate adjoint scale] W={W} H={H} m={m}", flush=True)
+    v = hip.Volume(torch.from_numpy(vol).cuda(), dmin, dmax)
+    rr = torch.empty((H * W, 2), dtype=torch.float32, device="cuda")
+    hip.generate(v, cam, gp, W, H, m, form="mixed", ray_range=rr)
+    gbar = torch.from_numpy(rng.normal(size=(H, W, m)).astype(np.float32)).cuda()
+    ref = hip.generate_backward(v, cam, gp, W, H, m, gbar, ray_range=rr).cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(ref) > 0
+    for e in (-40, 40):
+        ge = hip.generate_backward(v, cam, gp, W, H, m, gbar * 2.0 ** e, ray_range=rr).cpu().numpy()
+        rel = np.linalg.norm(ge.astype(np.float64) * 2.0 ** (-e) - ref) / np.linalg.norm(ref)
+        print(f"  scale 2^{e}: rel {rel:.2e}")
+        assert rel <= 1e-6
+    g0 = hip.generate_backward(v, cam, gp, W, H, m, torch.zeros_like(gbar), ray_range=rr)
+    assert torch.count_nonzero(g0).item() == 0
+
+
 def test_render_autograd_chain(hip):
     """torch.autograd through generate -> prepare -> reconstruct: the volume
     gradient equals the GPU adjoints applied by hand in sequence, and a
