@@ -607,10 +607,10 @@ struct GradTarget
 };
 
 // Accumulates gs times the trilinear weights of a sample. Consecutive
-// samples of a ray in the same cell are merged in registers; when the ray
-// moves on to a neighbouring cell along one axis, the four corners of the
-// shared face stay in registers and only the other four go to the target,
-// else all eight do. Clamp-to-edge duplicates receive both weights.
+// samples of a ray in the same cell are merged in registers; the eight
+// corners go to the target when the ray leaves the cell (one code path for
+// every lane: keeping a shared face in registers saved atomics but split the
+// wavefront over six cases). Clamp-to-edge duplicates receive both weights.
 struct CellScatter
 {
     int cx = -1, cy = 0, cz = 0;
@@ -625,56 +625,16 @@ struct CellScatter
         cx = -1;
     }
 
-    // the ray moved by +1 (UP) or -1 along the axis of corner bit B: the face
-    // it leaves goes to the target, the shared face moves to the other side
-    template <int B, bool UP> __device__ __forceinline__ void step_face(const GradTarget &tg)
-    {
-        constexpr uint32_t kHigh = (B == 1) ? 0xAAu : (B == 2) ? 0xCCu : 0xF0u;
-        tg.add<UP ? (~kHigh & 0xFFu) : kHigh>(cx, cy, cz, fl, w);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (!(i & B))
-            {
-                if (UP)
-                {
-                    w[i] = w[i | B];
-                    w[i | B] = 0.0f;
-                }
-                else
-                {
-                    w[i | B] = w[i];
-                    w[i] = 0.0f;
-                }
-            }
-    }
-
     template <bool POW2> __device__ __forceinline__ void add(const GradTarget &tg, const GradSample<POW2> &sm, float gs)
     {
         // the same corner with other neighbour flags is another cell (a
         // sample just outside the domain clamps onto the edge voxel)
         if (!(sm.cx == cx && sm.cy == cy && sm.cz == cz && sm.fl == fl))
         {
-            const int dx = sm.cx - cx, dy = sm.cy - cy, dz = sm.cz - cz;
-            const bool same_fl = cx >= 0 && sm.fl == fl;
-            if (same_fl && dy == 0 && dz == 0 && dx == 1 && (fl & 1u))
-                step_face<1, true>(tg);
-            else if (same_fl && dy == 0 && dz == 0 && dx == -1 && (fl & 1u))
-                step_face<1, false>(tg);
-            else if (same_fl && dx == 0 && dz == 0 && dy == 1 && (fl & 2u))
-                step_face<2, true>(tg);
-            else if (same_fl && dx == 0 && dz == 0 && dy == -1 && (fl & 2u))
-                step_face<2, false>(tg);
-            else if (same_fl && dx == 0 && dy == 0 && dz == 1 && (fl & 4u))
-                step_face<4, true>(tg);
-            else if (same_fl && dx == 0 && dy == 0 && dz == -1 && (fl & 4u))
-                step_face<4, false>(tg);
-            else
-            {
-                flush(tg);
+            flush(tg);
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    w[i] = 0.0f;
-            }
+            for (int i = 0; i < 8; ++i)
+                w[i] = 0.0f;
             cx = sm.cx;
             cy = sm.cy;
             cz = sm.cz;
