@@ -329,18 +329,19 @@ __device__ __forceinline__ float4 tf_sample(const float4 *tf, float fs, float bs
     return o;
 }
 
-// tf_sample with the weights of each table entry summed first (w01 = w0 + w1,
-// w23 = w2 + w3) and fused multiply-adds: the same lookup up to rounding, two
-// operations per channel instead of seven (MIXED form only).
+// tf_sample with the weights of each table entry summed first: w0 + w1 =
+// (1 - ax)(1 - ay) + ax (1 - ay) = 1 - ay and w2 + w3 = ay exactly, so the
+// front sample's fraction drops out; fused multiply-adds, two operations per
+// channel instead of seven. The same lookup up to rounding, closer to its
+// exact value (MIXED form only).
 __device__ __forceinline__ float4 tf_sample_mixed(const float4 *tf, float fs, float bs)
 {
     const float ipx = fs * 256.0f, ipy = bs * 256.0f;
-    const float ax = ipx - floorf(ipx);
     const float ay = ipy - floorf(ipy);
     const int xl = (int)floorf(ipx);
     const int x0 = iclamp(xl, 0, 255), x1 = iclamp(xl + 1, 0, 255);
     const float4 l = tf[x0], r = tf[x1];
-    const float w01 = (1.f - ax) * (1.f - ay) + ax * (1.f - ay), w23 = (1.f - ax) * ay + ax * ay;
+    const float w01 = 1.f - ay, w23 = ay;
     float4 o;
     o.x = __builtin_fmaf(w23, r.x, w01 * l.x);
     o.y = __builtin_fmaf(w23, r.y, w01 * l.y);

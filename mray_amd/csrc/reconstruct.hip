@@ -499,11 +499,10 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
 
                     // indices and weights of tf_sample_mixed
                     const float ipx = fs * 256.0f, ipy = bs * 256.0f;
-                    const float ax = ipx - floorf(ipx), ay = ipy - floorf(ipy);
+                    const float ay = ipy - floorf(ipy);
                     const int xl = (int)floorf(ipx);
                     const int x0 = iclamp(xl, 0, 255), x1 = iclamp(xl + 1, 0, 255);
-                    const float w01 = (1.f - ax) * (1.f - ay) + ax * (1.f - ay);
-                    const float w23 = (1.f - ax) * ay + ax * ay;
+                    const float w01 = 1.f - ay, w23 = ay;
                     const float4 lo4 = tf[x0], hi4 = tf[x1];
 
                     const double Td = (double)T;
