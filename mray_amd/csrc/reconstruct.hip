@@ -263,10 +263,17 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
             steps[pix] = 0;
         return;
     }
+    // the M loads are unconditional (index clamped into the pixel's nm >= 1
+    // entries, the surplus zeroed after), so that they are in flight together:
+    // one conditional load and wait after another let L2 evict the rows'
+    // cache lines between the loads (3.4x the multipliers' bytes from HBM)
     double lam[M];
 #pragma unroll
     for (int l = 0; l < M; ++l)
-        lam[l] = (l < nm) ? (double)lag[off + l] : 0.0;
+    {
+        const float v = lag[off + min(l, nm - 1)];
+        lam[l] = (l < nm) ? (double)v : 0.0;
+    }
     float blo = 0.f, bhi = 1.f;
     if (bounds)
     {
@@ -446,7 +453,8 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
 #pragma unroll
             for (int l = 0; l < M; ++l)
             {
-                lam[l] = (l < nm) ? lag[off + l] : 0.0f;
+                const float v = lag[off + min(l, nm - 1)]; // unconditional, as in the forward
+                lam[l] = (l < nm) ? v : 0.0f;
                 glam[l] = 0.0f;
             }
             float blo = 0.f, bhi = 1.f;
