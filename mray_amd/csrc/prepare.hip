@@ -127,23 +127,34 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     levinson_octaves<1, C, false>(A, E, g, nm, j_invalid, u_unused);
 
     // autocorrelation of P = 2 pi A (computeAutocorrelation), written over A
+    // (one pass over j for all slots: A_j, the same for the pixel's 8 lanes,
+    // is read once per j instead of once per slot and j)
     const double a0x = (nm > 0) ? A[0].x : 1.0;
     dc R[C];
 #pragma unroll
     for (int s = 0; s < C; ++s)
+        R[s] = dc{0.0, 0.0};
+    for (int j = 0; j < nm; ++j)
     {
-        const int k = g + 8 * s;
-        double rx = 0.0, ry = 0.0;
-        if (k < nm)
-            for (int j = 0; j < nm - k; ++j)
+        const dc a = A[j];
+#pragma unroll
+        for (int s = 0; s < C; ++s)
+        {
+            const int k = g + 8 * s;
+            if (j + k < nm)
             {
-                const dc a = A[j], b = A[j + k];
+                const dc b = A[j + k];
                 // a * conj(b)
-                rx = fma(a.x, b.x, fma(a.y, b.y, rx));
-                ry = fma(a.y, b.x, fma(-a.x, b.y, ry));
+                R[s].x = fma(a.x, b.x, fma(a.y, b.y, R[s].x));
+                R[s].y = fma(a.y, b.x, fma(-a.x, b.y, R[s].y));
             }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+    {
         const double sc = 4.0 * kPi * kPi;
-        R[s] = dc{sc * rx, sc * ry};
+        R[s] = dc{sc * R[s].x, sc * R[s].y};
     }
     wave_sync();
 #pragma unroll
@@ -158,21 +169,32 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     wave_sync();
 
     // imaginary correlation (computeImaginaryCorrelation) and normalisation
+    // (one pass over j for all slots, E_j read once per j)
     const double nf = 1.0 / (kPi * 2.0 * kPi * a0x);
+    double acc[C];
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+        acc[s] = 0.0;
+    for (int j = 0; j < nm; ++j)
+    {
+        const dc e = E[j];
+#pragma unroll
+        for (int s = 0; s < C; ++s)
+        {
+            const int k = g + 8 * s;
+            if (j + k < nm)
+            {
+                const dc r = A[k + j];
+                acc[s] = fma(r.x, e.y, fma(r.y, e.x, acc[s]));
+            }
+        }
+    }
 #pragma unroll
     for (int s = 0; s < C; ++s)
     {
         const int k = g + 8 * s;
         if (k < nm)
-        {
-            double acc = 0.0;
-            for (int j = 0; j < nm - k; ++j)
-            {
-                const dc r = A[k + j], e = E[j];
-                acc = fma(r.x, e.y, fma(r.y, e.x, acc));
-            }
-            out[off + k] = (float)(nf * acc);
-        }
+            out[off + k] = (float)(nf * acc[s]);
     }
 }
 
