@@ -138,6 +138,55 @@ def test_table_gradient_scale_invariance(hip):
     assert torch.count_nonzero(g0).item() == 0
 
 
+def test_reconstruct_adjoint_compact_matches_dense(hip):
+    """Compact multipliers (index with 0, 5 and m entries per pixel): the
+    reverse pass with the index equals the dense one on the zero-padded
+    multipliers (the series is the same), and pixels without moments add
+    their incoming gradient to table entry 0 (the zero ray's colour); both
+    paths sum the table gradient in fixed point. Bar 1e-5 normwise (float
+    atomics across workgroups)."""
+    import torch
+
+    O = Oracle()
+    name = list(CASES)[0]
+    dmin, dmax, cam, W, H, m, rp, lag, table, b = setup(O, name)
+    rng = np.random.default_rng(13)
+    npix = W * H
+    nm = np.where(np.arange(npix) % 3 == 0, 0, np.where(np.arange(npix) % 3 == 1, 5, m)).astype(np.int64)
+    index = np.concatenate([[0], np.cumsum(nm)]).astype(np.uint32)
+    lag2 = lag.reshape(npix, m)
+    compact = np.concatenate([lag2[p, :nm[p]] for p in range(npix)]).astype(np.float32)
+    dense = np.zeros_like(lag2)
+    for p in range(npix):
+        dense[p, :nm[p]] = lag2[p, :nm[p]]
+    g = rng.normal(size=(H, W, 4)).astype(np.float32)
+    g_dense = g.copy()
+    g_dense.reshape(npix, 4)[nm == 0] = 0.0
+    print(f"[reconstruct adjoint compact] case={name} W={W} H={H} m={m}", flush=True)
+    tab_d = torch.from_numpy(table).cuda()
+    b_d = torch.from_numpy(b.view(np.int32)).cuda()
+    c_d = torch.from_numpy(compact).cuda()
+    i_d = torch.from_numpy(index.view(np.int32)).cuda()
+    d_d = torch.from_numpy(dense.reshape(-1)).cuda()
+    rgba_c, _ = hip.reconstruct(cam, dmin, dmax, c_d, W, H, m, tab_d, rp, form="mixed", index=i_d, bounds=b_d)
+    rgba_d, _ = hip.reconstruct(cam, dmin, dmax, d_d, W, H, m, tab_d, rp, form="mixed", bounds=b_d)
+    gl_c, gt_c = hip.reconstruct_backward(cam, dmin, dmax, c_d, W, H, m, tab_d, rp, rgba_c,
+                                          torch.from_numpy(g).cuda(), index=i_d, bounds=b_d)
+    gl_d, gt_d = hip.reconstruct_backward(cam, dmin, dmax, d_d, W, H, m, tab_d, rp, rgba_d,
+                                          torch.from_numpy(g_dense).cuda(), bounds=b_d)
+    gl_c = gl_c.cpu().numpy()
+    gl_d = gl_d.cpu().numpy().reshape(npix, m)
+    expect = np.concatenate([gl_d[p, :nm[p]] for p in range(npix)])
+    rel_l = np.linalg.norm(gl_c - expect) / np.linalg.norm(expect)
+    zero_rays = g.reshape(npix, 4)[nm == 0].astype(np.float64).sum(axis=0)
+    gt_expect = gt_d.cpu().numpy().astype(np.float64)
+    gt_expect[0] += zero_rays
+    rel_t = np.linalg.norm(gt_c.cpu().numpy() - gt_expect) / np.linalg.norm(gt_expect)
+    print(f"  multipliers rel {rel_l:.2e}, table rel {rel_t:.2e}")
+    assert rel_l <= 1e-5
+    assert rel_t <= 1e-5
+
+
 def test_adjoint_finite_differences_on_gpu(hip):
     """Directional derivatives of L = sum(g * rgba) w.r.t. the table, central
     differences with eps = 1e-3 of the GPU forward, no early termination:
