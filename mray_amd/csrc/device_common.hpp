@@ -378,6 +378,11 @@ __device__ __forceinline__ float from_fixed(unsigned long long q, int k)
     return (float)ldexp((double)(long long)q, -k);
 }
 
+// whether sums bounded by b can be kept in fixed point (b finite: the
+// incoming gradients are; otherwise the callers add floats, so that infinities
+// and NaNs propagate as they would through float sums)
+__device__ __forceinline__ bool fixed_usable(float b) { return b < 3.0e38f; }
+
 // exponent of a scale for sums bounded by b: 2^k b < 2^61 (0 for b = 0 or
 // not finite)
 __device__ __forceinline__ int fixed_scale(float b)

@@ -837,7 +837,10 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         atomicMax(&bound_bits, __float_as_uint(lb));
     }
     __syncthreads();
-    const int kfix = fixed_scale(__uint_as_float(bound_bits) * (float)blockDim.x);
+    const float bsum = __uint_as_float(bound_bits) * (float)blockDim.x;
+    const int kfix = fixed_scale(bsum);
+    // not finite: no window, float atomics straight to the volume
+    const bool fixed = fixed_usable(bsum);
     // d f / d s of map_density
     auto dmap = [&](float sv) {
         const float u = div_c<POW2>(sv - a.data_min, a.data_max - a.data_min, a.data_rinv);
@@ -916,7 +919,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         tg.ex = __builtin_amdgcn_readfirstlane(bbox[3]) - tg.ox + 1;
         tg.ey = __builtin_amdgcn_readfirstlane(bbox[4]) - tg.oy + 1;
         tg.ez = __builtin_amdgcn_readfirstlane(bbox[5]) - tg.oz + 1;
-        tg.lds = tg.ex > 0 && tg.ey > 0 && tg.ez > 0 && (long long)tg.ex * tg.ey * tg.ez <= kWin;
+        tg.lds = fixed && tg.ex > 0 && tg.ey > 0 && tg.ez > 0 && (long long)tg.ex * tg.ey * tg.ez <= kWin;
         for (int k = 0; k < kChunk && !done; ++k, ++it)
         {
             const bool last = !(t + s < t_far) || it >= cap;

@@ -432,7 +432,16 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     if (grad_tf)
         atomicMax(&bound_bits, __float_as_uint(lane_bound));
     __syncthreads();
-    const int kfix = fixed_scale(__uint_as_float(bound_bits) * (float)blockDim.x);
+    const float bsum = __uint_as_float(bound_bits) * (float)blockDim.x;
+    const int kfix = fixed_scale(bsum);
+    const bool fixed = fixed_usable(bsum); // else float atomics straight to grad_tf
+    // adds v to channel ch of table entry e
+    auto tf_add = [&](int ch, int e, float v) {
+        if (fixed)
+            atomicAdd(&gtf[ch * 256 + e], to_fixed(v, kfix));
+        else
+            atomicAdd(&grad_tf[e * 4 + ch], v);
+    };
     if (active)
     {
         const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
@@ -441,10 +450,10 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
             // reconstruct_zero_ray: the colour is the table at (0, 0) = entry 0
             if (grad_tf)
             {
-                atomicAdd(&gtf[0], to_fixed(g.x, kfix));
-                atomicAdd(&gtf[256], to_fixed(g.y, kfix));
-                atomicAdd(&gtf[512], to_fixed(g.z, kfix));
-                atomicAdd(&gtf[768], to_fixed(g.w, kfix));
+                tf_add(0, 0, g.x);
+                tf_add(1, 0, g.y);
+                tf_add(2, 0, g.z);
+                tf_add(3, 0, g.w);
             }
         }
         else
@@ -490,8 +499,8 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
 #pragma unroll
                     for (int ch = 0; ch < 4; ++ch)
                     {
-                        atomicAdd(&gtf[ch * 256 + tf_x0], to_fixed(tf_a[ch], kfix));
-                        atomicAdd(&gtf[ch * 256 + tf_x1], to_fixed(tf_b[ch], kfix));
+                        tf_add(ch, tf_x0, tf_a[ch]);
+                        tf_add(ch, tf_x1, tf_b[ch]);
                         tf_a[ch] = tf_b[ch] = 0.0f;
                     }
                 };
@@ -579,7 +588,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
         }
     }
     __syncthreads();
-    if (grad_tf)
+    if (grad_tf && fixed)
         for (int i = tid; i < 1024; i += blockDim.x)
         {
             // channel i / 256 of entry i % 256

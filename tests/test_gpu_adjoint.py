@@ -108,7 +108,8 @@ def test_table_gradient_scale_invariance(hip):
     """The table gradient is summed per workgroup in 64-bit fixed point at a
     scale taken from a bound on each ray's contribution: scaling dL/drgba by
     2^-40 or 2^40 (exact in float) scales dL/dtable by the same factor, to
-    1e-6 normwise; a zero incoming gradient gives exactly zero."""
+    1e-6 normwise; a zero incoming gradient gives exactly zero, and a
+    non-finite one gives a non-finite table gradient."""
     import torch
 
     O = Oracle()
@@ -136,6 +137,13 @@ def test_table_gradient_scale_invariance(hip):
     _, g0 = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, rgba, torch.zeros_like(g),
                                      bounds=b_d, want_lagrange=False)
     assert torch.count_nonzero(g0).item() == 0
+    # a non-finite incoming gradient is not dropped by the fixed-point sums:
+    # that workgroup adds floats, and the table gradient is non-finite
+    gi = g.clone()
+    gi[H // 2, W // 2, 3] = float("inf")
+    _, gt_inf = hip.reconstruct_backward(cam, dmin, dmax, lag_d, W, H, m, tab_d, rp, rgba, gi, bounds=b_d,
+                                         want_lagrange=False)
+    assert not torch.isfinite(gt_inf).all()
 
 
 def test_reconstruct_adjoint_compact_matches_dense(hip):
@@ -417,7 +425,8 @@ def test_generate_adjoint_scale_invariance(hip):
     """The workgroup's voxel window sums in 64-bit fixed point at a scale
     from a bound on each ray's contribution: dL/dmoments scaled by 2^-40 or
     2^40 (exact in float) scales dL/dvoxels by the same factor (1e-6
-    normwise), and a zero incoming gradient gives exactly zero."""
+    normwise), a zero incoming gradient gives exactly zero, and a non-finite
+    one gives a non-finite gradient (not a silently dropped summand)."""
     import torch
 
     rng = np.random.default_rng(9)
@@ -442,6 +451,11 @@ def test_generate_adjoint_scale_invariance(hip):
         assert rel <= 1e-6
     g0 = hip.generate_backward(v, cam, gp, W, H, m, torch.zeros_like(gbar), ray_range=rr)
     assert torch.count_nonzero(g0).item() == 0
+    # a non-finite incoming gradient reaches the voxels (float atomics then)
+    gi = gbar.clone()
+    gi[H // 2, W // 2, 1] = float("inf")
+    g_inf = hip.generate_backward(v, cam, gp, W, H, m, gi, ray_range=rr)
+    assert not torch.isfinite(g_inf).all()
 
 
 def test_render_autograd_chain(hip):
