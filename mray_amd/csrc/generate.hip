@@ -376,7 +376,14 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         trap += width * ((double)f_prev + 0.5 * (double)g * width);
         const float delta = (node > 0) ? (float)((double)u - (double)node_phase) : 0.0f;
         const float h = __builtin_fmaf(g, delta, g_left * eps_left);
-        accumulate_node<M>(part, part_s, g_left - g, -delta * f_prev, h, zc, zs);
+        // a node with d = e = h = 0 (inside a stretch of zero density:
+        // both slopes and the start value vanish) adds exact zeros; where
+        // every lane of the wavefront has one, the update is skipped
+        {
+            const float dn = g_left - g, en = -delta * f_prev;
+            if (dn != 0.0f || en != 0.0f || h != 0.0f)
+                accumulate_node<M>(part, part_s, dn, en, h, zc, zs);
+        }
         g_left = g;
         eps_left = (float)(width - (double)dphi);
         f_prev = f;
