@@ -586,9 +586,8 @@ struct GradTarget
     int nx;
     uint32_t sxy;
 
-    // adds w[i] to corner i of the cell for the corners in MASK (corner i:
-    // bit 0 / 1 / 2 of i select x1 / y1 / z1)
-    template <uint32_t MASK>
+    // adds w[i] to corner i of the cell (corner i: bit 0 / 1 / 2 of i select
+    // x1 / y1 / z1)
     __device__ __forceinline__ void add(int cx, int cy, int cz, uint32_t fl, const float (&w)[8]) const
     {
         const int fx = fl & 1, fy = (fl >> 1) & 1, fz = (fl >> 2) & 1;
@@ -599,17 +598,14 @@ struct GradTarget
             const int dy = fy * ex, dz = fz * ex * ey;
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                if (MASK & (1u << i))
-                    atomicAdd(q0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), to_fixed(w[i], kfix));
+                atomicAdd(q0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), to_fixed(w[i], kfix));
             return;
         }
         float *p0 = grad + ((size_t)cz * sxy + (uint32_t)(cy * nx + cx));
         const uint32_t dy = fy ? (uint32_t)nx : 0u, dz = fz ? sxy : 0u;
 #pragma unroll
         for (int i = 0; i < 8; ++i)
-            if (MASK & (1u << i))
-                unsafeAtomicAdd(p0 + ((i & 1) ? (uint32_t)fx : 0u) + ((i & 2) ? dy : 0u) + ((i & 4) ? (size_t)dz : 0u),
-                                w[i]);
+            unsafeAtomicAdd(p0 + ((i & 1) ? (uint32_t)fx : 0u) + ((i & 2) ? dy : 0u) + ((i & 4) ? (size_t)dz : 0u), w[i]);
     }
 };
 
@@ -628,7 +624,7 @@ struct CellScatter
     {
         if (cx < 0)
             return;
-        tg.add<0xFFu>(cx, cy, cz, fl, w);
+        tg.add(cx, cy, cz, fl, w);
         cx = -1;
     }
 
@@ -844,7 +840,9 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         atomicMax(&bound_bits, __float_as_uint(lb));
     }
     __syncthreads();
-    const float bsum = __uint_as_float(bound_bits) * (float)blockDim.x;
+    // the same value in every lane: read as a scalar, so the branches on it
+    // are uniform
+    const float bsum = __uint_as_float(__builtin_amdgcn_readfirstlane(bound_bits)) * (float)blockDim.x;
     const int kfix = fixed_scale(bsum);
     // not finite: no window, float atomics straight to the volume
     const bool fixed = fixed_usable(bsum);

@@ -432,16 +432,16 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     if (grad_tf)
         atomicMax(&bound_bits, __float_as_uint(lane_bound));
     __syncthreads();
-    const float bsum = __uint_as_float(bound_bits) * (float)blockDim.x;
+    // the same value in every lane: read as a scalar, so the branches on it
+    // are uniform
+    const float bsum = __uint_as_float(__builtin_amdgcn_readfirstlane(bound_bits)) * (float)blockDim.x;
     const int kfix = fixed_scale(bsum);
-    const bool fixed = fixed_usable(bsum); // else float atomics straight to grad_tf
+    // not finite (a non-finite incoming gradient): the workgroup's table
+    // gradient is that value in every entry, rather than summands lost in
+    // their conversion
+    const bool fixed = fixed_usable(bsum);
     // adds v to channel ch of table entry e
-    auto tf_add = [&](int ch, int e, float v) {
-        if (fixed)
-            atomicAdd(&gtf[ch * 256 + e], to_fixed(v, kfix));
-        else
-            atomicAdd(&grad_tf[e * 4 + ch], v);
-    };
+    auto tf_add = [&](int ch, int e, float v) { atomicAdd(&gtf[ch * 256 + e], to_fixed(v, kfix)); };
     if (active)
     {
         const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
@@ -588,11 +588,11 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
         }
     }
     __syncthreads();
-    if (grad_tf && fixed)
+    if (grad_tf)
         for (int i = tid; i < 1024; i += blockDim.x)
         {
             // channel i / 256 of entry i % 256
-            const float v = from_fixed(gtf[i], kfix);
+            const float v = fixed ? from_fixed(gtf[i], kfix) : bsum;
             if (v != 0.0f)
                 atomicAdd(&grad_tf[(i & 255) * 4 + (i >> 8)], v);
         }
