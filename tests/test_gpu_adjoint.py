@@ -311,6 +311,44 @@ def test_prepare_adjoint_matches_fp64_finite_differences(hip, name):
     assert worst <= 1e-5
 
 
+@pytest.mark.parametrize("m", [72, 100])
+def test_prepare_adjoint_large_m(hip, m):
+    """Beyond 64 moments the preparation reverse runs one thread per pixel
+    with a global scratch: dL/dmoments against central differences of the
+    FP64 oracle preparation, as test_prepare_adjoint_matches_fp64_finite_differences
+    (moments from the oracle's generate)."""
+    import torch
+
+    O = Oracle()
+    vol = synth.marschner_lobb(32)
+    dmin, dmax = (-1, -1, -1), (1, 1, 1)
+    cam = synth.ortho_camera()
+    W, H = 10, 8
+    gp = GenParams(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    print(f"[prepare adjoint large m] m={m} W={W} H={H}", flush=True)
+    mo, _, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    mo = mo.reshape(-1, m)
+    bias = 5e-5
+    rng = np.random.default_rng(23)
+    gbar = rng.normal(size=(W * H, m)).astype(np.float32)
+    gm = hip.prepare_backward(torch.from_numpy(np.ascontiguousarray(mo)).cuda(), W, H, m, bias,
+                              torch.from_numpy(gbar).cuda()).cpu().numpy().astype(np.float64)
+    worst = 0.0
+    for p in range(0, W * H, 7):
+        x = mo[p].astype(np.float64)
+        d = rng.normal(size=m) * 0.1
+        eps = 1e-6
+        lp = O.prepare_std_f64(m, x + eps * d, bias)
+        lm = O.prepare_std_f64(m, x - eps * d, bias)
+        fd = float(gbar[p].astype(np.float64) @ (lp - lm)) / (2 * eps)
+        an = float(gm[p] @ d)
+        scale = float(np.abs(gm[p] * d).sum()) + 1e-12
+        worst = max(worst, abs(fd - an) / scale)
+    print(f"  worst |fd - adjoint| / sum|terms| {worst:.2e}")
+    assert worst <= 1e-5
+
+
 def test_pipeline_adjoint_reconstruct_prepare(hip):
     """dL/dmoments through the march and the preparation (GPU adjoints
     chained) against central differences of the FP64 pipeline (oracle

@@ -237,6 +237,36 @@ def test_prepare_parity(hip, O, name, form):
     assert excess.max() <= 0.0
 
 
+@pytest.mark.parametrize("m", [64, 72, 100, 128])
+def test_prepare_mixed_large_m(hip, O, m):
+    """MIXED preparation beyond the parity cases' 50 moments, up to the
+    kernel's 128 (16 element slots per lane): densities against the FP64
+    oracle at 41 phases with the float-storage bound of test_prepare_parity.
+    Moments from the oracle's generate (the mixed generate stops at 64)."""
+    import torch
+
+    vol = synth.marschner_lobb(32)
+    dmin, dmax = (-1, -1, -1), (1, 1, 1)
+    cam = synth.ortho_camera()
+    W, H = 12, 10
+    gp = GenParams(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                   prediction_coding=0)
+    print(f"[prepare large m] m={m} W={W} H={H}", flush=True)
+    mo_ref, _, ns_ref = O.generate(vol, dmin, dmax, cam, W, H, m, gp)
+    bias = 5e-5
+    lag = to_np(hip.prepare(torch.from_numpy(mo_ref).cuda(), W, H, m, bias, form="mixed")).reshape(-1, m)
+    phases = np.linspace(-np.pi, 0.0, 41)
+    worst = -1.0
+    for p in np.nonzero(ns_ref > 0)[0][::4]:
+        l64 = O.prepare_std_f64(m, mo_ref.reshape(-1, m)[p].astype(np.float64), bias)
+        r64 = densities64(O, m, l64, phases)
+        bound = (abs(l64[0]) + 2.0 * np.abs(l64[1:]).sum()) * 2.0 ** -24 / np.pi * 1.01
+        err = np.abs(densities64(O, m, lag[p], phases) - r64).max()
+        worst = max(worst, err - (bound + 2e-6))
+    print(f"  worst excess over allowance {worst:.2e}")
+    assert worst <= 0.0
+
+
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 @pytest.mark.parametrize("name", ["ml_ortho_m30", "ml_persp_m40", "mix_persp_m50", "mixw_persp_m50", "u8_ortho_m16",
                                   "tiny_m2"])
