@@ -85,6 +85,10 @@ CASES = {
     "u8_ortho_m16": (lambda: np.clip(synth.mixing_layer(32) * 255, 0, 255).astype(np.uint8), (0, 0, 0), (1, 1, 1),
                      synth.ortho_camera(pos=(0.5, 0.5, -1.0), right=(0.6, 0, 0), up=(0, 0.6, 0)), 24, 24, 16,
                      dict(step_size=1.0 / 64, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
+    # the largest mixed generate / reconstruct instantiation (M = 64)
+    "ml_ortho_m64": (lambda: synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 20, 16, 64,
+                     dict(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                          prediction_coding=0, bias=5e-5)),
     "tiny_m1": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 1,
                 dict(step_size=2.0 / 20, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
     "tiny_m2": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 2,
@@ -109,7 +113,7 @@ def gpu_volume(hip, vol, dmin, dmax):
 
 # opacity scale of the transfer function per case (the reference's
 # preintegration_step_size_factor), chosen for a partly transparent image
-TF_FACTOR = {"ml_ortho_m30": 4.0, "ml_persp_m40": 4.0, "mix_persp_m50": 40.0, "mixw_persp_m50": 40.0,
+TF_FACTOR = {"ml_ortho_m64": 4.0, "ml_ortho_m30": 4.0, "ml_persp_m40": 4.0, "mix_persp_m50": 40.0, "mixw_persp_m50": 40.0,
              "ml_nobounds_m7": 4.0, "u8_ortho_m16": 20.0, "tiny_m1": 4.0, "tiny_m2": 4.0}
 
 
@@ -269,7 +273,7 @@ def test_prepare_mixed_large_m(hip, O, m):
 
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 @pytest.mark.parametrize("name", ["ml_ortho_m30", "ml_persp_m40", "mix_persp_m50", "mixw_persp_m50", "u8_ortho_m16",
-                                  "tiny_m2"])
+                                  "tiny_m2", "ml_ortho_m64"])
 def test_reconstruct_parity(hip, O, name, form):
     """The march alone, on the FP32 oracle's multipliers: RMSE against the
     FP64 evaluation of the same multipliers <= 1e-4, or no worse than 1.5x the
