@@ -76,6 +76,23 @@ int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_pa
                       int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, float *d_ray_range,
                       mray_stream stream);
 
+/* ---- reference direct volume rendering (src/moment_raymarcher.cpp:74-81) -- */
+
+/* The reference renderer: the volume marched directly through the transfer
+ * function (no moments), the ground truth of the moment-based images.
+ * Replaces generate_reference(cam, tf, RegularGrid, img, ReferenceParameters)
+ * (src/moment_raymarcher.cpp:74-81) -> reference_host
+ * (src/moment_raymarcher.cpp:47-61) / generate_reference_device ->
+ * reference_ray (src/raymarch_reference.h:93-162), without single scattering
+ * (p->single_scattering with p->medium_scattering > 0 returns
+ * MRAY_ERR_UNSUPPORTED).
+ *   d_tf      256 float4 entries, as for mray_hip_reconstruct
+ *   d_rgba    W*H float4, un-clamped float RGBA (clamp+truncate gives the
+ *             reference's RGBA8)
+ *   d_steps   optional W*H int32: compositing steps per ray */
+int mray_hip_reference(mray_volume vol, const mray_camera *cam, const float *d_tf, const mray_ref_params *p, int W,
+                       int H, float *d_rgba, int32_t *d_steps, mray_stream stream);
+
 /* ---- prepare (src/moment_image.cpp:154-183, src/moment_image_device.cu:279-335) */
 
 /* Bias lerp + bounded-MESE Lagrange multipliers for every pixel (dense or

@@ -95,6 +95,20 @@ typedef struct mray_rec_params
     int32_t use_truncated_fourier; /* 0 */
 } mray_rec_params;
 
+/* ReferenceParameters (src/parameters.h:339-350) of the reference direct
+ * volume renderer, without single scattering (SingleScatteringParameters
+ * .enable = false, its default). Same names and defaults. */
+typedef struct mray_ref_params
+{
+    float step_size;             /* 1.0 */
+    float medium_emission;       /* 0.1 */
+    float medium_scattering;     /* 0.1 (unused without single scattering) */
+    float early_ray_termination; /* 0.95 */
+    float data_min;              /* 0.0 */
+    float data_max;              /* 1.0 */
+    int32_t single_scattering;   /* 0 (ss.enable); nonzero is not supported */
+} mray_ref_params;
+
 /* DensityBound (src/common.h:135-150) is stored as one uint32 per pixel:
  * lower | (upper << 16). */
 #define MRAY_BOUND_LO(b) ((uint16_t)((b)&0xFFFFu))

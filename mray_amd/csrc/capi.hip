@@ -172,6 +172,24 @@ int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_pa
     return check_status(rc, "mray_hip_generate");
 }
 
+int mray_hip_reference(mray_volume vol, const mray_camera *cam, const float *d_tf, const mray_ref_params *p, int W,
+                       int H, float *d_rgba, int32_t *d_steps, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_reference: call mray_hip_init first");
+    if (!vol || !cam || !d_tf || !p || !d_rgba)
+        return fail(MRAY_ERR_ARG, "mray_hip_reference: null argument");
+    if (bad_image(W, H, 1))
+        return fail(MRAY_ERR_ARG, "mray_hip_reference: bad image size %dx%d", W, H);
+    if (!(p->step_size > 0.0f) || !(p->data_max != p->data_min))
+        return fail(MRAY_ERR_ARG, "mray_hip_reference: step_size must be > 0 and data_max != data_min");
+    int rc = mray::launch_reference(vol->data, vol->dtype, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax, *cam, *p,
+                                    W, H, d_tf, d_rgba, d_steps, (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_reference: single scattering is not supported by this build");
+    return check_status(rc, "mray_hip_reference");
+}
+
 int mray_hip_prepare(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias, int form,
                      float *d_lagrange, mray_stream stream)
 {

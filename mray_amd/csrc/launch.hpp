@@ -22,6 +22,10 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
 int launch_prepare(const float *moments, const uint32_t *index, int W, int H, int m, float bias, int form,
                    float *lagrange, hipStream_t stream);
 
+int launch_reference(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                     const mray_camera &cam, const mray_ref_params &p, int W, int H, const float *tf, float *rgba,
+                     int32_t *steps, hipStream_t stream);
+
 int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
                        const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
                        const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream);

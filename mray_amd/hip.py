@@ -11,7 +11,7 @@ import pathlib
 
 import torch
 
-from .types import Camera, GenParams, RecParams, MRAY_DTYPE_F32, MRAY_DTYPE_U8, MRAY_FORM_MIXED, MRAY_FORM_PLAIN
+from .types import Camera, GenParams, RecParams, RefParams, MRAY_DTYPE_F32, MRAY_DTYPE_U8, MRAY_FORM_MIXED, MRAY_FORM_PLAIN
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmray_hip.so"
 
@@ -30,6 +30,8 @@ _SIGNATURES = {
     "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_reconstruct": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_reference": ([_vp, C.POINTER(Camera), _vp, C.POINTER(RefParams), C.c_int, C.c_int, _vp, _vp, _vp],
+                           C.c_int),
     "mray_hip_reconstruct_backward": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                        C.POINTER(RecParams), _vp, _vp, _vp, _vp, _vp], C.c_int),
     "mray_hip_generate_backward": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, _vp, _vp,
@@ -171,6 +173,25 @@ def prepare(moments: torch.Tensor, W: int, H: int, m: int, bias: float, form="mi
     _check(load_library().mray_hip_prepare(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias), form_id(form),
                                            _ptr(out), _stream(stream)), "mray_hip_prepare")
     return out
+
+
+def reference(vol: Volume, cam: dict, tf: torch.Tensor, p: RefParams, W: int, H: int, rgba=None, steps=None,
+              want_steps=False, stream=None):
+    """The reference direct volume renderer (mray_hip_reference): float RGBA
+    (H, W, 4) of the volume marched through the table, and optionally the
+    compositing steps per ray."""
+    dev = vol.voxels.device
+    if rgba is None:
+        rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    if steps is None and want_steps:
+        steps = torch.empty(H * W, dtype=torch.int32, device=dev)
+    c = Camera.from_dict(cam)
+    tf = tf.to(device=dev, dtype=torch.float32).contiguous()
+    if tf.numel() != 1024:
+        raise MrayError("transfer function table must have 256 RGBA entries")
+    _check(load_library().mray_hip_reference(vol.handle, C.byref(c), _ptr(tf), C.byref(p), W, H, _ptr(rgba),
+                                             _ptr(steps), _stream(stream)), "mray_hip_reference")
+    return rgba, steps
 
 
 def reconstruct(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m: int, tf: torch.Tensor,

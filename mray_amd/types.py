@@ -84,3 +84,24 @@ class RecParams(C.Structure):
                         data_min=0.0, data_max=1.0, step_size=0.01, use_truncated_fourier=0)
         defaults.update(kw)
         super().__init__(**defaults)
+
+
+class RefParams(C.Structure):
+    """ReferenceParameters of the reference direct volume renderer
+    (src/parameters.h:339-350) without single scattering; mirrors
+    mray_ref_params."""
+    _fields_ = [
+        ("step_size", C.c_float),
+        ("medium_emission", C.c_float),
+        ("medium_scattering", C.c_float),
+        ("early_ray_termination", C.c_float),
+        ("data_min", C.c_float),
+        ("data_max", C.c_float),
+        ("single_scattering", C.c_int32),
+    ]
+
+    def __init__(self, **kw):
+        defaults = dict(step_size=1.0, medium_emission=0.1, medium_scattering=0.1, early_ray_termination=0.95,
+                        data_min=0.0, data_max=1.0, single_scattering=0)
+        defaults.update(kw)
+        super().__init__(**defaults)

@@ -942,6 +942,98 @@ void orc_reconstruct_ray(const float org[3], const float dir[3], const float bmi
         *n_steps = ns;
 }
 
+/* ------------------------------------------------------------------------ */
+/* reference DVR (src/raymarch_reference.h:93-162)                           */
+/* ------------------------------------------------------------------------ */
+/* map_density (src/parameters.h:31-34) for ReferenceParameters */
+static inline float map_density_ref(float s, const mray_ref_params *p)
+{
+    return fclampr((s - p->data_min) / (p->data_max - p->data_min), 0.0f, 1.0f);
+}
+
+/* reference_ray: march while t < t_far sampling the volume itself, the TF
+ * lookup of (front, back) densities, the invalid-sample colour for samples
+ * >= 1e10, emission-absorption compositing with T = exp(-tau) (double exp,
+ * as the float argument promotes), early termination sets alpha = 1. */
+void orc_reference_ray(const orc_volume *v, const float org[3], const float dir[3], float znear, const float *tf,
+                       const mray_ref_params *p, float out[4], int32_t *n_steps)
+{
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, v->dmin, v->dmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    int ns = 0;
+    if (hit && (t_far > 0))
+    {
+        float L[3] = {0.0f, 0.0f, 0.0f};
+        float t = t_near, alpha = 0.0f, T = 1.0f, tau = 0.0f;
+        float pos[3];
+        sample_at(v, org, dir, t, pos);
+        float fs = map_density_ref(orc_volume_sample(v, pos), p);
+        while (t < t_far)
+        {
+            t += p->step_size;
+            ++ns;
+            sample_at(v, org, dir, t, pos);
+            float s = orc_volume_sample(v, pos);
+            float bs = map_density_ref(s, p);
+            float rgba[4];
+            orc_tf_sample(tf, fs, bs, rgba);
+            if (s >= 1e10f)
+            {
+                rgba[0] = 0.0f;
+                rgba[1] = 1.0f;
+                rgba[2] = 0.0f;
+                rgba[3] = 0.05f;
+            }
+            float sigma_t = rgba[3];
+            for (int c = 0; c < 3; ++c)
+            {
+                float Le = p->medium_emission * rgba[c];
+                L[c] += (Le + 0.0f * p->medium_scattering) * T;
+            }
+            tau += sigma_t;
+            T = (float)exp((double)(-tau));
+            alpha = 1.0f - T;
+            fs = bs;
+            if (alpha >= p->early_ray_termination)
+            {
+                alpha = 1.f;
+                break;
+            }
+        }
+        out[0] = L[0];
+        out[1] = L[1];
+        out[2] = L[2];
+        out[3] = alpha;
+    }
+    else
+        out[0] = out[1] = out[2] = out[3] = 0.f;
+    if (n_steps)
+        *n_steps = ns;
+}
+
+/* reference_host (src/moment_raymarcher.cpp:47-61): OpenMP over rows */
+void orc_reference_image(const orc_volume *v, const mray_camera *cam, int W, int H, const float *tf,
+                         const mray_ref_params *p, float *rgba, int32_t *n_steps, int row0, int row_step,
+                         int nthreads)
+{
+    if (row_step < 1)
+        row_step = 1;
+    int nrows = (H - row0 + row_step - 1) / row_step;
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int r = 0; r < nrows; ++r)
+    {
+        int y = row0 + r * row_step;
+        for (int x = 0; x < W; ++x)
+        {
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            size_t pix = (size_t)y * W + x;
+            orc_reference_ray(v, org, dir, cam->znear, tf, p, rgba + pix * 4, n_steps ? n_steps + pix : NULL);
+        }
+    }
+}
+
 void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
                            const float *data, const uint32_t *index, const uint32_t *bounds, int prediction_coded,
                            int coding_warp, const float *coding_params, const float *tf, const mray_rec_params *p,

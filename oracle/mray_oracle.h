@@ -86,6 +86,13 @@ void orc_tf_sample(const float *tf, float fs, float bs, float out[4]);
 void orc_reconstruct_ray(const float org[3], const float dir[3], const float bmin[3], const float bmax[3], float znear,
                          int m, const float *pmoments, float blo, float bhi, const float *tf,
                          const mray_rec_params *p, float out[4], int32_t *n_steps);
+/* ---- reference direct volume rendering (src/raymarch_reference.h:93-162,
+ * src/moment_raymarcher.cpp:47-61), without single scattering ---- */
+void orc_reference_ray(const orc_volume *v, const float org[3], const float dir[3], float znear, const float *tf,
+                       const mray_ref_params *p, float out[4], int32_t *n_steps);
+void orc_reference_image(const orc_volume *v, const mray_camera *cam, int W, int H, const float *tf,
+                         const mray_ref_params *p, float *rgba, int32_t *n_steps, int row0, int row_step,
+                         int nthreads);
 void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
                            const float *data, const uint32_t *index, const uint32_t *bounds, int prediction_coded,
                            int coding_warp, const float *coding_params, const float *tf, const mray_rec_params *p,

@@ -10,6 +10,7 @@
 #include "raymarch_generate.h"
 #include "raymarch_single_scattering_cache.h"
 #include "raymarch_reconstruct.h"
+#include "raymarch_reference.h"
 #include "moment_compaction.h"
 #include "moment_prediction_coding.h"
 #include "moment_preparation.h"
@@ -196,6 +197,48 @@ void ref_reconstruct(const mray_camera *cam, const float *dmin, const float *dma
     {
         ColorImageHost img(W, H);
         reconstruct(c, mi, tf, img, rp);
+        std::memcpy(rgba8, img.rgba.data(), (size_t)W * H * 4);
+    }
+}
+
+// Reference direct volume rendering: per-pixel float RGBA of reference_host's
+// loop (src/moment_raymarcher.cpp:47-61, reference_ray) before the RGBA8
+// quantisation, plus the RGBA8 image of the real ::generate_reference
+// (src/moment_raymarcher.cpp:74-81). Single scattering off.
+void ref_reference_render(const void *vol, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                          const mray_camera *cam, int W, int H, const uint8_t *tf_rgba8, float tf_step,
+                          const mray_ref_params *p, float *rgba, uint8_t *rgba8)
+{
+    auto grid = make_grid(vol, dtype, nx, ny, nz, dmin, dmax);
+    auto tf = make_tf(tf_rgba8, tf_step);
+    ReferenceParameters rp;
+    rp.step_size = p->step_size;
+    rp.medium_emission = p->medium_emission;
+    rp.medium_scattering = p->medium_scattering;
+    rp.early_ray_termination = p->early_ray_termination;
+    rp.data_min = p->data_min;
+    rp.data_max = p->data_max;
+    rp.ss.enable = false;
+    auto c = make_cam(cam);
+    if (rgba)
+    {
+#pragma omp parallel for
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+            {
+                Vec2f ndc(x / static_cast<float>(W) * 2.f - 1.f, y / static_cast<float>(H) * 2.f - 1.f);
+                Vec4f col = reference_ray(grid, cut::get_eye_ray(c, ndc), c.znear, tf, SingleScatteringBruteForce{}, rp);
+                float *o = rgba + ((size_t)y * W + x) * 4;
+                o[0] = col.x;
+                o[1] = col.y;
+                o[2] = col.z;
+                o[3] = col.w;
+            }
+    }
+    if (rgba8)
+    {
+        ColorImageHost img(W, H);
+        generate_reference(c, tf, grid, img, rp);
         std::memcpy(rgba8, img.rgba.data(), (size_t)W * H * 4);
     }
 }

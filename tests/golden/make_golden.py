@@ -24,7 +24,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 from mray_amd import synth  # noqa: E402
-from mray_amd.types import GenParams, RecParams  # noqa: E402
+from mray_amd.types import RefParams, GenParams, RecParams  # noqa: E402
 from oracle_ffi import Reference, ptr  # noqa: E402
 
 OUT = pathlib.Path(__file__).resolve().parent
@@ -221,6 +221,38 @@ def image_vectors(R: Reference) -> dict:
     return out
 
 
+def reference_dvr_vectors(R: Reference) -> dict:
+    """The reference's direct volume renderer (reference_ray, generate_reference):
+    float RGBA per pixel and the RGBA8 image, on small f32 and u8 volumes."""
+    out = {}
+    tf8 = synth.tf_ramps(0)
+    out["tf_rgba8"] = tf8
+    u8 = np.clip(synth.mixing_layer(20) * 255.0, 0, 255).astype(np.uint8)
+    cases = {
+        # name: (volume, dmin, dmax, camera, W, H, reference parameters, tf step)
+        "dvr_ml_ortho": (synth.marschner_lobb(24), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 20, 16,
+                         dict(step_size=2.0 / 48, medium_emission=1.0, early_ray_termination=0.99), 4 * 2.0 / 48),
+        "dvr_u8_persp": (u8, (0, 0, 0), (1, 1, 1),
+                         synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=16 / 14), 16, 14,
+                         dict(step_size=1.0 / 40, medium_emission=0.5, early_ray_termination=0.9, data_min=0.1,
+                              data_max=0.9), 20.0 / 40),
+        # opaque enough for early ray termination (alpha set to 1)
+        "dvr_ml_opaque": (synth.marschner_lobb(24), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 12, 10,
+                          dict(step_size=2.0 / 48, medium_emission=1.0, early_ray_termination=0.9), 60 * 2.0 / 48),
+        "dvr_ml_miss": (synth.marschner_lobb(24), (-1, -1, -1), (1, 1, 1),
+                        synth.perspective_camera(position=(1.5, 0.2, -2.5), fov_half_deg=25.0, aspect=18 / 12),
+                        18, 12, dict(step_size=2.0 / 40, medium_emission=1.0, early_ray_termination=0.95),
+                        4 * 2.0 / 40),
+    }
+    for name, (vol, dmin, dmax, cam, W, H, pk, tf_step) in cases.items():
+        p = RefParams(**pk)
+        rgba, rgba8 = R.reference(vol, dmin, dmax, cam, W, H, tf8, tf_step, p, want_rgba8=True)
+        out[f"{name}_volume"] = vol
+        out[f"{name}_rgba"] = rgba
+        out[f"{name}_rgba8"] = rgba8
+    return out
+
+
 def sph_vectors(R: Reference) -> dict:
     x, y, z, v = synth.particle_mixture(3000, seed=5, n_gauss=6)
     v = (0.5 + 0.5 * synth.hash_uniform(7, 1, np.arange(3000))).astype(np.float32)
@@ -244,11 +276,12 @@ def sph_vectors(R: Reference) -> dict:
 
 
 def main():
+    # python make_golden.py [name ...]: only the named fixtures (default all)
     R = Reference()
-    np.savez_compressed(OUT / "golden_mese.npz", **mese_vectors(R))
-    np.savez_compressed(OUT / "golden_coding.npz", **coding_vectors(R))
-    np.savez_compressed(OUT / "golden_images.npz", **image_vectors(R))
-    np.savez_compressed(OUT / "golden_sph.npz", **sph_vectors(R))
+    makers = {"mese": mese_vectors, "coding": coding_vectors, "images": image_vectors, "sph": sph_vectors,
+              "reference": reference_dvr_vectors}
+    for name in (sys.argv[1:] or list(makers)):
+        np.savez_compressed(OUT / f"golden_{name}.npz", **makers[name](R))
     for f in sorted(OUT.glob("golden_*.npz")):
         print(f.name, f.stat().st_size)
 

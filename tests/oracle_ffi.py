@@ -8,7 +8,7 @@ import pathlib
 
 import numpy as np
 
-from mray_amd.types import Camera, GenParams, RecParams
+from mray_amd.types import Camera, GenParams, RecParams, RefParams
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
@@ -117,6 +117,16 @@ class Oracle:
                                        ptr(bounds, _u32), prediction_coded, coding_warp, ptr(coding_params, _f),
                                        ptr(tf, _f), C.byref(rp), ptr(rgba, _f), ptr(ns, _i32), row0, row_step,
                                        nthreads())
+        return rgba, ns
+
+    def reference(self, vol, dmin, dmax, cam, W, H, tf, p: RefParams, row0=0, row_step=1):
+        """Reference direct volume rendering (orc_reference_image): float RGBA and steps per ray."""
+        v = self.volume(vol, dmin, dmax)
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float32)
+        ns = np.zeros(W * H, np.int32)
+        self.lib.orc_reference_image(C.byref(v), C.byref(c), W, H, ptr(tf, _f), C.byref(p), ptr(rgba, _f),
+                                     ptr(ns, _i32), row0, row_step, nthreads())
         return rgba, ns
 
     def reconstruct_f64(self, cam, dmin, dmax, W, H, m, moments64, tf, rp: RecParams, bounds=None):
@@ -248,6 +258,19 @@ class Reference:
                                   bmin, bmax, C.byref(c), W, H, m, C.byref(gp), ptr(data, _f), ptr(bounds, _u32),
                                   ptr(index, _u32), ptr(cp, _u16))
         return data[:n], bounds, index, cp
+
+    def reference(self, vol, dmin, dmax, cam, W, H, tf_rgba8, tf_step, p: RefParams, want_rgba8=False):
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float32)
+        rgba8 = np.zeros((H, W, 4), np.uint8) if want_rgba8 else None
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        nz, ny, nx = vol.shape
+        self.lib.ref_reference_render(vol.ctypes.data_as(C.c_void_p), 1 if vol.dtype == np.uint8 else 0, nx, ny, nz,
+                                      bmin, bmax, C.byref(c), W, H,
+                                      ptr(np.ascontiguousarray(tf_rgba8, np.uint8), _u8), C.c_float(tf_step),
+                                      C.byref(p), ptr(rgba, _f), ptr(rgba8, _u8))
+        return rgba, rgba8
 
     def reconstruct(self, cam, dmin, dmax, W, H, m, data, tf_rgba8, tf_step, rp: RecParams, index=None,
                     bounds=None, prediction_coded=0, coding_warp=0, coding_params=None, want_rgba8=False):

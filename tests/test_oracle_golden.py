@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from mray_amd import synth, tf as tfmod
-from mray_amd.types import GenParams, RecParams
+from mray_amd.types import GenParams, RecParams, RefParams
 from oracle_ffi import Oracle, Reference, ptr, REF_SO
 
 GOLD = pathlib.Path(__file__).resolve().parent / "golden"
@@ -325,6 +325,37 @@ def test_coding_params_like_reference_test(O):
     params = np.zeros(2 * (m - 1), np.float32)
     O.lib.orc_find_coding_params(W, H, m, ptr(comp, _f), ptr(index, _u32), ptr(params, _f))
     assert np.allclose(params[0::2], 0.6, atol=1e-6) and np.allclose(params[1::2], 0.5, atol=1e-6)
+
+
+# the reference direct volume renderer's cases in golden_reference.npz
+# (tests/golden/make_golden.py, reference_dvr_vectors)
+DVR_CASES = {
+    "dvr_ml_ortho": ((-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 20, 16,
+                     dict(step_size=2.0 / 48, medium_emission=1.0, early_ray_termination=0.99), 4 * 2.0 / 48),
+    "dvr_u8_persp": ((0, 0, 0), (1, 1, 1), synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=16 / 14), 16,
+                     14, dict(step_size=1.0 / 40, medium_emission=0.5, early_ray_termination=0.9, data_min=0.1,
+                              data_max=0.9), 20.0 / 40),
+    "dvr_ml_opaque": ((-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 12, 10,
+                      dict(step_size=2.0 / 48, medium_emission=1.0, early_ray_termination=0.9), 60 * 2.0 / 48),
+    "dvr_ml_miss": ((-1, -1, -1), (1, 1, 1),
+                    synth.perspective_camera(position=(1.5, 0.2, -2.5), fov_half_deg=25.0, aspect=18 / 12), 18, 12,
+                    dict(step_size=2.0 / 40, medium_emission=1.0, early_ray_termination=0.95), 4 * 2.0 / 40),
+}
+
+
+@pytest.mark.parametrize("case", list(DVR_CASES))
+def test_reference_dvr_matches_reference(O, case):
+    """The oracle's reference direct volume renderer reproduces the compiled
+    reference's reference_ray bit for bit (float RGBA) and its
+    generate_reference RGBA8 image (f32 and u8 volumes, rays that miss, early
+    termination)."""
+    g = dict(np.load(GOLD / "golden_reference.npz"))
+    dmin, dmax, cam, W, H, pk, tf_step = DVR_CASES[case]
+    table = O.tf(g["tf_rgba8"], tf_step)
+    rgba, ns = O.reference(g[f"{case}_volume"], dmin, dmax, cam, W, H, table, RefParams(**pk))
+    assert bits_equal(rgba, g[f"{case}_rgba"])
+    q = (np.clip(rgba, 0, 1) * np.float32(255)).astype(np.uint8)
+    assert np.array_equal(q, g[f"{case}_rgba8"])
 
 
 def test_sph_gather_matches_reference():
