@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--cpu-row-step", type=int, default=64, help="CPU baseline: every k-th row")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--no-dvr", action="store_true", help="skip the reference direct volume renderer")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the adjoint workload and print its figures (for kernel profiles)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -192,6 +193,27 @@ def main():
 
     adjoint = None if args.no_adjoint else adjoint_ratio(args.steps)
 
+    # the reference direct volume renderer (the ground truth of the moment
+    # images) on the same volume, camera, table and step: one frame per step
+    dvr = None
+    if not args.no_dvr:
+        from mray_amd.types import RefParams
+
+        refp = RefParams(step_size=step, medium_emission=rp.medium_emission,
+                         early_ray_termination=rp.early_ray_termination)
+        dvr_rgba = torch.empty_like(rgba)
+        hip.reference(vol, cam, table, refp, W, H, rgba=dvr_rgba)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            hip.reference(vol, cam, table, refp, W, H, rgba=dvr_rgba)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_dvr = e0.elapsed_time(e1) / args.steps
+        dvr = {"workload": "reference DVR, same volume, camera, table and step", "ms_per_frame": round(ms_dvr, 3),
+               "Mrays_per_s": round(W * H / (ms_dvr * 1e-3) / 1e6, 3)}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step, tf_factor)
@@ -224,6 +246,7 @@ def main():
         "roofline_frame": {"achieved_TFLOPs": round(total_flop / (ms_step * 1e-3) / 1e12, 3),
                            "frac": round(total_flop / (ms_step * 1e-3) / PEAK_FP32_VALU, 4)},
         "adjoint": adjoint,
+        "reference_dvr": dvr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

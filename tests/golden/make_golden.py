@@ -3,7 +3,8 @@
 Run in the build container (needs oracle/_ref/libmrayref.so, built by
 `make -C oracle ref` from the reference sources):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py reference  # golden_reference.npz only
 
 Inputs are (a) the vectors of the reference's own unit tests
 (tests/test_mese.cpp, tests/test_prediction_coding.cpp,
