@@ -1,7 +1,9 @@
 """A volume larger than 4 GiB: the mixed generate kernel then addresses
 voxels with 64-bit offsets (up to 4 GiB it uses 32-bit byte offsets from the
 base). 1025 x 1024 x 1024 f32 voxels (4.03 GiB), the last z slab lying past
-the 4 GiB mark; 16 x 16 perspective rays cross the whole volume.
+the 4 GiB mark; 16 x 16 perspective rays cross the whole volume. The
+reference renderer runs through the same voxels (bit-identical RGBA and
+step counts).
 
 Bars: sample counts and 16-bit bounds bit-exact against the FP32 oracle on
 the same voxels (they exercise every address); moments against the FP64
@@ -47,7 +49,16 @@ def test_generate_mixed_beyond_4gib():
                    prediction_coding=0)
     v = hip.Volume(vox, dmin, dmax)
     mo, b, ns = hip.generate(v, cam, gp, W, H, m, form="mixed", want_samples=True)
+    # the reference renderer through the same voxels (its 64-bit path too)
+    from mray_amd import tf as tfmod
+    from mray_amd.types import RefParams
+
+    refp = RefParams(step_size=gp.step_size, medium_emission=1.0, early_ray_termination=0.99)
+    table = tfmod.table_from_rgba8(synth.tf_ramps(0), 64 * gp.step_size)
+    dvr, dvr_steps = hip.reference(v, cam, torch.from_numpy(table).cuda(), refp, W, H, want_steps=True)
     torch.cuda.synchronize()
+    dvr = dvr.cpu().numpy()
+    dvr_steps = dvr_steps.cpu().numpy()
     mo = mo.cpu().numpy()
     b = b.cpu().numpy().view(np.uint32)
     ns = ns.cpu().numpy()
@@ -58,6 +69,9 @@ def test_generate_mixed_beyond_4gib():
     assert np.array_equal(ns, ns_ref), "sample counts differ"
     assert np.array_equal(b, b_ref), "16-bit bounds differ"
     assert np.any((b_ref >> 16) >= int(0.97 * 65535)), "no ray reached the slab past 4 GiB"
+    dvr_ref, dvr_ns = O.reference(vol, dmin, dmax, cam, W, H, table, refp)
+    assert np.array_equal(dvr_steps, dvr_ns), "reference renderer step counts differ"
+    assert np.array_equal(dvr.view(np.uint32), dvr_ref.view(np.uint32)), "reference renderer RGBA differs"
     mo64, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp, f64=True)
     hit = ns_ref > 0
     err = normwise_rel(mo, mo64)[hit]
