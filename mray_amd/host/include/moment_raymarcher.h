@@ -28,6 +28,13 @@ void generate(cut::Camera cam, const Particles &volume, MomentImageHost &img, co
 void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, ColorImageHost &cimg,
                  const ReconstructionParameters &params);
 
+// generate_reference (src/moment_raymarcher.h:40-41): the reference direct
+// volume renderer into an RGBA8 image, and the float RGBA of a W x H image
+void generate_reference(cut::Camera cam, const TransferFunction &tf, const RegularGrid &volume, ColorImageHost &img,
+                        const ReferenceParameters &params);
+void generate_reference(cut::Camera cam, const TransferFunction &tf, const RegularGrid &volume, int W, int H,
+                        vector<Vec4f> &rgba, const ReferenceParameters &params);
+
 // Float-RGBA overload (un-clamped), which the reference lacks.
 void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, vector<Vec4f> &rgba,
                  const ReconstructionParameters &params);

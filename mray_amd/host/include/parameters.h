@@ -38,5 +38,18 @@ struct ReconstructionParameters
     bool use_truncated_fourier = false;
 };
 
+// ReferenceParameters (src/parameters.h:339-350) without the single
+// scattering block (not part of this library)
+struct ReferenceParameters
+{
+    float step_size = 1.f;
+    float medium_emission = 0.1f;
+    float medium_scattering = 0.1f;
+    float early_ray_termination = 0.95f;
+    float data_min = 0.0f;
+    float data_max = 1.0f;
+};
+
 GenerationParameters parse_gen_params(std::unordered_map<string, string> &params);
+ReferenceParameters parse_ref_params(std::unordered_map<string, string> &params);
 ReconstructionParameters parse_rec_params(std::unordered_map<string, string> &params);

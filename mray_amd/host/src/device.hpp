@@ -134,4 +134,17 @@ inline mray_rec_params to_c(const ReconstructionParameters &p)
     return r;
 }
 
+inline mray_ref_params to_c(const ReferenceParameters &p)
+{
+    mray_ref_params r{};
+    r.step_size = p.step_size;
+    r.medium_emission = p.medium_emission;
+    r.medium_scattering = p.medium_scattering;
+    r.early_ray_termination = p.early_ray_termination;
+    r.data_min = p.data_min;
+    r.data_max = p.data_max;
+    r.single_scattering = 0;
+    return r;
+}
+
 } // namespace mray_host

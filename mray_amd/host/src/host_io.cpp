@@ -122,6 +122,19 @@ GenerationParameters parse_gen_params(std::unordered_map<string, string> &params
     return p;
 }
 
+// parse_ref_params (src/parameters.h:352-377)
+ReferenceParameters parse_ref_params(std::unordered_map<string, string> &params)
+{
+    ReferenceParameters p;
+    read_f(params, "step_size", p.step_size);
+    read_f(params, "medium_emission", p.medium_emission);
+    read_f(params, "medium_scattering", p.medium_scattering);
+    read_f(params, "early_ray_termination", p.early_ray_termination);
+    read_f(params, "data_min", p.data_min);
+    read_f(params, "data_max", p.data_max);
+    return p;
+}
+
 ReconstructionParameters parse_rec_params(std::unordered_map<string, string> &params)
 {
     ReconstructionParameters p;

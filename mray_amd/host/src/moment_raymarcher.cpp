@@ -218,3 +218,39 @@ void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunct
         for (int x = 0; x < mi.width; ++x)
             cimg.set_color(x, y, rgba[(size_t)y * mi.width + x]);
 }
+
+void generate_reference(cut::Camera cam, const TransferFunction &tf, const RegularGrid &volume, int W, int H,
+                        vector<Vec4f> &rgba, const ReferenceParameters &params)
+{
+    mray_host::ensure_init();
+    if (volume.data.size() * sizeof(float) < volume.num_voxels() * volume.get_byte_size())
+        throw invalid_data("volume data smaller than its size");
+    if (W <= 0 || H <= 0)
+        throw invalid_args("image size must be positive");
+    const float dmin[3] = {volume.domain.min.x, volume.domain.min.y, volume.domain.min.z};
+    const float dmax[3] = {volume.domain.max.x, volume.domain.max.y, volume.domain.max.z};
+    VolumeHandle vh;
+    check(mray_hip_volume_create(volume.voxels(),
+                                 volume.type == RegularGrid::DataType::U8 ? MRAY_DTYPE_U8 : MRAY_DTYPE_F32,
+                                 volume.size.x, volume.size.y, volume.size.z, dmin, dmax, &vh.v),
+          "volume_create");
+    const size_t npix = (size_t)W * H;
+    DeviceBuffer d_tf(tf.data.data(), sizeof(tf.data));
+    DeviceBuffer d_rgba(npix * sizeof(Vec4f));
+    mray_camera c = mray_host::to_c(cam);
+    mray_ref_params r = mray_host::to_c(params);
+    check(mray_hip_reference(vh.v, &c, d_tf.as<float>(), &r, W, H, d_rgba.as<float>(), nullptr, nullptr),
+          "reference");
+    rgba.resize(npix);
+    d_rgba.download(rgba.data(), npix * sizeof(Vec4f));
+}
+
+void generate_reference(cut::Camera cam, const TransferFunction &tf, const RegularGrid &volume, ColorImageHost &img,
+                        const ReferenceParameters &params)
+{
+    vector<Vec4f> rgba;
+    generate_reference(cam, tf, volume, img.width, img.height, rgba, params);
+    for (int y = 0; y < img.height; ++y)
+        for (int x = 0; x < img.width; ++x)
+            img.set_color(x, y, rgba[(size_t)y * img.width + x]);
+}

@@ -10,7 +10,12 @@
 //       tf, camera, preintegration_step_size_factor, background_r/g/b,
 //       ReconstructionParameters keys; float_rgba=<file> also writes the
 //       un-clamped float RGBA (4 floats per pixel, rows top-down)
-//   both: form=mixed (default) | plain
+//   mray --reference <output.ppm> <volume.raw> "k=v,..."
+//       width, height, tf, camera, preintegration_step_size_factor,
+//       background_r/g/b, ReferenceParameters keys, dims / dtype / domain as
+//       for --generate; float_rgba=<file> as for --reconstruct
+//       (src/main.cpp:134-168, 740-752)
+//   generate and reconstruct: form=mixed (default) | plain
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -149,6 +154,38 @@ void run_reconstruct(const string &mi_file, const string &output, unordered_map<
     }
 }
 
+// the reference renderer (src/main.cpp:134-168): raw volumes only
+void run_reference(const string &output, const string &dataset, unordered_map<string, string> p)
+{
+    auto ref_params = parse_ref_params(p);
+    float factor = 1.f;
+    if (p.count("preintegration_step_size_factor"))
+        factor = std::stof(p["preintegration_step_size_factor"]);
+    auto tf = read_from_disk(need(p, "tf").c_str(), ref_params.step_size * factor);
+    const int width = std::stoi(need(p, "width")), height = std::stoi(need(p, "height"));
+    auto cam = cut::load_from_file(need(p, "camera"), width, height);
+    auto volume = load_raw(dataset, p);
+    vector<Vec4f> rgba;
+    generate_reference(cam, tf, volume, width, height, rgba, ref_params);
+    ColorImageHost img(width, height);
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x)
+            img.set_color(x, y, rgba[(size_t)y * width + x]);
+    Vec3f bg(1.f, 1.f, 1.f);
+    if (p.count("background_r"))
+        bg.x = std::stof(p["background_r"]);
+    if (p.count("background_g"))
+        bg.y = std::stof(p["background_g"]);
+    if (p.count("background_b"))
+        bg.z = std::stof(p["background_b"]);
+    write_PPM(output, img, bg);
+    if (p.count("float_rgba"))
+    {
+        std::ofstream f(p["float_rgba"], std::ios::binary);
+        f.write(reinterpret_cast<const char *>(rgba.data()), rgba.size() * sizeof(Vec4f));
+    }
+}
+
 } // namespace
 
 int main(int argc, char *argv[])
@@ -156,7 +193,8 @@ int main(int argc, char *argv[])
     if (argc < 5)
     {
         std::cerr << "usage: mray --generate <moment_image> <volume.raw> \"k=v,...\"\n"
-                     "       mray --reconstruct <moment_image> <output.ppm> \"k=v,...\"\n";
+                     "       mray --reconstruct <moment_image> <output.ppm> \"k=v,...\"\n"
+                     "       mray --reference <output.ppm> <volume.raw> \"k=v,...\"\n";
         return 2;
     }
     const string mode(argv[1]);
@@ -166,6 +204,8 @@ int main(int argc, char *argv[])
             run_generate(argv[2], argv[3], parse_parameters(argv[4]));
         else if (mode == "--reconstruct")
             run_reconstruct(argv[2], argv[3], parse_parameters(argv[4]));
+        else if (mode == "--reference")
+            run_reference(argv[2], argv[3], parse_parameters(argv[4]));
         else
         {
             std::cerr << "unknown mode " << mode << "\n";

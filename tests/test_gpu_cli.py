@@ -6,7 +6,8 @@ Bars: bounds bit-exact and moments within 1e-5 normwise (generate); the
 compaction and prediction coding of the CLI's own dense moments bit-identical
 to the oracle's; in the PLAIN form the float RGBA reconstructed from the coded
 file bit-identical to the oracle's, and the PPM bytes equal to write_PPM of
-the oracle's image (src/color_image.cpp:29-57)."""
+the oracle's image (src/color_image.cpp:29-57). The --reference mode renders
+the volume with the reference renderer, bit-identical to the oracle."""
 import ctypes as C
 import pathlib
 import subprocess
@@ -129,6 +130,44 @@ def test_cli_round_trip(tmp_path, case):
     print(f"[cli] {case}: bit-identical rays {same.sum()}/{W * H}, alpha mean {ref[..., 3].mean():.3f}")
     assert same.all()
     assert (tmp_path / "out.ppm").read_bytes() == ppm_bytes(ref)
+
+
+@pytest.mark.parametrize("case", ["ortho_f32", "persp_u8"])
+def test_cli_reference(tmp_path, case):
+    """mray --reference (src/main.cpp:134-168, 740-752): the reference
+    renderer from a raw volume with the reference's parameter keys; float RGBA
+    bit-identical to the oracle and the PPM equal to write_PPM of it."""
+    from mray_amd.types import RefParams
+
+    O = Oracle()
+    if case == "ortho_f32":
+        vol, dmin, dmax = synth.marschner_lobb(40), (-1, -1, -1), (1, 1, 1)
+        cam = synth.ortho_camera()
+        W, H, step, factor, extra = 32, 28, 2.0 / 80, 8.0, ""
+        p = RefParams(step_size=step, medium_emission=1.0, early_ray_termination=0.99)
+    else:
+        vol = np.clip(synth.mixing_layer(32) * 255, 0, 255).astype(np.uint8)
+        dmin, dmax = (0, 0, 0), (1, 1, 1)
+        W, H, step, factor, extra = 30, 26, 0.5 / 32, 200.0, ",dtype=u8,data_min=0.1,data_max=0.9"
+        cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=W / H)
+        p = RefParams(step_size=step, medium_emission=0.8, early_ray_termination=0.95, data_min=0.1, data_max=0.9)
+    vol.tofile(tmp_path / "vol.raw")
+    camera_file(tmp_path / "cam.txt", cam)
+    rgba8 = synth.tf_ramps(0)
+    rgba8.tofile(tmp_path / "tf.bin")
+    nz, ny, nx = vol.shape
+    run("--reference", tmp_path / "ref.ppm", tmp_path / "vol.raw",
+        f"width={W},height={H},camera={tmp_path / 'cam.txt'},tf={tmp_path / 'tf.bin'},dims={nx}:{ny}:{nz},"
+        f"domain={dmin[0]}:{dmin[1]}:{dmin[2]}:{dmax[0]}:{dmax[1]}:{dmax[2]},step_size={step!r},"
+        f"medium_emission={p.medium_emission!r},early_ray_termination={p.early_ray_termination!r},"
+        f"preintegration_step_size_factor={factor}{extra},float_rgba={tmp_path / 'rgba.f32'}")
+    rgba = np.fromfile(tmp_path / "rgba.f32", np.float32).reshape(H, W, 4)
+    table = tfmod.table_from_rgba8(rgba8, step * factor)
+    ref, _ = O.reference(vol, dmin, dmax, cam, W, H, table, p)
+    same = (rgba.view(np.uint32) == ref.view(np.uint32)).all(axis=-1)
+    print(f"[cli] reference {case}: bit-identical rays {same.sum()}/{W * H}, alpha mean {ref[..., 3].mean():.3f}")
+    assert same.all()
+    assert (tmp_path / "ref.ppm").read_bytes() == ppm_bytes(ref)
 
 
 def test_cli_rejects_bad_input(tmp_path):
