@@ -209,6 +209,39 @@ int mray_hip_prediction_encode(float *d_data, const uint32_t *d_index, int W, in
 int mray_hip_prepare_coded(const float *d_moments, const uint32_t *d_index, int W, int H, int m, int coding_warp,
                            const uint16_t *d_coding_params, float *d_lagrange, mray_stream stream);
 
+/* ---- quantization (src/moment_quantization.cpp) ----------------------- */
+
+/* Packs a compact image into the quantized bit stream of the moment-image
+ * files (moment_quantization::quantize_prediction_coding(mi, table) when
+ * prediction_coded != 0, src/moment_quantization.cpp:244-283; else
+ * moment_quantization::quantize(mi, bits), :300-329, as write_moment_image
+ * calls them, src/moment_image_io.cpp:397-408). bits (host, n_bits entries,
+ * each 1..32) is the word width of moment l: the reference's quantization
+ * table for prediction-coded images (its default is 16 then
+ * quantization_bits), quantization_bits for every moment otherwise. Ranges:
+ * [0, 1] for prediction-coded moments and moment 0, [-1/pi, 1/pi] for the
+ * others. Words follow each other least significant bit first, pixels in row
+ * order, and a last partial byte is dropped as the reference's pack_bits
+ * drops it; *stream_bytes receives the stream length. d_stream NULL only
+ * measures. Fails with MRAY_ERR_ARG when the stream exceeds capacity or a
+ * pixel has more than n_bits moments. Waits for the stream (the length is
+ * read back); uses stream-ordered scratch. Bit-identical to the reference. */
+int mray_hip_quantize(const float *d_data, const uint32_t *d_index, int W, int H, int prediction_coded,
+                      const uint8_t *bits, int n_bits, uint8_t *d_stream, size_t capacity, size_t *stream_bytes,
+                      mray_stream stream);
+
+/* Unpacks a quantized stream into a compact image
+ * (moment_quantization::dequantize_prediction_coding / dequantize,
+ * src/moment_quantization.cpp:285-298, 331-360, as read_moment_image calls
+ * them, src/moment_image_io.cpp:169-175); same bits and ranges as
+ * mray_hip_quantize. Bits past the end of the stream (the dropped partial
+ * byte) read as zero, where the reference reads past its word list. Fails
+ * with MRAY_ERR_ARG when the stream is shorter than the image's words.
+ * Waits for the stream; bit-identical to the reference where the stream holds
+ * every word whole. */
+int mray_hip_dequantize(const uint8_t *d_stream, size_t stream_bytes, const uint32_t *d_index, int W, int H,
+                        int prediction_coded, const uint8_t *bits, int n_bits, float *d_data, mray_stream stream);
+
 /* ---- SPH particles (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh}) --- */
 
 /* Samples particles (device arrays of n positions and values) at every voxel

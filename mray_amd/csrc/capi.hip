@@ -370,6 +370,48 @@ int mray_hip_prepare_coded(const float *d_moments, const uint32_t *d_index, int 
     return check_status(rc, "mray_hip_prepare_coded");
 }
 
+int mray_hip_quantize(const float *d_data, const uint32_t *d_index, int W, int H, int prediction_coded,
+                      const uint8_t *bits, int n_bits, uint8_t *d_stream, size_t capacity, size_t *stream_bytes,
+                      mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_quantize: call mray_hip_init first");
+    if (!d_index || !bits || (d_stream && !d_data))
+        return fail(MRAY_ERR_ARG, "mray_hip_quantize: null argument");
+    if (bad_image(W, H, 1) || n_bits < 1 || n_bits > 256)
+        return fail(MRAY_ERR_ARG, "mray_hip_quantize: bad image size %dx%d or table length %d", W, H, n_bits);
+    size_t nbytes = 0;
+    int rc = mray::launch_quantize(d_data, d_index, W, H, prediction_coded, bits, n_bits, d_stream, capacity, &nbytes,
+                                   (hipStream_t)stream);
+    if (stream_bytes)
+        *stream_bytes = nbytes;
+    if (rc == MRAY_ERR_ARG)
+        return fail(rc, "mray_hip_quantize: widths must be 1..32, every pixel at most %d moments, and the stream "
+                        "(%zu bytes) within the capacity (%zu)", n_bits, nbytes, capacity);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_quantize: scratch allocation failed");
+    return check_status(rc, "mray_hip_quantize");
+}
+
+int mray_hip_dequantize(const uint8_t *d_stream, size_t stream_bytes, const uint32_t *d_index, int W, int H,
+                        int prediction_coded, const uint8_t *bits, int n_bits, float *d_data, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_dequantize: call mray_hip_init first");
+    if (!d_index || !bits || !d_data || (!d_stream && stream_bytes))
+        return fail(MRAY_ERR_ARG, "mray_hip_dequantize: null argument");
+    if (bad_image(W, H, 1) || n_bits < 1 || n_bits > 256)
+        return fail(MRAY_ERR_ARG, "mray_hip_dequantize: bad image size %dx%d or table length %d", W, H, n_bits);
+    int rc = mray::launch_dequantize(d_stream, stream_bytes, d_index, W, H, prediction_coded, bits, n_bits, d_data,
+                                     (hipStream_t)stream);
+    if (rc == MRAY_ERR_ARG)
+        return fail(rc, "mray_hip_dequantize: widths must be 1..32, every pixel at most %d moments, and the "
+                        "stream (%zu bytes) must hold every whole byte of the words", n_bits, stream_bytes);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_dequantize: scratch allocation failed");
+    return check_status(rc, "mray_hip_dequantize");
+}
+
 int mray_hip_particles_to_grid(const float *d_x, const float *d_y, const float *d_z, const float *d_v, uint32_t n,
                                float h, const float dom_min[3], const float dom_max[3], int nx, int ny, int nz,
                                float *d_grid, mray_stream stream)

@@ -46,6 +46,11 @@ int launch_prediction_encode(float *data, const uint32_t *index, int W, int H, i
                              uint16_t *qparams, hipStream_t stream);
 int launch_prepare_coded(const float *moments, const uint32_t *index, int W, int H, int m, int warp,
                          const uint16_t *qparams, float *lagrange, hipStream_t stream);
+int launch_quantize(const float *data, const uint32_t *index, int W, int H, int prediction_coded,
+                    const uint8_t *bits, int n_bits, uint8_t *stream_out, size_t capacity, size_t *stream_bytes,
+                    hipStream_t stream);
+int launch_dequantize(const uint8_t *stream_in, size_t stream_bytes, const uint32_t *index, int W, int H,
+                      int prediction_coded, const uint8_t *bits, int n_bits, float *data, hipStream_t stream);
 
 int launch_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
                              const float *dmin, const float *dmax, int nx, int ny, int nz, float *grid,

@@ -1,0 +1,471 @@
+// Quantization of compact moment images into the packed bit stream of the
+// moment-image files, and back (moment_quantization::quantize_prediction_coding,
+// quantize, dequantize_prediction_coding, dequantize:
+// src/moment_quantization.cpp:33-163, 244-360).
+//
+// Moment l of a pixel is a word of bits[l] bits (1..32); the words of all
+// pixels follow each other in row order, least significant bit first, and a
+// last partial byte is not part of the stream. A pixel's words are
+// contiguous, so the stream offset of pixel p is the exclusive prefix sum of
+// cum[count(p)] with cum the running sum of the widths:
+//   1. scan: per-tile sums of the pixel bit counts, a 64-bit scan of the tile
+//      sums, then per tile the offsets (64-bit: a 2048^2 image of 100 moments
+//      at 16 bits is 6.7 Gbit); offsets[W*H] is the total
+//   2. encode: one thread per pixel quantizes its words into a 64-bit shift
+//      register and stores each full 32-bit stream word. Words that lie
+//      inside the pixel's bit range are written with plain stores, the (at
+//      most two) words shared with the neighbouring pixels with atomicOr
+//      into a cleared word buffer; the stream is the buffer's first bytes
+//   3. decode: the stream is copied into a cleared word buffer (bits past its
+//      end read as zero), one thread per pixel reads its words from 64-bit
+//      windows
+// Values round like the reference's float code: clamp by comparisons,
+// (v - lo) / (hi - lo) and q / (2^bits - 1) correctly rounded, the product
+// and sum unfused (-ffp-contract=off), float to integer by truncation.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace mray
+{
+
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;
+
+struct QuantTable
+{
+    uint32_t cum[257]; // cum[n] = bits[0] + ... + bits[n-1]
+    uint8_t bits[256];
+    int32_t n_bits;    // entries of bits given; a pixel with more moments is an error
+    int32_t prediction_coded;
+};
+
+struct QuantTableLds
+{
+    uint32_t cum[257];
+    uint8_t bits[256];
+};
+
+__device__ __forceinline__ void load_table(const QuantTable &t, QuantTableLds &s)
+{
+    for (int i = threadIdx.x; i < 257; i += blockDim.x)
+        s.cum[i] = t.cum[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x)
+        s.bits[i] = t.bits[i];
+    __syncthreads();
+}
+
+// the pixel's bit count; a count beyond the table raises the error flag
+__device__ __forceinline__ uint32_t pixel_bits(const uint32_t *__restrict__ index, size_t p, const QuantTable &t,
+                                               const QuantTableLds &s, int *err)
+{
+    const uint32_t n = index[p + 1] - index[p];
+    if (n > (uint32_t)t.n_bits)
+    {
+        atomicOr(err, 1);
+        return 0;
+    }
+    return s.cum[n];
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *red)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t s = 0;
+    for (int i = 0; i < kThreads / 64; ++i)
+        s += red[i];
+    return s;
+}
+
+// exclusive scan of one value per thread (Hillis-Steele in LDS); buf[kThreads-1]
+// holds the total afterwards
+template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T *buf)
+{
+    const int t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (int o = 1; o < kThreads; o <<= 1)
+    {
+        const T add = (t >= o) ? buf[t - o] : T(0);
+        __syncthreads();
+        buf[t] += add;
+        __syncthreads();
+    }
+    return buf[t] - v;
+}
+
+// pass 1: bits of each tile of kTile pixels (at most 4096 * 8160 < 2^32)
+__global__ __launch_bounds__(kThreads) void quant_tile_sums(const uint32_t *__restrict__ index, size_t npix,
+                                                            QuantTable t, uint32_t *__restrict__ sums, int *err)
+{
+    __shared__ QuantTableLds s;
+    __shared__ uint32_t red[kThreads / 64];
+    load_table(t, s);
+    const size_t base = (size_t)blockIdx.x * kTile;
+    uint32_t sum = 0;
+    for (int i = 0; i < kItems; ++i)
+    {
+        const size_t p = base + (size_t)i * kThreads + threadIdx.x;
+        if (p < npix)
+            sum += pixel_bits(index, p, t, s, err);
+    }
+    sum = block_sum(sum, red);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = sum;
+}
+
+// pass 2: 64-bit exclusive scan of the tile sums (one workgroup, chunks of
+// kThreads with a running carry); the total goes to offsets[npix]
+__global__ __launch_bounds__(kThreads) void quant_tile_offsets(const uint32_t *__restrict__ sums, int ntiles,
+                                                               uint64_t *__restrict__ tile_off,
+                                                               uint64_t *__restrict__ offsets, size_t npix)
+{
+    __shared__ uint64_t buf[kThreads];
+    uint64_t carry = 0;
+    for (int c = 0; c < ntiles; c += kThreads)
+    {
+        const int i = c + threadIdx.x;
+        const uint64_t v = (i < ntiles) ? sums[i] : 0u;
+        const uint64_t ex = block_exclusive_scan<uint64_t>(v, buf);
+        const uint64_t chunk_total = buf[kThreads - 1];
+        if (i < ntiles)
+            tile_off[i] = carry + ex;
+        carry += chunk_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        offsets[npix] = carry;
+}
+
+// pass 3: per-pixel offsets inside each tile plus the tile's offset
+__global__ __launch_bounds__(kThreads) void quant_tile_apply(const uint32_t *__restrict__ index, size_t npix,
+                                                             QuantTable t, const uint64_t *__restrict__ tile_off,
+                                                             uint64_t *__restrict__ offsets)
+{
+    __shared__ QuantTableLds s;
+    __shared__ uint32_t tile[kTile];
+    __shared__ uint32_t buf[kThreads];
+    __shared__ int dummy_err;
+    load_table(t, s);
+    const int th = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * kTile;
+    for (int i = 0; i < kItems; ++i)
+    {
+        const size_t p = base + (size_t)i * kThreads + th;
+        tile[i * kThreads + th] = (p < npix) ? pixel_bits(index, p, t, s, &dummy_err) : 0u;
+    }
+    __syncthreads();
+    uint32_t sum = 0;
+    for (int i = 0; i < kItems; ++i)
+        sum += tile[th * kItems + i];
+    uint32_t run = block_exclusive_scan<uint32_t>(sum, buf);
+    const uint64_t toff = tile_off[blockIdx.x];
+    __syncthreads();
+    for (int i = 0; i < kItems; ++i)
+    {
+        const uint32_t v = tile[th * kItems + i];
+        tile[th * kItems + i] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int i = 0; i < kItems; ++i)
+    {
+        const size_t p = base + (size_t)i * kThreads + th;
+        if (p < npix)
+            offsets[p] = toff + tile[i * kThreads + th];
+    }
+}
+
+__device__ __forceinline__ float clampr(float v, float lo, float hi)
+{
+    v = (v > lo) ? v : lo;
+    return (v < hi) ? v : hi;
+}
+
+// moment l's range: [0, 1] for moment 0 and for prediction-coded images,
+// else [-1/pi, 1/pi] as floats of the double quotients
+__device__ __forceinline__ void quant_range(uint32_t l, int prediction_coded, float &lo, float &hi)
+{
+    const bool unit = (l == 0) || prediction_coded;
+    lo = unit ? 0.0f : (float)(-1.0 / M_PI);
+    hi = unit ? 1.0f : (float)(1.0 / M_PI);
+}
+
+// quantize (src/moment_quantization.cpp:33-50)
+__device__ __forceinline__ uint32_t quantize_value(float v, int bits, float lo, float hi)
+{
+    v = clampr(v, lo, hi);
+    if (bits == 32)
+        return __float_as_uint(v);
+    const uint32_t full = (1u << bits) - 1u;
+    const float norm = (v - lo) / (hi - lo);
+    return (uint32_t)(norm * (float)(int32_t)full) & full;
+}
+
+// dequantize (src/moment_quantization.cpp:52-63)
+__device__ __forceinline__ float dequantize_value(uint32_t q, int bits, float lo, float hi)
+{
+    if (bits == 32)
+        return clampr(__uint_as_float(q), lo, hi);
+    const float norm = (float)q / (float)(int32_t)((1u << bits) - 1u);
+    return norm * (hi - lo) + lo;
+}
+
+__global__ __launch_bounds__(kThreads) void fill_words(uint32_t *__restrict__ w, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        w[i] = 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void quant_encode(const float *__restrict__ data,
+                                                         const uint32_t *__restrict__ index, size_t npix,
+                                                         QuantTable t, const uint64_t *__restrict__ offsets,
+                                                         uint32_t *__restrict__ words)
+{
+    __shared__ QuantTableLds s;
+    load_table(t, s);
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix)
+        return;
+    const uint32_t start = index[p], n = index[p + 1] - start;
+    if (n == 0 || n > (uint32_t)t.n_bits)
+        return;
+    const uint64_t first = offsets[p], last = first + s.cum[n];
+    uint64_t w = first >> 5;
+    int fill = (int)(first & 31u);
+    uint64_t acc = 0;
+    auto emit = [&](uint32_t v) {
+        if ((w << 5) >= first && (w << 5) + 32u <= last)
+            words[w] = v;
+        else
+            atomicOr(&words[w], v);
+        ++w;
+    };
+    for (uint32_t l = 0; l < n; ++l)
+    {
+        float lo, hi;
+        quant_range(l, t.prediction_coded, lo, hi);
+        const int b = s.bits[l];
+        acc |= (uint64_t)quantize_value(data[start + l], b, lo, hi) << fill;
+        fill += b;
+        if (fill >= 32)
+        {
+            emit((uint32_t)acc);
+            acc >>= 32;
+            fill -= 32;
+        }
+    }
+    if (fill > 0)
+        emit((uint32_t)acc);
+}
+
+// byte i of the stream is byte i of the little-endian word buffer
+__global__ __launch_bounds__(kThreads) void copy_stream(const uint32_t *__restrict__ words, size_t nbytes,
+                                                        uint8_t *__restrict__ out)
+{
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(words);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = b[i];
+}
+
+__global__ __launch_bounds__(kThreads) void load_stream(const uint8_t *__restrict__ in, size_t nbytes,
+                                                        uint32_t *__restrict__ words, size_t nwords)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+    {
+        uint32_t v = 0;
+        for (int k = 0; k < 4; ++k)
+        {
+            const size_t e = i * 4 + k;
+            if (e < nbytes)
+                v |= (uint32_t)in[e] << (8 * k);
+        }
+        words[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void quant_decode(const uint32_t *__restrict__ words,
+                                                         const uint32_t *__restrict__ index, size_t npix,
+                                                         QuantTable t, const uint64_t *__restrict__ offsets,
+                                                         float *__restrict__ data)
+{
+    __shared__ QuantTableLds s;
+    load_table(t, s);
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix)
+        return;
+    const uint32_t start = index[p], n = index[p + 1] - start;
+    if (n > (uint32_t)t.n_bits)
+        return;
+    uint64_t bit = offsets[p];
+    for (uint32_t l = 0; l < n; ++l)
+    {
+        const int b = s.bits[l];
+        const uint64_t w = bit >> 5;
+        const uint64_t win = (uint64_t)words[w] | ((uint64_t)words[w + 1] << 32);
+        const uint32_t q = (uint32_t)((win >> (bit & 31u)) & ((b == 32) ? 0xFFFFFFFFull : ((1ull << b) - 1ull)));
+        float lo, hi;
+        quant_range(l, t.prediction_coded, lo, hi);
+        data[start + l] = dequantize_value(q, b, lo, hi);
+        bit += b;
+    }
+}
+
+unsigned grid_for(size_t n)
+{
+    return (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 65536);
+}
+
+int make_table(const uint8_t *bits, int n_bits, int prediction_coded, QuantTable &t)
+{
+    if (!bits || n_bits < 0 || n_bits > 256)
+        return MRAY_ERR_ARG;
+    std::memset(&t, 0, sizeof t);
+    t.n_bits = n_bits;
+    t.prediction_coded = prediction_coded ? 1 : 0;
+    for (int l = 0; l < n_bits; ++l)
+    {
+        if (bits[l] < 1 || bits[l] > 32)
+            return MRAY_ERR_ARG;
+        t.bits[l] = bits[l];
+        t.cum[l + 1] = t.cum[l] + bits[l];
+    }
+    return 0;
+}
+
+// the pixel offsets (offsets[npix] = total bits) and the error flag, the
+// total and flag copied back to the host
+int scan_offsets(const uint32_t *index, size_t npix, const QuantTable &t, uint64_t *offsets, uint64_t &total,
+                 hipStream_t stream)
+{
+    const int ntiles = (int)((npix + kTile - 1) / kTile);
+    uint8_t *scratch = nullptr;
+    const size_t sums_b = ((size_t)std::max(ntiles, 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t toff_b = (size_t)std::max(ntiles, 1) * sizeof(uint64_t);
+    if (hipMallocAsync((void **)&scratch, sums_b + toff_b + 256, stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    uint32_t *sums = reinterpret_cast<uint32_t *>(scratch);
+    uint64_t *toff = reinterpret_cast<uint64_t *>(scratch + sums_b);
+    int *err = reinterpret_cast<int *>(scratch + sums_b + toff_b);
+    hipLaunchKernelGGL(fill_words, dim3(1), dim3(kThreads), 0, stream, reinterpret_cast<uint32_t *>(err), (size_t)1);
+    if (ntiles == 0)
+        hipLaunchKernelGGL(fill_words, dim3(1), dim3(kThreads), 0, stream, reinterpret_cast<uint32_t *>(offsets),
+                           (size_t)2);
+    else
+    {
+        hipLaunchKernelGGL(quant_tile_sums, dim3(ntiles), dim3(kThreads), 0, stream, index, npix, t, sums, err);
+        hipLaunchKernelGGL(quant_tile_offsets, dim3(1), dim3(kThreads), 0, stream, sums, ntiles, toff, offsets,
+                           npix);
+        hipLaunchKernelGGL(quant_tile_apply, dim3(ntiles), dim3(kThreads), 0, stream, index, npix, t, toff,
+                           offsets);
+    }
+    struct
+    {
+        uint64_t total;
+        int err;
+    } h = {0, 0};
+    const bool copied =
+        hipMemcpyAsync(&h.total, offsets + npix, sizeof(uint64_t), hipMemcpyDeviceToHost, stream) == hipSuccess &&
+        hipMemcpyAsync(&h.err, err, sizeof(int), hipMemcpyDeviceToHost, stream) == hipSuccess;
+    const bool freed = hipFreeAsync(scratch, stream) == hipSuccess;
+    if (!copied || !freed || hipStreamSynchronize(stream) != hipSuccess)
+        return MRAY_ERR_HIP;
+    if (h.err)
+        return MRAY_ERR_ARG;
+    total = h.total;
+    return 0;
+}
+
+} // namespace
+
+int launch_quantize(const float *data, const uint32_t *index, int W, int H, int prediction_coded,
+                    const uint8_t *bits, int n_bits, uint8_t *stream_out, size_t capacity, size_t *stream_bytes,
+                    hipStream_t stream)
+{
+    QuantTable t;
+    int rc = make_table(bits, n_bits, prediction_coded, t);
+    if (rc)
+        return rc;
+    const size_t npix = (size_t)W * H;
+    uint64_t *offsets = nullptr;
+    if (hipMallocAsync((void **)&offsets, (npix + 1) * sizeof(uint64_t), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    uint64_t total = 0;
+    rc = scan_offsets(index, npix, t, offsets, total, stream);
+    const size_t nbytes = (size_t)(total / 8);
+    if (stream_bytes)
+        *stream_bytes = nbytes;
+    if (rc == 0 && stream_out && nbytes > capacity)
+        rc = MRAY_ERR_ARG;
+    if (rc == 0 && stream_out && nbytes > 0)
+    {
+        const size_t nwords = (size_t)((total + 31) / 32) + 1;
+        uint32_t *words = nullptr;
+        if (hipMallocAsync((void **)&words, nwords * sizeof(uint32_t), stream) != hipSuccess)
+            rc = MRAY_ERR_OOM;
+        else
+        {
+            hipLaunchKernelGGL(fill_words, dim3(grid_for(nwords)), dim3(kThreads), 0, stream, words, nwords);
+            hipLaunchKernelGGL(quant_encode, dim3((unsigned)((npix + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                               stream, data, index, npix, t, offsets, words);
+            hipLaunchKernelGGL(copy_stream, dim3(grid_for(nbytes)), dim3(kThreads), 0, stream, words, nbytes,
+                               stream_out);
+            if (hipFreeAsync(words, stream) != hipSuccess)
+                rc = MRAY_ERR_HIP;
+        }
+    }
+    if (hipFreeAsync(offsets, stream) != hipSuccess && rc == 0)
+        rc = MRAY_ERR_HIP;
+    return rc;
+}
+
+int launch_dequantize(const uint8_t *stream_in, size_t stream_bytes, const uint32_t *index, int W, int H,
+                      int prediction_coded, const uint8_t *bits, int n_bits, float *data, hipStream_t stream)
+{
+    QuantTable t;
+    int rc = make_table(bits, n_bits, prediction_coded, t);
+    if (rc)
+        return rc;
+    const size_t npix = (size_t)W * H;
+    uint64_t *offsets = nullptr;
+    if (hipMallocAsync((void **)&offsets, (npix + 1) * sizeof(uint64_t), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    uint64_t total = 0;
+    rc = scan_offsets(index, npix, t, offsets, total, stream);
+    // the stream holds every whole byte of the words
+    if (rc == 0 && stream_bytes < total / 8)
+        rc = MRAY_ERR_ARG;
+    if (rc == 0 && total > 0)
+    {
+        const size_t nwords = (size_t)((total + 31) / 32) + 1;
+        uint32_t *words = nullptr;
+        if (hipMallocAsync((void **)&words, nwords * sizeof(uint32_t), stream) != hipSuccess)
+            rc = MRAY_ERR_OOM;
+        else
+        {
+            const size_t used = std::min<size_t>(stream_bytes, (size_t)((total + 7) / 8));
+            hipLaunchKernelGGL(load_stream, dim3(grid_for(nwords)), dim3(kThreads), 0, stream, stream_in, used,
+                               words, nwords);
+            hipLaunchKernelGGL(quant_decode, dim3((unsigned)((npix + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                               stream, words, index, npix, t, offsets, data);
+            if (hipFreeAsync(words, stream) != hipSuccess)
+                rc = MRAY_ERR_HIP;
+        }
+    }
+    if (hipFreeAsync(offsets, stream) != hipSuccess && rc == 0)
+        rc = MRAY_ERR_HIP;
+    return rc;
+}
+
+} // namespace mray

@@ -42,6 +42,10 @@ _SIGNATURES = {
     "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_prediction_encode": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_prepare_coded": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_quantize": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, _vp, C.c_size_t,
+                           C.POINTER(C.c_size_t), _vp], C.c_int),
+    "mray_hip_dequantize": ([_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, _vp,
+                             _vp], C.c_int),
     "mray_hip_particles_to_grid": ([_vp, _vp, _vp, _vp, C.c_uint32, C.c_float, _f3, _f3, C.c_int, C.c_int, C.c_int,
                                     _vp, _vp], C.c_int),
     "mray_hip_synth_mixing_layer": ([_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -283,6 +287,53 @@ def compact(moments: torch.Tensor, index: torch.Tensor, W: int, H: int, m: int, 
     out = torch.empty(max(total, 1), dtype=torch.float32, device=moments.device)
     _check(load_library().mray_hip_compact(_ptr(moments), _ptr(index), W, H, m, _ptr(out), _stream(stream)),
            "mray_hip_compact")
+    return out[:total]
+
+
+def quantization_table(m: int, prediction_coded: bool, bits: int = 10, table=None):
+    """Word widths per moment (uint8, m entries): the given table (the
+    reference's loaded quantization_table), else the reference's default for
+    prediction-coded images (16 bits for moment 0, then `bits`;
+    get_prediction_coding_quantization_table, src/moment_quantization.cpp:186-203)
+    or `bits` for every moment (quantize(mi, bits))."""
+    t = (C.c_uint8 * m)(*([bits] * m))
+    if table is not None:
+        for i, b in enumerate(table[:m]):
+            t[i] = int(b)
+    elif prediction_coded:
+        t[0] = 16
+    return t
+
+
+def _table(bits):
+    return bits if isinstance(bits, C.Array) else (C.c_uint8 * len(bits))(*[int(b) for b in bits])
+
+
+def quantize(data: torch.Tensor, index: torch.Tensor, W: int, H: int, prediction_coded: bool, bits,
+             stream=None) -> torch.Tensor:
+    """The quantized bit stream of a compact image (uint8 tensor): words of
+    bits[l] bits per moment, least significant bit first (mray_hip_quantize)."""
+    t = _table(bits)
+    n = C.c_size_t(0)
+    lib = load_library()
+    _check(lib.mray_hip_quantize(_ptr(data), _ptr(index), W, H, int(bool(prediction_coded)), t, len(t), None, 0,
+                                 C.byref(n), _stream(stream)), "mray_hip_quantize")
+    out = torch.empty(max(n.value, 1), dtype=torch.uint8, device=index.device)
+    _check(lib.mray_hip_quantize(_ptr(data), _ptr(index), W, H, int(bool(prediction_coded)), t, len(t), _ptr(out),
+                                 out.numel(), C.byref(n), _stream(stream)), "mray_hip_quantize")
+    return out[:n.value]
+
+
+def dequantize(stream_bytes: torch.Tensor, index: torch.Tensor, W: int, H: int, prediction_coded: bool, bits,
+               stream=None) -> torch.Tensor:
+    """The compact image a quantized stream holds (mray_hip_dequantize)."""
+    t = _table(bits)
+    total = int(index[-1].to(torch.int64).item()) & 0xFFFFFFFF
+    out = torch.empty(max(total, 1), dtype=torch.float32, device=index.device)
+    sb = stream_bytes if stream_bytes.numel() > 0 else None
+    _check(load_library().mray_hip_dequantize(_ptr(sb), stream_bytes.numel(), _ptr(index), W, H,
+                                              int(bool(prediction_coded)), t, len(t), _ptr(out), _stream(stream)),
+           "mray_hip_dequantize")
     return out[:total]
 
 

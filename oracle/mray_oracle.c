@@ -1900,3 +1900,114 @@ void orc_particles_to_grid(const float *x, const float *y, const float *z, const
     free(start);
     free(count);
 }
+
+/* ------------------------------------------------------------------------ */
+/* quantization (src/moment_quantization.cpp)                               */
+/* ------------------------------------------------------------------------ */
+
+/* quantize (src/moment_quantization.cpp:33-50): clamp, normalise, scale by
+ * 2^bits - 1 in float and truncate; 32 bits keep the clamped float's bits */
+uint32_t orc_quantize_value(float value, int bits, float lo, float hi)
+{
+    value = fclampr(value, lo, hi);
+    if (bits == 32)
+    {
+        uint32_t u;
+        memcpy(&u, &value, sizeof u);
+        return u;
+    }
+    const float norm = (value - lo) / (hi - lo);
+    const uint32_t full = (uint32_t)((1u << bits) - 1u);
+    const uint32_t q = (uint32_t)(norm * (float)(int32_t)full);
+    return q & full;
+}
+
+/* dequantize (src/moment_quantization.cpp:52-63) */
+float orc_dequantize_value(uint32_t q, int bits, float lo, float hi)
+{
+    if (bits == 32)
+    {
+        float f;
+        memcpy(&f, &q, sizeof f);
+        return fclampr(f, lo, hi);
+    }
+    const float norm = (float)q / (float)(int32_t)((1u << bits) - 1u);
+    return norm * (hi - lo) + lo;
+}
+
+/* the value range of moment l: quantize_prediction_coding uses
+ * PREDICTION_CODING_QUANTIZATION_MIN/MAX = [0, 1] (src/moment_prediction_coding.h:32-33)
+ * for every moment, quantize [0, 1] for moment 0 and [-1/pi, 1/pi] (float
+ * parameters from the double quotients) for the others
+ * (src/moment_quantization.cpp:253-283, 300-329) */
+static void quant_range(int l, int prediction_coded, float *lo, float *hi)
+{
+    if (l == 0 || prediction_coded)
+    {
+        *lo = 0.0f;
+        *hi = 1.0f;
+    }
+    else
+    {
+        *lo = (float)(-1.0 / M_PI);
+        *hi = (float)(1.0 / M_PI);
+    }
+}
+
+/* quantize_prediction_coding / quantize of a compact image with pack_bits
+ * (src/moment_quantization.cpp:65-111, 244-283, 300-329): pixels without
+ * moments are skipped, the bits of each word are appended least significant
+ * first, and a last partial byte is not written */
+size_t orc_quantize_image(int W, int H, const float *data, const uint32_t *index, int prediction_coded,
+                          const uint8_t *bits, uint8_t *out)
+{
+    size_t nbytes = 0;
+    uint32_t buffer = 0;
+    int cur = 0;
+    for (size_t pix = 0; pix < (size_t)W * H; ++pix)
+    {
+        const uint32_t start = index[pix], n = index[pix + 1] - start;
+        for (uint32_t l = 0; l < n; ++l)
+        {
+            float lo, hi;
+            quant_range((int)l, prediction_coded, &lo, &hi);
+            const uint32_t q = orc_quantize_value(data[start + l], bits[l], lo, hi);
+            for (int b = 0; b < bits[l]; ++b)
+            {
+                if (q & (1u << b))
+                    buffer |= 1u << cur;
+                if (++cur == 8)
+                {
+                    out[nbytes++] = (uint8_t)buffer;
+                    buffer = 0;
+                    cur = 0;
+                }
+            }
+        }
+    }
+    return nbytes;
+}
+
+/* dequantize_prediction_coding / dequantize with unpack_bits
+ * (src/moment_quantization.cpp:113-163, 285-298, 331-360). Bits past the end
+ * of the stream (the dropped partial byte) read as zero; the reference reads
+ * past its word list there. */
+void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_coded, const uint8_t *bits,
+                          const uint8_t *in, size_t n_in, float *data)
+{
+    size_t bit = 0;
+    for (size_t pix = 0; pix < (size_t)W * H; ++pix)
+    {
+        const uint32_t start = index[pix], n = index[pix + 1] - start;
+        for (uint32_t l = 0; l < n; ++l)
+        {
+            uint32_t q = 0;
+            for (int b = 0; b < bits[l]; ++b, ++bit)
+                if ((bit >> 3) < n_in && (in[bit >> 3] >> (bit & 7)) & 1u)
+                    q |= 1u << b;
+            float lo, hi;
+            quant_range((int)l, prediction_coded, &lo, &hi);
+            data[start + l] = orc_dequantize_value(q, bits[l], lo, hi);
+        }
+    }
+}

@@ -142,6 +142,19 @@ void orc_particles_to_grid(const float *x, const float *y, const float *z, const
                            const float dmin[3], const float dmax[3], int nx, int ny, int nz, float *grid,
                            int nthreads);
 
+/* ---- quantization of compact moment images (src/moment_quantization.cpp) ----
+ * bits[l] is the word width of moment l (1..32). prediction_coded selects the
+ * ranges: [0, 1] for every moment (quantize_prediction_coding), else [0, 1]
+ * for moment 0 and [-1/pi, 1/pi] for the others (quantize). The stream packs
+ * the words of all pixels in row order, least significant bit first; a last
+ * partial byte is dropped as the reference's pack_bits does. */
+uint32_t orc_quantize_value(float value, int bits, float lo, float hi);
+float orc_dequantize_value(uint32_t q, int bits, float lo, float hi);
+size_t orc_quantize_image(int W, int H, const float *data, const uint32_t *index, int prediction_coded,
+                          const uint8_t *bits, uint8_t *out);
+void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_coded, const uint8_t *bits,
+                          const uint8_t *in, size_t n_in, float *data);
+
 #ifdef __cplusplus
 }
 #endif

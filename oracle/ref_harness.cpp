@@ -15,6 +15,7 @@
 #include "moment_prediction_coding.h"
 #include "moment_preparation.h"
 #include "coding_transform.h"
+#include "moment_quantization.h"
 #include "mese/MESE_4_10.h"
 
 #include <cstdio>
@@ -96,6 +97,20 @@ TransferFunction make_tf(const uint8_t *rgba8, float step)
     TransferFunction tf = read_from_disk(path, step);
     unlink(path);
     return tf;
+}
+} // namespace
+
+namespace
+{
+// a compact image of the given counts and data, as moment_image_io reads one
+MomentImageHost make_compact(int W, int H, const float *data, const uint32_t *index, int prediction_coded)
+{
+    MomentImageHost mi(W, H, 256, true, false);
+    mi.index.assign(index, index + (size_t)W * H + 1);
+    mi.data.assign(data, data + index[(size_t)W * H]);
+    mi.is_compact = true;
+    mi.prediction_code = prediction_coded != 0;
+    return mi;
 }
 } // namespace
 
@@ -349,5 +364,36 @@ void ref_particles_sample(const float *x, const float *y, const float *z, const 
 #pragma omp parallel for
     for (long i = 0; i < (long)npts; ++i)
         out[i] = parts.sample(Vec3f(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+
+// moment_quantization::quantize_prediction_coding(mi, table) /
+// moment_quantization::quantize(mi, bits) (src/moment_quantization.cpp:244-329).
+// Returns the stream length in bytes.
+long ref_quantize(int W, int H, const float *data, const uint32_t *index, int prediction_coded, const uint8_t *bits,
+                  uint8_t *out)
+{
+    auto mi = make_compact(W, H, data, index, prediction_coded);
+    vector<Byte> q;
+    if (prediction_coded)
+        q = moment_quantization::quantize_prediction_coding(mi, vector<Byte>(bits, bits + 256));
+    else
+        q = moment_quantization::quantize(mi, bits[0]);
+    std::memcpy(out, q.data(), q.size());
+    return (long)q.size();
+}
+
+// moment_quantization::dequantize_prediction_coding / dequantize
+// (src/moment_quantization.cpp:285-360), on streams whose word count is whole
+void ref_dequantize(int W, int H, const uint32_t *index, int prediction_coded, const uint8_t *bits,
+                    const uint8_t *in, long n_in, float *data)
+{
+    vector<float> zeros(index[(size_t)W * H], 0.0f);
+    auto mi = make_compact(W, H, zeros.data(), index, prediction_coded);
+    vector<Byte> qs(in, in + n_in);
+    if (prediction_coded)
+        moment_quantization::dequantize_prediction_coding(mi, vector<Byte>(bits, bits + 256), qs);
+    else
+        moment_quantization::dequantize(mi, bits[0], qs);
+    std::memcpy(data, mi.data.data(), mi.data.size() * sizeof(float));
 }
 }

@@ -276,11 +276,50 @@ def sph_vectors(R: Reference) -> dict:
     return dict(sph_x=x, sph_y=y, sph_z=z, sph_v=v, sph_h=f32([h]), sph_grid=res.reshape(n[2], n[1], n[0]))
 
 
+def quant_vectors(R: Reference) -> dict:
+    """moment_quantization's packed streams (quantize_prediction_coding with the
+    default and a loaded table, quantize at several widths) of compact images
+    with empty pixels, out-of-range and signed-zero values, and their
+    dequantized values where the stream holds every word whole."""
+    from oracle_ffi import quant_bits
+
+    rng = np.random.default_rng(2024)
+    table = [16, 14, 12, 12, 11, 10, 10, 9, 9, 8, 8, 8, 7, 7, 7, 6, 6, 6, 5, 5, 5, 5, 4, 4, 4, 4, 3]
+    cases = {"pc_default10": (1, quant_bits(256, 1, 10), True), "pc_table": (1, quant_bits(256, 1, table=table), True),
+             "pc_ragged12": (1, quant_bits(256, 1, 12), False), "plain8": (0, quant_bits(256, 0, 8), True),
+             "plain13": (0, quant_bits(256, 0, 13), True), "plain32": (0, quant_bits(256, 0, 32), True)}
+    out = {}
+    for name, (pc, bits, whole) in cases.items():
+        W, H = 19, 11
+        counts = rng.integers(0, 40, W * H).astype(np.uint32)
+        counts[rng.random(W * H) < 0.2] = 0
+        cum = np.concatenate([[0], np.cumsum(bits.astype(np.int64))])
+        # the last pixel's count sets whether the stream ends on a byte boundary
+        for c in range(1, 60):
+            counts[-1] = c
+            if (int(cum[counts].sum()) % 8 == 0) == whole:
+                break
+        index = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+        n = int(index[-1])
+        if pc:
+            data = rng.uniform(-0.05, 1.05, n).astype(np.float32)
+        else:
+            data = rng.uniform(-0.34, 0.34, n).astype(np.float32)
+        data[::13] = 0.0
+        data[::17] = -0.0
+        stream = R.quantize(W, H, data, index, pc, bits)
+        out.update({f"{name}_index": index, f"{name}_data": data, f"{name}_pc": np.int32(pc),
+                    f"{name}_bits": bits, f"{name}_stream": stream})
+        if whole:
+            out[f"{name}_decoded"] = R.dequantize(W, H, index, pc, bits, stream)
+    return out
+
+
 def main():
     # python make_golden.py [name ...]: only the named fixtures (default all)
     R = Reference()
     makers = {"mese": mese_vectors, "coding": coding_vectors, "images": image_vectors, "sph": sph_vectors,
-              "reference": reference_dvr_vectors}
+              "reference": reference_dvr_vectors, "quant": quant_vectors}
     for name in (sys.argv[1:] or list(makers)):
         np.savez_compressed(OUT / f"golden_{name}.npz", **makers[name](R))
     for f in sorted(OUT.glob("golden_*.npz")):

@@ -53,6 +53,26 @@ def nthreads():
     return int(os.environ.get("MRAY_ORACLE_THREADS", "0"))
 
 
+def quant_bits(m, prediction_coded, bits=10, table=None):
+    """The per-moment word widths the reference quantizes with: the table when
+    one is given, else get_prediction_coding_quantization_table's default
+    (16 bits for moment 0, then `bits`) for prediction-coded images and
+    `bits` for every moment otherwise (src/moment_quantization.cpp:186-203)."""
+    t = np.full(256, bits, np.uint8)
+    if table is not None:
+        t[:len(table)] = table
+    elif prediction_coded:
+        t[0] = 16
+    return t
+
+
+def stream_bytes(index, bits):
+    """floor(total bits / 8): the length of the reference's packed stream."""
+    counts = np.diff(index.astype(np.int64))
+    cum = np.concatenate([[0], np.cumsum(bits.astype(np.int64))])
+    return int(cum[counts].sum()) // 8
+
+
 class Oracle:
     """numpy-level wrappers over liboracle.so."""
 
@@ -216,6 +236,20 @@ class Oracle:
                                           ptr(out, _f))
         return out[:nm]
 
+    def quantize(self, W, H, data, index, prediction_coded, bits):
+        out = np.zeros(stream_bytes(index, bits) + 8, np.uint8)
+        self.lib.orc_quantize_image.restype = C.c_size_t
+        n = self.lib.orc_quantize_image(W, H, ptr(np.ascontiguousarray(data, np.float32), _f), ptr(index, _u32),
+                                        prediction_coded, ptr(bits, _u8), ptr(out, _u8))
+        return out[:n]
+
+    def dequantize(self, W, H, index, prediction_coded, bits, stream):
+        out = np.zeros(max(int(index[-1]), 1), np.float32)
+        st = np.ascontiguousarray(stream, np.uint8)
+        self.lib.orc_dequantize_image(W, H, ptr(index, _u32), prediction_coded, ptr(bits, _u8), ptr(st, _u8),
+                                      C.c_size_t(len(st)), ptr(out, _f))
+        return out[:int(index[-1])]
+
     def particles_to_grid(self, x, y, z, v, h, dmin, dmax, nx, ny, nz):
         g = np.zeros((nz, ny, nx), np.float32)
         bmin = (C.c_float * 3)(*dmin)
@@ -258,6 +292,20 @@ class Reference:
                                   bmin, bmax, C.byref(c), W, H, m, C.byref(gp), ptr(data, _f), ptr(bounds, _u32),
                                   ptr(index, _u32), ptr(cp, _u16))
         return data[:n], bounds, index, cp
+
+    def quantize(self, W, H, data, index, prediction_coded, bits):
+        out = np.zeros(stream_bytes(index, bits) + 8, np.uint8)
+        self.lib.ref_quantize.restype = C.c_long
+        n = self.lib.ref_quantize(W, H, ptr(np.ascontiguousarray(data, np.float32), _f), ptr(index, _u32),
+                                  prediction_coded, ptr(bits, _u8), ptr(out, _u8))
+        return out[:n]
+
+    def dequantize(self, W, H, index, prediction_coded, bits, stream):
+        out = np.zeros(max(int(index[-1]), 1), np.float32)
+        st = np.ascontiguousarray(stream, np.uint8)
+        self.lib.ref_dequantize(W, H, ptr(index, _u32), prediction_coded, ptr(bits, _u8), ptr(st, _u8),
+                                C.c_long(len(st)), ptr(out, _f))
+        return out[:int(index[-1])]
 
     def reference(self, vol, dmin, dmax, cam, W, H, tf_rgba8, tf_step, p: RefParams, want_rgba8=False):
         c = Camera.from_dict(cam)

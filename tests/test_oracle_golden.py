@@ -358,6 +358,61 @@ def test_reference_dvr_matches_reference(O, case):
     assert np.array_equal(q, g[f"{case}_rgba8"])
 
 
+QUANT_CASES = ["pc_default10", "pc_table", "pc_ragged12", "plain8", "plain13", "plain32"]
+
+
+@pytest.mark.parametrize("case", QUANT_CASES)
+def test_quantization_matches_reference(O, case):
+    """Packed streams and dequantized values of moment_quantization
+    (src/moment_quantization.cpp:244-360) bit for bit."""
+    g = dict(np.load(GOLD / "golden_quant.npz"))
+    index, data, pc, bits = g[f"{case}_index"], g[f"{case}_data"], int(g[f"{case}_pc"]), g[f"{case}_bits"]
+    W, H = 19, 11
+    stream = O.quantize(W, H, data, index, pc, bits)
+    assert np.array_equal(stream, g[f"{case}_stream"])
+    if f"{case}_decoded" in g:
+        dec = O.dequantize(W, H, index, pc, bits, stream)
+        assert bits_equal(dec, g[f"{case}_decoded"])
+
+
+def test_quantization_packing_and_steps(O):
+    """Independent of the reference: the stream is the concatenation of the
+    words least significant bit first (numpy's little-endian bit order), and
+    each value comes back within one quantization step of its clamp."""
+    from oracle_ffi import quant_bits
+
+    rng = np.random.default_rng(7)
+    W, H = 9, 5
+    counts = rng.integers(0, 12, W * H).astype(np.uint32)
+    index = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    n = int(index[-1])
+    for pc, b in ((1, 10), (0, 7), (0, 32)):
+        bits = quant_bits(256, pc, b)
+        lo = np.where((np.arange(n) - np.repeat(index[:-1], counts) == 0) | (pc == 1), 0.0, -1 / math.pi)
+        lo = lo.astype(np.float32)
+        hi = np.where(lo == 0, 1.0, 1 / math.pi).astype(np.float32)
+        data = (lo + (hi - lo) * rng.uniform(-0.1, 1.1, n)).astype(np.float32)
+        stream = O.quantize(W, H, data, index, pc, bits)
+        ls = np.concatenate([np.arange(c) for c in counts]).astype(np.int64)
+        widths = bits[ls].astype(np.int64)
+        dec = O.dequantize(W, H, index, pc, bits, stream)
+        # re-pack the words the decoder returns, little-endian bit order
+        words = []
+        for v, w, l0, h0 in zip(dec, widths, lo, hi):
+            if w == 32:
+                words.append(int(np.float32(v).view(np.uint32)))
+            else:
+                words.append(int(round((float(v) - float(l0)) / (float(h0) - float(l0)) * ((1 << w) - 1))))
+        bitlist = np.concatenate([[(q >> i) & 1 for i in range(w)] for q, w in zip(words, widths)]).astype(np.uint8)
+        nb = len(bitlist) // 8
+        assert np.array_equal(np.packbits(bitlist[:nb * 8], bitorder="little"), stream)
+        clamped = np.clip(data, lo, hi)
+        step = np.where(widths == 32, 0.0, (hi - lo) / ((2.0 ** np.minimum(widths, 31)) - 1))
+        whole = np.arange(n) < n - (1 if len(bitlist) % 8 else 0)
+        err = np.abs(dec - clamped)[whole]
+        assert np.all(err <= step[whole] * (1 + 1e-3) + 1e-7)
+
+
 def test_sph_gather_matches_reference():
     g = dict(np.load(GOLD / "golden_sph.npz"))
     O = Oracle()
