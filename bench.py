@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-dvr", action="store_true", help="skip the reference direct volume renderer")
+    ap.add_argument("--no-quant", action="store_true", help="skip the moment-image quantization")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the adjoint workload and print its figures (for kernel profiles)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -214,6 +215,33 @@ def main():
         dvr = {"workload": "reference DVR, same volume, camera, table and step", "ms_per_frame": round(ms_dvr, 3),
                "Mrays_per_s": round(W * H / (ms_dvr * 1e-3) / 1e6, 3)}
 
+    # quantization of the frame's moment image for a moment-image file (10-bit
+    # words, the [-1/pi, 1/pi] range of moments 1..m-1): encode and decode per
+    # call, the bit-offset scan and the length read-back included
+    quant = None
+    if not args.no_quant:
+        qindex = torch.arange(0, (W * H + 1) * m, m, dtype=torch.int64, device="cuda").to(torch.int32)
+        qbits = hip.quantization_table(m, False, 10)
+        qs = hip.quantize(mo, qindex, W, H, False, qbits)
+        dq = hip.dequantize(qs, qindex, W, H, False, qbits)
+        torch.cuda.synchronize()
+        times = {}
+        for name, fn in (("encode", lambda: hip.quantize(mo, qindex, W, H, False, qbits)),
+                         ("decode", lambda: hip.dequantize(qs, qindex, W, H, False, qbits))):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            times[name] = e0.elapsed_time(e1) / args.steps
+        # algorithmic bytes: the floats, the stream and the index (one pass each)
+        qbytes = W * H * m * 4 + qs.numel() + (W * H + 1) * 4
+        quant = {"workload": f"the {W}x{H} x {m} moment image, 10-bit words", "stream_bytes": int(qs.numel()),
+                 "ms": {k: round(v, 3) for k, v in times.items()},
+                 "GB_per_s": {k: round(qbytes / (v * 1e-3) / 1e9, 1) for k, v in times.items()}}
+        del qs, dq, qindex
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step, tf_factor)
@@ -247,6 +275,7 @@ def main():
                            "frac": round(total_flop / (ms_step * 1e-3) / PEAK_FP32_VALU, 4)},
         "adjoint": adjoint,
         "reference_dvr": dvr,
+        "quantization": quant,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -33,7 +33,10 @@ typedef struct mray_volume_s *mray_volume;
 typedef void *mray_stream;
 
 /* Selects the device and reads its properties. Replaces init_device
- * (src/cut/cuda_common.cpp:55-59). */
+ * (src/cut/cuda_common.cpp:55-59). Sets the release threshold of the
+ * device's default memory pool, which the library's stream-ordered scratch
+ * (hipMallocAsync) comes from, to its maximum: freed scratch stays in the
+ * pool for the next call instead of being returned at each synchronisation. */
 int mray_hip_init(int device);
 
 /* Thread-local description of the last error of this thread. */

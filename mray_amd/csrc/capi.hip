@@ -80,6 +80,15 @@ int mray_hip_init(int device)
     if (hipGetDeviceProperties(&prop, device) != hipSuccess)
         return fail(MRAY_ERR_HIP, "mray_hip_init: hipGetDeviceProperties failed");
     mray::set_lds_limit(prop.sharedMemPerBlock);
+    // the library's scratch comes from the device's default pool
+    // (hipMallocAsync); with the default release threshold of zero the pool
+    // returns freed memory at every synchronisation, and the next call maps
+    // it again. The pool keeps what it holds instead.
+    hipMemPool_t pool;
+    uint64_t keep = UINT64_MAX;
+    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess ||
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+        return fail(MRAY_ERR_HIP, "mray_hip_init: setting the memory pool's release threshold failed");
     g_init = true;
     return MRAY_OK;
 }

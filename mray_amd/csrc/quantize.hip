@@ -11,14 +11,18 @@
 //   1. scan: per-tile sums of the pixel bit counts, a 64-bit scan of the tile
 //      sums, then per tile the offsets (64-bit: a 2048^2 image of 100 moments
 //      at 16 bits is 6.7 Gbit); offsets[W*H] is the total
-//   2. encode: one thread per pixel quantizes its words into a 64-bit shift
-//      register and stores each full 32-bit stream word. Words that lie
-//      inside the pixel's bit range are written with plain stores, the (at
-//      most two) words shared with the neighbouring pixels with atomicOr
-//      into a cleared word buffer; the stream is the buffer's first bytes
-//   3. decode: the stream is copied into a cleared word buffer (bits past its
-//      end read as zero), one thread per pixel reads its words from 64-bit
-//      windows
+//   2. encode: the compact data is cut into chunks of kChunk consecutive
+//      elements (a map gives the pixel of each chunk's first element); one
+//      thread per chunk quantizes its words into a 64-bit shift register and
+//      stores each full 32-bit stream word. Words inside the chunk's bit
+//      range are written with plain stores, the (at most two) words shared
+//      with the neighbouring chunks with atomicOr into a cleared word
+//      buffer; the stream is the buffer's first bytes
+//   3. decode: the stream is copied into a zero-padded word buffer (bits past
+//      its end read as zero), one thread per chunk reads its words from
+//      64-bit windows
+// The stream copies in and out of the word buffer are DMA copies
+// (hipMemcpyAsync); no hipMemsetAsync is recorded.
 // Values round like the reference's float code: clamp by comparisons,
 // (v - lo) / (hi - lo) and q / (2^bits - 1) correctly rounded, the product
 // and sum unfused (-ffp-contract=off), float to integer by truncation.
@@ -38,6 +42,7 @@ namespace
 constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;
+constexpr int kChunk = 16; // elements per thread of the encode and decode
 
 struct QuantTable
 {
@@ -127,9 +132,20 @@ __global__ __launch_bounds__(kThreads) void quant_tile_sums(const uint32_t *__re
 
 // pass 2: 64-bit exclusive scan of the tile sums (one workgroup, chunks of
 // kThreads with a running carry); the total goes to offsets[npix]
+// what the host reads back after the scan, in one copy
+struct ScanSummary
+{
+    uint64_t total; // bits of the stream
+    uint32_t nelem; // elements of the compact image, index[W*H]
+    int32_t err;    // a pixel has more moments than the table
+};
+
 __global__ __launch_bounds__(kThreads) void quant_tile_offsets(const uint32_t *__restrict__ sums, int ntiles,
                                                                uint64_t *__restrict__ tile_off,
-                                                               uint64_t *__restrict__ offsets, size_t npix)
+                                                               uint64_t *__restrict__ offsets, size_t npix,
+                                                               const uint32_t *__restrict__ index,
+                                                               const int *__restrict__ err,
+                                                               ScanSummary *__restrict__ summary)
 {
     __shared__ uint64_t buf[kThreads];
     uint64_t carry = 0;
@@ -145,7 +161,12 @@ __global__ __launch_bounds__(kThreads) void quant_tile_offsets(const uint32_t *_
         __syncthreads();
     }
     if (threadIdx.x == 0)
+    {
         offsets[npix] = carry;
+        summary->total = carry;
+        summary->nelem = index[npix];
+        summary->err = *err;
+    }
 }
 
 // pass 3: per-pixel offsets inside each tile plus the tile's offset
@@ -228,97 +249,170 @@ __global__ __launch_bounds__(kThreads) void fill_words(uint32_t *__restrict__ w,
         w[i] = 0u;
 }
 
+// the pixel holding the first element of each chunk of kChunk elements of
+// the compact data (every chunk starts in exactly one non-empty pixel)
+__global__ __launch_bounds__(kThreads) void quant_chunk_map(const uint32_t *__restrict__ index, size_t npix,
+                                                            uint32_t *__restrict__ chunk_pixel)
+{
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix)
+        return;
+    const uint64_t e0 = index[p], e1 = index[p + 1];
+    for (uint64_t c = (e0 + kChunk - 1) / kChunk; c * kChunk < e1; ++c)
+        chunk_pixel[c] = (uint32_t)p;
+}
+
+// walks the elements of a chunk through the pixels they belong to: pixel p,
+// the end of its elements, and the element's moment l
+struct ElementCursor
+{
+    size_t p;
+    uint32_t pend, l;
+    __device__ __forceinline__ void at(const uint32_t *__restrict__ index, const uint32_t *__restrict__ chunk_pixel,
+                                       uint64_t c, uint64_t e)
+    {
+        p = chunk_pixel[c];
+        pend = index[p + 1];
+        l = (uint32_t)(e - index[p]);
+    }
+    // moves past the pixels that end at element k (empty ones included);
+    // k < index[W*H] bounds the walk
+    __device__ __forceinline__ void settle(const uint32_t *__restrict__ index, uint64_t k)
+    {
+        while (k == pend)
+        {
+            ++p;
+            pend = index[p + 1];
+            l = 0;
+        }
+    }
+};
+
+// one thread per chunk of kChunk consecutive elements (VEC: float4 loads of
+// a 16-byte aligned image): the chunk's words go into a 64-bit shift
+// register, each full 32-bit stream word inside the chunk's bit range is
+// stored, the first and last (shared with the neighbouring chunks) are
+// or-ed into the cleared buffer
+template <bool VEC>
 __global__ __launch_bounds__(kThreads) void quant_encode(const float *__restrict__ data,
-                                                         const uint32_t *__restrict__ index, size_t npix,
-                                                         QuantTable t, const uint64_t *__restrict__ offsets,
+                                                         const uint32_t *__restrict__ index, uint64_t nelem,
+                                                         QuantTable t, const uint32_t *__restrict__ chunk_pixel,
+                                                         const uint64_t *__restrict__ offsets,
                                                          uint32_t *__restrict__ words)
 {
     __shared__ QuantTableLds s;
     load_table(t, s);
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npix)
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t e = c * kChunk;
+    if (e >= nelem)
         return;
-    const uint32_t start = index[p], n = index[p + 1] - start;
-    if (n == 0 || n > (uint32_t)t.n_bits)
-        return;
-    const uint64_t first = offsets[p], last = first + s.cum[n];
+    const int cnt = (int)((nelem - e < (uint64_t)kChunk) ? (nelem - e) : (uint64_t)kChunk);
+    float v[kChunk];
+    if (VEC && cnt == kChunk)
+    {
+        const float4 *d4 = reinterpret_cast<const float4 *>(data + e);
+#pragma unroll
+        for (int i = 0; i < kChunk / 4; ++i)
+        {
+            const float4 x = d4[i];
+            v[4 * i] = x.x;
+            v[4 * i + 1] = x.y;
+            v[4 * i + 2] = x.z;
+            v[4 * i + 3] = x.w;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i)
+            v[i] = (i < cnt) ? data[e + i] : 0.0f;
+    }
+    ElementCursor cur;
+    cur.at(index, chunk_pixel, c, e);
+    const uint64_t first = offsets[cur.p] + s.cum[cur.l];
     uint64_t w = first >> 5;
     int fill = (int)(first & 31u);
     uint64_t acc = 0;
-    auto emit = [&](uint32_t v) {
-        if ((w << 5) >= first && (w << 5) + 32u <= last)
-            words[w] = v;
-        else
-            atomicOr(&words[w], v);
-        ++w;
-    };
-    for (uint32_t l = 0; l < n; ++l)
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i)
     {
-        float lo, hi;
-        quant_range(l, t.prediction_coded, lo, hi);
-        const int b = s.bits[l];
-        acc |= (uint64_t)quantize_value(data[start + l], b, lo, hi) << fill;
-        fill += b;
-        if (fill >= 32)
+        if (i < cnt)
         {
-            emit((uint32_t)acc);
-            acc >>= 32;
-            fill -= 32;
+            cur.settle(index, e + i);
+            float lo, hi;
+            quant_range(cur.l, t.prediction_coded, lo, hi);
+            const int b = s.bits[cur.l];
+            acc |= (uint64_t)quantize_value(v[i], b, lo, hi) << fill;
+            fill += b;
+            if (fill >= 32)
+            {
+                if ((w << 5) >= first)
+                    words[w] = (uint32_t)acc;
+                else
+                    atomicOr(&words[w], (uint32_t)acc);
+                ++w;
+                acc >>= 32;
+                fill -= 32;
+            }
+            ++cur.l;
         }
     }
     if (fill > 0)
-        emit((uint32_t)acc);
+        atomicOr(&words[w], (uint32_t)acc);
 }
 
-// byte i of the stream is byte i of the little-endian word buffer
-__global__ __launch_bounds__(kThreads) void copy_stream(const uint32_t *__restrict__ words, size_t nbytes,
-                                                        uint8_t *__restrict__ out)
-{
-    const uint8_t *b = reinterpret_cast<const uint8_t *>(words);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = b[i];
-}
-
-__global__ __launch_bounds__(kThreads) void load_stream(const uint8_t *__restrict__ in, size_t nbytes,
-                                                        uint32_t *__restrict__ words, size_t nwords)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
-    {
-        uint32_t v = 0;
-        for (int k = 0; k < 4; ++k)
-        {
-            const size_t e = i * 4 + k;
-            if (e < nbytes)
-                v |= (uint32_t)in[e] << (8 * k);
-        }
-        words[i] = v;
-    }
-}
-
+// one thread per chunk of kChunk elements: each word read from a 64-bit
+// window of the stream (VEC: float4 stores into a 16-byte aligned image)
+template <bool VEC>
 __global__ __launch_bounds__(kThreads) void quant_decode(const uint32_t *__restrict__ words,
-                                                         const uint32_t *__restrict__ index, size_t npix,
-                                                         QuantTable t, const uint64_t *__restrict__ offsets,
+                                                         const uint32_t *__restrict__ index, uint64_t nelem,
+                                                         QuantTable t, const uint32_t *__restrict__ chunk_pixel,
+                                                         const uint64_t *__restrict__ offsets,
                                                          float *__restrict__ data)
 {
     __shared__ QuantTableLds s;
     load_table(t, s);
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npix)
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t e = c * kChunk;
+    if (e >= nelem)
         return;
-    const uint32_t start = index[p], n = index[p + 1] - start;
-    if (n > (uint32_t)t.n_bits)
-        return;
-    uint64_t bit = offsets[p];
-    for (uint32_t l = 0; l < n; ++l)
+    const int cnt = (int)((nelem - e < (uint64_t)kChunk) ? (nelem - e) : (uint64_t)kChunk);
+    ElementCursor cur;
+    cur.at(index, chunk_pixel, c, e);
+    uint64_t bit = offsets[cur.p] + s.cum[cur.l];
+    float v[kChunk];
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i)
     {
-        const int b = s.bits[l];
-        const uint64_t w = bit >> 5;
-        const uint64_t win = (uint64_t)words[w] | ((uint64_t)words[w + 1] << 32);
-        const uint32_t q = (uint32_t)((win >> (bit & 31u)) & ((b == 32) ? 0xFFFFFFFFull : ((1ull << b) - 1ull)));
-        float lo, hi;
-        quant_range(l, t.prediction_coded, lo, hi);
-        data[start + l] = dequantize_value(q, b, lo, hi);
-        bit += b;
+        v[i] = 0.0f;
+        if (i < cnt)
+        {
+            cur.settle(index, e + i);
+            const int b = s.bits[cur.l];
+            const uint64_t w = bit >> 5;
+            const uint64_t win = (uint64_t)words[w] | ((uint64_t)words[w + 1] << 32);
+            const uint32_t q =
+                (uint32_t)((win >> (bit & 31u)) & ((b == 32) ? 0xFFFFFFFFull : ((1ull << b) - 1ull)));
+            float lo, hi;
+            quant_range(cur.l, t.prediction_coded, lo, hi);
+            v[i] = dequantize_value(q, b, lo, hi);
+            bit += b;
+            ++cur.l;
+        }
+    }
+    if (VEC && cnt == kChunk)
+    {
+        float4 *d4 = reinterpret_cast<float4 *>(data + e);
+#pragma unroll
+        for (int i = 0; i < kChunk / 4; ++i)
+            d4[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i)
+            if (i < cnt)
+                data[e + i] = v[i];
     }
 }
 
@@ -344,46 +438,39 @@ int make_table(const uint8_t *bits, int n_bits, int prediction_coded, QuantTable
     return 0;
 }
 
-// the pixel offsets (offsets[npix] = total bits) and the error flag, the
-// total and flag copied back to the host
+// the pixel offsets (offsets[npix] = total bits); the total, the element
+// count and the error flag are read back through a pinned host buffer
 int scan_offsets(const uint32_t *index, size_t npix, const QuantTable &t, uint64_t *offsets, uint64_t &total,
-                 hipStream_t stream)
+                 uint64_t &nelem, hipStream_t stream)
 {
-    const int ntiles = (int)((npix + kTile - 1) / kTile);
+    // one pinned buffer per host thread, kept until the process ends
+    static thread_local ScanSummary *host = nullptr;
+    if (!host && hipHostMalloc((void **)&host, sizeof(ScanSummary)) != hipSuccess)
+        return MRAY_ERR_OOM;
+    const int ntiles = (int)((npix + kTile - 1) / kTile); // npix >= 1
     uint8_t *scratch = nullptr;
-    const size_t sums_b = ((size_t)std::max(ntiles, 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
-    const size_t toff_b = (size_t)std::max(ntiles, 1) * sizeof(uint64_t);
+    const size_t sums_b = ((size_t)ntiles * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t toff_b = ((size_t)ntiles * sizeof(uint64_t) + 255) & ~(size_t)255;
     if (hipMallocAsync((void **)&scratch, sums_b + toff_b + 256, stream) != hipSuccess)
         return MRAY_ERR_OOM;
     uint32_t *sums = reinterpret_cast<uint32_t *>(scratch);
     uint64_t *toff = reinterpret_cast<uint64_t *>(scratch + sums_b);
     int *err = reinterpret_cast<int *>(scratch + sums_b + toff_b);
+    ScanSummary *summary = reinterpret_cast<ScanSummary *>(scratch + sums_b + toff_b + 64);
     hipLaunchKernelGGL(fill_words, dim3(1), dim3(kThreads), 0, stream, reinterpret_cast<uint32_t *>(err), (size_t)1);
-    if (ntiles == 0)
-        hipLaunchKernelGGL(fill_words, dim3(1), dim3(kThreads), 0, stream, reinterpret_cast<uint32_t *>(offsets),
-                           (size_t)2);
-    else
-    {
-        hipLaunchKernelGGL(quant_tile_sums, dim3(ntiles), dim3(kThreads), 0, stream, index, npix, t, sums, err);
-        hipLaunchKernelGGL(quant_tile_offsets, dim3(1), dim3(kThreads), 0, stream, sums, ntiles, toff, offsets,
-                           npix);
-        hipLaunchKernelGGL(quant_tile_apply, dim3(ntiles), dim3(kThreads), 0, stream, index, npix, t, toff,
-                           offsets);
-    }
-    struct
-    {
-        uint64_t total;
-        int err;
-    } h = {0, 0};
+    hipLaunchKernelGGL(quant_tile_sums, dim3(ntiles), dim3(kThreads), 0, stream, index, npix, t, sums, err);
+    hipLaunchKernelGGL(quant_tile_offsets, dim3(1), dim3(kThreads), 0, stream, sums, ntiles, toff, offsets, npix,
+                       index, err, summary);
+    hipLaunchKernelGGL(quant_tile_apply, dim3(ntiles), dim3(kThreads), 0, stream, index, npix, t, toff, offsets);
     const bool copied =
-        hipMemcpyAsync(&h.total, offsets + npix, sizeof(uint64_t), hipMemcpyDeviceToHost, stream) == hipSuccess &&
-        hipMemcpyAsync(&h.err, err, sizeof(int), hipMemcpyDeviceToHost, stream) == hipSuccess;
+        hipMemcpyAsync(host, summary, sizeof(ScanSummary), hipMemcpyDeviceToHost, stream) == hipSuccess;
     const bool freed = hipFreeAsync(scratch, stream) == hipSuccess;
     if (!copied || !freed || hipStreamSynchronize(stream) != hipSuccess)
         return MRAY_ERR_HIP;
-    if (h.err)
+    if (host->err)
         return MRAY_ERR_ARG;
-    total = h.total;
+    total = host->total;
+    nelem = host->nelem;
     return 0;
 }
 
@@ -401,8 +488,8 @@ int launch_quantize(const float *data, const uint32_t *index, int W, int H, int 
     uint64_t *offsets = nullptr;
     if (hipMallocAsync((void **)&offsets, (npix + 1) * sizeof(uint64_t), stream) != hipSuccess)
         return MRAY_ERR_OOM;
-    uint64_t total = 0;
-    rc = scan_offsets(index, npix, t, offsets, total, stream);
+    uint64_t total = 0, nelem = 0;
+    rc = scan_offsets(index, npix, t, offsets, total, nelem, stream);
     const size_t nbytes = (size_t)(total / 8);
     if (stream_bytes)
         *stream_bytes = nbytes;
@@ -411,17 +498,29 @@ int launch_quantize(const float *data, const uint32_t *index, int W, int H, int 
     if (rc == 0 && stream_out && nbytes > 0)
     {
         const size_t nwords = (size_t)((total + 31) / 32) + 1;
-        uint32_t *words = nullptr;
-        if (hipMallocAsync((void **)&words, nwords * sizeof(uint32_t), stream) != hipSuccess)
+        const uint64_t nchunks = (nelem + kChunk - 1) / kChunk;
+        const size_t words_b = (nwords * sizeof(uint32_t) + 255) & ~(size_t)255;
+        uint8_t *buf = nullptr;
+        if (hipMallocAsync((void **)&buf, words_b + nchunks * sizeof(uint32_t), stream) != hipSuccess)
             rc = MRAY_ERR_OOM;
         else
         {
+            uint32_t *words = reinterpret_cast<uint32_t *>(buf);
+            uint32_t *chunk_pixel = reinterpret_cast<uint32_t *>(buf + words_b);
+            const dim3 cgrid((unsigned)((nchunks + kThreads - 1) / kThreads));
             hipLaunchKernelGGL(fill_words, dim3(grid_for(nwords)), dim3(kThreads), 0, stream, words, nwords);
-            hipLaunchKernelGGL(quant_encode, dim3((unsigned)((npix + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                               stream, data, index, npix, t, offsets, words);
-            hipLaunchKernelGGL(copy_stream, dim3(grid_for(nbytes)), dim3(kThreads), 0, stream, words, nbytes,
-                               stream_out);
-            if (hipFreeAsync(words, stream) != hipSuccess)
+            hipLaunchKernelGGL(quant_chunk_map, dim3((unsigned)((npix + kThreads - 1) / kThreads)), dim3(kThreads),
+                               0, stream, index, npix, chunk_pixel);
+            if (((uintptr_t)data & 15u) == 0)
+                hipLaunchKernelGGL(quant_encode<true>, cgrid, dim3(kThreads), 0, stream, data, index, nelem, t,
+                                   chunk_pixel, offsets, words);
+            else
+                hipLaunchKernelGGL(quant_encode<false>, cgrid, dim3(kThreads), 0, stream, data, index, nelem, t,
+                                   chunk_pixel, offsets, words);
+            // byte i of the stream is byte i of the little-endian word buffer
+            if (hipMemcpyAsync(stream_out, words, nbytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                rc = MRAY_ERR_HIP;
+            if (hipFreeAsync(buf, stream) != hipSuccess)
                 rc = MRAY_ERR_HIP;
         }
     }
@@ -441,25 +540,41 @@ int launch_dequantize(const uint8_t *stream_in, size_t stream_bytes, const uint3
     uint64_t *offsets = nullptr;
     if (hipMallocAsync((void **)&offsets, (npix + 1) * sizeof(uint64_t), stream) != hipSuccess)
         return MRAY_ERR_OOM;
-    uint64_t total = 0;
-    rc = scan_offsets(index, npix, t, offsets, total, stream);
+    uint64_t total = 0, nelem = 0;
+    rc = scan_offsets(index, npix, t, offsets, total, nelem, stream);
     // the stream holds every whole byte of the words
     if (rc == 0 && stream_bytes < total / 8)
         rc = MRAY_ERR_ARG;
     if (rc == 0 && total > 0)
     {
         const size_t nwords = (size_t)((total + 31) / 32) + 1;
-        uint32_t *words = nullptr;
-        if (hipMallocAsync((void **)&words, nwords * sizeof(uint32_t), stream) != hipSuccess)
+        const uint64_t nchunks = (nelem + kChunk - 1) / kChunk;
+        const size_t words_b = (nwords * sizeof(uint32_t) + 255) & ~(size_t)255;
+        uint8_t *buf = nullptr;
+        if (hipMallocAsync((void **)&buf, words_b + nchunks * sizeof(uint32_t), stream) != hipSuccess)
             rc = MRAY_ERR_OOM;
         else
         {
+            uint32_t *words = reinterpret_cast<uint32_t *>(buf);
+            uint32_t *chunk_pixel = reinterpret_cast<uint32_t *>(buf + words_b);
+            const dim3 cgrid((unsigned)((nchunks + kThreads - 1) / kThreads));
             const size_t used = std::min<size_t>(stream_bytes, (size_t)((total + 7) / 8));
-            hipLaunchKernelGGL(load_stream, dim3(grid_for(nwords)), dim3(kThreads), 0, stream, stream_in, used,
-                               words, nwords);
-            hipLaunchKernelGGL(quant_decode, dim3((unsigned)((npix + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                               stream, words, index, npix, t, offsets, data);
-            if (hipFreeAsync(words, stream) != hipSuccess)
+            // the words from the one holding byte `used` on are cleared, then
+            // the stream's bytes copied over their start
+            const size_t tail = used / 4;
+            hipLaunchKernelGGL(fill_words, dim3(grid_for(nwords - tail)), dim3(kThreads), 0, stream, words + tail,
+                               nwords - tail);
+            if (used > 0 && hipMemcpyAsync(words, stream_in, used, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                rc = MRAY_ERR_HIP;
+            hipLaunchKernelGGL(quant_chunk_map, dim3((unsigned)((npix + kThreads - 1) / kThreads)), dim3(kThreads),
+                               0, stream, index, npix, chunk_pixel);
+            if (((uintptr_t)data & 15u) == 0)
+                hipLaunchKernelGGL(quant_decode<true>, cgrid, dim3(kThreads), 0, stream, words, index, nelem, t,
+                                   chunk_pixel, offsets, data);
+            else
+                hipLaunchKernelGGL(quant_decode<false>, cgrid, dim3(kThreads), 0, stream, words, index, nelem, t,
+                                   chunk_pixel, offsets, data);
+            if (hipFreeAsync(buf, stream) != hipSuccess)
                 rc = MRAY_ERR_HIP;
         }
     }

@@ -315,12 +315,14 @@ def quantize(data: torch.Tensor, index: torch.Tensor, W: int, H: int, prediction
     bits[l] bits per moment, least significant bit first (mray_hip_quantize)."""
     t = _table(bits)
     n = C.c_size_t(0)
-    lib = load_library()
-    _check(lib.mray_hip_quantize(_ptr(data), _ptr(index), W, H, int(bool(prediction_coded)), t, len(t), None, 0,
-                                 C.byref(n), _stream(stream)), "mray_hip_quantize")
-    out = torch.empty(max(n.value, 1), dtype=torch.uint8, device=index.device)
-    _check(lib.mray_hip_quantize(_ptr(data), _ptr(index), W, H, int(bool(prediction_coded)), t, len(t), _ptr(out),
-                                 out.numel(), C.byref(n), _stream(stream)), "mray_hip_quantize")
+    s = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(s):
+        total = int(index[-1].to(torch.int64).item()) & 0xFFFFFFFF
+    # every word is at most max(bits) wide: one call with that capacity
+    cap = (total * max(t) + 7) // 8
+    out = torch.empty(max(cap, 1), dtype=torch.uint8, device=index.device)
+    _check(load_library().mray_hip_quantize(_ptr(data), _ptr(index), W, H, int(bool(prediction_coded)), t, len(t),
+                                            _ptr(out), out.numel(), C.byref(n), _stream(stream)), "mray_hip_quantize")
     return out[:n.value]
 
 
