@@ -1,10 +1,15 @@
-// Moment-image files, uncompressed and unquantised (src/moment_image_io.cpp).
+// Moment-image files (src/moment_image_io.cpp): uncompressed, with the moment
+// data as floats or, for compact images, as the quantized bit stream of
+// moment_quantization (packed and unpacked on the GPU by mray_hip_quantize /
+// mray_hip_dequantize). LZ4 compression and entropy coding are not supported.
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iterator>
 
 #include "../include/color_image.h"
 #include "../include/moment_image_io.h"
+#include "device.hpp"
 
 namespace
 {
@@ -17,13 +22,62 @@ void get_bytes(std::fstream &in, void *p, size_t n, const std::string &file)
     if ((size_t)in.gcount() != n)
         throw invalid_data("truncated moment image " + file);
 }
+
+// the word widths of the quantized stream: for prediction-coded images the
+// file's table when it carries one, else get_prediction_coding_quantization_table's default (16 bits for
+// moment 0, then quantization_bits) for prediction-coded images and
+// quantization_bits for every moment otherwise
+// (src/moment_quantization.cpp:186-203, 285-360)
+vector<Byte> quant_widths(const MomentImageHost &mi, int bits, const vector<Byte> &file_table)
+{
+    vector<Byte> w(std::max(mi.num_moments, 1), static_cast<Byte>(bits));
+    if (mi.prediction_code && !file_table.empty())
+        std::copy(file_table.begin(), file_table.begin() + std::min(file_table.size(), w.size()), w.begin());
+    else if (mi.prediction_code)
+        w[0] = 16;
+    return w;
+}
+
+vector<Byte> quantize_on_device(const MomentImageHost &mi, const vector<Byte> &widths)
+{
+    using namespace mray_host;
+    ensure_init();
+    DeviceBuffer d_data(mi.data.data(), std::max<size_t>(mi.data.size(), 1) * sizeof(float));
+    DeviceBuffer d_index(mi.index.data(), mi.index.size() * sizeof(uint32_t));
+    const size_t cap = mi.data.size() * sizeof(float) + 8; // words of at most 32 bits
+    DeviceBuffer d_out(cap);
+    size_t n = 0;
+    check(mray_hip_quantize(d_data.as<float>(), d_index.as<uint32_t>(), mi.width, mi.height, mi.prediction_code,
+                            widths.data(), (int)widths.size(), d_out.as<uint8_t>(), cap, &n, nullptr),
+          "mray_hip_quantize");
+    vector<Byte> out(n);
+    d_out.download(out.data(), n);
+    return out;
+}
+
+void dequantize_on_device(MomentImageHost &mi, const vector<Byte> &widths, const vector<Byte> &bytes)
+{
+    using namespace mray_host;
+    ensure_init();
+    DeviceBuffer d_stream(bytes.data(), std::max<size_t>(bytes.size(), 1));
+    DeviceBuffer d_index(mi.index.data(), mi.index.size() * sizeof(uint32_t));
+    DeviceBuffer d_data(std::max<size_t>(mi.data.size(), 1) * sizeof(float));
+    check(mray_hip_dequantize(d_stream.as<uint8_t>(), bytes.size(), d_index.as<uint32_t>(), mi.width, mi.height,
+                              mi.prediction_code, widths.data(), (int)widths.size(), d_data.as<float>(), nullptr),
+          "mray_hip_dequantize");
+    d_data.download(mi.data.data(), mi.data.size() * sizeof(float));
+}
 } // namespace
 
 void write_moment_image(const MomentImageHost &mi, const std::string filename, bool compress, Byte quantizationBits,
                         bool entropy_coding)
 {
-    if (compress || quantizationBits != 32 || entropy_coding)
-        throw invalid_args("moment-image compression, quantisation and entropy coding are not supported here");
+    if (compress || entropy_coding)
+        throw invalid_args("moment-image compression and entropy coding are not supported here");
+    if (quantizationBits < 1 || quantizationBits > 32)
+        throw invalid_args("quantization_bits must be 1..32");
+    if (quantizationBits < 32 && !mi.is_compact)
+        throw invalid_args("quantised dense moment images (LZ4-compressed in the reference) are not supported here");
     if (mi.width > 65535 || mi.height > 65535 || mi.num_moments > 255)
         throw invalid_args("moment image too large for the file header");
     std::fstream out(filename, std::fstream::binary | std::fstream::out);
@@ -40,7 +94,7 @@ void write_moment_image(const MomentImageHost &mi, const std::string filename, b
     flags |= mi.has_error_bounds() << 5;
     flags |= mi.has_coding_params() << 7;
     put(out, flags);
-    put(out, static_cast<uint8_t>(32));
+    put(out, static_cast<uint8_t>(quantizationBits)); // no entropy coding, no table
     put(out, mi.domain.min);
     put(out, mi.domain.max);
     if (mi.is_compact)
@@ -61,7 +115,14 @@ void write_moment_image(const MomentImageHost &mi, const std::string filename, b
         put_bytes(out, mi.error_bounds.data(), mi.error_bounds.size() * sizeof(ErrorBound));
     if (mi.has_coding_params())
         put_bytes(out, mi.coding_params.data(), mi.coding_params.size() * sizeof(CodingParamType));
-    put_bytes(out, mi.data.data(), mi.data.size() * sizeof(float));
+    if (quantizationBits < 32)
+    {
+        // moment_quantization::quantize_prediction_coding / quantize (src/moment_image_io.cpp:397-408)
+        const vector<Byte> stream = quantize_on_device(mi, quant_widths(mi, quantizationBits, {}));
+        put_bytes(out, stream.data(), stream.size());
+    }
+    else
+        put_bytes(out, mi.data.data(), mi.data.size() * sizeof(float));
     if (!out)
         throw invalid_data("write error on " + filename);
 }
@@ -84,8 +145,11 @@ MomentImageHost load_moment_image(const std::string filename)
     get(in, mi.domain.max);
     if (!in)
         throw invalid_data("truncated moment-image header in " + filename);
-    if ((flags & 1) || (qbits & 0x3F) != 32 || (qbits & 0xC0))
-        throw invalid_args("compressed, quantised or entropy-coded moment images are not supported here");
+    const int bits = qbits & 0x3F;
+    if ((flags & 1) || (qbits & 0x40))
+        throw invalid_args("compressed or entropy-coded moment images are not supported here");
+    if (bits < 1 || bits > 32 || (bits < 32 && !(flags & 2)))
+        throw invalid_args("quantised moment images are supported compact, with 1..32 bits");
     mi.width = w;
     mi.height = h;
     mi.num_moments = num_moments;
@@ -93,6 +157,14 @@ MomentImageHost load_moment_image(const std::string filename)
     mi.is_compact = flags & 2;
     mi.prediction_code = flags & 4;
     const size_t npix = (size_t)w * h;
+    // the quantization table of a prediction-coded image follows the header
+    // (read_quantization_table, src/moment_image_io.cpp:110-115)
+    vector<Byte> file_table;
+    if (qbits & 0x80)
+    {
+        file_table.resize(num_moments);
+        get_bytes(in, file_table.data(), file_table.size(), filename);
+    }
     if (mi.is_compact)
     {
         vector<Byte> counts(npix);
@@ -123,7 +195,14 @@ MomentImageHost load_moment_image(const std::string filename)
     }
     const size_t n = mi.is_compact ? mi.index[npix] : npix * num_moments;
     mi.data.resize(n);
-    get_bytes(in, mi.data.data(), n * sizeof(float), filename);
+    if (bits < 32)
+    {
+        // the rest of the file is the stream (read_moment_bytes, src/moment_image_io.cpp:91-107)
+        const vector<Byte> stream((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        dequantize_on_device(mi, quant_widths(mi, bits, file_table), stream);
+    }
+    else
+        get_bytes(in, mi.data.data(), n * sizeof(float), filename);
     return mi;
 }
 

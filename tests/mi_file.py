@@ -1,5 +1,6 @@
-"""Reader for moment-image files (the uncompressed, unquantised variant of
-src/moment_image_io.cpp) used by the CLI tests."""
+"""Reader for moment-image files (the uncompressed variants of
+src/moment_image_io.cpp: float moments, or the quantized stream of a compact
+image without a table) used by the CLI tests."""
 import numpy as np
 
 
@@ -9,11 +10,12 @@ def read_moment_image(path):
     num_moments, warp, flags, qbits = b[4], b[5], b[6], b[7]
     dmin = np.frombuffer(b, np.float32, 3, 8)
     dmax = np.frombuffer(b, np.float32, 3, 20)
-    assert not (flags & 1) and (qbits & 0x3F) == 32
+    assert not (flags & 1) and not (qbits & 0xC0)
     off = 32
     npix = int(w) * int(h)
     out = dict(width=int(w), height=int(h), num_moments=int(num_moments), coding_warp=int(warp),
-               compact=bool(flags & 2), prediction_code=bool(flags & 4), domain=(dmin, dmax))
+               compact=bool(flags & 2), prediction_code=bool(flags & 4), domain=(dmin, dmax),
+               quantization_bits=int(qbits & 0x3F))
     if out["compact"]:
         counts = np.frombuffer(b, np.uint8, npix, off).astype(np.uint32)
         off += npix
@@ -29,6 +31,9 @@ def read_moment_image(path):
         out["coding_params"] = np.frombuffer(b, np.uint16, 2 * (int(num_moments) - 1), off).copy()
         off += 4 * (int(num_moments) - 1)
     n = int(out["index"][-1]) if out["compact"] else npix * int(num_moments)
+    if out["quantization_bits"] < 32:
+        out["stream"] = np.frombuffer(b, np.uint8, len(b) - off, off).copy()
+        return out
     out["data"] = np.frombuffer(b, np.float32, n, off).copy()
     assert off + 4 * n == len(b)
     return out

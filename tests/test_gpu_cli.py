@@ -132,6 +132,53 @@ def test_cli_round_trip(tmp_path, case):
     assert (tmp_path / "out.ppm").read_bytes() == ppm_bytes(ref)
 
 
+def test_cli_quantized_file(tmp_path):
+    """quantization_bits=10 on a compacted, prediction-coded image: the file
+    carries moment_quantization's stream of the coded moments (16 bits for
+    moment 0, then 10; src/moment_quantization.cpp:244-283) bit for bit, and
+    reconstructing it renders the oracle's image of the dequantized moments."""
+    from oracle_ffi import quant_bits
+
+    O = Oracle()
+    vol, dmin, dmax = synth.marschner_lobb(48), (-1, -1, -1), (1, 1, 1)
+    cam = synth.ortho_camera()
+    W, H, m, step = 40, 32, 24, 2.0 / 96
+    vol.tofile(tmp_path / "vol.raw")
+    camera_file(tmp_path / "cam.txt", cam)
+    rgba8 = synth.tf_ramps(0)
+    rgba8.tofile(tmp_path / "tf.bin")
+    common = (f"width={W},height={H},num_moments={m},camera={tmp_path / 'cam.txt'},dims=48:48:48,"
+              f"domain=-1:-1:-1:1:1:1,step_size={step!r},transform_to_bounds=1,transform_bound_eps=0.01,"
+              f"bias=1e-4,form=plain,compact_image=1,prediction_coding=1,error_threshold=0.001")
+    run("--generate", tmp_path / "coded.bin", tmp_path / "vol.raw", common)
+    run("--generate", tmp_path / "q10.bin", tmp_path / "vol.raw", common + ",quantization_bits=10")
+    coded = read_moment_image(tmp_path / "coded.bin")
+    q10 = read_moment_image(tmp_path / "q10.bin")
+    assert q10["quantization_bits"] == 10 and q10["compact"] and q10["prediction_code"]
+    assert np.array_equal(q10["index"], coded["index"])
+    assert np.array_equal(q10["coding_params"], coded["coding_params"])
+    bits = quant_bits(256, 1, 10)
+    stream = O.quantize(W, H, coded["data"], coded["index"], 1, bits)
+    print(f"[cli] quantized stream {len(q10['stream'])} bytes for {int(coded['index'][-1])} moments", flush=True)
+    assert np.array_equal(q10["stream"], stream)
+
+    factor = 4.0
+    rp = RecParams(step_size=step, bias=1e-4, early_ray_termination=0.99, medium_emission=1.0)
+    run("--reconstruct", tmp_path / "q10.bin", tmp_path / "out.ppm",
+        f"tf={tmp_path / 'tf.bin'},camera={tmp_path / 'cam.txt'},step_size={step!r},bias=1e-4,"
+        f"early_ray_termination=0.99,medium_emission=1.0,preintegration_step_size_factor={factor},form=plain,"
+        f"float_rgba={tmp_path / 'rgba.f32'}")
+    rgba = np.fromfile(tmp_path / "rgba.f32", np.float32).reshape(H, W, 4)
+    deq = O.dequantize(W, H, coded["index"], 1, bits, stream)
+    table = tfmod.table_from_rgba8(rgba8, step * factor)
+    params_f = (coded["coding_params"].astype(np.float32) / np.float32(65535.0)).astype(np.float32)
+    ref, _ = O.reconstruct(cam, dmin, dmax, W, H, m, deq, table, rp, index=coded["index"], bounds=coded["bounds"],
+                           prediction_coded=1, coding_warp=0, coding_params=params_f)
+    same = (rgba.view(np.uint32) == ref.view(np.uint32)).all(axis=-1)
+    print(f"[cli] quantized: bit-identical rays {same.sum()}/{W * H}", flush=True)
+    assert same.all()
+
+
 @pytest.mark.parametrize("case", ["ortho_f32", "persp_u8"])
 def test_cli_reference(tmp_path, case):
     """mray --reference (src/main.cpp:134-168, 740-752): the reference
