@@ -233,6 +233,20 @@ int mray_hip_quantize(const float *d_data, const uint32_t *d_index, int W, int H
                       const uint8_t *bits, int n_bits, uint8_t *d_stream, size_t capacity, size_t *stream_bytes,
                       mray_stream stream);
 
+/* The quantization table of a prediction-coded compact image
+ * (create_quantization_table, src/measure_coding_error.cpp:374-437, as
+ * ::generate_moments runs it on sample_moment_image of the image when
+ * quantization_bits < 32, src/main.cpp:318-327): table[0] = 16, table[1] =
+ * start_bits, then for each moment the fewest bits, down to 2 and at most the
+ * previous moment's, whose mean coding error stays below moment 1's at
+ * start_bits. The coding error of a pixel is the relative distance of its
+ * decoded trigonometric moments from the unquantized decode. Each measurement
+ * decodes every pixel on the GPU; the mean is summed on the host in pixel
+ * order, as the reference's CPU path sums it. table (host) receives m bytes.
+ * Waits for the stream. Same table as the reference. */
+int mray_hip_quantization_table(const float *d_data, const uint32_t *d_index, int W, int H, int m, int coding_warp,
+                                const uint16_t *d_coding_params, int start_bits, uint8_t *table, mray_stream stream);
+
 /* Unpacks a quantized stream into a compact image
  * (moment_quantization::dequantize_prediction_coding / dequantize,
  * src/moment_quantization.cpp:285-298, 331-360, as read_moment_image calls

@@ -402,6 +402,31 @@ int mray_hip_quantize(const float *d_data, const uint32_t *d_index, int W, int H
     return check_status(rc, "mray_hip_quantize");
 }
 
+int mray_hip_quantization_table(const float *d_data, const uint32_t *d_index, int W, int H, int m, int coding_warp,
+                                const uint16_t *d_coding_params, int start_bits, uint8_t *table, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_quantization_table: call mray_hip_init first");
+    if (!d_data || !d_index || !table)
+        return fail(MRAY_ERR_ARG, "mray_hip_quantization_table: null argument");
+    if (bad_image(W, H, m) || start_bits < 2 || start_bits > 31)
+        return fail(MRAY_ERR_ARG, "mray_hip_quantization_table: bad image size %dx%d, moment count %d or start bits %d",
+                    W, H, m, start_bits);
+    if (coding_warp != MRAY_CODING_WARP_DEFAULT_TRANSFORMED && coding_warp != MRAY_CODING_WARP_NONE)
+        return fail(MRAY_ERR_ARG, "mray_hip_quantization_table: unknown coding warp %d", coding_warp);
+    if (coding_warp == MRAY_CODING_WARP_DEFAULT_TRANSFORMED && m > 1 && !d_coding_params)
+        return fail(MRAY_ERR_ARG, "mray_hip_quantization_table: the default warp needs d_coding_params");
+    int rc = mray::launch_quantization_table(d_data, d_index, W, H, m, coding_warp, d_coding_params, start_bits, table,
+                                             (hipStream_t)stream);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_quantization_table: %d moments do not fit the LDS", m);
+    if (rc == MRAY_ERR_ARG)
+        return fail(rc, "mray_hip_quantization_table: a pixel has more than %d moments", m);
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_quantization_table: scratch allocation failed");
+    return check_status(rc, "mray_hip_quantization_table");
+}
+
 int mray_hip_dequantize(const uint8_t *d_stream, size_t stream_bytes, const uint32_t *d_index, int W, int H,
                         int prediction_coded, const uint8_t *bits, int n_bits, float *d_data, mray_stream stream)
 {

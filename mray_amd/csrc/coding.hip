@@ -24,6 +24,9 @@
 // columns; the workgroup size adapts so that the columns fit the LDS.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <vector>
+
 #include <algorithm>
 
 #include "launch.hpp"
@@ -499,6 +502,101 @@ __global__ __launch_bounds__(64) void prepare_coded_kernel(const float *__restri
     lagrange_from_levinson(nm, out + off, E, P, R);
 }
 
+// The coding error the quantization-curve search measures
+// (compute_reference / compute_coding_error with apply_quantization of one
+// moment, src/measure_coding_error.cpp:41-66, 92-182): the pixel's coded
+// moments, moment q_index floored to q_bits bits (q_index < 0: none), are
+// decoded as in prepare_coded_kernel and converted to trigonometric moments
+// (exponentialToTrigonometricMoments, src/mese/MESE_dynamic.h:78-97).
+// q_index < 0 writes those moments to ref; else err[pixel] = sqrt(sum_l
+// (ref_l - t_l)^2) / ref_0 (0 for empty pixels).
+__global__ __launch_bounds__(64) void coding_error_kernel(const float *__restrict__ mom,
+                                                          const uint32_t *__restrict__ index, int npix, int m,
+                                                          int warp, const uint16_t *__restrict__ qparams,
+                                                          int q_index, float q_scale, float *__restrict__ ref,
+                                                          float *__restrict__ err)
+{
+    extern __shared__ cf lds_cf[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int pix = blockIdx.x * nt + tid;
+    if (pix >= npix)
+        return;
+    const int nm = min((int)(index[pix + 1] - index[pix]), m);
+    const size_t off = index[pix];
+    if (nm <= 0)
+    {
+        if (q_index >= 0)
+            err[pix] = 0.0f;
+        return;
+    }
+    const float *in = mom + off;
+    // apply_quantization: floor(v * 2^b) / 2^b, exact with q_scale = 2^b
+    auto value = [&](int l) { return (l == q_index) ? floorf(in[l] * q_scale) / q_scale : in[l]; };
+    const Col E{lds_cf + tid, nt}, P{lds_cf + nt * m + tid, nt}, T{lds_cf + 2 * nt * m + tid, nt};
+
+    const float ph = 3.14159265f * value(0) - 1.57079633f;
+    const cf code0 = crmul(2.0f * 0.0795774715f, cmk(host_cosf(ph), host_sinf(ph)));
+    const cf axis = cmk(-code0.y, code0.x);
+    const cf factor_axis = cdivr(axis, sqrtf(cabs2(axis)));
+    E[0] = code0;
+    P[0] = cmk(1.0f / E[0].x, 0.0f);
+    const float eps = 1e-6f;
+    bool invalid = false;
+    for (int l = 1; l < nm; ++l)
+    {
+        cf u = crmul(coding_inv_warp(l, value(l), warp, qparams), factor_axis);
+        float a2 = cabs2(u);
+        if (invalid || a2 >= (1.0f - eps))
+        {
+            u = cmk(0.0f, 0.0f);
+            a2 = 0;
+            invalid = true;
+        }
+        cf g = cmk(0.0f, 0.0f);
+        for (int k = 1; k < l; ++k)
+            g = cadd(g, cmul(P[k], E[l - k]));
+        E[l] = cmul(cdiv(cmk(1.0f, 0.0f), P[0]), csub(u, g));
+        levinson_update(l, P, u, 1.0f / (1.0f - a2));
+    }
+
+    // exponentialToTrigonometricMoments; atan2 of the float parts in double
+    const cf gamma_0 = cdivr(E[0], 2.f);
+    const cf t0 = cdivr(gamma_0, 0.0795774715f);
+    const float zph = (float)atan2((double)t0.y, (double)t0.x);
+    const float tm0 = (zph + 1.57079633f) / 3.14159265f;
+    T[0] = cmk(tm0, 0.0f);
+    const cf den = cmul(cmk(0.0f, (float)(2.0f * M_PI)), gamma_0);
+    float sum = 0.0f;
+    const float *r = ref + off;
+    if (q_index >= 0)
+    {
+        const float d = r[0] - tm0;
+        sum = d * d;
+    }
+    else
+        ref[off] = tm0;
+    for (int l = 1; l < nm; ++l)
+    {
+        cf t = cmk(0.0f, 0.0f);
+        for (int j = 1; j < l; ++j)
+        {
+            const cf c = cdiv(cmk((float)(l - j), 0.0f), crmul((float)l, gamma_0));
+            t = cadd(t, cmul(cmul(c, E[j]), T[l - j]));
+        }
+        t = csub(cdiv(E[l], den), t);
+        T[l] = t;
+        if (q_index >= 0)
+        {
+            const float d = r[l] - t.x;
+            sum = sum + d * d;
+        }
+        else
+            ref[off + l] = t.x;
+    }
+    if (q_index >= 0)
+        err[pix] = sqrtf(sum) / r[0];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -594,6 +692,86 @@ int launch_prepare_coded(const float *moments, const uint32_t *index, int W, int
     hipLaunchKernelGGL(prepare_coded_kernel, dim3((npix + nt - 1) / nt), dim3(nt), (size_t)3 * m * nt * sizeof(cf),
                        stream, moments, index, npix, m, warp, qparams, lagrange);
     return 0;
+}
+
+// create_quantization_table (src/measure_coding_error.cpp:374-437): moment 0
+// at 16 bits, moment 1 at start_bits, then per moment the fewest bits (down to
+// 2, from the previous moment's) whose mean coding error stays below moment
+// 1's. Each measurement decodes the image on the GPU; the mean is get_mean's
+// float sum over the non-empty pixels in row order, on the host.
+int launch_quantization_table(const float *data, const uint32_t *index, int W, int H, int m, int warp,
+                              const uint16_t *qparams, int start_bits, uint8_t *table, hipStream_t stream)
+{
+    const int npix = W * H;
+    const int nt = column_block(3, m);
+    if (!nt)
+        return MRAY_ERR_UNSUPPORTED;
+    std::vector<uint32_t> h_index((size_t)npix + 1);
+    if (hipMemcpyAsync(h_index.data(), index, h_index.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return MRAY_ERR_HIP;
+    size_t nonempty = 0;
+    for (int p = 0; p < npix; ++p)
+    {
+        if (h_index[p + 1] - h_index[p] > (uint32_t)m)
+            return MRAY_ERR_ARG;
+        nonempty += h_index[p + 1] > h_index[p];
+    }
+    float *ref = nullptr, *err = nullptr;
+    if (hipMallocAsync((void **)&ref, ((size_t)h_index[npix] + 1) * sizeof(float), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    if (hipMallocAsync((void **)&err, (size_t)npix * sizeof(float), stream) != hipSuccess)
+    {
+        (void)hipFreeAsync(ref, stream);
+        return MRAY_ERR_OOM;
+    }
+    const dim3 grid((npix + nt - 1) / nt);
+    const size_t lds = (size_t)3 * m * nt * sizeof(cf);
+    hipLaunchKernelGGL(coding_error_kernel, grid, dim3(nt), lds, stream, data, index, npix, m, warp, qparams, -1,
+                       1.0f, ref, err);
+    std::vector<float> h_err(npix);
+    int rc = 0;
+    auto mean_error = [&](int idx, int b, float &mean) {
+        hipLaunchKernelGGL(coding_error_kernel, grid, dim3(nt), lds, stream, data, index, npix, m, warp, qparams,
+                           idx, std::ldexp(1.0f, b), ref, err);
+        if (hipMemcpyAsync(h_err.data(), err, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess)
+            return false;
+        mean = 0.0f;
+        for (int p = 0; p < npix; ++p)
+            if (h_index[p + 1] > h_index[p])
+                mean += h_err[p];
+        mean /= (float)nonempty;
+        return true;
+    };
+    std::fill(table, table + m, (uint8_t)0);
+    table[0] = 16;
+    if (m > 1)
+        table[1] = (uint8_t)start_bits;
+    float threshold = 0.0f;
+    if (m > 1 && !mean_error(1, start_bits, threshold))
+        rc = MRAY_ERR_HIP;
+    for (int idx = 2; rc == 0 && idx < m; ++idx)
+    {
+        table[idx] = table[idx - 1];
+        for (int b = table[idx - 1]; b >= 2; --b)
+        {
+            float e = 0.0f;
+            if (!mean_error(idx, b, e))
+            {
+                rc = MRAY_ERR_HIP;
+                break;
+            }
+            if (e >= threshold)
+                break;
+            table[idx] = (uint8_t)b;
+        }
+    }
+    if (hipFreeAsync(ref, stream) != hipSuccess || hipFreeAsync(err, stream) != hipSuccess)
+        rc = rc ? rc : MRAY_ERR_HIP;
+    return rc;
 }
 
 } // namespace mray

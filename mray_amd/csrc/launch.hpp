@@ -46,6 +46,8 @@ int launch_prediction_encode(float *data, const uint32_t *index, int W, int H, i
                              uint16_t *qparams, hipStream_t stream);
 int launch_prepare_coded(const float *moments, const uint32_t *index, int W, int H, int m, int warp,
                          const uint16_t *qparams, float *lagrange, hipStream_t stream);
+int launch_quantization_table(const float *data, const uint32_t *index, int W, int H, int m, int warp,
+                              const uint16_t *qparams, int start_bits, uint8_t *table, hipStream_t stream);
 int launch_quantize(const float *data, const uint32_t *index, int W, int H, int prediction_coded,
                     const uint8_t *bits, int n_bits, uint8_t *stream_out, size_t capacity, size_t *stream_bytes,
                     hipStream_t stream);

@@ -44,6 +44,8 @@ _SIGNATURES = {
     "mray_hip_prepare_coded": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
     "mray_hip_quantize": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, _vp, C.c_size_t,
                            C.POINTER(C.c_size_t), _vp], C.c_int),
+    "mray_hip_quantization_table": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                                     C.POINTER(C.c_uint8), _vp], C.c_int),
     "mray_hip_dequantize": ([_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, _vp,
                              _vp], C.c_int),
     "mray_hip_particles_to_grid": ([_vp, _vp, _vp, _vp, C.c_uint32, C.c_float, _f3, _f3, C.c_int, C.c_int, C.c_int,
@@ -337,6 +339,17 @@ def dequantize(stream_bytes: torch.Tensor, index: torch.Tensor, W: int, H: int, 
                                               int(bool(prediction_coded)), t, len(t), _ptr(out), _stream(stream)),
            "mray_hip_dequantize")
     return out[:total]
+
+
+def curve_table(data: torch.Tensor, index: torch.Tensor, W: int, H: int, m: int, coding_params, start_bits: int,
+                coding_warp: int = 0, stream=None):
+    """The quantization table the reference's curve search picks for a
+    prediction-coded compact image (mray_hip_quantization_table); m widths."""
+    t = (C.c_uint8 * m)()
+    _check(load_library().mray_hip_quantization_table(_ptr(data), _ptr(index), W, H, m, coding_warp,
+                                                      _ptr(coding_params), start_bits, t, _stream(stream)),
+           "mray_hip_quantization_table")
+    return list(t)
 
 
 def compact_image(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None, stream=None):

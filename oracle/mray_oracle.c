@@ -2011,3 +2011,93 @@ void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_co
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* quantization curve (src/measure_coding_error.cpp)                        */
+/* ------------------------------------------------------------------------ */
+
+/* compute_reference / compute_coding_error with apply_quantization of one
+ * moment (src/measure_coding_error.cpp:41-66, 92-182) */
+void orc_coding_error(int W, int H, const float *data, const uint32_t *index, int warp, const float *params,
+                      int q_index, int q_bits, float *ref, float *err)
+{
+    for (size_t pix = 0; pix < (size_t)W * H; ++pix)
+    {
+        const uint32_t start = index[pix];
+        const int nm = (int)(index[pix + 1] - start);
+        if (err)
+            err[pix] = 0.0f;
+        if (nm <= 0)
+            continue;
+        float in[MAXM];
+        cf code[MAXM], E[MAXM], P[MAXM], tmp[MAXM], T[MAXM];
+        memcpy(in, data + start, nm * sizeof(float));
+        if (q_index >= 0 && nm > q_index)
+        {
+            const float p = (float)pow(2.0, (double)q_bits); /* std::pow(2.0f, b): exact */
+            in[q_index] = floorf(in[q_index] * p) / p;
+        }
+        orc_transform_dequantization_real(nm, in, (float *)code, warp, params);
+        decode_f(nm, code, E, P, tmp);
+        exp_to_trig_f(nm, T, E);
+        if (q_index < 0)
+        {
+            for (int l = 0; l < nm; ++l)
+                ref[start + l] = T[l].x;
+            continue;
+        }
+        float sum = 0.0f;
+        for (int l = 0; l < nm; ++l)
+        {
+            const float d = ref[start + l] - T[l].x;
+            sum += d * d;
+        }
+        err[pix] = sqrtf(sum) / ref[start];
+    }
+}
+
+/* create_quantization_table with get_mean's float sum in pixel order
+ * (src/measure_coding_error.cpp:184-192, 360-437) */
+void orc_quantization_table(int W, int H, int m, const float *data, const uint32_t *index, int warp,
+                            const float *params, int start_bits, uint8_t *table)
+{
+    const size_t npix = (size_t)W * H;
+    float *ref = (float *)malloc((index[npix] + 1) * sizeof(float));
+    float *err = (float *)malloc(npix * sizeof(float));
+    size_t n = 0;
+    for (size_t p = 0; p < npix; ++p)
+        n += index[p + 1] > index[p];
+    orc_coding_error(W, H, data, index, warp, params, -1, 0, ref, NULL);
+#define MEAN_ERR(idx, b, out)                                                                                   \
+    do                                                                                                          \
+    {                                                                                                           \
+        orc_coding_error(W, H, data, index, warp, params, (idx), (b), ref, err);                                \
+        float mean_ = 0.0f;                                                                                     \
+        for (size_t p = 0; p < npix; ++p)                                                                       \
+            if (index[p + 1] > index[p])                                                                        \
+                mean_ += err[p];                                                                                \
+        out = mean_ / (float)n;                                                                                 \
+    } while (0)
+    for (int i = 0; i < m; ++i)
+        table[i] = 0;
+    table[0] = 16;
+    if (m > 1)
+        table[1] = (uint8_t)start_bits;
+    float threshold = 0.0f;
+    MEAN_ERR(1, start_bits, threshold);
+    for (int idx = 2; idx < m; ++idx)
+    {
+        table[idx] = table[idx - 1];
+        for (int b = table[idx - 1]; b >= 2; --b)
+        {
+            float e;
+            MEAN_ERR(idx, b, e);
+            if (e >= threshold)
+                break;
+            table[idx] = (uint8_t)b;
+        }
+    }
+#undef MEAN_ERR
+    free(ref);
+    free(err);
+}

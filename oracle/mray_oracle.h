@@ -155,6 +155,20 @@ size_t orc_quantize_image(int W, int H, const float *data, const uint32_t *index
 void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_coded, const uint8_t *bits,
                           const uint8_t *in, size_t n_in, float *data);
 
+/* ---- quantization curve (src/measure_coding_error.cpp:41-192, 374-437) ----
+ * Coding error of a prediction-coded compact image: every pixel decoded
+ * (transform_dequantization_real, decode, exponentialToTrigonometricMoments)
+ * with moment q_index floored to q_bits bits; err[pixel] = sqrt(sum_l (ref_l
+ * - t_l)^2) / ref_0 against the unquantized decode (0 for empty pixels).
+ * q_index < 0 writes the unquantized trigonometric moments to ref instead.
+ * params: 2(m-1) dequantized coding parameters. */
+void orc_coding_error(int W, int H, const float *data, const uint32_t *index, int warp, const float *params,
+                      int q_index, int q_bits, float *ref, float *err);
+/* create_quantization_table: table[0] = 16, table[1] = start_bits, then per
+ * moment the fewest bits (down to 2) whose mean error stays below moment 1's */
+void orc_quantization_table(int W, int H, int m, const float *data, const uint32_t *index, int warp,
+                            const float *params, int start_bits, uint8_t *table);
+
 #ifdef __cplusplus
 }
 #endif

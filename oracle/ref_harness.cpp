@@ -16,6 +16,7 @@
 #include "moment_preparation.h"
 #include "coding_transform.h"
 #include "moment_quantization.h"
+#include "measure_coding_error.h"
 #include "mese/MESE_4_10.h"
 
 #include <cstdio>
@@ -395,5 +396,24 @@ void ref_dequantize(int W, int H, const uint32_t *index, int prediction_coded, c
     else
         moment_quantization::dequantize(mi, bits[0], qs);
     std::memcpy(data, mi.data.data(), mi.data.size() * sizeof(float));
+}
+
+// create_quantization_table (src/measure_coding_error.cpp:374-437) on a
+// prediction-coded compact image; the table's first m entries
+void ref_quantization_table(int W, int H, int m, const float *data, const uint32_t *index, int warp,
+                            const uint16_t *coding_params, int start_bits, uint8_t *table)
+{
+    auto mi = make_compact(W, H, data, index, 1);
+    mi.num_moments = m;
+    mi.coding_warp = warp;
+    mi.coding_params.resize(m - 1);
+    for (int i = 0; i < m - 1; ++i)
+    {
+        mi.coding_params[i].param1 = coding_params[2 * i];
+        mi.coding_params[i].param2 = coding_params[2 * i + 1];
+    }
+    create_quantization_table(mi, start_bits);
+    auto t = moment_quantization::get_prediction_coding_quantization_table();
+    std::memcpy(table, t.data(), m);
 }
 }

@@ -315,11 +315,44 @@ def quant_vectors(R: Reference) -> dict:
     return out
 
 
+CURVE_CASES = {"ml": ("ml", 32, 32, 24, 1e-3), "mix": ("mix", 40, 40, 30, 1e-2), "mix64": ("mix", 64, 64, 40, 1e-3)}
+
+
+def curve_vectors(R: Reference) -> dict:
+    """create_quantization_table (the quantization-curve search) of coded
+    compact images the reference generated, at start widths 8, 10 and 14."""
+    import ctypes as C
+
+    from oracle_ffi import ptr, _f, _u8, _u16, _u32
+
+    out = {}
+    for name, (kind, W, H, m, thr) in CURVE_CASES.items():
+        if kind == "ml":
+            vol, dmin, dmax, cam = synth.marschner_lobb(40), (-1, -1, -1), (1, 1, 1), synth.ortho_camera()
+        else:
+            vol, dmin, dmax = synth.mixing_layer(40, width=0.05), (0, 0, 0), (1, 1, 1)
+            cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=1.0)
+        gp = GenParams(step_size=1.0 / 40, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=1,
+                       prediction_coding=1, error_threshold=thr, bias=1e-4)
+        data, _, index, cp = R.generate(vol, dmin, dmax, cam, W, H, m, gp)
+        data = np.ascontiguousarray(data, np.float32)
+        index = np.ascontiguousarray(index, np.uint32)
+        cp = np.ascontiguousarray(cp, np.uint16)
+        out.update({f"{name}_data": data, f"{name}_index": index, f"{name}_params": cp,
+                    f"{name}_shape": np.array([W, H, m], np.int32)})
+        for sb in (8, 10, 14):
+            t = np.zeros(m, np.uint8)
+            R.lib.ref_quantization_table(W, H, m, ptr(data, _f), ptr(index, _u32), 0, ptr(cp, _u16), sb, ptr(t, _u8))
+            out[f"{name}_table{sb}"] = t
+    return out
+
+
 def main():
     # python make_golden.py [name ...]: only the named fixtures (default all)
     R = Reference()
     makers = {"mese": mese_vectors, "coding": coding_vectors, "images": image_vectors, "sph": sph_vectors,
-              "reference": reference_dvr_vectors, "quant": quant_vectors}
+              "reference": reference_dvr_vectors, "quant": quant_vectors,
+              "curve": curve_vectors}
     for name in (sys.argv[1:] or list(makers)):
         np.savez_compressed(OUT / f"golden_{name}.npz", **makers[name](R))
     for f in sorted(OUT.glob("golden_*.npz")):

@@ -154,3 +154,18 @@ def test_quantize_empty_and_errors(hip):
     stream = hip.quantize(data, dev(index), W, H, True, bits)
     with pytest.raises(hip.MrayError, match="whole byte"):
         hip.dequantize(stream[:-1], dev(index), W, H, True, bits)
+
+
+@pytest.mark.parametrize("case", ["ml", "mix", "mix64"])
+def test_curve_table_matches_reference(hip, case):
+    """The quantization-curve search on the GPU (mray_hip_quantization_table)
+    picks the compiled reference's tables (tests/golden/golden_curve.npz) for
+    start widths 8, 10 and 14."""
+    g = dict(np.load(GOLD / "golden_curve.npz"))
+    W, H, m = (int(v) for v in g[f"{case}_shape"])
+    d_data, d_index = dev(g[f"{case}_data"]), dev(g[f"{case}_index"])
+    d_params = dev(g[f"{case}_params"].view(np.int16))
+    for sb in (8, 10, 14):
+        print(f"[quantize] curve case={case} W={W} H={H} m={m} start_bits={sb}", flush=True)
+        t = hip.curve_table(d_data, d_index, W, H, m, d_params, sb)
+        assert t == g[f"{case}_table{sb}"].tolist()

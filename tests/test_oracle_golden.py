@@ -413,6 +413,26 @@ def test_quantization_packing_and_steps(O):
         assert np.all(err <= step[whole] * (1 + 1e-3) + 1e-7)
 
 
+CURVE_CASES = ["ml", "mix", "mix64"]
+
+
+@pytest.mark.parametrize("case", CURVE_CASES)
+def test_quantization_curve_matches_reference(O, case):
+    """The quantization-curve search (create_quantization_table,
+    src/measure_coding_error.cpp:374-437) picks the compiled reference's
+    tables on coded images it generated."""
+    g = dict(np.load(GOLD / "golden_curve.npz"))
+    W, H, m = (int(v) for v in g[f"{case}_shape"])
+    data, index = g[f"{case}_data"], g[f"{case}_index"]
+    pf = (g[f"{case}_params"].astype(np.float32) / np.float32(65535.0)).astype(np.float32)
+    for sb in (8, 10, 14):
+        t = np.zeros(m, np.uint8)
+        O.lib.orc_quantization_table(W, H, m, ptr(data, C.POINTER(C.c_float)), ptr(index, C.POINTER(C.c_uint32)), 0,
+                                     ptr(pf, C.POINTER(C.c_float)), sb, ptr(t, C.POINTER(C.c_uint8)))
+        assert np.array_equal(t, g[f"{case}_table{sb}"]), (sb, t.tolist(), g[f"{case}_table{sb}"].tolist())
+        assert t[0] == 16 and t[1] == sb and np.all(np.diff(t[1:].astype(int)) <= 0) and t[-1] >= 2
+
+
 def test_sph_gather_matches_reference():
     g = dict(np.load(GOLD / "golden_sph.npz"))
     O = Oracle()
