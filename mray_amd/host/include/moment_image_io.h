@@ -1,7 +1,8 @@
 // Moment-image files (src/moment_image_io.cpp:255-530), the subset without
-// LZ4 compression, quantisation or entropy coding (compress = 0,
-// quantization_bits = 32, entropy_coding = 0): byte-compatible with the
-// reference for that subset. Header: uint16 width, height; uint8 moments,
+// LZ4 compression or entropy coding: float moments, or for compact images the
+// quantized stream (quantization_bits < 32, with the quantization table after
+// the header when one is set); byte-compatible with the reference for that
+// subset. Header: uint16 width, height; uint8 moments,
 // coding warp, flags (1 compressed, 2 compact, 4 prediction coded,
 // 16 bounds, 32 error bounds, 128 coding parameters), quantisation byte;
 // float3 domain min, max. Then, compact: per-pixel counts as uint8, bounds,
@@ -15,3 +16,20 @@ void write_moment_image(const MomentImageHost &mi, const std::string filename, b
                         Byte quantizationBits = 32, bool entropy_coding = false);
 
 MomentImageHost load_moment_image(const std::string filename);
+
+// moment_quantization's quantization table (src/moment_quantization.cpp:165-242):
+// once set, prediction-coded images are quantized with it and files carry it
+// after the header
+void set_prediction_coding_quantization_table(const vector<Byte> &table);
+bool uses_prediction_coding_quantization_table();
+vector<Byte> get_prediction_coding_quantization_table();
+
+// create_quantization_table (src/measure_coding_error.cpp:374-437) on the
+// GPU (mray_hip_quantization_table); sets the table
+void create_quantization_table(const MomentImageHost &img, int start_bits);
+
+// sample_moment_image (src/moment_image_sample.h:33-61): every
+// (width / w)-th column and (height / h)-th row of a compact image. Sizes
+// for which the reference's loops read or write outside the images (w != h
+// with h > w, or rows past the source) throw invalid_args.
+MomentImageHost sample_moment_image(const MomentImageHost &img, int w, int h);

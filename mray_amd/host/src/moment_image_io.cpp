@@ -28,6 +28,9 @@ void get_bytes(std::fstream &in, void *p, size_t n, const std::string &file)
 // moment 0, then quantization_bits) for prediction-coded images and
 // quantization_bits for every moment otherwise
 // (src/moment_quantization.cpp:186-203, 285-360)
+bool g_use_table = false; // s_UseQuantizationTable, s_QuantizationTable
+vector<Byte> g_table(255, 0);
+
 vector<Byte> quant_widths(const MomentImageHost &mi, int bits, const vector<Byte> &file_table)
 {
     vector<Byte> w(std::max(mi.num_moments, 1), static_cast<Byte>(bits));
@@ -94,9 +97,11 @@ void write_moment_image(const MomentImageHost &mi, const std::string filename, b
     flags |= mi.has_error_bounds() << 5;
     flags |= mi.has_coding_params() << 7;
     put(out, flags);
-    put(out, static_cast<uint8_t>(quantizationBits)); // no entropy coding, no table
+    put(out, static_cast<uint8_t>(quantizationBits | (g_use_table ? 0x80 : 0))); // no entropy coding
     put(out, mi.domain.min);
     put(out, mi.domain.max);
+    if (g_use_table) // write_quantization_table (src/moment_image_io.cpp:340-344)
+        put_bytes(out, g_table.data(), mi.num_moments);
     if (mi.is_compact)
     {
         vector<Byte> counts((size_t)mi.width * mi.height);
@@ -118,7 +123,9 @@ void write_moment_image(const MomentImageHost &mi, const std::string filename, b
     if (quantizationBits < 32)
     {
         // moment_quantization::quantize_prediction_coding / quantize (src/moment_image_io.cpp:397-408)
-        const vector<Byte> stream = quantize_on_device(mi, quant_widths(mi, quantizationBits, {}));
+        const vector<Byte> table =
+            g_use_table ? vector<Byte>(g_table.begin(), g_table.begin() + mi.num_moments) : vector<Byte>();
+        const vector<Byte> stream = quantize_on_device(mi, quant_widths(mi, quantizationBits, table));
         put_bytes(out, stream.data(), stream.size());
     }
     else
@@ -225,4 +232,78 @@ void write_PPM(string file, const ColorImageHost &img, Vec3f background_color)
             fwrite(&out, 1, 3, fp);
         }
     fclose(fp);
+}
+
+void set_prediction_coding_quantization_table(const vector<Byte> &table)
+{
+    g_use_table = true;
+    for (size_t i = 0; i < table.size() && i < g_table.size(); ++i)
+        g_table[i] = table[i];
+}
+
+bool uses_prediction_coding_quantization_table() { return g_use_table; }
+
+vector<Byte> get_prediction_coding_quantization_table() { return g_table; }
+
+void create_quantization_table(const MomentImageHost &img, int start_bits)
+{
+    using namespace mray_host;
+    if (!img.prediction_code || !img.is_compact)
+        throw invalid_args("the quantization curve needs a compact prediction-coded image");
+    ensure_init();
+    DeviceBuffer d_data(img.data.data(), std::max<size_t>(img.data.size(), 1) * sizeof(float));
+    DeviceBuffer d_index(img.index.data(), img.index.size() * sizeof(uint32_t));
+    DeviceBuffer d_params(img.coding_params.data(),
+                          std::max<size_t>(img.coding_params.size(), 1) * sizeof(CodingParamType));
+    vector<Byte> table(img.num_moments);
+    check(mray_hip_quantization_table(d_data.as<float>(), d_index.as<uint32_t>(), img.width, img.height,
+                                      img.num_moments, img.coding_warp,
+                                      img.coding_params.empty() ? nullptr : d_params.as<uint16_t>(), start_bits,
+                                      table.data(), nullptr),
+          "mray_hip_quantization_table");
+    set_prediction_coding_quantization_table(table);
+}
+
+MomentImageHost sample_moment_image(const MomentImageHost &img, int w, int h)
+{
+    if (!img.is_compact || w <= 0 || h <= 0)
+        throw invalid_args("sample_moment_image needs a compact image and a positive size");
+    const int fx = img.width / w, fy = img.height / h;
+    // the reference's loops: rows y < w, columns x < h, pixel y * w + x
+    if (h > w || (w - 1) * fy >= img.height || (h - 1) * fx >= img.width)
+        throw invalid_args("sample_moment_image: the reference's loops leave the images at this size");
+    const int m = img.num_moments;
+    vector<uint32_t> counts((size_t)w * h, 0);
+    vector<float> dense((size_t)w * h * m, 0.0f);
+    for (int y = 0; y < w; ++y)
+        for (int x = 0; x < h; ++x)
+        {
+            const size_t src = (size_t)(y * fy) * img.width + x * fx;
+            const uint32_t n = img.index[src + 1] - img.index[src];
+            const size_t dst = (size_t)y * w + x;
+            counts[dst] = n;
+            for (uint32_t l = 0; l < n; ++l)
+                dense[dst * m + l] = img.data[img.index[src] + l];
+        }
+    MomentImageHost s;
+    s.width = w;
+    s.height = h;
+    s.num_moments = m;
+    s.is_compact = true;
+    s.prediction_code = img.prediction_code;
+    s.coding_warp = img.coding_warp;
+    s.coding_params = img.coding_params;
+    s.index.resize((size_t)w * h + 1);
+    uint32_t sum = 0;
+    for (size_t i = 0; i < counts.size(); ++i)
+    {
+        s.index[i] = sum;
+        sum += counts[i];
+    }
+    s.index[counts.size()] = sum;
+    s.data.resize(sum);
+    for (size_t i = 0; i < counts.size(); ++i)
+        for (uint32_t l = 0; l < counts[i]; ++l)
+            s.data[s.index[i] + l] = dense[i * m + l];
+    return s;
 }

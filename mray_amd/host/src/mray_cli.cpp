@@ -120,6 +120,14 @@ void run_generate(const string &mi_file, const string &dataset, unordered_map<st
     auto volume = load_raw(dataset, p);
     mi.domain = volume.domain;
     generate(cam, volume, mi, gen_params);
+    // the quantization curve of a quantized prediction-coded image, on a
+    // sample of it (src/main.cpp:318-327)
+    if (gen_params.prediction_coding && gen_params.quantization_bits < 32)
+    {
+        const int sw = std::min(mi.width, gen_params.quantization_curve_sampling);
+        const int sh = std::min(mi.height, gen_params.quantization_curve_sampling);
+        create_quantization_table(sample_moment_image(mi, sw, sh), gen_params.quantization_bits);
+    }
     write_moment_image(mi, mi_file, gen_params.compress, gen_params.quantization_bits, gen_params.entropy_coding);
 }
 

@@ -1,6 +1,6 @@
 """Reader for moment-image files (the uncompressed variants of
 src/moment_image_io.cpp: float moments, or the quantized stream of a compact
-image without a table) used by the CLI tests."""
+image, with or without its quantization table) used by the CLI tests."""
 import numpy as np
 
 
@@ -10,12 +10,15 @@ def read_moment_image(path):
     num_moments, warp, flags, qbits = b[4], b[5], b[6], b[7]
     dmin = np.frombuffer(b, np.float32, 3, 8)
     dmax = np.frombuffer(b, np.float32, 3, 20)
-    assert not (flags & 1) and not (qbits & 0xC0)
+    assert not (flags & 1) and not (qbits & 0x40)
     off = 32
     npix = int(w) * int(h)
     out = dict(width=int(w), height=int(h), num_moments=int(num_moments), coding_warp=int(warp),
                compact=bool(flags & 2), prediction_code=bool(flags & 4), domain=(dmin, dmax),
                quantization_bits=int(qbits & 0x3F))
+    if qbits & 0x80:  # the quantization table follows the header
+        out["quant_table"] = np.frombuffer(b, np.uint8, int(num_moments), off).copy()
+        off += int(num_moments)
     if out["compact"]:
         counts = np.frombuffer(b, np.uint8, npix, off).astype(np.uint32)
         off += npix

@@ -133,16 +133,18 @@ def test_cli_round_trip(tmp_path, case):
 
 
 def test_cli_quantized_file(tmp_path):
-    """quantization_bits=10 on a compacted, prediction-coded image: the file
-    carries moment_quantization's stream of the coded moments (16 bits for
-    moment 0, then 10; src/moment_quantization.cpp:244-283) bit for bit, and
+    """quantization_bits=10 on a compacted, prediction-coded image: the CLI
+    runs the quantization-curve search on the image (36x36, under the 64-pixel
+    sampling, so the sample is the image; src/main.cpp:318-327), the file
+    carries the table and moment_quantization's stream of the coded moments
+    with it (src/moment_quantization.cpp:244-283) bit for bit, and
     reconstructing it renders the oracle's image of the dequantized moments."""
     from oracle_ffi import quant_bits
 
     O = Oracle()
     vol, dmin, dmax = synth.marschner_lobb(48), (-1, -1, -1), (1, 1, 1)
     cam = synth.ortho_camera()
-    W, H, m, step = 40, 32, 24, 2.0 / 96
+    W, H, m, step = 36, 36, 24, 2.0 / 96
     vol.tofile(tmp_path / "vol.raw")
     camera_file(tmp_path / "cam.txt", cam)
     rgba8 = synth.tf_ramps(0)
@@ -157,7 +159,15 @@ def test_cli_quantized_file(tmp_path):
     assert q10["quantization_bits"] == 10 and q10["compact"] and q10["prediction_code"]
     assert np.array_equal(q10["index"], coded["index"])
     assert np.array_equal(q10["coding_params"], coded["coding_params"])
-    bits = quant_bits(256, 1, 10)
+    pf = (coded["coding_params"].astype(np.float32) / np.float32(65535.0)).astype(np.float32)
+    table = np.zeros(m, np.uint8)
+    O.lib.orc_quantization_table(W, H, m, coded["data"].ctypes.data_as(C.POINTER(C.c_float)),
+                                 coded["index"].ctypes.data_as(C.POINTER(C.c_uint32)), 0,
+                                 pf.ctypes.data_as(C.POINTER(C.c_float)), 10,
+                                 table.ctypes.data_as(C.POINTER(C.c_uint8)))
+    print(f"[cli] quantization table {table.tolist()}", flush=True)
+    assert np.array_equal(q10["quant_table"], table)
+    bits = quant_bits(256, 1, table=table)
     stream = O.quantize(W, H, coded["data"], coded["index"], 1, bits)
     print(f"[cli] quantized stream {len(q10['stream'])} bytes for {int(coded['index'][-1])} moments", flush=True)
     assert np.array_equal(q10["stream"], stream)
