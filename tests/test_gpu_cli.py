@@ -172,6 +172,26 @@ def test_cli_quantized_file(tmp_path):
     print(f"[cli] quantized stream {len(q10['stream'])} bytes for {int(coded['index'][-1])} moments", flush=True)
     assert np.array_equal(q10["stream"], stream)
 
+    # a 16x16 sample (every 2nd row and column, sample_moment_image,
+    # src/moment_image_sample.h:33-61) chooses the table
+    run("--generate", tmp_path / "q10s.bin", tmp_path / "vol.raw",
+        common + ",quantization_bits=10,quantization_curve_sampling=16")
+    sw, f = 16, W // 16
+    pix = [(y * f) * W + x * f for y in range(sw) for x in range(sw)]
+    cnt = np.diff(coded["index"].astype(np.int64))
+    s_index = np.concatenate([[0], np.cumsum(cnt[pix])]).astype(np.uint32)
+    s_data = np.concatenate([coded["data"][coded["index"][p]:coded["index"][p + 1]] for p in pix]).astype(np.float32)
+    s_table = np.zeros(m, np.uint8)
+    O.lib.orc_quantization_table(sw, sw, m, s_data.ctypes.data_as(C.POINTER(C.c_float)),
+                                 s_index.ctypes.data_as(C.POINTER(C.c_uint32)), 0,
+                                 pf.ctypes.data_as(C.POINTER(C.c_float)), 10,
+                                 s_table.ctypes.data_as(C.POINTER(C.c_uint8)))
+    q10s = read_moment_image(tmp_path / "q10s.bin")
+    print(f"[cli] sampled quantization table {s_table.tolist()}", flush=True)
+    assert np.array_equal(q10s["quant_table"], s_table)
+    assert np.array_equal(q10s["stream"], O.quantize(W, H, coded["data"], coded["index"], 1,
+                                                     quant_bits(256, 1, table=s_table)))
+
     factor = 4.0
     rp = RecParams(step_size=step, bias=1e-4, early_ray_termination=0.99, medium_emission=1.0)
     run("--reconstruct", tmp_path / "q10.bin", tmp_path / "out.ppm",
