@@ -241,6 +241,23 @@ def main():
                  "ms": {k: round(v, 3) for k, v in times.items()},
                  "GB_per_s": {k: round(qbytes / (v * 1e-3) / 1e9, 1) for k, v in times.items()}}
         del qs, dq, qindex
+        # the quantization-curve search of generate with prediction coding and
+        # 10 bits: the frame compacted (error threshold 1e-3) and coded, every
+        # 16th row and column (the CLI's 64x64 sample)
+        cdata, cindex = hip.compact_image(mo, W, H, m, 1e-3, bounds=bnd)
+        cparams = hip.prediction_encode(cdata, cindex, W, H, m, 0, 1e-4)
+        torch.cuda.synchronize()
+        hi, hd = cindex.cpu().numpy().view(np.uint32).astype(np.int64), cdata.cpu().numpy()
+        sw, f = 64, W // 64
+        pix = [(y * f) * W + x * f for y in range(sw) for x in range(sw)]
+        s_index = np.concatenate([[0], np.cumsum(hi[1:][pix] - hi[:-1][pix])]).astype(np.uint32)
+        s_data = np.concatenate([hd[hi[p]:hi[p + 1]] for p in pix]).astype(np.float32)
+        d_si, d_sd = torch.from_numpy(s_index.view(np.int32)).cuda(), torch.from_numpy(s_data).cuda()
+        t0 = time.perf_counter()
+        curve = hip.curve_table(d_sd, d_si, sw, sw, m, cparams, 10)
+        quant["curve_search"] = {"workload": f"{sw}x{sw} sample of the coded frame, {int(s_index[-1])} moments",
+                                 "ms": round((time.perf_counter() - t0) * 1e3, 2), "table": curve}
+        del cdata, cindex
 
     cpu = None
     if not args.no_cpu:
