@@ -130,8 +130,6 @@ __global__ __launch_bounds__(kThreads) void quant_tile_sums(const uint32_t *__re
         sums[blockIdx.x] = sum;
 }
 
-// pass 2: 64-bit exclusive scan of the tile sums (one workgroup, chunks of
-// kThreads with a running carry); the total goes to offsets[npix]
 // what the host reads back after the scan, in one copy
 struct ScanSummary
 {
@@ -140,6 +138,9 @@ struct ScanSummary
     int32_t err;    // a pixel has more moments than the table
 };
 
+// pass 2: 64-bit exclusive scan of the tile sums (one workgroup, chunks of
+// kThreads with a running carry); the total goes to offsets[npix] and, with
+// the element count and the error flag, to the summary
 __global__ __launch_bounds__(kThreads) void quant_tile_offsets(const uint32_t *__restrict__ sums, int ntiles,
                                                                uint64_t *__restrict__ tile_off,
                                                                uint64_t *__restrict__ offsets, size_t npix,
