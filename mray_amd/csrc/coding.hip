@@ -509,13 +509,17 @@ __global__ __launch_bounds__(64) void prepare_coded_kernel(const float *__restri
 // decoded as in prepare_coded_kernel and converted to trigonometric moments
 // (exponentialToTrigonometricMoments, src/mese/MESE_dynamic.h:78-97).
 // q_index < 0 writes those moments to ref; else err[pixel] = sqrt(sum_l
-// (ref_l - t_l)^2) / ref_0 (0 for empty pixels).
+// (ref_l - t_l)^2) / ref_0 (0 for empty pixels). Grid row y measures
+// q_bits = q_bits_hi - y into err + y * npix, so one launch covers the widths
+// the search tries for a moment.
 __global__ __launch_bounds__(64) void coding_error_kernel(const float *__restrict__ mom,
                                                           const uint32_t *__restrict__ index, int npix, int m,
                                                           int warp, const uint16_t *__restrict__ qparams,
-                                                          int q_index, float q_scale, float *__restrict__ ref,
+                                                          int q_index, int q_bits_hi, float *__restrict__ ref,
                                                           float *__restrict__ err)
 {
+    const float q_scale = ldexpf(1.0f, q_bits_hi - (int)blockIdx.y); // 2^b, exact
+    err += (size_t)blockIdx.y * npix;
     extern __shared__ cf lds_cf[];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int pix = blockIdx.x * nt + tid;
@@ -718,32 +722,40 @@ int launch_quantization_table(const float *data, const uint32_t *index, int W, i
             return MRAY_ERR_ARG;
         nonempty += h_index[p + 1] > h_index[p];
     }
+    // widths tried per moment: the previous moment's down to 2, at most 30
+    constexpr int kMaxVariants = 30;
     float *ref = nullptr, *err = nullptr;
     if (hipMallocAsync((void **)&ref, ((size_t)h_index[npix] + 1) * sizeof(float), stream) != hipSuccess)
         return MRAY_ERR_OOM;
-    if (hipMallocAsync((void **)&err, (size_t)npix * sizeof(float), stream) != hipSuccess)
+    if (hipMallocAsync((void **)&err, (size_t)npix * kMaxVariants * sizeof(float), stream) != hipSuccess)
     {
         (void)hipFreeAsync(ref, stream);
         return MRAY_ERR_OOM;
     }
     const dim3 grid((npix + nt - 1) / nt);
     const size_t lds = (size_t)3 * m * nt * sizeof(cf);
-    hipLaunchKernelGGL(coding_error_kernel, grid, dim3(nt), lds, stream, data, index, npix, m, warp, qparams, -1,
-                       1.0f, ref, err);
-    std::vector<float> h_err(npix);
+    hipLaunchKernelGGL(coding_error_kernel, grid, dim3(nt), lds, stream, data, index, npix, m, warp, qparams, -1, 0,
+                       ref, err);
+    std::vector<float> h_err((size_t)npix * kMaxVariants);
     int rc = 0;
-    auto mean_error = [&](int idx, int b, float &mean) {
-        hipLaunchKernelGGL(coding_error_kernel, grid, dim3(nt), lds, stream, data, index, npix, m, warp, qparams,
-                           idx, std::ldexp(1.0f, b), ref, err);
-        if (hipMemcpyAsync(h_err.data(), err, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, stream) !=
+    // the mean errors of moment idx at widths b_hi, b_hi - 1, ..., b_hi - n + 1
+    // (one launch); each mean is get_mean's float sum in pixel order
+    auto mean_errors = [&](int idx, int b_hi, int n, float *mean) {
+        hipLaunchKernelGGL(coding_error_kernel, dim3(grid.x, n), dim3(nt), lds, stream, data, index, npix, m, warp,
+                           qparams, idx, b_hi, ref, err);
+        if (hipMemcpyAsync(h_err.data(), err, (size_t)npix * n * sizeof(float), hipMemcpyDeviceToHost, stream) !=
                 hipSuccess ||
             hipStreamSynchronize(stream) != hipSuccess)
             return false;
-        mean = 0.0f;
-        for (int p = 0; p < npix; ++p)
-            if (h_index[p + 1] > h_index[p])
-                mean += h_err[p];
-        mean /= (float)nonempty;
+        for (int v = 0; v < n; ++v)
+        {
+            float s = 0.0f;
+            const float *e = h_err.data() + (size_t)v * npix;
+            for (int p = 0; p < npix; ++p)
+                if (h_index[p + 1] > h_index[p])
+                    s += e[p];
+            mean[v] = s / (float)nonempty;
+        }
         return true;
     };
     std::fill(table, table + m, (uint8_t)0);
@@ -751,23 +763,25 @@ int launch_quantization_table(const float *data, const uint32_t *index, int W, i
     if (m > 1)
         table[1] = (uint8_t)start_bits;
     float threshold = 0.0f;
-    if (m > 1 && !mean_error(1, start_bits, threshold))
+    if (m > 1 && !mean_errors(1, start_bits, 1, &threshold))
         rc = MRAY_ERR_HIP;
+    float means[kMaxVariants];
     for (int idx = 2; rc == 0 && idx < m; ++idx)
     {
+        // the reference measures b = table[idx-1], ..., 2 until an error
+        // reaches the threshold; all of them are measured at once and the same
+        // first one ends the walk
         table[idx] = table[idx - 1];
-        for (int b = table[idx - 1]; b >= 2; --b)
+        const int b_hi = table[idx - 1], n = b_hi - 1;
+        if (n <= 0)
+            continue;
+        if (!mean_errors(idx, b_hi, n, means))
         {
-            float e = 0.0f;
-            if (!mean_error(idx, b, e))
-            {
-                rc = MRAY_ERR_HIP;
-                break;
-            }
-            if (e >= threshold)
-                break;
-            table[idx] = (uint8_t)b;
+            rc = MRAY_ERR_HIP;
+            break;
         }
+        for (int v = 0; v < n && means[v] < threshold; ++v)
+            table[idx] = (uint8_t)(b_hi - v);
     }
     if (hipFreeAsync(ref, stream) != hipSuccess || hipFreeAsync(err, stream) != hipSuccess)
         rc = rc ? rc : MRAY_ERR_HIP;
