@@ -722,12 +722,15 @@ int launch_quantization_table(const float *data, const uint32_t *index, int W, i
             return MRAY_ERR_ARG;
         nonempty += h_index[p + 1] > h_index[p];
     }
-    // widths tried per moment: the previous moment's down to 2, at most 30
-    constexpr int kMaxVariants = 30;
+    // widths tried per moment: the previous moment's down to 2; no width
+    // exceeds start_bits (table[1]), so at most start_bits - 1 of them
+    if (start_bits < 2 || start_bits > 31)
+        return MRAY_ERR_ARG;
+    const int max_variants = start_bits - 1;
     float *ref = nullptr, *err = nullptr;
     if (hipMallocAsync((void **)&ref, ((size_t)h_index[npix] + 1) * sizeof(float), stream) != hipSuccess)
         return MRAY_ERR_OOM;
-    if (hipMallocAsync((void **)&err, (size_t)npix * kMaxVariants * sizeof(float), stream) != hipSuccess)
+    if (hipMallocAsync((void **)&err, (size_t)npix * max_variants * sizeof(float), stream) != hipSuccess)
     {
         (void)hipFreeAsync(ref, stream);
         return MRAY_ERR_OOM;
@@ -736,11 +739,13 @@ int launch_quantization_table(const float *data, const uint32_t *index, int W, i
     const size_t lds = (size_t)3 * m * nt * sizeof(cf);
     hipLaunchKernelGGL(coding_error_kernel, grid, dim3(nt), lds, stream, data, index, npix, m, warp, qparams, -1, 0,
                        ref, err);
-    std::vector<float> h_err((size_t)npix * kMaxVariants);
+    std::vector<float> h_err((size_t)npix * max_variants);
     int rc = 0;
     // the mean errors of moment idx at widths b_hi, b_hi - 1, ..., b_hi - n + 1
     // (one launch); each mean is get_mean's float sum in pixel order
     auto mean_errors = [&](int idx, int b_hi, int n, float *mean) {
+        if (n > max_variants)
+            return false;
         hipLaunchKernelGGL(coding_error_kernel, dim3(grid.x, n), dim3(nt), lds, stream, data, index, npix, m, warp,
                            qparams, idx, b_hi, ref, err);
         if (hipMemcpyAsync(h_err.data(), err, (size_t)npix * n * sizeof(float), hipMemcpyDeviceToHost, stream) !=
@@ -765,7 +770,7 @@ int launch_quantization_table(const float *data, const uint32_t *index, int W, i
     float threshold = 0.0f;
     if (m > 1 && !mean_errors(1, start_bits, 1, &threshold))
         rc = MRAY_ERR_HIP;
-    float means[kMaxVariants];
+    float means[31];
     for (int idx = 2; rc == 0 && idx < m; ++idx)
     {
         // the reference measures b = table[idx-1], ..., 2 until an error

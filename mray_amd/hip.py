@@ -228,9 +228,12 @@ def reconstruct_backward(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, 
     g_tf = torch.zeros((256, 4), dtype=torch.float32, device=dev) if want_tf else None
     c = Camera.from_dict(cam)
     tf = tf.to(device=dev, dtype=torch.float32).contiguous()
+    # contiguous copies are bound to names so they outlive the call
+    rgba_c = rgba.contiguous()
+    grad_c = grad_rgba.contiguous()
     _check(load_library().mray_hip_reconstruct_backward(C.byref(c), _f3(*dmin), _f3(*dmax), _ptr(lagrange),
                                                         _ptr(index), _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp),
-                                                        _ptr(rgba.contiguous()), _ptr(grad_rgba.contiguous()),
+                                                        _ptr(rgba_c), _ptr(grad_c),
                                                         _ptr(g_lag), _ptr(g_tf), _stream(stream)),
            "mray_hip_reconstruct_backward")
     return g_lag, g_tf
@@ -243,8 +246,9 @@ def generate_backward(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: 
     if out is None:
         out = torch.zeros_like(vol.voxels, dtype=torch.float32)
     c = Camera.from_dict(cam)
+    grad_c = grad_moments.contiguous()
     _check(load_library().mray_hip_generate_backward(vol.handle, C.byref(c), C.byref(gp), W, H, m,
-                                                     _ptr(grad_moments.contiguous()), _ptr(ray_range), _ptr(out),
+                                                     _ptr(grad_c), _ptr(ray_range), _ptr(out),
                                                      _stream(stream)),
            "mray_hip_generate_backward")
     return out
@@ -254,8 +258,9 @@ def prepare_backward(moments: torch.Tensor, W: int, H: int, m: int, bias: float,
                      index=None, stream=None) -> torch.Tensor:
     """dL/dmoments for a given dL/dlagrange of prepare (MIXED mathematics)."""
     out = _like(moments)
+    grad_c = grad_lagrange.contiguous()
     _check(load_library().mray_hip_prepare_backward(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias),
-                                                    _ptr(grad_lagrange.contiguous()), _ptr(out), _stream(stream)),
+                                                    _ptr(grad_c), _ptr(out), _stream(stream)),
            "mray_hip_prepare_backward")
     return out
 

@@ -62,9 +62,11 @@ struct MomentImageHost
     // Prediction coding of a compact image (src/moment_image.cpp:96-152).
     void prediction_encode(int warp, float bias);
 
-    // Lagrange multipliers of one pixel (src/moment_image.cpp:154-183).
-    vector<float> prepare_moments(int x, int y, float bias) const;
+    // Lagrange multipliers of one pixel (src/moment_image.cpp:154-183);
+    // form 1 = mixed (FP64 recursion, the default), 0 = plain (the
+    // reference's FP32 arithmetic).
+    vector<float> prepare_moments(int x, int y, float bias, int form = 1) const;
 
     // Lagrange multipliers of all pixels, in the layout of data[].
-    vector<float> prepare_all_moments(float bias) const;
+    vector<float> prepare_all_moments(float bias, int form = 1) const;
 };

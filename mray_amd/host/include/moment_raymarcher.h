@@ -1,7 +1,10 @@
 // generate / reconstruct with the reference's signatures
 // (src/moment_raymarcher.h:48-64), running on one MI355X through
 // libmray_hip.so. Errors are thrown as std::runtime_error carrying the C
-// ABI's message.
+// ABI's message. The library holds no mutable global state: the kernel form
+// (MRAY_FORM_MIXED, the default, or MRAY_FORM_PLAIN, the reference's FP32
+// arithmetic) is the kernel_form field of the parameters of each call, so
+// calls from different threads are independent.
 #pragma once
 
 #include "color_image.h"
@@ -10,11 +13,6 @@
 #include "parameters.h"
 #include "transfer_function.h"
 #include "volume.h"
-
-// Kernel form used by generate / prepare / reconstruct: MRAY_FORM_MIXED (1,
-// the default) or MRAY_FORM_PLAIN (0, the reference's FP32 arithmetic).
-void set_kernel_form(int form);
-int get_kernel_form();
 
 // Ray-marches the volume into img (dense moments, bounds when img has
 // bounds), then compacts and prediction-codes it as params ask.

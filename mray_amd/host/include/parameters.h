@@ -23,6 +23,9 @@ struct GenerationParameters
     int quantization_curve_sampling = 64;
     bool transform_to_bounds = false;
     float transform_bound_eps = 0.01f;
+    // kernel form of this call (not in the reference, which has one CUDA
+    // form): 1 = mixed (default), 0 = plain, the reference's FP32 arithmetic
+    int kernel_form = 1;
 };
 
 struct ReconstructionParameters
@@ -36,6 +39,8 @@ struct ReconstructionParameters
     float step_size = 0.01f;
     float step_size_write_samples = 1.5f;
     bool use_truncated_fourier = false;
+    // kernel form of this call: 1 = mixed (default), 0 = plain
+    int kernel_form = 1;
 };
 
 // ReferenceParameters (src/parameters.h:339-350) without the single

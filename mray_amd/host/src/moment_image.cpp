@@ -56,7 +56,7 @@ void MomentImageHost::prediction_encode(int warp, float bias)
     coding_warp = warp;
 }
 
-vector<float> MomentImageHost::prepare_all_moments(float bias) const
+vector<float> MomentImageHost::prepare_all_moments(float bias, int form) const
 {
     mray_host::ensure_init();
     const size_t npix = (size_t)width * height;
@@ -74,7 +74,7 @@ vector<float> MomentImageHost::prepare_all_moments(float bias) const
               "prepare_coded");
     }
     else
-        check(mray_hip_prepare(d_data.as<float>(), idx, width, height, num_moments, bias, get_kernel_form(),
+        check(mray_hip_prepare(d_data.as<float>(), idx, width, height, num_moments, bias, form,
                                d_lag.as<float>(), nullptr),
               "prepare");
     vector<float> out(data.size());
@@ -82,9 +82,9 @@ vector<float> MomentImageHost::prepare_all_moments(float bias) const
     return out;
 }
 
-vector<float> MomentImageHost::prepare_moments(int x, int y, float bias) const
+vector<float> MomentImageHost::prepare_moments(int x, int y, float bias, int form) const
 {
-    auto all = prepare_all_moments(bias);
+    auto all = prepare_all_moments(bias, form);
     const size_t idx = get_idx(x, y);
     return vector<float>(all.begin() + idx, all.begin() + idx + get_num_moments(x, y));
 }

@@ -13,7 +13,11 @@ using mray_host::DeviceBuffer;
 
 namespace
 {
-int g_form = MRAY_FORM_MIXED;
+void require_form(int form)
+{
+    if (form != MRAY_FORM_PLAIN && form != MRAY_FORM_MIXED)
+        throw invalid_args("kernel_form must be MRAY_FORM_PLAIN (0) or MRAY_FORM_MIXED (1)");
+}
 
 struct VolumeHandle
 {
@@ -36,6 +40,7 @@ void require_size(const MomentImageHost &img)
 void generate_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &img, const GenerationParameters &params)
 {
     require_size(img);
+    require_form(params.kernel_form);
     if (params.prediction_coding && !params.compact_image)
         throw invalid_args("prediction coding needs a compact image (compact_image=1)");
     const int W = img.width, H = img.height, m = img.num_moments;
@@ -45,7 +50,7 @@ void generate_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &img, 
     DeviceBuffer d_mom(npix * m * sizeof(float));
     const bool want_bounds = img.has_bounds() || params.transform_to_bounds;
     DeviceBuffer d_bounds(want_bounds ? npix * 4 : 0);
-    check(mray_hip_generate(vol, &c, &g, g_form, W, H, m, d_mom.as<float>(), d_bounds.as<uint32_t>(), nullptr, nullptr,
+    check(mray_hip_generate(vol, &c, &g, params.kernel_form, W, H, m, d_mom.as<float>(), d_bounds.as<uint32_t>(), nullptr, nullptr,
                             nullptr),
           "generate");
     if (img.has_bounds())
@@ -93,7 +98,7 @@ void generate_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &img, 
 // multipliers (or trigonometric moments for the truncated Fourier
 // reconstruction) of every pixel into d_lag, layout of mi.data
 void prepare_on_device(const MomentImageHost &mi, const DeviceBuffer &d_data, const DeviceBuffer &d_index,
-                       float bias, bool truncated_fourier, DeviceBuffer &d_lag)
+                       float bias, bool truncated_fourier, int form, DeviceBuffer &d_lag)
 {
     const int W = mi.width, H = mi.height, m = mi.num_moments;
     const uint32_t *idx = mi.is_compact ? d_index.as<uint32_t>() : nullptr;
@@ -118,7 +123,7 @@ void prepare_on_device(const MomentImageHost &mi, const DeviceBuffer &d_data, co
               "prepare_coded");
         return;
     }
-    check(mray_hip_prepare(d_data.as<float>(), idx, W, H, m, bias, g_form, d_lag.as<float>(), nullptr), "prepare");
+    check(mray_hip_prepare(d_data.as<float>(), idx, W, H, m, bias, form, d_lag.as<float>(), nullptr), "prepare");
 }
 
 void reconstruct_float(const cut::Camera &cam, const MomentImageHost &mi, const TransferFunction &tf,
@@ -126,6 +131,7 @@ void reconstruct_float(const cut::Camera &cam, const MomentImageHost &mi, const 
 {
     mray_host::ensure_init();
     require_size(mi);
+    require_form(params.kernel_form);
     const int W = mi.width, H = mi.height, m = mi.num_moments;
     const size_t npix = (size_t)W * H;
     if (mi.is_compact && mi.index.size() != npix + 1)
@@ -136,7 +142,7 @@ void reconstruct_float(const cut::Camera &cam, const MomentImageHost &mi, const 
     DeviceBuffer d_index(mi.is_compact ? mi.index.data() : nullptr, mi.is_compact ? (npix + 1) * 4 : 0);
     DeviceBuffer d_bounds(mi.has_bounds() ? mi.bounds.data() : nullptr, mi.has_bounds() ? npix * 4 : 0);
     DeviceBuffer d_lag(std::max<size_t>(mi.data.size(), 1) * sizeof(float));
-    prepare_on_device(mi, d_data, d_index, params.bias, params.use_truncated_fourier, d_lag);
+    prepare_on_device(mi, d_data, d_index, params.bias, params.use_truncated_fourier, params.kernel_form, d_lag);
     DeviceBuffer d_tf(tf.data.data(), sizeof(tf.data));
     DeviceBuffer d_rgba(npix * sizeof(Vec4f));
     mray_camera c = mray_host::to_c(cam);
@@ -145,21 +151,12 @@ void reconstruct_float(const cut::Camera &cam, const MomentImageHost &mi, const 
     const float dmax[3] = {mi.domain.max.x, mi.domain.max.y, mi.domain.max.z};
     check(mray_hip_reconstruct(&c, dmin, dmax, d_lag.as<float>(), mi.is_compact ? d_index.as<uint32_t>() : nullptr,
                                mi.has_bounds() ? d_bounds.as<uint32_t>() : nullptr, W, H, m, d_tf.as<float>(), &r,
-                               g_form, d_rgba.as<float>(), nullptr, nullptr),
+                               params.kernel_form, d_rgba.as<float>(), nullptr, nullptr),
           "reconstruct");
     rgba.resize(npix);
     d_rgba.download(rgba.data(), npix * sizeof(Vec4f));
 }
 } // namespace
-
-void set_kernel_form(int form)
-{
-    if (form != MRAY_FORM_PLAIN && form != MRAY_FORM_MIXED)
-        throw invalid_args("kernel form must be MRAY_FORM_PLAIN or MRAY_FORM_MIXED");
-    g_form = form;
-}
-
-int get_kernel_form() { return g_form; }
 
 void generate(cut::Camera cam, const RegularGrid &volume, MomentImageHost &img, const GenerationParameters &params)
 {

@@ -24,6 +24,7 @@
 
 #include "../include/moment_image_io.h"
 #include "../include/moment_raymarcher.h"
+#include "../../../include/mray_types.h"
 
 using std::unordered_map;
 
@@ -66,18 +67,15 @@ const string &need(unordered_map<string, string> &p, const char *key)
     return it->second;
 }
 
-void apply_form(unordered_map<string, string> &p)
+// the form=plain|mixed option: the kernel form of this run
+int parse_form(unordered_map<string, string> &p)
 {
     auto it = p.find("form");
-    if (it != p.end())
-    {
-        if (it->second == "plain")
-            set_kernel_form(0);
-        else if (it->second == "mixed")
-            set_kernel_form(1);
-        else
-            throw invalid_args("form must be plain or mixed");
-    }
+    if (it == p.end() || it->second == "mixed")
+        return MRAY_FORM_MIXED;
+    if (it->second == "plain")
+        return MRAY_FORM_PLAIN;
+    throw invalid_args("form must be plain or mixed");
 }
 
 RegularGrid load_raw(const string &file, unordered_map<string, string> &p)
@@ -109,12 +107,12 @@ RegularGrid load_raw(const string &file, unordered_map<string, string> &p)
 
 void run_generate(const string &mi_file, const string &dataset, unordered_map<string, string> p)
 {
-    apply_form(p);
     const int width = std::stoi(need(p, "width")), height = std::stoi(need(p, "height"));
     const int num_moments = std::stoi(need(p, "num_moments"));
     if (num_moments > 255)
         throw invalid_args("Number of moments cannot be greater than 255");
     auto gen_params = parse_gen_params(p);
+    gen_params.kernel_form = parse_form(p);
     auto cam = cut::load_from_file(need(p, "camera"), width, height);
     MomentImageHost mi(width, height, num_moments, gen_params.compact_image, gen_params.transform_to_bounds);
     auto volume = load_raw(dataset, p);
@@ -133,8 +131,8 @@ void run_generate(const string &mi_file, const string &dataset, unordered_map<st
 
 void run_reconstruct(const string &mi_file, const string &output, unordered_map<string, string> p)
 {
-    apply_form(p);
     auto rec_params = parse_rec_params(p);
+    rec_params.kernel_form = parse_form(p);
     float factor = 1.f;
     if (p.count("preintegration_step_size_factor"))
         factor = std::stof(p["preintegration_step_size_factor"]);

@@ -730,13 +730,14 @@ void orc_prepare_moments_std(int m, const float *moments, float bias, float *pmo
 /* ------------------------------------------------------------------------ */
 /* bounded MESE, FP64 (same algorithm, double arithmetic, exact functions)   */
 /* ------------------------------------------------------------------------ */
-void orc_prepare_moments_std_f64(int m, const double *moments, double bias, double *out)
+/* trigonometricToExponentialMoments (src/mese/MESE_dynamic.h:35-54) in
+ * double: gamma_0 = e^{i(pi t_0 - pi/2)} / (4 pi), then the power series of
+ * exp(2 pi i sum_{l>=1} t_l z^l), gamma_l = (2 pi i / l) sum_{j<l} (l - j)
+ * t_{l-j} gamma_j, and gamma_0 doubled at the end. exp_c: m interleaved
+ * complex doubles. */
+void orc_trig_to_exp_f64(int m, const double *t, double *exp_c)
 {
-    cd E[MAXM], P[MAXM], R[MAXM], tmp[MAXM];
-    double t[MAXM];
-    t[0] = (1 - bias) * moments[0] + bias * 0.5;
-    for (int i = 1; i < m; ++i)
-        t[i] = (1 - bias) * moments[i];
+    cd *E = (cd *)exp_c;
     double ph = M_PI * t[0] - M_PI / 2;
     E[0] = cd_rmul(1.0 / (4.0 * M_PI), cd_make(cos(ph), sin(ph)));
     for (int l = 1; l < m; ++l)
@@ -747,7 +748,17 @@ void orc_prepare_moments_std_f64(int m, const double *moments, double bias, doub
         E[l] = cd_mul(acc, cd_make(0.0, 2.0 * M_PI / l));
     }
     E[0] = cd_rmul(2.0, E[0]);
-    /* Levinson with the reference's invalid guard */
+}
+
+/* levinsonsAlgorithm (src/mese/MESE_dynamic.h:100-135) in double, with the
+ * reference's guard: once |u|^2 >= 1 that and every later reflection
+ * coefficient is 0. Solves T a = e_0 for the Hermitian Toeplitz matrix whose
+ * first column is first_col (diagonal Re(first_col_0)). */
+void orc_levinson_f64(int m, const double *first_col_c, double *sol_c)
+{
+    const cd *E = (const cd *)first_col_c;
+    cd *P = (cd *)sol_c;
+    cd tmp[MAXM];
     P[0] = cd_make(1.0 / E[0].x, 0.0);
     int invalid = 0;
     for (int j = 1; j < m; ++j)
@@ -772,24 +783,65 @@ void orc_prepare_moments_std_f64(int m, const double *moments, double bias, doub
             P[k] = cd_rmul(f, cd_sub(P[k], cd_mul(tmp[k], u)));
         P[j] = cd_rmul(f, cd_rmul(-tmp[j].x, u));
     }
-    for (int i = 0; i < m; ++i)
-        P[i] = cd_rmul(2.0 * M_PI, P[i]);
+}
+
+/* computeAutocorrelation (src/mese/MESE_dynamic.h:137-147) in double:
+ * out_k = sum_j s_j conj(s_{j+k}) */
+void orc_autocorrelation_f64(int m, const double *sig_c, double *out_c)
+{
+    const cd *s = (const cd *)sig_c;
+    cd *R = (cd *)out_c;
     for (int k = 0; k < m; ++k)
     {
         cd acc = cd_make(0, 0);
         for (int j = 0; j < m - k; ++j)
-            acc = cd_add(acc, cd_mul(P[j], cd_conj(P[j + k])));
+            acc = cd_add(acc, cd_mul(s[j], cd_conj(s[j + k])));
         R[k] = acc;
     }
-    E[0] = cd_rmul(0.5, E[0]);
-    double nf = 1.0 / (M_PI * P[0].x);
+}
+
+/* computeImaginaryCorrelation (src/mese/MESE_dynamic.h:161-171) in double:
+ * out_k = Im(sum_j lhs_{k+j} rhs_j) */
+void orc_imag_correlation_f64(int m, const double *lhs_c, const double *rhs_c, double *out)
+{
+    const cd *L = (const cd *)lhs_c, *Rr = (const cd *)rhs_c;
     for (int k = 0; k < m; ++k)
     {
         double acc = 0.0;
         for (int j = 0; j < m - k; ++j)
-            acc += R[k + j].x * E[j].y + R[k + j].y * E[j].x;
-        out[k] = nf * acc;
+            acc += L[k + j].x * Rr[j].y + L[k + j].y * Rr[j].x;
+        out[k] = acc;
     }
+}
+
+/* prepareMESELagrange, real overload (src/mese/MESE_dynamic.h:194-211), in
+ * double: exponential moments, Levinson, scale by 2 pi, autocorrelation,
+ * gamma_0 halved, imaginary correlation, normalised by 1 / (pi P_0). */
+void orc_prepare_mese_f64(int m, const double *t, double *out)
+{
+    cd E[MAXM], P[MAXM], R[MAXM];
+    orc_trig_to_exp_f64(m, t, (double *)E);
+    orc_levinson_f64(m, (const double *)E, (double *)P);
+    for (int i = 0; i < m; ++i)
+        P[i] = cd_rmul(2.0 * M_PI, P[i]);
+    orc_autocorrelation_f64(m, (const double *)P, (double *)R);
+    E[0] = cd_rmul(0.5, E[0]);
+    orc_imag_correlation_f64(m, (const double *)R, (const double *)E, out);
+    double nf = 1.0 / (M_PI * P[0].x);
+    for (int k = 0; k < m; ++k)
+        out[k] = nf * out[k];
+}
+
+/* prepare_moments_from_std_coding (src/moment_preparation.h:55-64) in
+ * double: the bias lerp (m_0 towards 0.5, the others towards 0), then
+ * prepareMESELagrange */
+void orc_prepare_moments_std_f64(int m, const double *moments, double bias, double *out)
+{
+    double t[MAXM];
+    t[0] = (1 - bias) * moments[0] + bias * 0.5;
+    for (int i = 1; i < m; ++i)
+        t[i] = (1 - bias) * moments[i];
+    orc_prepare_mese_f64(m, t, out);
 }
 
 /* fast_atan (src/mese/complex_algebra.h:131-152) evaluated in double: the

@@ -74,7 +74,13 @@ float orc_evaluate_fourier(int m, float cx, float cy, const float *coeff);
 float orc_evaluate_mese(int m, float phase, const float *lagrange);
 void orc_prepare_moments_std(int m, const float *moments, float bias, float *pmoments);
 
-/* ---- bounded MESE, FP64 ---- */
+/* ---- bounded MESE, FP64 ----
+ * complex arrays are interleaved (re, im) double pairs. */
+void orc_trig_to_exp_f64(int m, const double *trig, double *exp_c);
+void orc_levinson_f64(int m, const double *first_col_c, double *sol_c);
+void orc_autocorrelation_f64(int m, const double *sig_c, double *out_c);
+void orc_imag_correlation_f64(int m, const double *lhs_c, const double *rhs_c, double *out);
+void orc_prepare_mese_f64(int m, const double *trig, double *lagrange);
 void orc_prepare_moments_std_f64(int m, const double *moments, double bias, double *pmoments);
 double orc_evaluate_mese_f64(int m, double phase, const double *lagrange);
 

@@ -349,6 +349,11 @@ def test_pipeline_mixed_vs_f64(hip, O, name):
     if rmse32 <= 1e-4:
         assert rmse <= 1e-4
     else:
+        # the sharp mixing-layer cases at 50 moments: the FP32 reference
+        # pipeline is measured 1.5e-3 (mixw) and 2.2e-3 (mix) RMSE from the
+        # FP64 oracle on the CPU, whose own functions are pinned independently
+        # (tests/test_oracle_f64.py); the relaxed bar applies only that far
+        assert rmse32 <= 4e-3, f"FP32 oracle {rmse32:.2e} from the FP64 oracle, beyond its measured distance"
         assert rmse <= rmse32
 
 
