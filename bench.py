@@ -39,8 +39,12 @@ def parse():
     ap.add_argument("--res", type=int, default=1024, help="image edge")
     ap.add_argument("--moments", type=int, default=50)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-row-step", type=int, default=64, help="CPU baseline: every k-th row")
+    ap.add_argument("--cpu-row-step", type=int, default=16, help="CPU baseline: every k-th row")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs, the median is reported")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision-row-step", type=int, default=64,
+                    help="image RMSE against the FP64 oracle on every k-th row")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-dvr", action="store_true", help="skip the reference direct volume renderer")
     ap.add_argument("--no-quant", action="store_true", help="skip the moment-image quantization")
@@ -193,6 +197,10 @@ def main():
     total_flop = f_gen + f_rec + f_prep
 
     adjoint = None if args.no_adjoint else adjoint_ratio(args.steps)
+    adjoint_c1 = None
+    if not args.no_adjoint:
+        adjoint_c1 = adjoint_frame(vol, vox, cam, dmin, dmax, gp, rp, table, W, H, m, args.steps,
+                                   f"the timed frame ({n}^3 mixing layer, {W}x{H} rays, {m} moments)")
 
     # the reference direct volume renderer (the ground truth of the moment
     # images) on the same volume, camera, table and step: one frame per step
@@ -259,9 +267,43 @@ def main():
                                  "ms": round((time.perf_counter() - t0) * 1e3, 2), "table": curve}
         del cdata, cindex
 
-    cpu = None
+    # images of both forms at the bench's MESE bias and at the reference
+    # eval's 5e-5 (eval/marschner_lobb/marschner_lobb.json), for the precision
+    # figures against the FP64 oracle
+    gpu_images = {}
+    if not args.no_precision:
+        for form in ("mixed", "plain"):
+            hip.generate(vol, cam, gp, W, H, m, form=form, moments=mo, bounds=bnd)
+            for bias in (rp.bias, 5e-5):
+                hip.prepare(mo, W, H, m, bias, form=form, out=lag)
+                img, st = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table,
+                                          RecParams(step_size=step, bias=bias, early_ray_termination=0.99,
+                                                    medium_emission=1.0), form=form, bounds=bnd, want_steps=True)
+                gpu_images[(form, bias)] = (img.cpu().numpy(), st.cpu().numpy())
+        torch.cuda.synchronize()
+
+    cpu, fp32_rows = None, None
+    vol_np = None
+    if not args.no_cpu or not args.no_precision:
+        print("[bench] copying the volume to the host", file=sys.stderr, flush=True)
+        vol_np = vox.cpu().numpy()
     if not args.no_cpu:
-        cpu = cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step, tf_factor)
+        cpu, rows32 = cpu_baseline(vol_np, dmin, dmax, cam, W, H, m, gp, rp, args.cpu_row_step, tf_factor,
+                                   runs=args.cpu_runs)
+        if args.precision_row_step % args.cpu_row_step == 0:
+            fp32_rows = rows32[:2]
+    precision = None
+    if not args.no_precision:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle_ffi import Oracle
+
+        O = Oracle()
+        if fp32_rows is None:
+            mo32, b32, _ = O.generate(vol_np, dmin, dmax, cam, W, H, m, gp, row0=0, row_step=args.precision_row_step)
+            fp32_rows = (mo32, b32)
+        table_np = tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size * tf_factor)
+        precision = precision_vs_fp64(O, vol_np, dmin, dmax, cam, W, H, m, gp, rp, table_np, gpu_images, fp32_rows,
+                                      args.precision_row_step)
 
     line = {
         "metric": "Mrays/s gen+reconstruct, mixed vs plain form at RMSE<=1e-4; adjoint/forward",
@@ -291,8 +333,10 @@ def main():
         "roofline_frame": {"achieved_TFLOPs": round(total_flop / (ms_step * 1e-3) / 1e12, 3),
                            "frac": round(total_flop / (ms_step * 1e-3) / PEAK_FP32_VALU, 4)},
         "adjoint": adjoint,
+        "adjoint_frame": adjoint_c1,
         "reference_dvr": dvr,
         "quantization": quant,
+        "rmse_vs_fp64": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -362,31 +406,144 @@ def adjoint_ratio(steps):
             "table_adjoint_over_forward": round(ms_tf / ms_fwd, 3)}
 
 
-def cpu_baseline(vox, dmin, dmax, cam, W, H, m, gp, rp, row_step, tf_factor):
-    """The C oracle (reference FP32 arithmetic, OpenMP over rows) on every
-    row_step-th row of the same frame; extrapolated to Mrays/s."""
+def host_info():
+    """CPU model (lscpu) and the cores this process may run on."""
+    import subprocess
+
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
+
+
+def adjoint_frame(vol, vox, cam, dmin, dmax, gp, rp, table, W, H, m, steps, workload):
+    """Adjoint / forward time on a frame of the timed workload: forward =
+    generate + prepare + reconstruct (mixed), adjoint = reconstruct reverse +
+    prepare reverse + generate reverse to the voxels (a gradient volume of the
+    volume's size, zeroed per pass)."""
+    import torch
+
+    from mray_amd import hip
+
+    print(f"[bench] adjoint on {workload}", file=sys.stderr, flush=True)
+    g = torch.randn((H, W, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
+    gvol = torch.zeros_like(vox)
+    rr = torch.empty((H * W, 2), dtype=torch.float32, device="cuda")
+
+    def fwd():
+        mo, b, _ = hip.generate(vol, cam, gp, W, H, m, form="mixed", ray_range=rr)
+        lag = hip.prepare(mo, W, H, m, rp.bias, form="mixed")
+        rgba, _ = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form="mixed", bounds=b)
+        return mo, b, lag, rgba
+
+    def adj(mo, b, lag, rgba):
+        g_lag, _ = hip.reconstruct_backward(cam, dmin, dmax, lag, W, H, m, table, rp, rgba, g, bounds=b)
+        g_mo = hip.prepare_backward(mo, W, H, m, rp.bias, g_lag)
+        gvol.zero_()
+        hip.generate_backward(vol, cam, gp, W, H, m, g_mo, out=gvol, ray_range=rr)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    state = fwd()
+    adj(*state)
+    k = max(2, steps)
+    ms_fwd = timed(fwd, k)
+    ms_adj = timed(lambda: adj(*state), k)
+    del gvol
+    return {"workload": workload, "forward_ms": round(ms_fwd, 3), "adjoint_ms": round(ms_adj, 3),
+            "adjoint_over_forward": round(ms_adj / ms_fwd, 3)}
+
+
+def cpu_baseline(vol, dmin, dmax, cam, W, H, m, gp, rp, row_step, tf_factor, runs=3):
+    """The C oracle (the reference CPU path's loops and FP32 arithmetic, OpenMP
+    over rows with the default static schedule) on every row_step-th row of the
+    same frame, extrapolated to Mrays/s: the median of `runs` runs. Threads:
+    OMP_NUM_THREADS, the host cores this job is given (else every core it may
+    run on). Returns the baseline and the FP32 moments, bounds and image of the
+    sampled rows (for the precision figures)."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_ffi import Oracle
     from mray_amd import synth, tf as tfmod
 
     O = Oracle()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     os.environ["MRAY_ORACLE_THREADS"] = str(threads)
-    print(f"[bench] cpu baseline: copying volume to host, {threads} threads, every {row_step}th row",
+    info = host_info()
+    print(f"[bench] cpu baseline: {threads} threads ({info['cpu_model']}), every {row_step}th row, {runs} runs",
           file=sys.stderr, flush=True)
-    vol = vox.cpu().numpy()
     table = tfmod.table_from_rgba8(synth.tf_ramps(0), rp.step_size * tf_factor)
-    t0 = time.perf_counter()
-    mo, b, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp, row0=0, row_step=row_step)
-    t1 = time.perf_counter()
-    rgba, _ = O.reconstruct(cam, dmin, dmax, W, H, m, mo.reshape(-1), table, rp, bounds=b, row0=0,
-                            row_step=row_step)
-    t2 = time.perf_counter()
+    t_gen, t_rec = [], []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        mo, b, _ = O.generate(vol, dmin, dmax, cam, W, H, m, gp, row0=0, row_step=row_step)
+        t1 = time.perf_counter()
+        rgba, st = O.reconstruct(cam, dmin, dmax, W, H, m, mo.reshape(-1), table, rp, bounds=b, row0=0,
+                                 row_step=row_step)
+        t2 = time.perf_counter()
+        t_gen.append(t1 - t0)
+        t_rec.append(t2 - t1)
+        print(f"[bench] cpu baseline run {r + 1}/{runs}: generate {t1 - t0:.2f} s, prepare+reconstruct "
+              f"{t2 - t1:.2f} s", file=sys.stderr, flush=True)
+    tot = [a + b_ for a, b_ in zip(t_gen, t_rec)]
+    k = int(np.argsort(tot)[len(tot) // 2])
     rows = (H + row_step - 1) // row_step
     rays = rows * W
-    return {"value": round(rays / (t2 - t0) / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"rows 0,{row_step},... ({rays} of {W * H} rays) of the same frame; generate "
-                      f"{t1 - t0:.1f} s + prepare/reconstruct {t2 - t1:.1f} s"}
+    base = {"value": round(rays / tot[k] / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"rows 0,{row_step},... ({rays} of {W * H} rays) of the same frame, median of {runs} runs "
+                      f"(generate {t_gen[k]:.2f} s + prepare/reconstruct {t_rec[k]:.2f} s); "
+                      f"{threads} OpenMP threads",
+            "omp_max_threads": int(O.lib.orc_omp_max_threads()), **info}
+    return base, (mo, b, rgba, st)
+
+
+def precision_vs_fp64(O, vol, dmin, dmax, cam, W, H, m, gp, rp, table_np, gpu_images, fp32, row_step):
+    """Float RGBA RMSE against the FP64 oracle (the reference's discretisation
+    in FP64 with exact functions) on every row_step-th row of the frame, for
+    the GPU images given per (form, bias) and for the FP32 oracle; rays whose
+    early-termination step count differs from the FP64 march (a flip on
+    rounding at the threshold) are counted apart."""
+    from mray_amd.types import RecParams
+
+    rows = np.arange(0, H, row_step)
+    print(f"[bench] precision: FP64 oracle on {len(rows)} rows", file=sys.stderr, flush=True)
+    mo64, b64 = O.generate(vol, dmin, dmax, cam, W, H, m, gp, f64=True, row0=0, row_step=row_step)
+    mo32, b32 = fp32
+    out = {"rows": f"0,{row_step},... ({len(rows) * W} rays)"}
+    for bias in sorted({k[1] for k in gpu_images}, reverse=True):
+        rpb = RecParams(step_size=rp.step_size, bias=bias, early_ray_termination=rp.early_ray_termination,
+                        medium_emission=rp.medium_emission)
+        r64, s64 = O.reconstruct_f64(cam, dmin, dmax, W, H, m, mo64, table_np, rpb, bounds=b64, row0=0,
+                                     row_step=row_step)
+        r32, s32 = O.reconstruct(cam, dmin, dmax, W, H, m, mo32.reshape(-1), table_np, rpb, bounds=b32, row0=0,
+                                 row_step=row_step)
+        ref = r64[rows].reshape(-1, 4)
+        st64 = s64.reshape(H, W)[rows].reshape(-1)
+        entry = {}
+        imgs = dict(gpu_images)
+        imgs[("fp32_oracle", bias)] = (r32, s32)
+        for (form, bb), (img, st) in imgs.items():
+            if bb != bias:
+                continue
+            d = img[rows].reshape(-1, 4).astype(np.float64) - ref
+            same = st.reshape(H, W)[rows].reshape(-1) == st64
+            entry[form] = {"rmse": float(f"{np.sqrt((d[same] ** 2).mean()):.3e}"),
+                           "max": float(f"{np.abs(d[same]).max():.3e}"),
+                           "rmse_all_rays": float(f"{np.sqrt((d ** 2).mean()):.3e}"),
+                           "ert_flips": int(np.count_nonzero(~same))}
+        out[f"bias_{bias:g}"] = entry
+    return out
 
 
 if __name__ == "__main__":

@@ -494,6 +494,9 @@ void orc_generate_image_f64(const orc_volume *v, const mray_camera *cam, int W, 
 /* bounded MESE, FP32 (src/mese/MESE_dynamic.h, src/mese/complex_algebra.h) */
 /* ------------------------------------------------------------------------ */
 
+/* threads an OpenMP parallel region of this library gets by default */
+int orc_omp_max_threads(void) { return omp_get_max_threads(); }
+
 /* fast_atan (src/mese/complex_algebra.h:131-152) */
 float orc_fast_atan(float tan)
 {
@@ -1206,12 +1209,24 @@ void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], cons
                                const double *moments, const uint32_t *bounds, const float *tf,
                                const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
 {
+    orc_reconstruct_rows_f64(cam, bmin, bmax, W, H, m, moments, bounds, tf, rp, rgba, n_steps, 0, 1, nthreads);
+}
+
+void orc_reconstruct_rows_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                              const double *moments, const uint32_t *bounds, const float *tf,
+                              const mray_rec_params *rp, double *rgba, int32_t *n_steps, int row0, int row_step,
+                              int nthreads)
+{
     double tfd[1024];
     for (int i = 0; i < 1024; ++i)
         tfd[i] = tf[i];
+    if (row_step < 1)
+        row_step = 1;
+    const int nrows = (H - row0 + row_step - 1) / row_step;
 #pragma omp parallel for num_threads(set_threads(nthreads))
-    for (int y = 0; y < H; ++y)
+    for (int r = 0; r < nrows; ++r)
     {
+        const int y = row0 + r * row_step;
         double pm[MAXM];
         for (int x = 0; x < W; ++x)
         {

@@ -106,6 +106,12 @@ void orc_reconstruct_image(const mray_camera *cam, const float bmin[3], const fl
 void orc_reconstruct_image_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
                                const double *moments, const uint32_t *bounds, const float *tf,
                                const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);
+/* orc_reconstruct_image_f64 on rows row0, row0 + row_step, ... only */
+void orc_reconstruct_rows_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                              const double *moments, const uint32_t *bounds, const float *tf,
+                              const mray_rec_params *rp, double *rgba, int32_t *n_steps, int row0, int row_step,
+                              int nthreads);
+int orc_omp_max_threads(void);
 void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
                                   int m, const float *lagrange, const uint32_t *bounds, const float *tf,
                                   const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);

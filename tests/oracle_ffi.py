@@ -149,14 +149,16 @@ class Oracle:
                                      ptr(ns, _i32), row0, row_step, nthreads())
         return rgba, ns
 
-    def reconstruct_f64(self, cam, dmin, dmax, W, H, m, moments64, tf, rp: RecParams, bounds=None):
+    def reconstruct_f64(self, cam, dmin, dmax, W, H, m, moments64, tf, rp: RecParams, bounds=None, row0=0,
+                        row_step=1):
         c = Camera.from_dict(cam)
         rgba = np.zeros((H, W, 4), np.float64)
         ns = np.zeros(W * H, np.int32)
         bmin = (C.c_float * 3)(*dmin)
         bmax = (C.c_float * 3)(*dmax)
-        self.lib.orc_reconstruct_image_f64(C.byref(c), bmin, bmax, W, H, m, ptr(moments64, _d), ptr(bounds, _u32),
-                                           ptr(tf, _f), C.byref(rp), ptr(rgba, _d), ptr(ns, _i32), nthreads())
+        self.lib.orc_reconstruct_rows_f64(C.byref(c), bmin, bmax, W, H, m, ptr(moments64, _d), ptr(bounds, _u32),
+                                          ptr(tf, _f), C.byref(rp), ptr(rgba, _d), ptr(ns, _i32), row0, row_step,
+                                          nthreads())
         return rgba, ns
 
     def reconstruct_lagrange_f64(self, cam, dmin, dmax, W, H, m, lag32, tf, rp: RecParams, bounds=None):
