@@ -183,6 +183,17 @@ int mray_hip_compute_num_moments(const float *d_moments, const uint32_t *d_bound
  * entries). Uses stream-ordered scratch (hipMallocAsync). */
 int mray_hip_moment_index(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream);
 
+/* mray_hip_moment_index without allocating anything. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc). Same
+ * result as mray_hip_moment_index. Its scratch (the scan's tile sums, 4
+ * bytes per 4096 pixels) is a 64 KiB device array per GPU that the library
+ * holds until the process ends, so W*H is limited to 67,108,864 pixels
+ * (8192 x 8192; larger sizes return MRAY_ERR_ARG). All calls on one GPU share
+ * that array: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller. */
+int mray_hip_moment_index_noalloc(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream);
+
 /* Copies the first count(pixel) moments of each pixel of a dense image into
  * the compact layout (compaction_copy_kernel, src/moment_image_device.cu:65-81;
  * MomentImageHost::compact, src/moment_image.cpp:69-94). d_compact holds

@@ -89,6 +89,9 @@ int mray_hip_init(int device)
     if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess ||
         hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
         return fail(MRAY_ERR_HIP, "mray_hip_init: setting the memory pool's release threshold failed");
+    // the device address of the allocation-free entry points' scratch array
+    if (mray::noalloc_scratch_init(device, nullptr) != 0)
+        return fail(MRAY_ERR_HIP, "mray_hip_init: hipGetSymbolAddress of the scratch array failed");
     g_init = true;
     return MRAY_OK;
 }
@@ -322,6 +325,23 @@ int mray_hip_moment_index(const uint32_t *d_counts, int W, int H, uint32_t *d_in
     if (rc == MRAY_ERR_OOM)
         return fail(rc, "mray_hip_moment_index: scratch allocation failed");
     return check_status(rc, "mray_hip_moment_index");
+}
+
+int mray_hip_moment_index_noalloc(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_moment_index_noalloc: call mray_hip_init first");
+    if (!d_counts || !d_index)
+        return fail(MRAY_ERR_ARG, "mray_hip_moment_index_noalloc: null argument");
+    if (bad_image(W, H, 1))
+        return fail(MRAY_ERR_ARG, "mray_hip_moment_index_noalloc: bad image size %dx%d", W, H);
+    if ((size_t)W * H > mray::noalloc_max_pixels())
+        return fail(MRAY_ERR_ARG, "mray_hip_moment_index_noalloc: %dx%d pixels exceed its scratch limit of %zu pixels",
+                    W, H, mray::noalloc_max_pixels());
+    int rc = mray::launch_moment_index_noalloc(d_counts, W * H, d_index, (hipStream_t)stream);
+    if (rc == MRAY_ERR_HIP)
+        return fail(rc, "mray_hip_moment_index_noalloc: resolving the scratch array's device address failed");
+    return check_status(rc, "mray_hip_moment_index_noalloc");
 }
 
 int mray_hip_compact(const float *d_dense, const uint32_t *d_index, int W, int H, int m, float *d_compact,

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include <algorithm>
@@ -639,6 +640,62 @@ int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipSt
     hipLaunchKernelGGL(scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, stream, sums, ntiles, index, npix);
     hipLaunchKernelGGL(scan_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, stream, counts, npix, sums, index);
     return hipFreeAsync(sums, stream) == hipSuccess ? 0 : MRAY_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// the index scan without allocating: its tile sums live in a file-scope device
+// array, one copy per GPU, held until the process ends. Calls on one GPU share
+// it, so calls that overlap on one GPU race; every call writes each sum it
+// reads before reading it, so nothing has to start at zero.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxTiles = 16384; // 64 KiB: images up to 16384 * 4096 pixels
+static __device__ __align__(256) uint32_t scratch[kNoAllocMaxTiles];
+
+namespace
+{
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+// The device address of scratch on `device`, which must be current: resolved
+// by hipGetSymbolAddress once per device (the array's name is a host-side
+// stand-in, never a device address).
+int noalloc_scratch_init(int device, uint32_t **out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return MRAY_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device])
+    {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+            return MRAY_ERR_HIP;
+        g_scratch_ptr[device] = static_cast<uint32_t *>(ptr);
+    }
+    if (out)
+        *out = g_scratch_ptr[device];
+    return 0;
+}
+
+size_t noalloc_max_pixels() { return (size_t)kNoAllocMaxTiles * kScanTile; }
+
+int launch_moment_index_noalloc(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream)
+{
+    const int ntiles = (npix + kScanTile - 1) / kScanTile;
+    if (ntiles > kNoAllocMaxTiles)
+        return MRAY_ERR_ARG;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess)
+        return MRAY_ERR_HIP;
+    uint32_t *sums = nullptr;
+    const int rc = noalloc_scratch_init(device, &sums);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(scan_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, stream, counts, npix, sums);
+    hipLaunchKernelGGL(scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, stream, sums, ntiles, index, npix);
+    hipLaunchKernelGGL(scan_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, stream, counts, npix, sums, index);
+    return 0;
 }
 
 int launch_compact(const float *dense, const uint32_t *index, int W, int H, int m, float *out, hipStream_t stream)

@@ -41,6 +41,9 @@ int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, i
 int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H, int m, float error_threshold,
                        uint32_t *counts, hipStream_t stream);
 int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);
+int launch_moment_index_noalloc(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);
+int noalloc_scratch_init(int device, uint32_t **out);
+size_t noalloc_max_pixels();
 int launch_compact(const float *dense, const uint32_t *index, int W, int H, int m, float *out, hipStream_t stream);
 int launch_prediction_encode(float *data, const uint32_t *index, int W, int H, int m, int warp, float bias,
                              uint16_t *qparams, hipStream_t stream);

@@ -39,6 +39,7 @@ _SIGNATURES = {
     "mray_hip_prepare_backward": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp], C.c_int),
     "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_moment_index_noalloc": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_prediction_encode": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_prepare_coded": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
@@ -283,6 +284,14 @@ def moment_index(counts: torch.Tensor, W: int, H: int, out=None, stream=None) ->
         out = torch.empty(H * W + 1, dtype=torch.int32, device=counts.device)
     _check(load_library().mray_hip_moment_index(_ptr(counts), W, H, _ptr(out), _stream(stream)),
            "mray_hip_moment_index")
+    return out
+
+
+def moment_index_noalloc(counts: torch.Tensor, W: int, H: int, out: torch.Tensor, stream=None) -> torch.Tensor:
+    """moment_index into a caller-provided (H*W+1,) int32 tensor, with no device
+    allocation by the library (mray_hip_moment_index_noalloc); W*H up to 8192^2."""
+    _check(load_library().mray_hip_moment_index_noalloc(_ptr(counts), W, H, _ptr(out), _stream(stream)),
+           "mray_hip_moment_index_noalloc")
     return out
 
 
