@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-dvr", action="store_true", help="skip the reference direct volume renderer")
     ap.add_argument("--no-quant", action="store_true", help="skip the moment-image quantization")
+    ap.add_argument("--no-large", action="store_true", help="skip the large noise-and-blobs volume")
+    ap.add_argument("--large-only", action="store_true", help="time only the large noise-and-blobs volume")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the adjoint workload and print its figures (for kernel profiles)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -103,6 +105,9 @@ def main():
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_ratio(args.steps)}), flush=True)
         return
+    if args.large_only:
+        print(json.dumps({"large_volume": large_volume(args.steps)}), flush=True)
+        return
     n, W, H, m = args.n, args.res, args.res, args.moments
     dmin, dmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
     cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=25.0, znear=0.01, aspect=W / H)
@@ -134,13 +139,20 @@ def main():
 
     mo = torch.empty((H, W, m), dtype=torch.float32, device="cuda")
     lag = torch.empty_like(mo)
+    # the mixed form keeps its multipliers in FP64 between prepare and march
+    lag64 = torch.empty((H, W, m), dtype=torch.float64, device="cuda")
     bnd = torch.empty(H * W, dtype=torch.int32, device="cuda")
     rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
 
+    def prep(form, bias, out_f32=None):
+        if form == "mixed":
+            return hip.prepare(mo, W, H, m, bias, form="mixed64", out=lag64)
+        return hip.prepare(mo, W, H, m, bias, form=form, out=lag if out_f32 is None else out_f32)
+
     # untimed pass: per-ray sample counts for the algorithmic FLOP count
     _, _, ng = hip.generate(vol, cam, gp, W, H, m, form="mixed", moments=mo, bounds=bnd, want_samples=True)
-    hip.prepare(mo, W, H, m, rp.bias, form="mixed", out=lag)
-    _, nr = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form="mixed", bounds=bnd, rgba=rgba,
+    prep("mixed", rp.bias)
+    _, nr = hip.reconstruct(cam, dmin, dmax, lag64, W, H, m, table, rp, form="mixed", bounds=bnd, rgba=rgba,
                             want_steps=True)
     torch.cuda.synchronize()
     ng, nr = ng.cpu().numpy(), nr.cpu().numpy()
@@ -153,9 +165,9 @@ def main():
             e[0].record()
             hip.generate(vol, cam, gp, W, H, m, form=form, moments=mo, bounds=bnd)
             e[1].record()
-            hip.prepare(mo, W, H, m, rp.bias, form=form, out=lag)
+            lg = prep(form, rp.bias)
             e[2].record()
-            hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form=form, bounds=bnd, rgba=rgba)
+            hip.reconstruct(cam, dmin, dmax, lg, W, H, m, table, rp, form=form, bounds=bnd, rgba=rgba)
             e[3].record()
             if i >= warmup:
                 ev["gen"].append(e)
@@ -275,12 +287,14 @@ def main():
         for form in ("mixed", "plain"):
             hip.generate(vol, cam, gp, W, H, m, form=form, moments=mo, bounds=bnd)
             for bias in (rp.bias, 5e-5):
-                hip.prepare(mo, W, H, m, bias, form=form, out=lag)
-                img, st = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table,
+                lg = prep(form, bias)
+                img, st = hip.reconstruct(cam, dmin, dmax, lg, W, H, m, table,
                                           RecParams(step_size=step, bias=bias, early_ray_termination=0.99,
                                                     medium_emission=1.0), form=form, bounds=bnd, want_steps=True)
                 gpu_images[(form, bias)] = (img.cpu().numpy(), st.cpu().numpy())
         torch.cuda.synchronize()
+
+    large = None if args.no_large else large_volume(args.steps)
 
     cpu, fp32_rows = None, None
     vol_np = None
@@ -319,7 +333,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"mixing layer {n}^3 f32 (resident in HBM), perspective {W}x{H} rays, {m} moments, "
-                               f"step 0.5 voxel; generate + prepare + reconstruct, mixed form",
+                               f"step 0.5 voxel; generate + prepare + reconstruct, mixed form (FP64 multipliers)",
                    "volume": [n, n, n], "image": [W, H], "moments": m, "form": "mixed"},
         "plain": plain,
         "mixed_over_plain": round(mrays / plain["value"], 3) if plain else None,
@@ -335,6 +349,7 @@ def main():
         "adjoint": adjoint,
         "adjoint_frame": adjoint_c1,
         "reference_dvr": dvr,
+        "large_volume": large,
         "quantization": quant,
         "rmse_vs_fp64": precision,
         "cpu_baseline": cpu,
@@ -420,6 +435,97 @@ def host_info():
     except (OSError, subprocess.SubprocessError):
         pass
     return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
+
+
+def large_volume(steps, plain_steps=1):
+    """The large-volume workload: 2048 x 2048 x 1920 uint8 noise-and-blobs
+    volume (7.5 GB, resident), perspective 2048 x 2048 rays, step 1 voxel, 100
+    moments generated, per-pixel counts from the compaction at error
+    threshold 1e-3 clamped to [8, 100], compact preparation and
+    reconstruction. One frame = generate + compaction (counts, clamp, index,
+    compact copy) + prepare + reconstruct."""
+    import torch
+
+    from mray_amd import hip, synth, tf as tfmod
+    from mray_amd.types import GenParams, RecParams
+
+    nz, ny, nx = synth.blob_volume_shape()
+    W = H = 2048
+    m = 100
+    dmin, dmax = (0.0, 0.0, 0.0), (1.0, ny / nx, nz / nx)
+    cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=25.0, znear=0.01, aspect=1.0)
+    step = 1.0 / nx
+    gp = GenParams(step_size=step, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=1,
+                   prediction_coding=0, error_threshold=1e-3, bias=1e-4)
+    rp = RecParams(step_size=step, bias=1e-3, early_ray_termination=0.99, medium_emission=1.0)
+    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), step * 128.0)).cuda()
+    print(f"[bench] large volume: generating {nx}x{ny}x{nz} uint8 on the GPU", file=sys.stderr, flush=True)
+    vox = hip.synth_noise_blobs(nx, ny, nz, seed=2)
+    vol = hip.Volume(vox, dmin, dmax)
+    mo = torch.empty((H, W, m), dtype=torch.float32, device="cuda")
+    bnd = torch.empty(H * W, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    def frame(form, ev=None, want=False):
+        rec = lambda i: ev[i].record() if ev else None
+        rec(0)
+        _, _, ng = hip.generate(vol, cam, gp, W, H, m, form=form, moments=mo, bounds=bnd, want_samples=want)
+        rec(1)
+        data, index = hip.compact_image(mo, W, H, m, gp.error_threshold, bounds=bnd, count_range=(8, 100))
+        rec(2)
+        lag = hip.prepare(data, W, H, m, rp.bias, form="mixed64" if form == "mixed" else form, index=index)
+        rec(3)
+        img, nr = hip.reconstruct(cam, dmin, dmax, lag, W, H, m, table, rp, form=form, index=index, bounds=bnd,
+                                  want_steps=want)
+        rec(4)
+        return ng, nr, index, img
+
+    ng, nr, index, img = frame("mixed", want=True)
+    torch.cuda.synchronize()
+    ng, nr = ng.cpu().numpy().astype(np.float64), nr.cpu().numpy().astype(np.float64)
+    counts = np.diff(index.cpu().numpy().view(np.uint32).astype(np.int64)).astype(np.float64)
+    alpha_mean = float(img[..., 3].mean())
+    del index, img
+
+    def timed(form, k):
+        evs = []
+        t0 = time.perf_counter()
+        for _ in range(k):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+            frame(form, ev)
+            evs.append(ev)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / k
+        parts = {name: round(float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs])), 3)
+                 for i, name in enumerate(("generate", "compaction", "prepare", "reconstruct"))}
+        return wall, parts
+
+    print("[bench] large volume: mixed", file=sys.stderr, flush=True)
+    frame("mixed")
+    ms_mixed, parts = timed("mixed", max(2, steps // 2))
+    out = {"workload": f"noise-and-blobs {nx}x{ny}x{nz} uint8 (resident), perspective {W}x{H} rays, step 1 voxel, "
+                       f"{m} moments generated, compaction 1e-3 clamped to [8, 100], compact prepare + reconstruct",
+           "Mrays_per_s": round(W * H / (ms_mixed * 1e-3) / 1e6, 3), "ms_per_frame": round(ms_mixed, 3),
+           "ms": parts, "mean_moments": round(float(counts[counts > 0].mean()), 2),
+           "empty_pixels": int(np.count_nonzero(counts == 0)),
+           "samples_per_ray": {"generate": round(float(ng.mean()), 1), "reconstruct": round(float(nr.mean()), 1)},
+           "image_alpha_mean": round(alpha_mean, 3)}
+    # contract formula: generate at m, compaction 6 * 20 m^2 per pixel, then
+    # each pixel's own count in prepare and reconstruct
+    flop = {"generate": float(ng.sum()) * (8 * (m - 1) + 30), "compaction": W * H * 6.0 * 20 * m * m,
+            "prepare": float((20.0 * counts * counts).sum()),
+            "reconstruct": float((nr * (8 * (np.maximum(counts, 1) - 1) + 48)).sum())}
+    out["roofline"] = {k: round(flop[k] / (parts[k] * 1e-3) / PEAK_FP32_VALU, 4) for k in flop}
+    out["roofline_frame"] = round(sum(flop.values()) / (ms_mixed * 1e-3) / PEAK_FP32_VALU, 4)
+    if plain_steps > 0:
+        print("[bench] large volume: plain", file=sys.stderr, flush=True)
+        ms_plain, parts_p = timed("plain", plain_steps)
+        out["plain"] = {"Mrays_per_s": round(W * H / (ms_plain * 1e-3) / 1e6, 3), "ms_per_frame": round(ms_plain, 3),
+                        "ms": parts_p}
+        out["mixed_over_plain"] = round(ms_plain / ms_mixed, 3)
+    del vol, vox, mo
+    torch.cuda.empty_cache()
+    return out
 
 
 def adjoint_frame(vol, vox, cam, dmin, dmax, gp, rp, table, W, H, m, steps, workload):

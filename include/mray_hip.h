@@ -56,9 +56,24 @@ int mray_hip_volume_create(const void *host_voxels, int dtype, int nx, int ny, i
 int mray_hip_volume_wrap(const void *d_voxels, int dtype, int nx, int ny, int nz, const float dom_min[3],
                          const float dom_max[3], mray_volume *out);
 
+/* A particle set marched directly (generate(cam, const Particles&, ...),
+ * src/moment_raymarcher.h:48-49): n particles at device positions d_x, d_y,
+ * d_z with values d_v, smoothing length h, ray-marching domain [dom_min,
+ * dom_max] (Particles::domain). The particles are copied into the reference's
+ * uniform grid of cell size h, sorted stably by 4^3-tiled cell index
+ * (Particles::create_grid, src/volume.cpp:55-81); every sample of
+ * mray_hip_generate on this handle is Particles::sample (src/volume.cpp:83-94):
+ * the cubic-spline weighted mean of the values of particles closer than h,
+ * 0 where there are none. Synchronises the device once (the grid is sized
+ * from the particles' bounding box). mray_hip_reference and
+ * mray_hip_generate_backward do not take particle sets. */
+int mray_hip_volume_from_particles(const float *d_x, const float *d_y, const float *d_z, const float *d_v, uint32_t n,
+                                   float h, const float dom_min[3], const float dom_max[3], mray_volume *out);
+
 int mray_hip_volume_destroy(mray_volume vol);
 
-/* Device pointer of the volume's voxels (read-only view). */
+/* Device pointer of the volume's voxels (read-only view); for a particle set
+ * the cell-sorted records (x, y, z, value) as float4. */
 const void *mray_hip_volume_data(mray_volume vol);
 
 /* ---- generate (src/moment_raymarcher.h:48-49) -------------------------- */
@@ -72,7 +87,9 @@ const void *mray_hip_volume_data(mray_volume vol);
  *   d_ray_range  optional W*H x 2 floats: per ray the (lo, hi) the moments
  *                were normalised with (before the 16-bit packing of
  *                d_bounds), for mray_hip_generate_backward; NULL to skip
- * form: MRAY_FORM_PLAIN or MRAY_FORM_MIXED. Compaction and prediction coding
+ * form: MRAY_FORM_PLAIN or MRAY_FORM_MIXED (m up to 255 in both; MIXED splits
+ * more than 64 moments into chunks of at most 56, each marched by its own
+ * workgroups). Compaction and prediction coding
  * are separate calls (mray_hip_compute_num_moments, mray_hip_moment_index,
  * mray_hip_compact, mray_hip_prediction_encode). */
 int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int form, int W, int H,
@@ -124,6 +141,23 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
                          int m, const float *d_tf, const mray_rec_params *p, int form, float *d_rgba,
                          int32_t *d_steps, mray_stream stream);
 
+/* ---- MIXED with FP64 multipliers -------------------------------------- */
+
+/* mray_hip_prepare (MIXED) writing the multipliers in FP64 (d_lagrange:
+ * doubles, the layout of d_moments), and mray_hip_reconstruct (MIXED)
+ * reading them: the pair the reference runs inside reconstruct
+ * (src/moment_raymarcher.cpp:329-353 -> prepare_moments_device +
+ * reconstruct_moments_kernel, src/moment_raymarcher.cu:386-453), without the
+ * float rounding of the multipliers in between. At 50 moments the
+ * multipliers reach |lambda| ~ 1e3 and their float rounding alone moves the
+ * image by ~1e-4 RMSE; these keep it to the float storage of the moments. */
+int mray_hip_prepare_f64(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
+                         double *d_lagrange, mray_stream stream);
+int mray_hip_reconstruct_f64(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                             const double *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W,
+                             int H, int m, const float *d_tf, const mray_rec_params *p, float *d_rgba,
+                             int32_t *d_steps, mray_stream stream);
+
 /* ---- adjoint (no counterpart in the reference, which has no gradients) ---- */
 
 /* Reverse mode of mray_hip_reconstruct (MIXED arithmetic) for
@@ -135,7 +169,8 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
  * O(pixels x moments)); bounds are constants, the transfer-function lookup
  * is differentiated as the reference evaluates it (entries from the front
  * sample, weight from the back sample) and the density through the
- * fast_atan polynomial. Up to 64 moments. */
+ * fast_atan polynomial. Above 64 moments two (up to 128) or four lanes work on
+ * a ray, 64 multipliers each. */
 int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
                                   const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W,
                                   int H, int m, const float *d_tf, const mray_rec_params *p, const float *d_rgba,
@@ -148,7 +183,8 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
  * segments, map_density (where it does not clamp) and the trilinear weights;
  * the per-ray bounds are constants. d_ray_range: the forward's per-ray
  * normalisation (its d_ray_range output), or NULL to recompute it with an
- * extra pass over the samples. F32 volumes, up to 64 moments; float
+ * extra pass over the samples. F32 volumes; above 64 moments the moments are
+ * split into chunks of at most 56 whose passes add their shares; float
  * atomics, so the summation order (and the last bits) vary between runs. */
 int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int W, int H, int m,
                                const float *d_grad_moments, const float *d_ray_range, float *d_grad_volume,
@@ -182,6 +218,15 @@ int mray_hip_compute_num_moments(const float *d_moments, const uint32_t *d_bound
  * (src/moment_image_device.cu:83-110). d_index may be d_counts (W*H+1
  * entries). Uses stream-ordered scratch (hipMallocAsync). */
 int mray_hip_moment_index(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream);
+
+/* Clamps the moment counts of non-empty pixels (count > 0) to [min_moments,
+ * max_moments] in place; empty pixels keep 0. No counterpart in the
+ * reference, whose compute_num_moments has no lower limit: the per-pixel
+ * range of a workload (8 to 100 moments for the large noise-and-blobs
+ * volume), applied between mray_hip_compute_num_moments and
+ * mray_hip_moment_index. The dense moments must hold max_moments per pixel. */
+int mray_hip_clamp_num_moments(uint32_t *d_counts, int W, int H, int min_moments, int max_moments,
+                               mray_stream stream);
 
 /* mray_hip_moment_index without allocating anything. No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
@@ -296,6 +341,15 @@ int mray_hip_synth_mixing_layer(float *d_out, int n, const double *amp, const do
 
 /* Marschner-Lobb on [-1,1]^3 (fM = 6, a = 0.25), n^3 floats into d_out. */
 int mray_hip_synth_marschner_lobb(float *d_out, int n, mray_stream stream);
+
+/* Noise-and-blobs uint8 volume, nx*ny*nz bytes ([z][y][x]) into d_out: at
+ * p = (x, y, z) / nx, v = clamp(0.5 smoothstep(0.3, 0.7, 0.5 + 0.5 noise(p)) +
+ * sum_k amps[k] exp(-|p - c_k|^2 / (2 sigmas[k]^2)), 0, 1), stored as
+ * floor(255 v + 0.5); noise: 6 octaves of hashed value noise from a lattice
+ * of 4 per unit. Host arrays: centres 3 * n_blobs floats (x, y, z), sigmas
+ * and amps n_blobs floats, n_blobs <= 64. */
+int mray_hip_synth_noise_blobs(uint8_t *d_out, int nx, int ny, int nz, const float *centres, const float *sigmas,
+                               const float *amps, int n_blobs, uint64_t seed, mray_stream stream);
 
 #ifdef __cplusplus
 }

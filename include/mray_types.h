@@ -24,11 +24,12 @@ enum mray_status
     MRAY_ERR_NOT_INIT = -5     /* mray_hip_init was not called */
 };
 
-/* Voxel types of RegularGrid (src/volume.h:33-37). */
+/* Voxel types of RegularGrid (src/volume.h:33-37), and particle sets. */
 enum mray_dtype
 {
     MRAY_DTYPE_F32 = 0,
-    MRAY_DTYPE_U8 = 1
+    MRAY_DTYPE_U8 = 1,
+    MRAY_DTYPE_PARTICLES = 2 /* a particle set (Particles, src/volume.h:132-160), mray_hip_volume_from_particles */
 };
 
 /* Kernel forms. PLAIN follows the reference per-sample arithmetic (per-moment

@@ -11,6 +11,7 @@
 
 #include "../../include/mray_hip.h"
 #include "launch.hpp"
+#include "sph_common.hpp"
 
 namespace mray
 {
@@ -150,18 +151,67 @@ int mray_hip_volume_wrap(const void *d_voxels, int dtype, int nx, int ny, int nz
     return MRAY_OK;
 }
 
+int mray_hip_volume_from_particles(const float *d_x, const float *d_y, const float *d_z, const float *d_v, uint32_t n,
+                                   float h, const float dom_min[3], const float dom_max[3], mray_volume *out)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_volume_from_particles: call mray_hip_init first");
+    if (!out || !d_x || !d_y || !d_z || !d_v || !dom_min || !dom_max || n == 0)
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_from_particles: null argument or no particles");
+    if (!(h > 0.0f))
+        return fail(MRAY_ERR_ARG, "mray_hip_volume_from_particles: smoothing length must be > 0");
+    for (int i = 0; i < 3; ++i)
+        if (!(dom_max[i] > dom_min[i]))
+            return fail(MRAY_ERR_ARG, "mray_hip_volume_from_particles: empty domain on axis %d", i);
+    auto *sb = new (std::nothrow) mray::SphBuffers;
+    auto *v = new (std::nothrow) mray_volume_s;
+    if (!sb || !v)
+    {
+        delete sb;
+        delete v;
+        return fail(MRAY_ERR_OOM, "mray_hip_volume_from_particles: out of host memory");
+    }
+    const int rc = mray::sph_build(d_x, d_y, d_z, d_v, n, h, true, *sb, nullptr);
+    if (rc != 0 || hipStreamSynchronize(nullptr) != hipSuccess)
+    {
+        delete sb;
+        delete v;
+        return fail(rc ? rc : MRAY_ERR_HIP, "mray_hip_volume_from_particles: building the cell grid failed (%d)", rc);
+    }
+    v->data = sb;
+    v->owned = true;
+    v->dtype = MRAY_DTYPE_PARTICLES;
+    v->nx = v->ny = v->nz = 1;
+    std::memcpy(v->dmin, dom_min, sizeof(v->dmin));
+    std::memcpy(v->dmax, dom_max, sizeof(v->dmax));
+    (void)hipGetDevice(&v->device);
+    *out = v;
+    return MRAY_OK;
+}
+
 int mray_hip_volume_destroy(mray_volume vol)
 {
     if (!vol)
         return MRAY_OK;
     int rc = MRAY_OK;
-    if (vol->owned && hipFree(const_cast<void *>(vol->data)) != hipSuccess)
+    if (vol->dtype == MRAY_DTYPE_PARTICLES)
+    {
+        auto *sb = static_cast<mray::SphBuffers *>(const_cast<void *>(vol->data));
+        mray::sph_free(*sb, true, nullptr);
+        delete sb;
+    }
+    else if (vol->owned && hipFree(const_cast<void *>(vol->data)) != hipSuccess)
         rc = fail(MRAY_ERR_HIP, "mray_hip_volume_destroy: hipFree failed");
     delete vol;
     return rc;
 }
 
-const void *mray_hip_volume_data(mray_volume vol) { return vol ? vol->data : nullptr; }
+const void *mray_hip_volume_data(mray_volume vol)
+{
+    if (vol && vol->dtype == MRAY_DTYPE_PARTICLES)
+        return static_cast<const mray::SphBuffers *>(vol->data)->parts;
+    return vol ? vol->data : nullptr;
+}
 
 int mray_hip_generate(mray_volume vol, const mray_camera *cam, const mray_gen_params *p, int form, int W, int H,
                       int m, float *d_moments, uint32_t *d_bounds, int32_t *d_samples, float *d_ray_range,
@@ -195,6 +245,8 @@ int mray_hip_reference(mray_volume vol, const mray_camera *cam, const float *d_t
         return fail(MRAY_ERR_ARG, "mray_hip_reference: bad image size %dx%d", W, H);
     if (!(p->step_size > 0.0f) || !(p->data_max != p->data_min))
         return fail(MRAY_ERR_ARG, "mray_hip_reference: step_size must be > 0 and data_max != data_min");
+    if (vol->dtype == MRAY_DTYPE_PARTICLES)
+        return fail(MRAY_ERR_UNSUPPORTED, "mray_hip_reference: regular grids only");
     int rc = mray::launch_reference(vol->data, vol->dtype, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax, *cam, *p,
                                     W, H, d_tf, d_rgba, d_steps, (hipStream_t)stream);
     if (rc == MRAY_ERR_UNSUPPORTED)
@@ -237,6 +289,37 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
     return check_status(rc, "mray_hip_reconstruct");
 }
 
+int mray_hip_prepare_f64(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
+                         double *d_lagrange, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_prepare_f64: call mray_hip_init first");
+    if (!d_moments || !d_lagrange)
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_f64: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_prepare_f64: bad image size %dx%d or moment count %d", W, H, m);
+    int rc = mray::launch_prepare_f64(d_moments, d_index, W, H, m, bias, d_lagrange, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_prepare_f64");
+}
+
+int mray_hip_reconstruct_f64(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                             const double *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W,
+                             int H, int m, const float *d_tf, const mray_rec_params *p, float *d_rgba,
+                             int32_t *d_steps, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_reconstruct_f64: call mray_hip_init first");
+    if (!cam || !dom_min || !dom_max || !d_lagrange || !d_tf || !p || !d_rgba)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_f64: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_f64: bad image size %dx%d or moment count %d", W, H, m);
+    if (!(p->step_size > 0.0f))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_f64: step_size must be > 0");
+    int rc = mray::launch_reconstruct_f64(*cam, dom_min, dom_max, d_lagrange, d_index, d_bounds, W, H, m, d_tf, *p,
+                                          d_rgba, d_steps, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_reconstruct_f64");
+}
+
 int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
                                   const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds, int W,
                                   int H, int m, const float *d_tf, const mray_rec_params *p, const float *d_rgba,
@@ -256,7 +339,7 @@ int mray_hip_reconstruct_backward(const mray_camera *cam, const float dom_min[3]
     int rc = mray::launch_reconstruct_adjoint(*cam, dom_min, dom_max, d_lagrange, d_index, d_bounds, W, H, m, d_tf, *p,
                                               d_rgba, d_grad_rgba, d_grad_lagrange, d_grad_tf, (hipStream_t)stream);
     if (rc == MRAY_ERR_UNSUPPORTED)
-        return fail(rc, "mray_hip_reconstruct_backward: at most 64 moments");
+        return fail(rc, "mray_hip_reconstruct_backward: configuration not supported");
     return check_status(rc, "mray_hip_reconstruct_backward");
 }
 
@@ -278,7 +361,7 @@ int mray_hip_generate_backward(mray_volume vol, const mray_camera *cam, const mr
                                            *cam, *p, W, H, m, d_grad_moments, d_ray_range, d_grad_volume,
                                            (hipStream_t)stream);
     if (rc == MRAY_ERR_UNSUPPORTED)
-        return fail(rc, "mray_hip_generate_backward: at most 64 moments");
+        return fail(rc, "mray_hip_generate_backward: configuration not supported");
     return check_status(rc, "mray_hip_generate_backward");
 }
 
@@ -325,6 +408,20 @@ int mray_hip_moment_index(const uint32_t *d_counts, int W, int H, uint32_t *d_in
     if (rc == MRAY_ERR_OOM)
         return fail(rc, "mray_hip_moment_index: scratch allocation failed");
     return check_status(rc, "mray_hip_moment_index");
+}
+
+int mray_hip_clamp_num_moments(uint32_t *d_counts, int W, int H, int min_moments, int max_moments,
+                               mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_clamp_num_moments: call mray_hip_init first");
+    if (!d_counts)
+        return fail(MRAY_ERR_ARG, "mray_hip_clamp_num_moments: null argument");
+    if (bad_image(W, H, 1) || min_moments < 1 || max_moments < min_moments || max_moments > 255)
+        return fail(MRAY_ERR_ARG, "mray_hip_clamp_num_moments: bad image size %dx%d or range [%d, %d]", W, H,
+                    min_moments, max_moments);
+    int rc = mray::launch_clamp_counts(d_counts, W * H, min_moments, max_moments, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_clamp_num_moments");
 }
 
 int mray_hip_moment_index_noalloc(const uint32_t *d_counts, int W, int H, uint32_t *d_index, mray_stream stream)
@@ -494,6 +591,19 @@ int mray_hip_synth_mixing_layer(float *d_out, int n, const double *amp, const do
         return fail(MRAY_ERR_ARG, "mray_hip_synth_mixing_layer: bad argument");
     int rc = mray::launch_synth_mixing_layer(d_out, n, amp, theta, phi, seed, (hipStream_t)stream);
     return check_status(rc, "mray_hip_synth_mixing_layer");
+}
+
+int mray_hip_synth_noise_blobs(uint8_t *d_out, int nx, int ny, int nz, const float *centres, const float *sigmas,
+                               const float *amps, int n_blobs, uint64_t seed, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_synth_noise_blobs: call mray_hip_init first");
+    if (!d_out || nx <= 0 || ny <= 0 || nz <= 0 || n_blobs < 0 || n_blobs > 64 ||
+        (n_blobs > 0 && (!centres || !sigmas || !amps)))
+        return fail(MRAY_ERR_ARG, "mray_hip_synth_noise_blobs: bad argument");
+    int rc = mray::launch_synth_noise_blobs(d_out, nx, ny, nz, centres, sigmas, amps, n_blobs, seed,
+                                            (hipStream_t)stream);
+    return check_status(rc, "mray_hip_synth_noise_blobs");
 }
 
 int mray_hip_synth_marschner_lobb(float *d_out, int n, mray_stream stream)

@@ -630,6 +630,24 @@ int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H,
     return 0;
 }
 
+// counts of non-empty pixels clamped to [lo, hi]; empty pixels stay empty
+__global__ void clamp_counts_kernel(uint32_t *__restrict__ counts, int npix, uint32_t lo, uint32_t hi)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix)
+        return;
+    const uint32_t c = counts[i];
+    if (c > 0u)
+        counts[i] = c < lo ? lo : (c > hi ? hi : c);
+}
+
+int launch_clamp_counts(uint32_t *counts, int npix, int lo, int hi, hipStream_t stream)
+{
+    hipLaunchKernelGGL(clamp_counts_kernel, dim3((npix + 255) / 256), dim3(256), 0, stream, counts, npix,
+                       (uint32_t)lo, (uint32_t)hi);
+    return 0;
+}
+
 int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream)
 {
     const int ntiles = (npix + kScanTile - 1) / kScanTile;

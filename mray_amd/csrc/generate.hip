@@ -19,18 +19,21 @@
 //         (eps); both enter to first order, exactly enough to reproduce the
 //         reference's arithmetic-free result to ~1e-13:
 //             + sum_k [ -delta_k f_k cos(j p_k) + h_k sin(j p_k) / j ].
-//         A node costs one sincos of its (float) phase and two Chebyshev
-//         recurrences over j (cos and sin of j phi): 5 or 6 VALU ops per
-//         moment (accumulate_node) instead of 2(m-1) sin/cos. The phasor is not carried from node to
-//         node by rotation: the reference's node phases jitter by ulps around
-//         the uniform grid, and a rotated phasor drifts off cos(p_k) by more
-//         than the high orders tolerate.
-//         Accumulation is blocked: kFold nodes go into an FP32 partial sum
-//         that is then added to an FP64 total; trig_0 (the exact integral of
-//         the reference's lines) is accumulated in FP64. Moments live in
-//         registers (template M >= m).
+//         A node costs one FP64 sincos of its (float) phase and two
+//         Chebyshev recurrences over j (cos and sin of j phi): 5 or 6 FP64
+//         FMAs per moment (accumulate_node) instead of 2(m-1) sin/cos; gfx950
+//         issues FP64 FMAs at the rate of unpacked FP32 ones, so the sums,
+//         the recurrences and trig_0 (the exact integral of the reference's
+//         lines) are all FP64 and the moments are rounded to float once. The
+//         phasor is not carried from node to node by rotation: the
+//         reference's node phases jitter by ulps around the uniform grid, and
+//         a rotated phasor drifts off cos(p_k) by more than the high orders
+//         tolerate. Moments live in registers (template M >= m); more than 64
+//         are split into chunks over workgroups (CHUNK).
 #include "device_common.hpp"
 #include "launch.hpp"
+#include "sph_common.hpp"
+#include <type_traits>
 
 namespace mray
 {
@@ -182,70 +185,34 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 // MIXED
 // ---------------------------------------------------------------------------
-constexpr int kFold = 16;   // nodes per FP32 partial sum before the FP64 add
 
-// sin and cos of a node phase p in [-3.3, 0.1]: p = k pi/2 + r with k =
-// rint(2p / pi) in {-2, -1, 0}, r taken off in two float parts of pi/2 (k * hi
-// is exact for these k, so r is within an ulp), then Taylor polynomials on
-// |r| <= pi/4 (truncation below 3e-9 relative), about as accurate as the
-// device library's sincosf for this range and without its large-argument path.
-__device__ __forceinline__ void sincos_phase(float p, float &s, float &c)
-{
-    const float k = rintf(p * 0.636619772f);
-    float r = __builtin_fmaf(-k, 1.57079637f, p);
-    r = __builtin_fmaf(-k, -4.37113900e-8f, r);
-    const float r2 = r * r;
-    float ps = __builtin_fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);
-    ps = __builtin_fmaf(r2, ps, 8.33333333e-3f);
-    ps = __builtin_fmaf(r2, ps, -0.166666667f);
-    const float sr = __builtin_fmaf(r * r2, ps, r);
-    float pc = __builtin_fmaf(r2, -2.75573192e-7f, 2.48015873e-5f);
-    pc = __builtin_fmaf(r2, pc, -1.38888889e-3f);
-    pc = __builtin_fmaf(r2, pc, 4.16666667e-2f);
-    pc = __builtin_fmaf(r2, pc, -0.5f);
-    const float cr = __builtin_fmaf(r2, pc, 1.0f);
-    const int q = (int)k & 3; // 0, 3 (k = -1) or 2 (k = -2)
-    s = (q == 0) ? sr : (q == 3) ? -cr : -sr;
-    c = (q == 0) ? cr : (q == 3) ? sr : -cr;
-}
-
-// The moment's partial sum for j = 1..M-1, scaled by j^2, over a node is
-// (d + j^2 e) cos(j phi) + j h sin(j phi). SPLIT keeps the sine term apart,
-// part[j] += (d + j^2 e) cos(j phi), part_s[j] += h sin(j phi), the sum being
-// part[j] + j part_s[j]: five operations per moment instead of six, for M more
-// registers; taken for the moment counts where that keeps the kernel's
-// wavefronts per SIMD (8, 12, 20, 28 and 40 to 50 on gfx950).
-// cos and sin of j phi by the Chebyshev recurrence v_j = 2x v_{j-1} - v_{j-2}
-// from x = cos(phi), y = sin(phi).
+// One node's terms for moments j = 1..M-1:
+//     sc[j] += (d + j^2 e) cos(j phi)   (FP64)
+//     ss[j] += h sin(j phi)             (FP32)
+// the moment being (sc[j] + j ss[j]) / j^2. cos(j phi) by the Chebyshev
+// recurrence v_j = 2x v_{j-1} - v_{j-2} in FP64 from x = cos(phi) (gfx950
+// issues FP64 FMAs at the rate of unpacked FP32 ones); the sine terms carry
+// only the first-order corrections of the reference's ulp offsets (h is a few
+// ulps of the slopes), so their recurrence from y = sin(phi) and their sums
+// are FP32: five operations and three registers per moment.
 template <int M>
-constexpr bool kSplitSine = M == 8 || M == 12 || M == 20 || M == 28 || (M >= 40 && M <= 50);
-
-template <int M>
-__device__ __forceinline__ void accumulate_node(float (&part)[M], float (&part_s)[M], float d, float e, float h,
-                                                float x, float y)
+__device__ __forceinline__ void accumulate_node(double (&sc)[M], float (&ss)[M], double d, double e, float h,
+                                                double x, float y)
 {
-    constexpr bool SPLIT = kSplitSine<M>;
-    const float x2 = x + x;
-    float cm2 = 1.0f, cm1 = x;
+    const double x2 = x + x;
+    const float x2f = (float)x2;
+    double cm2 = 1.0, cm1 = x;
     float sm2 = 0.0f, sm1 = y;
-    part[1] = __builtin_fmaf(d + e, cm1, part[1]);
-    if constexpr (SPLIT)
-        part_s[1] = __builtin_fmaf(h, sm1, part_s[1]);
-    else
-        part[1] = __builtin_fmaf(h, sm1, part[1]);
+    sc[1] = fma(d + e, cm1, sc[1]);
+    ss[1] = __builtin_fmaf(h, sm1, ss[1]);
 #pragma unroll
     for (int j = 2; j < M; ++j)
     {
-        const float c = __builtin_fmaf(x2, cm1, -cm2);
-        const float sn = __builtin_fmaf(x2, sm1, -sm2);
-        const float w = __builtin_fmaf((float)(j * j), e, d);
-        if constexpr (SPLIT)
-        {
-            part[j] = __builtin_fmaf(w, c, part[j]);
-            part_s[j] = __builtin_fmaf(h, sn, part_s[j]);
-        }
-        else
-            part[j] = __builtin_fmaf((float)j * h, sn, __builtin_fmaf(w, c, part[j]));
+        const double c = fma(x2, cm1, -cm2);
+        const float sn = __builtin_fmaf(x2f, sm1, -sm2);
+        const double w = fma((double)(j * j), e, d);
+        sc[j] = fma(w, c, sc[j]);
+        ss[j] = __builtin_fmaf(h, sn, ss[j]);
         cm2 = cm1;
         cm1 = c;
         sm2 = sm1;
@@ -253,24 +220,55 @@ __device__ __forceinline__ void accumulate_node(float (&part)[M], float (&part_s
     }
 }
 
-// adds the partial sums to the FP64 totals and restarts them
-template <int M> __device__ __forceinline__ void fold(float (&part)[M], float (&part_s)[M], double (&sum)[M])
+// accumulate_node for the moment chunk j = j0 + i, i = 0..M-1, of a march
+// whose moments are split into chunks (more than fit in one thread's
+// registers): (c0, s0) and (c1, s1) are cos and sin of j0 phi and (j0 + 1) phi,
+// the recurrences run from there. The weight d + j^2 e is D + i (E2 + i e)
+// with D = d + j0^2 e and E2 = 2 j0 e (one more operation per moment than
+// with a compile-time j). Slot i = 0 of the chunk j0 = 0 (moment 0) gets
+// d, which nothing reads.
+template <int M>
+__device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&ss)[M], double d, double e, float h,
+                                                      double x, double c0, float s0, double c1, float s1, double j0)
 {
+    const double x2 = x + x;
+    const float x2f = (float)x2;
+    const double D = fma(j0 * j0, e, d);
+    const double E2 = (j0 + j0) * e;
+    double cm2 = c0, cm1 = c1;
+    float sm2 = s0, sm1 = s1;
+    sc[0] = fma(D, c0, sc[0]);
+    ss[0] = __builtin_fmaf(h, s0, ss[0]);
 #pragma unroll
-    for (int j = 1; j < M; ++j)
+    for (int i = 1; i < M; ++i)
     {
-        if constexpr (kSplitSine<M>)
+        double c;
+        float sn;
+        if (i == 1)
         {
-            sum[j] += (double)__builtin_fmaf((float)j, part_s[j], part[j]);
-            part_s[j] = 0.0f;
+            c = c1;
+            sn = s1;
         }
         else
-            sum[j] += (double)part[j];
-        part[j] = 0.0f;
+        {
+            c = fma(x2, cm1, -cm2);
+            sn = __builtin_fmaf(x2f, sm1, -sm2);
+            cm2 = cm1;
+            cm1 = c;
+            sm2 = sm1;
+            sm1 = sn;
+        }
+        const double w = fma((double)(i * i), e, fma((double)i, E2, D));
+        sc[i] = fma(w, c, sc[i]);
+        ss[i] = __builtin_fmaf(h, sn, ss[i]);
     }
 }
 
-template <typename T, int M, bool OFF32, bool POW2>
+// CHUNK: the moments are split into chunks of M over blockIdx.y (more than
+// one thread's registers hold); each chunk's workgroups march the same rays
+// and accumulate moments j0 .. j0 + M - 1, j0 = blockIdx.y * M; chunk 0 also
+// writes moment 0, the bounds, the ray range and the sample counts.
+template <typename T, int M, bool OFF32, bool POW2, bool CHUNK = false>
 __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
                                                              int32_t *__restrict__ samples,
@@ -287,6 +285,9 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     if (x >= a.W || y >= a.H)
         return;
     const int m = a.m;
+    const int j0 = CHUNK ? (int)blockIdx.y * M : 0;
+    const bool first_chunk = !CHUNK || blockIdx.y == 0;
+    const int jend = CHUNK ? min(m, j0 + M) : m;
     Ray r = pixel_ray(cam, a.W, a.H, x, y);
     float t_near, t_far;
     bool hit = intersect_aabb(r, vol.dmin, vol.dmax, t_near, t_far);
@@ -296,14 +297,17 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     if (!(hit && (t_far > 0)))
     {
         if (!(a.transform_to_bounds && a.compact_image))
-            for (int j = 0; j < m; ++j)
+            for (int j = j0; j < jend; ++j)
                 mo[j] = 0.f;
-        if (bounds)
-            bounds[pix] = 0;
-        if (ray_range)
-            ray_range[pix] = make_float2(0.0f, 1.0f);
-        if (samples)
-            samples[pix] = 0;
+        if (first_chunk)
+        {
+            if (bounds)
+                bounds[pix] = 0;
+            if (ray_range)
+                ray_range[pix] = make_float2(0.0f, 1.0f);
+            if (samples)
+                samples[pix] = 0;
+        }
         return;
     }
 
@@ -328,22 +332,43 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     }
     else
         node_phase = (float)(-kPi);
-    float zs, zc;
-    sincos_phase(node_phase, zs, zc);
-    const float z0s = zs, z0c = zc;
+    // cos and sin of the node phase (FP64, of the float phase)
+    double zs, zc;
+    sincos_small((double)node_phase, zs, zc);
+    const double z0s = zs, z0c = zc;
+    // CHUNK: cos and sin of j0 phi and (j0 + 1) phi at the current node (the
+    // product j0 phi is exact in FP64)
+    const double j0d = (double)j0;
+    double kc = 1.0, ks = 0.0, kc1 = zc, ks1 = zs;
+    auto chunk_start = [&](float ph) {
+        if constexpr (CHUNK)
+        {
+            sincos_small(j0d * (double)ph, ks, kc);
+            kc1 = fma(kc, zc, -ks * zs);
+            ks1 = fma(ks, zc, kc * zs);
+        }
+    };
+    chunk_start(node_phase);
+    const double q0s = ks, q0s1 = ks1;
 
-    float part[M], part_s[M];
-    double sum[M];
+    // sums of the cosine (FP64) and sine (FP32) terms
+    double sc[M];
+    float ss[M];
 #pragma unroll
     for (int j = 0; j < M; ++j)
     {
-        part[j] = 0.0f;
-        part_s[j] = 0.0f;
-        sum[j] = 0.0;
+        sc[j] = 0.0;
+        ss[j] = 0.0f;
     }
+    auto accumulate = [&](double d, double e, double h) {
+        if constexpr (CHUNK)
+            accumulate_node_chunk<M>(sc, ss, d, e, (float)h, zc, kc, (float)ks, kc1, (float)ks1, j0d);
+        else
+            accumulate_node<M>(sc, ss, d, e, (float)h, zc, (float)zs);
+    };
     double trap = 0.0;
-    float g_left = 0.0f;   // slope of the segment ending at the current node
-    float eps_left = 0.0f; // its width minus dphi
+    float g_left = 0.0f;    // slope of the segment ending at the current node
+    double eps_left = 0.0;  // its width minus dphi
     int node = 0;
 
     const int cap = march_cap(t_near, t_far, s);
@@ -374,24 +399,25 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         const float g = div_rn(f - f_prev, dphi, rdphi);
         const double width = (double)p - (double)u;
         trap += width * ((double)f_prev + 0.5 * (double)g * width);
-        const float delta = (node > 0) ? (float)((double)u - (double)node_phase) : 0.0f;
-        const float h = __builtin_fmaf(g, delta, g_left * eps_left);
+        // the offset of the segment start from the node (exact: u and the
+        // node phase are floats a few ulps apart) and the terms of the node
+        const double delta = (node > 0) ? (double)u - (double)node_phase : 0.0;
+        const double h = fma((double)g, delta, (double)g_left * eps_left);
         // a node with d = e = h = 0 (inside a stretch of zero density:
         // both slopes and the start value vanish) adds exact zeros; where
         // every lane of the wavefront has one, the update is skipped
         {
-            const float dn = g_left - g, en = -delta * f_prev;
-            if (dn != 0.0f || en != 0.0f || h != 0.0f)
-                accumulate_node<M>(part, part_s, dn, en, h, zc, zs);
+            const double dn = (double)g_left - (double)g, en = -delta * (double)f_prev;
+            if (dn != 0.0 || en != 0.0 || h != 0.0)
+                accumulate(dn, en, h);
         }
         g_left = g;
-        eps_left = (float)(width - (double)dphi);
+        eps_left = width - (double)dphi;
         f_prev = f;
         node_phase = p;
         ++node;
-        sincos_phase(p, zs, zc);
-        if (node % kFold == 0)
-            fold<M>(part, part_s, sum);
+        sincos_small((double)p, zs, zc);
+        chunk_start(p);
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
@@ -402,85 +428,113 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const float dl = 0.0f - node_phase;
     const float g_last = (f_last - f_prev) / dl;
     trap += (double)dl * ((double)f_prev + 0.5 * (double)g_last * (double)dl);
-    accumulate_node<M>(part, part_s, g_left - g_last, 0.0f, g_left * eps_left, zc, zs);
+    accumulate((double)g_left - (double)g_last, 0.0, (double)g_left * eps_left);
     // node at phase 0: cos(j*0) = 1, sin(j*0) = 0
 #pragma unroll
-    for (int j = 1; j < M; ++j)
-        part[j] += g_last;
-    fold<M>(part, part_s, sum);
+    for (int j = CHUNK ? 0 : 1; j < M; ++j)
+        sc[j] += (double)g_last;
 
-    // trig_j = sum_j / j^2 - f_0 sin(j phi_0) / j, sin by Chebyshev from phi_0.
-    float trig[M];
+    // trig_j = (sc_j + j ss_j) / j^2 - f_0 sin(j phi_0) / j in FP64, sin by
+    // Chebyshev from phi_0 (from j0 phi_0 and (j0 + 1) phi_0 for a chunk),
+    // then moment_j = trig_j / pi, normalised and rounded to float once
+    double trig[M];
     {
-        const float x2 = z0c + z0c;
-        float sm2 = 0.0f, sm1 = z0s;
+        const double x2 = z0c + z0c;
+        double sm2 = CHUNK ? q0s : 0.0, sm1 = CHUNK ? q0s1 : z0s;
 #pragma unroll
-        for (int j = 1; j < M; ++j)
+        for (int i = CHUNK ? 0 : 1; i < M; ++i)
         {
-            float sj;
-            if (j == 1)
-                sj = z0s;
+            double sj;
+            if (CHUNK && i == 0)
+                sj = q0s;
+            else if (i == 1)
+                sj = CHUNK ? q0s1 : z0s;
             else
             {
-                sj = __builtin_fmaf(x2, sm1, -sm2);
+                sj = fma(x2, sm1, -sm2);
                 sm2 = sm1;
                 sm1 = sj;
             }
-            const float jf = (float)j;
-            trig[j] = (float)sum[j] / (jf * jf) - f_first * sj / jf;
+            const double jd = j0d + (double)i;
+            const double num = fma(jd, (double)ss[i], sc[i]);
+            trig[i] = (jd == 0.0) ? 0.0 : num / (jd * jd) - (double)f_first * sj / jd;
         }
     }
-    trig[0] = (float)trap;
+    if (first_chunk)
+        trig[0] = trap;
 
     uint32_t b;
+    constexpr double kInvPi = 0.31830988618379067154;
     if (a.transform_to_bounds)
     {
         lo = rmax(lo - a.bound_eps, 0.0f);
         hi = rmin(hi + a.bound_eps, 1.0f);
         const double inv = 1.0 / (double)(hi - lo);
-        mo[0] = (float)((double)(trig[0] * kInvPiF - lo) * inv);
+        if (first_chunk)
+            mo[0] = (float)((trig[0] * kInvPi - (double)lo) * inv);
 #pragma unroll
-        for (int j = 1; j < M; ++j)
-            if (j < m)
-                mo[j] = (float)((double)(trig[j] * kInvPiF) * inv);
+        for (int i = 0; i < M; ++i)
+        {
+            const int j = j0 + i;
+            if (j >= 1 && j < jend)
+                mo[j] = (float)(trig[i] * kInvPi * inv);
+        }
     }
     else
     {
 #pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m)
-                mo[j] = trig[j] * kInvPiF;
+        for (int i = 0; i < M; ++i)
+        {
+            const int j = j0 + i;
+            if (j < jend)
+                mo[j] = (float)(trig[i] * kInvPi);
+        }
     }
-    b = pack_bound(lo, hi);
-    if (bounds)
-        bounds[pix] = b;
-    if (ray_range)
-        ray_range[pix] = make_float2(lo, hi);
-    if (samples)
-        samples[pix] = ns;
+    if (first_chunk)
+    {
+        b = pack_bound(lo, hi);
+        if (bounds)
+            bounds[pix] = b;
+        if (ray_range)
+            ray_range[pix] = make_float2(lo, hi);
+        if (samples)
+            samples[pix] = ns;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T, int M>
+// chunks > 0 launches the moment chunks of M over blockIdx.y (CHUNK)
+template <typename T, int M, bool CHUNK = false>
 static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
-                         int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks)
+                         int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks, int chunks = 1)
 {
     // byte offsets of every voxel fit in 32 bits up to 4 GiB of volume
     const bool off32 = (uint64_t)v.nx * v.ny * v.nz * sizeof(T) <= (uint64_t)0xFFFFFFFFu + 1u;
     // power-of-two domain extents and data range: divisions become products
     const bool pow2 = exact_pow2(v.dmax.x - v.dmin.x) && exact_pow2(v.dmax.y - v.dmin.y) &&
                       exact_pow2(v.dmax.z - v.dmin.z) && exact_pow2(a.data_max - a.data_min);
-    if (off32 && pow2)
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, true>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b, s,
-                           rr);
-    else if (off32)
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b,
+    const dim3 grid(blocks, chunks);
+    if constexpr (std::is_same<T, SphParticle>::value)
+    {
+        // particle sets: the gather ignores OFF32 and POW2 (the data range
+        // division takes the exact path)
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false, CHUNK>), grid, dim3(256), 0, st, v, c, a, o, b,
                            s, rr);
+    }
     else
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, false, false>), dim3(blocks), dim3(256), 0, st, v, c, a, o, b,
-                           s, rr);
+    {
+        if (off32 && pow2)
+            hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, true, CHUNK>), grid, dim3(256), 0, st, v, c, a, o, b,
+                               s, rr);
+        else if (off32)
+            hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false, CHUNK>), grid, dim3(256), 0, st, v, c, a, o,
+                               b, s, rr);
+        else
+            hipLaunchKernelGGL((generate_mixed_kernel<T, M, false, false, CHUNK>), grid, dim3(256), 0, st, v, c, a, o,
+                               b, s, rr);
+    }
 }
 
 template <typename T>
@@ -511,8 +565,23 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
     MRAY_GEN_CASE(52)
     MRAY_GEN_CASE(56)
     MRAY_GEN_CASE(60)
-    MRAY_GEN_CASE(64)
 #undef MRAY_GEN_CASE
+    // 61..255 moments: K chunks of at most 56 (an unchunked M = 64 and a
+    // chunked M = 48 lose half their occupancy), each M >= ceil(m / K) wide
+    if (m <= 255)
+    {
+        const int chunks = (m + 55) / 56;
+        const int per = (m + chunks - 1) / chunks;
+#define MRAY_GEN_CHUNK_CASE(MM)                                                                                       \
+    if (per <= MM)                                                                                                    \
+    {                                                                                                                 \
+        launch_mixed<T, MM, true>(v, c, a, o, b, s, rr, st, blocks, chunks);                                          \
+        return 0;                                                                                                     \
+    }
+        MRAY_GEN_CHUNK_CASE(40)
+        MRAY_GEN_CHUNK_CASE(56)
+#undef MRAY_GEN_CHUNK_CASE
+    }
     return MRAY_ERR_UNSUPPORTED;
 }
 
@@ -655,41 +724,6 @@ struct CellScatter
     }
 };
 
-// sin and cos of x in FP64 for |x| <= 4 (node phases lie in [-pi, 0]): the
-// nearest multiple k of pi/2 is taken off in two parts (the high part has
-// 33 significant bits, so k * hi is exact), then Taylor series to r^20 on
-// |r| <= pi/4 (truncation below 1e-21).
-__device__ __forceinline__ void sincos_small(double x, double &s, double &c)
-{
-    const double k = rint(x * 0.63661977236758134);
-    const double r = fma(-k, 6.07710050650619224932e-11, fma(-k, 1.57079632673412561417e+00, x));
-    const double r2 = r * r;
-    double ps = 1.0 / 121645100408832000.0;   // 1/19!
-    ps = fma(ps, -r2, 1.0 / 355687428096000.0); // 1/17!
-    ps = fma(ps, -r2, 1.0 / 1307674368000.0);
-    ps = fma(ps, -r2, 1.0 / 6227020800.0);
-    ps = fma(ps, -r2, 1.0 / 39916800.0);
-    ps = fma(ps, -r2, 1.0 / 362880.0);
-    ps = fma(ps, -r2, 1.0 / 5040.0);
-    ps = fma(ps, -r2, 1.0 / 120.0);
-    ps = fma(ps, -r2, 1.0 / 6.0);
-    const double sr = fma(-r * r2, ps, r);
-    double pc = 1.0 / 2432902008176640000.0;  // 1/20!
-    pc = fma(pc, -r2, 1.0 / 6402373705728000.0); // 1/18!
-    pc = fma(pc, -r2, 1.0 / 20922789888000.0);
-    pc = fma(pc, -r2, 1.0 / 87178291200.0);
-    pc = fma(pc, -r2, 1.0 / 479001600.0);
-    pc = fma(pc, -r2, 1.0 / 3628800.0);
-    pc = fma(pc, -r2, 1.0 / 40320.0);
-    pc = fma(pc, -r2, 1.0 / 720.0);
-    pc = fma(pc, -r2, 1.0 / 24.0);
-    pc = fma(pc, -r2, 0.5);
-    const double cr = fma(-r2, pc, 1.0);
-    const int q = (int)k & 3;
-    s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
-    c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
-}
-
 // Reverse mode of the generate march for dL/dmoments (dense W*H*m). The
 // derivative of the exact integral of the reference's piecewise-linear
 // segment [pp, pn] (length sg, slope denominator seg, r = sg / seg) is
@@ -709,7 +743,10 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &c)
 // constants (a first pass recomputes them for the normalisation); cos and
 // sin of the node phases come from sincos_small. One thread per ray, nothing
 // stored per sample.
-template <int M, bool POW2>
+// CHUNK: moments j0 .. j0 + M - 1 with j0 = blockIdx.y * M (more moments
+// than one thread's registers hold); the reverse is linear in the incoming
+// gradient, so the chunks' passes add their shares to the volume gradient.
+template <int M, bool POW2, bool CHUNK = false>
 __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float> vol, DevCamera cam, GenArgs a,
                                                                const float *__restrict__ grad_mom,
                                                                const float2 *__restrict__ ray_range,
@@ -797,34 +834,79 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         inv = 1.0 / (double)(hi - lo);
     }
     // dL/dtrig_j = dL/dmoment_j * inv / pi
-    // Chebyshev coefficients g_j / j^2 of C in FP64 registers
+    // Chebyshev coefficients g_j / j^2 of C in FP64 registers (slot i holds
+    // moment j0 + i; moment 0 enters through g0, in the first chunk)
+    const int j0 = CHUNK ? (int)blockIdx.y * M : 0;
     double cc[M];
     const double scale = inv / kPi;
-    const double g0 = active ? (double)grad_mom[pix * m] * scale : 0.0;
-    cc[0] = 0.0;
+    const double g0 = (active && j0 == 0) ? (double)grad_mom[pix * m] * scale : 0.0;
 #pragma unroll
-    for (int j = 1; j < M; ++j)
-        cc[j] = (active && j < m) ? (double)grad_mom[pix * m + j] * scale / (double)(j * j) : 0.0;
+    for (int i = 0; i < M; ++i)
+    {
+        const int j = j0 + i;
+        cc[i] = (active && j >= 1 && j < m) ? (double)grad_mom[pix * m + j] * scale / (double)(j * j) : 0.0;
+    }
     // C and S at a node with cos / sin (co, si) of its phase: Clenshaw's
     // recurrence b_l = c_l + 2x b_{l+1} - b_{l+2} gives C = x b_1 - b_2, and
     // its derivative in x, b'_l = 2 b_{l+1} + 2x b'_{l+1} - b'_{l+2}, gives
     // S = -dC/dphi = sin(phi) dC/dx = sin(phi) (b_1 + x b'_1 - b'_2), since
     // S = sum j c_j sin(j phi) for the same coefficients c_j = g_j / j^2.
+    //
+    // CHUNK: the chunk's sum is C = b_0 T_j0 - b_1 T_j0-1 (Clenshaw down to
+    // slot 0), and with T'_n(cos phi) sin(phi) = n sin(n phi),
+    // S = sin(phi) (d_0 T_j0 - d_1 T_j0-1) + b_0 j0 sin(j0 phi)
+    //     - b_1 (j0 - 1) sin((j0 - 1) phi);
+    // cos and sin of j0 phi by binary powering of the node's phasor (j0 is
+    // the same in the whole workgroup), so that they are the Chebyshev values
+    // of the x the recurrence uses.
     auto sums = [&](double co, double si, double &C, double &S) {
         const double x2 = co + co;
         double b1 = 0.0, b2 = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-        for (int l = M - 1; l >= 1; --l)
+        if constexpr (!CHUNK)
         {
-            const double tb = fma(x2, b1, cc[l] - b2);
-            const double td = fma(x2, d1, fma(2.0, b1, -d2));
-            b2 = b1;
-            b1 = tb;
-            d2 = d1;
-            d1 = td;
+#pragma unroll
+            for (int l = M - 1; l >= 1; --l)
+            {
+                const double tb = fma(x2, b1, cc[l] - b2);
+                const double td = fma(x2, d1, fma(2.0, b1, -d2));
+                b2 = b1;
+                b1 = tb;
+                d2 = d1;
+                d1 = td;
+            }
+            C = fma(co, b1, -b2);
+            S = si * (b1 + fma(co, d1, -d2));
         }
-        C = fma(co, b1, -b2);
-        S = si * (b1 + fma(co, d1, -d2));
+        else
+        {
+#pragma unroll
+            for (int l = M - 1; l >= 0; --l)
+            {
+                const double tb = fma(x2, b1, cc[l] - b2);
+                const double td = fma(x2, d1, fma(2.0, b1, -d2));
+                b2 = b1;
+                b1 = tb;
+                d2 = d1;
+                d1 = td;
+            }
+            // b1, b2, d1, d2 now hold b_0, b_1, d_0, d_1
+            double pc = 1.0, ps = 0.0, bc = co, bs = si;
+            for (int e = j0; e > 0; e >>= 1)
+            {
+                if (e & 1)
+                {
+                    const double nc = fma(pc, bc, -ps * bs);
+                    ps = fma(pc, bs, ps * bc);
+                    pc = nc;
+                }
+                const double sc = fma(bc, bc, -bs * bs);
+                bs = 2.0 * bc * bs;
+                bc = sc;
+            }
+            const double qc = fma(pc, co, ps * si), qs = fma(ps, co, -pc * si); // (j0 - 1) phi
+            C = fma(b1, pc, -b2 * qc);
+            S = fma(si, fma(d1, pc, -d2 * qc), fma(b1 * (double)j0, ps, -b2 * (double)(j0 - 1) * qs));
+        }
     };
     const float dscale = 1.0f / (a.data_max - a.data_min);
     // fixed-point scale of the window: a node adds at most |fbar| |dscale| to
@@ -834,8 +916,8 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     {
         double sj = 0.0;
 #pragma unroll
-        for (int j = 1; j < M; ++j)
-            sj = fma((double)j, fabs(cc[j]), sj);
+        for (int i = 0; i < M; ++i)
+            sj = fma((double)(j0 + i), fabs(cc[i]), sj);
         const float lb = active ? (float)(8.0 * (sj + fabs(g0)) * (double)fabsf(dscale) * (kChunk + 1)) : 0.0f;
         atomicMax(&bound_bits, __float_as_uint(lb));
     }
@@ -1055,6 +1137,25 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
                                grad_moments, rr, grad_volume);                                                        \
         return 0;                                                                                                     \
     }
+    // 65..255 moments: chunks of at most 56 over blockIdx.y
+#define MRAY_GADJ_CHUNK_CASE(MM)                                                                                      \
+    if (m > 64 && m <= 255 && per <= MM)                                                                              \
+    {                                                                                                                 \
+        const dim3 grid(blocks, chunks);                                                                              \
+        if (pow2)                                                                                                     \
+            hipLaunchKernelGGL((generate_adjoint_kernel<MM, true, true>), grid, dim3(256), 0, stream, v, c, a,        \
+                               grad_moments, rr, grad_volume);                                                        \
+        else                                                                                                          \
+            hipLaunchKernelGGL((generate_adjoint_kernel<MM, false, true>), grid, dim3(256), 0, stream, v, c, a,       \
+                               grad_moments, rr, grad_volume);                                                        \
+        return 0;                                                                                                     \
+    }
+    const int chunks = (m + 55) / 56;
+    const int per = (m + chunks - 1) / chunks;
+    MRAY_GADJ_CHUNK_CASE(40)
+    MRAY_GADJ_CHUNK_CASE(48)
+    MRAY_GADJ_CHUNK_CASE(56)
+#undef MRAY_GADJ_CHUNK_CASE
     MRAY_GADJ_CASE(8)
     MRAY_GADJ_CASE(16)
     MRAY_GADJ_CASE(24)
@@ -1086,6 +1187,26 @@ int launch_generate(const void *vox, int dtype, int nx, int ny, int nz, const fl
     DevCamera c = make_dev_camera(cam);
     uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
     uint32_t blocks = (ntiles + 3) / 4;
+    if (dtype == MRAY_DTYPE_PARTICLES)
+    {
+        // a particle set: vox is its SphBuffers (host struct of device buffers)
+        const SphBuffers *sb = static_cast<const SphBuffers *>(vox);
+        VolumeView<SphParticle> v;
+        v.s = sb->view();
+        v.nx = v.ny = v.nz = 1;
+        v.dmin = {dmin[0], dmin[1], dmin[2]};
+        v.dmax = {dmax[0], dmax[1], dmax[2]};
+        v.rext = {host_reciprocal(dmax[0] - dmin[0]), host_reciprocal(dmax[1] - dmin[1]),
+                  host_reciprocal(dmax[2] - dmin[2])};
+        if (form == MRAY_FORM_PLAIN)
+        {
+            int bs = plain_block(m);
+            hipLaunchKernelGGL(generate_plain_kernel<SphParticle>, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
+                               bs * m * sizeof(float), stream, v, c, a, moments, bounds, samples, rr);
+            return 0;
+        }
+        return dispatch_mixed<SphParticle>(v, c, a, moments, bounds, samples, rr, stream, blocks);
+    }
     if (dtype == MRAY_DTYPE_F32)
     {
         VolumeView<float> v = make_volume_view((const float *)vox, nx, ny, nz, dmin, dmax);

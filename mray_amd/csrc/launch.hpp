@@ -21,6 +21,8 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
 
 int launch_prepare(const float *moments, const uint32_t *index, int W, int H, int m, float bias, int form,
                    float *lagrange, hipStream_t stream);
+int launch_prepare_f64(const float *moments, const uint32_t *index, int W, int H, int m, float bias,
+                       double *lagrange, hipStream_t stream);
 
 int launch_reference(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
                      const mray_camera &cam, const mray_ref_params &p, int W, int H, const float *tf, float *rgba,
@@ -29,6 +31,10 @@ int launch_reference(const void *vox, int dtype, int nx, int ny, int nz, const f
 int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
                        const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
                        const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream);
+
+int launch_reconstruct_f64(const mray_camera &cam, const float *dmin, const float *dmax, const double *lagrange,
+                           const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                           const mray_rec_params &p, float *rgba, int32_t *steps, hipStream_t stream);
 
 int launch_reconstruct_adjoint(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
                                const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
@@ -41,6 +47,7 @@ int launch_prepare_adjoint(const float *moments, const uint32_t *index, int W, i
 int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H, int m, float error_threshold,
                        uint32_t *counts, hipStream_t stream);
 int launch_moment_index(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);
+int launch_clamp_counts(uint32_t *counts, int npix, int lo, int hi, hipStream_t stream);
 int launch_moment_index_noalloc(const uint32_t *counts, int npix, uint32_t *index, hipStream_t stream);
 int noalloc_scratch_init(int device, uint32_t **out);
 size_t noalloc_max_pixels();
@@ -64,5 +71,7 @@ int launch_particles_to_grid(const float *x, const float *y, const float *z, con
 int launch_synth_mixing_layer(float *out, int n, const double *amp, const double *theta, const double *phi,
                               uint64_t seed, hipStream_t stream);
 int launch_synth_marschner_lobb(float *out, int n, hipStream_t stream);
+int launch_synth_noise_blobs(uint8_t *out, int nx, int ny, int nz, const float *centres, const float *sigmas,
+                             const float *amps, int n_blobs, uint64_t seed, hipStream_t stream);
 
 } // namespace mray

@@ -56,10 +56,17 @@ __global__ __launch_bounds__(64) void prepare_plain_kernel(const float *__restri
 // MIXED (FP64, 8 lanes per pixel)
 // ---------------------------------------------------------------------------
 
-template <int C>
+// per-pixel LDS stride of the mixed kernels, in complex doubles: MP + 4, so
+// the eight pixels' arrays start 64 bytes (16 banks) apart (a stride of MP
+// elements puts them all in the same banks), except at C = 32, where the
+// padding would take the two arrays of eight pixels past 64 KiB
+constexpr int prepare_pixel_stride(int C) { return C >= 32 ? 8 * C : 8 * C + 4; }
+
+// OutT: float multipliers, or FP64 ones for mray_hip_prepare_f64
+template <int C, typename OutT = float>
 __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restrict__ mom,
                                                            const uint32_t *__restrict__ index, int npix, int m,
-                                                           float bias, float *__restrict__ out)
+                                                           float bias, OutT *__restrict__ out)
 {
     // Per pixel (8 per wavefront) two LDS arrays of MP complex doubles: E
     // (exponential moments) and A (Levinson coefficients, later the
@@ -69,10 +76,7 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     // indices are clamped into range and the surplus terms are exact zeros):
     // the loads of a step issue together instead of one round trip per term.
     constexpr int MP = 8 * C; // elements per array and pixel
-    // per-pixel stride MP + 4: the eight pixels' arrays start 64 bytes (16
-    // banks) apart (a stride of MP elements would put them all in the same
-    // banks)
-    constexpr int PS = MP + 4;
+    constexpr int PS = prepare_pixel_stride(C);
     extern __shared__ dc lds_c[];
     const int lane = threadIdx.x;
     const int g = lane & 7;
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(64) void prepare_mixed_kernel(const float *__restri
     {
         const int k = g + 8 * s;
         if (k < nm)
-            out[off + k] = (float)(nf * acc[s]);
+            out[off + k] = (OutT)(nf * acc[s]);
     }
 }
 
@@ -202,28 +206,24 @@ size_t g_lds_limit = 65536;
 
 void set_lds_limit(size_t bytes) { g_lds_limit = bytes; }
 
-int launch_prepare(const float *moments, const uint32_t *index, int W, int H, int m, float bias, int form,
-                   float *lagrange, hipStream_t stream)
+namespace
 {
-    const int npix = W * H;
-    if (form == MRAY_FORM_PLAIN)
-    {
-        size_t lds = (size_t)3 * 64 * m * sizeof(cf);
-        if (lds > g_lds_limit)
-            return MRAY_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL(prepare_plain_kernel, dim3((npix + 63) / 64), dim3(64), lds, stream, moments, index, npix,
-                           m, bias, lagrange);
-        return 0;
-    }
-    const int C = (m + 7) / 8;
-    const size_t lds = (size_t)8 * (8 * C + 4) * 2 * sizeof(dc);
+// the MIXED preparation into float or FP64 multipliers
+template <typename OutT>
+int launch_prepare_mixed(const float *moments, const uint32_t *index, int npix, int m, float bias, OutT *lagrange,
+                         hipStream_t stream)
+{
+    int C = (m + 7) / 8;
+    if (C > 16)
+        C = (C + 3) / 4 * 4; // 20, 24, 28, 32
+    const size_t lds = (size_t)8 * prepare_pixel_stride(C) * 2 * sizeof(dc);
     if (lds > g_lds_limit)
         return MRAY_ERR_UNSUPPORTED;
     dim3 grid((npix + 7) / 8);
 #define MRAY_PREP_CASE(CC)                                                                                            \
     case CC:                                                                                                          \
-        hipLaunchKernelGGL(prepare_mixed_kernel<CC>, grid, dim3(64), lds, stream, moments, index, npix, m, bias,      \
-                           lagrange);                                                                                 \
+        hipLaunchKernelGGL((prepare_mixed_kernel<CC, OutT>), grid, dim3(64), lds, stream, moments, index, npix, m,    \
+                           bias, lagrange);                                                                           \
         return 0;
     switch (C)
     {
@@ -243,10 +243,37 @@ int launch_prepare(const float *moments, const uint32_t *index, int W, int H, in
         MRAY_PREP_CASE(14)
         MRAY_PREP_CASE(15)
         MRAY_PREP_CASE(16)
+        MRAY_PREP_CASE(20)
+        MRAY_PREP_CASE(24)
+        MRAY_PREP_CASE(28)
+        MRAY_PREP_CASE(32)
     default:
         return MRAY_ERR_UNSUPPORTED;
     }
 #undef MRAY_PREP_CASE
+}
+} // namespace
+
+int launch_prepare(const float *moments, const uint32_t *index, int W, int H, int m, float bias, int form,
+                   float *lagrange, hipStream_t stream)
+{
+    const int npix = W * H;
+    if (form == MRAY_FORM_PLAIN)
+    {
+        size_t lds = (size_t)3 * 64 * m * sizeof(cf);
+        if (lds > g_lds_limit)
+            return MRAY_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL(prepare_plain_kernel, dim3((npix + 63) / 64), dim3(64), lds, stream, moments, index, npix,
+                           m, bias, lagrange);
+        return 0;
+    }
+    return launch_prepare_mixed<float>(moments, index, npix, m, bias, lagrange, stream);
+}
+
+int launch_prepare_f64(const float *moments, const uint32_t *index, int W, int H, int m, float bias,
+                       double *lagrange, hipStream_t stream)
+{
+    return launch_prepare_mixed<double>(moments, index, W * H, m, bias, lagrange, stream);
 }
 
 } // namespace mray

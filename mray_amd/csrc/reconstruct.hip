@@ -163,15 +163,35 @@ __global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, R
 // ---------------------------------------------------------------------------
 // MIXED
 // ---------------------------------------------------------------------------
-// sin x for x in [0, pi/4]: Taylor series to x^9 (truncation below 2e-9),
-// about one float ulp; no range reduction needed on this interval
-__device__ __forceinline__ float sin_quarter(float x)
+// sin x in FP64 for x in [0, pi/4]: Taylor series to x^17 (truncation below
+// 1e-19); no range reduction needed on this interval
+__device__ __forceinline__ double sin_quarter(double x)
 {
-    const float x2 = x * x;
-    float c = __builtin_fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
-    c = __builtin_fmaf(x2, c, 8.33333333e-3f);
-    c = __builtin_fmaf(x2, c, -0.166666667f);
-    return __builtin_fmaf(x * x2, c, x);
+    const double x2 = x * x;
+    double c = 1.0 / 355687428096000.0; // 1/17!
+    c = fma(c, -x2, 1.0 / 1307674368000.0);
+    c = fma(c, -x2, 1.0 / 6227020800.0);
+    c = fma(c, -x2, 1.0 / 39916800.0);
+    c = fma(c, -x2, 1.0 / 362880.0);
+    c = fma(c, -x2, 1.0 / 5040.0);
+    c = fma(c, -x2, 1.0 / 120.0);
+    c = fma(c, -x2, 1.0 / 6.0);
+    return fma(-x * x2, c, x);
+}
+
+// x = cos(theta), theta = -p in [0, pi], from the half angle h = theta / 2
+// (exact in float): 1 - 2 sin^2(h), or for h > pi/4, 2 cos^2(h) - 1 with
+// cos(h) = sin(pi/2 - h), so that 1 - x (resp. 1 + x) keeps its relative
+// accuracy next to theta = 0 (resp. pi), where the high orders need it; the
+// sine in FP64 (a float sine puts ~1e-7 of phase error on every order, which
+// the large multipliers of 50 moments turn into ~1e-4 of density)
+__device__ __forceinline__ double cos_theta_of(float p)
+{
+    const double h = -0.5 * (double)p;
+    const bool reflect = h > 0.78539816339744831;
+    const double q = sin_quarter(reflect ? 1.57079632679489662 - h : h);
+    const double q2 = 2.0 * q * q;
+    return reflect ? q2 - 1.0 : 1.0 - q2;
 }
 
 template <int M, typename L>
@@ -181,18 +201,7 @@ __device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p, do
     // (entries l >= m hold 0), by Clenshaw's recurrence in FP64 (2 operations
     // per term; gfx950 issues FP64 FMAs at the non-packed FP32 rate):
     //     b_l = lambda_l + 2x b_{l+1} - b_{l+2},  sum_{l>=1} = x b_1 - b_2.
-    // x = cos(theta) comes from the half angle, x = 1 - 2 sin^2(theta/2) or,
-    // for theta/2 > pi/4, x = 2 cos^2(theta/2) - 1 with cos(theta/2) =
-    // sin(pi/2 - theta/2), so that 1 - x (resp. 1 + x) keeps its relative
-    // accuracy next to theta = 0 (resp. pi), where the high orders need it.
-    const float h = -0.5f * p;
-    const bool reflect = h > 0.785398163f;
-    // pi/2 - h with pi/2 = 1.57079637f - 4.37113900e-8f; the first difference
-    // is exact for h in [pi/4, pi/2]
-    const float xq = reflect ? (1.57079637f - h) + -4.37113900e-8f : h;
-    const double q = (double)sin_quarter(xq);
-    const double q2 = 2.0 * q * q;
-    const double x = reflect ? q2 - 1.0 : 1.0 - q2;
+    const double x = cos_theta_of(p);
     if (cos_theta)
         *cos_theta = x;
     const double x2 = x + x;
@@ -229,9 +238,10 @@ __device__ __forceinline__ float fast_atan_rcp(float tan)
     return negative ? (-x) : x;
 }
 
-template <int M>
+// LagT: float multipliers, or FP64 ones kept from mray_hip_prepare_f64
+template <int M, typename LagT = float>
 __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, RecArgs a,
-                                                                const float *__restrict__ lag,
+                                                                const LagT *__restrict__ lag,
                                                                 const uint32_t *__restrict__ index,
                                                                 const uint32_t *__restrict__ bounds,
                                                                 const float4 *__restrict__ tf_g,
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
 #pragma unroll
     for (int l = 0; l < M; ++l)
     {
-        const float v = lag[off + min(l, nm - 1)];
+        const LagT v = lag[off + min(l, nm - 1)];
         lam[l] = (l < nm) ? (double)v : 0.0;
     }
     float blo = 0.f, bhi = 1.f;
@@ -331,6 +341,201 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
 }
 
 // ---------------------------------------------------------------------------
+// MIXED with the multipliers split over K lanes per ray (more moments than one
+// thread's registers hold)
+// ---------------------------------------------------------------------------
+
+// T_64(x), T_63(x), T_128(x), T_127(x), T_192(x), T_191(x) by doubling in
+// FP64 from x (T_2a = 2 T_a^2 - 1, T_2a-1 = 2 T_a T_a-1 - x,
+// T_a+b = 2 T_a T_b - T_a-b): the Chebyshev polynomials of the same x the
+// recurrences use, so that a chunk's Clenshaw sum continues the series
+// exactly where the previous chunk stops (a start from an independent cos(j0
+// theta) would differ from it by j0 times the error of x).
+__device__ __forceinline__ void chunk_starts(double x, double (&tn)[4], double (&tn1)[4])
+{
+    double a = x, a1 = 1.0; // T_1, T_0
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+    {
+        const double n = fma(2.0 * a, a, -1.0);
+        const double n1 = fma(2.0 * a, a1, -x);
+        a = n;
+        a1 = n1;
+    }
+    // a = T_64, a1 = T_63
+    const double t128 = fma(2.0 * a, a, -1.0), t127 = fma(2.0 * a, a1, -x);
+    const double t65 = fma(2.0 * x, a, -a1);
+    tn[0] = 1.0;
+    tn1[0] = x; // T_-1 = T_1
+    tn[1] = a;
+    tn1[1] = a1;
+    tn[2] = t128;
+    tn1[2] = t127;
+    tn[3] = fma(2.0 * t128, a, -a);    // T_192 = 2 T_128 T_64 - T_64
+    tn1[3] = fma(2.0 * t128, a1, -t65); // T_191 = 2 T_128 T_63 - T_65
+}
+
+// The series lambda_0 + 2 sum_l lambda_l cos(l theta), theta = -p, with the
+// multipliers split over K adjacent lanes (lane k holds lambda_64k ..
+// lambda_64k+63 in lam, zero past the pixel's count): each lane's Clenshaw
+// sum over its chunk from T_64k, T_64k-1 of the shared x = cos(theta) (taken
+// from the half angle as in series_clenshaw), the K partial sums added by an
+// xor butterfly, so every lane of the ray gets the same value bit for bit.
+// Also returns x and the lane's T_64k, T_64k-1 (for the reverse's forward
+// recurrence over its chunk).
+template <int K, typename L>
+__device__ __forceinline__ double series_split(const L (&lam)[64], int k, float p, double &x_out, double &t_j0,
+                                               double &t_j0m1)
+{
+    const double xx = cos_theta_of(p);
+    double tn[4], tn1[4];
+    chunk_starts(xx, tn, tn1);
+    double s0 = tn[0], s1 = tn1[0];
+#pragma unroll
+    for (int c = 1; c < K; ++c)
+    {
+        s0 = (k == c) ? tn[c] : s0;
+        s1 = (k == c) ? tn1[c] : s1;
+    }
+    const double x2 = xx + xx;
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int i = 63; i >= 0; --i)
+    {
+        const double t = fma(x2, b1, (double)lam[i] - b2);
+        b2 = b1;
+        b1 = t;
+    }
+    // lane 0 takes lambda_0 / 2 off its sum, so that twice the total is
+    // lambda_0 + 2 sum_{l>=1} lambda_l T_l
+    const double half0 = (k == 0) ? 0.5 * (double)lam[0] : 0.0;
+    double part = fma(b1, s0, fma(-b2, s1, -half0));
+#pragma unroll
+    for (int o = 1; o < K; o <<= 1)
+        part += __shfl_xor(part, o, 64);
+    x_out = xx;
+    t_j0 = s0;
+    t_j0m1 = s1;
+    return 2.0 * part;
+}
+
+// K lanes per ray (K = 2 or 4, adjacent lanes), 64 / K rays per wavefront in
+// 8 x (8 / K) pixel tiles; lane k holds multipliers 64k .. 64k + 63. Each lane
+// sums its chunk by Clenshaw's recurrence from T_64k, T_64k-1; the K partial
+// sums are added by an xor butterfly, which gives every lane of the ray the
+// same series value bit for bit, so all K lanes take the same steps and only
+// lane 0 writes.
+template <int K, typename LagT = float>
+__global__ __launch_bounds__(256) void reconstruct_mixed_split_kernel(DevCamera cam, RecArgs a,
+                                                                      const LagT *__restrict__ lag,
+                                                                      const uint32_t *__restrict__ index,
+                                                                      const uint32_t *__restrict__ bounds,
+                                                                      const float4 *__restrict__ tf_g,
+                                                                      float *__restrict__ rgba,
+                                                                      int32_t *__restrict__ steps)
+{
+    constexpr int M = 64;
+    constexpr int RPT = 8 / K; // pixel rows per tile
+    __shared__ float4 tf[256];
+    const int tid = threadIdx.x;
+    tf[tid] = tf_g[tid];
+    __syncthreads();
+
+    const int lane = tid & 63;
+    const int k = lane % K, ray = lane / K;
+    const int tiles_x = (a.W + 7) / 8;
+    const uint32_t ntiles = (uint32_t)tiles_x * (uint32_t)((a.H + RPT - 1) / RPT);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    if (tile >= ntiles)
+        return;
+    const int x = (int)(tile % tiles_x) * 8 + (ray & 7);
+    const int y = (int)(tile / tiles_x) * RPT + (ray >> 3);
+    if (x >= a.W || y >= a.H)
+        return; // all K lanes of the ray together
+    const size_t pix = (size_t)y * a.W + x;
+    const int nm = index ? min((int)(index[pix + 1] - index[pix]), a.m) : a.m;
+    const size_t off = index ? (size_t)index[pix] : pix * (size_t)a.m;
+    if (nm == 0)
+    {
+        if (k == 0)
+        {
+            float o[4];
+            zero_ray(tf, o);
+            store_rgba(rgba, pix, o[0], o[1], o[2], o[3]);
+            if (steps)
+                steps[pix] = 0;
+        }
+        return;
+    }
+    const int j0 = k * M;
+    double lam[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+    {
+        const LagT v = lag[off + min(j0 + i, nm - 1)];
+        lam[i] = (j0 + i < nm) ? (double)v : 0.0;
+    }
+    float blo = 0.f, bhi = 1.f;
+    if (bounds)
+    {
+        uint32_t b = bounds[pix];
+        blo = (b & 0xFFFFu) / 65535.f;
+        bhi = (b >> 16) / 65535.f;
+    }
+    const float brange = bhi - blo;
+    const bool fourier = a.fourier != 0;
+    auto recon = [&](float p) {
+        p = rclamp(p, (float)(-kPi), 0.0f);
+        double xx, tj0, tj0m1;
+        const float sv = (float)series_split<K>(lam, k, p, xx, tj0, tj0m1);
+        float rho = fourier ? sv : __builtin_fmaf(fast_atan_rcp(sv), 0.318309886f, 0.5f);
+        return __builtin_fmaf(rho, brange, blo);
+    };
+
+    Ray r = pixel_ray(cam, a.W, a.H, x, y);
+    float t_near, t_far;
+    bool hit = intersect_aabb(r, a.dmin, a.dmax, t_near, t_far);
+    t_near = rmax(t_near, cam.znear);
+    int ns = 0;
+    float Lr = 0.f, Lg = 0.f, Lb = 0.f, alpha = 0.f;
+    if (hit && (t_far > 0))
+    {
+        float t = t_near;
+        float p = (float)((double)((t - t_near) / (t_far - t_near)) * kPi - kPi);
+        const float dphi = (float)(kPi * (double)(a.step / (t_far - t_near)));
+        float T = 1.0f, tau = 0.0f;
+        float fs = recon(p);
+        const int cap = march_cap(t_near, t_far, a.step) + 1;
+        for (int it = 0; t <= t_far && it < cap; ++it)
+        {
+            t += a.step;
+            p += dphi;
+            ++ns;
+            const float bs = recon(p);
+            const float4 c = tf_sample_mixed(tf, fs, bs);
+            Lr = __builtin_fmaf(a.emission * c.x, T, Lr);
+            Lg = __builtin_fmaf(a.emission * c.y, T, Lg);
+            Lb = __builtin_fmaf(a.emission * c.z, T, Lb);
+            tau += c.w;
+            T = __expf(-tau);
+            alpha = 1.0f - T;
+            fs = bs;
+            if (alpha >= a.ert)
+            {
+                alpha = 1.f;
+                break;
+            }
+        }
+    }
+    if (k == 0)
+    {
+        store_rgba(rgba, pix, Lr, Lg, Lb, alpha);
+        if (steps)
+            steps[pix] = ns;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // ADJOINT of the MIXED march
 // ---------------------------------------------------------------------------
 
@@ -368,7 +573,11 @@ __device__ __forceinline__ float fast_atan_deriv(float t)
 // one entry, so that no partial sum reaches 2^61.
 
 
-template <int M>
+// K > 1: K adjacent lanes per ray (M = 64 multipliers each, as
+// reconstruct_mixed_split_kernel): every lane recomputes the ray's march with
+// the shared series value, accumulates the gradient of its own multipliers,
+// and only lane 0 adds to the table gradient.
+template <int M, int K = 1>
 __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     DevCamera cam, RecArgs a, const float *__restrict__ lag, const uint32_t *__restrict__ index,
     const uint32_t *__restrict__ bounds, const float4 *__restrict__ tf_g, const float4 *__restrict__ rgba_fwd,
@@ -390,14 +599,21 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     // the largest colour of the table (non-negative floats order as their bits)
     atomicMax(&cmax_bits, __float_as_uint(fmaxf(fabsf(tv.x), fmaxf(fabsf(tv.y), fabsf(tv.z)))));
 
+    static_assert(K == 1 || M == 64, "lane-split reverse holds 64 multipliers per lane");
     const int lane = tid & 63;
-    const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
+    const int k = lane % K; // chunk of the lane (0 for K = 1)
+    const int j0 = k * M;
+    constexpr int RPT = 8 / K; // pixel rows per tile
+    const int tiles_x = (a.W + 7) / 8;
+    const uint32_t ntiles = (uint32_t)tiles_x * (uint32_t)((a.H + RPT - 1) / RPT);
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
     int x = 0, y = 0;
     bool active = tile < ntiles;
     if (active)
     {
-        tile_pixel(tile, a.W, lane, x, y);
+        const int ray = lane / K;
+        x = (int)(tile % tiles_x) * 8 + (ray & 7);
+        y = (int)(tile / tiles_x) * RPT + (ray >> 3);
         active = x < a.W && y < a.H;
     }
     const size_t pix = active ? (size_t)y * a.W + x : 0;
@@ -414,7 +630,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     // |dL/dc| = |g_rgb e| T and |dL/dsigma| <= sum |g_ch| (|L_ch| + |P_ch|) + |g_a| T_end
     // (weights w01 + w23 = 1), at most cap samples; twice that for rounding
     float lane_bound = 0.0f;
-    if (active && grad_tf)
+    if (active && grad_tf && k == 0)
     {
         const float ga = fabsf(g.x) + fabsf(g.y) + fabsf(g.z);
         if (nm == 0)
@@ -448,7 +664,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
         if (nm == 0)
         {
             // reconstruct_zero_ray: the colour is the table at (0, 0) = entry 0
-            if (grad_tf)
+            if (grad_tf && k == 0)
             {
                 tf_add(0, 0, g.x);
                 tf_add(1, 0, g.y);
@@ -462,10 +678,23 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
 #pragma unroll
             for (int l = 0; l < M; ++l)
             {
-                const float v = lag[off + min(l, nm - 1)]; // unconditional, as in the forward
-                lam[l] = (l < nm) ? v : 0.0f;
+                const float v = lag[off + min(j0 + l, nm - 1)]; // unconditional, as in the forward
+                lam[l] = (j0 + l < nm) ? v : 0.0f;
                 glam[l] = 0.0f;
             }
+            // the series at phase pc; c1 = cos(theta), tj0 / tj0m1 = T_j0, T_j0-1
+            // of this lane's chunk (1 and cos(theta) for K = 1)
+            auto series_at = [&](float pc, double &c1, double &tj0, double &tj0m1) {
+                if constexpr (K == 1)
+                {
+                    const double sv = series_clenshaw<M>(lam, pc, &c1);
+                    tj0 = 1.0;
+                    tj0m1 = c1;
+                    return sv;
+                }
+                else
+                    return series_split<K>(lam, k, pc, c1, tj0, tj0m1);
+            };
             float blo = 0.f, bhi = 1.f;
             if (bounds)
             {
@@ -488,7 +717,8 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     const float rho = fourier ? s : __builtin_fmaf(fast_atan_rcp(s), 0.318309886f, 0.5f);
                     return __builtin_fmaf(rho, brange, blo);
                 };
-                float fs = rho_of((float)series_clenshaw<M>(lam, rclamp(p, (float)(-kPi), 0.0f)));
+                double c1, tj0, tj0m1;
+                float fs = rho_of((float)series_at(rclamp(p, (float)(-kPi), 0.0f), c1, tj0, tj0m1));
                 double pre[3] = {0.0, 0.0, 0.0};
                 // table gradient of the current entry pair (x0, x1), not yet in gtf
                 int tf_x0 = -1, tf_x1 = -1;
@@ -509,8 +739,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     t += a.step;
                     p += dphi;
                     const float pc = rclamp(p, (float)(-kPi), 0.0f);
-                    double c1;
-                    const float S = (float)series_clenshaw<M>(lam, pc, &c1);
+                    const float S = (float)series_at(pc, c1, tj0, tj0m1);
                     const float bs = rho_of(S);
                     const float4 c = tf_sample_mixed(tf, fs, bs);
 
@@ -530,7 +759,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     const double gsig = -((double)g.x * ((double)L.x - pre[0]) + (double)g.y * ((double)L.y - pre[1]) +
                                           (double)g.z * ((double)L.z - pre[2])) +
                                         (double)g.w * t_end;
-                    if (grad_tf)
+                    if (grad_tf && k == 0)
                     {
                         // consecutive samples that read the same table pair are
                         // summed in registers; the LDS atomics (contended: the
@@ -556,11 +785,11 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     if (dS != 0.0f)
                     {
                         // cos(l theta), theta = -pc, by the Chebyshev recurrence
-                        // from the series' own cos(theta)
-                        double cm = 1.0, cl = c1;
-                        glam[0] += dS;
+                        // from the series' own cos(theta), over the lane's
+                        // chunk l = j0 .. j0 + M - 1
+                        double cm = tj0m1, cl = tj0;
 #pragma unroll
-                        for (int l = 1; l < M; ++l)
+                        for (int l = 0; l < M; ++l)
                         {
                             glam[l] = __builtin_fmaf(dS, (float)cl, glam[l]);
                             const double cn = fma(2.0 * c1, cl, -cm);
@@ -574,7 +803,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                     if (1.0f - T >= a.ert)
                         break;
                 }
-                if (grad_tf)
+                if (grad_tf && k == 0)
                     tf_flush();
             }
             if (grad_lag)
@@ -582,8 +811,8 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
                 // static indices keep glam in registers
 #pragma unroll
                 for (int l = 0; l < M; ++l)
-                    if (l < nm)
-                        grad_lag[off + l] = l == 0 ? glam[0] : 2.0f * glam[l];
+                    if (j0 + l < nm)
+                        grad_lag[off + j0 + l] = (j0 + l == 0) ? glam[l] : 2.0f * glam[l];
             }
         }
     }
@@ -635,12 +864,29 @@ int launch_reconstruct_adjoint(const mray_camera &cam, const float *dmin, const 
     MRAY_ADJ_CASE(56)
     MRAY_ADJ_CASE(64)
 #undef MRAY_ADJ_CASE
+    // 65..256: two or four lanes per ray, 64 multipliers each
+    if (m <= 256)
+    {
+        if (m <= 128)
+        {
+            const uint32_t tiles2 = (uint32_t)(((W + 7) / 8) * ((H + 3) / 4));
+            hipLaunchKernelGGL((reconstruct_adjoint_kernel<64, 2>), dim3((tiles2 + 3) / 4), dim3(256), 0, stream, c,
+                               a, lagrange, index, bounds, tf4, fw, gr, grad_lagrange, grad_tf);
+        }
+        else
+        {
+            const uint32_t tiles4 = (uint32_t)(((W + 7) / 8) * ((H + 1) / 2));
+            hipLaunchKernelGGL((reconstruct_adjoint_kernel<64, 4>), dim3((tiles4 + 3) / 4), dim3(256), 0, stream, c,
+                               a, lagrange, index, bounds, tf4, fw, gr, grad_lagrange, grad_tf);
+        }
+        return 0;
+    }
     return MRAY_ERR_UNSUPPORTED;
 }
 
-int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
-                       const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
-                       const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream)
+namespace
+{
+RecArgs rec_args(const float *dmin, const float *dmax, int W, int H, int m, const mray_rec_params &p)
 {
     RecArgs a;
     a.W = W;
@@ -652,22 +898,22 @@ int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *d
     a.fourier = p.use_truncated_fourier;
     a.dmin = {dmin[0], dmin[1], dmin[2]};
     a.dmax = {dmax[0], dmax[1], dmax[2]};
-    DevCamera c = make_dev_camera(cam);
+    return a;
+}
+
+// the MIXED march on float or FP64 multipliers
+template <typename LagT>
+int launch_mixed_march(const DevCamera &c, const RecArgs &a, const LagT *lagrange, const uint32_t *index,
+                       const uint32_t *bounds, const float4 *tf4, float *rgba, int32_t *steps, hipStream_t stream)
+{
+    const int W = a.W, H = a.H, m = a.m;
     const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
-    const float4 *tf4 = reinterpret_cast<const float4 *>(tf);
-    if (form == MRAY_FORM_PLAIN)
-    {
-        int bs = (256 * m * (int)sizeof(float) <= 60000) ? 256 : 64;
-        hipLaunchKernelGGL(reconstruct_plain_kernel, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
-                           (size_t)bs * m * sizeof(float), stream, c, a, lagrange, index, bounds, tf4, rgba, steps);
-        return 0;
-    }
     const uint32_t blocks = (ntiles + 3) / 4;
 #define MRAY_REC_CASE(MM)                                                                                             \
     if (m <= MM)                                                                                                      \
     {                                                                                                                 \
-        hipLaunchKernelGGL(reconstruct_mixed_kernel<MM>, dim3(blocks), dim3(256), 0, stream, c, a, lagrange, index,   \
-                           bounds, tf4, rgba, steps);                                                                 \
+        hipLaunchKernelGGL((reconstruct_mixed_kernel<MM, LagT>), dim3(blocks), dim3(256), 0, stream, c, a, lagrange,  \
+                           index, bounds, tf4, rgba, steps);                                                          \
         return 0;                                                                                                     \
     }
     MRAY_REC_CASE(4)
@@ -697,7 +943,42 @@ int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *d
     MRAY_REC_CASE(120)
     MRAY_REC_CASE(128)
 #undef MRAY_REC_CASE
+    if (m <= 256)
+    {
+        // four lanes per ray, 16 rays per wavefront in 8 x 2 tiles
+        const uint32_t tiles4 = (uint32_t)(((W + 7) / 8) * ((H + 1) / 2));
+        hipLaunchKernelGGL((reconstruct_mixed_split_kernel<4, LagT>), dim3((tiles4 + 3) / 4), dim3(256), 0, stream, c,
+                           a, lagrange, index, bounds, tf4, rgba, steps);
+        return 0;
+    }
     return MRAY_ERR_UNSUPPORTED;
+}
+} // namespace
+
+int launch_reconstruct(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
+                       const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                       const mray_rec_params &p, int form, float *rgba, int32_t *steps, hipStream_t stream)
+{
+    const RecArgs a = rec_args(dmin, dmax, W, H, m, p);
+    const DevCamera c = make_dev_camera(cam);
+    const float4 *tf4 = reinterpret_cast<const float4 *>(tf);
+    if (form == MRAY_FORM_PLAIN)
+    {
+        const uint32_t ntiles = (uint32_t)(((W + 7) / 8) * ((H + 7) / 8));
+        int bs = (256 * m * (int)sizeof(float) <= 60000) ? 256 : 64;
+        hipLaunchKernelGGL(reconstruct_plain_kernel, dim3((ntiles + bs / 64 - 1) / (bs / 64)), dim3(bs),
+                           (size_t)bs * m * sizeof(float), stream, c, a, lagrange, index, bounds, tf4, rgba, steps);
+        return 0;
+    }
+    return launch_mixed_march<float>(c, a, lagrange, index, bounds, tf4, rgba, steps, stream);
+}
+
+int launch_reconstruct_f64(const mray_camera &cam, const float *dmin, const float *dmax, const double *lagrange,
+                           const uint32_t *index, const uint32_t *bounds, int W, int H, int m, const float *tf,
+                           const mray_rec_params &p, float *rgba, int32_t *steps, hipStream_t stream)
+{
+    return launch_mixed_march<double>(make_dev_camera(cam), rec_args(dmin, dmax, W, H, m, p), lagrange, index, bounds,
+                                      reinterpret_cast<const float4 *>(tf), rgba, steps, stream);
 }
 
 } // namespace mray

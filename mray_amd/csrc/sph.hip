@@ -25,16 +25,14 @@
 
 #include <cstring>
 
-#include "device_common.hpp"
 #include "launch.hpp"
+#include "sph_common.hpp"
 
 namespace mray
 {
 
 namespace
 {
-
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int float_key(float f)
 {
@@ -47,42 +45,6 @@ __host__ __device__ inline float key_float(int k)
     float f;
     std::memcpy(&f, &i, 4);
     return f;
-}
-
-struct CellGrid
-{
-    uint32_t sx, sy, sz; // padded cell counts
-    float minp[3];
-    float inv_cell;
-    float r2;
-};
-
-// getIndexFromCell with 4^3 tiles (src/particle_grid.cuh:64-83)
-__device__ __forceinline__ uint32_t cell_index(const CellGrid &g, int cx, int cy, int cz)
-{
-    const int tsx = 4, tsy = 4, tsz = (g.sz == 1) ? 1 : 4;
-    const int width_tiles = ((int)g.sx + tsx - 1) / tsx;
-    const int height_tiles = ((int)g.sy + tsy - 1) / tsy;
-    const int tile_x = cx / tsx, tile_y = cy / tsy, tile_z = cz / tsz;
-    const int in_x = cx % tsx, in_y = cy % tsy, in_z = cz % tsz;
-    return (uint32_t)((tile_z * height_tiles * width_tiles + tile_y * width_tiles + tile_x) * (tsx * tsy * tsz) +
-                      in_z * tsx * tsy + in_y * tsx + in_x);
-}
-
-__device__ __forceinline__ int cell_coord(const CellGrid &g, float p, int k)
-{
-    return (int)((p - g.minp[k]) * g.inv_cell);
-}
-
-// cubicSpline (src/cut/kernels.h:55-64)
-__device__ __forceinline__ float cubic_spline(float x)
-{
-    const float sigma = 0.5641895835477563f;
-    if (x < 1.f)
-        return sigma * (1.f - (3.f / 2.f) * x * x * (1.f - x * 0.5f));
-    else if (x < 2.f)
-        return sigma / 4.f * (2.f - x) * (2.f - x) * (2.f - x);
-    return 0.0f;
 }
 
 __global__ void bbox_init_kernel(int *__restrict__ keys)
@@ -177,10 +139,7 @@ __global__ __launch_bounds__(256) void cell_range_kernel(const uint32_t *__restr
 }
 
 // Particles::sample at every voxel position
-__global__ __launch_bounds__(256) void sph_gather_kernel(const float4 *__restrict__ parts,
-                                                         const uint32_t *__restrict__ start,
-                                                         const uint32_t *__restrict__ end, CellGrid g, float inv_h,
-                                                         float3 dmin, float3 dmax, int nx, int ny, int nz,
+__global__ __launch_bounds__(256) void sph_gather_kernel(SphView sv, float3 dmin, float3 dmax, int nx, int ny, int nz,
                                                          float *__restrict__ grid)
 {
     const int ix = blockIdx.x * 8 + (threadIdx.x & 7);
@@ -188,50 +147,22 @@ __global__ __launch_bounds__(256) void sph_gather_kernel(const float4 *__restric
     const int iz = blockIdx.z * 4 + (threadIdx.x >> 6);
     if (ix >= nx || iy >= ny || iz >= nz)
         return;
-    const float p[3] = {dmin.x + (float)ix / (float)nx * (dmax.x - dmin.x),
-                        dmin.y + (float)iy / (float)ny * (dmax.y - dmin.y),
-                        dmin.z + (float)iz / (float)nz * (dmax.z - dmin.z)};
-    const int c[3] = {cell_coord(g, p[0], 0), cell_coord(g, p[1], 1), cell_coord(g, p[2], 2)};
-    float value = 0.0f, wsum = 0.0f;
-    const int z0 = max(c[2] - 1, 0), z1 = min(c[2] + 2, (int)g.sz);
-    const int y0 = max(c[1] - 1, 0), y1 = min(c[1] + 2, (int)g.sy);
-    const int x0 = max(c[0] - 1, 0), x1 = min(c[0] + 2, (int)g.sx);
-    for (int cz = z0; cz < z1; ++cz)
-        for (int cy = y0; cy < y1; ++cy)
-            for (int cx = x0; cx < x1; ++cx)
-            {
-                const uint32_t ci = cell_index(g, cx, cy, cz);
-                const uint32_t s = start[ci];
-                if (s == kEmpty)
-                    continue;
-                const uint32_t e = end[ci];
-                for (uint32_t q = s; q < e; ++q)
-                {
-                    const float4 pt = parts[q];
-                    const float dx = p[0] - pt.x, dy = p[1] - pt.y, dz = p[2] - pt.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    if (d2 < g.r2)
-                    {
-                        const float ex = pt.x - p[0], ey = pt.y - p[1], ez = pt.z - p[2];
-                        const float w = cubic_spline(sqrtf(ex * ex + ey * ey + ez * ez) * inv_h);
-                        wsum += w;
-                        value += w * pt.w;
-                    }
-                }
-            }
-    if (wsum != 0.0f)
-        value /= wsum;
-    grid[((size_t)iz * ny + iy) * nx + ix] = value;
+    const float px = dmin.x + (float)ix / (float)nx * (dmax.x - dmin.x);
+    const float py = dmin.y + (float)iy / (float)ny * (dmax.y - dmin.y);
+    const float pz = dmin.z + (float)iz / (float)nz * (dmax.z - dmin.z);
+    grid[((size_t)iz * ny + iy) * nx + ix] = sph_sample(sv, px, py, pz);
 }
 
 uint32_t next_multiple(uint32_t x, int mul) { return ((x - 1) / mul + 1) * mul; }
 
 } // namespace
 
-int launch_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
-                             const float *dmin, const float *dmax, int nx, int ny, int nz, float *grid,
-                             hipStream_t stream)
+int sph_build(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h, bool persistent,
+              SphBuffers &out, hipStream_t stream)
 {
+    auto alloc = [&](void **p, size_t bytes) {
+        return persistent ? hipMalloc(p, bytes) : hipMallocAsync(p, bytes, stream);
+    };
     // bounding box (createGridFromDomainCPU, src/particle_grid.cpp:38-55)
     int *d_keys6 = nullptr;
     if (hipMallocAsync((void **)&d_keys6, 6 * sizeof(int), stream) != hipSuccess)
@@ -266,33 +197,38 @@ int launch_particles_to_grid(const float *x, const float *y, const float *z, con
     g.inv_cell = (float)(1.0 / (double)h);
     g.r2 = h * h;
     const size_t ncells = (size_t)g.sx * g.sy * g.sz;
-    if (ncells >= (size_t)kEmpty)
+    if (ncells >= (size_t)kSphEmpty)
         return MRAY_ERR_UNSUPPORTED;
     int end_bit = 1;
     while (end_bit < 32 && ((uint64_t)1 << end_bit) < ncells)
         ++end_bit;
 
     // keys, stable sort, sorted records, cell ranges
-    uint32_t *keys = nullptr, *ids = nullptr, *keys_s = nullptr, *ids_s = nullptr, *start = nullptr, *end = nullptr;
-    float4 *parts = nullptr;
+    uint32_t *keys = nullptr, *ids = nullptr, *keys_s = nullptr, *ids_s = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     int rc = 0;
-    auto alloc = [&](void **p, size_t bytes) {
+    auto scratch = [&](void **p, size_t bytes) {
         if (rc == 0 && hipMallocAsync(p, bytes, stream) != hipSuccess)
             rc = MRAY_ERR_OOM;
     };
-    alloc((void **)&keys, (size_t)n * 4);
-    alloc((void **)&ids, (size_t)n * 4);
-    alloc((void **)&keys_s, (size_t)n * 4);
-    alloc((void **)&ids_s, (size_t)n * 4);
-    alloc((void **)&parts, (size_t)n * sizeof(float4));
-    alloc((void **)&start, ncells * 4);
-    alloc((void **)&end, ncells * 4);
+    out = SphBuffers{};
+    out.g = g;
+    out.inv_h = 1.f / h;
+    out.n = n;
+    out.ncells = ncells;
+    scratch((void **)&keys, (size_t)n * 4);
+    scratch((void **)&ids, (size_t)n * 4);
+    scratch((void **)&keys_s, (size_t)n * 4);
+    scratch((void **)&ids_s, (size_t)n * 4);
+    if (rc == 0 && (alloc((void **)&out.parts, (size_t)n * sizeof(float4)) != hipSuccess ||
+                    alloc((void **)&out.start, ncells * 4) != hipSuccess ||
+                    alloc((void **)&out.end, ncells * 4) != hipSuccess))
+        rc = MRAY_ERR_OOM;
     if (rc == 0 && hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_s, ids, ids_s, (int)n, 0, end_bit,
                                                       stream) != hipSuccess)
         rc = MRAY_ERR_HIP;
-    alloc(&tmp, tmp_bytes);
+    scratch(&tmp, tmp_bytes);
     if (rc == 0)
     {
         const unsigned pb = (unsigned)((n + 255) / 256);
@@ -304,20 +240,42 @@ int launch_particles_to_grid(const float *x, const float *y, const float *z, con
     if (rc == 0)
     {
         const unsigned pb = (unsigned)((n + 255) / 256);
-        hipLaunchKernelGGL(gather_particles_kernel, dim3(pb), dim3(256), 0, stream, x, y, z, v, ids_s, n, parts);
+        hipLaunchKernelGGL(gather_particles_kernel, dim3(pb), dim3(256), 0, stream, x, y, z, v, ids_s, n, out.parts);
         const unsigned fb = (unsigned)std::min<size_t>((ncells + 255) / 256, 65536);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(fb), dim3(256), 0, stream, start, ncells, kEmpty);
-        hipLaunchKernelGGL(cell_range_kernel, dim3(pb), dim3(256), 0, stream, keys_s, n, start, end);
-        dim3 blocks((nx + 7) / 8, (ny + 7) / 8, (nz + 3) / 4);
-        hipLaunchKernelGGL(sph_gather_kernel, blocks, dim3(256), 0, stream, parts, start, end, g, 1.f / h,
-                           make_float3(dmin[0], dmin[1], dmin[2]), make_float3(dmax[0], dmax[1], dmax[2]), nx, ny,
-                           nz, grid);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(fb), dim3(256), 0, stream, out.start, ncells, kSphEmpty);
+        hipLaunchKernelGGL(cell_range_kernel, dim3(pb), dim3(256), 0, stream, keys_s, n, out.start, out.end);
     }
-    for (void *p : {(void *)keys, (void *)ids, (void *)keys_s, (void *)ids_s, (void *)parts, (void *)start, (void *)end,
-                    tmp})
+    for (void *p : {(void *)keys, (void *)ids, (void *)keys_s, (void *)ids_s, tmp})
         if (p)
             (void)hipFreeAsync(p, stream);
+    if (rc != 0)
+        sph_free(out, persistent, stream);
     return rc;
+}
+
+void sph_free(SphBuffers &b, bool persistent, hipStream_t stream)
+{
+    for (void *p : {(void *)b.parts, (void *)b.start, (void *)b.end})
+        if (p)
+            (void)(persistent ? hipFree(p) : hipFreeAsync(p, stream));
+    b.parts = nullptr;
+    b.start = b.end = nullptr;
+}
+
+int launch_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
+                             const float *dmin, const float *dmax, int nx, int ny, int nz, float *grid,
+                             hipStream_t stream)
+{
+    SphBuffers b;
+    int rc = sph_build(x, y, z, v, n, h, false, b, stream);
+    if (rc != 0)
+        return rc;
+    dim3 blocks((nx + 7) / 8, (ny + 7) / 8, (nz + 3) / 4);
+    hipLaunchKernelGGL(sph_gather_kernel, blocks, dim3(256), 0, stream, b.view(),
+                       make_float3(dmin[0], dmin[1], dmin[2]), make_float3(dmax[0], dmax[1], dmax[2]), nx, ny, nz,
+                       grid);
+    sph_free(b, false, stream);
+    return 0;
 }
 
 } // namespace mray

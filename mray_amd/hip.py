@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import pathlib
 
+import numpy as np
 import torch
 
 from .types import Camera, GenParams, RecParams, RefParams, MRAY_DTYPE_F32, MRAY_DTYPE_U8, MRAY_FORM_MIXED, MRAY_FORM_PLAIN
@@ -24,10 +25,14 @@ _SIGNATURES = {
     "mray_hip_volume_create": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f3, _f3, C.POINTER(_vp)], C.c_int),
     "mray_hip_volume_wrap": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f3, _f3, C.POINTER(_vp)], C.c_int),
     "mray_hip_volume_destroy": ([_vp], C.c_int),
+    "mray_hip_volume_from_particles": ([_vp, _vp, _vp, _vp, C.c_uint32, C.c_float, _f3, _f3, C.POINTER(_vp)], C.c_int),
     "mray_hip_volume_data": ([_vp], _vp),
     "mray_hip_generate": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, C.c_int, _vp,
                            _vp, _vp, _vp, _vp], C.c_int),
     "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_prepare_f64": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
+    "mray_hip_reconstruct_f64": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                  C.POINTER(RecParams), _vp, _vp, _vp], C.c_int),
     "mray_hip_reconstruct": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                               C.POINTER(RecParams), C.c_int, _vp, _vp, _vp], C.c_int),
     "mray_hip_reference": ([_vp, C.POINTER(Camera), _vp, C.POINTER(RefParams), C.c_int, C.c_int, _vp, _vp, _vp],
@@ -40,6 +45,7 @@ _SIGNATURES = {
     "mray_hip_compute_num_moments": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_moment_index": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_moment_index_noalloc": ([_vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_clamp_num_moments": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp], C.c_int),
     "mray_hip_compact": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "mray_hip_prediction_encode": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_prepare_coded": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
@@ -54,6 +60,8 @@ _SIGNATURES = {
     "mray_hip_synth_mixing_layer": ([_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.c_uint64, _vp], C.c_int),
     "mray_hip_synth_marschner_lobb": ([_vp, C.c_int, _vp], C.c_int),
+    "mray_hip_synth_noise_blobs": ([_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), C.c_int, C.c_uint64, _vp], C.c_int),
 }
 
 _lib = None
@@ -154,11 +162,32 @@ class Volume:
             pass
 
 
+class ParticleVolume(Volume):
+    """A particle set marched directly (mray_hip_volume_from_particles): every
+    generate sample is the reference's Particles::sample of the particles
+    (device arrays x, y, z, values) with smoothing length h, over the domain
+    [dmin, dmax]."""
+
+    def __init__(self, x: torch.Tensor, y: torch.Tensor, z: torch.Tensor, v: torch.Tensor, h: float, dmin, dmax):
+        init(x.device.index or 0)
+        self.voxels = None
+        self._parts = [t.contiguous() for t in (x, y, z, v)]
+        if len({t.numel() for t in self._parts}) != 1 or any(t.dtype != torch.float32 for t in self._parts):
+            raise MrayError("particle arrays must be float32 and of equal length")
+        self.dmin = tuple(float(c) for c in dmin)
+        self.dmax = tuple(float(c) for c in dmax)
+        hnd = _vp()
+        _check(load_library().mray_hip_volume_from_particles(*[_ptr(t) for t in self._parts], self._parts[0].numel(),
+                                                             C.c_float(h), _f3(*self.dmin), _f3(*self.dmax),
+                                                             C.byref(hnd)), "mray_hip_volume_from_particles")
+        self.handle = hnd
+
+
 def generate(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, form="mixed", moments=None,
              bounds=None, samples=None, want_samples=False, ray_range=None, stream=None):
     """Dense moments (H, W, m), packed bounds (H*W,) uint32 (or None), samples per ray (or None).
     ray_range: optional (H*W, 2) float32 tensor that receives each ray's normalisation (lo, hi)."""
-    dev = vol.voxels.device
+    dev = vol.voxels.device if vol.voxels is not None else vol._parts[0].device
     if moments is None:
         moments = torch.empty((H, W, m), dtype=torch.float32, device=dev)
     if bounds is None and gp.transform_to_bounds:
@@ -175,6 +204,15 @@ def generate(vol: Volume, cam: dict, gp: GenParams, W: int, H: int, m: int, form
 
 def prepare(moments: torch.Tensor, W: int, H: int, m: int, bias: float, form="mixed", index=None, out=None,
             stream=None):
+    """Lagrange multipliers, layout of the moments: float for "plain" and
+    "mixed", FP64 for "mixed64" (mray_hip_prepare_f64)."""
+    if form == "mixed64":
+        if out is None:
+            out = torch.empty(max(moments.numel(), 1), dtype=torch.float64,
+                              device=moments.device)[: moments.numel()].view(moments.shape)
+        _check(load_library().mray_hip_prepare_f64(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias), _ptr(out),
+                                                   _stream(stream)), "mray_hip_prepare_f64")
+        return out
     if out is None:
         out = _like(moments)
     _check(load_library().mray_hip_prepare(_ptr(moments), _ptr(index), W, H, m, C.c_float(bias), form_id(form),
@@ -214,6 +252,13 @@ def reconstruct(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m
     tf = tf.to(device=dev, dtype=torch.float32).contiguous()
     if tf.numel() != 1024:
         raise MrayError("transfer function table must have 256 RGBA entries")
+    if lagrange.dtype == torch.float64:
+        # FP64 multipliers of prepare(..., form="mixed64"): the mixed march
+        _check(load_library().mray_hip_reconstruct_f64(C.byref(c), _f3(*dmin), _f3(*dmax), _ptr(lagrange),
+                                                       _ptr(index), _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp),
+                                                       _ptr(rgba), _ptr(steps), _stream(stream)),
+               "mray_hip_reconstruct_f64")
+        return rgba, steps
     _check(load_library().mray_hip_reconstruct(C.byref(c), _f3(*dmin), _f3(*dmax), _ptr(lagrange), _ptr(index),
                                                _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp), form_id(form),
                                                _ptr(rgba), _ptr(steps), _stream(stream)), "mray_hip_reconstruct")
@@ -366,9 +411,20 @@ def curve_table(data: torch.Tensor, index: torch.Tensor, W: int, H: int, m: int,
     return list(t)
 
 
-def compact_image(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None, stream=None):
-    """The compaction of generate with compact_image set: (compact data, index)."""
+def clamp_num_moments(counts: torch.Tensor, W: int, H: int, lo: int, hi: int, stream=None) -> torch.Tensor:
+    """Counts of non-empty pixels clamped to [lo, hi] in place (mray_hip_clamp_num_moments)."""
+    _check(load_library().mray_hip_clamp_num_moments(_ptr(counts), W, H, lo, hi, _stream(stream)),
+           "mray_hip_clamp_num_moments")
+    return counts
+
+
+def compact_image(moments: torch.Tensor, W: int, H: int, m: int, error_threshold: float, bounds=None, stream=None,
+                  count_range=None):
+    """The compaction of generate with compact_image set: (compact data, index).
+    count_range (lo, hi): the moment counts of non-empty pixels clamped to it."""
     counts = compute_num_moments(moments, W, H, m, error_threshold, bounds=bounds, stream=stream)
+    if count_range is not None:
+        clamp_num_moments(counts, W, H, count_range[0], count_range[1], stream=stream)
     index = moment_index(counts, W, H, stream=stream)
     return compact(moments, index, W, H, m, stream=stream), index
 
@@ -426,4 +482,24 @@ def synth_marschner_lobb(n: int, device="cuda", stream=None) -> torch.Tensor:
     out = torch.empty((n, n, n), dtype=torch.float32, device=device)
     _check(load_library().mray_hip_synth_marschner_lobb(_ptr(out), n, _stream(stream)),
            "mray_hip_synth_marschner_lobb")
+    return out
+
+
+def synth_noise_blobs(nx: int, ny: int, nz: int, seed: int = 2, device="cuda", stream=None) -> torch.Tensor:
+    """The noise-and-blobs uint8 volume (nz, ny, nx) with the seeded blob
+    parameters of synth.blob_params (normalised coordinates, so a smaller
+    volume samples the same field)."""
+    from .synth import blob_params
+
+    init(torch.device(device).index or 0)
+    centres, sigmas, amps = blob_params(seed)
+    out = torch.empty((nz, ny, nx), dtype=torch.uint8, device=device)
+    fp = C.POINTER(C.c_float)
+    cc = np.ascontiguousarray(centres, np.float32)
+    ss = np.ascontiguousarray(sigmas, np.float32)
+    aa = np.ascontiguousarray(amps, np.float32)
+    _check(load_library().mray_hip_synth_noise_blobs(_ptr(out), nx, ny, nz, cc.ctypes.data_as(fp),
+                                                     ss.ctypes.data_as(fp), aa.ctypes.data_as(fp), len(aa),
+                                                     C.c_uint64(seed), _stream(stream)),
+           "mray_hip_synth_noise_blobs")
     return out

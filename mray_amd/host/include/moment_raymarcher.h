@@ -18,8 +18,9 @@
 // bounds), then compacts and prediction-codes it as params ask.
 void generate(cut::Camera cam, const RegularGrid &volume, MomentImageHost &img, const GenerationParameters &params);
 
-// Particles are first sampled onto a grid of spacing smoothing_length / 2
-// (Particles::get_step_size) over their domain, then marched as a grid.
+// Particles are marched directly: every ray sample is Particles::sample (the
+// cubic-spline weighted mean of the values of particles closer than the
+// smoothing length, through the reference's cell grid).
 void generate(cut::Camera cam, const Particles &volume, MomentImageHost &img, const GenerationParameters &params);
 
 // Renders the moment image into RGBA8 (clamped and truncated, as the reference).

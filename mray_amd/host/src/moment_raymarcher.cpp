@@ -181,20 +181,16 @@ void generate(cut::Camera cam, const Particles &volume, MomentImageHost &img, co
     const uint32_t n = volume.size();
     if (n == 0 || volume.y.size() != n || volume.z.size() != n || volume.values.size() != n)
         throw invalid_data("particle arrays must be non-empty and of equal length");
-    const float spacing = volume.get_step_size();
     const Vec3f lo = volume.domain.min, hi = volume.domain.max;
-    auto cells = [&](float a, float b) { return std::max(1, (int)std::ceil((b - a) / spacing)); };
-    const int nx = cells(lo.x, hi.x), ny = cells(lo.y, hi.y), nz = cells(lo.z, hi.z);
     DeviceBuffer dx(volume.x.data(), n * 4), dy(volume.y.data(), n * 4), dz(volume.z.data(), n * 4),
         dv(volume.values.data(), n * 4);
-    DeviceBuffer grid((size_t)nx * ny * nz * sizeof(float));
     const float dmin[3] = {lo.x, lo.y, lo.z};
     const float dmax[3] = {hi.x, hi.y, hi.z};
-    check(mray_hip_particles_to_grid(dx.as<float>(), dy.as<float>(), dz.as<float>(), dv.as<float>(), n,
-                                     volume.smoothing_length, dmin, dmax, nx, ny, nz, grid.as<float>(), nullptr),
-          "particles_to_grid");
+    // the particle set itself: every ray sample is Particles::sample
     VolumeHandle vh;
-    check(mray_hip_volume_wrap(grid.ptr, MRAY_DTYPE_F32, nx, ny, nz, dmin, dmax, &vh.v), "volume_wrap");
+    check(mray_hip_volume_from_particles(dx.as<float>(), dy.as<float>(), dz.as<float>(), dv.as<float>(), n,
+                                         volume.smoothing_length, dmin, dmax, &vh.v),
+          "volume_from_particles");
     generate_on(vh.v, cam, img, params);
 }
 

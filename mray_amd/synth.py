@@ -12,6 +12,8 @@ Workloads (sizes as in the benchmark configurations):
     along +z;
   * mixing layer on [0,1]^3: 0.5 + 0.5 tanh((z - 0.5 - h(x,y)) / 0.01) plus
     4-octave value noise of amplitude 0.05, perspective camera;
+  * large uint8 volume: 6-octave value noise through a smoothstep plus 64
+    Gaussian blobs of 20-80 voxels (at 2048 x 2048 x 1920), seed 2;
   * 256-entry RGBA8 transfer functions made of piecewise-linear ramps;
   * SPH particles drawn from a mixture of anisotropic Gaussians.
 """
@@ -130,6 +132,35 @@ def mixing_layer(n: int, seed: int = 1, width: float = 0.01) -> np.ndarray:
         noise = value_noise(x2, y2, np.full_like(x2, z), seed, 4, 8)
         out[iz] = (layer + 0.05 * noise).astype(np.float32)
     return out
+
+
+# the noise-and-blobs volume: normalised coordinates p = voxel index / nx on
+# every axis (isotropic voxels, domain [0,1] x [0,ny/nx] x [0,nz/nx]);
+# density v = clamp(BLOB_NOISE_WEIGHT * smoothstep(lo, hi, 0.5 + 0.5 noise6(p))
+# + sum_k a_k exp(-|p - c_k|^2 / (2 s_k^2)), 0, 1), stored as floor(255 v + 0.5)
+BLOB_FULL_SHAPE = (1920, 2048, 2048)  # (nz, ny, nx) of the full-size volume
+BLOB_NOISE_BASE = 4                    # lattice of octave 0 (per unit of p)
+BLOB_NOISE_OCTAVES = 6
+BLOB_NOISE_WEIGHT = 0.5
+BLOB_SMOOTH = (0.3, 0.7)               # smoothstep edges
+
+
+def blob_params(seed: int = 2, n_blobs: int = 64):
+    """Centres (n, 3) as (x, y, z) in normalised coordinates inside the
+    full-size box, widths s_k from 20 to 80 voxels of the full-size volume
+    (divided by its nx), amplitudes a_k from 0.4 to 1.0."""
+    nz, ny, nx = BLOB_FULL_SHAPE
+    b = np.arange(n_blobs)
+    ext = np.array([1.0, ny / nx, nz / nx])
+    centres = np.stack([hash_uniform(seed, 40 + d, b) * ext[d] for d in range(3)], axis=1)
+    sigmas = (20.0 + 60.0 * hash_uniform(seed, 43, b)) / nx
+    amps = 0.4 + 0.6 * hash_uniform(seed, 44, b)
+    return centres.astype(np.float32), sigmas.astype(np.float32), amps.astype(np.float32)
+
+
+def blob_volume_shape(scale: int = 1):
+    """(nz, ny, nx) of the noise-and-blobs volume at 1/scale of full size."""
+    return tuple(n // scale for n in BLOB_FULL_SHAPE)
 
 
 # --------------------------------------------------------------------------

@@ -178,6 +178,8 @@ static inline float vol_get(const orc_volume *v, int x, int y, int z)
 
 float orc_volume_sample(const orc_volume *v, const float p[3])
 {
+    if (v->dtype == MRAY_DTYPE_PARTICLES)
+        return orc_sph_sample((const orc_sph *)v->data, p);
     float ip[3], a[3];
     int il[3];
     const int n[3] = {v->nx, v->ny, v->nz};
@@ -205,6 +207,11 @@ float orc_volume_sample(const orc_volume *v, const float p[3])
 
 double orc_volume_sample_f64(const orc_volume *v, const double p[3])
 {
+    if (v->dtype == MRAY_DTYPE_PARTICLES)
+    {
+        const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+        return orc_sph_sample((const orc_sph *)v->data, pf);
+    }
     double a[3];
     int il[3];
     const int n[3] = {v->nx, v->ny, v->nz};
@@ -1383,9 +1390,22 @@ void orc_reconstruct_lagrange64_f64(const mray_camera *cam, const float bmin[3],
                                     int m, const double *lagrange, const uint32_t *bounds, const double *tf,
                                     const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads)
 {
+    orc_reconstruct_lagrange64_rows_f64(cam, bmin, bmax, W, H, m, lagrange, bounds, tf, rp, rgba, n_steps, 0, 1,
+                                        nthreads);
+}
+
+void orc_reconstruct_lagrange64_rows_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W,
+                                         int H, int m, const double *lagrange, const uint32_t *bounds,
+                                         const double *tf, const mray_rec_params *rp, double *rgba, int32_t *n_steps,
+                                         int row0, int row_step, int nthreads)
+{
+    if (row_step < 1)
+        row_step = 1;
+    const int nrows = (H - row0 + row_step - 1) / row_step;
 #pragma omp parallel for num_threads(set_threads(nthreads))
-    for (int y = 0; y < H; ++y)
+    for (int r = 0; r < nrows; ++r)
     {
+        const int y = row0 + r * row_step;
         for (int x = 0; x < W; ++x)
         {
             size_t pix = (size_t)y * W + x;
@@ -1849,16 +1869,22 @@ static int kv_cmp(const void *a, const void *b)
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
 }
 
-/* Particles::create_grid + Particles::sample evaluated at the voxel positions
- * of an nx*ny*nz grid over [dmin, dmax] (voxel i at dmin + i*(dmax-dmin)/n,
- * the RegularGrid::sample convention). Particles are ordered by cell with a
- * stable order inside a cell (the reference's std::sort leaves that order
- * unspecified). */
-void orc_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
-                           const float dmin[3], const float dmax[3], int nx, int ny, int nz, float *grid,
-                           int nthreads)
+struct orc_sph
 {
-    /* createGridFromDomainCPU (src/particle_grid.cpp:38-55) + createGridFromDomain (.cuh:389-400) */
+    sph_grid g;
+    float *sx, *sy, *sz, *sv; /* particles in cell order */
+    uint32_t *start, *count;  /* per cell: first particle (0xFFFFFFFF when empty), count */
+    float inv_h;
+};
+
+/* Particles::create_grid (src/volume.cpp:55-81): the uniform grid of cell
+ * size h over the particles' bounding box (createGridFromDomainCPU,
+ * src/particle_grid.cpp:38-55; createGridFromDomain,
+ * src/particle_grid.cuh:389-400), particles ordered by cell with a stable
+ * order inside a cell (the reference's std::sort leaves that order
+ * unspecified). */
+orc_sph *orc_sph_create(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h)
+{
     float mn[3] = {x[0], y[0], z[0]}, mx[3] = {x[0], y[0], z[0]};
     for (uint32_t i = 1; i < n; ++i)
     {
@@ -1869,59 +1895,121 @@ void orc_particles_to_grid(const float *x, const float *y, const float *z, const
             mx[k] = (mx[k] < q[k]) ? q[k] : mx[k];
         }
     }
-    sph_grid g;
+    orc_sph *s = (orc_sph *)calloc(1, sizeof(orc_sph));
+    sph_grid *g = &s->g;
     float cell = h;
     float inv_cellf = 1 / cell; /* Vec3 / scalar multiplies by the reciprocal (src/cut/cuda_math.h:467-472) */
     uint32_t gs[3];
     for (int k = 0; k < 3; ++k)
         gs[k] = (uint32_t)((mx[k] - mn[k]) * inv_cellf + 1);
-    g.sx = next_multiple(gs[0], 4);
-    g.sy = next_multiple(gs[1], 4);
-    g.sz = (gs[2] > 1) ? next_multiple(gs[2], 4) : gs[2];
+    g->sx = next_multiple(gs[0], 4);
+    g->sy = next_multiple(gs[1], 4);
+    g->sz = (gs[2] > 1) ? next_multiple(gs[2], 4) : gs[2];
     for (int k = 0; k < 3; ++k)
-        g.minp[k] = mn[k];
-    g.inv_cell = (float)(1.0 / (double)cell);
-    g.r2 = cell * cell;
-    g.total = g.sx * g.sy * g.sz;
+        g->minp[k] = mn[k];
+    g->inv_cell = (float)(1.0 / (double)cell);
+    g->r2 = cell * cell;
+    g->total = g->sx * g->sy * g->sz;
 
     kv *keys = (kv *)malloc(sizeof(kv) * n);
     for (uint32_t i = 0; i < n; ++i)
     {
         const float q[3] = {x[i], y[i], z[i]};
         int c[3];
-        sph_get_cell(&g, q, c);
+        sph_get_cell(g, q, c);
         /* the largest coordinate can round to one cell past a padded size
          * (the reference then indexes past its cell array): clamp */
-        const uint32_t s3[3] = {g.sx, g.sy, g.sz};
+        const uint32_t s3[3] = {g->sx, g->sy, g->sz};
         for (int k = 0; k < 3; ++k)
             c[k] = c[k] < 0 ? 0 : (c[k] >= (int)s3[k] ? (int)s3[k] - 1 : c[k]);
-        keys[i].key = sph_cell_index(&g, c[0], c[1], c[2]);
+        keys[i].key = sph_cell_index(g, c[0], c[1], c[2]);
         keys[i].idx = i;
     }
     qsort(keys, n, sizeof(kv), kv_cmp);
-    float *sx = (float *)malloc(sizeof(float) * n * 4);
-    float *sy = sx + n, *sz = sy + n, *sv = sz + n;
+    s->sx = (float *)malloc(sizeof(float) * n * 4);
+    s->sy = s->sx + n;
+    s->sz = s->sy + n;
+    s->sv = s->sz + n;
     for (uint32_t i = 0; i < n; ++i)
     {
-        sx[i] = x[keys[i].idx];
-        sy[i] = y[keys[i].idx];
-        sz[i] = z[keys[i].idx];
-        sv[i] = v[keys[i].idx];
+        s->sx[i] = x[keys[i].idx];
+        s->sy[i] = y[keys[i].idx];
+        s->sz[i] = z[keys[i].idx];
+        s->sv[i] = v[keys[i].idx];
     }
-    uint32_t *start = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)g.total);
-    uint32_t *count = (uint32_t *)calloc((size_t)g.total, sizeof(uint32_t));
-    for (uint32_t c = 0; c < g.total; ++c)
-        start[c] = 0xFFFFFFFFu;
+    s->start = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)g->total);
+    s->count = (uint32_t *)calloc((size_t)g->total, sizeof(uint32_t));
+    for (uint32_t c = 0; c < g->total; ++c)
+        s->start[c] = 0xFFFFFFFFu;
     for (uint32_t i = 0; i < n; ++i)
     {
         uint32_t c = keys[i].key;
         if (i == 0 || c != keys[i - 1].key)
-            start[c] = i;
-        count[c]++;
+            s->start[c] = i;
+        s->count[c]++;
     }
     free(keys);
+    s->inv_h = 1.f / h;
+    return s;
+}
 
-    float inv_h = 1.f / h;
+void orc_sph_destroy(orc_sph *s)
+{
+    if (!s)
+        return;
+    free(s->sx);
+    free(s->start);
+    free(s->count);
+    free(s);
+}
+
+/* Particles::sample (src/volume.cpp:83-94): InterpolateParticlesOp
+ * (src/particle_interpolate.h:35-58) over the particles of the 27 cells
+ * around p (cells in z, y, x order, particles in cell order) closer than h,
+ * normalised by the weight sum when it is not 0 */
+float orc_sph_sample(const orc_sph *s, const float p[3])
+{
+    const sph_grid *g = &s->g;
+    int c[3];
+    sph_get_cell(g, p, c);
+    float value = 0.0f, wsum = 0.0f;
+    int z0 = (c[2] - 1 > 0) ? c[2] - 1 : 0, z1 = (c[2] + 2 < (int)g->sz) ? c[2] + 2 : (int)g->sz;
+    int y0 = (c[1] - 1 > 0) ? c[1] - 1 : 0, y1 = (c[1] + 2 < (int)g->sy) ? c[1] + 2 : (int)g->sy;
+    int x0 = (c[0] - 1 > 0) ? c[0] - 1 : 0, x1 = (c[0] + 2 < (int)g->sx) ? c[0] + 2 : (int)g->sx;
+    for (int cz = z0; cz < z1; ++cz)
+        for (int cy = y0; cy < y1; ++cy)
+            for (int cx = x0; cx < x1; ++cx)
+            {
+                uint32_t ci = sph_cell_index(g, cx, cy, cz);
+                if (s->start[ci] == 0xFFFFFFFFu)
+                    continue;
+                for (uint32_t q = s->start[ci]; q < s->start[ci] + s->count[ci]; ++q)
+                {
+                    float dx = p[0] - s->sx[q], dy = p[1] - s->sy[q], dz = p[2] - s->sz[q];
+                    float d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < g->r2)
+                    {
+                        /* InterpolateParticlesOp (src/particle_interpolate.h:45-51) */
+                        float ex = s->sx[q] - p[0], ey = s->sy[q] - p[1], ez = s->sz[q] - p[2];
+                        float w = orc_cubic_spline(sqrtf(ex * ex + ey * ey + ez * ez) * s->inv_h);
+                        wsum += w;
+                        value += w * s->sv[q];
+                    }
+                }
+            }
+    if (wsum != 0.0f)
+        value /= wsum;
+    return value;
+}
+
+/* Particles::sample evaluated at the voxel positions of an nx*ny*nz grid over
+ * [dmin, dmax] (voxel i at dmin + i*(dmax-dmin)/n, the RegularGrid::sample
+ * convention). */
+void orc_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
+                           const float dmin[3], const float dmax[3], int nx, int ny, int nz, float *grid,
+                           int nthreads)
+{
+    orc_sph *s = orc_sph_create(x, y, z, v, n, h);
 #pragma omp parallel for num_threads(set_threads(nthreads)) collapse(2)
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
@@ -1932,40 +2020,9 @@ void orc_particles_to_grid(const float *x, const float *y, const float *z, const
                 const int nn[3] = {nx, ny, nz};
                 for (int k = 0; k < 3; ++k)
                     p[k] = dmin[k] + (float)ii[k] / (float)nn[k] * (dmax[k] - dmin[k]);
-                int c[3];
-                sph_get_cell(&g, p, c);
-                float value = 0.0f, wsum = 0.0f;
-                int z0 = (c[2] - 1 > 0) ? c[2] - 1 : 0, z1 = (c[2] + 2 < (int)g.sz) ? c[2] + 2 : (int)g.sz;
-                int y0 = (c[1] - 1 > 0) ? c[1] - 1 : 0, y1 = (c[1] + 2 < (int)g.sy) ? c[1] + 2 : (int)g.sy;
-                int x0 = (c[0] - 1 > 0) ? c[0] - 1 : 0, x1 = (c[0] + 2 < (int)g.sx) ? c[0] + 2 : (int)g.sx;
-                for (int cz = z0; cz < z1; ++cz)
-                    for (int cy = y0; cy < y1; ++cy)
-                        for (int cx = x0; cx < x1; ++cx)
-                        {
-                            uint32_t ci = sph_cell_index(&g, cx, cy, cz);
-                            if (start[ci] == 0xFFFFFFFFu)
-                                continue;
-                            for (uint32_t q = start[ci]; q < start[ci] + count[ci]; ++q)
-                            {
-                                float dx = p[0] - sx[q], dy = p[1] - sy[q], dz = p[2] - sz[q];
-                                float d2 = dx * dx + dy * dy + dz * dz;
-                                if (d2 < g.r2)
-                                {
-                                    /* InterpolateParticlesOp (src/particle_interpolate.h:45-51) */
-                                    float ex = sx[q] - p[0], ey = sy[q] - p[1], ez = sz[q] - p[2];
-                                    float w = orc_cubic_spline(sqrtf(ex * ex + ey * ey + ez * ez) * inv_h);
-                                    wsum += w;
-                                    value += w * sv[q];
-                                }
-                            }
-                        }
-                if (wsum != 0.0f)
-                    value /= wsum;
-                grid[((size_t)iz * ny + iy) * nx + ix] = value;
+                grid[((size_t)iz * ny + iy) * nx + ix] = orc_sph_sample(s, p);
             }
-    free(sx);
-    free(start);
-    free(count);
+    orc_sph_destroy(s);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -31,7 +31,7 @@ extern "C" {
 
 typedef struct orc_volume
 {
-    const void *data; /* F32: float[nz][ny][nx], U8: uint8_t[nz][ny][nx] */
+    const void *data; /* F32: float[nz][ny][nx], U8: uint8_t[nz][ny][nx], PARTICLES: orc_sph */
     int32_t dtype;    /* MRAY_DTYPE_F32 / MRAY_DTYPE_U8 */
     int32_t nx, ny, nz;
     float dmin[3], dmax[3];
@@ -116,6 +116,11 @@ void orc_reconstruct_lagrange_f64(const mray_camera *cam, const float bmin[3], c
                                   int m, const float *lagrange, const uint32_t *bounds, const float *tf,
                                   const mray_rec_params *p, double *rgba, int32_t *n_steps, int nthreads);
 
+/* the FP64 march on FP64 multipliers and table, rows row0, row0 + row_step, ... */
+void orc_reconstruct_lagrange64_rows_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W,
+                                         int H, int m, const double *lagrange, const uint32_t *bounds,
+                                         const double *tf, const mray_rec_params *rp, double *rgba, int32_t *n_steps,
+                                         int row0, int row_step, int nthreads);
 void orc_reconstruct_lagrange64_f64(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H,
                                     int m, const double *lagrange, const uint32_t *bounds, const double *tf,
                                     const mray_rec_params *rp, double *rgba, int32_t *n_steps, int nthreads);
@@ -148,7 +153,14 @@ void orc_prediction_encode_image(int W, int H, int m, float *data, const uint32_
                                  uint16_t *coding_params_q, int nthreads);
 void orc_prepare_moments_pred(int m, const float *moments, int warp, const float *params, float *pmoments);
 
-/* ---- SPH gather (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh}, src/particle_interpolate.h) ---- */
+/* ---- SPH gather (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh}, src/particle_interpolate.h) ----
+ * A particle set (orc_sph_create) is also a volume: an orc_volume with dtype
+ * MRAY_DTYPE_PARTICLES and data the orc_sph samples it (Particles::sample)
+ * wherever generate samples a grid. */
+typedef struct orc_sph orc_sph;
+orc_sph *orc_sph_create(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h);
+void orc_sph_destroy(orc_sph *s);
+float orc_sph_sample(const orc_sph *s, const float p[3]);
 float orc_cubic_spline(float q);
 void orc_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
                            const float dmin[3], const float dmax[3], int nx, int ny, int nz, float *grid,

@@ -367,6 +367,25 @@ void ref_particles_sample(const float *x, const float *y, const float *z, const 
         out[i] = parts.sample(Vec3f(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
 }
 
+// ::generate on a Particles (src/moment_raymarcher.cpp:168-183 ->
+// generate_host over Particles::sample): dense moments and bounds.
+void ref_generate_particles(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
+                            const float *dmin, const float *dmax, const mray_camera *cam, int W, int H, int m,
+                            const mray_gen_params *p, float *data, uint32_t *bounds)
+{
+    Range<Vec3f> dom(Vec3f(dmin[0], dmin[1], dmin[2]), Vec3f(dmax[0], dmax[1], dmax[2]));
+    Particles parts(dom, vector<float>(x, x + n), vector<float>(y, y + n), vector<float>(z, z + n),
+                    vector<float>(v, v + n), h);
+    auto gp = make_gen(p);
+    MomentImageHost mi(W, H, m, gp.compact_image, gp.transform_to_bounds);
+    mi.domain = dom;
+    generate(make_cam(cam), parts, mi, gp);
+    std::memcpy(data, mi.data.data(), mi.data.size() * sizeof(float));
+    if (bounds && mi.has_bounds())
+        for (size_t i = 0; i < mi.bounds.size(); ++i)
+            bounds[i] = (uint32_t)mi.bounds[i].lower | ((uint32_t)mi.bounds[i].upper << 16);
+}
+
 // moment_quantization::quantize_prediction_coding(mi, table) /
 // moment_quantization::quantize(mi, bits) (src/moment_quantization.cpp:244-329).
 // Returns the stream length in bytes.

@@ -273,7 +273,13 @@ def sph_vectors(R: Reference) -> dict:
     bmax = (C.c_float * 3)(*dmax)
     R.lib.ref_particles_sample(ptr(x, _f), ptr(y, _f), ptr(z, _f), ptr(v, _f), C.c_uint32(len(x)), C.c_float(h),
                                bmin, bmax, ptr(pts, _f), C.c_uint32(len(pts)), ptr(res, _f))
-    return dict(sph_x=x, sph_y=y, sph_z=z, sph_v=v, sph_h=f32([h]), sph_grid=res.reshape(n[2], n[1], n[0]))
+    # ::generate on the particle set itself: Particles::sample at every ray sample
+    cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=24.0, aspect=1.0)
+    gp = GenParams(step_size=0.02, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    mo, b = R.generate_particles(x, y, z, v, float(h), dmin, dmax, cam, 14, 14, 12, gp)
+    return dict(sph_x=x, sph_y=y, sph_z=z, sph_v=v, sph_h=f32([h]), sph_grid=res.reshape(n[2], n[1], n[0]),
+                sph_march_moments=mo, sph_march_bounds=b)
 
 
 def quant_vectors(R: Reference) -> dict:

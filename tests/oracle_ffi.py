@@ -73,6 +73,26 @@ def stream_bytes(index, bits):
     return int(cum[counts].sum()) // 8
 
 
+class Particles:
+    """A particle set of the oracle (orc_sph_create): Particles::create_grid
+    of the reference; pass it to Oracle.generate in place of a volume."""
+
+    def __init__(self, oracle, x, y, z, v, h):
+        self._lib = oracle.lib
+        self._arrays = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, v)]
+        fp = C.POINTER(C.c_float)
+        self._lib.orc_sph_create.restype = C.c_void_p
+        self._lib.orc_sph_create.argtypes = [fp, fp, fp, fp, C.c_uint32, C.c_float]
+        self._lib.orc_sph_destroy.argtypes = [C.c_void_p]
+        self.handle = self._lib.orc_sph_create(*[a.ctypes.data_as(fp) for a in self._arrays], len(self._arrays[0]),
+                                               C.c_float(h))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.orc_sph_destroy(self.handle)
+            self.handle = None
+
+
 class Oracle:
     """numpy-level wrappers over liboracle.so."""
 
@@ -95,8 +115,18 @@ class Oracle:
         L.orc_generate_ray.restype = C.c_uint32
 
     @staticmethod
-    def volume(vol: np.ndarray, dmin, dmax):
+    def volume(vol, dmin, dmax):
         v = OrcVolume()
+        if isinstance(vol, Particles):
+            # a particle set sampled by Particles::sample at every ray sample
+            v.data = vol.handle
+            v.dtype = 2
+            v.nz = v.ny = v.nx = 1
+            for i in range(3):
+                v.dmin[i] = dmin[i]
+                v.dmax[i] = dmax[i]
+            v._keep = vol
+            return v
         v.data = vol.ctypes.data
         v.dtype = 1 if vol.dtype == np.uint8 else 0
         v.nz, v.ny, v.nx = vol.shape
@@ -294,6 +324,18 @@ class Reference:
                                   bmin, bmax, C.byref(c), W, H, m, C.byref(gp), ptr(data, _f), ptr(bounds, _u32),
                                   ptr(index, _u32), ptr(cp, _u16))
         return data[:n], bounds, index, cp
+
+    def generate_particles(self, x, y, z, v, h, dmin, dmax, cam, W, H, m, gp: GenParams):
+        """::generate on a Particles (per-sample Particles::sample): dense moments, bounds."""
+        c = Camera.from_dict(cam)
+        data = np.zeros(W * H * m, np.float32)
+        bounds = np.zeros(W * H, np.uint32)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        arr = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, v)]
+        self.lib.ref_generate_particles(*[ptr(a, _f) for a in arr], C.c_uint32(len(arr[0])), C.c_float(h), bmin,
+                                        bmax, C.byref(c), W, H, m, C.byref(gp), ptr(data, _f), ptr(bounds, _u32))
+        return data.reshape(H, W, m), bounds
 
     def quantize(self, W, H, data, index, prediction_coded, bits):
         out = np.zeros(stream_bytes(index, bits) + 8, np.uint8)

@@ -85,10 +85,24 @@ CASES = {
     "u8_ortho_m16": (lambda: np.clip(synth.mixing_layer(32) * 255, 0, 255).astype(np.uint8), (0, 0, 0), (1, 1, 1),
                      synth.ortho_camera(pos=(0.5, 0.5, -1.0), right=(0.6, 0, 0), up=(0, 0.6, 0)), 24, 24, 16,
                      dict(step_size=1.0 / 64, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
-    # the largest mixed generate / reconstruct instantiation (M = 64)
+    # 64 moments: the mixed generate in two moment chunks
     "ml_ortho_m64": (lambda: synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 20, 16, 64,
                      dict(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
                           prediction_coding=0, bias=5e-5)),
+    # beyond one thread's registers: moment chunks over workgroups (generate),
+    # 20-32 element slots per lane (prepare), four lanes per ray above 128
+    # moments (reconstruct)
+    "ml_ortho_m100": (lambda: synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 20, 16, 100,
+                      dict(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                           prediction_coding=0, bias=5e-5)),
+    "ml_persp_m200": (lambda: synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1),
+                      synth.perspective_camera(position=(0.15, -0.1, -2.6), fov_half_deg=24.0, znear=0.01,
+                                               aspect=22 / 18), 22, 18, 200,
+                      dict(step_size=2.0 / 64, transform_to_bounds=1, transform_bound_eps=0.005, compact_image=0,
+                           prediction_coding=0)),
+    "u8_ortho_m255": (lambda: np.clip(synth.mixing_layer(24, width=0.05) * 255, 0, 255).astype(np.uint8), (0, 0, 0),
+                      (1, 1, 1), synth.ortho_camera(pos=(0.5, 0.5, -1.0), right=(0.6, 0, 0), up=(0, 0.6, 0)), 17, 15,
+                      255, dict(step_size=1.0 / 48, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
     "tiny_m1": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 1,
                 dict(step_size=2.0 / 20, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
     "tiny_m2": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 2,
@@ -114,14 +128,16 @@ def gpu_volume(hip, vol, dmin, dmax):
 # opacity scale of the transfer function per case (the reference's
 # preintegration_step_size_factor), chosen for a partly transparent image
 TF_FACTOR = {"ml_ortho_m64": 4.0, "ml_ortho_m30": 4.0, "ml_persp_m40": 4.0, "mix_persp_m50": 40.0, "mixw_persp_m50": 40.0,
-             "ml_nobounds_m7": 4.0, "u8_ortho_m16": 20.0, "tiny_m1": 4.0, "tiny_m2": 4.0}
+             "ml_nobounds_m7": 4.0, "u8_ortho_m16": 20.0, "tiny_m1": 4.0, "tiny_m2": 4.0, "ml_ortho_m100": 4.0,
+             "ml_persp_m200": 4.0, "u8_ortho_m255": 20.0}
 
 
 # MESE bias (lerp toward the moments of a uniform density) per case: the
 # sharp mixing-layer and uint8 volumes at 16-50 moments sit so close to the
 # boundary of the moment space that at 5e-5 even float storage of exact
 # moments moves the image by ~6e-4 RMSE; 1e-3 makes them well posed.
-REC_BIAS = {"mix_persp_m50": 1e-3, "mixw_persp_m50": 1e-3, "u8_ortho_m16": 1e-3}
+REC_BIAS = {"mix_persp_m50": 1e-3, "mixw_persp_m50": 1e-3, "u8_ortho_m16": 1e-3, "u8_ortho_m255": 1e-3,
+            "ml_persp_m200": 1e-3}
 
 
 def rec_params(gp, ert=0.99, name=None):
@@ -200,7 +216,7 @@ def densities64(O, m, lag, phases):
 
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 @pytest.mark.parametrize("name", ["ml_ortho_m30", "mix_persp_m50", "mixw_persp_m50", "tiny_m1", "tiny_m2",
-                                  "ml_nobounds_m7"])
+                                  "ml_nobounds_m7", "ml_ortho_m100"])
 def test_prepare_parity(hip, O, name, form):
     """Densities of the GPU multipliers against the oracle's, at 41 phases.
 
@@ -241,12 +257,13 @@ def test_prepare_parity(hip, O, name, form):
     assert excess.max() <= 0.0
 
 
-@pytest.mark.parametrize("m", [64, 72, 100, 128])
+@pytest.mark.parametrize("m", [64, 72, 100, 128, 160, 200, 248, 255])
 def test_prepare_mixed_large_m(hip, O, m):
-    """MIXED preparation beyond the parity cases' 50 moments, up to the
-    kernel's 128 (16 element slots per lane): densities against the FP64
+    """MIXED preparation beyond the parity cases' 50 moments, up to 255 (16
+    element slots per lane up to 128, then 20-32; at 32 the per-pixel LDS
+    arrays lose their bank padding to fit 64 KiB): densities against the FP64
     oracle at 41 phases with the float-storage bound of test_prepare_parity.
-    Moments from the oracle's generate (the mixed generate stops at 64)."""
+    Moments from the oracle's generate."""
     import torch
 
     vol = synth.marschner_lobb(32)
@@ -273,7 +290,7 @@ def test_prepare_mixed_large_m(hip, O, m):
 
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 @pytest.mark.parametrize("name", ["ml_ortho_m30", "ml_persp_m40", "mix_persp_m50", "mixw_persp_m50", "u8_ortho_m16",
-                                  "tiny_m2", "ml_ortho_m64"])
+                                  "tiny_m2", "ml_ortho_m64", "ml_ortho_m100", "ml_persp_m200", "u8_ortho_m255"])
 def test_reconstruct_parity(hip, O, name, form):
     """The march alone, on the FP32 oracle's multipliers: RMSE against the
     FP64 evaluation of the same multipliers <= 1e-4, or no worse than 1.5x the
@@ -317,7 +334,8 @@ def test_reconstruct_parity(hip, O, name, form):
         assert ident.all()
 
 
-@pytest.mark.parametrize("name", ["ml_ortho_m30", "mixw_persp_m50", "ml_persp_m40", "mix_persp_m50"])
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "mixw_persp_m50", "ml_persp_m40", "mix_persp_m50", "ml_ortho_m100",
+                                  "ml_persp_m200"])
 def test_pipeline_mixed_vs_f64(hip, O, name):
     """Final float RGBA of the mixed pipeline against the FP64 oracle: RMSE <= 1e-4
     where the reference's own FP32 pipeline meets that; on the ill-conditioned

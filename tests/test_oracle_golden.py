@@ -453,3 +453,28 @@ def test_oracle_matches_live_reference_on_fresh_inputs():
     mo, b, _ = O.generate(vol, (-1, -1, -1), (1, 1, 1), cam, 30, 26, 20, gp)
     rd, rb, _, _ = R.generate(vol, (-1, -1, -1), (1, 1, 1), cam, 30, 26, 20, gp)
     assert bits_equal(mo.reshape(-1), rd) and np.array_equal(b, rb)
+
+
+def test_particle_march_matches_reference():
+    """generate on a particle set, Particles::sample at every ray sample
+    (src/volume.cpp:83-94 inside generate_ray): the oracle's march over its
+    particle set against the compiled reference's ::generate on a Particles
+    (golden_sph.npz). The reference's std::sort leaves the order inside a cell
+    unspecified, so sample sums may differ in the last bits: moments to 1e-5
+    normwise, 16-bit bounds within one step."""
+    g = dict(np.load(GOLD / "golden_sph.npz"))
+    O = Oracle()
+    from oracle_ffi import Particles
+
+    parts = Particles(O, g["sph_x"], g["sph_y"], g["sph_z"], g["sph_v"], float(g["sph_h"][0]))
+    cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=24.0, aspect=1.0)
+    gp = GenParams(step_size=0.02, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0)
+    mo, b, ns = O.generate(parts, (0, 0, 0), (1, 1, 0.5), cam, 14, 14, 12, gp)
+    ref = g["sph_march_moments"].astype(np.float64).reshape(-1, 12)
+    err = np.linalg.norm(mo.reshape(-1, 12) - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-3)
+    assert err.max() <= 1e-5
+    lo, hi = b & 0xFFFF, b >> 16
+    rlo, rhi = g["sph_march_bounds"] & 0xFFFF, g["sph_march_bounds"] >> 16
+    assert np.abs(lo.astype(int) - rlo.astype(int)).max() <= 1 and np.abs(hi.astype(int) - rhi.astype(int)).max() <= 1
+    assert np.count_nonzero(ns) > 0.5 * ns.size
