@@ -315,6 +315,48 @@ int mray_hip_quantization_table(const float *d_data, const uint32_t *d_index, in
 int mray_hip_dequantize(const uint8_t *d_stream, size_t stream_bytes, const uint32_t *d_index, int W, int H,
                         int prediction_coded, const uint8_t *bits, int n_bits, float *d_data, mray_stream stream);
 
+/* The quantized values of a dense image as the LZ4 path of
+ * write_static_image stores them (moment_quantization::quantize(moments,
+ * num_moments, bits), src/moment_quantization.cpp:361-385, called at
+ * src/moment_image_io.cpp:423-426): n values (pixel-major, m per pixel),
+ * value i quantized to bits bits over [0, 1] for moment 0 (i % m == 0) and
+ * [-1/pi, 1/pi] otherwise, stored as (bits + 7) / 8 little-endian bytes.
+ * d_out receives n * ((bits + 7) / 8) bytes. Bit-identical to the reference. */
+int mray_hip_quantize_dense(const float *d_data, size_t n, int m, int bits, uint8_t *d_out, mray_stream stream);
+
+/* The inverse (moment_quantization::dequantize(qs, num_moments, bits),
+ * src/moment_quantization.cpp:387-407): n values from n * ((bits + 7) / 8)
+ * bytes. */
+int mray_hip_dequantize_dense(const uint8_t *d_in, size_t n, int m, int bits, float *d_data, mray_stream stream);
+
+/* The entropy-coded moment section of a compact image (entropy_encode,
+ * src/moment_image_coding.cpp:202-217, as write_moment_image calls it with
+ * entropy_coding set, src/moment_image_io.cpp:383-396): u32 count of
+ * non-empty pixels, their moment 0 quantized to bits[0] bits as u16 words in
+ * pixel order, then for l = 1..m-1 a u32 length and the adaptive arithmetic
+ * code (Amir Said's FastAC coder with its adaptive multi-symbol model,
+ * src/entropy_coding/arithmetic_codec.cpp) of moment l, quantized to bits[l]
+ * bits (1..14), of every pixel with more than l moments. bits (host) holds m
+ * widths; value ranges are those of mray_hip_quantize. h_out is HOST memory:
+ * the section is written there when it fits capacity; *out_bytes receives its
+ * length either way (h_out NULL only measures it; a too-small capacity fails
+ * with MRAY_ERR_ARG). One wavefront codes each moment's stream on the GPU;
+ * uses stream-ordered scratch of about 4 bytes per moment. Waits for the
+ * stream. Byte-identical to the reference. */
+int mray_hip_entropy_encode(const float *d_data, const uint32_t *d_index, int W, int H, int m, int prediction_coded,
+                            const uint8_t *bits, uint8_t *h_out, size_t capacity, size_t *out_bytes,
+                            mray_stream stream);
+
+/* Decodes an entropy-coded moment section (entropy_decode,
+ * src/moment_image_coding.cpp:219-234, as read_moment_image calls it,
+ * src/moment_image_io.cpp:161-168) from HOST memory h_in (n_in bytes) into
+ * the compact data of the image with index d_index; same widths and ranges
+ * as mray_hip_entropy_encode. Bytes past a stream's end read as zero. Fails
+ * with MRAY_ERR_ARG when a length in the section runs past n_in. Waits for
+ * the stream. Bit-identical to the reference. */
+int mray_hip_entropy_decode(const uint8_t *h_in, size_t n_in, const uint32_t *d_index, int W, int H, int m,
+                            int prediction_coded, const uint8_t *bits, float *d_data, mray_stream stream);
+
 /* ---- SPH particles (src/volume.cpp:55-94, src/particle_grid.{cpp,cuh}) --- */
 
 /* Samples particles (device arrays of n positions and values) at every voxel

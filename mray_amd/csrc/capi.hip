@@ -563,6 +563,72 @@ int mray_hip_dequantize(const uint8_t *d_stream, size_t stream_bytes, const uint
     return check_status(rc, "mray_hip_dequantize");
 }
 
+int mray_hip_quantize_dense(const float *d_data, size_t n, int m, int bits, uint8_t *d_out, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_quantize_dense: call mray_hip_init first");
+    if ((n && (!d_data || !d_out)) || m < 1 || m > 255 || bits < 1 || bits > 32)
+        return fail(MRAY_ERR_ARG, "mray_hip_quantize_dense: null argument, moment count %d or width %d", m, bits);
+    return check_status(mray::launch_quantize_dense(d_data, n, m, bits, d_out, (hipStream_t)stream),
+                        "mray_hip_quantize_dense");
+}
+
+int mray_hip_dequantize_dense(const uint8_t *d_in, size_t n, int m, int bits, float *d_data, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_dequantize_dense: call mray_hip_init first");
+    if ((n && (!d_in || !d_data)) || m < 1 || m > 255 || bits < 1 || bits > 32)
+        return fail(MRAY_ERR_ARG, "mray_hip_dequantize_dense: null argument, moment count %d or width %d", m, bits);
+    return check_status(mray::launch_dequantize_dense(d_in, n, m, bits, d_data, (hipStream_t)stream),
+                        "mray_hip_dequantize_dense");
+}
+
+int mray_hip_entropy_encode(const float *d_data, const uint32_t *d_index, int W, int H, int m, int prediction_coded,
+                            const uint8_t *bits, uint8_t *h_out, size_t capacity, size_t *out_bytes,
+                            mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_entropy_encode: call mray_hip_init first");
+    if (!d_index || !bits || !d_data)
+        return fail(MRAY_ERR_ARG, "mray_hip_entropy_encode: null argument");
+    if (bad_image(W, H, 1) || m < 1 || m > 255)
+        return fail(MRAY_ERR_ARG, "mray_hip_entropy_encode: bad image size %dx%d or moment count %d", W, H, m);
+    size_t n = 0;
+    int rc = mray::launch_entropy_encode(d_data, d_index, W, H, m, prediction_coded, bits, h_out, capacity, &n,
+                                         (hipStream_t)stream);
+    if (out_bytes)
+        *out_bytes = n;
+    if (rc == MRAY_ERR_ARG)
+        return fail(rc, "mray_hip_entropy_encode: widths must be 1..32 for moment 0 and 1..14 above, every pixel at "
+                        "most %d moments, and the section (%zu bytes) within the capacity (%zu)", m, n, capacity);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_entropy_encode: the device cannot hold the model of the widest alphabet in LDS");
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_entropy_encode: scratch allocation failed");
+    return check_status(rc, "mray_hip_entropy_encode");
+}
+
+int mray_hip_entropy_decode(const uint8_t *h_in, size_t n_in, const uint32_t *d_index, int W, int H, int m,
+                            int prediction_coded, const uint8_t *bits, float *d_data, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_entropy_decode: call mray_hip_init first");
+    if (!h_in || !d_index || !bits || !d_data)
+        return fail(MRAY_ERR_ARG, "mray_hip_entropy_decode: null argument");
+    if (bad_image(W, H, 1) || m < 1 || m > 255)
+        return fail(MRAY_ERR_ARG, "mray_hip_entropy_decode: bad image size %dx%d or moment count %d", W, H, m);
+    int rc = mray::launch_entropy_decode(h_in, n_in, d_index, W, H, m, prediction_coded, bits, d_data,
+                                         (hipStream_t)stream);
+    if (rc == MRAY_ERR_ARG)
+        return fail(rc, "mray_hip_entropy_decode: widths must be 1..32 for moment 0 and 1..14 above, and the "
+                        "section (%zu bytes) must hold every length its headers give", n_in);
+    if (rc == MRAY_ERR_UNSUPPORTED)
+        return fail(rc, "mray_hip_entropy_decode: the device cannot hold the model of the widest alphabet in LDS");
+    if (rc == MRAY_ERR_OOM)
+        return fail(rc, "mray_hip_entropy_decode: scratch allocation failed");
+    return check_status(rc, "mray_hip_entropy_decode");
+}
+
 int mray_hip_particles_to_grid(const float *d_x, const float *d_y, const float *d_z, const float *d_v, uint32_t n,
                                float h, const float dom_min[3], const float dom_max[3], int nx, int ny, int nz,
                                float *d_grid, mray_stream stream)

@@ -63,6 +63,12 @@ int launch_quantize(const float *data, const uint32_t *index, int W, int H, int 
                     hipStream_t stream);
 int launch_dequantize(const uint8_t *stream_in, size_t stream_bytes, const uint32_t *index, int W, int H,
                       int prediction_coded, const uint8_t *bits, int n_bits, float *data, hipStream_t stream);
+int launch_quantize_dense(const float *data, size_t n, int m, int bits, uint8_t *out, hipStream_t stream);
+int launch_dequantize_dense(const uint8_t *in, size_t n, int m, int bits, float *data, hipStream_t stream);
+int launch_entropy_encode(const float *data, const uint32_t *index, int W, int H, int m, int prediction_coded,
+                          const uint8_t *bits, uint8_t *out, size_t capacity, size_t *out_bytes, hipStream_t stream);
+int launch_entropy_decode(const uint8_t *in, size_t n_in, const uint32_t *index, int W, int H, int m,
+                          int prediction_coded, const uint8_t *bits, float *data, hipStream_t stream);
 
 int launch_particles_to_grid(const float *x, const float *y, const float *z, const float *v, uint32_t n, float h,
                              const float *dmin, const float *dmax, int nx, int ny, int nz, float *grid,

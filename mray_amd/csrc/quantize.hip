@@ -475,7 +475,57 @@ int scan_offsets(const uint32_t *index, size_t npix, const QuantTable &t, uint64
     return 0;
 }
 
+// dense images with one value per byte group (moment_quantization::quantize
+// / dequantize of a moment vector, src/moment_quantization.cpp:361-407):
+// value i = moment i % m of its pixel, (bits + 7) / 8 little-endian bytes
+__global__ __launch_bounds__(kThreads) void quant_dense_encode(const float *__restrict__ data, size_t n, int m,
+                                                               int bits, uint8_t *__restrict__ out)
+{
+    const int nb = (bits + 7) / 8;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    {
+        float lo, hi;
+        quant_range((uint32_t)(i % (size_t)m), 0, lo, hi);
+        const uint32_t q = quantize_value(data[i], bits, lo, hi);
+        for (int b = 0; b < nb; ++b)
+            out[i * nb + b] = (uint8_t)(q >> (8 * b));
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void quant_dense_decode(const uint8_t *__restrict__ in, size_t n, int m,
+                                                               int bits, float *__restrict__ data)
+{
+    const int nb = (bits + 7) / 8;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    {
+        uint32_t q = 0;
+        for (int b = 0; b < nb; ++b)
+            q |= (uint32_t)in[i * nb + b] << (8 * b);
+        float lo, hi;
+        quant_range((uint32_t)(i % (size_t)m), 0, lo, hi);
+        data[i] = dequantize_value(q, bits, lo, hi);
+    }
+}
+
 } // namespace
+
+int launch_quantize_dense(const float *data, size_t n, int m, int bits, uint8_t *out, hipStream_t stream)
+{
+    if (m < 1 || bits < 1 || bits > 32)
+        return MRAY_ERR_ARG;
+    if (n)
+        hipLaunchKernelGGL(quant_dense_encode, dim3(grid_for(n)), dim3(kThreads), 0, stream, data, n, m, bits, out);
+    return hipGetLastError() == hipSuccess ? 0 : MRAY_ERR_HIP;
+}
+
+int launch_dequantize_dense(const uint8_t *in, size_t n, int m, int bits, float *data, hipStream_t stream)
+{
+    if (m < 1 || bits < 1 || bits > 32)
+        return MRAY_ERR_ARG;
+    if (n)
+        hipLaunchKernelGGL(quant_dense_decode, dim3(grid_for(n)), dim3(kThreads), 0, stream, in, n, m, bits, data);
+    return hipGetLastError() == hipSuccess ? 0 : MRAY_ERR_HIP;
+}
 
 int launch_quantize(const float *data, const uint32_t *index, int W, int H, int prediction_coded,
                     const uint8_t *bits, int n_bits, uint8_t *stream_out, size_t capacity, size_t *stream_bytes,

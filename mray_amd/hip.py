@@ -55,6 +55,12 @@ _SIGNATURES = {
                                      C.POINTER(C.c_uint8), _vp], C.c_int),
     "mray_hip_dequantize": ([_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, _vp,
                              _vp], C.c_int),
+    "mray_hip_quantize_dense": ([_vp, C.c_size_t, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_dequantize_dense": ([_vp, C.c_size_t, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_entropy_encode": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), _vp,
+                                 C.c_size_t, C.POINTER(C.c_size_t), _vp], C.c_int),
+    "mray_hip_entropy_decode": ([_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8),
+                                 _vp, _vp], C.c_int),
     "mray_hip_particles_to_grid": ([_vp, _vp, _vp, _vp, C.c_uint32, C.c_float, _f3, _f3, C.c_int, C.c_int, C.c_int,
                                     _vp, _vp], C.c_int),
     "mray_hip_synth_mixing_layer": ([_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -397,6 +403,58 @@ def dequantize(stream_bytes: torch.Tensor, index: torch.Tensor, W: int, H: int, 
     _check(load_library().mray_hip_dequantize(_ptr(sb), stream_bytes.numel(), _ptr(index), W, H,
                                               int(bool(prediction_coded)), t, len(t), _ptr(out), _stream(stream)),
            "mray_hip_dequantize")
+    return out[:total]
+
+
+def quantize_dense(data: torch.Tensor, m: int, bits: int, stream=None) -> torch.Tensor:
+    """The byte-aligned quantized values of a dense image (mray_hip_quantize_dense):
+    (bits + 7) // 8 little-endian bytes per value."""
+    n = data.numel()
+    out = torch.empty(max(n * ((bits + 7) // 8), 1), dtype=torch.uint8, device=data.device)
+    _check(load_library().mray_hip_quantize_dense(_ptr(data), n, m, bits, _ptr(out), _stream(stream)),
+           "mray_hip_quantize_dense")
+    return out[:n * ((bits + 7) // 8)]
+
+
+def dequantize_dense(qs: torch.Tensor, n: int, m: int, bits: int, stream=None) -> torch.Tensor:
+    """The values byte-aligned quantized data holds (mray_hip_dequantize_dense)."""
+    out = torch.empty(max(n, 1), dtype=torch.float32, device=qs.device)
+    _check(load_library().mray_hip_dequantize_dense(_ptr(qs), n, m, bits, _ptr(out), _stream(stream)),
+           "mray_hip_dequantize_dense")
+    return out[:n]
+
+
+def entropy_encode(data: torch.Tensor, index: torch.Tensor, W: int, H: int, m: int, prediction_coded: bool, bits,
+                   stream=None):
+    """The entropy-coded moment section of a compact image as a host numpy
+    uint8 array (mray_hip_entropy_encode): moment-0 words, then one adaptive
+    arithmetic code per moment of bits[l] bits."""
+    t = _table(bits)
+    if len(t) < m:
+        raise ValueError("entropy_encode needs m widths")
+    total = int(index[-1].to(torch.int64).item()) & 0xFFFFFFFF
+    # at most 4 bytes per symbol, 2 per moment 0, and the headers
+    cap = 4 + 2 * W * H + 20 * m + 4 * total + 64
+    out = np.zeros(cap, np.uint8)
+    n = C.c_size_t(0)
+    _check(load_library().mray_hip_entropy_encode(_ptr(data), _ptr(index), W, H, m, int(bool(prediction_coded)), t,
+                                                  out.ctypes.data, cap, C.byref(n), _stream(stream)),
+           "mray_hip_entropy_encode")
+    return out[:n.value].copy()
+
+
+def entropy_decode(section, index: torch.Tensor, W: int, H: int, m: int, prediction_coded: bool, bits,
+                   stream=None) -> torch.Tensor:
+    """The compact image an entropy-coded section (host bytes) holds (mray_hip_entropy_decode)."""
+    t = _table(bits)
+    if len(t) < m:
+        raise ValueError("entropy_decode needs m widths")
+    sec = np.ascontiguousarray(section, np.uint8)
+    total = int(index[-1].to(torch.int64).item()) & 0xFFFFFFFF
+    out = torch.empty(max(total, 1), dtype=torch.float32, device=index.device)
+    _check(load_library().mray_hip_entropy_decode(sec.ctypes.data, sec.size, _ptr(index), W, H, m,
+                                                  int(bool(prediction_coded)), t, _ptr(out), _stream(stream)),
+           "mray_hip_entropy_decode")
     return out[:total]
 
 

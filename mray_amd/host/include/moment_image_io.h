@@ -1,13 +1,15 @@
-// Moment-image files (src/moment_image_io.cpp:255-530), the subset without
-// LZ4 compression or entropy coding: float moments, or for compact images the
-// quantized stream (quantization_bits < 32, with the quantization table after
-// the header when one is set); byte-compatible with the reference for that
-// subset. Header: uint16 width, height; uint8 moments,
-// coding warp, flags (1 compressed, 2 compact, 4 prediction coded,
-// 16 bounds, 32 error bounds, 128 coding parameters), quantisation byte;
-// float3 domain min, max. Then, compact: per-pixel counts as uint8, bounds,
-// error bounds, coding parameters, moments; dense: bounds, error bounds,
-// coding parameters, moments. Other variants throw invalid_args.
+// Moment-image files (src/moment_image_io.cpp:255-530), byte-compatible
+// with the reference in every variant: header (uint16 width, height; uint8
+// moments, coding warp, flags (1 compressed, 2 compact, 4 prediction coded,
+// 16 bounds, 32 error bounds, 128 coding parameters), quantization byte (bits
+// 0-5 quantization_bits, 64 entropy coded, 128 table present); float3
+// domain min, max), then the quantization table when set, then for compact
+// images the per-pixel counts as uint8, bounds, error bounds, coding
+// parameters and moments (floats, the packed quantized stream, or the
+// entropy-coded section), for dense images the same without the counts and
+// with the moments as floats or, compressed, LZ4 of byte-aligned quantized
+// values. With compression every section is an LZ4 block behind its u32
+// length (raw when LZ4 does not shrink it).
 #pragma once
 
 #include "moment_image.h"

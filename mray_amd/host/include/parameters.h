@@ -16,9 +16,9 @@ struct GenerationParameters
     float bias = 1e-4f;
     bool compact_image = true;
     bool prediction_coding = true;
-    bool compress = false;          // moment-image file options: LZ4 compression and entropy
-    bool entropy_coding = false;    // coding are rejected; quantization_bits < 32 writes the
-    Byte quantization_bits = 32;    // quantized stream of a compact image
+    bool compress = false;          // moment-image file options: LZ4 sections,
+    bool entropy_coding = false;    // arithmetic-coded moments (compact images),
+    Byte quantization_bits = 32;    // and the word width of quantized moments
     int coding_warp = 0;
     int quantization_curve_sampling = 64;
     bool transform_to_bounds = false;

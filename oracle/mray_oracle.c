@@ -2137,6 +2137,348 @@ void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_co
 }
 
 /* ------------------------------------------------------------------------ */
+/* entropy coding (src/moment_image_coding.cpp, src/entropy_coding/)        */
+/* ------------------------------------------------------------------------ */
+
+/* Amir Said's FastAC floating-point arithmetic coder (vendored by the
+ * reference in src/entropy_coding/arithmetic_codec.{h,cpp}) with its adaptive
+ * multi-symbol model, restated from the published algorithm:
+ *   - the interval is [base, base + length) in doubles; a symbol s narrows it
+ *     to [base + length d(s), base + length d(s + 1)), d(A) read as 1
+ *     (the last symbol uses base + length); 2^-47 of leakage is taken off the
+ *     length after every symbol;
+ *   - base >= 1 propagates a carry into the bytes already written;
+ *   - length <= 2^-16 shifts out two bytes (16 bits) of base at a time;
+ *   - the model: counts start at 1, d(k) = sum_{i<k} count_i / total with
+ *     total the sum of the counts, recomputed every `cycle` symbols; the
+ *     counts are halved (rounding up) when total would pass 2^17, and cycle
+ *     grows by 5/4 per update up to 8 (A + 6), starting from (A + 6) / 2.
+ * Arithmetic_Codec::encode / decode (Adaptive_Data_Model):
+ * src/entropy_coding/arithmetic_codec.cpp:445-519; renormalisation and carry
+ * :76-112; start / stop :575-675; the model :838-919. */
+#define AC_OUT 65536.0
+#define AC_MINLEN (1.0 / 65536.0)
+#define AC_LSB (1.0 / 16777216.0 / 16777216.0)
+#define AC_LEAK (2.0 * AC_LSB)
+#define AC_MAXCOUNT (1u << 17)
+
+typedef struct
+{
+    unsigned n;      /* alphabet size */
+    unsigned *count; /* symbol counts */
+    double *dist;    /* cumulative distribution d(k) */
+    unsigned total, cycle, until;
+} ac_model;
+
+static void ac_model_update(ac_model *M)
+{
+    M->total += M->cycle;
+    if (M->total > AC_MAXCOUNT)
+    {
+        M->total = 0;
+        for (unsigned k = 0; k < M->n; ++k)
+        {
+            M->count[k] = (M->count[k] + 1) >> 1;
+            M->total += M->count[k];
+        }
+    }
+    const double scale = 1.0 / (double)M->total;
+    unsigned sum = 0;
+    for (unsigned k = 0; k < M->n; ++k)
+    {
+        M->dist[k] = scale * (double)sum;
+        sum += M->count[k];
+    }
+    M->cycle = (5 * M->cycle) >> 2;
+    const unsigned cap = (M->n + 6) << 3;
+    if (M->cycle > cap)
+        M->cycle = cap;
+    M->until = M->cycle;
+}
+
+static int ac_model_init(ac_model *M, unsigned n)
+{
+    M->n = n;
+    M->count = (unsigned *)malloc(n * sizeof(unsigned));
+    M->dist = (double *)malloc(n * sizeof(double));
+    if (!M->count || !M->dist)
+        return -1;
+    for (unsigned k = 0; k < n; ++k)
+        M->count[k] = 1;
+    M->total = 0;
+    M->cycle = n;
+    ac_model_update(M);
+    M->cycle = M->until = (n + 6) >> 1;
+    return 0;
+}
+
+static void ac_model_free(ac_model *M)
+{
+    free(M->count);
+    free(M->dist);
+}
+
+typedef struct
+{
+    double base, length;
+    uint8_t *buf;
+    size_t pos, cap;
+    int overflow;
+} ac_encoder;
+
+static void ac_put(ac_encoder *E, unsigned b)
+{
+    if (E->pos < E->cap)
+        E->buf[E->pos] = (uint8_t)b;
+    else
+        E->overflow = 1;
+    ++E->pos;
+}
+
+static void ac_carry(ac_encoder *E)
+{
+    E->base -= 1.0;
+    size_t p = E->pos - 1;
+    while (p < E->cap && E->buf[p] == 0xFFu)
+        E->buf[p--] = 0;
+    if (p < E->cap)
+        ++E->buf[p];
+}
+
+static void ac_encode(ac_encoder *E, ac_model *M, unsigned s)
+{
+    const double y = (s == M->n - 1) ? E->base + E->length : E->base + E->length * M->dist[s + 1];
+    E->base += E->length * M->dist[s];
+    E->length = y - E->base;
+    if (E->base >= 1.0)
+        ac_carry(E);
+    if ((E->length -= AC_LEAK) <= AC_MINLEN)
+    {
+        do
+        {
+            E->base *= AC_OUT;
+            const unsigned a = (unsigned)E->base;
+            ac_put(E, a >> 8);
+            ac_put(E, a & 0xFFu);
+            E->base -= (double)a;
+        } while ((E->length *= AC_OUT) <= AC_MINLEN);
+    }
+    ++M->count[s];
+    if (--M->until == 0)
+        ac_model_update(M);
+}
+
+static void ac_finish(ac_encoder *E)
+{
+    const unsigned a = (unsigned)(AC_OUT * E->base), b = (unsigned)(AC_OUT * (E->base + E->length));
+    int last;
+    if (b - a < 2)
+    {
+        E->base += 0.5 * AC_MINLEN;
+        last = 3;
+    }
+    else if ((b >> 8) - (a >> 8) < 2)
+    {
+        E->base += AC_MINLEN;
+        last = 2;
+    }
+    else
+    {
+        E->base += 256.0 * AC_MINLEN;
+        last = 1;
+    }
+    if (E->base >= 1.0)
+        ac_carry(E);
+    do
+    {
+        E->base *= 256.0;
+        const unsigned c = (unsigned)E->base;
+        ac_put(E, c);
+        E->base -= (double)c;
+    } while (--last);
+}
+
+typedef struct
+{
+    double base, length, value;
+    const uint8_t *buf;
+    size_t pos, n;
+} ac_decoder;
+
+/* bytes past the end of the code read as zero */
+static double ac_byte(const ac_decoder *D, size_t i) { return (double)(i < D->n ? D->buf[i] : 0u); }
+
+static void ac_decoder_start(ac_decoder *D, const uint8_t *buf, size_t n)
+{
+    D->buf = buf;
+    D->n = n;
+    D->length = 1.0;
+    D->base = D->value = 0.0;
+    for (size_t k = 0; k < 6; ++k)
+        D->value = 256.0 * D->value + AC_LSB * ac_byte(D, k);
+    D->pos = 6;
+}
+
+static unsigned ac_decode(ac_decoder *D, ac_model *M)
+{
+    unsigned s = 0, n = M->n, mid = n >> 1;
+    double x = D->base, y = D->base + D->length;
+    const double sv = D->value + AC_LSB;
+    do
+    {
+        const double z = D->base + D->length * M->dist[mid];
+        if (z > sv)
+        {
+            y = z;
+            n = mid;
+        }
+        else
+        {
+            x = z;
+            s = mid;
+        }
+    } while ((mid = (s + n) >> 1) != s);
+    D->base = x;
+    D->length = y - x;
+    if (D->base >= 1.0)
+    {
+        D->base -= 1.0;
+        D->value -= 1.0;
+    }
+    if ((D->length -= AC_LEAK) <= AC_MINLEN)
+    {
+        do
+        {
+            D->base *= AC_OUT;
+            const unsigned a = (unsigned)D->base;
+            D->base -= (double)a;
+            D->value = (AC_OUT * D->value - (double)a) + AC_LSB * (256.0 * ac_byte(D, D->pos) + ac_byte(D, D->pos + 1));
+            D->pos += 2;
+        } while ((D->length *= AC_OUT) <= AC_MINLEN);
+    }
+    ++M->count[s];
+    if (--M->until == 0)
+        ac_model_update(M);
+    return s;
+}
+
+static void put_u32(uint8_t *p, uint32_t v) { memcpy(p, &v, 4); }
+static uint32_t get_u32(const uint8_t *p)
+{
+    uint32_t v;
+    memcpy(&v, p, 4);
+    return v;
+}
+
+/* entropy_encode (src/moment_image_coding.cpp:98-154, 202-217, write :73-96):
+ * u32 count of non-empty pixels, their moment 0 quantized to bits[0] bits as
+ * u16 words in pixel order, then for l = 1..m-1 a u32 byte count and the
+ * arithmetic code of moment l of every pixel with more than l moments
+ * (quantized to bits[l] bits, alphabet 2^bits[l]). Returns the section's
+ * length; nothing is written past cap. -1 when a width is out of range. */
+long orc_entropy_encode(int W, int H, int m, const float *data, const uint32_t *index, int prediction_coded,
+                        const uint8_t *bits, uint8_t *out, size_t cap)
+{
+    const size_t npix = (size_t)W * H;
+    for (int l = 1; l < m; ++l)
+        if (bits[l] < 1 || bits[l] > 14)
+            return -1;
+    size_t pos = 4, nz = 0;
+    for (size_t p = 0; p < npix; ++p)
+        if (index[p + 1] > index[p])
+        {
+            const uint16_t q = (uint16_t)orc_quantize_value(data[index[p]], bits[0], 0.0f, 1.0f);
+            if (pos + 2 <= cap)
+                memcpy(out + pos, &q, 2);
+            pos += 2;
+            ++nz;
+        }
+    if (cap >= 4)
+        put_u32(out, (uint32_t)nz);
+    for (int l = 1; l < m; ++l)
+    {
+        float lo, hi;
+        quant_range(l, prediction_coded, &lo, &hi);
+        ac_model M;
+        if (ac_model_init(&M, 1u << bits[l]))
+            return -1;
+        size_t nsym = 0;
+        for (size_t p = 0; p < npix; ++p)
+            nsym += (index[p + 1] - index[p]) > (uint32_t)l;
+        ac_encoder E = {0.0, 1.0, NULL, 0, 4 * nsym + 16, 0};
+        E.buf = (uint8_t *)malloc(E.cap);
+        if (!E.buf)
+            return -1;
+        for (size_t p = 0; p < npix; ++p)
+            if ((index[p + 1] - index[p]) > (uint32_t)l)
+                ac_encode(&E, &M, orc_quantize_value(data[index[p] + l], bits[l], lo, hi));
+        ac_finish(&E);
+        if (pos + 4 <= cap)
+            put_u32(out + pos, (uint32_t)E.pos);
+        pos += 4;
+        if (pos + E.pos <= cap)
+            memcpy(out + pos, E.buf, E.pos);
+        pos += E.pos;
+        free(E.buf);
+        ac_model_free(&M);
+        if (E.overflow)
+            return -1;
+    }
+    return (long)pos;
+}
+
+/* entropy_decode (src/moment_image_coding.cpp:156-200, 219-234, read :42-71)
+ * of a section written by orc_entropy_encode into the compact data of the
+ * image with the given index. Returns 0, or -1 when the section is shorter
+ * than its headers say or a width is out of range. */
+int orc_entropy_decode(int W, int H, int m, const uint8_t *in, size_t n_in, const uint32_t *index,
+                       int prediction_coded, const uint8_t *bits, float *data)
+{
+    const size_t npix = (size_t)W * H;
+    for (int l = 1; l < m; ++l)
+        if (bits[l] < 1 || bits[l] > 14)
+            return -1;
+    if (n_in < 4)
+        return -1;
+    const size_t nz = get_u32(in);
+    size_t pos = 4 + 2 * nz;
+    if (pos > n_in)
+        return -1;
+    size_t k = 0;
+    for (size_t p = 0; p < npix; ++p)
+        if (index[p + 1] > index[p])
+        {
+            uint16_t q = 0;
+            if (k < nz)
+                memcpy(&q, in + 4 + 2 * k, 2);
+            ++k;
+            data[index[p]] = orc_dequantize_value(q, bits[0], 0.0f, 1.0f);
+        }
+    for (int l = 1; l < m; ++l)
+    {
+        if (pos + 4 > n_in)
+            return -1;
+        const size_t size = get_u32(in + pos);
+        pos += 4;
+        if (pos + size > n_in)
+            return -1;
+        float lo, hi;
+        quant_range(l, prediction_coded, &lo, &hi);
+        ac_model M;
+        if (ac_model_init(&M, 1u << bits[l]))
+            return -1;
+        ac_decoder D;
+        ac_decoder_start(&D, in + pos, size);
+        for (size_t p = 0; p < npix; ++p)
+            if ((index[p + 1] - index[p]) > (uint32_t)l)
+                data[index[p] + l] = orc_dequantize_value(ac_decode(&D, &M), bits[l], lo, hi);
+        ac_model_free(&M);
+        pos += size;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* quantization curve (src/measure_coding_error.cpp)                        */
 /* ------------------------------------------------------------------------ */
 

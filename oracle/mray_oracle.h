@@ -179,6 +179,17 @@ size_t orc_quantize_image(int W, int H, const float *data, const uint32_t *index
 void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_coded, const uint8_t *bits,
                           const uint8_t *in, size_t n_in, float *data);
 
+/* ---- entropy coding (src/moment_image_coding.cpp, FastAC) ----
+ * The entropy-coded moment section of a compact image: u32 count of
+ * non-empty pixels, their moment 0 as u16 words (bits[0] bits), then per
+ * moment l >= 1 a u32 length and the adaptive arithmetic code of the moment
+ * l words (bits[l] in 1..14) of every pixel with more than l moments.
+ * encode returns the length (-1 on a bad width); decode returns 0 or -1. */
+long orc_entropy_encode(int W, int H, int m, const float *data, const uint32_t *index, int prediction_coded,
+                        const uint8_t *bits, uint8_t *out, size_t cap);
+int orc_entropy_decode(int W, int H, int m, const uint8_t *in, size_t n_in, const uint32_t *index,
+                       int prediction_coded, const uint8_t *bits, float *data);
+
 /* ---- quantization curve (src/measure_coding_error.cpp:41-192, 374-437) ----
  * Coding error of a prediction-coded compact image: every pixel decoded
  * (transform_dequantization_real, decode, exponentialToTrigonometricMoments)

@@ -17,6 +17,8 @@
 #include "coding_transform.h"
 #include "moment_quantization.h"
 #include "measure_coding_error.h"
+#include "moment_image_coding.h"
+#include "moment_image_io.h"
 #include "mese/MESE_4_10.h"
 
 #include <cstdio>
@@ -434,5 +436,116 @@ void ref_quantization_table(int W, int H, int m, const float *data, const uint32
     create_quantization_table(mi, start_bits);
     auto t = moment_quantization::get_prediction_coding_quantization_table();
     std::memcpy(table, t.data(), m);
+}
+
+// entropy_encode (src/moment_image_coding.cpp:202-217) of a compact image of
+// m moments with per-moment widths bits (m entries); returns the section's
+// length, the bytes in out when it fits cap
+long ref_entropy_encode(int W, int H, int m, const float *data, const uint32_t *index, int prediction_coded,
+                        const uint8_t *bits, uint8_t *out, long cap)
+{
+    auto mi = make_compact(W, H, data, index, prediction_coded);
+    mi.num_moments = m;
+    auto bytes = entropy_encode(mi, vector<Byte>(bits, bits + m));
+    if ((long)bytes.size() <= cap)
+        std::memcpy(out, bytes.data(), bytes.size());
+    return (long)bytes.size();
+}
+
+// entropy_decode (src/moment_image_coding.cpp:219-234) into compact data
+void ref_entropy_decode(int W, int H, int m, const uint8_t *in, long n_in, const uint32_t *index,
+                        int prediction_coded, const uint8_t *bits, float *data)
+{
+    vector<float> zeros(index[(size_t)W * H], 0.0f);
+    auto mi = make_compact(W, H, zeros.data(), index, prediction_coded);
+    mi.num_moments = m;
+    // the decoder reads up to two bytes past each stream
+    vector<Byte> bytes(in, in + n_in);
+    entropy_decode(mi, vector<Byte>(bits, bits + m), bytes);
+    std::memcpy(data, mi.data.data(), mi.data.size() * sizeof(float));
+}
+
+// write_moment_image (src/moment_image_io.cpp:515-530) of an image given by
+// its arrays: compact with index (W*H+1) or dense (W*H*m floats); bounds
+// (W*H packed lower | upper << 16), error_bounds (W*H u16) and
+// coding_params (2(m-1) u16) when non-null. table (m entries) sets the
+// prediction-coding quantization table first; the reference keeps it set
+// for every later write in the process.
+void ref_write_moment_image(const char *path, int W, int H, int m, int compact, int prediction_coded, int warp,
+                            const float *dmin, const float *dmax, const uint32_t *index, const float *data,
+                            long n_data, const uint32_t *bounds, const uint16_t *error_bounds,
+                            const uint16_t *coding_params, const uint8_t *table, int compress, int bits,
+                            int entropy)
+{
+    MomentImageHost mi(W, H, m, false, false);
+    mi.domain = Range<Vec3f>(Vec3f(dmin[0], dmin[1], dmin[2]), Vec3f(dmax[0], dmax[1], dmax[2]));
+    mi.coding_warp = warp;
+    mi.prediction_code = prediction_coded != 0;
+    mi.data.assign(data, data + n_data);
+    if (compact)
+    {
+        mi.is_compact = true;
+        mi.index.assign(index, index + (size_t)W * H + 1);
+    }
+    if (bounds)
+    {
+        mi.bounds.resize((size_t)W * H);
+        for (size_t i = 0; i < mi.bounds.size(); ++i)
+        {
+            mi.bounds[i].lower = (uint16_t)(bounds[i] & 0xFFFFu);
+            mi.bounds[i].upper = (uint16_t)(bounds[i] >> 16);
+        }
+    }
+    if (error_bounds)
+        mi.error_bounds.assign(error_bounds, error_bounds + (size_t)W * H);
+    if (coding_params)
+    {
+        mi.coding_params.resize(m - 1);
+        for (int i = 0; i < m - 1; ++i)
+        {
+            mi.coding_params[i].param1 = coding_params[2 * i];
+            mi.coding_params[i].param2 = coding_params[2 * i + 1];
+        }
+    }
+    if (table)
+    {
+        vector<Byte> t(255, 0);
+        std::memcpy(t.data(), table, m);
+        moment_quantization::set_prediction_coding_quantization_table(t);
+    }
+    write_moment_image(mi, path, compress != 0, (Byte)bits, entropy != 0);
+}
+
+// load_moment_image (src/moment_image_io.cpp:286-313): header fields
+// (W, H, m, warp, compact, prediction coded, has bounds, has error bounds,
+// has coding params, data length), then the arrays when the pointers are
+// non-null (index W*H+1, data, bounds packed, coding params 2(m-1))
+void ref_load_moment_image(const char *path, long *hdr, uint32_t *index, float *data, uint32_t *bounds,
+                           uint16_t *coding_params)
+{
+    auto mi = load_moment_image(path);
+    hdr[0] = mi.width;
+    hdr[1] = mi.height;
+    hdr[2] = mi.num_moments;
+    hdr[3] = mi.coding_warp;
+    hdr[4] = mi.is_compact;
+    hdr[5] = mi.prediction_code;
+    hdr[6] = mi.has_bounds();
+    hdr[7] = mi.has_error_bounds();
+    hdr[8] = mi.has_coding_params();
+    hdr[9] = (long)mi.data.size();
+    if (index && !mi.index.empty())
+        std::memcpy(index, mi.index.data(), mi.index.size() * sizeof(uint32_t));
+    if (data)
+        std::memcpy(data, mi.data.data(), mi.data.size() * sizeof(float));
+    if (bounds)
+        for (size_t i = 0; i < mi.bounds.size(); ++i)
+            bounds[i] = (uint32_t)mi.bounds[i].lower | ((uint32_t)mi.bounds[i].upper << 16);
+    if (coding_params)
+        for (size_t i = 0; i < mi.coding_params.size(); ++i)
+        {
+            coding_params[2 * i] = mi.coding_params[i].param1;
+            coding_params[2 * i + 1] = mi.coding_params[i].param2;
+        }
 }
 }

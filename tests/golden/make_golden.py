@@ -6,6 +6,9 @@ Run in the build container (needs oracle/_ref/libmrayref.so, built by
     python tests/golden/make_golden.py            # every fixture
     python tests/golden/make_golden.py reference  # golden_reference.npz only
 
+The moment-image files are written in a child process of this script: the
+reference keeps its quantization table as process state.
+
 Inputs are (a) the vectors of the reference's own unit tests
 (tests/test_mese.cpp, tests/test_prediction_coding.cpp,
 tests/test_moment_compaction.cpp) and (b) small synthetic workloads from
@@ -353,12 +356,157 @@ def curve_vectors(R: Reference) -> dict:
     return out
 
 
+def _entropy_image(rng, W, H, m, pc, empty=0.2):
+    counts = rng.integers(1, m + 1, W * H).astype(np.uint32)
+    counts[rng.random(W * H) < empty] = 0
+    index = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    n = int(index[-1])
+    if pc:
+        data = rng.beta(2.0, 2.0, n).astype(np.float32)
+    else:
+        data = rng.normal(0.0, 0.08, n).astype(np.float32)
+        data[index[:-1][counts > 0]] = rng.random(int(np.count_nonzero(counts)))
+    return index, data
+
+
+def halving_data(W, H, levels, level):
+    """The 2-moment image of the halving case: moment 0 = (p % 97) / 97,
+    moment 1 = levels[level[p]]."""
+    data = np.empty(2 * W * H, np.float32)
+    data[0::2] = (np.arange(W * H) % 97) / np.float32(97.0)
+    data[1::2] = levels[level]
+    return data
+
+
+def entropy_vectors(R: Reference) -> dict:
+    """entropy_encode / entropy_decode (the FastAC adaptive arithmetic coder
+    per moment) of compact images: the reference's section bytes and the
+    values it decodes from them."""
+    rng = np.random.default_rng(77)
+    out = {}
+    table12 = np.array([16, 11, 10, 10, 9, 8, 8, 7, 6, 6, 5, 4], np.uint8)
+    cases = {"plain8": (16, 12, 10, 0, np.full(10, 8, np.uint8)),
+             "pc_table": (23, 17, 12, 1, table12),
+             "w14": (20, 20, 6, 0, np.full(6, 14, np.uint8)),
+             "w1": (30, 7, 5, 0, np.full(5, 1, np.uint8)),
+             "pc_w32_zeroth": (9, 9, 4, 1, np.array([32, 8, 8, 8], np.uint8))}
+    for name, (W, H, m, pc, bits) in cases.items():
+        index, data = _entropy_image(rng, W, H, m, pc)
+        sec = R.entropy_encode(W, H, m, data, index, pc, bits)
+        out.update({f"{name}_shape": np.array([W, H, m, pc], np.int32), f"{name}_index": index,
+                    f"{name}_data": data, f"{name}_bits": bits, f"{name}_section": sec,
+                    f"{name}_decoded": R.entropy_decode(W, H, m, sec, index, pc, bits)})
+    # a stream long enough for the model's count halving (total > 2^17):
+    # 380 x 380 pixels of 2 moments (the index is 2 p, not stored), moment 1
+    # drawn from 24 levels with a skewed distribution (a compact fixture)
+    W = H = 380
+    index = (np.arange(W * H + 1) * 2).astype(np.uint32)
+    levels = np.linspace(-0.3, 0.3, 24).astype(np.float32)
+    p = np.exp(-0.5 * ((np.arange(24) - 11.5) / 2.0) ** 2)
+    level = rng.choice(24, W * H, p=p / p.sum()).astype(np.uint8)
+    data = halving_data(W, H, levels, level)
+    bits = np.array([16, 7], np.uint8)
+    sec = R.entropy_encode(W, H, 2, data, index, 0, bits)
+    dec = R.entropy_decode(W, H, 2, sec, index, 0, bits)
+    import hashlib
+
+    # the section (mostly the 16-bit moment-0 words) is kept as its SHA-256;
+    # the decoded moment 1 as the set of its values per level
+    out.update({"halving_shape": np.array([W, H, 2, 0], np.int32), "halving_levels": levels,
+                "halving_level": level, "halving_bits": bits, "halving_section_len": np.int64(len(sec)),
+                "halving_section_sha256": np.frombuffer(hashlib.sha256(sec.tobytes()).digest(), np.uint8),
+                "halving_stream1_tail": sec[-64:],
+                "halving_decoded_levels": np.array([dec[1::2][level == k][0] if np.any(level == k) else np.nan
+                                                    for k in range(24)], np.float32)})
+    return out
+
+
+# the moment-image files, in writing order: the quantization table is set
+# before the "tab" cases and, as in the reference, stays set after them
+FILE_CASES = [("c_f32", "A", 0, 32, 0), ("c_q10", "A", 0, 10, 0), ("c_lz4_f32", "A", 1, 32, 0),
+              ("c_lz4_q10", "A", 1, 10, 0), ("c_ent8", "A", 0, 8, 1), ("c_lz4_ent10", "A", 1, 10, 1),
+              ("c_eb_lz4", "E", 1, 32, 0), ("d_f32", "D", 0, 32, 0), ("d_q10_raw", "D", 0, 10, 0),
+              ("d_lz4_f32", "D", 1, 32, 0), ("d_lz4_q12", "D", 1, 12, 0), ("d_lz4_q16", "D", 1, 16, 0),
+              ("pc_q10", "P", 0, 10, 0), ("pc_lz4_q10", "P", 1, 10, 0), ("pc_tab_q10", "P", 0, 10, 0),
+              ("pc_tab_lz4_ent10", "P", 1, 10, 1), ("c_tab_lz4_ent10", "A", 1, 10, 1)]
+# the reference's reader cannot read compressed quantized dense images: it
+# hands the quantization byte in front of the LZ4 block to LZ4
+FILE_UNREADABLE = ("d_lz4_q12", "d_lz4_q16")
+
+
+def _file_images(R: Reference) -> dict:
+    vol, dmin, dmax = synth.marschner_lobb(32), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    cam = synth.ortho_camera()
+    imgs = {}
+    for key, (W, H, m, compact, pc) in {"A": (24, 20, 20, 1, 0), "D": (16, 12, 12, 0, 0),
+                                        "P": (24, 20, 20, 1, 1)}.items():
+        gp = GenParams(step_size=1.0 / 32, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=compact,
+                       prediction_coding=pc, error_threshold=2e-3, bias=1e-4)
+        data, bounds, index, cp = R.generate(vol, dmin, dmax, cam, W, H, m, gp)
+        imgs[key] = dict(W=W, H=H, m=m, dmin=dmin, dmax=dmax, prediction_coded=pc, warp=0,
+                         data=np.ascontiguousarray(data, np.float32), bounds=bounds,
+                         index=index if compact else None, coding_params=cp if pc else None)
+    e = dict(imgs["A"])
+    e["error_bounds"] = np.random.default_rng(5).integers(0, 65536, e["W"] * e["H"]).astype(np.uint16)
+    imgs["E"] = e
+    return imgs
+
+
+def _files_worker(outfile):
+    """Writes every file case in a fresh process (the reference's table is
+    process state) and saves inputs, file bytes and what the reference reads."""
+    import tempfile
+
+    from oracle_ffi import ptr, _f, _u8, _u16, _u32
+
+    R = Reference()
+    imgs = _file_images(R)
+    out = {}
+    for key, img in imgs.items():
+        for k in ("data", "bounds", "index", "coding_params", "error_bounds"):
+            if img.get(k) is not None:
+                out[f"img{key}_{k}"] = img[k]
+        out[f"img{key}_shape"] = np.array([img["W"], img["H"], img["m"], img["prediction_coded"]], np.int32)
+    P = imgs["P"]
+    table = None
+    with tempfile.TemporaryDirectory() as td:
+        for name, key, compress, bits, entropy in FILE_CASES:
+            if name.startswith("pc_tab") and table is None:
+                # the quantization curve of image P at 10 bits sets the table
+                table = np.zeros(P["m"], np.uint8)
+                R.lib.ref_quantization_table(P["W"], P["H"], P["m"], ptr(P["data"], _f), ptr(P["index"], _u32), 0,
+                                             ptr(P["coding_params"], _u16), 10, ptr(table, _u8))
+                out["table"] = table
+            path = pathlib.Path(td) / f"{name}.bin"
+            R.write_moment_image(path, imgs[key], compress, bits, entropy)
+            out[f"{name}_file"] = np.frombuffer(path.read_bytes(), np.uint8).copy()
+            if name not in FILE_UNREADABLE:
+                out[f"{name}_read"] = R.load_moment_image(path, 0)["data"]
+    np.savez_compressed(outfile, **out)
+
+
+def file_vectors(R: Reference) -> dict:
+    import subprocess
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as td:
+        f = pathlib.Path(td) / "files.npz"
+        subprocess.run([sys.executable, __file__, "_files_worker", str(f)], check=True)
+        with np.load(f) as z:
+            out = {k: z[k] for k in z.files}
+    out["order"] = np.array([c[0] for c in FILE_CASES])
+    return out
+
+
 def main():
     # python make_golden.py [name ...]: only the named fixtures (default all)
     R = Reference()
+    if sys.argv[1:2] == ["_files_worker"]:
+        _files_worker(sys.argv[2])
+        return
     makers = {"mese": mese_vectors, "coding": coding_vectors, "images": image_vectors, "sph": sph_vectors,
               "reference": reference_dvr_vectors, "quant": quant_vectors,
-              "curve": curve_vectors}
+              "curve": curve_vectors, "entropy": entropy_vectors, "files": file_vectors}
     for name in (sys.argv[1:] or list(makers)):
         np.savez_compressed(OUT / f"golden_{name}.npz", **makers[name](R))
     for f in sorted(OUT.glob("golden_*.npz")):

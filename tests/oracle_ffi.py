@@ -66,6 +66,13 @@ def quant_bits(m, prediction_coded, bits=10, table=None):
     return t
 
 
+def entropy_capacity(index, m):
+    """An upper bound of an entropy-coded section's length: the layout's
+    headers, 2 bytes per moment 0 and 4 bytes (+16 per stream) per symbol."""
+    npix = len(index) - 1
+    return 4 + 2 * npix + (m - 1) * 20 + 4 * int(index[-1]) + 64
+
+
 def stream_bytes(index, bits):
     """floor(total bits / 8): the length of the reference's packed stream."""
     counts = np.diff(index.astype(np.int64))
@@ -282,6 +289,26 @@ class Oracle:
                                       C.c_size_t(len(st)), ptr(out, _f))
         return out[:int(index[-1])]
 
+    def entropy_encode(self, W, H, m, data, index, prediction_coded, bits):
+        """The entropy-coded moment section (bytes) of a compact image."""
+        cap = entropy_capacity(index, m)
+        out = np.zeros(cap, np.uint8)
+        self.lib.orc_entropy_encode.restype = C.c_long
+        n = self.lib.orc_entropy_encode(W, H, m, ptr(np.ascontiguousarray(data, np.float32), _f), ptr(index, _u32),
+                                        prediction_coded, ptr(np.ascontiguousarray(bits, np.uint8), _u8),
+                                        ptr(out, _u8), C.c_size_t(cap))
+        assert 0 <= n <= cap, n
+        return out[:n]
+
+    def entropy_decode(self, W, H, m, section, index, prediction_coded, bits):
+        out = np.zeros(max(int(index[-1]), 1), np.float32)
+        sec = np.ascontiguousarray(section, np.uint8)
+        rc = self.lib.orc_entropy_decode(W, H, m, ptr(sec, _u8), C.c_size_t(len(sec)), ptr(index, _u32),
+                                         prediction_coded, ptr(np.ascontiguousarray(bits, np.uint8), _u8),
+                                         ptr(out, _f))
+        assert rc == 0
+        return out[:int(index[-1])]
+
     def particles_to_grid(self, x, y, z, v, h, dmin, dmax, nx, ny, nz):
         g = np.zeros((nz, ny, nx), np.float32)
         bmin = (C.c_float * 3)(*dmin)
@@ -350,6 +377,52 @@ class Reference:
         self.lib.ref_dequantize(W, H, ptr(index, _u32), prediction_coded, ptr(bits, _u8), ptr(st, _u8),
                                 C.c_long(len(st)), ptr(out, _f))
         return out[:int(index[-1])]
+
+    def entropy_encode(self, W, H, m, data, index, prediction_coded, bits):
+        cap = entropy_capacity(index, m)
+        out = np.zeros(cap, np.uint8)
+        self.lib.ref_entropy_encode.restype = C.c_long
+        n = self.lib.ref_entropy_encode(W, H, m, ptr(np.ascontiguousarray(data, np.float32), _f), ptr(index, _u32),
+                                        prediction_coded, ptr(np.ascontiguousarray(bits, np.uint8), _u8),
+                                        ptr(out, _u8), C.c_long(cap))
+        assert 0 <= n <= cap, n
+        return out[:n]
+
+    def entropy_decode(self, W, H, m, section, index, prediction_coded, bits):
+        out = np.zeros(max(int(index[-1]), 1), np.float32)
+        # the reference's decoder reads two bytes past a stream; give it zeros
+        sec = np.concatenate([np.ascontiguousarray(section, np.uint8), np.zeros(16, np.uint8)])
+        self.lib.ref_entropy_decode(W, H, m, ptr(sec, _u8), C.c_long(len(section)), ptr(index, _u32),
+                                    prediction_coded, ptr(np.ascontiguousarray(bits, np.uint8), _u8), ptr(out, _f))
+        return out[:int(index[-1])]
+
+    def write_moment_image(self, path, img, compress, bits, entropy, table=None):
+        """write_moment_image of a MomentFile-like image (see moment_file())."""
+        bmin = (C.c_float * 3)(*img["dmin"])
+        bmax = (C.c_float * 3)(*img["dmax"])
+        data = np.ascontiguousarray(img["data"], np.float32)
+        opt = lambda k, t: None if img.get(k) is None else np.ascontiguousarray(img[k], t)
+        index, bounds, eb, cp = opt("index", np.uint32), opt("bounds", np.uint32), opt("error_bounds", np.uint16), \
+            opt("coding_params", np.uint16)
+        tb = None if table is None else np.ascontiguousarray(table, np.uint8)
+        self.lib.ref_write_moment_image(str(path).encode(), img["W"], img["H"], img["m"], int(index is not None),
+                                        int(img.get("prediction_coded", 0)), int(img.get("warp", 0)), bmin, bmax,
+                                        ptr(index, _u32), ptr(data, _f), C.c_long(len(data)), ptr(bounds, _u32),
+                                        ptr(eb, _u16), ptr(cp, _u16), ptr(tb, _u8), int(compress), int(bits),
+                                        int(entropy))
+
+    def load_moment_image(self, path, n_data_max):
+        hdr = np.zeros(10, np.int64)
+        self.lib.ref_load_moment_image(str(path).encode(), ptr(hdr, C.POINTER(C.c_long)), None, None, None, None)
+        W, H, m = int(hdr[0]), int(hdr[1]), int(hdr[2])
+        index = np.zeros(W * H + 1, np.uint32) if hdr[4] else None
+        data = np.zeros(max(int(hdr[9]), 1), np.float32)
+        bounds = np.zeros(W * H, np.uint32) if hdr[6] else None
+        cp = np.zeros(max(2 * (m - 1), 1), np.uint16) if hdr[8] else None
+        self.lib.ref_load_moment_image(str(path).encode(), ptr(hdr, C.POINTER(C.c_long)), ptr(index, _u32),
+                                       ptr(data, _f), ptr(bounds, _u32), ptr(cp, _u16))
+        return dict(W=W, H=H, m=m, warp=int(hdr[3]), compact=bool(hdr[4]), prediction_coded=bool(hdr[5]),
+                    index=index, data=data[:int(hdr[9])], bounds=bounds, coding_params=cp)
 
     def reference(self, vol, dmin, dmax, cam, W, H, tf_rgba8, tf_step, p: RefParams, want_rgba8=False):
         c = Camera.from_dict(cam)

@@ -57,3 +57,15 @@ def test_parameter_structs_match_header(cname, cls):
         base = ct._type_ if hasattr(ct, "_length_") else ct
         expect = {"float": C.c_float, "int32_t": C.c_int32}[t]
         assert base is expect
+
+
+def test_host_library_exports_every_declared_function():
+    from mray_amd import moment_file
+
+    if not moment_file.LIB_PATH.exists():
+        pytest.skip("libmray_host.so not built (make -C mray_amd)")
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "mray_host.h").read_text(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(mray_host_[a-z0-9_]+)\s*\(", text)))
+    assert len(names) >= 4
+    lib = C.CDLL(str(moment_file.LIB_PATH))
+    assert not [n for n in names if not hasattr(lib, n)]

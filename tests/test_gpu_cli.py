@@ -208,6 +208,22 @@ def test_cli_quantized_file(tmp_path):
     print(f"[cli] quantized: bit-identical rays {same.sum()}/{W * H}", flush=True)
     assert same.all()
 
+    # the reference eval's file options (eval/marschner_lobb/marschner_lobb.json:
+    # compress=1, entropy_coding=1, quantization_bits=10): LZ4 sections and
+    # arithmetic-coded moments hold the same quantized values
+    run("--generate", tmp_path / "ent.bin", tmp_path / "vol.raw",
+        common + ",quantization_bits=10,compress=1,entropy_coding=1")
+    raw = (tmp_path / "ent.bin").read_bytes()
+    assert raw[6] & 1 and raw[7] == (10 | 0x40 | 0x80)
+    print(f"[cli] eval file options: {len(raw)} bytes (packed quantized file "
+          f"{(tmp_path / 'q10.bin').stat().st_size})", flush=True)
+    run("--reconstruct", tmp_path / "ent.bin", tmp_path / "out_ent.ppm",
+        f"tf={tmp_path / 'tf.bin'},camera={tmp_path / 'cam.txt'},step_size={step!r},bias=1e-4,"
+        f"early_ray_termination=0.99,medium_emission=1.0,preintegration_step_size_factor={factor},form=plain,"
+        f"float_rgba={tmp_path / 'rgba_ent.f32'}")
+    rgba_ent = np.fromfile(tmp_path / "rgba_ent.f32", np.float32).reshape(H, W, 4)
+    assert np.array_equal(rgba_ent.view(np.uint32), ref.view(np.uint32))
+
 
 @pytest.mark.parametrize("case", ["ortho_f32", "persp_u8"])
 def test_cli_reference(tmp_path, case):
