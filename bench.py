@@ -277,6 +277,25 @@ def main():
         curve = hip.curve_table(d_sd, d_si, sw, sw, m, cparams, 10)
         quant["curve_search"] = {"workload": f"{sw}x{sw} sample of the coded frame, {int(s_index[-1])} moments",
                                  "ms": round((time.perf_counter() - t0) * 1e3, 2), "table": curve}
+        # the arithmetic coder of the reference eval's file options (compact,
+        # prediction-coded, 10-bit curve table, entropy_coding=1): the section
+        # from device moments to host bytes and back, per call
+        sec = hip.entropy_encode(cdata, cindex, W, H, m, True, curve)
+        times = {}
+        for name, fn in (("encode", lambda: hip.entropy_encode(cdata, cindex, W, H, m, True, curve)),
+                         ("decode", lambda: hip.entropy_decode(sec, cindex, W, H, m, True, curve))):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(max(1, args.steps // 2)):
+                fn()
+            torch.cuda.synchronize()
+            times[name] = (time.perf_counter() - t0) * 1e3 / max(1, args.steps // 2)
+        nmom = int(hi[-1])
+        quant["entropy_coding"] = {"workload": f"the compacted, prediction-coded frame ({nmom} moments), "
+                                               f"curve table, one stream per moment",
+                                   "section_bytes": int(sec.size), "bits_per_moment": round(8 * sec.size / nmom, 3),
+                                   "ms": {k: round(v, 2) for k, v in times.items()},
+                                   "Msymbols_per_s": {k: round(nmom / (v * 1e-3) / 1e6, 1) for k, v in times.items()}}
         del cdata, cindex
 
     # images of both forms at the bench's MESE bias and at the reference

@@ -315,6 +315,45 @@ int mray_hip_quantization_table(const float *d_data, const uint32_t *d_index, in
 int mray_hip_dequantize(const uint8_t *d_stream, size_t stream_bytes, const uint32_t *d_index, int W, int H,
                         int prediction_coded, const uint8_t *bits, int n_bits, float *d_data, mray_stream stream);
 
+/* ---- error bounds and uncertainty (src/raymarch_reconstruct.h:203-367,
+ *      src/error_measures.h, src/moment_raymarcher.cu:693-950) ---- */
+
+/* Per-pixel error bounds of a moment image against its volume
+ * (reconstruct_error, src/moment_raymarcher.h:73-76, as --error-bounds runs it,
+ * src/main.cpp:333-367): along each pixel's ray, from t_near in steps of
+ * p->step_size while t <= t_far, the reconstruction from the pixel's
+ * multipliers (d_lagrange as mray_hip_prepare with MRAY_FORM_PLAIN writes
+ * them; d_index for compact images, d_bounds when the image has density
+ * bounds) at the phase of t against map_density of the volume sample at t;
+ * the error is the maximum, the RMSE or the P^2 estimate of the
+ * p->error_percentile percentile of |difference| (p->error_type). Written as
+ * the uint16 e * 65535, truncated (MomentImageHost::set_error_bound). The
+ * reference's FP32 arithmetic: bit-identical to its CPU path. A pixel without
+ * moments reconstructs with an empty series (the reference reads an empty
+ * array there); a ray of fewer than 3 samples has percentile 0 (the
+ * reference reads uninitialised markers). */
+int mray_hip_error_bounds(mray_volume vol, const mray_camera *cam, const float *d_lagrange, const uint32_t *d_index,
+                          const uint32_t *d_bounds, int W, int H, int m, const mray_error_params *p,
+                          uint16_t *d_error_bounds, mray_stream stream);
+
+/* Uncertainty renderings (reconstruct_uncertainty_interpolation /
+ * reconstruct_uncertainty_convolution, src/moment_raymarcher.h:78-86) into
+ * float RGBA, from PLAIN multipliers and the W*H uint16 error bounds e:
+ * mode 0 ray-marches (1 - s) lower + s upper, s = bound_interpolation,
+ * lower / upper the reconstruction -/+ e clamped to the pixel's density
+ * bounds, with the preintegrated 256-entry table d_tf of mray_hip_reconstruct;
+ * mode 1 classifies one sample per step with the 256 x 256 RGBA table d_tf
+ * (row-major, the box filter of read_from_disk_uncertainty for
+ * MRAY_ERROR_MAX and MRAY_ERROR_PERCENTILE, the gaussian filter for
+ * MRAY_ERROR_RMSE) at (lower, upper) for MRAY_ERROR_MAX and at (mean, e)
+ * otherwise. The reference's FP32 arithmetic: bit-identical to its CPU path.
+ * Single scattering is not supported. */
+int mray_hip_reconstruct_uncertainty(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                                     const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds,
+                                     const uint16_t *d_error_bounds, int W, int H, int m, const float *d_tf,
+                                     const mray_rec_params *p, int mode, int error_type, float bound_interpolation,
+                                     float *d_rgba, mray_stream stream);
+
 /* The quantized values of a dense image as the LZ4 path of
  * write_static_image stores them (moment_quantization::quantize(moments,
  * num_moments, bits), src/moment_quantization.cpp:361-385, called at

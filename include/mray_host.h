@@ -40,6 +40,13 @@ int mray_host_write_moment_image(const char *path, int W, int H, int m, int pred
 int mray_host_load_moment_image(const char *path, int64_t hdr[10], uint32_t *index, float *data, uint32_t *bounds,
                                 uint16_t *error_bounds, uint16_t *coding_params);
 
+/* The uncertainty transfer function of a 256 x RGBA8 transfer function
+ * (read_from_disk_uncertainty, src/transfer_function.cpp:233-248): the box
+ * filter for MRAY_ERROR_MAX and MRAY_ERROR_PERCENTILE, the gaussian filter for
+ * MRAY_ERROR_RMSE, at step_size; table receives 256 x 256 x 4 floats, row y
+ * (upper bound or deviation) by column x (lower bound or mean). */
+int mray_host_uncertainty_tf(const uint8_t *rgba8, float step_size, int error_type, float *table);
+
 /* The prediction-coding quantization table: returns 1 and copies its 255
  * entries into table when one is set, else 0. */
 int mray_host_quantization_table(uint8_t table[255]);

@@ -110,6 +110,27 @@ typedef struct mray_ref_params
     int32_t single_scattering;   /* 0 (ss.enable); nonzero is not supported */
 } mray_ref_params;
 
+/* ErrorType (src/common.h:152-157) */
+enum mray_error_type
+{
+    MRAY_ERROR_MAX = 0,
+    MRAY_ERROR_RMSE = 1,
+    MRAY_ERROR_PERCENTILE = 2
+};
+
+/* ErrorReconstructionParameters (src/parameters.h:124-137) without the file
+ * options (compress, entropy_coding, quantization_bits), which the host
+ * library takes where it writes the file. Same names and defaults. */
+typedef struct mray_error_params
+{
+    int32_t error_type;       /* 2 (MRAY_ERROR_PERCENTILE) */
+    int32_t error_percentile; /* 90 */
+    float bias;               /* 1e-6 */
+    float data_min;           /* 0.0 */
+    float data_max;           /* 1.0 */
+    float step_size;          /* 0.01 */
+} mray_error_params;
+
 /* DensityBound (src/common.h:135-150) is stored as one uint32 per pixel:
  * lower | (upper << 16). */
 #define MRAY_BOUND_LO(b) ((uint16_t)((b)&0xFFFFu))

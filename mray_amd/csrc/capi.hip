@@ -289,6 +289,51 @@ int mray_hip_reconstruct(const mray_camera *cam, const float dom_min[3], const f
     return check_status(rc, "mray_hip_reconstruct");
 }
 
+int mray_hip_error_bounds(mray_volume vol, const mray_camera *cam, const float *d_lagrange, const uint32_t *d_index,
+                          const uint32_t *d_bounds, int W, int H, int m, const mray_error_params *p,
+                          uint16_t *d_error_bounds, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_error_bounds: call mray_hip_init first");
+    if (!vol || !cam || !d_lagrange || !p || !d_error_bounds)
+        return fail(MRAY_ERR_ARG, "mray_hip_error_bounds: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_error_bounds: bad image size %dx%d or moment count %d", W, H, m);
+    if (!(p->step_size > 0.0f) || !(p->data_max != p->data_min))
+        return fail(MRAY_ERR_ARG, "mray_hip_error_bounds: step_size must be > 0 and data_max != data_min");
+    if (p->error_type < MRAY_ERROR_MAX || p->error_type > MRAY_ERROR_PERCENTILE)
+        return fail(MRAY_ERR_ARG, "mray_hip_error_bounds: unknown error_type %d", p->error_type);
+    int rc = mray::launch_error_bounds(vol->data, vol->dtype, vol->nx, vol->ny, vol->nz, vol->dmin, vol->dmax, *cam,
+                                       d_lagrange, d_index, d_bounds, W, H, m, *p, d_error_bounds,
+                                       (hipStream_t)stream);
+    return check_status(rc, "mray_hip_error_bounds");
+}
+
+int mray_hip_reconstruct_uncertainty(const mray_camera *cam, const float dom_min[3], const float dom_max[3],
+                                     const float *d_lagrange, const uint32_t *d_index, const uint32_t *d_bounds,
+                                     const uint16_t *d_error_bounds, int W, int H, int m, const float *d_tf,
+                                     const mray_rec_params *p, int mode, int error_type, float bound_interpolation,
+                                     float *d_rgba, mray_stream stream)
+{
+    if (!g_init)
+        return fail(MRAY_ERR_NOT_INIT, "mray_hip_reconstruct_uncertainty: call mray_hip_init first");
+    if (!cam || !dom_min || !dom_max || !d_lagrange || !d_error_bounds || !d_tf || !p || !d_rgba)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_uncertainty: null argument");
+    if (bad_image(W, H, m))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_uncertainty: bad image size %dx%d or moment count %d", W, H,
+                    m);
+    if (mode != 0 && mode != 1)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_uncertainty: mode must be 0 (interpolation) or 1 "
+                                  "(convolution)");
+    if (error_type < MRAY_ERROR_MAX || error_type > MRAY_ERROR_PERCENTILE)
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_uncertainty: unknown error_type %d", error_type);
+    if (!(p->step_size > 0.0f))
+        return fail(MRAY_ERR_ARG, "mray_hip_reconstruct_uncertainty: step_size must be > 0");
+    int rc = mray::launch_uncertainty(*cam, dom_min, dom_max, d_lagrange, d_index, d_bounds, d_error_bounds, W, H, m,
+                                      d_tf, *p, mode, error_type, bound_interpolation, d_rgba, (hipStream_t)stream);
+    return check_status(rc, "mray_hip_reconstruct_uncertainty");
+}
+
 int mray_hip_prepare_f64(const float *d_moments, const uint32_t *d_index, int W, int H, int m, float bias,
                          double *d_lagrange, mray_stream stream)
 {

@@ -63,6 +63,13 @@ int launch_quantize(const float *data, const uint32_t *index, int W, int H, int 
                     hipStream_t stream);
 int launch_dequantize(const uint8_t *stream_in, size_t stream_bytes, const uint32_t *index, int W, int H,
                       int prediction_coded, const uint8_t *bits, int n_bits, float *data, hipStream_t stream);
+int launch_error_bounds(const void *vox, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                        const mray_camera &cam, const float *lagrange, const uint32_t *index, const uint32_t *bounds,
+                        int W, int H, int m, const mray_error_params &p, uint16_t *out, hipStream_t stream);
+int launch_uncertainty(const mray_camera &cam, const float *dmin, const float *dmax, const float *lagrange,
+                       const uint32_t *index, const uint32_t *bounds, const uint16_t *error_bounds, int W, int H, int m,
+                       const float *tf, const mray_rec_params &p, int mode, int error_type, float bound_interpolation,
+                       float *rgba, hipStream_t stream);
 int launch_quantize_dense(const float *data, size_t n, int m, int bits, uint8_t *out, hipStream_t stream);
 int launch_dequantize_dense(const uint8_t *in, size_t n, int m, int bits, float *data, hipStream_t stream);
 int launch_entropy_encode(const float *data, const uint32_t *index, int W, int H, int m, int prediction_coded,

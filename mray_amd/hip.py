@@ -12,7 +12,7 @@ import pathlib
 import numpy as np
 import torch
 
-from .types import Camera, GenParams, RecParams, RefParams, MRAY_DTYPE_F32, MRAY_DTYPE_U8, MRAY_FORM_MIXED, MRAY_FORM_PLAIN
+from .types import Camera, ErrParams, GenParams, RecParams, RefParams, MRAY_DTYPE_F32, MRAY_DTYPE_U8, MRAY_FORM_MIXED, MRAY_FORM_PLAIN
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmray_hip.so"
 
@@ -30,6 +30,9 @@ _SIGNATURES = {
     "mray_hip_generate": ([_vp, C.POINTER(Camera), C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, C.c_int, _vp,
                            _vp, _vp, _vp, _vp], C.c_int),
     "mray_hip_prepare": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp], C.c_int),
+    "mray_hip_error_bounds": ([_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
+    "mray_hip_reconstruct_uncertainty": ([_vp, _f3, _f3, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                          C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_prepare_f64": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp], C.c_int),
     "mray_hip_reconstruct_f64": ([C.POINTER(Camera), _f3, _f3, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                   C.POINTER(RecParams), _vp, _vp, _vp], C.c_int),
@@ -269,6 +272,41 @@ def reconstruct(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m
                                                _ptr(bounds), W, H, m, _ptr(tf), C.byref(rp), form_id(form),
                                                _ptr(rgba), _ptr(steps), _stream(stream)), "mray_hip_reconstruct")
     return rgba, steps
+
+
+def error_bounds(vol: "Volume", cam: dict, lagrange: torch.Tensor, W: int, H: int, m: int, ep: ErrParams,
+                 index=None, bounds=None, out=None, stream=None) -> torch.Tensor:
+    """Per-pixel uint16 error bounds (as an int16 tensor of W*H) of the
+    reconstruction from PLAIN multipliers against the volume (mray_hip_error_bounds)."""
+    if out is None:
+        out = torch.empty(W * H, dtype=torch.int16, device=lagrange.device)
+    c = Camera.from_dict(cam)
+    _check(load_library().mray_hip_error_bounds(vol.handle, C.byref(c), _ptr(lagrange), _ptr(index), _ptr(bounds), W,
+                                                H, m, C.byref(ep), _ptr(out), _stream(stream)),
+           "mray_hip_error_bounds")
+    return out
+
+
+def reconstruct_uncertainty(cam: dict, dmin, dmax, lagrange: torch.Tensor, error_bounds: torch.Tensor, W: int, H: int,
+                            m: int, tf: torch.Tensor, rp: RecParams, mode: int, error_type: int = 1,
+                            bound_interpolation: float = 0.5, index=None, bounds=None, rgba=None,
+                            stream=None) -> torch.Tensor:
+    """Float RGBA (H, W, 4) of the uncertainty renderings
+    (mray_hip_reconstruct_uncertainty): mode 0 interpolation with the
+    preintegrated table (256, 4), mode 1 convolution with a 2D table (256, 256, 4)."""
+    dev = lagrange.device
+    if rgba is None:
+        rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    tf = tf.to(device=dev, dtype=torch.float32).contiguous()
+    if tf.numel() != (1024 if mode == 0 else 256 * 256 * 4):
+        raise MrayError("mode 0 takes the 256-entry table, mode 1 the 256 x 256 table")
+    c = Camera.from_dict(cam)
+    _check(load_library().mray_hip_reconstruct_uncertainty(C.byref(c), _f3(*dmin), _f3(*dmax), _ptr(lagrange),
+                                                           _ptr(index), _ptr(bounds), _ptr(error_bounds), W, H, m,
+                                                           _ptr(tf), C.byref(rp), mode, error_type,
+                                                           bound_interpolation, _ptr(rgba), _stream(stream)),
+           "mray_hip_reconstruct_uncertainty")
+    return rgba
 
 
 def reconstruct_backward(cam: dict, dmin, dmax, lagrange: torch.Tensor, W: int, H: int, m: int, tf: torch.Tensor,

@@ -37,3 +37,26 @@ void generate_reference(cut::Camera cam, const TransferFunction &tf, const Regul
 // Float-RGBA overload (un-clamped), which the reference lacks.
 void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf, vector<Vec4f> &rgba,
                  const ReconstructionParameters &params);
+
+// reconstruct_error (src/moment_raymarcher.h:73-76): the error bound of every
+// pixel's reconstruction against the volume (maximum, RMSE or a percentile
+// of the difference along its ray) into mi.error_bounds (which it creates).
+// Runs in the reference's FP32 arithmetic (MRAY_FORM_PLAIN multipliers):
+// the bounds are those of the reference.
+void reconstruct_error(cut::Camera cam, const RegularGrid &volume, MomentImageHost &mi,
+                       const ErrorReconstructionParameters &params);
+void reconstruct_error(cut::Camera cam, const Particles &volume, MomentImageHost &mi,
+                       const ErrorReconstructionParameters &params);
+
+// The uncertainty renderings (src/moment_raymarcher.h:78-86) of an image with
+// error bounds, in the reference's FP32 arithmetic: interpolation between
+// the bounds with the preintegrated tf (params.bound_interpolation), and
+// convolution with an uncertainty tf of read_from_disk_uncertainty
+// (params.error_type). Single scattering is not supported.
+void reconstruct_uncertainty_interpolation(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf,
+                                           ColorImageHost &cimg, const UncertaintyReconstructionParameters &params);
+void reconstruct_uncertainty_convolution(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf,
+                                         ColorImageHost &cimg, const UncertaintyReconstructionParameters &params);
+// float-RGBA overloads (un-clamped), mode 0 interpolation, 1 convolution
+void reconstruct_uncertainty(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf,
+                             vector<Vec4f> &rgba, const UncertaintyReconstructionParameters &params, int mode);

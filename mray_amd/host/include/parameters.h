@@ -55,6 +55,31 @@ struct ReferenceParameters
     float data_max = 1.0f;
 };
 
+// ErrorReconstructionParameters (src/parameters.h:124-158); error_type:
+// 0 maximum, 1 RMSE, 2 percentile (ErrorType, src/common.h:152-157)
+struct ErrorReconstructionParameters
+{
+    int error_type = 2;
+    int error_percentile = 90;
+    float bias = 1e-6f;
+    float data_min = 0.0f;
+    float data_max = 1.0f;
+    float step_size = 0.01f;
+    bool compress = false;
+    bool entropy_coding = false;
+    Byte quantization_bits = 32;
+};
+
+// UncertaintyReconstructionParameters (src/parameters.h:319-337)
+struct UncertaintyReconstructionParameters
+{
+    ReconstructionParameters rec;
+    float bound_interpolation = 0.5f;
+    int error_type = 1;
+};
+
 GenerationParameters parse_gen_params(std::unordered_map<string, string> &params);
+ErrorReconstructionParameters parse_error_params(std::unordered_map<string, string> &params);
+UncertaintyReconstructionParameters parse_uncertainty_rec_params(std::unordered_map<string, string> &params);
 ReferenceParameters parse_ref_params(std::unordered_map<string, string> &params);
 ReconstructionParameters parse_rec_params(std::unordered_map<string, string> &params);

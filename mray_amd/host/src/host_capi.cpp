@@ -5,6 +5,7 @@
 
 #include "../../../include/mray_host.h"
 #include "../include/moment_image_io.h"
+#include "../include/transfer_function.h"
 
 namespace
 {
@@ -100,6 +101,23 @@ int mray_host_load_moment_image(const char *path, int64_t hdr[10], uint32_t *ind
             std::memcpy(error_bounds, mi.error_bounds.data(), mi.error_bounds.size() * sizeof(uint16_t));
         if (coding_params && mi.has_coding_params())
             std::memcpy(coding_params, mi.coding_params.data(), mi.coding_params.size() * 2 * sizeof(uint16_t));
+    });
+}
+
+int mray_host_uncertainty_tf(const uint8_t *rgba8, float step_size, int error_type, float *table)
+{
+    return guarded("mray_host_uncertainty_tf", [&] {
+        if (!rgba8 || !table)
+            throw invalid_args("null argument");
+        const auto tf1d = tf_from_rgba8(rgba8);
+        TransferFunction tf;
+        if (error_type == MRAY_ERROR_MAX || error_type == MRAY_ERROR_PERCENTILE)
+            tf = box_filter(tf1d, step_size);
+        else if (error_type == MRAY_ERROR_RMSE)
+            tf = gaussian_filter(tf1d, step_size);
+        else
+            throw invalid_args("error_type must be 0 (max), 1 (RMSE) or 2 (percentile)");
+        std::memcpy(table, tf.table2d.data(), tf.table2d.size() * sizeof(Vec4f));
     });
 }
 

@@ -4,6 +4,7 @@
 #include <fstream>
 #include <iostream>
 
+#include "../../../include/mray_types.h"
 #include "../include/cut/camera.h"
 #include "../include/parameters.h"
 #include "../include/transfer_function.h"
@@ -81,6 +82,110 @@ TransferFunction read_from_disk(const char *filepath, float step_size)
 
 namespace
 {
+// sample1D (src/transfer_function.cpp:32-44)
+Vec4f sample1d(const std::array<Vec4f, TransferFunction::RESOLUTION> &tf1d, float x)
+{
+    const float ip = x * static_cast<float>(TransferFunction::RESOLUTION);
+    const float a = ip - std::floor(ip);
+    const int xl = (int)std::floor(ip);
+    const Vec4f &l = tf1d[std::min(std::max(xl, 0), TransferFunction::RESOLUTION - 1)];
+    const Vec4f &r = tf1d[std::min(std::max(xl + 1, 0), TransferFunction::RESOLUTION - 1)];
+    return Vec4f((1.f - a) * l.x + a * r.x, (1.f - a) * l.y + a * r.y, (1.f - a) * l.z + a * r.z,
+                 (1.f - a) * l.w + a * r.w);
+}
+
+// normal_cdf (src/cut/gaussian.h:33-59): Abramowitz and Stegun 7.1.26 with
+// the reference's float / double promotions and the float exp
+float normal_cdf(float x)
+{
+    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f, a5 = 1.061405429f,
+                p = 0.3275911f;
+    const int sign = (x < 0) ? -1 : 1;
+    x = (float)((double)std::fabs(x) / 1.41421356237);
+    const float t = (float)(1.0 / (1.0 + (double)(p * x)));
+    const float y = (float)(1.0 - (double)((((((a5 * t + a4) * t) + a3) * t + a2) * t + a1) * t * std::exp(-x * x)));
+    return (float)(0.5 * (1.0 + (double)((float)sign * y)));
+}
+} // namespace
+
+// box_filter (src/transfer_function.cpp:152-185)
+TransferFunction box_filter(const std::array<Vec4f, TransferFunction::RESOLUTION> &tf1d, float step_size)
+{
+    const int R = TransferFunction::RESOLUTION;
+    TransferFunction tf;
+    tf.table2d.assign((size_t)R * R, Vec4f());
+    for (int y = 0; y < R; ++y)
+        for (int x = 0; x < R; ++x)
+        {
+            const float sf = x / float(R), sb = y / float(R);
+            const int num_steps = std::abs(y - x) + 2;
+            const float dw = 1.0f / float(num_steps);
+            Vec4f rgba(0.0f);
+            for (int i = 0; i < num_steps; ++i)
+            {
+                const float w = float(i) / float(num_steps - 1);
+                const Vec4f t = sample1d(tf1d, w * sb + (1 - w) * sf);
+                const float a = dw * t.w * step_size;
+                rgba.x += t.x * a;
+                rgba.y += t.y * a;
+                rgba.z += t.z * a;
+                rgba.w += a;
+            }
+            tf.table2d[(size_t)y * R + x] = rgba;
+        }
+    return tf;
+}
+
+// gaussian_filter (src/transfer_function.cpp:187-231)
+TransferFunction gaussian_filter(const std::array<Vec4f, TransferFunction::RESOLUTION> &tf1d, float step_size)
+{
+    const int R = TransferFunction::RESOLUTION;
+    TransferFunction tf;
+    tf.table2d.assign((size_t)R * R, Vec4f());
+    for (int y = 0; y < R; ++y)
+        for (int x = 0; x < R; ++x)
+        {
+            const float mean = x / float(R), sd = y / float(R);
+            Vec4f rgba(0.0f);
+            if (sd != 0.0f)
+            {
+                const float delta = 1.0f / static_cast<float>(R);
+                for (int i = 0; i < R; ++i)
+                {
+                    const float xi = i * delta;
+                    const float weight = normal_cdf((xi + delta - mean) / sd) - normal_cdf((xi - mean) / sd);
+                    const float a = weight * tf1d[i].w * step_size;
+                    rgba.x += tf1d[i].x * a;
+                    rgba.y += tf1d[i].y * a;
+                    rgba.z += tf1d[i].z * a;
+                    rgba.w += a;
+                }
+            }
+            else
+            {
+                rgba = tf1d[x];
+                rgba.w *= step_size;
+                rgba.x = rgba.x * rgba.w;
+                rgba.y = rgba.y * rgba.w;
+                rgba.z = rgba.z * rgba.w;
+            }
+            tf.table2d[(size_t)y * R + x] = rgba;
+        }
+    return tf;
+}
+
+TransferFunction read_from_disk_uncertainty(const char *filepath, float step_size, int error_type)
+{
+    const auto tf1d = read_tf(filepath);
+    if (error_type == MRAY_ERROR_MAX || error_type == MRAY_ERROR_PERCENTILE)
+        return box_filter(tf1d, step_size);
+    if (error_type == MRAY_ERROR_RMSE)
+        return gaussian_filter(tf1d, step_size);
+    throw invalid_args("error_type must be 0 (max), 1 (RMSE) or 2 (percentile)");
+}
+
+namespace
+{
 template <class T> void read_f(std::unordered_map<string, string> &p, const char *k, T &v)
 {
     auto it = p.find(k);
@@ -132,6 +237,29 @@ ReferenceParameters parse_ref_params(std::unordered_map<string, string> &params)
     read_f(params, "early_ray_termination", p.early_ray_termination);
     read_f(params, "data_min", p.data_min);
     read_f(params, "data_max", p.data_max);
+    return p;
+}
+
+ErrorReconstructionParameters parse_error_params(std::unordered_map<string, string> &params)
+{
+    // parse_error_params (src/parameters.h:139-158): the file options are not
+    // parsed there, so --error-bounds writes uncompressed float files
+    ErrorReconstructionParameters p;
+    read_i(params, "error_type", p.error_type);
+    read_i(params, "error_percentile", p.error_percentile);
+    read_f(params, "bias", p.bias);
+    read_f(params, "data_min", p.data_min);
+    read_f(params, "data_max", p.data_max);
+    read_f(params, "step_size", p.step_size);
+    return p;
+}
+
+UncertaintyReconstructionParameters parse_uncertainty_rec_params(std::unordered_map<string, string> &params)
+{
+    UncertaintyReconstructionParameters p;
+    p.rec = parse_rec_params(params);
+    read_f(params, "bound_interpolation", p.bound_interpolation);
+    read_i(params, "error_type", p.error_type);
     return p;
 }
 

@@ -35,6 +35,46 @@ void require_size(const MomentImageHost &img)
         throw invalid_args("moment image needs width, height > 0 and 1..255 moments");
 }
 
+// a device copy of the grid (freed with the handle)
+void upload_volume(const RegularGrid &volume, VolumeHandle &vh)
+{
+    if (volume.data.size() * sizeof(float) < volume.num_voxels() * volume.get_byte_size())
+        throw invalid_data("volume data smaller than its size");
+    const float dmin[3] = {volume.domain.min.x, volume.domain.min.y, volume.domain.min.z};
+    const float dmax[3] = {volume.domain.max.x, volume.domain.max.y, volume.domain.max.z};
+    check(mray_hip_volume_create(volume.voxels(),
+                                 volume.type == RegularGrid::DataType::U8 ? MRAY_DTYPE_U8 : MRAY_DTYPE_F32,
+                                 volume.size.x, volume.size.y, volume.size.z, dmin, dmax, &vh.v),
+          "volume_create");
+}
+
+// the particle set sorted into its cell grid on the device: every ray sample
+// is Particles::sample
+void upload_particles(const Particles &volume, VolumeHandle &vh)
+{
+    const uint32_t n = volume.size();
+    if (n == 0 || volume.y.size() != n || volume.z.size() != n || volume.values.size() != n)
+        throw invalid_data("particle arrays must be non-empty and of equal length");
+    const Vec3f lo = volume.domain.min, hi = volume.domain.max;
+    DeviceBuffer dx(volume.x.data(), n * 4), dy(volume.y.data(), n * 4), dz(volume.z.data(), n * 4),
+        dv(volume.values.data(), n * 4);
+    const float dmin[3] = {lo.x, lo.y, lo.z};
+    const float dmax[3] = {hi.x, hi.y, hi.z};
+    check(mray_hip_volume_from_particles(dx.as<float>(), dy.as<float>(), dz.as<float>(), dv.as<float>(), n,
+                                         volume.smoothing_length, dmin, dmax, &vh.v),
+          "volume_from_particles");
+}
+
+// clamped and truncated RGBA8 (ColorImageHost::set_color)
+void to_rgba8(const vector<Vec4f> &rgba, int W, int H, ColorImageHost &cimg)
+{
+    if (cimg.width != W || cimg.height != H)
+        throw invalid_args("colour image and moment image differ in size");
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            cimg.set_color(x, y, rgba[(size_t)y * W + x]);
+}
+
 // the generate kernel on a device-resident volume, then the compaction and
 // prediction coding that generate_host applies
 void generate_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &img, const GenerationParameters &params)
@@ -162,15 +202,8 @@ void generate(cut::Camera cam, const RegularGrid &volume, MomentImageHost &img, 
 {
     mray_host::ensure_init();
     img.domain = volume.domain;
-    if (volume.data.size() * sizeof(float) < volume.num_voxels() * volume.get_byte_size())
-        throw invalid_data("volume data smaller than its size");
-    const float dmin[3] = {volume.domain.min.x, volume.domain.min.y, volume.domain.min.z};
-    const float dmax[3] = {volume.domain.max.x, volume.domain.max.y, volume.domain.max.z};
     VolumeHandle vh;
-    check(mray_hip_volume_create(volume.voxels(),
-                                 volume.type == RegularGrid::DataType::U8 ? MRAY_DTYPE_U8 : MRAY_DTYPE_F32,
-                                 volume.size.x, volume.size.y, volume.size.z, dmin, dmax, &vh.v),
-          "volume_create");
+    upload_volume(volume, vh);
     generate_on(vh.v, cam, img, params);
 }
 
@@ -178,19 +211,8 @@ void generate(cut::Camera cam, const Particles &volume, MomentImageHost &img, co
 {
     mray_host::ensure_init();
     img.domain = volume.domain;
-    const uint32_t n = volume.size();
-    if (n == 0 || volume.y.size() != n || volume.z.size() != n || volume.values.size() != n)
-        throw invalid_data("particle arrays must be non-empty and of equal length");
-    const Vec3f lo = volume.domain.min, hi = volume.domain.max;
-    DeviceBuffer dx(volume.x.data(), n * 4), dy(volume.y.data(), n * 4), dz(volume.z.data(), n * 4),
-        dv(volume.values.data(), n * 4);
-    const float dmin[3] = {lo.x, lo.y, lo.z};
-    const float dmax[3] = {hi.x, hi.y, hi.z};
-    // the particle set itself: every ray sample is Particles::sample
     VolumeHandle vh;
-    check(mray_hip_volume_from_particles(dx.as<float>(), dy.as<float>(), dz.as<float>(), dv.as<float>(), n,
-                                         volume.smoothing_length, dmin, dmax, &vh.v),
-          "volume_from_particles");
+    upload_particles(volume, vh);
     generate_on(vh.v, cam, img, params);
 }
 
@@ -207,26 +229,17 @@ void reconstruct(cut::Camera cam, const MomentImageHost &mi, const TransferFunct
         throw invalid_args("colour image and moment image differ in size");
     vector<Vec4f> rgba;
     reconstruct_float(cam, mi, tf, rgba, params);
-    for (int y = 0; y < mi.height; ++y)
-        for (int x = 0; x < mi.width; ++x)
-            cimg.set_color(x, y, rgba[(size_t)y * mi.width + x]);
+    to_rgba8(rgba, mi.width, mi.height, cimg);
 }
 
 void generate_reference(cut::Camera cam, const TransferFunction &tf, const RegularGrid &volume, int W, int H,
                         vector<Vec4f> &rgba, const ReferenceParameters &params)
 {
     mray_host::ensure_init();
-    if (volume.data.size() * sizeof(float) < volume.num_voxels() * volume.get_byte_size())
-        throw invalid_data("volume data smaller than its size");
     if (W <= 0 || H <= 0)
         throw invalid_args("image size must be positive");
-    const float dmin[3] = {volume.domain.min.x, volume.domain.min.y, volume.domain.min.z};
-    const float dmax[3] = {volume.domain.max.x, volume.domain.max.y, volume.domain.max.z};
     VolumeHandle vh;
-    check(mray_hip_volume_create(volume.voxels(),
-                                 volume.type == RegularGrid::DataType::U8 ? MRAY_DTYPE_U8 : MRAY_DTYPE_F32,
-                                 volume.size.x, volume.size.y, volume.size.z, dmin, dmax, &vh.v),
-          "volume_create");
+    upload_volume(volume, vh);
     const size_t npix = (size_t)W * H;
     DeviceBuffer d_tf(tf.data.data(), sizeof(tf.data));
     DeviceBuffer d_rgba(npix * sizeof(Vec4f));
@@ -246,4 +259,111 @@ void generate_reference(cut::Camera cam, const TransferFunction &tf, const Regul
     for (int y = 0; y < img.height; ++y)
         for (int x = 0; x < img.width; ++x)
             img.set_color(x, y, rgba[(size_t)y * img.width + x]);
+}
+
+namespace
+{
+void error_bounds_on(mray_volume vol, const cut::Camera &cam, MomentImageHost &mi,
+                     const ErrorReconstructionParameters &params)
+{
+    mray_host::ensure_init();
+    require_size(mi);
+    const int W = mi.width, H = mi.height, m = mi.num_moments;
+    const size_t npix = (size_t)W * H;
+    if (mi.is_compact && mi.index.size() != npix + 1)
+        throw invalid_data("compact moment image without index");
+    std::unique_ptr<DeviceBuffer> d_data_p(mray_host::upload_nonempty(mi.data));
+    DeviceBuffer d_index(mi.is_compact ? mi.index.data() : nullptr, mi.is_compact ? (npix + 1) * 4 : 0);
+    DeviceBuffer d_bounds(mi.has_bounds() ? mi.bounds.data() : nullptr, mi.has_bounds() ? npix * 4 : 0);
+    DeviceBuffer d_lag(std::max<size_t>(mi.data.size(), 1) * sizeof(float));
+    prepare_on_device(mi, *d_data_p, d_index, params.bias, false, MRAY_FORM_PLAIN, d_lag);
+    DeviceBuffer d_err(npix * sizeof(uint16_t));
+    mray_camera c = mray_host::to_c(cam);
+    mray_error_params e{};
+    e.error_type = params.error_type;
+    e.error_percentile = params.error_percentile;
+    e.bias = params.bias;
+    e.data_min = params.data_min;
+    e.data_max = params.data_max;
+    e.step_size = params.step_size;
+    check(mray_hip_error_bounds(vol, &c, d_lag.as<float>(), mi.is_compact ? d_index.as<uint32_t>() : nullptr,
+                                mi.has_bounds() ? d_bounds.as<uint32_t>() : nullptr, W, H, m, &e,
+                                d_err.as<uint16_t>(), nullptr),
+          "error_bounds");
+    mi.error_bounds.resize(npix);
+    d_err.download(mi.error_bounds.data(), npix * sizeof(uint16_t));
+}
+} // namespace
+
+void reconstruct_error(cut::Camera cam, const RegularGrid &volume, MomentImageHost &mi,
+                       const ErrorReconstructionParameters &params)
+{
+    mray_host::ensure_init();
+    VolumeHandle h;
+    upload_volume(volume, h);
+    error_bounds_on(h.v, cam, mi, params);
+}
+
+void reconstruct_error(cut::Camera cam, const Particles &volume, MomentImageHost &mi,
+                       const ErrorReconstructionParameters &params)
+{
+    mray_host::ensure_init();
+    VolumeHandle h;
+    upload_particles(volume, h);
+    error_bounds_on(h.v, cam, mi, params);
+}
+
+void reconstruct_uncertainty(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf,
+                             vector<Vec4f> &rgba, const UncertaintyReconstructionParameters &params, int mode)
+{
+    mray_host::ensure_init();
+    require_size(mi);
+    if (mode == 1 && tf.table2d.size() != (size_t)TransferFunction::RESOLUTION * TransferFunction::RESOLUTION)
+        throw invalid_args("the uncertainty convolution needs an uncertainty transfer function "
+                           "(read_from_disk_uncertainty)");
+    if (!mi.has_error_bounds())
+        throw invalid_args("the uncertainty renderings need a moment image with error bounds");
+    const int W = mi.width, H = mi.height, m = mi.num_moments;
+    const size_t npix = (size_t)W * H;
+    if (mi.is_compact && mi.index.size() != npix + 1)
+        throw invalid_data("compact moment image without index");
+    std::unique_ptr<DeviceBuffer> d_data_p(mray_host::upload_nonempty(mi.data));
+    DeviceBuffer d_index(mi.is_compact ? mi.index.data() : nullptr, mi.is_compact ? (npix + 1) * 4 : 0);
+    DeviceBuffer d_bounds(mi.has_bounds() ? mi.bounds.data() : nullptr, mi.has_bounds() ? npix * 4 : 0);
+    DeviceBuffer d_err(mi.error_bounds.data(), npix * sizeof(uint16_t));
+    DeviceBuffer d_lag(std::max<size_t>(mi.data.size(), 1) * sizeof(float));
+    prepare_on_device(mi, *d_data_p, d_index, params.rec.bias, params.rec.use_truncated_fourier, MRAY_FORM_PLAIN,
+                      d_lag);
+    DeviceBuffer d_tf(mode == 0 ? (const void *)tf.data.data() : (const void *)tf.table2d.data(),
+                      mode == 0 ? sizeof(tf.data) : tf.table2d.size() * sizeof(Vec4f));
+    DeviceBuffer d_rgba(npix * sizeof(Vec4f));
+    mray_camera c = mray_host::to_c(cam);
+    mray_rec_params r = mray_host::to_c(params.rec);
+    const float dmin[3] = {mi.domain.min.x, mi.domain.min.y, mi.domain.min.z};
+    const float dmax[3] = {mi.domain.max.x, mi.domain.max.y, mi.domain.max.z};
+    check(mray_hip_reconstruct_uncertainty(&c, dmin, dmax, d_lag.as<float>(),
+                                           mi.is_compact ? d_index.as<uint32_t>() : nullptr,
+                                           mi.has_bounds() ? d_bounds.as<uint32_t>() : nullptr,
+                                           d_err.as<uint16_t>(), W, H, m, d_tf.as<float>(), &r, mode,
+                                           params.error_type, params.bound_interpolation, d_rgba.as<float>(),
+                                           nullptr),
+          "reconstruct_uncertainty");
+    rgba.resize(npix);
+    d_rgba.download(rgba.data(), npix * sizeof(Vec4f));
+}
+
+void reconstruct_uncertainty_interpolation(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf,
+                                           ColorImageHost &cimg, const UncertaintyReconstructionParameters &params)
+{
+    vector<Vec4f> rgba;
+    reconstruct_uncertainty(cam, mi, tf, rgba, params, 0);
+    to_rgba8(rgba, mi.width, mi.height, cimg);
+}
+
+void reconstruct_uncertainty_convolution(cut::Camera cam, const MomentImageHost &mi, const TransferFunction &tf,
+                                         ColorImageHost &cimg, const UncertaintyReconstructionParameters &params)
+{
+    vector<Vec4f> rgba;
+    reconstruct_uncertainty(cam, mi, tf, rgba, params, 1);
+    to_rgba8(rgba, mi.width, mi.height, cimg);
 }

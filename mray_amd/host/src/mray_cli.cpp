@@ -15,6 +15,18 @@
 //       background_r/g/b, ReferenceParameters keys, dims / dtype / domain as
 //       for --generate; float_rgba=<file> as for --reconstruct
 //       (src/main.cpp:134-168, 740-752)
+//   mray --error-bounds <moment_image.bin> <volume.raw> "k=v,..."
+//       camera, ErrorReconstructionParameters keys (error_type,
+//       error_percentile, bias, data_min, data_max, step_size), dims / dtype
+//       / domain as for --generate; adds the per-pixel error bounds to the
+//       image and rewrites it with float moments (src/main.cpp:333-367)
+//   mray --uncertainty-interpolate <moment_image.bin> <output.ppm> "k=v,..."
+//   mray --uncertainty-convolution <moment_image.bin> <output.ppm> "k=v,..."
+//       as --reconstruct, plus bound_interpolation / error_type
+//       (src/main.cpp:475-519); the convolution classifies with
+//       read_from_disk_uncertainty's table of the tf at step_size *
+//       preintegration_step_size_factor; both in the reference's FP32
+//       arithmetic
 //   generate and reconstruct: form=mixed (default) | plain
 #include <cstdio>
 #include <fstream>
@@ -160,6 +172,62 @@ void run_reconstruct(const string &mi_file, const string &output, unordered_map<
     }
 }
 
+Vec3f background(unordered_map<string, string> &p)
+{
+    Vec3f bg(1.f, 1.f, 1.f);
+    if (p.count("background_r"))
+        bg.x = std::stof(p["background_r"]);
+    if (p.count("background_g"))
+        bg.y = std::stof(p["background_g"]);
+    if (p.count("background_b"))
+        bg.z = std::stof(p["background_b"]);
+    return bg;
+}
+
+void write_outputs(const string &output, const vector<Vec4f> &rgba, int W, int H, unordered_map<string, string> &p)
+{
+    ColorImageHost img(W, H);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            img.set_color(x, y, rgba[(size_t)y * W + x]);
+    write_PPM(output, img, background(p));
+    if (p.count("float_rgba"))
+    {
+        std::ofstream f(p["float_rgba"], std::ios::binary);
+        f.write(reinterpret_cast<const char *>(rgba.data()), rgba.size() * sizeof(Vec4f));
+    }
+}
+
+// compute_errors (src/main.cpp:333-367): raw volumes only
+void run_error_bounds(const string &mi_file, const string &dataset, unordered_map<string, string> p)
+{
+    auto error_params = parse_error_params(p);
+    auto mi = load_moment_image(mi_file);
+    auto cam = cut::load_from_file(need(p, "camera"), mi.width, mi.height);
+    auto volume = load_raw(dataset, p);
+    mi.domain = volume.domain;
+    reconstruct_error(cam, volume, mi, error_params);
+    write_moment_image(mi, mi_file, error_params.compress, error_params.quantization_bits,
+                       error_params.entropy_coding);
+}
+
+// reconstruct_uncertainty_interpolate / _convolution (src/main.cpp:475-519)
+void run_uncertainty(const string &mi_file, const string &output, unordered_map<string, string> p, int mode)
+{
+    auto up = parse_uncertainty_rec_params(p);
+    float factor = 1.f;
+    if (p.count("preintegration_step_size_factor"))
+        factor = std::stof(p["preintegration_step_size_factor"]);
+    auto tf = mode == 0 ? read_from_disk(need(p, "tf").c_str(), up.rec.step_size * factor)
+                        : read_from_disk_uncertainty(need(p, "tf").c_str(), up.rec.step_size * factor,
+                                                     up.error_type);
+    auto mi = load_moment_image(mi_file);
+    auto cam = cut::load_from_file(need(p, "camera"), mi.width, mi.height);
+    vector<Vec4f> rgba;
+    reconstruct_uncertainty(cam, mi, tf, rgba, up, mode);
+    write_outputs(output, rgba, mi.width, mi.height, p);
+}
+
 // the reference renderer (src/main.cpp:134-168): raw volumes only
 void run_reference(const string &output, const string &dataset, unordered_map<string, string> p)
 {
@@ -200,7 +268,10 @@ int main(int argc, char *argv[])
     {
         std::cerr << "usage: mray --generate <moment_image> <volume.raw> \"k=v,...\"\n"
                      "       mray --reconstruct <moment_image> <output.ppm> \"k=v,...\"\n"
-                     "       mray --reference <output.ppm> <volume.raw> \"k=v,...\"\n";
+                     "       mray --reference <output.ppm> <volume.raw> \"k=v,...\"\n"
+                     "       mray --error-bounds <moment_image> <volume.raw> \"k=v,...\"\n"
+                     "       mray --uncertainty-interpolate <moment_image> <output.ppm> \"k=v,...\"\n"
+                     "       mray --uncertainty-convolution <moment_image> <output.ppm> \"k=v,...\"\n";
         return 2;
     }
     const string mode(argv[1]);
@@ -212,6 +283,12 @@ int main(int argc, char *argv[])
             run_reconstruct(argv[2], argv[3], parse_parameters(argv[4]));
         else if (mode == "--reference")
             run_reference(argv[2], argv[3], parse_parameters(argv[4]));
+        else if (mode == "--error-bounds")
+            run_error_bounds(argv[2], argv[3], parse_parameters(argv[4]));
+        else if (mode == "--uncertainty-interpolate")
+            run_uncertainty(argv[2], argv[3], parse_parameters(argv[4]), 0);
+        else if (mode == "--uncertainty-convolution")
+            run_uncertainty(argv[2], argv[3], parse_parameters(argv[4]), 1);
         else
         {
             std::cerr << "unknown mode " << mode << "\n";

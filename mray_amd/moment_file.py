@@ -37,6 +37,7 @@ def load_library() -> C.CDLL:
         lib.mray_host_load_moment_image.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                                     C.c_void_p, C.c_void_p]
         lib.mray_host_quantization_table.argtypes = [C.c_void_p]
+        lib.mray_host_uncertainty_tf.argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_void_p]
         _lib = lib
     return _lib
 

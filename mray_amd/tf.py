@@ -43,3 +43,20 @@ def read_tf_file(path) -> np.ndarray:
     if len(data) < 1024:
         raise ValueError(f"{path}: transfer function needs 1024 bytes, got {len(data)}")
     return np.frombuffer(data[:1024], dtype=np.uint8).reshape(256, 4).copy()
+
+
+def uncertainty_table(rgba8: np.ndarray, step_size: float, error_type: int) -> np.ndarray:
+    """The 256 x 256 x RGBA uncertainty transfer function of the C++ host
+    library (mray_host_uncertainty_tf, read_from_disk_uncertainty of
+    src/transfer_function.cpp:233-248): box filter for the maximum error and
+    the percentile (error_type 0, 2), gaussian filter for the RMSE (1)."""
+    from . import moment_file
+
+    out = np.zeros((256, 256, 4), np.float32)
+    rgba8 = np.ascontiguousarray(rgba8, np.uint8)
+    if rgba8.size != 1024:
+        raise ValueError("rgba8 must hold 256 RGBA entries")
+    moment_file._check(moment_file.load_library().mray_host_uncertainty_tf(rgba8.ctypes.data, float(step_size),
+                                                                          int(error_type), out.ctypes.data),
+                       "mray_host_uncertainty_tf")
+    return out

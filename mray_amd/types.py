@@ -67,6 +67,23 @@ class GenParams(C.Structure):
         super().__init__(**defaults)
 
 
+class ErrParams(C.Structure):
+    """mray_error_params (ErrorReconstructionParameters without the file options)."""
+    _fields_ = [
+        ("error_type", C.c_int32),
+        ("error_percentile", C.c_int32),
+        ("bias", C.c_float),
+        ("data_min", C.c_float),
+        ("data_max", C.c_float),
+        ("step_size", C.c_float),
+    ]
+
+    def __init__(self, **kw):
+        defaults = dict(error_type=2, error_percentile=90, bias=1e-6, data_min=0.0, data_max=1.0, step_size=0.01)
+        defaults.update(kw)
+        super().__init__(**defaults)
+
+
 class RecParams(C.Structure):
     _fields_ = [
         ("bias", C.c_float),

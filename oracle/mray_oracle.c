@@ -2137,6 +2137,509 @@ void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_co
 }
 
 /* ------------------------------------------------------------------------ */
+/* error bounds and uncertainty (src/raymarch_reconstruct.h:203-367,        */
+/* src/error_measures.h, src/transfer_function.cpp:99-248)                  */
+/* ------------------------------------------------------------------------ */
+
+/* TransferFunction::sample on a full 256 x 256 table (src/transfer_function.h:44-59):
+ * the (x, y + 1) corner is weighted by a.x (1 - a.y) and (x + 1, y) by
+ * (1 - a.x) a.y, as the reference writes it */
+void orc_tf2d_sample(const float *tf, float fs, float bs, float out[4])
+{
+    float ipx = fs * 256.0f, ipy = bs * 256.0f;
+    float ax = ipx - floorf(ipx), ay = ipy - floorf(ipy);
+    int xl = (int)floorf(ipx), yl = (int)floorf(ipy);
+    int x0 = iclampr(xl, 0, 255), x1 = iclampr(xl + 1, 0, 255);
+    int y0 = iclampr(yl, 0, 255), y1 = iclampr(yl + 1, 0, 255);
+    const float *ll = tf + ((size_t)y0 * 256 + x0) * 4, *lr = tf + ((size_t)y1 * 256 + x0) * 4;
+    const float *rr = tf + ((size_t)y1 * 256 + x1) * 4, *rl = tf + ((size_t)y0 * 256 + x1) * 4;
+    float w0 = (1.f - ax) * (1.f - ay), w1 = ax * (1.f - ay), w2 = (1.f - ax) * ay, w3 = ax * ay;
+    for (int c = 0; c < 4; ++c)
+        out[c] = w0 * ll[c] + w1 * lr[c] + w2 * rl[c] + w3 * rr[c];
+}
+
+/* read_tf (src/transfer_function.cpp:99-119) */
+static void read_tf1d(const uint8_t *rgba8, float tf1d[256][4])
+{
+    for (int i = 0; i < 256; ++i)
+    {
+        tf1d[i][0] = powf(rgba8[i * 4 + 0] / 255.f, 2.2f);
+        tf1d[i][1] = powf(rgba8[i * 4 + 1] / 255.f, 2.2f);
+        tf1d[i][2] = powf(rgba8[i * 4 + 2] / 255.f, 2.2f);
+        tf1d[i][3] = rgba8[i * 4 + 3] / 255.f;
+    }
+}
+
+/* sample1D (src/transfer_function.cpp:32-44) */
+static void sample1d(const float tf1d[256][4], float x, float out[4])
+{
+    float ip = x * 256.0f;
+    float a = ip - floorf(ip);
+    int xl = (int)floorf(ip);
+    int i0 = iclampr(xl, 0, 255), i1 = iclampr(xl + 1, 0, 255);
+    for (int c = 0; c < 4; ++c)
+        out[c] = (1.f - a) * tf1d[i0][c] + a * tf1d[i1][c];
+}
+
+/* box_filter (src/transfer_function.cpp:152-185): entry (x, y) averages the
+ * 1D function over the |y - x| + 2 points from x / 256 to y / 256 */
+void orc_tf_box_filter(const uint8_t *rgba8, float step_size, float *table)
+{
+    float tf1d[256][4];
+    read_tf1d(rgba8, tf1d);
+#pragma omp parallel for
+    for (int y = 0; y < 256; ++y)
+        for (int x = 0; x < 256; ++x)
+        {
+            float sf = x / 256.0f, sb = y / 256.0f;
+            int num_steps = abs(y - x) + 2;
+            float dw = 1.0f / (float)num_steps;
+            float rgba[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int i = 0; i < num_steps; ++i)
+            {
+                float w = (float)i / (float)(num_steps - 1);
+                float sv = w * sb + (1 - w) * sf;
+                float t[4];
+                sample1d(tf1d, sv, t);
+                float a = dw * t[3] * step_size;
+                rgba[0] += t[0] * a;
+                rgba[1] += t[1] * a;
+                rgba[2] += t[2] * a;
+                rgba[3] += a;
+            }
+            memcpy(table + ((size_t)y * 256 + x) * 4, rgba, sizeof rgba);
+        }
+}
+
+/* normal_cdf (src/cut/gaussian.h:33-59): Abramowitz and Stegun 7.1.26 with
+ * the reference's float / double promotions */
+static float normal_cdf_std(float x)
+{
+    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f, a5 = 1.061405429f,
+                p = 0.3275911f;
+    int sign = 1;
+    if (x < 0)
+        sign = -1;
+    x = (float)((double)fabsf(x) / 1.41421356237);
+    float t = (float)(1.0 / (1.0 + (double)(p * x)));
+    float y = (float)(1.0 - (double)((((((a5 * t + a4) * t) + a3) * t + a2) * t + a1) * t * expf(-x * x)));
+    return (float)(0.5 * (1.0 + (double)((float)sign * y)));
+}
+
+static float normal_cdf3(float x, float mean, float sigma) { return normal_cdf_std((x - mean) / sigma); }
+
+/* gaussian_filter (src/transfer_function.cpp:187-231): entry (x, y) is the
+ * 1D function weighted by the normal distribution of mean x / 256 and
+ * standard deviation y / 256 over the 256 bins; row 0 is the function */
+void orc_tf_gaussian_filter(const uint8_t *rgba8, float step_size, float *table)
+{
+    float tf1d[256][4];
+    read_tf1d(rgba8, tf1d);
+#pragma omp parallel for
+    for (int y = 0; y < 256; ++y)
+        for (int x = 0; x < 256; ++x)
+        {
+            float mean = x / 256.0f, sd = y / 256.0f;
+            float rgba[4] = {0.f, 0.f, 0.f, 0.f};
+            if (sd != 0.0f)
+            {
+                float delta = 1.0f / 256.0f;
+                for (int i = 0; i < 256; ++i)
+                {
+                    float xi = i * delta;
+                    float weight = normal_cdf3(xi + delta, mean, sd) - normal_cdf3(xi, mean, sd);
+                    float a = weight * tf1d[i][3] * step_size;
+                    rgba[0] += tf1d[i][0] * a;
+                    rgba[1] += tf1d[i][1] * a;
+                    rgba[2] += tf1d[i][2] * a;
+                    rgba[3] += a;
+                }
+            }
+            else
+            {
+                rgba[3] = tf1d[x][3] * step_size;
+                rgba[0] = tf1d[x][0] * rgba[3];
+                rgba[1] = tf1d[x][1] * rgba[3];
+                rgba[2] = tf1d[x][2] * rgba[3];
+            }
+            memcpy(table + ((size_t)y * 256 + x) * 4, rgba, sizeof rgba);
+        }
+}
+
+/* ErrorMax, ErrorRMSE and ErrorPercentile (the P^2 estimator of the
+ * percentile, src/error_measures.h:29-231) */
+typedef struct
+{
+    int type;
+    float maxe, squares;
+    int count;
+    float q;
+    int init;
+    float h[5];
+    int pos[5];
+    float des[5];
+} err_acc;
+
+static void err_init(err_acc *e, int type, int percentile)
+{
+    memset(e, 0, sizeof *e);
+    e->type = type;
+    e->q = (float)percentile / 100.f;
+}
+
+static void sort5(float *v)
+{
+    /* sort_five_numbers (src/error_measures.h:58-98): ascending */
+    for (int i = 1; i < 5; ++i)
+        for (int j = i; j > 0 && v[j] < v[j - 1]; --j)
+        {
+            float t = v[j];
+            v[j] = v[j - 1];
+            v[j - 1] = t;
+        }
+}
+
+static float p2_parabolic(const err_acc *e, int i, int d)
+{
+    float t1 = d / (float)(e->pos[i + 1] - e->pos[i - 1]);
+    float t2 = (e->pos[i] - e->pos[i - 1] + d) * (e->h[i + 1] - e->h[i]) / (float)(e->pos[i + 1] - e->pos[i]);
+    float t3 = (e->pos[i + 1] - e->pos[i] - d) * (e->h[i] - e->h[i - 1]) / (float)(e->pos[i] - e->pos[i - 1]);
+    return e->h[i] + t1 * (t2 + t3);
+}
+
+static float p2_linear(const err_acc *e, int i, int d)
+{
+    return e->h[i] + d * (e->h[i + d] - e->h[i]) / (float)(e->pos[i + d] - e->pos[i]);
+}
+
+static void err_update(err_acc *e, float ref, float pred)
+{
+    if (e->type == MRAY_ERROR_MAX)
+    {
+        e->maxe = fmaxr(e->maxe, fabsf(pred - ref));
+        return;
+    }
+    if (e->type == MRAY_ERROR_RMSE)
+    {
+        e->count += 1;
+        float d = pred - ref;
+        e->squares += d * d;
+        return;
+    }
+    float value = fabsf(pred - ref);
+    if (e->init < 5)
+    {
+        e->h[e->init++] = value;
+        if (e->init == 5)
+        {
+            sort5(e->h);
+            for (int i = 0; i < 5; ++i)
+                e->pos[i] = 1 + i;
+            e->des[0] = 1.0f;
+            e->des[1] = 1.0f + 2.0f * e->q;
+            e->des[2] = 1.0f + 4.0f * e->q;
+            e->des[3] = 3.0f + 2.0f * e->q;
+            e->des[4] = 5.0f;
+        }
+        return;
+    }
+    int move = 0;
+    if (value < e->h[0])
+    {
+        e->h[0] = value;
+        move = 1;
+    }
+    else if (value < e->h[1])
+        move = 1;
+    else if (value < e->h[2])
+        move = 2;
+    else if (value < e->h[3])
+        move = 3;
+    else if (value < e->h[4])
+        move = 4;
+    else if (value > e->h[4])
+    {
+        e->h[4] = value;
+        move = 4;
+    }
+    for (int k = move; k < 5; ++k)
+        e->pos[k] += 1;
+    e->des[1] += e->q / 2.0f;
+    e->des[2] += e->q;
+    e->des[3] += (1.0f + e->q) / 2.f;
+    e->des[4] += 1.0f;
+    /* adjust_heights: markers 1..3 (marker 4's desired position and position
+     * both grow by exactly 1 per update, so it never moves) */
+    for (int i = 1; i < 4; ++i)
+    {
+        float di = e->des[i] - e->pos[i];
+        if ((di >= 1 && (e->pos[i + 1] - e->pos[i] > 1)) || (di <= -1 && (e->pos[i - 1] - e->pos[i] < -1)))
+        {
+            int sdi = (di > 0) ? 1 : -1;
+            float qp = p2_parabolic(e, i, sdi);
+            if ((e->h[i - 1] < qp) && (qp < e->h[i + 1]))
+                e->h[i] = qp;
+            else
+                e->h[i] = p2_linear(e, i, sdi);
+            e->pos[i] += sdi;
+        }
+    }
+}
+
+/* the estimate; fewer than 3 percentile samples read marker heights the
+ * reference leaves uninitialised, here 0 */
+static float err_value(const err_acc *e)
+{
+    if (e->type == MRAY_ERROR_MAX)
+        return e->maxe;
+    if (e->type == MRAY_ERROR_RMSE)
+        return sqrtf(e->squares / (float)e->count);
+    return e->h[2];
+}
+
+/* reconstruct_error (src/raymarch_reconstruct.h:323-367): from t_near in
+ * steps of step_size while t <= t_far, the reconstruction at the phase of t
+ * against the mapped volume sample at t */
+float orc_error_bound_ray(const orc_volume *v, const float org[3], const float dir[3], float znear, int m,
+                          const float *pm, float blo, float bhi, const mray_error_params *ep)
+{
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, v->dmin, v->dmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    if (!(hit && (t_far > 0)))
+        return 0.0f;
+    err_acc e;
+    err_init(&e, ep->error_type, ep->error_percentile);
+    mray_gen_params mp;
+    memset(&mp, 0, sizeof mp);
+    mp.data_min = ep->data_min;
+    mp.data_max = ep->data_max;
+    float t = t_near;
+    while (t <= t_far)
+    {
+        float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+        float fs = mese_reconstruct(m, pm, blo, bhi, p, 0);
+        float pos[3];
+        sample_at(v, org, dir, t, pos);
+        float ref = map_density(orc_volume_sample(v, pos), &mp);
+        err_update(&e, ref, fs);
+        t += ep->step_size;
+    }
+    return err_value(&e);
+}
+
+/* MomentImageHost::set_error_bound (src/moment_image.h:102-106) */
+static uint16_t quantize_error_bound(float e) { return (uint16_t)(uint32_t)(int32_t)(e * 65535.f); }
+
+/* reconstruct_error_host (src/moment_raymarcher.cpp:516-539): every pixel's
+ * multipliers prepared (prediction-coded or standard), then the error of the
+ * reconstruction along its ray; a pixel without moments reconstructs with
+ * an empty series (0), where the reference reads the first entry of an
+ * empty multiplier array */
+void orc_error_bounds(const orc_volume *v, const mray_camera *cam, int W, int H, int m, const float *data,
+                      const uint32_t *index, const uint32_t *bounds, int prediction_coded, int coding_warp,
+                      const float *coding_params, const mray_error_params *ep, uint16_t *out, int nthreads)
+{
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+    {
+        float pm[MAXM];
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            int nm = index ? (int)(index[pix + 1] - index[pix]) : m;
+            size_t off = index ? index[pix] : pix * (size_t)m;
+            if (nm > 0 && prediction_coded)
+                orc_prepare_moments_pred(nm, data + off, coding_warp, coding_params, pm);
+            else if (nm > 0)
+                orc_prepare_moments_std(nm, data + off, ep->bias, pm);
+            else
+                pm[0] = 0.0f;
+            float blo = 0.f, bhi = 1.f;
+            if (bounds)
+            {
+                blo = MRAY_BOUND_LO(bounds[pix]) / 65535.f;
+                bhi = MRAY_BOUND_HI(bounds[pix]) / 65535.f;
+            }
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            out[pix] = quantize_error_bound(orc_error_bound_ray(v, org, dir, cam->znear, nm > 0 ? nm : 1, pm, blo, bhi,
+                                                                ep));
+        }
+    }
+}
+
+/* The density bounds at phase p (ErrorBoundMESEReconstruction and
+ * MeanStdDevMESEReconstruction, src/reconstruction_techniques.h:99-135):
+ * [clamp(rho - e), clamp(rho + e)] to the pixel's bounds, or (rho, e) */
+static void density_bounds(int m, const float *pm, float blo, float bhi, float p, int fourier, float e,
+                           int mean_std, float out[2])
+{
+    float rho = mese_reconstruct(m, pm, blo, bhi, p, fourier);
+    if (mean_std)
+    {
+        out[0] = rho;
+        out[1] = e;
+    }
+    else
+    {
+        out[0] = fclampr(rho - e, blo, bhi);
+        out[1] = fclampr(rho + e, blo, bhi);
+    }
+}
+
+/* reconstruct_ray with ErrorBoundInterpolator (src/reconstruction_techniques.h:140-156):
+ * the preintegrated march over (1 - s) lower + s upper */
+static void uncertainty_interp_ray(const float org[3], const float dir[3], const float bmin[3], const float bmax[3],
+                                   float znear, int m, const float *pm, float blo, float bhi, float e, float s,
+                                   const float *tf, const mray_rec_params *rp, float out[4])
+{
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    out[0] = out[1] = out[2] = out[3] = 0.f;
+    if (!(hit && (t_far > 0)))
+        return;
+    float step = rp->step_size;
+    int fourier = rp->use_truncated_fourier;
+    float t = t_near;
+    float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+    float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+    float L[3] = {0.0f, 0.0f, 0.0f};
+    float alpha = 0.0f, T = 1.0f, tau = 0.0f;
+    float b[2];
+    density_bounds(m, pm, blo, bhi, p, fourier, e, 0, b);
+    float fs = (1 - s) * b[0] + s * b[1];
+    while (t <= t_far)
+    {
+        t += step;
+        p += phase_step;
+        density_bounds(m, pm, blo, bhi, p, fourier, e, 0, b);
+        float bs = (1 - s) * b[0] + s * b[1];
+        float rgba[4];
+        orc_tf_sample(tf, fs, bs, rgba);
+        for (int c = 0; c < 3; ++c)
+            L[c] += (rp->medium_emission * rgba[c] + 0.0f * rp->medium_scattering) * T;
+        tau += rgba[3];
+        T = (float)exp((double)(-tau));
+        alpha = 1.0f - T;
+        fs = bs;
+        if (alpha >= rp->early_ray_termination)
+        {
+            alpha = 1.f;
+            break;
+        }
+    }
+    out[0] = L[0];
+    out[1] = L[1];
+    out[2] = L[2];
+    out[3] = alpha;
+}
+
+/* reconstruct_ray_uncertainty_tf (src/raymarch_reconstruct.h:203-257): one
+ * sample of the bounds per step, classified by the 2D uncertainty table */
+static void uncertainty_conv_ray(const float org[3], const float dir[3], const float bmin[3], const float bmax[3],
+                                 float znear, int m, const float *pm, float blo, float bhi, float e, int mean_std,
+                                 const float *tf2d, const mray_rec_params *rp, float out[4])
+{
+    float t_near, t_far;
+    int hit = orc_intersect_aabb(org, dir, bmin, bmax, &t_near, &t_far);
+    t_near = fmaxr(t_near, znear);
+    out[0] = out[1] = out[2] = out[3] = 0.f;
+    if (!(hit && (t_far > 0)))
+        return;
+    float step = rp->step_size;
+    int fourier = rp->use_truncated_fourier;
+    float t = t_near;
+    float p = (float)((double)((t - t_near) / (t_far - t_near)) * M_PI - M_PI);
+    float phase_step = (float)(M_PI * (double)(step / (t_far - t_near)));
+    float L[3] = {0.0f, 0.0f, 0.0f};
+    float alpha = 0.0f, T = 1.0f, tau = 0.0f;
+    while (t <= t_far)
+    {
+        t += step;
+        p += phase_step;
+        float b[2], rgba[4];
+        density_bounds(m, pm, blo, bhi, p, fourier, e, mean_std, b);
+        orc_tf2d_sample(tf2d, b[0], b[1], rgba);
+        for (int c = 0; c < 3; ++c)
+            L[c] += (rp->medium_emission * rgba[c] + 0.0f * rp->medium_scattering) * T;
+        tau += rgba[3];
+        T = (float)exp((double)(-tau));
+        alpha = 1.0f - T;
+        if (alpha >= rp->early_ray_termination)
+        {
+            alpha = 1.f;
+            break;
+        }
+    }
+    out[0] = L[0];
+    out[1] = L[1];
+    out[2] = L[2];
+    out[3] = alpha;
+}
+
+/* reconstruct_uncertainty_interpolation_host / _convolution_host
+ * (src/moment_raymarcher.cpp:563-712): mode 0 interpolates between the
+ * bounds (s = bound_interpolation, tf the preintegrated table), mode 1
+ * classifies (lower, upper) for MRAY_ERROR_MAX and (mean, error) otherwise
+ * with the 2D uncertainty table. error_bounds: W*H uint16 (e / 65535). */
+void orc_uncertainty_image(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                           const float *data, const uint32_t *index, const uint32_t *bounds,
+                           const uint16_t *error_bounds, int prediction_coded, int coding_warp,
+                           const float *coding_params, const float *tf, const mray_rec_params *p, int mode,
+                           int error_type, float bound_interpolation, float *rgba, int nthreads)
+{
+#pragma omp parallel for num_threads(set_threads(nthreads))
+    for (int y = 0; y < H; ++y)
+    {
+        float pm[MAXM];
+        for (int x = 0; x < W; ++x)
+        {
+            size_t pix = (size_t)y * W + x;
+            int nm = index ? (int)(index[pix + 1] - index[pix]) : m;
+            size_t off = index ? index[pix] : pix * (size_t)m;
+            float *out = rgba + pix * 4;
+            if (nm == 0)
+            {
+                float c[4];
+                if (mode == 0)
+                    orc_tf_sample(tf, 0.0f, 0.0f, c);
+                else
+                    orc_tf2d_sample(tf, 0.0f, 0.0f, c);
+                out[0] = c[0];
+                out[1] = c[1];
+                out[2] = c[2];
+                out[3] = 0.0f;
+                continue;
+            }
+            if (p->use_truncated_fourier)
+                for (int l = 0; l < nm; ++l)
+                    pm[l] = prediction_coded ? 0.0f : data[off + l];
+            else if (prediction_coded)
+                orc_prepare_moments_pred(nm, data + off, coding_warp, coding_params, pm);
+            else
+                orc_prepare_moments_std(nm, data + off, p->bias, pm);
+            float blo = 0.f, bhi = 1.f;
+            if (bounds)
+            {
+                blo = MRAY_BOUND_LO(bounds[pix]) / 65535.f;
+                bhi = MRAY_BOUND_HI(bounds[pix]) / 65535.f;
+            }
+            const float e = (float)error_bounds[pix] / 65535.f;
+            float org[3], dir[3];
+            orc_pixel_ray(cam, W, H, x, y, org, dir);
+            if (mode == 0)
+                uncertainty_interp_ray(org, dir, bmin, bmax, cam->znear, nm, pm, blo, bhi, e, bound_interpolation, tf, p,
+                                       out);
+            else
+                uncertainty_conv_ray(org, dir, bmin, bmax, cam->znear, nm, pm, blo, bhi, e,
+                                     error_type != MRAY_ERROR_MAX, tf, p, out);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* entropy coding (src/moment_image_coding.cpp, src/entropy_coding/)        */
 /* ------------------------------------------------------------------------ */
 

@@ -179,6 +179,29 @@ size_t orc_quantize_image(int W, int H, const float *data, const uint32_t *index
 void orc_dequantize_image(int W, int H, const uint32_t *index, int prediction_coded, const uint8_t *bits,
                           const uint8_t *in, size_t n_in, float *data);
 
+/* ---- error bounds and uncertainty (src/raymarch_reconstruct.h:203-367,
+ * src/error_measures.h, src/transfer_function.cpp:99-248) ----
+ * orc_tf2d_sample samples a full 256 x 256 x RGBA table; the box (max error,
+ * percentile) and gaussian (RMSE) uncertainty tables are built from a 256 x
+ * RGBA8 transfer function. orc_error_bounds writes the per-pixel uint16
+ * error bound (e * 65535, truncated) of reconstruct_error_host;
+ * orc_uncertainty_image renders float RGBA with the interpolated bounds
+ * (mode 0, tf: the preintegrated 256 x 4 table) or the bound convolution
+ * (mode 1, tf: a 2D table). */
+void orc_tf2d_sample(const float *tf, float fs, float bs, float out[4]);
+void orc_tf_box_filter(const uint8_t *rgba8, float step_size, float *table);
+void orc_tf_gaussian_filter(const uint8_t *rgba8, float step_size, float *table);
+float orc_error_bound_ray(const orc_volume *v, const float org[3], const float dir[3], float znear, int m,
+                          const float *pm, float blo, float bhi, const mray_error_params *ep);
+void orc_error_bounds(const orc_volume *v, const mray_camera *cam, int W, int H, int m, const float *data,
+                      const uint32_t *index, const uint32_t *bounds, int prediction_coded, int coding_warp,
+                      const float *coding_params, const mray_error_params *ep, uint16_t *out, int nthreads);
+void orc_uncertainty_image(const mray_camera *cam, const float bmin[3], const float bmax[3], int W, int H, int m,
+                           const float *data, const uint32_t *index, const uint32_t *bounds,
+                           const uint16_t *error_bounds, int prediction_coded, int coding_warp,
+                           const float *coding_params, const float *tf, const mray_rec_params *p, int mode,
+                           int error_type, float bound_interpolation, float *rgba, int nthreads);
+
 /* ---- entropy coding (src/moment_image_coding.cpp, FastAC) ----
  * The entropy-coded moment section of a compact image: u32 count of
  * non-empty pixels, their moment 0 as u16 words (bits[0] bits), then per

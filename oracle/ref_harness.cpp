@@ -548,4 +548,172 @@ void ref_load_moment_image(const char *path, long *hdr, uint32_t *index, float *
             coding_params[2 * i + 1] = mi.coding_params[i].param2;
         }
 }
+
+// read_from_disk_uncertainty (src/transfer_function.cpp:233-248) of a
+// 256 x RGBA8 transfer function: the 256 x 256 x RGBA table
+void ref_tf_uncertainty(const uint8_t *rgba8, float step, int error_type, float *table)
+{
+    char path[] = "/tmp/mrayref_utfXXXXXX";
+    int fd = mkstemp(path);
+    FILE *f = fdopen(fd, "wb");
+    fwrite(rgba8, 1, 1024, f);
+    fclose(f);
+    TransferFunction tf = read_from_disk_uncertainty(path, step, error_type);
+    unlink(path);
+    std::memcpy(table, tf.data.data(), sizeof(float) * 4 * 256 * 256);
+}
+
+namespace
+{
+MomentImageHost make_image(const float *dmin, const float *dmax, int W, int H, int m, const float *data, long n_data,
+                           const uint32_t *index, const uint32_t *bounds, const uint16_t *error_bounds,
+                           int prediction_coded, int coding_warp, const uint16_t *coding_params)
+{
+    MomentImageHost mi(W, H, m, index != nullptr, bounds != nullptr);
+    mi.domain = Range<Vec3f>(Vec3f(dmin[0], dmin[1], dmin[2]), Vec3f(dmax[0], dmax[1], dmax[2]));
+    mi.data.assign(data, data + n_data);
+    if (index)
+    {
+        mi.index.assign(index, index + (size_t)W * H + 1);
+        mi.is_compact = true;
+    }
+    if (bounds)
+        for (size_t i = 0; i < (size_t)W * H; ++i)
+        {
+            mi.bounds[i].lower = (uint16_t)(bounds[i] & 0xFFFF);
+            mi.bounds[i].upper = (uint16_t)(bounds[i] >> 16);
+        }
+    if (error_bounds)
+        mi.error_bounds.assign(error_bounds, error_bounds + (size_t)W * H);
+    if (prediction_coded)
+    {
+        mi.prediction_code = true;
+        mi.coding_warp = coding_warp;
+        if (coding_warp == CODING_WARP_DEFAULT_TRANSFORMED)
+            for (int l = 0; l < m - 1; ++l)
+                mi.coding_params.push_back(CodingParamType{coding_params[2 * l], coding_params[2 * l + 1]});
+    }
+    return mi;
+}
+} // namespace
+
+// reconstruct_error (src/moment_raymarcher.cpp:516-561, the host path) of a
+// moment image against its volume: the per-pixel uint16 error bounds
+void ref_error_bounds(const void *vol, int dtype, int nx, int ny, int nz, const float *dmin, const float *dmax,
+                      const mray_camera *cam, int W, int H, int m, const float *data, long n_data,
+                      const uint32_t *index, const uint32_t *bounds, int prediction_coded, int coding_warp,
+                      const uint16_t *coding_params, const mray_error_params *ep, uint16_t *out)
+{
+    auto grid = make_grid(vol, dtype, nx, ny, nz, dmin, dmax);
+    auto mi = make_image(dmin, dmax, W, H, m, data, n_data, index, bounds, nullptr, prediction_coded, coding_warp,
+                         coding_params);
+    mi.add_error_bounds();
+    ErrorReconstructionParameters p;
+    p.error_type = ep->error_type;
+    p.error_percentile = ep->error_percentile;
+    p.bias = ep->bias;
+    p.data_min = ep->data_min;
+    p.data_max = ep->data_max;
+    p.step_size = ep->step_size;
+    reconstruct_error(make_cam(cam), grid, mi, p);
+    std::memcpy(out, mi.error_bounds.data(), (size_t)W * H * sizeof(uint16_t));
+}
+
+// The uncertainty renderings (src/moment_raymarcher.cpp:563-712): float RGBA
+// of the host loops before RGBA8 quantisation, and the RGBA8 image of the
+// real functions. mode 0: reconstruct_uncertainty_interpolation with the
+// preintegrated table of tf_rgba8 at tf_step; mode 1:
+// reconstruct_uncertainty_convolution with read_from_disk_uncertainty's
+// table at tf_step. Single scattering off.
+void ref_uncertainty(const mray_camera *cam, const float *dmin, const float *dmax, int W, int H, int m,
+                     const float *data, long n_data, const uint32_t *index, const uint32_t *bounds,
+                     const uint16_t *error_bounds, int prediction_coded, int coding_warp,
+                     const uint16_t *coding_params, const uint8_t *tf_rgba8, float tf_step, const mray_rec_params *p,
+                     int mode, int error_type, float bound_interpolation, float *rgba, uint8_t *rgba8)
+{
+    auto mi = make_image(dmin, dmax, W, H, m, data, n_data, index, bounds, error_bounds, prediction_coded,
+                         coding_warp, coding_params);
+    TransferFunction tf;
+    if (mode == 0)
+        tf = make_tf(tf_rgba8, tf_step);
+    else
+        ref_tf_uncertainty(tf_rgba8, tf_step, error_type, reinterpret_cast<float *>(tf.data.data()));
+    UncertaintyReconstructionParameters up;
+    up.rec = make_rec(p);
+    up.bound_interpolation = bound_interpolation;
+    up.error_type = error_type;
+    auto c = make_cam(cam);
+    const auto &rp = up.rec;
+    if (rgba)
+    {
+#pragma omp parallel for
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+            {
+                Vec2f ndc(x / static_cast<float>(W) * 2.f - 1.f, y / static_cast<float>(H) * 2.f - 1.f);
+                Vec4f col(0.0f);
+                const float e = mi.get_error_bound(x, y);
+                const auto ray = cut::get_eye_ray(c, ndc);
+                if (mi.get_num_moments(x, y) == 0)
+                    col = reconstruct_zero_ray(tf);
+                else if (rp.use_truncated_fourier)
+                {
+                    vector<float> pm(mi.get_num_moments(x, y));
+                    mi.get_trig_moments(x, y, pm);
+                    TruncatedFourierReconstructionDefault rec(pm.size(), pm.data(), mi.get_bounds(x, y));
+                    if (mode == 0)
+                        col = reconstruct_ray(ray, mi.domain, c.znear,
+                                              ErrorBoundInterpolator<ErrorBoundMESEReconstruction<
+                                                  TruncatedFourierReconstructionDefault>>(
+                                                  ErrorBoundMESEReconstruction<TruncatedFourierReconstructionDefault>(
+                                                      rec, e),
+                                                  bound_interpolation),
+                                              tf, NoSingleScattering{}, rp);
+                    else if (error_type == MAX_ERROR)
+                        col = reconstruct_ray_uncertainty_tf(
+                            ray, mi.domain, c.znear,
+                            ErrorBoundMESEReconstruction<TruncatedFourierReconstructionDefault>(rec, e), tf,
+                            NoSingleScattering{}, rp);
+                    else
+                        col = reconstruct_ray_uncertainty_tf(
+                            ray, mi.domain, c.znear,
+                            MeanStdDevMESEReconstruction<TruncatedFourierReconstructionDefault>(rec, e), tf,
+                            NoSingleScattering{}, rp);
+                }
+                else
+                {
+                    auto pm = mi.prepare_moments(x, y, rp.bias);
+                    MESEReconstructionDefault rec(pm.size(), pm.data(), mi.get_bounds(x, y));
+                    if (mode == 0)
+                        col = reconstruct_ray(
+                            ray, mi.domain, c.znear,
+                            ErrorBoundInterpolator<ErrorBoundMESEReconstruction<MESEReconstructionDefault>>(
+                                ErrorBoundMESEReconstruction<MESEReconstructionDefault>(rec, e), bound_interpolation),
+                            tf, NoSingleScattering{}, rp);
+                    else if (error_type == MAX_ERROR)
+                        col = reconstruct_ray_uncertainty_tf(
+                            ray, mi.domain, c.znear, ErrorBoundMESEReconstruction<MESEReconstructionDefault>(rec, e),
+                            tf, NoSingleScattering{}, rp);
+                    else
+                        col = reconstruct_ray_uncertainty_tf(
+                            ray, mi.domain, c.znear, MeanStdDevMESEReconstruction<MESEReconstructionDefault>(rec, e),
+                            tf, NoSingleScattering{}, rp);
+                }
+                float *o = rgba + ((size_t)y * W + x) * 4;
+                o[0] = col.x;
+                o[1] = col.y;
+                o[2] = col.z;
+                o[3] = col.w;
+            }
+    }
+    if (rgba8)
+    {
+        ColorImageHost img(W, H);
+        if (mode == 0)
+            reconstruct_uncertainty_interpolation(c, mi, tf, img, up);
+        else
+            reconstruct_uncertainty_convolution(c, mi, tf, img, up);
+        std::memcpy(rgba8, img.rgba.data(), (size_t)W * H * 4);
+    }
+}
 }

@@ -369,6 +369,72 @@ def _entropy_image(rng, W, H, m, pc, empty=0.2):
     return index, data
 
 
+UNC_CASES = {
+    # name: (volume, W, H, m, compact, prediction_coded, camera, gen step, error step)
+    "ml_ortho": ("ml", 24, 20, 16, 1, 0, "ortho", 1.0 / 32, 1.0 / 40),
+    "mix_persp_u8": ("mix_u8", 26, 22, 12, 1, 0, "persp", 1.0 / 32, 1.0 / 24),
+    "ml_coded": ("ml", 20, 18, 14, 1, 1, "ortho", 1.0 / 32, 1.0 / 40),
+    "ml_dense_coarse": ("ml", 16, 14, 10, 0, 0, "ortho", 1.0 / 32, 0.45),
+}
+
+
+def uncertainty_volume(kind):
+    if kind == "ml":
+        return synth.marschner_lobb(32), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    vol = np.clip(synth.mixing_layer(32, width=0.05) * 255, 0, 255).astype(np.uint8)
+    return vol, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+
+
+def uncertainty_camera(kind, W, H):
+    if kind == "ortho":
+        return synth.ortho_camera()
+    return synth.perspective_camera(position=(0.5, 0.5, -1.2), fov_half_deg=30.0, znear=0.01, aspect=W / H)
+
+
+def uncertainty_vectors(R: Reference) -> dict:
+    """Error bounds (reconstruct_error: max, RMSE and the P^2 percentile),
+    the uncertainty transfer functions (box and gaussian filters; every 17th
+    row and column plus a SHA-256 of each full table) and the uncertainty
+    renderings (interpolation, convolution per error type) of the reference."""
+    import hashlib
+
+    from mray_amd.types import ErrParams
+
+    out = {}
+    rgba8 = synth.tf_ramps(0)
+    for et in (0, 1):
+        t = R.tf_uncertainty(rgba8, 0.1, et)
+        out[f"tf{et}_sub"] = t[::17, ::17].copy()
+        out[f"tf{et}_sha256"] = np.frombuffer(hashlib.sha256(t.tobytes()).digest(), np.uint8)
+    for name, (vk, W, H, m, compact, pc, ck, gstep, estep) in UNC_CASES.items():
+        vol, dmin, dmax = uncertainty_volume(vk)
+        cam = uncertainty_camera(ck, W, H)
+        gp = GenParams(step_size=gstep, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=compact,
+                       prediction_coding=pc, error_threshold=2e-3, bias=1e-4)
+        data, bounds, index, cp = R.generate(vol, dmin, dmax, cam, W, H, m, gp)
+        data = np.ascontiguousarray(data, np.float32)
+        idx = index if compact else None
+        cpq = cp if pc else None
+        out.update({f"{name}_data": data, f"{name}_bounds": bounds, f"{name}_shape": np.array([W, H, m], np.int32)})
+        if compact:
+            out[f"{name}_index"] = index
+        if pc:
+            out[f"{name}_coding_params"] = cp
+        for et in (0, 1, 2):
+            ep = ErrParams(error_type=et, error_percentile=90 if et == 2 else 95, bias=1e-4, step_size=estep)
+            out[f"{name}_err{et}"] = R.error_bounds(vol, dmin, dmax, cam, W, H, m, data, ep, index=idx, bounds=bounds,
+                                                   prediction_coded=pc, coding_params=cpq)
+        rp = RecParams(step_size=1.0 / 40, bias=1e-4, early_ray_termination=0.99, medium_emission=1.0)
+        for mode, et, s in ((0, 2, 0.3), (1, 0, 0.0), (1, 1, 0.0), (1, 2, 0.0)):
+            eb = out[f"{name}_err{et}"]
+            rgba, rgba8_img = R.uncertainty(cam, dmin, dmax, W, H, m, data, eb, rgba8, rp.step_size * 4, rp, mode, et, s,
+                                            index=idx, bounds=bounds, prediction_coded=pc, coding_params=cpq,
+                                            want_rgba8=True)
+            out[f"{name}_unc{mode}{et}"] = rgba
+            out[f"{name}_unc{mode}{et}_rgba8"] = rgba8_img
+    return out
+
+
 def halving_data(W, H, levels, level):
     """The 2-moment image of the halving case: moment 0 = (p % 97) / 97,
     moment 1 = levels[level[p]]."""
@@ -506,7 +572,8 @@ def main():
         return
     makers = {"mese": mese_vectors, "coding": coding_vectors, "images": image_vectors, "sph": sph_vectors,
               "reference": reference_dvr_vectors, "quant": quant_vectors,
-              "curve": curve_vectors, "entropy": entropy_vectors, "files": file_vectors}
+              "curve": curve_vectors, "entropy": entropy_vectors, "files": file_vectors,
+              "uncertainty": uncertainty_vectors}
     for name in (sys.argv[1:] or list(makers)):
         np.savez_compressed(OUT / f"golden_{name}.npz", **makers[name](R))
     for f in sorted(OUT.glob("golden_*.npz")):

@@ -8,7 +8,7 @@ import pathlib
 
 import numpy as np
 
-from mray_amd.types import Camera, GenParams, RecParams, RefParams
+from mray_amd.types import Camera, ErrParams, GenParams, RecParams, RefParams
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
@@ -289,6 +289,42 @@ class Oracle:
                                       C.c_size_t(len(st)), ptr(out, _f))
         return out[:int(index[-1])]
 
+    def tf_uncertainty(self, rgba8, step, error_type):
+        """The 2D uncertainty table (256, 256, 4): box filter for the max error
+        and the percentile, gaussian for the RMSE."""
+        out = np.zeros((256, 256, 4), np.float32)
+        fn = self.lib.orc_tf_gaussian_filter if error_type == 1 else self.lib.orc_tf_box_filter
+        fn(ptr(np.ascontiguousarray(rgba8, np.uint8), _u8), C.c_float(step), ptr(out, _f))
+        return out
+
+    def error_bounds(self, vol, dmin, dmax, cam, W, H, m, data, ep: ErrParams, index=None, bounds=None,
+                     prediction_coded=0, coding_warp=0, coding_params=None):
+        v = self.volume(vol, dmin, dmax)
+        c = Camera.from_dict(cam)
+        out = np.zeros(W * H, np.uint16)
+        self.lib.orc_error_bounds(C.byref(v), C.byref(c), W, H, m, ptr(np.ascontiguousarray(data, np.float32), _f),
+                                  ptr(index, _u32), ptr(bounds, _u32), prediction_coded, coding_warp,
+                                  ptr(coding_params, _f), C.byref(ep), ptr(out, _u16), nthreads())
+        return out
+
+    def uncertainty(self, cam, dmin, dmax, W, H, m, data, error_bounds, tf, rp: RecParams, mode, error_type=1,
+                    bound_interpolation=0.5, index=None, bounds=None, prediction_coded=0, coding_warp=0,
+                    coding_params=None):
+        """Float RGBA of the uncertainty renderings (mode 0 interpolation with
+        the preintegrated table tf (256, 4); mode 1 convolution with a 2D
+        table tf (256, 256, 4))."""
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float32)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        self.lib.orc_uncertainty_image(C.byref(c), bmin, bmax, W, H, m,
+                                       ptr(np.ascontiguousarray(data, np.float32), _f), ptr(index, _u32),
+                                       ptr(bounds, _u32), ptr(np.ascontiguousarray(error_bounds, np.uint16), _u16),
+                                       prediction_coded, coding_warp, ptr(coding_params, _f),
+                                       ptr(np.ascontiguousarray(tf, np.float32), _f), C.byref(rp), mode, error_type,
+                                       C.c_float(bound_interpolation), ptr(rgba, _f), nthreads())
+        return rgba
+
     def entropy_encode(self, W, H, m, data, index, prediction_coded, bits):
         """The entropy-coded moment section (bytes) of a compact image."""
         cap = entropy_capacity(index, m)
@@ -377,6 +413,43 @@ class Reference:
         self.lib.ref_dequantize(W, H, ptr(index, _u32), prediction_coded, ptr(bits, _u8), ptr(st, _u8),
                                 C.c_long(len(st)), ptr(out, _f))
         return out[:int(index[-1])]
+
+    def tf_uncertainty(self, rgba8, step, error_type):
+        out = np.zeros((256, 256, 4), np.float32)
+        self.lib.ref_tf_uncertainty(ptr(np.ascontiguousarray(rgba8, np.uint8), _u8), C.c_float(step), error_type,
+                                    ptr(out, _f))
+        return out
+
+    def error_bounds(self, vol, dmin, dmax, cam, W, H, m, data, ep: ErrParams, index=None, bounds=None,
+                     prediction_coded=0, coding_warp=0, coding_params=None):
+        c = Camera.from_dict(cam)
+        out = np.zeros(W * H, np.uint16)
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        nz, ny, nx = vol.shape
+        data = np.ascontiguousarray(data, np.float32)
+        self.lib.ref_error_bounds(vol.ctypes.data_as(C.c_void_p), 1 if vol.dtype == np.uint8 else 0, nx, ny, nz,
+                                  bmin, bmax, C.byref(c), W, H, m, ptr(data, _f), C.c_long(len(data)),
+                                  ptr(index, _u32), ptr(bounds, _u32), prediction_coded, coding_warp,
+                                  ptr(coding_params, _u16), C.byref(ep), ptr(out, _u16))
+        return out
+
+    def uncertainty(self, cam, dmin, dmax, W, H, m, data, error_bounds, tf_rgba8, tf_step, rp: RecParams, mode,
+                    error_type=1, bound_interpolation=0.5, index=None, bounds=None, prediction_coded=0,
+                    coding_warp=0, coding_params=None, want_rgba8=False):
+        c = Camera.from_dict(cam)
+        rgba = np.zeros((H, W, 4), np.float32)
+        rgba8 = np.zeros((H, W, 4), np.uint8) if want_rgba8 else None
+        bmin = (C.c_float * 3)(*dmin)
+        bmax = (C.c_float * 3)(*dmax)
+        data = np.ascontiguousarray(data, np.float32)
+        self.lib.ref_uncertainty(C.byref(c), bmin, bmax, W, H, m, ptr(data, _f), C.c_long(len(data)),
+                                 ptr(index, _u32), ptr(bounds, _u32),
+                                 ptr(np.ascontiguousarray(error_bounds, np.uint16), _u16), prediction_coded,
+                                 coding_warp, ptr(coding_params, _u16),
+                                 ptr(np.ascontiguousarray(tf_rgba8, np.uint8), _u8), C.c_float(tf_step), C.byref(rp),
+                                 mode, error_type, C.c_float(bound_interpolation), ptr(rgba, _f), ptr(rgba8, _u8))
+        return rgba, rgba8
 
     def entropy_encode(self, W, H, m, data, index, prediction_coded, bits):
         cap = entropy_capacity(index, m)
