@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-quant", action="store_true", help="skip the moment-image quantization")
     ap.add_argument("--no-large", action="store_true", help="skip the large noise-and-blobs volume")
     ap.add_argument("--large-only", action="store_true", help="time only the large noise-and-blobs volume")
+    ap.add_argument("--no-particles", action="store_true", help="skip the 20 M particle workload")
+    ap.add_argument("--particles-only", action="store_true", help="time only the 20 M particle workload")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the adjoint workload and print its figures (for kernel profiles)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -107,6 +109,9 @@ def main():
         return
     if args.large_only:
         print(json.dumps({"large_volume": large_volume(args.steps)}), flush=True)
+        return
+    if args.particles_only:
+        print(json.dumps({"particles": particles_frame(args.steps)}), flush=True)
         return
     n, W, H, m = args.n, args.res, args.res, args.moments
     dmin, dmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
@@ -314,6 +319,7 @@ def main():
         torch.cuda.synchronize()
 
     large = None if args.no_large else large_volume(args.steps)
+    particles = None if args.no_particles else particles_frame(args.steps)
 
     cpu, fp32_rows = None, None
     vol_np = None
@@ -369,6 +375,7 @@ def main():
         "adjoint_frame": adjoint_c1,
         "reference_dvr": dvr,
         "large_volume": large,
+        "particles": particles,
         "quantization": quant,
         "rmse_vs_fp64": precision,
         "cpu_baseline": cpu,
@@ -543,6 +550,59 @@ def large_volume(steps, plain_steps=1):
                         "ms": parts_p}
         out["mixed_over_plain"] = round(ms_plain / ms_mixed, 3)
     del vol, vox, mo
+    torch.cuda.empty_cache()
+    return out
+
+
+def particles_frame(steps, n_particles=20_000_000):
+    """The particle workload: 20 M particles from a mixture of 32 anisotropic
+    Gaussians in [0,1] x [0,1] x [0,0.5] (seed 5, uniform mass), smoothing
+    length 0.004, gathered onto a 512 x 512 x 256 grid with the reference's
+    Particles::sample, then a mixed frame at 1024 x 1024 rays, 40 moments,
+    step 0.5 voxel, and its adjoint back to the grid values."""
+    import torch
+
+    from mray_amd import hip, synth, tf as tfmod
+    from mray_amd.types import GenParams, RecParams
+
+    nx, ny, nz = 512, 512, 256
+    W = H = 1024
+    m = 40
+    h = 0.004
+    dmin, dmax = (0.0, 0.0, 0.0), (1.0, 1.0, 0.5)
+    print(f"[bench] particles: {n_particles} particles (host generator)", file=sys.stderr, flush=True)
+    x, y, z, v = synth.particle_mixture(n_particles, seed=5, n_gauss=32, box=(1.0, 1.0, 0.5))
+    dx, dy, dz, dv = (torch.from_numpy(a).cuda() for a in (x, y, z, v))
+    del x, y, z, v
+    grid = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+    hip.particles_to_grid(dx, dy, dz, dv, h, dmin, dmax, nx, ny, nz, out=grid)
+    torch.cuda.synchronize()
+    print(f"[bench] particles: gather onto {nx}x{ny}x{nz}", file=sys.stderr, flush=True)
+    k = max(2, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        hip.particles_to_grid(dx, dy, dz, dv, h, dmin, dmax, nx, ny, nz, out=grid)
+    torch.cuda.synchronize()
+    ms_splat = (time.perf_counter() - t0) * 1e3 / k
+    occupied = float((grid > 0).float().mean())
+    del dx, dy, dz, dv
+    vol = hip.Volume(grid, dmin, dmax)
+    step = 0.5 / nx
+    cam = synth.perspective_camera(position=(0.5, 0.5, -1.0), fov_half_deg=25.0, znear=0.01, aspect=1.0)
+    gp = GenParams(step_size=step, transform_to_bounds=1, transform_bound_eps=0.01, compact_image=0,
+                   prediction_coding=0, bias=1e-4)
+    rp = RecParams(step_size=step, bias=1e-3, early_ray_termination=0.99, medium_emission=1.0)
+    table = torch.from_numpy(tfmod.table_from_rgba8(synth.tf_ramps(0), step * 64.0)).cuda()
+    adj = adjoint_frame(vol, grid, cam, dmin, dmax, gp, rp, table, W, H, m, steps,
+                        f"{W}x{H} rays, {m} moments, step 0.5 voxel on the gathered grid")
+    out = {"workload": f"{n_particles} particles (32-Gaussian mixture, h = {h}) gathered onto {nx}x{ny}x{nz} f32, "
+                       f"perspective {W}x{H} rays, {m} moments, step 0.5 voxel; mixed forward and its adjoint "
+                       f"to the grid values",
+           "gather_ms": round(ms_splat, 3), "gather_Mvoxels_per_s": round(nx * ny * nz / (ms_splat * 1e-3) / 1e6, 1),
+           "occupied_voxels": round(occupied, 4), "forward_ms": adj["forward_ms"], "adjoint_ms": adj["adjoint_ms"],
+           "adjoint_over_forward": adj["adjoint_over_forward"],
+           "Mrays_per_s": round(W * H / (adj["forward_ms"] * 1e-3) / 1e6, 3)}
+    del vol, grid
     torch.cuda.empty_cache()
     return out
 
