@@ -20,11 +20,13 @@
 //         reference's arithmetic-free result to ~1e-13:
 //             + sum_k [ -delta_k f_k cos(j p_k) + h_k sin(j p_k) / j ].
 //         A node costs one FP64 sincos of its (float) phase and two
-//         Chebyshev recurrences over j (cos and sin of j phi): 5 or 6 FP64
-//         FMAs per moment (accumulate_node) instead of 2(m-1) sin/cos; gfx950
-//         issues FP64 FMAs at the rate of unpacked FP32 ones, so the sums,
-//         the recurrences and trig_0 (the exact integral of the reference's
-//         lines) are all FP64 and the moments are rounded to float once. The
+//         Chebyshev recurrences over j (cos and sin of j phi): five
+//         operations per moment (accumulate_node) instead of 2(m-1)
+//         sin/cos. gfx950 issues FP64 FMAs at the rate of unpacked FP32
+//         ones, so the cosine recurrence and sums (the moments' bulk) and
+//         trig_0 (the exact integral of the reference's lines) are FP64; the
+//         sine terms carry only the first-order ulp corrections and stay
+//         FP32. The moments are rounded to float once. The
 //         phasor is not carried from node to node by rotation: the
 //         reference's node phases jitter by ulps around the uniform grid, and
 //         a rotated phasor drifts off cos(p_k) by more than the high orders
