@@ -19,19 +19,19 @@
 //         (eps); both enter to first order, exactly enough to reproduce the
 //         reference's arithmetic-free result to ~1e-13:
 //             + sum_k [ -delta_k f_k cos(j p_k) + h_k sin(j p_k) / j ].
-//         A node costs one FP64 sincos of its (float) phase and two
-//         Chebyshev recurrences over j (cos and sin of j phi): five
-//         operations per moment (accumulate_node) instead of 2(m-1)
-//         sin/cos. gfx950 issues FP64 FMAs at the rate of unpacked FP32
-//         ones, so the cosine recurrence and sums (the moments' bulk) and
-//         trig_0 (the exact integral of the reference's lines) are FP64; the
-//         sine terms carry only the first-order ulp corrections and stay
-//         FP32. The moments are rounded to float once. The
-//         phasor is not carried from node to node by rotation: the
-//         reference's node phases jitter by ulps around the uniform grid, and
-//         a rotated phasor drifts off cos(p_k) by more than the high orders
-//         tolerate. Moments live in registers (template M >= m); more than 64
-//         are split into chunks over workgroups (CHUNK).
+//         A node costs cos and sin of its (float) phase and two Chebyshev
+//         recurrences over j (cos and sin of j phi): five operations per
+//         moment (accumulate_node) instead of 2(m-1) sin/cos. gfx950 issues
+//         FP64 FMAs at the rate of unpacked FP32 ones, so the cosine
+//         recurrence and sums (the moments' bulk) and trig_0 (the exact
+//         integral of the reference's lines) are FP64; the sine terms carry
+//         only the first-order ulp corrections and stay FP32. The moments are
+//         rounded to float once. The node's phasor is the previous one turned
+//         by the exact FP64 difference of the two float phases (the
+//         reference's phases jitter by ulps around the uniform grid, so a
+//         fixed step would drift off cos(p_k)). Moments live in registers
+//         (template M >= m); more than 60 are split into chunks over
+//         workgroups (CHUNK).
 #include "device_common.hpp"
 #include "launch.hpp"
 #include "sph_common.hpp"
@@ -196,7 +196,11 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
 // issues FP64 FMAs at the rate of unpacked FP32 ones); the sine terms carry
 // only the first-order corrections of the reference's ulp offsets (h is a few
 // ulps of the slopes), so their recurrence from y = sin(phi) and their sums
-// are FP32: five operations and three registers per moment.
+// are FP32. The weight d + j^2 e is built up by the odd numbers,
+// w_j = w_{j-1} + (2j - 1) e, one FMA whose small odd factor is an
+// instruction literal (an FMA by j^2 into a fresh register would need a copy
+// of d first: gfx950's three-operand forms take no literal): five operations
+// and three registers per moment.
 template <int M>
 __device__ __forceinline__ void accumulate_node(double (&sc)[M], float (&ss)[M], double d, double e, float h,
                                                 double x, float y)
@@ -205,14 +209,15 @@ __device__ __forceinline__ void accumulate_node(double (&sc)[M], float (&ss)[M],
     const float x2f = (float)x2;
     double cm2 = 1.0, cm1 = x;
     float sm2 = 0.0f, sm1 = y;
-    sc[1] = fma(d + e, cm1, sc[1]);
+    double w = d + e;
+    sc[1] = fma(w, cm1, sc[1]);
     ss[1] = __builtin_fmaf(h, sm1, ss[1]);
 #pragma unroll
     for (int j = 2; j < M; ++j)
     {
         const double c = fma(x2, cm1, -cm2);
         const float sn = __builtin_fmaf(x2f, sm1, -sm2);
-        const double w = fma((double)(j * j), e, d);
+        w = fma((double)(2 * j - 1), e, w);
         sc[j] = fma(w, c, sc[j]);
         ss[j] = __builtin_fmaf(h, sn, ss[j]);
         cm2 = cm1;
@@ -227,8 +232,9 @@ __device__ __forceinline__ void accumulate_node(double (&sc)[M], float (&ss)[M],
 // registers): (c0, s0) and (c1, s1) are cos and sin of j0 phi and (j0 + 1) phi,
 // the recurrences run from there. The weight d + j^2 e is D + i (E2 + i e)
 // with D = d + j0^2 e and E2 = 2 j0 e (one more operation per moment than
-// with a compile-time j). Slot i = 0 of the chunk j0 = 0 (moment 0) gets
-// d, which nothing reads.
+// with a compile-time j; building it up as in accumulate_node would save that
+// one but lengthen the registers' live ranges past 256). Slot i = 0 of the
+// chunk j0 = 0 (moment 0) gets d, which nothing reads.
 template <int M>
 __device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&ss)[M], double d, double e, float h,
                                                       double x, double c0, float s0, double c1, float s1, double j0)
@@ -382,6 +388,29 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     };
     PendingSample<T, OFF32, POW2> pending;
     auto density = [&](float sv) { return map_density_r<POW2>(sv, a.data_min, a.data_max, a.data_rinv); };
+    // cos and sin of the next node's phase p: those of the current node turned
+    // by dp = p - node_phase, exact in FP64 (the difference of two floats
+    // whose magnitudes are within 2^29 of each other), with sin dp and cos dp
+    // from their Taylor series to dp^5 and dp^6 (|dp| <= 2^-6: truncation
+    // below 5e-17 of sin and 1e-19 of cos); the rotation's rounding adds
+    // ~1e-16 per node. Larger steps (rays of a few dozen samples), the
+    // chunked march and the march of 60 moments (whose registers are full)
+    // take the direct sincos.
+    constexpr bool kRotate = !CHUNK && M <= 56;
+    auto advance_phase = [&](float p) {
+        const double dp = (double)p - (double)node_phase;
+        if (kRotate && fabs(dp) <= 0.015625)
+        {
+            const double d2 = dp * dp;
+            const double sd = fma(dp * d2, fma(d2, 1.0 / 120.0, -1.0 / 6.0), dp);
+            const double cd = fma(fma(fma(d2, -1.0 / 720.0, 1.0 / 24.0), d2, -0.5), d2, 1.0);
+            const double nc = fma(zc, cd, -zs * sd);
+            zs = fma(zs, cd, zc * sd);
+            zc = nc;
+        }
+        else
+            sincos_small((double)p, zs, zc);
+    };
     issue_at(pending, (t + s < t_far) ? t + s : t_far);
     for (int it = 0; t + s < t_far && it < cap; ++it)
     {
@@ -416,9 +445,9 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         g_left = g;
         eps_left = width - (double)dphi;
         f_prev = f;
+        advance_phase(p);
         node_phase = p;
         ++node;
-        sincos_small((double)p, zs, zc);
         chunk_start(p);
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
@@ -568,8 +597,10 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
     MRAY_GEN_CASE(56)
     MRAY_GEN_CASE(60)
 #undef MRAY_GEN_CASE
-    // 61..255 moments: K chunks of at most 56 (an unchunked M = 64 and a
-    // chunked M = 48 lose half their occupancy), each M >= ceil(m / K) wide
+    // 61..255 moments: K chunks of at most 56 (an unchunked M = 64 loses half
+    // its occupancy), each M >= ceil(m / K) wide; M = 50 serves 81..100
+    // moments in two chunks within 256 registers also with the 64-bit
+    // addressing of volumes above 4 GiB (M = 56 there takes 262)
     if (m <= 255)
     {
         const int chunks = (m + 55) / 56;
@@ -581,6 +612,7 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
         return 0;                                                                                                     \
     }
         MRAY_GEN_CHUNK_CASE(40)
+        MRAY_GEN_CHUNK_CASE(50)
         MRAY_GEN_CHUNK_CASE(56)
 #undef MRAY_GEN_CHUNK_CASE
     }

@@ -263,6 +263,60 @@ def test_cli_reference(tmp_path, case):
     assert (tmp_path / "ref.ppm").read_bytes() == ppm_bytes(ref)
 
 
+def test_cli_file_round_trips(tmp_path):
+    """Two file cases of the reference's reader and writer through the CLI:
+
+    * --error-bounds loads a file that carries a quantization table and
+      re-writes it (src/main.cpp:333-367): loading installs the table
+      (read_quantization_table, src/moment_image_io.cpp:110-115), so the
+      re-written float file carries the same table, and its moments are the
+      values the reference's reader returned for the original file;
+    * a dense image with quantization_bits < 32 and no compression is written
+      as raw floats (write_static_image, src/moment_image_io.cpp), the same
+      moments as the float file, and reconstructs to the same image."""
+    files = np.load(ROOT / "tests" / "golden" / "golden_files.npz")
+    W, H, m, _ = (int(v) for v in files["imgP_shape"])
+    (tmp_path / "tab.bin").write_bytes(files["pc_tab_q10_file"].tobytes())
+    before = read_moment_image(tmp_path / "tab.bin")
+    assert before["quantization_bits"] == 10 and np.array_equal(before["quant_table"], files["table"][:m])
+    vol = synth.marschner_lobb(40)
+    vol.tofile(tmp_path / "vol.raw")
+    camera_file(tmp_path / "cam.txt", synth.ortho_camera())
+    run("--error-bounds", tmp_path / "tab.bin", tmp_path / "vol.raw",
+        f"camera={tmp_path / 'cam.txt'},dims=40:40:40,domain=-1:-1:-1:1:1:1,error_type=2,bias=1e-4,step_size=0.03")
+    raw = (tmp_path / "tab.bin").read_bytes()
+    after = read_moment_image(tmp_path / "tab.bin")
+    print(f"[cli] re-written tabled file: flags {raw[6]:#x}, bits byte {raw[7]:#x}", flush=True)
+    assert raw[6] & 32 and raw[7] == (32 | 0x80)
+    assert np.array_equal(after["quant_table"], files["table"][:m])
+    assert np.array_equal(after["index"], before["index"])
+    assert np.array_equal(after["coding_params"], before["coding_params"])
+    assert np.array_equal(after["data"].view(np.uint32), files["pc_tab_q10_read"].view(np.uint32))
+
+    vol = synth.marschner_lobb(32)
+    vol.tofile(tmp_path / "vol32.raw")
+    rgba8 = synth.tf_ramps(0)
+    rgba8.tofile(tmp_path / "tf.bin")
+    W, H, m, step = 20, 18, 12, 2.0 / 64
+    dense = (f"width={W},height={H},num_moments={m},camera={tmp_path / 'cam.txt'},dims=32:32:32,"
+             f"domain=-1:-1:-1:1:1:1,step_size={step!r},transform_to_bounds=1,transform_bound_eps=0.01,"
+             f"bias=1e-4,form=plain,compact_image=0,prediction_coding=0")
+    run("--generate", tmp_path / "f32.bin", tmp_path / "vol32.raw", dense)
+    run("--generate", tmp_path / "q10.bin", tmp_path / "vol32.raw", dense + ",quantization_bits=10")
+    f32, q10 = read_moment_image(tmp_path / "f32.bin"), read_moment_image(tmp_path / "q10.bin")
+    assert q10["quantization_bits"] == 10 and not q10["compact"]
+    assert len(q10["stream"]) == 4 * W * H * m
+    assert np.array_equal(q10["stream"].view(np.uint32), f32["data"].view(np.uint32))
+    rec = (f"tf={tmp_path / 'tf.bin'},camera={tmp_path / 'cam.txt'},step_size={step!r},bias=1e-4,"
+           f"early_ray_termination=0.99,medium_emission=1.0,preintegration_step_size_factor=4.0,form=plain")
+    for name in ("f32", "q10"):
+        run("--reconstruct", tmp_path / f"{name}.bin", tmp_path / f"{name}.ppm",
+            rec + f",float_rgba={tmp_path / f'{name}.f32'}")
+    a = np.fromfile(tmp_path / "f32.f32", np.float32)
+    assert np.array_equal(a.view(np.uint32), np.fromfile(tmp_path / "q10.f32", np.uint32))
+    assert a.reshape(H, W, 4)[..., 3].max() > 0.05
+
+
 def test_cli_rejects_bad_input(tmp_path):
     r = subprocess.run([str(MRAY), "--reconstruct", str(tmp_path / "missing.bin"), str(tmp_path / "o.ppm"),
                         "tf=x,camera=y"], capture_output=True, text=True, timeout=60)
