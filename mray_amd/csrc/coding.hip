@@ -40,57 +40,127 @@ namespace mray
 // compaction: number of moments per pixel
 // ---------------------------------------------------------------------------
 
-// evaluate_unknown_moments (src/moment_compaction.h:35-60): extend the first n
-// exponential moments to m with the MESE (Peters 2019, Eq. B.3).
-__device__ void evaluate_unknown_moments(int n_known, Col E, int m, Col newE, Col lev)
+// The search keeps two per-pixel arrays of m complex numbers: the exponential
+// moments E (fixed for the pixel; a global scratch column) and a work array X
+// in LDS that holds at once the three arrays the reference keeps separately,
+// since their live parts never overlap:
+//   X[0 .. n_known-1]  the Levinson solution lev (scaled by 2 pi),
+//   X[n_known .. m-1]  the extended exponential moments newE, each of which
+//                      is overwritten by the bounded moment B[l] computed
+//                      from it;
+// newE[l] for l < n_known is E[l] (the reference copies it), and B[l] for
+// l < n_known is the real moment l. One LDS column per pixel instead of three
+// fits three times the pixels per CU. The sums run in the reference's order
+// with its operations, so the counts are bit for bit its counts.
+
+// evaluate_rrmse (src/moment_compaction.h:84-111) with evaluate_unknown_moments
+// (:35-60: extend the first n_known exponential moments to m with the MESE,
+// Peters 2019, Eq. B.3) and exponential_to_bounded_moments (:65-79, including
+// its use of the l-th exponential moment inside the j loop).
+// acc + sum_{j = j0 .. j1} a(j) b(j), added in order of j; the operands of
+// eight terms are read before their products are summed, so that their loads
+// (global or LDS) are in flight together
+template <class FA, class FB> __device__ __forceinline__ cf dot_run(cf acc, int j0, int j1, FA a, FB b)
 {
-    const int mm = m - 1;
+    int j = j0;
+    for (; j + 7 <= j1; j += 8)
+    {
+        cf av[8], bv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+        {
+            av[i] = a(j + i);
+            bv[i] = b(j + i);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            acc = cadd(acc, cmul(av[i], bv[i]));
+    }
+    for (; j <= j1; ++j)
+        acc = cadd(acc, cmul(a(j), b(j)));
+    return acc;
+}
+
+// levinson (mese_f32.hpp) with its sums in runs of eight, so that the loads
+// of E (global) and P (LDS) of eight terms issue together
+__device__ __forceinline__ void levinson_unrolled(int m, Col P, Col E)
+{
+    P[0] = cmk(1.0f / E[0].x, 0.0f);
+    bool invalid = false;
+    for (int j = 1; j < m; ++j)
+    {
+        cf u = cmk(0.0f, 0.0f);
+        if (!invalid)
+            u = dot_run(u, 0, j - 1, [&](int k) { return P[k]; }, [&](int k) { return E[j - k]; });
+        float a2 = u.x * u.x + u.y * u.y;
+        if (a2 >= 1.0f)
+        {
+            u = cmk(0.0f, 0.0f);
+            a2 = 0;
+            invalid = true;
+        }
+        const float factor = 1.0f / (1.0f - a2);
+        const float p0 = P[0].x;
+#pragma unroll 4
+        for (int k = 1; 2 * k <= j; ++k)
+        {
+            const int q = j - k;
+            const cf a = P[k], b = P[q];
+            P[k] = crmul(factor, csub(a, cmul(cconj(b), u)));
+            if (q != k)
+                P[q] = crmul(factor, csub(b, cmul(cconj(a), u)));
+        }
+        P[0] = cmk(factor * p0, 0.0f);
+        P[j] = crmul(factor, crmul(-p0, u));
+    }
+}
+
+__device__ float evaluate_rrmse(int n_known, int m, const float *moments, Col E, Col X, float norm)
+{
     const int n = n_known - 1;
-    levinson(n_known, lev, E);
+    const int mm = m - 1;
+    levinson_unrolled(n_known, X, E);
     for (int i = 0; i < n + 1; ++i)
-        lev[i] = crmul(6.28318531f, lev[i]);
-    for (int l = 0; l < n + 1; ++l)
-        newE[l] = E[l];
-    const cf inv_lev0 = cdiv(cmk(-1.0f, 0.0f), lev[0]);
+        X[i] = crmul(6.28318531f, X[i]);
+    const cf inv_lev0 = cdiv(cmk(-1.0f, 0.0f), X[0]);
     for (int k = 0; k < mm - n; ++k)
     {
-        cf acc = cmk(0.0f, 0.0f);
-        for (int j = 1; j < n + 1; ++j)
-            acc = cadd(acc, cmul(newE[j + k], lev[n + 1 - j]));
-        newE[n + k + 1] = cmul(acc, inv_lev0);
+        // acc = sum_{j=1..n} newE[j + k] lev[n + 1 - j]: newE[j + k] is E up to
+        // index n (j <= n - k), then the extension
+        const int jsplit = n - k;
+        cf acc = dot_run(
+            cmk(0.0f, 0.0f), 1, jsplit, [&](int j) { return E[j + k]; }, [&](int j) { return X[n + 1 - j]; });
+        acc = dot_run(
+            acc, (jsplit < 0 ? 0 : jsplit) + 1, n, [&](int j) { return X[j + k]; },
+            [&](int j) { return X[n + 1 - j]; });
+        X[n + k + 1] = cmul(acc, inv_lev0);
     }
-}
-
-// exponential_to_bounded_moments (src/moment_compaction.h:65-79), including
-// its use of the l-th exponential moment inside the j loop.
-__device__ void exponential_to_bounded_moments(int m, Col E, Col B, int n)
-{
+    // lev is no longer needed: its slots take the known bounded moments, the
+    // real moments (as the reference's ext array holds them)
+    for (int q = 0; q < n_known; ++q)
+        X[q] = cmk(moments[q], 0.0f);
+    // bounded moments B[l], l >= n_known, over the extension
     const cf gamma_0 = cdivr(E[0], 2.f);
-    for (int l = n; l < m; ++l)
+    for (int l = n_known; l < m; ++l)
     {
-        B[l] = cmk(0.0f, 0.0f);
-        for (int j = 0; j < l; ++j)
+        const cf El = X[l];
+        const cf d = crmul((float)l, gamma_0);
+        // j = 0 reads B[l] itself, still zero
+        cf acc = cmk(0.0f, 0.0f);
+        acc = cadd(acc, cmul(cmul(cdiv(cmk((float)l, 0.0f), d), El), acc));
+#pragma unroll 8
+        for (int j = 1; j < l; ++j)
         {
-            const cf c = cdiv(cmk((float)(l - j), 0.0f), crmul((float)l, gamma_0));
-            B[l] = cadd(B[l], cmul(cmul(c, E[l]), B[l - j]));
+            const cf c = cdiv(cmk((float)(l - j), 0.0f), d);
+            acc = cadd(acc, cmul(cmul(c, El), X[l - j]));
         }
         const cf den = cmul(cmul(cmk(0.0f, 2.0f), cmk((float)kPi, 0.0f)), gamma_0);
-        B[l] = csub(cdiv(E[l], den), B[l]);
+        X[l] = csub(cdiv(El, den), acc);
     }
-}
-
-// evaluate_rrmse (src/moment_compaction.h:84-111)
-__device__ float evaluate_rrmse(int n, int m, const float *moments, Col tE, Col tExt, Col tLev, float norm)
-{
-    evaluate_unknown_moments(n, tE, m, tExt, tLev);
-    const Col ext = tLev;
-    for (int l = 0; l < n; ++l)
-        ext[l] = cmk(moments[l], 0.0f);
-    exponential_to_bounded_moments(m, tExt, ext, n);
     float sum = 0.0f;
-    for (int k = n; k < m; ++k)
+    for (int k = n_known; k < m; ++k)
     {
-        const float diff = ext[k].x - moments[k];
+        const float diff = X[k].x - moments[k];
         sum += diff * diff;
     }
     return sqrtf(sum) * norm;
@@ -98,18 +168,17 @@ __device__ float evaluate_rrmse(int n, int m, const float *moments, Col tE, Col 
 
 // determine_best_number_moments (src/moment_compaction.h:113-141): binary
 // search on the rRMSE starting at m/2 with step m/4.
-__device__ int determine_best_number_moments(int m, const float *moments, float error_threshold, Col tE, Col tExt,
-                                             Col tLev)
+__device__ int determine_best_number_moments(int m, const float *moments, float error_threshold, Col E, Col X)
 {
     if (error_threshold == 0.0f)
         return m;
     const float norm = 1.0f / moments[0];
-    trig_to_exp(m, tE, [&](int i) { return moments[i]; });
+    trig_to_exp(m, E, [&](int i) { return moments[i]; });
     int n = m / 2;
     int s = m / 4;
     while (s > 0)
     {
-        const float rrmse = evaluate_rrmse(n, m, moments, tE, tExt, tLev, norm);
+        const float rrmse = evaluate_rrmse(n, m, moments, E, X, norm);
         const float d = rrmse - error_threshold;
         if (((d < 0) ? -d : d) < 1e-8f)
             return n;
@@ -122,29 +191,33 @@ __device__ int determine_best_number_moments(int m, const float *moments, float 
     return n;
 }
 
-// compute_num_moments (src/moment_compaction.h:147-174), one pixel per thread
+// compute_num_moments (src/moment_compaction.h:147-174), one pixel per thread.
+// The grid strides over the pixels in groups of blockDim.x; thread slot
+// (blockIdx.x * blockDim.x + tid) owns column slot of the exponential-moment
+// scratch (gridDim.x * blockDim.x columns of m).
 __global__ __launch_bounds__(64) void num_moments_kernel(const float *__restrict__ dense,
                                                          const uint32_t *__restrict__ bounds, int npix, int m,
-                                                         float error_threshold, uint32_t *__restrict__ counts)
+                                                         float error_threshold, uint32_t *__restrict__ counts,
+                                                         cf *__restrict__ exp_scratch)
 {
     extern __shared__ cf lds_cf[];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int pix = blockIdx.x * nt + tid;
-    if (pix >= npix)
-        return;
-    const float *mo = dense + (size_t)pix * m;
-    bool is_zero;
-    if (bounds)
-        is_zero = (bounds[pix] & 0xFFFFu) == (bounds[pix] >> 16);
-    else
-        is_zero = mo[0] == 0.0f;
-    int n = 0;
-    if (!is_zero)
+    const int nslots = (int)gridDim.x * nt;
+    const Col E{exp_scratch + (size_t)blockIdx.x * nt + tid, nslots};
+    const Col X{lds_cf + tid, nt};
+    for (int base = (int)blockIdx.x * nt; base < npix; base += nslots)
     {
-        const Col tE{lds_cf + tid, nt}, tExt{lds_cf + nt * m + tid, nt}, tLev{lds_cf + 2 * nt * m + tid, nt};
-        n = determine_best_number_moments(m, mo, error_threshold, tE, tExt, tLev);
+        const int pix = base + tid;
+        if (pix >= npix)
+            continue;
+        const float *mo = dense + (size_t)pix * m;
+        bool is_zero;
+        if (bounds)
+            is_zero = (bounds[pix] & 0xFFFFu) == (bounds[pix] >> 16);
+        else
+            is_zero = mo[0] == 0.0f;
+        counts[pix] = is_zero ? 0u : (uint32_t)determine_best_number_moments(m, mo, error_threshold, E, X);
     }
-    counts[pix] = (uint32_t)n;
 }
 
 // ---------------------------------------------------------------------------
@@ -622,12 +695,27 @@ int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H,
                        uint32_t *counts, hipStream_t stream)
 {
     const int npix = W * H;
-    const int nt = column_block(3, m);
+    const int nt = column_block(1, m);
     if (!nt)
         return MRAY_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL(num_moments_kernel, dim3((npix + nt - 1) / nt), dim3(nt), (size_t)3 * m * nt * sizeof(cf),
-                       stream, dense, bounds, npix, m, error_threshold, counts);
-    return 0;
+    if (npix <= 0)
+        return 0;
+    // twice the workgroups the CUs hold at once (LDS-bound), at most one per
+    // pixel group; the scratch has one column of m per thread of the grid
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return MRAY_ERR_HIP;
+    const size_t lds = (size_t)m * nt * sizeof(cf);
+    const int per_cu = (int)std::max<size_t>(1, (size_t)163840 / lds);
+    const int groups = (npix + nt - 1) / nt;
+    const int blocks = std::min(groups, 2 * cus * per_cu);
+    cf *scratch_e = nullptr;
+    if (hipMallocAsync((void **)&scratch_e, (size_t)blocks * nt * m * sizeof(cf), stream) != hipSuccess)
+        return MRAY_ERR_OOM;
+    hipLaunchKernelGGL(num_moments_kernel, dim3(blocks), dim3(nt), lds, stream, dense, bounds, npix, m,
+                       error_threshold, counts, scratch_e);
+    return hipFreeAsync(scratch_e, stream) == hipSuccess ? 0 : MRAY_ERR_HIP;
 }
 
 // counts of non-empty pixels clamped to [lo, hi]; empty pixels stay empty

@@ -1118,19 +1118,36 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         __syncthreads();
         if (tg.lds)
         {
+            // the window's voxel i at (lx, ly, lz), i = lx + ex (ly + ey lz):
+            // each thread steps through i by blockDim.x, carrying lx and ly
+            // over (no integer divisions in the loop)
             const int n = tg.ex * tg.ey * tg.ez;
-            for (int i = tid; i < n; i += blockDim.x)
+            const int step = (int)blockDim.x;
+            const int sx = step % tg.ex, sy = (step / tg.ex) % tg.ey, sz = step / (tg.ex * tg.ey);
+            int lx = tid % tg.ex, ly = (tid / tg.ex) % tg.ey, lz = tid / (tg.ex * tg.ey);
+            for (int i = tid; i < n; i += step)
             {
                 const unsigned long long q = win[i];
                 if (q != 0ull)
                 {
                     win[i] = 0ull;
                     const float v = from_fixed(q, kfix);
-                    const int lx = i % tg.ex, rest = i / tg.ex;
-                    const int ly = rest % tg.ey, lz = rest / tg.ey;
                     unsafeAtomicAdd(grad_vol + ((size_t)(tg.oz + lz) * tg.sxy +
                                                 (uint32_t)((tg.oy + ly) * vol.nx + tg.ox + lx)),
                                     v);
+                }
+                lx += sx;
+                ly += sy;
+                lz += sz;
+                if (lx >= tg.ex)
+                {
+                    lx -= tg.ex;
+                    ++ly;
+                }
+                if (ly >= tg.ey)
+                {
+                    ly -= tg.ey;
+                    ++lz;
                 }
             }
         }
@@ -1190,13 +1207,22 @@ int launch_generate_adjoint(const float *vox, int nx, int ny, int nz, const floa
     MRAY_GADJ_CHUNK_CASE(48)
     MRAY_GADJ_CHUNK_CASE(56)
 #undef MRAY_GADJ_CHUNK_CASE
+    // the Clenshaw sums run over all M slots: widths close to the common
+    // moment counts
     MRAY_GADJ_CASE(8)
     MRAY_GADJ_CASE(16)
+    MRAY_GADJ_CASE(20)
     MRAY_GADJ_CASE(24)
+    MRAY_GADJ_CASE(28)
     MRAY_GADJ_CASE(32)
+    MRAY_GADJ_CASE(36)
     MRAY_GADJ_CASE(40)
+    MRAY_GADJ_CASE(44)
     MRAY_GADJ_CASE(48)
+    MRAY_GADJ_CASE(50)
+    MRAY_GADJ_CASE(52)
     MRAY_GADJ_CASE(56)
+    MRAY_GADJ_CASE(60)
     MRAY_GADJ_CASE(64)
 #undef MRAY_GADJ_CASE
     return MRAY_ERR_UNSUPPORTED;
