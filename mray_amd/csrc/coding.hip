@@ -29,6 +29,7 @@
 #include <vector>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "launch.hpp"
 #include "mese_f32.hpp"
@@ -57,23 +58,28 @@ namespace mray
 // (:35-60: extend the first n_known exponential moments to m with the MESE,
 // Peters 2019, Eq. B.3) and exponential_to_bounded_moments (:65-79, including
 // its use of the l-th exponential moment inside the j loop).
-// acc + sum_{j = j0 .. j1} a(j) b(j), added in order of j; the operands of
-// eight terms are read before their products are summed, so that their loads
-// (global or LDS) are in flight together
+// The search runs about one wavefront per SIMD (the LDS columns bound the
+// pixels per CU), so nothing hides the latency of a load but the loads issued
+// with it: the loops read the operands of a run of kRun terms (global E, LDS
+// X) before they combine them, in the reference's order. Registers are not
+// what bounds the occupancy here, so the runs are long.
+constexpr int kRun = 16;
+
+// acc + sum_{j = j0 .. j1} a(j) b(j), added in order of j
 template <class FA, class FB> __device__ __forceinline__ cf dot_run(cf acc, int j0, int j1, FA a, FB b)
 {
     int j = j0;
-    for (; j + 7 <= j1; j += 8)
+    for (; j + kRun - 1 <= j1; j += kRun)
     {
-        cf av[8], bv[8];
+        cf av[kRun], bv[kRun];
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < kRun; ++i)
         {
             av[i] = a(j + i);
             bv[i] = b(j + i);
         }
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < kRun; ++i)
             acc = cadd(acc, cmul(av[i], bv[i]));
     }
     for (; j <= j1; ++j)
@@ -81,8 +87,9 @@ template <class FA, class FB> __device__ __forceinline__ cf dot_run(cf acc, int 
     return acc;
 }
 
-// levinson (mese_f32.hpp) with its sums in runs of eight, so that the loads
-// of E (global) and P (LDS) of eight terms issue together
+// levinson (mese_f32.hpp) with its sums in runs, and its pair updates
+// (P_k, P_{j-k}) four pairs at a time: the pairs of one step are disjoint, so
+// reading four before writing them changes nothing
 __device__ __forceinline__ void levinson_unrolled(int m, Col P, Col E)
 {
     P[0] = cmk(1.0f / E[0].x, 0.0f);
@@ -101,8 +108,27 @@ __device__ __forceinline__ void levinson_unrolled(int m, Col P, Col E)
         }
         const float factor = 1.0f / (1.0f - a2);
         const float p0 = P[0].x;
-#pragma unroll 4
-        for (int k = 1; 2 * k <= j; ++k)
+        const int kend = j / 2;
+        int k = 1;
+        for (; k + 3 <= kend; k += 4)
+        {
+            cf av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+                av[i] = P[k + i];
+                bv[i] = P[j - k - i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+                const int q = j - k - i;
+                P[k + i] = crmul(factor, csub(av[i], cmul(cconj(bv[i]), u)));
+                if (q != k + i)
+                    P[q] = crmul(factor, csub(bv[i], cmul(cconj(av[i]), u)));
+            }
+        }
+        for (; k <= kend; ++k)
         {
             const int q = j - k;
             const cf a = P[k], b = P[q];
@@ -137,8 +163,21 @@ __device__ float evaluate_rrmse(int n_known, int m, const float *moments, Col E,
     }
     // lev is no longer needed: its slots take the known bounded moments, the
     // real moments (as the reference's ext array holds them)
-    for (int q = 0; q < n_known; ++q)
-        X[q] = cmk(moments[q], 0.0f);
+    {
+        int q = 0;
+        for (; q + kRun <= n_known; q += kRun)
+        {
+            float mv[kRun];
+#pragma unroll
+            for (int i = 0; i < kRun; ++i)
+                mv[i] = moments[q + i];
+#pragma unroll
+            for (int i = 0; i < kRun; ++i)
+                X[q + i] = cmk(mv[i], 0.0f);
+        }
+        for (; q < n_known; ++q)
+            X[q] = cmk(moments[q], 0.0f);
+    }
     // bounded moments B[l], l >= n_known, over the extension
     const cf gamma_0 = cdivr(E[0], 2.f);
     for (int l = n_known; l < m; ++l)
@@ -148,17 +187,60 @@ __device__ float evaluate_rrmse(int n_known, int m, const float *moments, Col E,
         // j = 0 reads B[l] itself, still zero
         cf acc = cmk(0.0f, 0.0f);
         acc = cadd(acc, cmul(cmul(cdiv(cmk((float)l, 0.0f), d), El), acc));
-#pragma unroll 8
-        for (int j = 1; j < l; ++j)
-        {
-            const cf c = cdiv(cmk((float)(l - j), 0.0f), d);
-            acc = cadd(acc, cmul(cmul(c, El), X[l - j]));
-        }
+        // c = (l - j) / d by the reference's complex division of the real
+        // (l - j, 0): ((l - j) d.x + 0 d.y, -(l - j) d.y + 0 d.x) / |d|^2.
+        // With d.x and d.y normal, (l - j) d.x and (l - j) d.y are nonzero
+        // and finite, and adding the product by zero (+-0) leaves them as
+        // they are: the same bits without those two products and sums.
+        const float dinv = 1.0f / cabs2(d);
+        auto sum_terms = [&](auto plain) {
+            auto quotient = [&](int r) {
+                const float fr = (float)r;
+                if constexpr (decltype(plain)::value)
+                    return cmk((fr * d.x) * dinv, (-fr * d.y) * dinv);
+                else
+                    return cdiv(cmk(fr, 0.0f), d);
+            };
+            int j = 1;
+            for (; j + kRun - 1 < l; j += kRun)
+            {
+                cf bv[kRun];
+#pragma unroll
+                for (int i = 0; i < kRun; ++i)
+                    bv[i] = X[l - j - i];
+#pragma unroll
+                for (int i = 0; i < kRun; ++i)
+                    acc = cadd(acc, cmul(cmul(quotient(l - j - i), El), bv[i]));
+            }
+            for (; j < l; ++j)
+                acc = cadd(acc, cmul(cmul(quotient(l - j), El), X[l - j]));
+        };
+        if (__builtin_isnormal(d.x) && __builtin_isnormal(d.y))
+            sum_terms(std::true_type{});
+        else
+            sum_terms(std::false_type{});
         const cf den = cmul(cmul(cmk(0.0f, 2.0f), cmk((float)kPi, 0.0f)), gamma_0);
         X[l] = csub(cdiv(El, den), acc);
     }
     float sum = 0.0f;
-    for (int k = n_known; k < m; ++k)
+    int k = n_known;
+    for (; k + kRun <= m; k += kRun)
+    {
+        float xv[kRun], mv[kRun];
+#pragma unroll
+        for (int i = 0; i < kRun; ++i)
+        {
+            xv[i] = X[k + i].x;
+            mv[i] = moments[k + i];
+        }
+#pragma unroll
+        for (int i = 0; i < kRun; ++i)
+        {
+            const float diff = xv[i] - mv[i];
+            sum += diff * diff;
+        }
+    }
+    for (; k < m; ++k)
     {
         const float diff = X[k].x - moments[k];
         sum += diff * diff;
@@ -194,7 +276,10 @@ __device__ int determine_best_number_moments(int m, const float *moments, float 
 // compute_num_moments (src/moment_compaction.h:147-174), one pixel per thread.
 // The grid strides over the pixels in groups of blockDim.x; thread slot
 // (blockIdx.x * blockDim.x + tid) owns column slot of the exponential-moment
-// scratch (gridDim.x * blockDim.x columns of m).
+// scratch (gridDim.x * blockDim.x columns of m). (Lanes whose searches sit
+// at different n idle in each other's loop iterations; dealing the pixels out
+// by n level by level within a workgroup measured slower, the barriers
+// costing more than the idling at the moment counts this search serves.)
 __global__ __launch_bounds__(64) void num_moments_kernel(const float *__restrict__ dense,
                                                          const uint32_t *__restrict__ bounds, int npix, int m,
                                                          float error_threshold, uint32_t *__restrict__ counts,
@@ -707,7 +792,7 @@ int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H,
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return MRAY_ERR_HIP;
     const size_t lds = (size_t)m * nt * sizeof(cf);
-    const int per_cu = (int)std::max<size_t>(1, (size_t)163840 / lds);
+    const int per_cu = (int)std::max<size_t>(1, g_lds_limit / lds);
     const int groups = (npix + nt - 1) / nt;
     const int blocks = std::min(groups, 2 * cus * per_cu);
     cf *scratch_e = nullptr;
