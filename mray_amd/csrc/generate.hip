@@ -230,11 +230,10 @@ __device__ __forceinline__ void accumulate_node(double (&sc)[M], float (&ss)[M],
 // accumulate_node for the moment chunk j = j0 + i, i = 0..M-1, of a march
 // whose moments are split into chunks (more than fit in one thread's
 // registers): (c0, s0) and (c1, s1) are cos and sin of j0 phi and (j0 + 1) phi,
-// the recurrences run from there. The weight d + j^2 e is D + i (E2 + i e)
-// with D = d + j0^2 e and E2 = 2 j0 e (one more operation per moment than
-// with a compile-time j; building it up as in accumulate_node would save that
-// one but lengthen the registers' live ranges past 256). Slot i = 0 of the
-// chunk j0 = 0 (moment 0) gets d, which nothing reads.
+// the recurrences run from there. The weight d + j^2 e = D + i E2 + i^2 e
+// (D = d + j0^2 e, E2 = 2 j0 e) is built up as w_i = w_{i-1} + (2i - 1) e
+// + E2: one operation per moment more than accumulate_node's. Slot i = 0 of
+// the chunk j0 = 0 (moment 0) gets d, which nothing reads.
 template <int M>
 __device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&ss)[M], double d, double e, float h,
                                                       double x, double c0, float s0, double c1, float s1, double j0)
@@ -245,7 +244,8 @@ __device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&s
     const double E2 = (j0 + j0) * e;
     double cm2 = c0, cm1 = c1;
     float sm2 = s0, sm1 = s1;
-    sc[0] = fma(D, c0, sc[0]);
+    double w = D;
+    sc[0] = fma(w, c0, sc[0]);
     ss[0] = __builtin_fmaf(h, s0, ss[0]);
 #pragma unroll
     for (int i = 1; i < M; ++i)
@@ -266,7 +266,7 @@ __device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&s
             sm2 = sm1;
             sm1 = sn;
         }
-        const double w = fma((double)(i * i), e, fma((double)i, E2, D));
+        w = fma((double)(2 * i - 1), e, w) + E2;
         sc[i] = fma(w, c, sc[i]);
         ss[i] = __builtin_fmaf(h, sn, ss[i]);
     }
@@ -393,10 +393,9 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     // whose magnitudes are within 2^29 of each other), with sin dp and cos dp
     // from their Taylor series to dp^5 and dp^6 (|dp| <= 2^-6: truncation
     // below 5e-17 of sin and 1e-19 of cos); the rotation's rounding adds
-    // ~1e-16 per node. Larger steps (rays of a few dozen samples), the
-    // chunked march and the march of 60 moments (whose registers are full)
-    // take the direct sincos.
-    constexpr bool kRotate = !CHUNK && M <= 56;
+    // ~1e-16 per node. Larger steps (rays of a few dozen samples) and the
+    // march of 60 moments (whose registers are full) take the direct sincos.
+    constexpr bool kRotate = M <= 56;
     auto advance_phase = [&](float p) {
         const double dp = (double)p - (double)node_phase;
         if (kRotate && fabs(dp) <= 0.015625)
@@ -597,13 +596,11 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
     MRAY_GEN_CASE(56)
     MRAY_GEN_CASE(60)
 #undef MRAY_GEN_CASE
-    // 61..255 moments: K chunks of at most 56 (an unchunked M = 64 loses half
-    // its occupancy), each M >= ceil(m / K) wide; M = 50 serves 81..100
-    // moments in two chunks within 256 registers also with the 64-bit
-    // addressing of volumes above 4 GiB (M = 56 there takes 262)
+    // 61..255 moments: K chunks of at most 50 (an unchunked M = 64 loses half
+    // its occupancy, and so does a chunk of 56), each M >= ceil(m / K) wide
     if (m <= 255)
     {
-        const int chunks = (m + 55) / 56;
+        const int chunks = (m + 49) / 50;
         const int per = (m + chunks - 1) / chunks;
 #define MRAY_GEN_CHUNK_CASE(MM)                                                                                       \
     if (per <= MM)                                                                                                    \
@@ -613,7 +610,6 @@ static int dispatch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenA
     }
         MRAY_GEN_CHUNK_CASE(40)
         MRAY_GEN_CHUNK_CASE(50)
-        MRAY_GEN_CHUNK_CASE(56)
 #undef MRAY_GEN_CHUNK_CASE
     }
     return MRAY_ERR_UNSUPPORTED;
