@@ -316,39 +316,21 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
         const double p0 = Pl[0].x / f;
         // previous iterate: Pold_k = P_k + u conj(P_{j-k}) (1 <= k < j),
         // Pold_0 = p0, Pold_j = 0; its partner Poq_k = Pold_{j-k} comes from
-        // the same two values, P_{j-k} + u conj(P_k)
-        dc Pq[S], Pbq[S], Pold[S], Poq[S];
+        // the same two values, P_{j-k} + u conj(P_k). Two passes over the
+        // slots, the second re-reading P_{j-k} and Pb_{j-k} from LDS, so that
+        // no per-slot temporaries stay live across the step (they would take
+        // the kernel past 256 VGPRs, to one wavefront per SIMD).
         dc acc = dmk(0.0, 0.0);
 #pragma unroll
         for (int s = 0; s < S; ++s)
         {
             const int k = g + 8 * s;
-            const int r = (k >= 1 && k < j) ? j - k : 0;
-            Pq[s] = Pl[r];
-            Pbq[s] = Yl[r];
             if (k >= 1 && k < j)
             {
-                Pold[s] = dadd(R.P[s], dmul(u, dconj(Pq[s])));
-                Poq[s] = dadd(Pq[s], dmul(u, dconj(R.P[s])));
-                acc = dadd(acc, dmul(R.Pb[s], Poq[s]));
-            }
-            else if (k == 0)
-            {
-                Pold[s] = dmk(p0, 0.0);
-                Poq[s] = dmk(0.0, 0.0);
-            }
-            else if (k == j)
-            {
-                Pold[s] = dmk(0.0, 0.0);
-                Poq[s] = dmk(p0, 0.0);
-            }
-            else
-            {
-                Pold[s] = R.P[s];
-                Poq[s] = dmk(0.0, 0.0);
+                const dc pq = Pl[j - k];
+                acc = dadd(acc, dmul(R.Pb[s], dadd(pq, dmul(u, dconj(R.P[s])))));
             }
         }
-        wave_sync();
         acc = dc{grp8_sum(acc.x), grp8_sum(acc.y)};
         dc ub = drmul(2.0 * f * f * fb, u);
         ub = dsub(ub, drmul(f, acc));
@@ -357,9 +339,32 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
         for (int s = 0; s < S; ++s)
         {
             const int k = g + 8 * s;
+            const int r = (k >= 1 && k < j) ? j - k : 0;
+            const dc pq = Pl[r], pbq = Yl[r];
+            dc Pold, Poq;
+            if (k >= 1 && k < j)
+            {
+                Pold = dadd(R.P[s], dmul(u, dconj(pq)));
+                Poq = dadd(pq, dmul(u, dconj(R.P[s])));
+            }
+            else if (k == 0)
+            {
+                Pold = dmk(p0, 0.0);
+                Poq = dmk(0.0, 0.0);
+            }
+            else if (k == j)
+            {
+                Pold = dmk(0.0, 0.0);
+                Poq = dmk(p0, 0.0);
+            }
+            else
+            {
+                Pold = R.P[s];
+                Poq = dmk(0.0, 0.0);
+            }
             dc Pbn = R.Pb[s];
             if (k >= 1 && k < j)
-                Pbn = drmul(f, dsub(R.Pb[s], dmul(dconj(Pbq[s]), u)));
+                Pbn = drmul(f, dsub(R.Pb[s], dmul(dconj(pbq), u)));
             else if (k == 0)
                 Pbn = dmk(f * p0b - f * ddot(pjb, u), 0.0);
             else if (k == j)
@@ -370,11 +375,12 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
                 if (k < j)
                     Pbn = dadd(Pbn, dmul(ub, dconj(E[j - k])));
                 if (k >= 1 && k <= j) // P_{j-k} of the previous iterate is Poq
-                    R.Eb[s] = dadd(R.Eb[s], dmul(ub, dconj(Poq[s])));
+                    R.Eb[s] = dadd(R.Eb[s], dmul(ub, dconj(Poq)));
             }
-            R.P[s] = Pold[s];
+            R.P[s] = Pold;
             R.Pb[s] = Pbn;
         }
+        wave_sync();
     }
     if constexpr (S > 1)
         levinson_reverse_octaves<S - 1, C>(R, Pl, Yl, E, g, nm, j_invalid);

@@ -672,16 +672,25 @@ template <bool POW2> struct GradSample
 };
 
 // Where the gradient of a cell's eight voxels goes: a workgroup's LDS window
-// over a box of voxels (summed there in 64-bit fixed point at scale 2^kfix,
-// added to the volume once per voxel and chunk of the march) or, for corners
-// outside the window, straight to the volume with float atomics.
+// over a box of voxels (summed there, added to the volume once per voxel and
+// chunk of the march) or, for corners
+// outside the window, straight to the volume with float atomics. The window's
+// 64 KiB hold 8192 voxels in 64-bit fixed point (integer LDS atomics, the
+// fast kind) or 16384 in float (LDS float atomics cost several times more,
+// but less than the global atomics a box too large for 8192 would take).
+enum WinMode : int
+{
+    kWinGlobal = 0, // no window: float atomics to the volume
+    kWinFixed = 1,  // 64-bit fixed point at scale 2^kfix
+    kWinFloat = 2,  // float
+};
 struct GradTarget
 {
     float *grad;
-    unsigned long long *win; // LDS window, x fastest
+    unsigned long long *win; // LDS window, x fastest (as 8192 x u64 or 16384 x f32)
     int kfix;
     int ox, oy, oz, ex, ey, ez;
-    bool lds;
+    int mode; // WinMode, the same in the whole workgroup
     int nx;
     uint32_t sxy;
 
@@ -691,13 +700,24 @@ struct GradTarget
     {
         const int fx = fl & 1, fy = (fl >> 1) & 1, fz = (fl >> 2) & 1;
         const int lx = cx - ox, ly = cy - oy, lz = cz - oz;
-        if (lds && lx >= 0 && ly >= 0 && lz >= 0 && lx + fx < ex && ly + fy < ey && lz + fz < ez)
+        if (mode != kWinGlobal && lx >= 0 && ly >= 0 && lz >= 0 && lx + fx < ex && ly + fy < ey && lz + fz < ez)
         {
-            unsigned long long *q0 = win + (lx + ex * (ly + ey * lz));
+            const int i0 = lx + ex * (ly + ey * lz);
             const int dy = fy * ex, dz = fz * ex * ey;
+            if (mode == kWinFixed)
+            {
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                atomicAdd(q0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), to_fixed(w[i], kfix));
+                for (int i = 0; i < 8; ++i)
+                    atomicAdd(win + i0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0),
+                              to_fixed(w[i], kfix));
+            }
+            else
+            {
+                float *wf = reinterpret_cast<float *>(win);
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    atomicAdd(wf + i0 + ((i & 1) ? fx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), w[i]);
+            }
             return;
         }
         float *p0 = grad + ((size_t)cz * sxy + (uint32_t)(cy * nx + cx));
@@ -786,7 +806,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
     // they agree on the box of voxels they can touch; gradients go to an LDS
     // window over that box (when it fits), which is then added to the volume
     // with one atomic per voxel instead of one per ray, sample and corner.
-    constexpr int kWin = 8192; // voxels of LDS window (64-bit fixed point)
+    constexpr int kWin = 8192; // voxels of LDS window in 64-bit fixed point (twice as many in float)
     constexpr int kChunk = 32; // nodes per chunk and ray
     __shared__ unsigned long long win[kWin];
     __shared__ int bbox[6];
@@ -1036,7 +1056,13 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         tg.ex = __builtin_amdgcn_readfirstlane(bbox[3]) - tg.ox + 1;
         tg.ey = __builtin_amdgcn_readfirstlane(bbox[4]) - tg.oy + 1;
         tg.ez = __builtin_amdgcn_readfirstlane(bbox[5]) - tg.oz + 1;
-        tg.lds = fixed && tg.ex > 0 && tg.ey > 0 && tg.ez > 0 && (long long)tg.ex * tg.ey * tg.ez <= kWin;
+        {
+            const long long nbox = (tg.ex > 0 && tg.ey > 0 && tg.ez > 0) ? (long long)tg.ex * tg.ey * tg.ez : -1;
+            tg.mode = (nbox < 0) ? kWinGlobal
+                      : (fixed && nbox <= kWin) ? kWinFixed
+                      : (nbox <= 2 * kWin) ? kWinFloat
+                                           : kWinGlobal;
+        }
         for (int k = 0; k < kChunk && !done; ++k, ++it)
         {
             const bool last = !(t + s < t_far) || it >= cap;
@@ -1112,8 +1138,10 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
         }
         cells.flush(tg);
         __syncthreads();
-        if (tg.lds)
+        if (tg.mode != kWinGlobal)
         {
+            // the window back to all-zero bits (a float sum may end as -0)
+            // as its sums go to the volume
             // the window's voxel i at (lx, ly, lz), i = lx + ex (ly + ey lz):
             // each thread steps through i by blockDim.x, carrying lx and ly
             // over (no integer divisions in the loop)
@@ -1121,17 +1149,35 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
             const int step = (int)blockDim.x;
             const int sx = step % tg.ex, sy = (step / tg.ex) % tg.ey, sz = step / (tg.ex * tg.ey);
             int lx = tid % tg.ex, ly = (tid / tg.ex) % tg.ey, lz = tid / (tg.ex * tg.ey);
+            uint32_t *win32 = reinterpret_cast<uint32_t *>(win);
             for (int i = tid; i < n; i += step)
             {
-                const unsigned long long q = win[i];
-                if (q != 0ull)
+                float v = 0.0f;
+                bool any = false;
+                if (tg.mode == kWinFixed)
                 {
-                    win[i] = 0ull;
-                    const float v = from_fixed(q, kfix);
+                    const unsigned long long q = win[i];
+                    any = q != 0ull;
+                    if (any)
+                    {
+                        win[i] = 0ull;
+                        v = from_fixed(q, kfix);
+                    }
+                }
+                else
+                {
+                    const uint32_t b = win32[i];
+                    any = b != 0u;
+                    if (any)
+                    {
+                        win32[i] = 0u;
+                        v = __uint_as_float(b);
+                    }
+                }
+                if (any)
                     unsafeAtomicAdd(grad_vol + ((size_t)(tg.oz + lz) * tg.sxy +
                                                 (uint32_t)((tg.oy + ly) * vol.nx + tg.ox + lx)),
                                     v);
-                }
                 lx += sx;
                 ly += sy;
                 lz += sz;
