@@ -216,6 +216,30 @@ __device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p, do
     return fma(2.0, fma(x, b1, -b2), (double)lam[0]);
 }
 
+// series_clenshaw at two phases in one pass: two independent recurrences
+// interleaved (each term's FMA waits on its predecessor; two chains give a
+// wavefront twice the independent work), each with series_clenshaw's
+// operations, so both values are series_clenshaw's bit for bit
+template <int M, typename L>
+__device__ __forceinline__ void series_clenshaw2(const L (&lam)[M], float pa, float pb, double &sa, double &sb)
+{
+    const double xa = cos_theta_of(pa), xb = cos_theta_of(pb);
+    const double x2a = xa + xa, x2b = xb + xb;
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int l = M - 1; l >= 1; --l)
+    {
+        const double ta = fma(x2a, a1, (double)lam[l] - a2);
+        const double tb = fma(x2b, b1, (double)lam[l] - b2);
+        a2 = a1;
+        a1 = ta;
+        b2 = b1;
+        b1 = tb;
+    }
+    sa = fma(2.0, fma(xa, a1, -a2), (double)lam[0]);
+    sb = fma(2.0, fma(xb, b1, -b2), (double)lam[0]);
+}
+
 // fast_atan with the reciprocal from the hardware approximation (about one
 // ulp) instead of the IEEE division and fused multiply-adds in the
 // polynomial; the MIXED form only
@@ -293,11 +317,20 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
     }
     const float brange = bhi - blo;
     const bool fourier = a.fourier != 0;
-    auto recon = [&](float p) {
-        p = rclamp(p, (float)(-kPi), 0.0f);
-        float s = (float)series_clenshaw<M>(lam, p);
-        float rho = fourier ? s : __builtin_fmaf(fast_atan_rcp(s), 0.318309886f, 0.5f);
+    auto density_of = [&](double series) {
+        const float s = (float)series;
+        const float rho = fourier ? s : __builtin_fmaf(fast_atan_rcp(s), 0.318309886f, 0.5f);
         return __builtin_fmaf(rho, brange, blo);
+    };
+    auto recon = [&](float p) { return density_of(series_clenshaw<M>(lam, rclamp(p, (float)(-kPi), 0.0f))); };
+    // the densities at p + dphi and p + 2 dphi (the march's float phase
+    // steps), from one interleaved pass
+    auto recon2 = [&](float p, float dphi, float &d1, float &d2) {
+        const float p1 = p + dphi, p2 = p1 + dphi;
+        double s1, s2;
+        series_clenshaw2<M>(lam, rclamp(p1, (float)(-kPi), 0.0f), rclamp(p2, (float)(-kPi), 0.0f), s1, s2);
+        d1 = density_of(s1);
+        d2 = density_of(s2);
     };
 
     Ray r = pixel_ray(cam, a.W, a.H, x, y);
@@ -314,12 +347,11 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
         float T = 1.0f, tau = 0.0f;
         float fs = recon(p);
         const int cap = march_cap(t_near, t_far, a.step) + 1;
-        for (int it = 0; t <= t_far && it < cap; ++it)
-        {
+        // one step of the reference's loop on the back sample bs
+        auto step = [&](float bs) {
             t += a.step;
             p += dphi;
             ++ns;
-            const float bs = recon(p);
             const float4 c = tf_sample_mixed(tf, fs, bs);
             Lr = __builtin_fmaf(a.emission * c.x, T, Lr);
             Lg = __builtin_fmaf(a.emission * c.y, T, Lg);
@@ -328,11 +360,38 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
             T = __expf(-tau);
             alpha = 1.0f - T;
             fs = bs;
-            if (alpha >= a.ert)
-            {
+            const bool stop = alpha >= a.ert;
+            if (stop)
                 alpha = 1.f;
-                break;
+            return stop;
+        };
+        // the loop (t <= t_far, at most cap steps, early termination) two
+        // steps at a time: both back samples from one interleaved pass (the
+        // second unused when the first step ends the march); above 96
+        // moments one at a time (the second chain's registers would cost
+        // occupancy)
+        int it = 0;
+        if constexpr (M <= 96)
+        {
+            while (t <= t_far && it < cap)
+            {
+                float b1, b2;
+                recon2(p, dphi, b1, b2);
+                if (step(b1))
+                    break;
+                ++it;
+                if (!(t <= t_far && it < cap))
+                    break;
+                if (step(b2))
+                    break;
+                ++it;
             }
+        }
+        else
+        {
+            for (; t <= t_far && it < cap; ++it)
+                if (step(recon(p + dphi)))
+                    break;
         }
     }
     store_rgba(rgba, pix, Lr, Lg, Lb, alpha);
