@@ -89,12 +89,40 @@ template <class FA, class FB> __device__ __forceinline__ cf dot_run(cf acc, int 
 
 // levinson (mese_f32.hpp) with its sums in runs, and its pair updates
 // (P_k, P_{j-k}) four pairs at a time: the pairs of one step are disjoint, so
-// reading four before writing them changes nothing
-__device__ __forceinline__ void levinson_unrolled(int m, Col P, Col E)
+// reading four before writing them changes nothing.
+//
+// Resumable: the recursion's state after its steps j < n (P_0 .. P_{n-1} and
+// whether the |u|^2 >= 1 guard has fired) is the result for n_known = n, and
+// the steps from there on are the same operations whatever came before, so
+// levinson(n') for n' > n continues from that state bit for bit. `from`
+// holds such a state for n_from (0: none), and the state reached is stored to
+// `to` (before the caller scales the solution).
+__device__ __forceinline__ void levinson_resume(int m, Col P, Col E, Col from, int n_from, bool inv_from, Col to,
+                                                bool &inv_to)
 {
-    P[0] = cmk(1.0f / E[0].x, 0.0f);
+    int j0 = 1;
     bool invalid = false;
-    for (int j = 1; j < m; ++j)
+    if (n_from > 0 && n_from <= m)
+    {
+        int i = 0;
+        for (; i + kRun <= n_from; i += kRun)
+        {
+            cf v[kRun];
+#pragma unroll
+            for (int r = 0; r < kRun; ++r)
+                v[r] = from[i + r];
+#pragma unroll
+            for (int r = 0; r < kRun; ++r)
+                P[i + r] = v[r];
+        }
+        for (; i < n_from; ++i)
+            P[i] = from[i];
+        j0 = n_from;
+        invalid = inv_from;
+    }
+    else
+        P[0] = cmk(1.0f / E[0].x, 0.0f);
+    for (int j = j0; j < m; ++j)
     {
         cf u = cmk(0.0f, 0.0f);
         if (!invalid)
@@ -139,13 +167,32 @@ __device__ __forceinline__ void levinson_unrolled(int m, Col P, Col E)
         P[0] = cmk(factor * p0, 0.0f);
         P[j] = crmul(factor, crmul(-p0, u));
     }
+    {
+        int i = 0;
+        for (; i + kRun <= m; i += kRun)
+        {
+            cf v[kRun];
+#pragma unroll
+            for (int r = 0; r < kRun; ++r)
+                v[r] = P[i + r];
+#pragma unroll
+            for (int r = 0; r < kRun; ++r)
+                to[i + r] = v[r];
+        }
+        for (; i < m; ++i)
+            to[i] = P[i];
+    }
+    inv_to = invalid;
 }
 
-__device__ float evaluate_rrmse(int n_known, int m, const float *moments, Col E, Col X, float norm)
+// evaluate_rrmse with the Levinson recursion resumed from the state `from`
+// (the result for n_from <= n_known, or none), its own state stored to `to`
+__device__ float evaluate_rrmse(int n_known, int m, const float *moments, Col E, Col X, float norm, Col from,
+                                int n_from, bool inv_from, Col to, bool &inv_to)
 {
     const int n = n_known - 1;
     const int mm = m - 1;
-    levinson_unrolled(n_known, X, E);
+    levinson_resume(n_known, X, E, from, n_from, inv_from, to, inv_to);
     for (int i = 0; i < n + 1; ++i)
         X[i] = crmul(6.28318531f, X[i]);
     const cf inv_lev0 = cdiv(cmk(-1.0f, 0.0f), X[0]);
@@ -249,8 +296,12 @@ __device__ float evaluate_rrmse(int n_known, int m, const float *moments, Col E,
 }
 
 // determine_best_number_moments (src/moment_compaction.h:113-141): binary
-// search on the rRMSE starting at m/2 with step m/4.
-__device__ int determine_best_number_moments(int m, const float *moments, float error_threshold, Col E, Col X)
+// search on the rRMSE starting at m/2 with step m/4. After a step up from n
+// every later n exceeds it (the later steps add up to less than s), so the
+// Levinson state at n is kept (scratch columns A and B take turns) and every
+// later evaluation resumes the recursion there instead of from step 1.
+__device__ int determine_best_number_moments(int m, const float *moments, float error_threshold, Col E, Col X,
+                                             Col A, Col B)
 {
     if (error_threshold == 0.0f)
         return m;
@@ -258,14 +309,24 @@ __device__ int determine_best_number_moments(int m, const float *moments, float 
     trig_to_exp(m, E, [&](int i) { return moments[i]; });
     int n = m / 2;
     int s = m / 4;
+    int n_kept = 0;
+    bool inv_kept = false;
     while (s > 0)
     {
-        const float rrmse = evaluate_rrmse(n, m, moments, E, X, norm);
+        bool inv_n = false;
+        const float rrmse = evaluate_rrmse(n, m, moments, E, X, norm, A, n_kept, inv_kept, B, inv_n);
         const float d = rrmse - error_threshold;
         if (((d < 0) ? -d : d) < 1e-8f)
             return n;
         else if (rrmse > error_threshold)
+        {
+            n_kept = n;
+            inv_kept = inv_n;
+            const Col t = A;
+            A = B;
+            B = t;
             n += s;
+        }
         else
             n -= s;
         s = s / 2;
@@ -275,8 +336,8 @@ __device__ int determine_best_number_moments(int m, const float *moments, float 
 
 // compute_num_moments (src/moment_compaction.h:147-174), one pixel per thread.
 // The grid strides over the pixels in groups of blockDim.x; thread slot
-// (blockIdx.x * blockDim.x + tid) owns column slot of the exponential-moment
-// scratch (gridDim.x * blockDim.x columns of m). (Lanes whose searches sit
+// (blockIdx.x * blockDim.x + tid) owns column slot of each of the scratch's
+// three arrays of gridDim.x * blockDim.x columns of m. (Lanes whose searches sit
 // at different n idle in each other's loop iterations; dealing the pixels out
 // by n level by level within a workgroup measured slower, the barriers
 // costing more than the idling at the moment counts this search serves.)
@@ -288,7 +349,11 @@ __global__ __launch_bounds__(64) void num_moments_kernel(const float *__restrict
     extern __shared__ cf lds_cf[];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nslots = (int)gridDim.x * nt;
-    const Col E{exp_scratch + (size_t)blockIdx.x * nt + tid, nslots};
+    // three scratch columns of m per thread: the exponential moments and two
+    // Levinson states
+    cf *col = exp_scratch + (size_t)blockIdx.x * nt + tid;
+    const size_t span = (size_t)m * nslots;
+    const Col E{col, nslots}, A{col + span, nslots}, B{col + 2 * span, nslots};
     const Col X{lds_cf + tid, nt};
     for (int base = (int)blockIdx.x * nt; base < npix; base += nslots)
     {
@@ -301,7 +366,7 @@ __global__ __launch_bounds__(64) void num_moments_kernel(const float *__restrict
             is_zero = (bounds[pix] & 0xFFFFu) == (bounds[pix] >> 16);
         else
             is_zero = mo[0] == 0.0f;
-        counts[pix] = is_zero ? 0u : (uint32_t)determine_best_number_moments(m, mo, error_threshold, E, X);
+        counts[pix] = is_zero ? 0u : (uint32_t)determine_best_number_moments(m, mo, error_threshold, E, X, A, B);
     }
 }
 
@@ -796,7 +861,7 @@ int launch_num_moments(const float *dense, const uint32_t *bounds, int W, int H,
     const int groups = (npix + nt - 1) / nt;
     const int blocks = std::min(groups, 2 * cus * per_cu);
     cf *scratch_e = nullptr;
-    if (hipMallocAsync((void **)&scratch_e, (size_t)blocks * nt * m * sizeof(cf), stream) != hipSuccess)
+    if (hipMallocAsync((void **)&scratch_e, (size_t)3 * blocks * nt * m * sizeof(cf), stream) != hipSuccess)
         return MRAY_ERR_OOM;
     hipLaunchKernelGGL(num_moments_kernel, dim3(blocks), dim3(nt), lds, stream, dense, bounds, npix, m,
                        error_threshold, counts, scratch_e);
