@@ -69,6 +69,35 @@ def test_num_moments_on_reference_vectors(hip, O):
         assert np.array_equal(c.astype(np.int32), g["compaction_counts"][i])
 
 
+@pytest.mark.parametrize("m", [16, 64, 100, 255])
+def test_num_moments_search_paths(hip, O, m):
+    """The count search against the oracle where its binary search takes
+    every path: moments of smooth densities (searches that step up, as the
+    large frames' do, and the Levinson recursion resumes from the kept
+    state) and random moments outside the moment space (the |u|^2 >= 1 guard
+    fires and the resumed recursion keeps it), at three thresholds."""
+    rng = np.random.default_rng(m)
+    n = 384
+    # smooth: moments of sums of two bumps on [-pi, 0] (the bounded moments
+    # of a density in [0, 1]); random: uniform values
+    phi = np.linspace(-np.pi, 0.0, 513)
+    smooth = np.empty((n // 2, m), np.float64)
+    for i in range(n // 2):
+        c, w = rng.uniform(-3.0, -0.2, 2), rng.uniform(0.05, 0.6, 2)
+        f = np.clip(0.6 * np.exp(-((phi - c[0]) / w[0]) ** 2) + 0.4 * np.exp(-((phi - c[1]) / w[1]) ** 2), 0, 1)
+        for j in range(m):
+            smooth[i, j] = np.trapezoid(f * np.cos(j * phi), phi) / np.pi
+    smooth[:, 0] = np.clip(smooth[:, 0], 0.02, 0.98)
+    rand = np.column_stack([rng.uniform(0.1, 0.9, n // 2), rng.uniform(-0.3, 0.3, (n // 2, m - 1))])
+    mo = np.ascontiguousarray(np.vstack([smooth, rand]).astype(np.float32))
+    d = dev(mo)
+    for th in (1e-2, 1e-3, 1e-4):
+        print(f"[num_moments] m={m} threshold {th:g}", flush=True)
+        ref = O.num_moments(n, 1, m, mo.reshape(-1), None, th)
+        got = host_u32(hip.compute_num_moments(d, n, 1, m, th))
+        assert np.array_equal(got, ref), (m, th, np.flatnonzero(got != ref)[:8])
+
+
 def test_coded_pipeline_matches_reference_outputs(hip, O):
     """generate (oracle) -> compaction -> prediction coding -> coded preparation ->
     PLAIN march, against the compiled reference's own outputs for the same
