@@ -17,12 +17,16 @@
 //     product the reference stores); its periodic update is a wave-wide scan.
 //     The encoder looks up a run of symbols at once: the model is constant
 //     between updates, whose positions are known in advance;
-//   - the decoder finds a symbol's interval with a 64-way search over the
-//     model (two levels for 1024 symbols): the interval ends
-//     base + length d(k) increase strictly with k (d grows by at least 2^-17
-//     per symbol and length exceeds 2^-16, so the gaps are far above double
-//     rounding), so the largest k with base + length d(k) <= value + 2^-48 is
-//     the symbol the reference's bisection finds.
+//   - the decoder finds a symbol's interval by testing 64 candidates at once:
+//     the interval ends base + length d(k) increase strictly with k (d grows
+//     by at least 2^-17 per symbol and length exceeds 2^-16, so the gaps are
+//     far above double rounding), so the largest k with
+//     base + length d(k) <= value + 2^-48 is the symbol the reference's
+//     bisection finds. A table over the model's cumulative counts (rebuilt at
+//     each model update) gives a symbol at or below it from a float estimate
+//     of the count at the value, and within 64 of it, so one test of 64
+//     candidates usually settles it; the code bytes come from a 256-byte
+//     window held in the wavefront's registers.
 // All double arithmetic is unfused and correctly rounded (-ffp-contract=off),
 // in the reference's order, so codes and decoded values are bit-identical.
 #include <hip/hip_runtime.h>
@@ -173,6 +177,48 @@ struct Model
 
     __device__ __forceinline__ double d(uint32_t k) const { return scale * (double)cum[k]; }
 };
+
+// The decoder's lookup table over the model's cumulative counts: tab[i] is
+// the symbol whose count range [cum[k], cum[k+1]) holds i * 2^kTabShift. A
+// count known to within +-2 then names a symbol at or below the one decoded,
+// at most 2^kTabShift + 4 symbols below it (every count is at least 1), so 64
+// candidates from there hold it.
+// The model's total stays at most max(2^17, 9 A + 48) < 2^18 for alphabets
+// of up to 2^14 symbols: an update adds at most 8 (A + 6), and one that passes
+// 2^17 halves the counts (rounding up), to at most half the sum plus A / 2.
+constexpr int kTabShift = 5;
+constexpr uint32_t kTabEntries = (1u << 18) >> kTabShift;
+
+__device__ void build_table(const Model &M, uint16_t *tab)
+{
+    const int lane = threadIdx.x;
+    const uint32_t used = (M.total >> kTabShift) + 1;
+    const uint32_t per = (used + 63) / 64;
+    const uint32_t i0 = min(used, lane * per), i1 = min(used, i0 + per);
+    if (i0 < i1)
+    {
+        // the symbol holding i0 << kTabShift (cum[0] = 0), then a walk
+        const uint32_t c0 = i0 << kTabShift;
+        uint32_t lo = 0, hi = M.A - 1;
+        while (lo < hi)
+        {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (M.cum[mid] <= c0)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        uint32_t k = lo;
+        for (uint32_t i = i0; i < i1; ++i)
+        {
+            const uint32_t c = i << kTabShift;
+            while (k + 1 < M.A && M.cum[k + 1] <= c)
+                ++k;
+            tab[i] = (uint16_t)k;
+        }
+    }
+    __syncthreads();
+}
 
 // pixel p's moment count
 __device__ __forceinline__ uint32_t count_of(const uint32_t *__restrict__ index, uint32_t p, uint32_t &start)
@@ -375,14 +421,36 @@ __global__ __launch_bounds__(64) void entropy_encode_kernel(const float *__restr
 }
 
 // The decoder's state (Arithmetic_Codec in mode 2); bytes past the code's
-// end read as zero.
+// end read as zero. Lane i holds code bytes wbase + 4i .. wbase + 4i + 3 (a
+// 256-byte window over the code, refilled when the read position leaves it).
 struct Decoder
 {
     double base, length, value;
     const uint8_t *in;
     size_t pos, n;
+    uint32_t win;
+    size_t wbase;
 
-    __device__ __forceinline__ double byte(size_t i) const { return (double)(i < n ? in[i] : 0u); }
+    __device__ void load_window(size_t b)
+    {
+        wbase = b;
+        const size_t q = b + 4 * (size_t)threadIdx.x;
+        uint32_t w = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            w |= (q + r < n ? (uint32_t)in[q + r] : 0u) << (8 * r);
+        win = w;
+    }
+
+    // code byte i (the same i in every lane)
+    __device__ __forceinline__ double byte(size_t i)
+    {
+        if (i - wbase >= 256)
+            load_window(i & ~(size_t)3);
+        const uint32_t off = (uint32_t)(i - wbase);
+        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)win, (int)(off >> 2));
+        return (double)((w >> (8 * (off & 3u))) & 0xFFu);
+    }
 
     // start_decoder (:590-604)
     __device__ void start(const uint8_t *code, size_t size)
@@ -391,35 +459,65 @@ struct Decoder
         n = size;
         length = 1.0;
         base = value = 0.0;
+        load_window(0);
         for (size_t k = 0; k < 6; ++k)
             value = 256.0 * value + kLsb * byte(k);
         pos = 6;
     }
 
     // decode(Adaptive_Data_Model) (:477-519) without the model's count update
-    __device__ uint32_t decode(const Model &M)
+    __device__ uint32_t decode(const Model &M, const uint16_t *tab)
     {
         const double sv = value + kLsb;
-        // the largest k with base + length d(k) <= sv (k = 0 when none)
-        uint32_t k0 = 0, span = M.A;
-        while (span > 1)
+        // the cumulative count at sv, (sv - base) total / length, to within a
+        // tenth of a count in float (total < 2^18; the hardware reciprocal is
+        // good to about an ulp); the table's symbol for two counts below it
+        // is at or below the symbol decoded
+        const float est = (float)(sv - base) * ((float)M.total * __builtin_amdgcn_rcpf((float)length));
+        const int c_lo = (est > 2.0f) ? (int)est - 2 : 0;
+        uint32_t k0 = tab[min((uint32_t)c_lo >> kTabShift, M.total >> kTabShift)];
+        // the largest k >= k0 with base + length d(k) <= sv, 64 candidates a
+        // round (the qualifying ones are a prefix)
+        uint32_t s = 0, j = 0;
+        double z = 0.0;
+        for (;;)
         {
-            const uint32_t step = (span + 63) / 64;
-            const uint32_t off = (uint32_t)threadIdx.x * step;
-            bool le = false;
-            if (off < span)
-            {
-                const double z = base + length * M.d(k0 + off);
-                le = !(z > sv);
-            }
+            const uint32_t k = k0 + (uint32_t)threadIdx.x;
+            // candidates past the alphabet read its last symbol and do not count
+            z = base + length * M.d(min(k, M.A - 1u));
+            const bool le = (k < M.A) && !(z > sv);
             const uint64_t b = __ballot(le);
-            const uint32_t j = b ? 63u - (uint32_t)__clzll((unsigned long long)b) : 0u;
-            k0 += j * step;
-            span = min(step, span - j * step);
+            if (b == 0ull)
+            {
+                // k0 qualifies by construction; should it not, search from
+                // symbol 0, and with none qualifying take symbol 0 (as the
+                // reference's bisection does)
+                if (k0 == 0)
+                {
+                    s = 0;
+                    j = 0;
+                    break;
+                }
+                k0 = 0;
+                continue;
+            }
+            j = 63u - (uint32_t)__clzll((unsigned long long)b);
+            if (j < 63u || k0 + 63u >= M.A - 1u)
+            {
+                s = k0 + j;
+                break;
+            }
+            k0 += 63u;
         }
-        const uint32_t s = k0;
-        const double x = base + length * M.d(s);
-        const double y = (s + 1 < M.A) ? base + length * M.d(s + 1) : base + length;
+        // x and y: lane j's and lane j + 1's ends (the same expressions)
+        const double x = readlane_f64(z, (int)j);
+        double y;
+        if (s + 1 >= M.A)
+            y = base + length;
+        else if (j < 63u)
+            y = readlane_f64(z, (int)j + 1);
+        else
+            y = base + length * M.d(s + 1);
         base = x;
         length = y - x;
         if (base >= 1.0)
@@ -482,6 +580,8 @@ __global__ __launch_bounds__(64) void entropy_decode_kernel(const uint8_t *__res
 
     Model M;
     M.reset(lds_model, 1u << bits);
+    uint16_t *tab = reinterpret_cast<uint16_t *>(lds_model + 2 * M.A);
+    build_table(M, tab);
     // stream l: [code_off[2 l], code_off[2 l + 1])
     Decoder D;
     D.start(section + code_off[2 * l], (size_t)(code_off[2 * l + 1] - code_off[2 * l]));
@@ -498,13 +598,16 @@ __global__ __launch_bounds__(64) void entropy_decode_kernel(const uint8_t *__res
         {
             const int i = __ffsll((unsigned long long)mask) - 1;
             mask &= mask - 1;
-            const uint32_t s = D.decode(M);
+            const uint32_t s = D.decode(M, tab);
             if (lane == i)
                 mine = s;
             if (lane == 0)
                 M.count[s] += 1;
             if (--M.until == 0)
+            {
                 M.update();
+                build_table(M, tab);
+            }
         }
         if (has)
             data[start + l] = ent_dequantize(mine, bits, lo, hi);
@@ -712,7 +815,7 @@ int launch_entropy_decode(const uint8_t *in, size_t n_in, const uint32_t *index,
     if (hipMallocAsync((void **)&buf, sec_b + off.size() * sizeof(uint64_t), stream) != hipSuccess)
         return MRAY_ERR_OOM;
     uint64_t *d_off = reinterpret_cast<uint64_t *>(buf + sec_b);
-    const size_t lds = (size_t)amax * 2 * sizeof(uint32_t);
+    const size_t lds = (size_t)amax * 2 * sizeof(uint32_t) + kTabEntries * sizeof(uint16_t);
     rc = set_lds(reinterpret_cast<const void *>(entropy_decode_kernel), lds);
     if (rc == 0 && (hipMemcpyAsync(buf, in, (size_t)pos, hipMemcpyHostToDevice, stream) != hipSuccess ||
                     hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice, stream) !=
