@@ -14,9 +14,9 @@
 //         as the reference CPU path.
 //
 //  MIXED  multipliers in registers (template M >= m); the series
-//         lambda_0 + 2 sum_l lambda_l cos(l theta) by Reinsch's form of
-//         Clenshaw's recurrence in FP64 (3 VALU ops per term, one sincos of
-//         the half angle per sample): at 50 moments the multipliers reach
+//         lambda_0 + 2 sum_l lambda_l cos(l theta) by Clenshaw's recurrence
+//         in FP64 (2 VALU ops per term, cos theta from an FP64 sine of the
+//         half angle per sample): at 50 moments the multipliers reach
 //         |lambda| ~ 1e3 and an FP32 series loses ~1e-4 of density to
 //         cancellation, while FP64 FMAs cost the same issue slots as FP32
 //         ones on gfx950. The 256-entry transfer function is staged in LDS
@@ -216,10 +216,10 @@ __device__ __forceinline__ double series_clenshaw(const L (&lam)[M], float p, do
     return fma(2.0, fma(x, b1, -b2), (double)lam[0]);
 }
 
-// series_clenshaw at two phases in one pass: two independent recurrences
-// interleaved (each term's FMA waits on its predecessor; two chains give a
-// wavefront twice the independent work), each with series_clenshaw's
-// operations, so both values are series_clenshaw's bit for bit
+// series_clenshaw at two phases in one pass, each recurrence with
+// series_clenshaw's operations, so both values are series_clenshaw's bit for
+// bit (the compiler runs the two recurrences one after the other; forced to
+// interleave them term by term, the march took as long)
 template <int M, typename L>
 __device__ __forceinline__ void series_clenshaw2(const L (&lam)[M], float pa, float pb, double &sa, double &sb)
 {
@@ -366,10 +366,10 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
             return stop;
         };
         // the loop (t <= t_far, at most cap steps, early termination) two
-        // steps at a time: both back samples from one interleaved pass (the
-        // second unused when the first step ends the march); above 96
-        // moments one at a time (the second chain's registers would cost
-        // occupancy)
+        // steps at a time: both back samples from one pass (the second unused
+        // when the first step ends the march), half the loop's tests and
+        // branches; above 96 moments one at a time (the second sample's
+        // registers would cost occupancy)
         int it = 0;
         if constexpr (M <= 96)
         {
