@@ -58,9 +58,20 @@ __global__ __launch_bounds__(64) void prepare_plain_kernel(const float *__restri
 
 // per-pixel LDS stride of the mixed kernels, in complex doubles: MP + 4, so
 // the eight pixels' arrays start 64 bytes (16 banks) apart (a stride of MP
-// elements puts them all in the same banks), except at C = 32, where the
-// padding would take the two arrays of eight pixels past 64 KiB
-constexpr int prepare_pixel_stride(int C) { return C >= 32 ? 8 * C : 8 * C + 4; }
+// elements puts them all in the same banks), except where the padding's 1 KiB
+// per workgroup costs a workgroup per CU that is worth more than the bank
+// conflicts it saves. Measured at 1024 x 1024 pixels, FP64 multipliers (ms,
+// padded / unpadded): C = 10 (80 moments) 12.0 / 11.3, C = 11 (88) 16.1 /
+// 14.5, C = 13 (100) 23.9 / 20.7, C = 20 (160) 85.3 / 66.9; where dropping
+// the padding also adds a workgroup per CU but loses: C = 5 (40) 2.75 /
+// 2.95, C = 7 (50) 4.38 / 4.59, C = 8 (64) 6.81 / 7.17, C = 16 (128) 44.0 /
+// 44.8; elsewhere the workgroups per CU are the same and the padding gains
+// 1-3 %. At C = 32 the padding would take the two arrays of eight pixels past
+// 64 KiB.
+constexpr int prepare_pixel_stride(int C)
+{
+    return (C >= 32 || C == 10 || C == 11 || C == 13 || C == 20) ? 8 * C : 8 * C + 4;
+}
 
 // OutT: float multipliers, or FP64 ones for mray_hip_prepare_f64
 template <int C, typename OutT = float>

@@ -257,11 +257,12 @@ def test_prepare_parity(hip, O, name, form):
     assert excess.max() <= 0.0
 
 
-@pytest.mark.parametrize("m", [64, 72, 100, 128, 160, 200, 248, 255])
+@pytest.mark.parametrize("m", [64, 72, 80, 88, 100, 128, 160, 200, 248, 255])
 def test_prepare_mixed_large_m(hip, O, m):
     """MIXED preparation beyond the parity cases' 50 moments, up to 255 (16
-    element slots per lane up to 128, then 20-32; at 32 the per-pixel LDS
-    arrays lose their bank padding to fit 64 KiB): densities against the FP64
+    element slots per lane up to 128, then 20-32; at 10, 11, 13, 20 and 32
+    slots, 80, 88, 100, 160 and 255 moments here, the per-pixel LDS arrays
+    have no bank padding): densities against the FP64
     oracle at 41 phases with the float-storage bound of test_prepare_parity.
     Moments from the oracle's generate."""
     import torch
