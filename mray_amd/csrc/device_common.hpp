@@ -430,15 +430,34 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &c)
     c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
 }
 
-// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD gets a
-// contiguous run of tiles (neighbouring rays share its L2).
-__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks)
+// Blocks are dealt round-robin to the 8 XCDs (block b runs on XCD b mod 8).
+// Remap so that each XCD takes runs of `run` consecutive blocks (a row of
+// tiles) in turn: the j-th block on XCD k is block j mod run of run
+// 8 (j / run) + k. Within a run neighbouring rays share the XCD's L2, and
+// the runs are dealt out so that every XCD gets rows from all over the
+// image: a perspective frame's rays are far shorter near its top and bottom
+// edges, and a contiguous band of rows per XCD left the busiest XCD with
+// 1.29x the mean work (1024 x 1024 rays into a cube at 25 degrees; 1.00x
+// dealt out by rows). Blocks past the last whole round of 8 runs keep their
+// index.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks, uint32_t run)
 {
     constexpr uint32_t kXcd = 8;
-    uint32_t per = nblocks / kXcd;
-    if (per == 0 || bid >= per * kXcd)
+    const uint32_t span = kXcd * run;
+    if (bid >= nblocks / span * span)
         return bid;
-    return (bid % kXcd) * per + bid / kXcd;
+    const uint32_t j = bid / kXcd, k = bid % kXcd;
+    return ((j / run) * kXcd + k) * run + j % run;
+}
+
+// This block's index in the tile order of an image W pixels wide, its blocks
+// holding tiles_per_block 8-pixel-wide tiles each, dealt out by tile rows
+// (xcd_swizzle)
+__device__ __forceinline__ uint32_t xcd_tile_block(int W, uint32_t tiles_per_block)
+{
+    const uint32_t tiles_x = (uint32_t)(W + 7) / 8;
+    const uint32_t run = max(1u, (tiles_x + tiles_per_block - 1) / tiles_per_block);
+    return xcd_swizzle(blockIdx.x, gridDim.x, run);
 }
 
 // 8x8 pixel tile per 64-lane wavefront; tiles in row-major order.

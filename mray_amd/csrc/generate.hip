@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void generate_plain_kernel(VolumeView<T> vol, 
     const int lane = tid & 63;
     const int stride = blockDim.x;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x / 64) + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, blockDim.x / 64) * (blockDim.x / 64) + (tid >> 6);
     if (tile >= ntiles)
         return;
     int x, y;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, 4) * 4 + (tid >> 6);
     if (tile >= ntiles)
         return;
     int x, y;
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(256) void generate_adjoint_kernel(VolumeView<float>
 
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, 4) * 4 + (tid >> 6);
     int x = 0, y = 0;
     bool active = tile < ntiles;
     if (active)

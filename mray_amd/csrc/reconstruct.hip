@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void reconstruct_plain_kernel(DevCamera cam, R
 
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x / 64) + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, blockDim.x / 64) * (blockDim.x / 64) + (tid >> 6);
     if (tile >= ntiles)
         return;
     int x, y;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_kernel(DevCamera cam, R
 
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, 4) * 4 + (tid >> 6);
     if (tile >= ntiles)
         return;
     int x, y;
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(256) void reconstruct_mixed_split_kernel(DevCamera 
     const int k = lane % K, ray = lane / K;
     const int tiles_x = (a.W + 7) / 8;
     const uint32_t ntiles = (uint32_t)tiles_x * (uint32_t)((a.H + RPT - 1) / RPT);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, 4) * 4 + (tid >> 6);
     if (tile >= ntiles)
         return;
     const int x = (int)(tile % tiles_x) * 8 + (ray & 7);
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(256) void reconstruct_adjoint_kernel(
     constexpr int RPT = 8 / K; // pixel rows per tile
     const int tiles_x = (a.W + 7) / 8;
     const uint32_t ntiles = (uint32_t)tiles_x * (uint32_t)((a.H + RPT - 1) / RPT);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, 4) * 4 + (tid >> 6);
     int x = 0, y = 0;
     bool active = tile < ntiles;
     if (active)

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void reference_kernel(VolumeView<T> vol, DevCa
 
     const int lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, 4) * 4 + (tid >> 6);
     if (tile >= ntiles)
         return;
     int x, y;

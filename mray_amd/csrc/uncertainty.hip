@@ -263,7 +263,7 @@ __device__ __forceinline__ bool pixel_of(const UArgs &a, int &x, int &y)
 {
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t ntiles = ((a.W + 7) / 8) * ((a.H + 7) / 8);
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x / 64) + (tid >> 6);
+    const uint32_t tile = xcd_tile_block(a.W, blockDim.x / 64) * (blockDim.x / 64) + (tid >> 6);
     if (tile >= ntiles)
         return false;
     tile_pixel(tile, a.W, lane, x, y);
