@@ -43,6 +43,17 @@ __device__ __forceinline__ dc dmul(dc a, dc b) { return dc{a.x * b.x - a.y * b.y
 __device__ __forceinline__ dc dconj(dc a) { return dc{a.x, -a.y}; }
 __device__ __forceinline__ dc drmul(double s, dc a) { return dc{s * a.x, s * a.y}; }
 __device__ __forceinline__ double ddot(dc a, dc b) { return a.x * b.x + a.y * b.y; } // Re(conj(a) b)
+// a + b c and a - b c by fused multiply-adds (the reverse passes only: their
+// values are the gradient, not a replay of the reference's arithmetic)
+__device__ __forceinline__ dc dfma(dc a, dc b, dc c)
+{
+    return dc{fma(b.x, c.x, fma(-b.y, c.y, a.x)), fma(b.x, c.y, fma(b.y, c.x, a.y))};
+}
+__device__ __forceinline__ dc dfms(dc a, dc b, dc c)
+{
+    return dc{fma(-b.x, c.x, fma(b.y, c.y, a.x)), fma(-b.x, c.y, fma(-b.y, c.x, a.y))};
+}
+__device__ __forceinline__ dc dsel(bool p, dc a, dc b) { return dc{p ? a.x : b.x, p ? a.y : b.y}; }
 
 // strided per-thread column of a scratch array
 struct Dcol
@@ -297,13 +308,18 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
     {
         const dc u = grp8_get(R.U[S - 1], j & 7);
         const bool valid = j < j_invalid;
-        const double f = recip_nr(1.0 - (u.x * u.x + u.y * u.y));
+        // 1 / f = 1 - |u|^2: the forward step's factor and its reciprocal
+        // (dividing by f would cost an FP64 division per use)
+        const double finv = 1.0 - (u.x * u.x + u.y * u.y);
+        const double f = recip_nr(finv);
         double fbs = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s)
-            if (g + 8 * s <= j)
-                fbs += ddot(R.Pb[s], R.P[s]);
-        const double fb = grp8_sum(fbs) / f;
+        {
+            const double t = fma(R.Pb[s].x, R.P[s].x, fma(R.Pb[s].y, R.P[s].y, fbs));
+            fbs = (g + 8 * s <= j) ? t : fbs;
+        }
+        const double fb = grp8_sum(fbs) * finv;
 #pragma unroll
         for (int s = 0; s < S; ++s)
         {
@@ -313,28 +329,29 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
         wave_sync();
         const dc pjb = Yl[j];
         const double p0b = Yl[0].x;
-        const double p0 = Pl[0].x / f;
+        const double p0 = Pl[0].x * finv;
         // previous iterate: Pold_k = P_k + u conj(P_{j-k}) (1 <= k < j),
         // Pold_0 = p0, Pold_j = 0; its partner Poq_k = Pold_{j-k} comes from
         // the same two values, P_{j-k} + u conj(P_k). Two passes over the
         // slots, the second re-reading P_{j-k} and Pb_{j-k} from LDS, so that
         // no per-slot temporaries stay live across the step (they would take
-        // the kernel past 256 VGPRs, to one wavefront per SIMD).
+        // the kernel past 256 VGPRs, to one wavefront per SIMD). The cases of
+        // a slot (1 <= k < j, k = 0, k = j, k > j) are selected, not branched
+        // on: a wavefront's lanes hold different k, so it runs every case.
         dc acc = dmk(0.0, 0.0);
 #pragma unroll
         for (int s = 0; s < S; ++s)
         {
             const int k = g + 8 * s;
-            if (k >= 1 && k < j)
-            {
-                const dc pq = Pl[j - k];
-                acc = dadd(acc, dmul(R.Pb[s], dadd(pq, dmul(u, dconj(R.P[s])))));
-            }
+            const bool inner = k >= 1 && k < j;
+            const dc pq = Pl[inner ? j - k : 0];
+            acc = dsel(inner, dfma(acc, R.Pb[s], dfma(pq, u, dconj(R.P[s]))), acc);
         }
         acc = dc{grp8_sum(acc.x), grp8_sum(acc.y)};
         dc ub = drmul(2.0 * f * f * fb, u);
         ub = dsub(ub, drmul(f, acc));
         ub = dsub(ub, drmul(f * p0, pjb));
+        const double pb0 = f * p0b - f * ddot(pjb, u);
 #pragma unroll
         for (int s = 0; s < S; ++s)
         {
@@ -344,8 +361,8 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
             dc Pold, Poq;
             if (k >= 1 && k < j)
             {
-                Pold = dadd(R.P[s], dmul(u, dconj(pq)));
-                Poq = dadd(pq, dmul(u, dconj(R.P[s])));
+                Pold = dfma(R.P[s], u, dconj(pq));
+                Poq = dfma(pq, u, dconj(R.P[s]));
             }
             else if (k == 0)
             {
@@ -364,18 +381,18 @@ __device__ __forceinline__ void levinson_reverse_octaves(AdjRegs<C> &R, dc *Pl, 
             }
             dc Pbn = R.Pb[s];
             if (k >= 1 && k < j)
-                Pbn = drmul(f, dsub(R.Pb[s], dmul(dconj(pbq), u)));
+                Pbn = drmul(f, dfms(R.Pb[s], dconj(pbq), u));
             else if (k == 0)
-                Pbn = dmk(f * p0b - f * ddot(pjb, u), 0.0);
+                Pbn = dmk(pb0, 0.0);
             else if (k == j)
                 Pbn = dmk(0.0, 0.0);
             if (valid)
             {
-                // u = sum_{k<j} P_k E_{j-k}
+                // u = sum_{k<j} P_k E_{j-k}; P_{j-k} of the previous iterate is Poq
                 if (k < j)
-                    Pbn = dadd(Pbn, dmul(ub, dconj(E[j - k])));
-                if (k >= 1 && k <= j) // P_{j-k} of the previous iterate is Poq
-                    R.Eb[s] = dadd(R.Eb[s], dmul(ub, dconj(Poq)));
+                    Pbn = dfma(Pbn, ub, dconj(E[j - k]));
+                if (k >= 1 && k <= j)
+                    R.Eb[s] = dfma(R.Eb[s], ub, dconj(Poq));
             }
             R.P[s] = Pold;
             R.Pb[s] = Pbn;
@@ -396,7 +413,7 @@ __device__ __forceinline__ void herglotz_reverse_octaves(AdjRegs<C> &R, const do
     for (int l = hi; l >= ((S == 1) ? 1 : 8 * (S - 1)); --l)
     {
         const dc Ebl = grp8_get(R.Eb[S - 1], l & 7);
-        const double c = 2.0 * kPi / (double)l;
+        const double c = 2.0 * kPi * recip_nr((double)l);
         const dc accb = dmk(c * Ebl.y, -c * Ebl.x);
 #pragma unroll
         for (int s = 0; s < S; ++s)
@@ -405,12 +422,10 @@ __device__ __forceinline__ void herglotz_reverse_octaves(AdjRegs<C> &R, const do
             const int r = (k <= l) ? l - k : 0;
             const double Tl = T[r];
             dc El = E[r];
-            if (r == 0) // the recurrence ran on E_0 before its doubling
-                El = drmul(0.5, El);
-            if (k < l)
-                R.Eb[s] = dadd(R.Eb[s], drmul((double)(l - k) * Tl, accb));
-            if (k >= 1 && k <= l)
-                R.Tb[s] += (double)k * ddot(accb, El);
+            // the recurrence ran on E_0 before its doubling
+            El = dsel(r == 0, drmul(0.5, El), El);
+            R.Eb[s] = dsel(k < l, dadd(R.Eb[s], drmul((double)(l - k) * Tl, accb)), R.Eb[s]);
+            R.Tb[s] = (k >= 1 && k <= l) ? fma((double)k, ddot(accb, El), R.Tb[s]) : R.Tb[s];
         }
     }
     if constexpr (S > 1)
