@@ -432,8 +432,12 @@ __device__ __forceinline__ void herglotz_reverse_octaves(AdjRegs<C> &R, const do
         herglotz_reverse_octaves<S - 1, C>(R, T, E, g, nm);
 }
 
+// Up to 5 slots the kernel is held to 168 VGPRs, three wavefronts per SIMD
+// (at 5 slots at the cost of a 36-byte spill): 1024 x 1024 pixels at 32
+// moments 6.81 -> 5.83 ms, at 40 moments 9.81 -> 9.55 ms; from 6 slots the
+// spills cost more (50 moments: 15.7 -> 17.1 ms)
 template <int C>
-__global__ __launch_bounds__(64) void prepare_adjoint_grp_kernel(const float *__restrict__ mom,
+__global__ __launch_bounds__(64, (C <= 5) ? 3 : 1) void prepare_adjoint_grp_kernel(const float *__restrict__ mom,
                                                                  const uint32_t *__restrict__ index, int npix,
                                                                  int m, float bias,
                                                                  const float *__restrict__ grad_lag,
