@@ -52,6 +52,11 @@ CASES = {
     "ml_persp_m200": (lambda: synth.marschner_lobb(32), (-1, -1, -1), (1, 1, 1),
                       synth.perspective_camera(position=(0.15, -0.1, -2.6), fov_half_deg=24.0, aspect=1.0), 18, 18,
                       200, 2.0 / 64, 4.0, 1e-3),
+    # enough tile rows for the workgroups' dealing out to the XCDs to remap
+    # (136 x 200 rays: 107 workgroups of 4 tiles, 80 of them remapped)
+    "mixw_persp_m16_rows": (lambda: synth.mixing_layer(40, width=0.05), (0, 0, 0), (1, 1, 1),
+                            synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=136 / 200), 136, 200, 16,
+                            0.5 / 40, 40.0, 1e-3),
 }
 
 
@@ -398,25 +403,26 @@ def test_pipeline_adjoint_reconstruct_prepare(hip):
         assert abs(fd - an) <= 2e-3 * abs(fd) + 1e-6
 
 
-@pytest.mark.parametrize("persp,m", [(False, 16), (True, 40), (False, 100), (True, 200)])
-def test_generate_adjoint_matches_fp64_finite_differences(hip, persp, m):
+@pytest.mark.parametrize("persp,m,W,H", [(False, 16, 18, 14), (True, 40, 20, 20), (False, 100, 18, 14),
+                                         (True, 200, 20, 20), (True, 16, 136, 96)])
+def test_generate_adjoint_matches_fp64_finite_differences(hip, persp, m, W, H):
     """dL/dvolume of generate (L = sum gbar * moments) against central
     differences of the FP64 oracle's generate. Without bound normalisation
     and with a data range that never clamps, the moments are linear in the
     voxels, so central differences are exact up to the float storage of the
-    perturbed volume (relative 1e-5 at eps = 1e-2). Bar: 2e-4 relative."""
+    perturbed volume (relative 1e-5 at eps = 1e-2). Bar: 2e-4 relative.
+    136 x 96 rays are 51 workgroups of 4 tiles, 40 of them remapped when the
+    tile rows are dealt out to the XCDs."""
     import torch
 
     O = Oracle()
     rng = np.random.default_rng(41)
     if persp:
         vol, dmin, dmax = synth.marschner_lobb(24), (-1, -1, -1), (1, 1, 1)
-        cam = synth.perspective_camera(position=(0.1, 0.05, -2.6), fov_half_deg=22.0, aspect=1.0)
-        W = H = 20
+        cam = synth.perspective_camera(position=(0.1, 0.05, -2.6), fov_half_deg=22.0, aspect=W / H)
     else:
         vol, dmin, dmax = synth.marschner_lobb(20), (-1, -1, -1), (1, 1, 1)
         cam = synth.ortho_camera()
-        W, H = 18, 14
     gp = GenParams(step_size=2.0 / 48, transform_to_bounds=0, data_min=-0.1, data_max=1.1, compact_image=0,
                    prediction_coding=0)
     gbar = rng.normal(size=(H, W, m))

@@ -103,6 +103,14 @@ CASES = {
     "u8_ortho_m255": (lambda: np.clip(synth.mixing_layer(24, width=0.05) * 255, 0, 255).astype(np.uint8), (0, 0, 0),
                       (1, 1, 1), synth.ortho_camera(pos=(0.5, 0.5, -1.0), right=(0.6, 0, 0), up=(0, 0.6, 0)), 17, 15,
                       255, dict(step_size=1.0 / 48, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
+    # tile rows dealt out to the XCDs: 136 x 200 rays are 17 x 25 tiles, 107
+    # workgroups of 4 tiles, 5 per tile row; two whole rounds of 8 rows' runs
+    # of workgroups are remapped, the last 27 workgroups keep their index (the
+    # smaller cases here fit no whole round)
+    "mixw_persp_m24_rows": (lambda: synth.mixing_layer(48, width=0.05), (0, 0, 0), (1, 1, 1),
+                            synth.perspective_camera(position=(0.5, 0.5, -1.0), aspect=136 / 200), 136, 200, 24,
+                            dict(step_size=0.5 / 48, transform_to_bounds=1, transform_bound_eps=0.01,
+                                 compact_image=0, prediction_coding=0)),
     "tiny_m1": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 1,
                 dict(step_size=2.0 / 20, transform_to_bounds=1, compact_image=0, prediction_coding=0)),
     "tiny_m2": (lambda: synth.marschner_lobb(16), (-1, -1, -1), (1, 1, 1), synth.ortho_camera(), 9, 7, 2,
@@ -127,7 +135,7 @@ def gpu_volume(hip, vol, dmin, dmax):
 
 # opacity scale of the transfer function per case (the reference's
 # preintegration_step_size_factor), chosen for a partly transparent image
-TF_FACTOR = {"ml_ortho_m64": 4.0, "ml_ortho_m30": 4.0, "ml_persp_m40": 4.0, "mix_persp_m50": 40.0, "mixw_persp_m50": 40.0,
+TF_FACTOR = {"mixw_persp_m24_rows": 40.0, "ml_ortho_m64": 4.0, "ml_ortho_m30": 4.0, "ml_persp_m40": 4.0, "mix_persp_m50": 40.0, "mixw_persp_m50": 40.0,
              "ml_nobounds_m7": 4.0, "u8_ortho_m16": 20.0, "tiny_m1": 4.0, "tiny_m2": 4.0, "ml_ortho_m100": 4.0,
              "ml_persp_m200": 4.0, "u8_ortho_m255": 20.0}
 
@@ -136,7 +144,7 @@ TF_FACTOR = {"ml_ortho_m64": 4.0, "ml_ortho_m30": 4.0, "ml_persp_m40": 4.0, "mix
 # sharp mixing-layer and uint8 volumes at 16-50 moments sit so close to the
 # boundary of the moment space that at 5e-5 even float storage of exact
 # moments moves the image by ~6e-4 RMSE; 1e-3 makes them well posed.
-REC_BIAS = {"mix_persp_m50": 1e-3, "mixw_persp_m50": 1e-3, "u8_ortho_m16": 1e-3, "u8_ortho_m255": 1e-3,
+REC_BIAS = {"mixw_persp_m24_rows": 1e-3, "mix_persp_m50": 1e-3, "mixw_persp_m50": 1e-3, "u8_ortho_m16": 1e-3, "u8_ortho_m255": 1e-3,
             "ml_persp_m200": 1e-3}
 
 
@@ -291,7 +299,8 @@ def test_prepare_mixed_large_m(hip, O, m):
 
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 @pytest.mark.parametrize("name", ["ml_ortho_m30", "ml_persp_m40", "mix_persp_m50", "mixw_persp_m50", "u8_ortho_m16",
-                                  "tiny_m2", "ml_ortho_m64", "ml_ortho_m100", "ml_persp_m200", "u8_ortho_m255"])
+                                  "tiny_m2", "ml_ortho_m64", "ml_ortho_m100", "ml_persp_m200", "u8_ortho_m255",
+                                  "mixw_persp_m24_rows"])
 def test_reconstruct_parity(hip, O, name, form):
     """The march alone, on the FP32 oracle's multipliers: RMSE against the
     FP64 evaluation of the same multipliers <= 1e-4, or no worse than 1.5x the
@@ -376,7 +385,7 @@ def test_pipeline_mixed_vs_f64(hip, O, name):
         assert rmse <= rmse32
 
 
-@pytest.mark.parametrize("name", ["ml_ortho_m30", "mixw_persp_m50"])
+@pytest.mark.parametrize("name", ["ml_ortho_m30", "mixw_persp_m50", "mixw_persp_m24_rows"])
 def test_pipeline_plain_vs_f32(hip, O, name):
     import torch
 
