@@ -272,10 +272,12 @@ __device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&s
     }
 }
 
-// CHUNK: the moments are split into chunks of M over blockIdx.y (more than
-// one thread's registers hold); each chunk's workgroups march the same rays
-// and accumulate moments j0 .. j0 + M - 1, j0 = blockIdx.y * M; chunk 0 also
-// writes moment 0, the bounds, the ray range and the sample counts.
+// CHUNK: the moments are split into chunks of M (more than one thread's
+// registers hold); each chunk's workgroups march the same rays and
+// accumulate moments j0 .. j0 + M - 1. The first chunk is the unchunked
+// kernel (M moments of an image of m > M: it writes moments 0 .. M - 1, the
+// bounds, the ray range and the sample counts, and needs no per-node phasor
+// of j0 phi); the CHUNK kernel runs the later chunks, j0 = (blockIdx.y + 1) M.
 template <typename T, int M, bool OFF32, bool POW2, bool CHUNK = false>
 __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
@@ -293,8 +295,8 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     if (x >= a.W || y >= a.H)
         return;
     const int m = a.m;
-    const int j0 = CHUNK ? (int)blockIdx.y * M : 0;
-    const bool first_chunk = !CHUNK || blockIdx.y == 0;
+    const int j0 = CHUNK ? (int)(blockIdx.y + 1) * M : 0;
+    const bool first_chunk = !CHUNK;
     const int jend = CHUNK ? min(m, j0 + M) : m;
     Ray r = pixel_ray(cam, a.W, a.H, x, y);
     float t_near, t_far;
@@ -535,11 +537,19 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// chunks > 0 launches the moment chunks of M over blockIdx.y (CHUNK)
+// chunks > 1 marches the moments in chunks of M: the unchunked kernel for
+// the first, then the later ones over blockIdx.y (CHUNK)
 template <typename T, int M, bool CHUNK = false>
 static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
                          int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks, int chunks = 1)
 {
+    if constexpr (CHUNK)
+    {
+        launch_mixed<T, M, false>(v, c, a, o, b, s, rr, st, blocks);
+        if (chunks <= 1)
+            return;
+        --chunks;
+    }
     // byte offsets of every voxel fit in 32 bits up to 4 GiB of volume
     const bool off32 = (uint64_t)v.nx * v.ny * v.nz * sizeof(T) <= (uint64_t)0xFFFFFFFFu + 1u;
     // power-of-two domain extents and data range: divisions become products
