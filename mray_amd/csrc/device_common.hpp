@@ -394,15 +394,10 @@ __device__ __forceinline__ int fixed_scale(float b)
     return 61 - ex;
 }
 
-// sin and cos of x in FP64 for |x| < 1e5 (node phases in [-pi, 0], and
-// their products with moment indices below 256): the nearest multiple k of
-// pi/2 is taken off in two parts (the high part has 33 significant bits, so
-// k * hi is exact for |k| < 2^20), then Taylor series to r^20 on |r| <= pi/4
-// (truncation below 1e-21).
-__device__ __forceinline__ void sincos_small(double x, double &s, double &c)
+// sin and cos of r in FP64 for |r| <= pi/4: Taylor series to r^19 and r^20
+// (truncation below 1e-21)
+__device__ __forceinline__ void sincos_quarter(double r, double &sr, double &cr)
 {
-    const double k = rint(x * 0.63661977236758134);
-    const double r = fma(-k, 6.07710050650619224932e-11, fma(-k, 1.57079632673412561417e+00, x));
     const double r2 = r * r;
     double ps = 1.0 / 121645100408832000.0;   // 1/19!
     ps = fma(ps, -r2, 1.0 / 355687428096000.0); // 1/17!
@@ -413,7 +408,7 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &c)
     ps = fma(ps, -r2, 1.0 / 5040.0);
     ps = fma(ps, -r2, 1.0 / 120.0);
     ps = fma(ps, -r2, 1.0 / 6.0);
-    const double sr = fma(-r * r2, ps, r);
+    sr = fma(-r * r2, ps, r);
     double pc = 1.0 / 2432902008176640000.0;  // 1/20!
     pc = fma(pc, -r2, 1.0 / 6402373705728000.0); // 1/18!
     pc = fma(pc, -r2, 1.0 / 20922789888000.0);
@@ -424,7 +419,19 @@ __device__ __forceinline__ void sincos_small(double x, double &s, double &c)
     pc = fma(pc, -r2, 1.0 / 720.0);
     pc = fma(pc, -r2, 1.0 / 24.0);
     pc = fma(pc, -r2, 0.5);
-    const double cr = fma(-r2, pc, 1.0);
+    cr = fma(-r2, pc, 1.0);
+}
+
+// sin and cos of x in FP64 for |x| < 1e5 (node phases in [-pi, 0], and
+// their products with moment indices below 256): the nearest multiple k of
+// pi/2 is taken off in two parts (the high part has 33 significant bits, so
+// k * hi is exact for |k| < 2^20), then sincos_quarter on |r| <= pi/4.
+__device__ __forceinline__ void sincos_small(double x, double &s, double &c)
+{
+    const double k = rint(x * 0.63661977236758134);
+    const double r = fma(-k, 6.07710050650619224932e-11, fma(-k, 1.57079632673412561417e+00, x));
+    double sr, cr;
+    sincos_quarter(r, sr, cr);
     const int q = (int)k & 3;
     s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
     c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;

@@ -272,13 +272,56 @@ __device__ __forceinline__ void accumulate_node_chunk(double (&sc)[M], float (&s
     }
 }
 
+// accumulate_node_chunk for a chunk start J0 known at compile time: the
+// weight d + j^2 e continues the odd-number recurrence of accumulate_node
+// from D = d + J0^2 e, w_j = w_{j-1} + (2j - 1) e with the odd factor an
+// instruction literal, five operations per moment as in the first chunk.
+template <int M, int J0>
+__device__ __forceinline__ void accumulate_node_chunk_c(double (&sc)[M], float (&ss)[M], double d, double e, float h,
+                                                        double x, double c0, float s0, double c1, float s1)
+{
+    const double x2 = x + x;
+    const float x2f = (float)x2;
+    double cm2 = c0, cm1 = c1;
+    float sm2 = s0, sm1 = s1;
+    double w = fma((double)(J0 * J0), e, d);
+    sc[0] = fma(w, c0, sc[0]);
+    ss[0] = __builtin_fmaf(h, s0, ss[0]);
+#pragma unroll
+    for (int i = 1; i < M; ++i)
+    {
+        double c;
+        float sn;
+        if (i == 1)
+        {
+            c = c1;
+            sn = s1;
+        }
+        else
+        {
+            c = fma(x2, cm1, -cm2);
+            sn = __builtin_fmaf(x2f, sm1, -sm2);
+            cm2 = cm1;
+            cm1 = c;
+            sm2 = sm1;
+            sm1 = sn;
+        }
+        w = fma((double)(2 * (J0 + i) - 1), e, w);
+        sc[i] = fma(w, c, sc[i]);
+        ss[i] = __builtin_fmaf(h, sn, ss[i]);
+    }
+}
+
 // CHUNK: the moments are split into chunks of M (more than one thread's
 // registers hold); each chunk's workgroups march the same rays and
 // accumulate moments j0 .. j0 + M - 1. The first chunk is the unchunked
 // kernel (M moments of an image of m > M: it writes moments 0 .. M - 1, the
 // bounds, the ray range and the sample counts, and needs no per-node phasor
-// of j0 phi); the CHUNK kernel runs the later chunks, j0 = (blockIdx.y + 1) M.
-template <typename T, int M, bool OFF32, bool POW2, bool CHUNK = false>
+// of j0 phi); the CHUNK kernel runs the later chunks, j0 = (blockIdx.y + 1) M,
+// or j0 = J0C when that is nonzero (the second of two chunks: its weights
+// take literal odd factors, and cos and sin of j0 phi follow the node's
+// phasor by a rotation instead of a sincos per node).
+template <typename T, int M, bool OFF32, bool POW2, bool CHUNK = false, int J0C = 0>
 __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, DevCamera cam, GenArgs a,
                                                              float *__restrict__ out, uint32_t *__restrict__ bounds,
                                                              int32_t *__restrict__ samples,
@@ -295,7 +338,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     if (x >= a.W || y >= a.H)
         return;
     const int m = a.m;
-    const int j0 = CHUNK ? (int)(blockIdx.y + 1) * M : 0;
+    const int j0 = CHUNK ? (J0C ? J0C : (int)(blockIdx.y + 1) * M) : 0;
     const bool first_chunk = !CHUNK;
     const int jend = CHUNK ? min(m, j0 + M) : m;
     Ray r = pixel_ray(cam, a.W, a.H, x, y);
@@ -371,7 +414,9 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         ss[j] = 0.0f;
     }
     auto accumulate = [&](double d, double e, double h) {
-        if constexpr (CHUNK)
+        if constexpr (CHUNK && J0C > 0)
+            accumulate_node_chunk_c<M, J0C>(sc, ss, d, e, (float)h, zc, kc, (float)ks, kc1, (float)ks1);
+        else if constexpr (CHUNK)
             accumulate_node_chunk<M>(sc, ss, d, e, (float)h, zc, kc, (float)ks, kc1, (float)ks1, j0d);
         else
             accumulate_node<M>(sc, ss, d, e, (float)h, zc, (float)zs);
@@ -397,9 +442,16 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
     // below 5e-17 of sin and 1e-19 of cos); the rotation's rounding adds
     // ~1e-16 per node. Larger steps (rays of a few dozen samples) and the
     // march of 60 moments (whose registers are full) take the direct sincos.
+    //
+    // A chunk from J0C (compile time) turns cos and sin of j0 phi by j0 dp
+    // likewise (j0 dp rounded once), with sincos_quarter (|j0 dp| <= 50 / 64
+    // < pi / 4); otherwise, or past that step, chunk_next takes them from a
+    // sincos of j0 p.
     constexpr bool kRotate = M <= 56;
+    bool chunk_turned = false;
     auto advance_phase = [&](float p) {
         const double dp = (double)p - (double)node_phase;
+        chunk_turned = false;
         if (kRotate && fabs(dp) <= 0.015625)
         {
             const double d2 = dp * dp;
@@ -408,9 +460,29 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
             const double nc = fma(zc, cd, -zs * sd);
             zs = fma(zs, cd, zc * sd);
             zc = nc;
+            if constexpr (CHUNK && J0C > 0)
+            {
+                static_assert(J0C <= 50, "the turn by J0C dp must stay within pi / 4");
+                double sj, cj;
+                sincos_quarter((double)J0C * dp, sj, cj);
+                const double nkc = fma(kc, cj, -ks * sj);
+                ks = fma(ks, cj, kc * sj);
+                kc = nkc;
+                chunk_turned = true;
+            }
         }
         else
             sincos_small((double)p, zs, zc);
+    };
+    // cos and sin of j0 p and (j0 + 1) p at the node just reached
+    auto chunk_next = [&](float ph) {
+        if constexpr (CHUNK)
+        {
+            if (!chunk_turned)
+                sincos_small(j0d * (double)ph, ks, kc);
+            kc1 = fma(kc, zc, -ks * zs);
+            ks1 = fma(ks, zc, kc * zs);
+        }
     };
     issue_at(pending, (t + s < t_far) ? t + s : t_far);
     for (int it = 0; t + s < t_far && it < cap; ++it)
@@ -449,7 +521,7 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
         advance_phase(p);
         node_phase = p;
         ++node;
-        chunk_start(p);
+        chunk_next(p);
     }
     // last segment [p_K, 0] (src/raymarch_generate.h:107-114): it starts
     // exactly at p_K and its line ends exactly at f_last
@@ -537,19 +609,11 @@ __global__ __launch_bounds__(256) void generate_mixed_kernel(VolumeView<T> vol, 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// chunks > 1 marches the moments in chunks of M: the unchunked kernel for
-// the first, then the later ones over blockIdx.y (CHUNK)
-template <typename T, int M, bool CHUNK = false>
-static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
-                         int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks, int chunks = 1)
+// one launch of generate_mixed_kernel over `chunks` rows of blockIdx.y
+template <typename T, int M, bool CHUNK, int J0C>
+static void launch_mixed_kernel(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
+                                int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks, int chunks)
 {
-    if constexpr (CHUNK)
-    {
-        launch_mixed<T, M, false>(v, c, a, o, b, s, rr, st, blocks);
-        if (chunks <= 1)
-            return;
-        --chunks;
-    }
     // byte offsets of every voxel fit in 32 bits up to 4 GiB of volume
     const bool off32 = (uint64_t)v.nx * v.ny * v.nz * sizeof(T) <= (uint64_t)0xFFFFFFFFu + 1u;
     // power-of-two domain extents and data range: divisions become products
@@ -560,20 +624,37 @@ static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenAr
     {
         // particle sets: the gather ignores OFF32 and POW2 (the data range
         // division takes the exact path)
-        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false, CHUNK>), grid, dim3(256), 0, st, v, c, a, o, b,
-                           s, rr);
+        hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false, CHUNK, J0C>), grid, dim3(256), 0, st, v, c, a,
+                           o, b, s, rr);
     }
     else
     {
         if (off32 && pow2)
-            hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, true, CHUNK>), grid, dim3(256), 0, st, v, c, a, o, b,
-                               s, rr);
+            hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, true, CHUNK, J0C>), grid, dim3(256), 0, st, v, c, a,
+                               o, b, s, rr);
         else if (off32)
-            hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false, CHUNK>), grid, dim3(256), 0, st, v, c, a, o,
-                               b, s, rr);
+            hipLaunchKernelGGL((generate_mixed_kernel<T, M, true, false, CHUNK, J0C>), grid, dim3(256), 0, st, v, c,
+                               a, o, b, s, rr);
         else
-            hipLaunchKernelGGL((generate_mixed_kernel<T, M, false, false, CHUNK>), grid, dim3(256), 0, st, v, c, a, o,
-                               b, s, rr);
+            hipLaunchKernelGGL((generate_mixed_kernel<T, M, false, false, CHUNK, J0C>), grid, dim3(256), 0, st, v, c,
+                               a, o, b, s, rr);
+    }
+}
+
+// CHUNK marches the moments in chunks of M: the unchunked kernel for the
+// first, then the second from j0 = M as a compile-time start, or the later
+// ones over blockIdx.y
+template <typename T, int M, bool CHUNK = false>
+static void launch_mixed(const VolumeView<T> &v, const DevCamera &c, const GenArgs &a, float *o, uint32_t *b,
+                         int32_t *s, float2 *rr, hipStream_t st, uint32_t blocks, int chunks = 1)
+{
+    launch_mixed_kernel<T, M, false, 0>(v, c, a, o, b, s, rr, st, blocks, 1);
+    if constexpr (CHUNK)
+    {
+        if (chunks == 2)
+            launch_mixed_kernel<T, M, true, M>(v, c, a, o, b, s, rr, st, blocks, 1);
+        else if (chunks > 2)
+            launch_mixed_kernel<T, M, true, 0>(v, c, a, o, b, s, rr, st, blocks, chunks - 1);
     }
 }
 
