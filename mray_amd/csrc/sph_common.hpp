@@ -65,7 +65,10 @@ struct SphView
     float inv_h;
 };
 
-// Particles::sample at p
+// Particles::sample at p. The loads are issued ahead of their use, in the
+// same summation order: a row's (up to) three cells' ranges together, and a
+// cell's particles four records at a time (one dependent load per particle
+// left the gather waiting on memory).
 __device__ __forceinline__ float sph_sample(const SphView &s, float px, float py, float pz)
 {
     const CellGrid &g = s.g;
@@ -74,29 +77,49 @@ __device__ __forceinline__ float sph_sample(const SphView &s, float px, float py
     const int z0 = max(c[2] - 1, 0), z1 = min(c[2] + 2, (int)g.sz);
     const int y0 = max(c[1] - 1, 0), y1 = min(c[1] + 2, (int)g.sy);
     const int x0 = max(c[0] - 1, 0), x1 = min(c[0] + 2, (int)g.sx);
+    auto add = [&](const float4 &pt) {
+        const float dx = px - pt.x, dy = py - pt.y, dz = pz - pt.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < g.r2)
+        {
+            const float ex = pt.x - px, ey = pt.y - py, ez = pt.z - pz;
+            const float w = cubic_spline(sqrtf(ex * ex + ey * ey + ez * ez) * s.inv_h);
+            wsum += w;
+            value += w * pt.w;
+        }
+    };
     for (int cz = z0; cz < z1; ++cz)
         for (int cy = y0; cy < y1; ++cy)
-            for (int cx = x0; cx < x1; ++cx)
+        {
+            // the row's cells x0 .. x1 - 1 (at most three)
+            uint32_t st[3], en[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
             {
-                const uint32_t ci = cell_index(g, cx, cy, cz);
-                const uint32_t st = s.start[ci];
-                if (st == kSphEmpty)
-                    continue;
-                const uint32_t e = s.end[ci];
-                for (uint32_t q = st; q < e; ++q)
-                {
-                    const float4 pt = s.parts[q];
-                    const float dx = px - pt.x, dy = py - pt.y, dz = pz - pt.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    if (d2 < g.r2)
-                    {
-                        const float ex = pt.x - px, ey = pt.y - py, ez = pt.z - pz;
-                        const float w = cubic_spline(sqrtf(ex * ex + ey * ey + ez * ez) * s.inv_h);
-                        wsum += w;
-                        value += w * pt.w;
-                    }
-                }
+                const bool in = x0 + i < x1;
+                const uint32_t ci = cell_index(g, in ? x0 + i : x0, cy, cz);
+                st[i] = in ? s.start[ci] : kSphEmpty;
+                en[i] = in ? s.end[ci] : 0u;
             }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                if (st[i] == kSphEmpty)
+                    continue;
+                const uint32_t e = en[i];
+                uint32_t q = st[i];
+                for (; q + 4 <= e; q += 4)
+                {
+                    const float4 p0 = s.parts[q], p1 = s.parts[q + 1], p2 = s.parts[q + 2], p3 = s.parts[q + 3];
+                    add(p0);
+                    add(p1);
+                    add(p2);
+                    add(p3);
+                }
+                for (; q < e; ++q)
+                    add(s.parts[q]);
+            }
+        }
     if (wsum != 0.0f)
         value /= wsum;
     return value;
