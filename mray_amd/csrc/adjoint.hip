@@ -55,6 +55,40 @@ __device__ __forceinline__ dc dfms(dc a, dc b, dc c)
 }
 __device__ __forceinline__ dc dsel(bool p, dc a, dc b) { return dc{p ? a.x : b.x, p ? a.y : b.y}; }
 
+// for t in [lo, hi): acc(ld(t)) in order of t, the loads of four terms issued
+// before their sums (one LDS round trip per four terms, not per term)
+template <class L, class A> __device__ __forceinline__ void run4(int lo, int hi, L ld, A acc)
+{
+    int t = lo;
+    for (; t + 4 <= hi; t += 4)
+    {
+        decltype(ld(t)) v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            v[i] = ld(t + i);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            acc(v[i]);
+    }
+    for (; t < hi; ++t)
+        acc(ld(t));
+}
+
+struct dc2
+{
+    dc a, b;
+};
+struct dcd
+{
+    dc a;
+    double d;
+};
+struct dcdc
+{
+    dc a, b;
+    double d;
+};
+
 // strided per-thread column of a scratch array
 struct Dcol
 {
@@ -508,12 +542,12 @@ __global__ __launch_bounds__(64, (C <= 5) ? 3 : 1) void prepare_adjoint_grp_kern
         const int kk = g + 8 * s;
         double rx = 0.0, ry = 0.0;
         if (kk < nm)
-            for (int j = 0; j < nm - kk; ++j)
-            {
-                const dc a = P[j], b = P[j + kk];
-                rx = fma(a.x, b.x, fma(a.y, b.y, rx));
-                ry = fma(a.y, b.x, fma(-a.x, b.y, ry));
-            }
+            run4(
+                0, nm - kk, [&](int j) { return dc2{P[j], P[j + kk]}; },
+                [&](const dc2 &v) {
+                    rx = fma(v.a.x, v.b.x, fma(v.a.y, v.b.y, rx));
+                    ry = fma(v.a.y, v.b.x, fma(-v.a.x, v.b.y, ry));
+                });
         Rk[s] = dc{rx, ry};
     }
     wave_sync();
@@ -558,15 +592,13 @@ __global__ __launch_bounds__(64, (C <= 5) ? 3 : 1) void prepare_adjoint_grp_kern
                 ck = fma(r.x, eh.y, r.y * eh.x);
                 eb = dc{lbt * r.y, lbt * r.x};
             }
-            for (int t = 1; t < nm - n; ++t)
-            {
-                const dc r = X[n + t];
-                const dc eh = E[t];
-                const double lbt = Yd[t];
-                ck = fma(r.x, eh.y, fma(r.y, eh.x, ck));
-                eb.x = fma(lbt, r.y, eb.x);
-                eb.y = fma(lbt, r.x, eb.y);
-            }
+            run4(
+                1, nm - n, [&](int t) { return dcdc{X[n + t], E[t], Yd[t]}; },
+                [&](const dcdc &v) {
+                    ck = fma(v.a.x, v.b.y, fma(v.a.y, v.b.x, ck));
+                    eb.x = fma(v.d, v.a.y, eb.x);
+                    eb.y = fma(v.d, v.a.x, eb.y);
+                });
             const double h = (n == 0) ? 0.5 : 1.0;
             eb = drmul(h * nf, eb);
             // Rb_n = nf sum_{kk <= n} lb_kk (Eh_{n-kk}.y, Eh_{n-kk}.x), kk = n peeled
@@ -575,13 +607,12 @@ __global__ __launch_bounds__(64, (C <= 5) ? 3 : 1) void prepare_adjoint_grp_kern
                 const double lbk = Yd[n];
                 rb = dc{lbk * eh.y, lbk * eh.x};
             }
-            for (int kk = 0; kk < n; ++kk)
-            {
-                const double lbk = Yd[kk];
-                const dc eh = E[n - kk];
-                rb.x = fma(lbk, eh.y, rb.x);
-                rb.y = fma(lbk, eh.x, rb.y);
-            }
+            run4(
+                0, n, [&](int kk) { return dcd{E[n - kk], Yd[kk]}; },
+                [&](const dcd &v) {
+                    rb.x = fma(v.d, v.a.y, rb.x);
+                    rb.y = fma(v.d, v.a.x, rb.y);
+                });
             rb = drmul(nf, rb);
             nfb_s = fma(Yd[n], ck, nfb_s);
         }
@@ -606,18 +637,18 @@ __global__ __launch_bounds__(64, (C <= 5) ? 3 : 1) void prepare_adjoint_grp_kern
         if (j < nm)
         {
             // Rb_kk P_{j+kk} over kk < nm - j, conj(Rb_kk) P_{j-kk} over kk <= j
-            for (int kk = 0; kk < nm - j; ++kk)
-            {
-                const dc rb = X[kk], b = P[j + kk];
-                pb.x = fma(rb.x, b.x, fma(-rb.y, b.y, pb.x));
-                pb.y = fma(rb.x, b.y, fma(rb.y, b.x, pb.y));
-            }
-            for (int kk = 0; kk <= j; ++kk)
-            {
-                const dc rb = X[kk], a = P[j - kk];
-                pb.x = fma(rb.x, a.x, fma(rb.y, a.y, pb.x));
-                pb.y = fma(rb.x, a.y, fma(-rb.y, a.x, pb.y));
-            }
+            run4(
+                0, nm - j, [&](int kk) { return dc2{X[kk], P[j + kk]}; },
+                [&](const dc2 &v) {
+                    pb.x = fma(v.a.x, v.b.x, fma(-v.a.y, v.b.y, pb.x));
+                    pb.y = fma(v.a.x, v.b.y, fma(v.a.y, v.b.x, pb.y));
+                });
+            run4(
+                0, j + 1, [&](int kk) { return dc2{X[kk], P[j - kk]}; },
+                [&](const dc2 &v) {
+                    pb.x = fma(v.a.x, v.b.x, fma(v.a.y, v.b.y, pb.x));
+                    pb.y = fma(v.a.x, v.b.y, fma(-v.a.y, v.b.x, pb.y));
+                });
         }
         R.Pb[s] = drmul(2.0 * kPi, pb);
         R.P[s] = drmul(1.0 / (2.0 * kPi), R.P[s]);
